@@ -1,0 +1,240 @@
+"""hpvcd_amd — MI355X reconstruction back end for HEVC/HEIC decoding.
+
+Python host mirror of the C ABI in ``include/hpvcd.h``.  The reference
+(awxkee/hpvcd v0.3.1) exposes ``Decoder::{decode, decode_rgb8, decode_yuv}``
+(src/decoder.rs:224-234) over ``decode_heic`` (src/lib.rs:1476); its
+reconstruction half runs per TU through ``ExecContext`` (src/exec.rs:33).
+Here the record producer (the seeded syntax generator, standing in for the
+CABAC parse) emits reconstruction records, and ``Job`` reconstructs whole
+batches of pictures on one GPU through hand-written gfx950 kernels.
+
+Errors surface as :class:`HpvcdError` carrying the ``hpvcd_status`` code
+(1..13 mirror ``DecodeError``, src/error.rs:30-58).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as N
+from ._native import HpvcdError, check
+
+__all__ = ["RecordSet", "Gpu", "Job", "HpvcdError", "WORKLOADS", "gen_config", "N"]
+
+
+# --------------------------------------------------------------------------------------
+# workloads: the synthetic picture sets the benchmark and the parity tests use
+# --------------------------------------------------------------------------------------
+WORKLOADS: Dict[str, dict] = {
+    # 512x512 HEVC Main intra, 4:2:0 8-bit, CTB 64, min CU 8, QP 32 -> RGB8
+    "single512": dict(tile_width=512, tile_height=512, grid_cols=1, grid_rows=1, out_width=512, out_height=512,
+                      chroma_idc=1, bit_depth=8, qp=32, config_id=0, out_kind=N.OUT_RGB8),
+    # 4032x3024 HEIC grid of 8x6 tiles of 512x512, 4:2:0 8-bit, QP 27 -> RGB8
+    "grid4032": dict(tile_width=512, tile_height=512, grid_cols=8, grid_rows=6, out_width=4032, out_height=3024,
+                     chroma_idc=1, bit_depth=8, qp=27, config_id=1, out_kind=N.OUT_RGB8),
+    # 4096x2048 RExt 4:4:4 10-bit grid of 8x4 tiles, QP 22, TS on 4x4 p=0.2, implicit RDPCM,
+    # cross-component prediction log2_res_scale in [-3, 3] -> RGB16
+    "rext4096": dict(tile_width=512, tile_height=512, grid_cols=8, grid_rows=4, out_width=4096, out_height=2048,
+                     chroma_idc=3, bit_depth=10, qp=22, config_id=2, out_kind=N.OUT_RGB16,
+                     transform_skip=1, log2_max_ts=2, p_ts=0.2, implicit_rdpcm=1, cross_component=1,
+                     log2_res_scale_max=3),
+}
+
+
+def gen_config(**overrides) -> N.GenConfig:
+    """Default distributions (split 0.5, NxN 0.5, TU split 0.4, cbf 0.6, sig 0.3,
+    geometric levels 0.5, SAO uniform, deblocking on) with overrides applied."""
+    cfg = N.GenConfig()
+    N.lib().hpvcd_gen_default_config(C.byref(cfg))
+    for k, v in overrides.items():
+        if not hasattr(cfg, k):
+            raise KeyError(f"unknown generator field {k!r}")
+        setattr(cfg, k, v)
+    return cfg
+
+
+class RecordSet:
+    """An owned set of reconstruction records (one hpvcd_records*)."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+        self.batch = N.Batch()
+        N.lib().hpvcd_records_view(self._h, C.byref(self.batch))
+
+    @classmethod
+    def generate(cls, workload: Optional[str] = None, **overrides) -> "RecordSet":
+        params = dict(WORKLOADS[workload]) if workload else {}
+        params.update(overrides)
+        cfg = gen_config(**params)
+        h = C.c_void_p()
+        check(N.lib().hpvcd_gen_records(C.byref(cfg), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def concat(cls, sets: Sequence["RecordSet"]) -> "RecordSet":
+        arr = (C.c_void_p * len(sets))(*[s._h.value for s in sets])
+        h = C.c_void_p()
+        check(N.lib().hpvcd_records_concat(arr, len(sets), C.byref(h)))
+        return cls(h.value)
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().hpvcd_records_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- views -------------------------------------------------------------------
+    @property
+    def num_pictures(self) -> int:
+        return self.batch.num_pictures
+
+    def picture(self, i: int) -> N.Picture:
+        return self.batch.pictures[i]
+
+    def tbs(self) -> np.ndarray:
+        n = self.batch.num_tbs
+        if n == 0:
+            return np.zeros(0, dtype=_TB_DTYPE)
+        buf = (N.TB * n).from_address(C.addressof(self.batch.tbs.contents))
+        return np.frombuffer(buf, dtype=_TB_DTYPE).copy()
+
+    def coeffs(self) -> np.ndarray:
+        n = self.batch.num_coeffs
+        return np.ctypeslib.as_array(self.batch.coeffs, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+
+    def cells(self) -> np.ndarray:
+        n = self.batch.num_cells
+        return np.ctypeslib.as_array(self.batch.cell_info, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+
+    def outputs(self) -> List[N.Output]:
+        return [self.batch.outputs[i] for i in range(self.batch.num_outputs)]
+
+    def output_buffers(self) -> List[np.ndarray]:
+        bufs = []
+        for o in self.outputs():
+            ch = 1 if o.kind in (N.OUT_LUMA8, N.OUT_LUMA16) else 3
+            dt = np.uint16 if o.kind in (N.OUT_RGB16, N.OUT_LUMA16) else np.uint8
+            bufs.append(np.zeros((o.height, o.width, ch), dtype=dt))
+        return bufs
+
+    def stats(self) -> dict:
+        """Coded samples S, nonzero levels, visible output pixels and output bytes."""
+        s = 0
+        sample_bytes = 0
+        for i in range(self.num_pictures):
+            p = self.picture(i)
+            sw = 2 if p.chroma_idc in (1, 2) else 1
+            sh = 2 if p.chroma_idc == 1 else 1
+            cs = 0 if p.chroma_idc == 0 else 2 * ((p.width + sw - 1) // sw) * ((p.height + sh - 1) // sh)
+            n = p.width * p.height + cs
+            s += n
+            sample_bytes += n * (1 if p.bit_depth_luma == 8 else 2)
+        out_px = sum(o.width * o.height for o in self.outputs())
+        out_bytes = sum(N.lib().hpvcd_output_bytes(C.byref(o)) for o in self.outputs())
+        return dict(samples=s, sample_bytes=sample_bytes, nnz=int(self.batch.num_coeffs), out_pixels=out_px,
+                    out_bytes=int(out_bytes), pictures=self.num_pictures, tbs=int(self.batch.num_tbs))
+
+
+_TB_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2n", "u1"), ("comp", "u1"), ("mode", "u1"), ("qp", "u1"),
+                      ("flags", "<u2"), ("nx", "u1"), ("res_scale", "i1"), ("coeff_off", "<u4"),
+                      ("coeff_cnt", "<u4"), ("order", "<u4")])
+assert _TB_DTYPE.itemsize == C.sizeof(N.TB)
+
+
+class Gpu:
+    """One GPU and its stream (hpvcd_gpu_open)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(N.lib().hpvcd_gpu_open(device, C.byref(h)))
+        self._h = h
+
+    def mem_info(self) -> Tuple[int, int]:
+        f, t = C.c_size_t(), C.c_size_t()
+        check(N.lib().hpvcd_gpu_mem_info(self._h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def sync(self) -> None:
+        check(N.lib().hpvcd_gpu_sync(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().hpvcd_gpu_close(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def job(self, records: RecordSet) -> "Job":
+        return Job(self, records)
+
+    def decode(self, records: RecordSet) -> List[np.ndarray]:
+        """Upload, reconstruct and download every output image of the batch."""
+        with Job(self, records) as j:
+            j.run()
+            return j.download()
+
+
+class Job:
+    """A batch resident in HBM (hpvcd_job_create)."""
+
+    def __init__(self, gpu: Gpu, records: RecordSet):
+        self.gpu = gpu
+        self.records = records
+        h = C.c_void_p()
+        check(N.lib().hpvcd_job_create(gpu._h, C.byref(records.batch), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().hpvcd_job_free(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def device_bytes(self) -> int:
+        return int(N.lib().hpvcd_job_device_bytes(self._h))
+
+    def run(self, mode: int = N.MODE_BATCHED, stop: int = N.STAGE_RGB) -> None:
+        check(N.lib().hpvcd_job_run(self._h, mode, stop))
+
+    def download(self) -> List[np.ndarray]:
+        bufs = self.records.output_buffers()
+        ptrs = (C.c_void_p * len(bufs))(*[b.ctypes.data for b in bufs])
+        check(N.lib().hpvcd_job_download(self._h, ptrs))
+        return bufs
+
+    def download_planes(self, picture: int) -> Tuple[np.ndarray, Optional[np.ndarray], Optional[np.ndarray]]:
+        p = self.records.picture(picture)
+        y = np.zeros((p.height, p.width), np.uint16)
+        if p.chroma_idc:
+            sw = 2 if p.chroma_idc in (1, 2) else 1
+            sh = 2 if p.chroma_idc == 1 else 1
+            cb = np.zeros(((p.height + sh - 1) // sh, (p.width + sw - 1) // sw), np.uint16)
+            cr = np.zeros_like(cb)
+            check(N.lib().hpvcd_job_download_planes(self._h, picture, y.ctypes.data, cb.ctypes.data, cr.ctypes.data))
+            return y, cb, cr
+        check(N.lib().hpvcd_job_download_planes(self._h, picture, y.ctypes.data, None, None))
+        return y, None, None
+
+    def time(self, mode: int = N.MODE_GRAPH, warmup: int = 2, iters: int = 10, breakdown: bool = True):
+        ms = C.c_double()
+        km = (C.c_double * 5)()
+        check(N.lib().hpvcd_job_time(self._h, mode, warmup, iters, C.byref(ms), km if breakdown else None))
+        return ms.value, {N.KERNEL_NAMES[i]: km[i] for i in range(5)} if breakdown else None

@@ -1,0 +1,202 @@
+"""ctypes mirror of include/hpvcd_records.h and include/hpvcd.h.
+
+Loads ``hpvcd_amd/libhpvcd.so`` (built in-tree by ``hpvcd_amd.build``).  A
+missing library raises ``ImportError`` with the build command; there is no
+fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhpvcd.so")
+
+# ---- records -----------------------------------------------------------------
+PF_STRONG_INTRA_SMOOTHING = 1 << 0
+PF_INTRA_SMOOTHING_DISABLED = 1 << 1
+PF_INTRA_BOUNDARY_FILTER_OFF = 1 << 2
+PF_EXTENDED_PRECISION = 1 << 3
+PF_IMPLICIT_RDPCM = 1 << 4
+PF_TS_ROTATION = 1 << 5
+PF_SAO_LUMA = 1 << 6
+PF_SAO_CHROMA = 1 << 7
+PF_PCM_LF_DISABLED = 1 << 8
+PF_CONSTRAINED_INTRA = 1 << 9
+PF_LF_ACROSS_TILES = 1 << 10
+PF_LF_ACROSS_SLICES = 1 << 11
+PF_FULL_RANGE = 1 << 12
+PF_TILES = 1 << 13
+PF_DEBLOCK_DISABLED = 1 << 14
+
+TB_CBF = 1 << 0
+TB_TS = 1 << 1
+TB_TQB = 1 << 2
+TB_INTER = 1 << 3
+TB_RDPCM_H = 1 << 4
+TB_RDPCM_V = 1 << 5
+
+CELL_TQB = 1 << 4
+CELL_PCM = 1 << 5
+CELL_INTRA = 1 << 6
+
+OUT_RGB8 = 0
+OUT_RGB16 = 1
+OUT_RGB8_FROM16 = 2
+OUT_LUMA8 = 3
+OUT_LUMA16 = 4
+
+SCALING_ENTRY_BYTES = 66
+SCALING_SET_BYTES = 4 * 6 * SCALING_ENTRY_BYTES
+
+MODE_BATCHED = 0
+MODE_PLAIN = 1
+MODE_GRAPH = 2
+
+STAGE_RECON = 0
+STAGE_DEBLOCK = 1
+STAGE_SAO = 2
+STAGE_RGB = 3
+
+KERNEL_NAMES = ("residual", "intra", "deblock", "sao", "rgb")
+
+
+class SliceParams(C.Structure):
+    _fields_ = [("disabled", C.c_uint8), ("beta_offset_div2", C.c_int8), ("tc_offset_div2", C.c_int8),
+                ("lf_across", C.c_uint8)]
+
+
+class Picture(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("chroma_idc", C.c_uint8), ("bit_depth_luma", C.c_uint8), ("bit_depth_chroma", C.c_uint8),
+        ("log2_ctb", C.c_uint8),
+        ("flags", C.c_uint32),
+        ("pps_cb_qp_offset", C.c_int8), ("pps_cr_qp_offset", C.c_int8),
+        ("matrix", C.c_uint8), ("num_slices", C.c_uint8),
+        ("log2_sao_offset_scale_luma", C.c_int8), ("log2_sao_offset_scale_chroma", C.c_int8),
+        ("out_kind", C.c_uint8), ("_pad0", C.c_uint8),
+        ("slices", SliceParams * 8),
+        ("slice_qp", C.c_int32),
+        ("crop_left", C.c_int32), ("crop_top", C.c_int32), ("win_w", C.c_int32), ("win_h", C.c_int32),
+        ("dst_x", C.c_int32), ("dst_y", C.c_int32), ("copy_w", C.c_int32), ("copy_h", C.c_int32),
+        ("out_index", C.c_uint32),
+        ("scaling_index", C.c_int32),
+        ("tb_begin", C.c_uint32), ("tb_count", C.c_uint32),
+        ("coeff_begin", C.c_uint32), ("coeff_count", C.c_uint32),
+        ("ctb_begin", C.c_uint32),
+        ("cell_begin", C.c_uint32),
+    ]
+
+
+class TB(C.Structure):
+    _fields_ = [
+        ("x", C.c_uint16), ("y", C.c_uint16),
+        ("log2n", C.c_uint8), ("comp", C.c_uint8), ("mode", C.c_uint8), ("qp", C.c_uint8),
+        ("flags", C.c_uint16),
+        ("nx", C.c_uint8), ("res_scale", C.c_int8),
+        ("coeff_off", C.c_uint32), ("coeff_cnt", C.c_uint32), ("order", C.c_uint32),
+    ]
+
+
+class Sao(C.Structure):
+    _fields_ = [("type_idx", C.c_uint8 * 3), ("band_pos", C.c_uint8 * 3), ("eo_class", C.c_uint8 * 3),
+                ("_pad", C.c_uint8 * 3), ("offsets", (C.c_int16 * 4) * 3)]
+
+
+class Output(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("kind", C.c_uint8), ("channels", C.c_uint8),
+                ("bytes_per_sample", C.c_uint8), ("_pad", C.c_uint8)]
+
+
+class Batch(C.Structure):
+    _fields_ = [
+        ("num_pictures", C.c_uint32), ("pictures", C.POINTER(Picture)),
+        ("num_tbs", C.c_uint32), ("tbs", C.POINTER(TB)),
+        ("num_coeffs", C.c_uint32), ("coeffs", C.POINTER(C.c_uint32)),
+        ("num_ctbs", C.c_uint32), ("sao", C.POINTER(Sao)),
+        ("num_cells", C.c_uint32), ("cell_info", C.POINTER(C.c_uint32)),
+        ("num_scaling", C.c_uint32), ("scaling", C.POINTER(C.c_uint8)),
+        ("num_outputs", C.c_uint32), ("outputs", C.POINTER(Output)),
+    ]
+
+
+class GenConfig(C.Structure):
+    _fields_ = [
+        ("tile_width", C.c_int32), ("tile_height", C.c_int32),
+        ("grid_cols", C.c_int32), ("grid_rows", C.c_int32),
+        ("out_width", C.c_int32), ("out_height", C.c_int32),
+        ("chroma_idc", C.c_uint8), ("bit_depth", C.c_uint8), ("log2_ctb", C.c_uint8), ("log2_min_cb", C.c_uint8),
+        ("log2_min_tb", C.c_uint8), ("log2_max_tb", C.c_uint8), ("max_transform_depth_intra", C.c_uint8),
+        ("qp", C.c_uint8),
+        ("log2_max_ts", C.c_uint8),
+        ("transform_skip", C.c_uint8), ("implicit_rdpcm", C.c_uint8), ("cross_component", C.c_uint8),
+        ("extended_precision", C.c_uint8),
+        ("strong_intra_smoothing", C.c_uint8), ("sao", C.c_uint8), ("deblock", C.c_uint8),
+        ("ts_rotation", C.c_uint8),
+        ("matrix", C.c_uint8), ("full_range", C.c_uint8), ("out_kind", C.c_uint8), ("num_outputs", C.c_uint8),
+        ("log2_res_scale_max", C.c_uint8),
+        ("_pad", C.c_uint8 * 3),
+        ("p_split_cu", C.c_float), ("p_nxn", C.c_float), ("p_split_tu", C.c_float), ("p_cbf", C.c_float),
+        ("p_sig", C.c_float), ("p_level_geom", C.c_float), ("p_ts", C.c_float), ("p_tqb", C.c_float),
+        ("config_id", C.c_uint32),
+        ("base_seed", C.c_uint64),
+    ]
+
+
+# every symbol include/hpvcd.h declares, with its ctypes signature
+_P = C.c_void_p
+SIGNATURES = {
+    "hpvcd_last_error": (C.c_char_p, []),
+    "hpvcd_version": (C.c_char_p, []),
+    "hpvcd_gen_default_config": (None, [C.POINTER(GenConfig)]),
+    "hpvcd_gen_records": (C.c_int, [C.POINTER(GenConfig), C.POINTER(_P)]),
+    "hpvcd_records_concat": (C.c_int, [C.POINTER(_P), C.c_uint32, C.POINTER(_P)]),
+    "hpvcd_records_view": (None, [_P, C.POINTER(Batch)]),
+    "hpvcd_records_free": (None, [_P]),
+    "hpvcd_output_bytes": (C.c_size_t, [C.POINTER(Output)]),
+    "hpvcd_gpu_open": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "hpvcd_gpu_close": (None, [_P]),
+    "hpvcd_gpu_mem_info": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "hpvcd_job_create": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(_P)]),
+    "hpvcd_job_free": (None, [_P]),
+    "hpvcd_job_device_bytes": (C.c_size_t, [_P]),
+    "hpvcd_job_run": (C.c_int, [_P, C.c_int, C.c_int]),
+    "hpvcd_gpu_sync": (C.c_int, [_P]),
+    "hpvcd_job_download": (C.c_int, [_P, C.POINTER(_P)]),
+    "hpvcd_job_download_planes": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
+    "hpvcd_job_time": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "hpvcd_gpu_decode_batch": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(_P)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded native library (raises ImportError when it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m hpvcd_amd.build` "
+                          "(or __graft_entry__.build()); there is no non-native fallback")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class HpvcdError(RuntimeError):
+    """A non-OK hpvcd_status; ``status`` holds the code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"hpvcd status {status}: {message}")
+        self.status = status
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise HpvcdError(status, lib().hpvcd_last_error().decode(errors="replace"))

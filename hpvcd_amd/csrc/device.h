@@ -1,0 +1,54 @@
+// device.h — device-side descriptors shared by the HIP kernels and the host
+// runtime that prepares them.  Plain structs, no device types.
+#pragma once
+
+#include <cstdint>
+
+#include "../../include/hpvcd_records.h"
+
+namespace hpvcd {
+
+// Per-picture descriptor as the kernels see it (host-derived geometry plus
+// offsets into the job's device pools).
+struct DevPic {
+    int32_t w, h, cw, ch;           // luma / chroma plane sizes
+    int32_t gw, gh;                 // 4x4 luma cell grid
+    int32_t sub_w_log2, sub_h_log2;
+    int32_t ctb_log2, ctb_cols, ctb_rows;
+    int32_t bd, bd_c;
+    uint32_t flags;                 // HPVCD_PF_*
+    int32_t num_slices;
+    hpvcd_slice_params slices[HPVCD_MAX_SLICES];
+    int32_t pps_cb_qp_offset, pps_cr_qp_offset;
+    int32_t chroma_idc, matrix;
+    int32_t crop_left, crop_top, win_w, win_h, dst_x, dst_y, copy_w, copy_h;
+    int32_t out_kind, out_width;    // output image kind / row length in pixels
+    uint64_t out_off;               // byte offset of the output image in the output pool
+    uint64_t plane_off[3];          // element offsets of Y/Cb/Cr in the sample pools
+    uint64_t res_off[3];            // element offsets of Y/Cb/Cr in the residual pool
+    uint32_t tb_begin;              // first TB of the picture in the job's TB array
+    uint32_t coeff_begin;           // first coefficient
+    uint32_t ctb_begin;             // first SAO record
+    uint32_t cell_begin;            // first cell (info/order)
+    int32_t scaling_index;          // -1 = flat
+    uint32_t suppress_any;          // some cell is TQB or PCM-with-loop-filter-disabled
+    uint32_t sao_across_slices;     // pps flag && every slice's flag (SAO neighbour gate)
+    uint32_t deblock_on;            // some slice of the picture enables deblocking
+};
+
+// One (picture, plane, CTB) work unit of the intra reconstruction wavefront.
+struct DevUnit {
+    uint32_t pic;
+    uint16_t cx, cy;        // CTB coordinates
+    uint8_t plane;          // 0 Y, 1 Cb, 2 Cr
+    uint8_t nlevels;
+    uint16_t ntb;
+    uint32_t tb_list;       // offset into unit_tbs
+    uint32_t level_list;    // offset into level_off (nlevels + 1 entries)
+    int32_t dep0, dep1;     // unit indices that must finish first (-1 = none)
+};
+
+// TB flag bits added by the runtime (upper byte of hpvcd_tb.flags).
+constexpr uint16_t TBX_HAS_RES = 1u << 8;  // residual samples exist in the residual pool
+
+}  // namespace hpvcd

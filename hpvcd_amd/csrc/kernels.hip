@@ -1,0 +1,921 @@
+// kernels.hip — gfx950 (CDNA4, wave64) reconstruction kernels.
+//
+//   k_residual<R, LOG2N>  dequantisation + inverse DCT/DST or transform skip /
+//                         bypass, rotation, RDPCM, cross-component prediction;
+//                         batched over every TB of one size class
+//                         (reference: src/transform.rs:489-910,
+//                         src/decode.rs:4724-4870, 5280-5504, 8418)
+//   k_intra<T, R>         intra prediction + residual add as a persistent
+//                         CTU wavefront; Y, Cb and Cr planes are independent
+//                         units (reference: src/intra.rs:84-374,
+//                         src/decode.rs:4641-4990, 5170-5280)
+//   k_deblock_*<T>        vertical then horizontal edge passes, luma then
+//                         chroma (reference: src/decode.rs:1774-2143,
+//                         src/deblock.rs:216-330)
+//   k_sao<T>              band / edge offset, out of place from the
+//                         deblocked planes (reference: src/decode.rs:2258,
+//                         src/sao.rs:179-470)
+//   k_rgb<T>              YUV -> RGB8/RGB16 with crop and grid placement
+//                         (reference: src/yuv.rs:187-396, src/lib.rs:1610)
+//
+// None of this is a dense contraction; all arithmetic is integer and stays
+// bit-exact with the CPU reference.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.h"
+
+namespace hpvcd {
+
+// ---------------------------------------------------------------------------
+// shared helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int64_t clampl(int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int32_t wrap16(int32_t v) { return (int32_t)(int16_t)(uint16_t)(uint32_t)v; }
+__device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+
+// Orders this wave's LDS traffic: DS operations of one wave execute in
+// order, so a compiler barrier is enough between a write and another lane's
+// read of the same location.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// HEVC transform matrix row k of the 32-point DCT, column m (H.265 8.6.4.2):
+// c(k*(2m+1) mod 128) with c(j) the normative integer approximation of
+// 64*sqrt(2)*cos(j*pi/64).
+__constant__ int8_t kCos64[33] = {64, 90, 90, 90, 89, 88, 87, 85, 83, 82, 80, 78, 75, 73, 70, 67, 64,
+                                  61, 57, 54, 50, 46, 43, 38, 36, 31, 25, 22, 18, 13, 9,  4,  0};
+__constant__ int8_t kDst4[4][4] = {{29, 55, 74, 84}, {74, 74, 0, -74}, {84, -29, -74, 55}, {55, -84, 74, -29}};
+__constant__ int16_t kLevelScale[6] = {40, 45, 51, 57, 64, 72};
+__constant__ uint8_t kBeta[52] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  6,  7,
+                                  8,  9,  10, 11, 12, 13, 14, 15, 16, 17, 18, 20, 22, 24, 26, 28, 30, 32,
+                                  34, 36, 38, 40, 42, 44, 46, 48, 50, 52, 54, 56, 58, 60, 62, 64};
+__constant__ uint8_t kTc[54] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,  0,  1,  1,  1,  1,  1,  1,  1,  1,  1,
+                                2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 5, 5, 6, 6, 7, 8, 9, 10, 11, 13, 14, 16, 18, 20, 22, 24};
+__constant__ int8_t kIntraAngle[33] = {32,  26,  21,  17,  13,  9,  5,  2,  0,  -2, -5, -9, -13, -17, -21, -26, -32,
+                                       -26, -21, -17, -13, -9, -5, -2, 0,  2,  5,  9,  13,  17,  21,  26,  32};
+__constant__ int16_t kInvAngle[15] = {-4096, -1638, -910, -630, -482, -390, -315, -256,
+                                      -315,  -390,  -482, -630, -910, -1638, -4096};
+
+__device__ __forceinline__ int dct_coef32(int k, int m) {
+    if (k == 0) return 64;
+    int a = (k * (2 * m + 1)) & 127;
+    if (a < 32) return kCos64[a];
+    if (a < 64) return -kCos64[64 - a];
+    if (a < 96) return -kCos64[a - 64];
+    return kCos64[128 - a];
+}
+
+__device__ __forceinline__ int max_range_of(int bd, bool ep) { return ep ? (bd + 6 > 15 ? bd + 6 : 15) : 15; }
+
+__device__ __forceinline__ int qpc_of(int qpi, int chroma_idc, int bd) {
+    qpi = clampi(qpi, -6 * (bd - 8), 57);
+    if (chroma_idc != 1) return qpi < 51 ? qpi : 51;
+    if (qpi < 30) return qpi;
+    if (qpi > 43) return qpi - 6;
+    const int t[14] = {29, 30, 31, 32, 33, 33, 34, 34, 35, 35, 36, 36, 37, 37};
+    return t[qpi - 30];
+}
+
+// ---------------------------------------------------------------------------
+// K1: residual (dequant + inverse transform / TS / bypass + RExt ops + CCP)
+// ---------------------------------------------------------------------------
+template <typename R, int LOG2N>
+__global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pics, const hpvcd_tb* __restrict__ tbs,
+                                                  const uint32_t* __restrict__ tb_pic,
+                                                  const uint32_t* __restrict__ list, uint32_t count,
+                                                  const uint32_t* __restrict__ coeffs,
+                                                  const uint8_t* __restrict__ scaling, R* __restrict__ res_pool,
+                                                  int ccp_pass) {
+    constexpr int N = 1 << LOG2N;
+    constexpr int NN = N * N;
+    constexpr int TPB = NN < 256 ? NN : 256;   // threads per TB
+    constexpr int TBS = 256 / TPB;             // TBs per workgroup
+    constexpr int EPT = NN / TPB;              // elements per thread
+    __shared__ int32_t blk[TBS][NN];
+    __shared__ int32_t tmp[TBS][NN];
+    __shared__ int8_t mat[N][N];
+
+    const int tid = threadIdx.x;
+    const int slot = tid / TPB, lt = tid % TPB;
+    const uint32_t i = blockIdx.x * TBS + slot;
+    const bool valid = i < count;
+
+    for (int e = tid; e < NN; e += 256) mat[e / N][e % N] = (int8_t)dct_coef32((e / N) * (32 / N), e % N);
+    for (int e = 0; e < EPT; e++) blk[slot][lt + e * TPB] = 0;
+
+    uint32_t tbi = valid ? list[i] : 0;
+    hpvcd_tb tb = tbs[tbi];
+    const DevPic& P = pics[tb_pic[tbi]];
+    const int comp = tb.comp;
+    const int bd = comp ? P.bd_c : P.bd;
+    const bool ep = (P.flags & HPVCD_PF_EXTENDED_PRECISION) != 0;
+    const bool inter = (tb.flags & HPVCD_TB_INTER) != 0;
+    const bool ts = (tb.flags & HPVCD_TB_TS) != 0;
+    const bool tqb = (tb.flags & HPVCD_TB_TQB) != 0;
+    const bool cbf = (tb.flags & HPVCD_TB_CBF) != 0;
+    const bool i16 = bd <= 8 && !ep && (comp == 0 || tb.res_scale == 0);
+    const int max_range = max_range_of(bd, ep);
+    const bool transform = cbf && !ts && !tqb;
+    const bool use_dst = transform && comp == 0 && LOG2N == 2 && !inter;
+    // RExt residual ops (src/decode.rs:4356 rext_post_ops)
+    bool rotate = false;
+    int rdpcm = -1;
+    if (cbf && (ts || tqb)) {
+        rotate = ts && !tqb && (P.flags & HPVCD_PF_TS_ROTATION) && N == 4 && !inter;
+        if (inter) {
+            if (tb.flags & HPVCD_TB_RDPCM_V) rdpcm = 1;
+            else if (tb.flags & HPVCD_TB_RDPCM_H) rdpcm = 0;
+        } else if ((P.flags & HPVCD_PF_IMPLICIT_RDPCM) && (tb.mode == 10 || tb.mode == 26)) {
+            rdpcm = tb.mode == 26 ? 1 : 0;
+        }
+    }
+    __syncthreads();
+
+    // scatter + dequantise the nonzero levels (src/transform.rs:764-910)
+    if (valid && cbf) {
+        const int log2n = LOG2N;
+        const int64_t bd_shift = bd + log2n + 10 - max_range;
+        const int max_qp = 51 + 6 * (bd - 8);
+        const int qp = clampi(tb.qp, 0, max_qp);
+        const int64_t factor = (int64_t)kLevelScale[qp % 6] * ((int64_t)1 << (qp / 6)) * 16;
+        const int64_t add = (int64_t)1 << (bd_shift - 1);
+        const int64_t cmin = -((int64_t)1 << max_range), cmax = ((int64_t)1 << max_range) - 1;
+        int tr_shift = max_range - bd - log2n;
+        if (ep && tr_shift < 0) tr_shift = 0;
+        const int32_t tr_add = tr_shift > 0 ? 1 << (tr_shift - 1) : 0;
+        const uint8_t* sc = nullptr;
+        if (!tqb && P.scaling_index >= 0 && !(ts && N != 4)) {
+            const int size_id = LOG2N - 2;
+            const int matrix_id = (comp < 2 ? comp : 2) + (inter ? 3 : 0);
+            sc = scaling + (size_t)P.scaling_index * HPVCD_SCALING_SET_BYTES +
+                 (size_id * 6 + matrix_id) * HPVCD_SCALING_ENTRY_BYTES;
+            if (sc[65]) sc = nullptr;  // flat-16 list == flat path
+        }
+        const uint32_t* co = coeffs + P.coeff_begin + tb.coeff_off;
+        for (uint32_t k = lt; k < tb.coeff_cnt; k += TPB) {
+            uint32_t word = co[k];
+            int pos = HPVCD_COEFF_POS(word);
+            int32_t level = HPVCD_COEFF_LEVEL(word);
+            int32_t v;
+            if (tqb) {
+                v = i16 ? clampi(level, -32768, 32767) : level;
+            } else {
+                int64_t f = factor;
+                if (sc) {
+                    int idx = pos;
+                    int64_t m;
+                    if (LOG2N == 2) m = sc[idx];
+                    else if (idx == 0 && LOG2N >= 4) m = sc[64];
+                    else {
+                        int yy = idx >> LOG2N, xx = idx & (N - 1);
+                        int down = LOG2N > 3 ? LOG2N - 3 : 0;
+                        m = sc[(yy >> down) * 8 + (xx >> down)];
+                    }
+                    f = (factor / 16) * m;
+                }
+                int64_t l = clampl(level, cmin, cmax);
+                v = (int32_t)clampl((l * f + add) >> bd_shift, cmin, cmax);
+                if (ts) {
+                    v = tr_shift >= 0 ? (wadd(v, tr_add) >> tr_shift) : (int32_t)((uint32_t)v << (-tr_shift));
+                    if (i16) v = wrap16(v);
+                }
+            }
+            blk[slot][pos] = v;
+        }
+    }
+    __syncthreads();
+
+    // stage 1 (columns) or rotation copy
+    const int32_t cl_min = -(1 << max_range), cl_max = (1 << max_range) - 1;
+    for (int e = 0; e < EPT; e++) {
+        const int o = lt + e * TPB;
+        const int m = o / N, c = o % N;  // tmp[m][c]
+        int32_t v = 0;
+        if (transform) {
+            if (c < tb.nx) {
+                int32_t acc = 0;
+                if (use_dst) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) acc += (int32_t)kDst4[k][m] * blk[slot][k * N + c];
+                } else {
+#pragma unroll 8
+                    for (int k = 0; k < N; k++) acc += (int32_t)mat[k][m] * blk[slot][k * N + c];
+                }
+                v = clampi((acc + 64) >> 7, cl_min, cl_max);
+            }
+        } else {
+            v = blk[slot][rotate ? NN - 1 - o : o];
+        }
+        tmp[slot][o] = v;
+    }
+    __syncthreads();
+
+    // RDPCM accumulation (src/decode.rs:8418), one thread per row/column
+    if (valid && !transform && rdpcm >= 0 && lt < N) {
+        if (rdpcm == 1) {
+            for (int y = 1; y < N; y++) {
+                int32_t v = wadd(tmp[slot][y * N + lt], tmp[slot][(y - 1) * N + lt]);
+                tmp[slot][y * N + lt] = i16 ? wrap16(v) : v;
+            }
+        } else {
+            for (int x = 1; x < N; x++) {
+                int32_t v = wadd(tmp[slot][lt * N + x], tmp[slot][lt * N + x - 1]);
+                tmp[slot][lt * N + x] = i16 ? wrap16(v) : v;
+            }
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+
+    const int shift2 = max_range + 5 - bd;
+    const int32_t add2 = 1 << (shift2 - 1);
+    const int pw = comp ? P.cw : P.w, ph = comp ? P.ch : P.h;
+    R* out = res_pool + P.res_off[comp];
+    const R* lres = res_pool + P.res_off[0];
+    for (int e = 0; e < EPT; e++) {
+        const int o = lt + e * TPB;
+        const int r = o / N, m = o % N;
+        int32_t v;
+        if (transform) {
+            int32_t acc = 0;
+            if (use_dst) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) acc += (int32_t)kDst4[k][m] * tmp[slot][r * N + k];
+            } else {
+#pragma unroll 8
+                for (int k = 0; k < N; k++) acc += (int32_t)mat[k][m] * tmp[slot][r * N + k];
+            }
+            v = wadd(acc, add2) >> shift2;
+            if (i16) v = wrap16(v);
+        } else if (cbf) {
+            v = tmp[slot][o];
+        } else {
+            v = 0;  // cross-component only
+        }
+        const int px = tb.x + m, py = tb.y + r;
+        if (px >= pw || py >= ph) continue;
+        if (ccp_pass && tb.res_scale != 0) {
+            // src/decode.rs:5488 apply_cross_comp_residual
+            int64_t ly = (int64_t)lres[(size_t)py * P.w + px];
+            int64_t sy = (ly << P.bd_c) >> P.bd;
+            int64_t delta = ((int64_t)tb.res_scale * sy) >> 3;
+            v = (int32_t)clampl((int64_t)v + delta, INT32_MIN, INT32_MAX);
+        }
+        out[(size_t)py * pw + px] = (R)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K2: intra prediction + reconstruction, persistent CTU wavefront
+// ---------------------------------------------------------------------------
+constexpr int kMaxCtb = 64;
+constexpr int kWinW = 33;   // luma cell window: 1 + 2 * 64 / 4 columns
+constexpr int kWinH = 17;   // 1 + 64 / 4 rows
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+struct IntraLds {
+    uint16_t body[kMaxCtb * kMaxCtb];
+    uint16_t top[2 * kMaxCtb + 1];
+    uint16_t left[kMaxCtb];
+    uint32_t ord[kWinH * kWinW];
+    uint32_t inf[kWinH * kWinW];
+    int32_t raw[4][132];
+    int32_t ref[4][132];
+    uint32_t unit;
+};
+
+template <typename T, typename R>
+__global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, const hpvcd_tb* __restrict__ tbs,
+                                               const DevUnit* __restrict__ units,
+                                               const uint32_t* __restrict__ ticket_units, uint32_t n_units,
+                                               const uint32_t* __restrict__ unit_tbs,
+                                               const uint16_t* __restrict__ level_off,
+                                               const uint32_t* __restrict__ cell_order,
+                                               const uint32_t* __restrict__ cell_info, T* __restrict__ samples,
+                                               const R* __restrict__ res_pool, uint32_t* counter, uint32_t* done,
+                                               uint32_t* err) {
+    __shared__ IntraLds L;
+    __shared__ R resb[kMaxCtb * kMaxCtb];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+
+    for (;;) {
+        if (tid == 0) L.unit = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t ticket = L.unit;
+        __syncthreads();
+        if (ticket >= n_units) return;
+        const uint32_t ui = ticket_units[ticket];
+        const DevUnit U = units[ui];
+        const DevPic& P = pics[U.pic];
+        const int plane = U.plane;
+        const int sw = plane ? P.sub_w_log2 : 0, sh = plane ? P.sub_h_log2 : 0;
+        const int ctb = 1 << P.ctb_log2;
+        const int ctbw = ctb >> sw, ctbh = ctb >> sh;       // unit size in plane samples
+        const int pw = plane ? P.cw : P.w, ph = plane ? P.ch : P.h;
+        const int ux0 = U.cx * ctbw, uy0 = U.cy * ctbh;     // unit origin (plane samples)
+        const int X0 = U.cx << P.ctb_log2, Y0 = U.cy << P.ctb_log2;  // luma origin
+        const int gx0 = (X0 >> 2) - 1, gy0 = (Y0 >> 2) - 1;          // cell window origin
+        const int bd = plane ? P.bd_c : P.bd;
+        const int32_t maxv = (1 << bd) - 1;
+        T* pl = samples + P.plane_off[plane];
+        const R* rp = res_pool + P.res_off[plane];
+        const bool need_info = (P.flags & (HPVCD_PF_TILES | HPVCD_PF_CONSTRAINED_INTRA)) || P.num_slices > 1;
+
+        // prefetch what does not depend on neighbours: residuals and cells
+        for (int e = tid; e < ctbw * ctbh; e += 256) {
+            int ry = e / ctbw, rx = e - ry * ctbw;
+            int px = ux0 + rx, py = uy0 + ry;
+            resb[ry * kMaxCtb + rx] = (px < pw && py < ph) ? rp[(size_t)py * pw + px] : (R)0;
+        }
+        for (int e = tid; e < kWinH * kWinW; e += 256) {
+            int wy = e / kWinW, wx = e - wy * kWinW;
+            int gx = gx0 + wx, gy = gy0 + wy;
+            bool in = gx >= 0 && gy >= 0 && gx < P.gw && gy < P.gh;
+            size_t g = (size_t)P.cell_begin + (size_t)gy * P.gw + gx;
+            L.ord[e] = in ? cell_order[g] : 0xffffffffu;
+            L.inf[e] = (in && need_info) ? cell_info[g] : 0u;
+        }
+        // wait for the left and above-right (or above) units
+        if (tid == 0) {
+            int deps[2] = {U.dep0, U.dep1};
+            for (int d = 0; d < 2; d++) {
+                if (deps[d] < 0) continue;
+                uint32_t spins = 0;
+                while (__hip_atomic_load(&done[deps[d]], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(err, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // neighbour samples: the row above (2 units wide + corner) and the
+        // column to the left
+        for (int e = tid; e < 2 * ctbw + 1; e += 256) {
+            int px = ux0 - 1 + e, py = uy0 - 1;
+            L.top[e] = (py >= 0 && px >= 0 && px < pw) ? (uint16_t)pl[(size_t)py * pw + px] : 0;
+        }
+        for (int e = tid; e < ctbh; e += 256) {
+            int px = ux0 - 1, py = uy0 + e;
+            L.left[e] = (px >= 0 && py < ph) ? (uint16_t)pl[(size_t)py * pw + px] : 0;
+        }
+        __syncthreads();
+
+        const uint32_t* tlist = unit_tbs + U.tb_list;
+        const uint16_t* lev = level_off + U.level_list;
+        for (int lv = 0; lv < U.nlevels; lv++) {
+            const int lo = lev[lv], hi = lev[lv + 1];
+            for (int j = lo + wave; j < hi; j += 4) {
+                const hpvcd_tb tb = tbs[tlist[j]];
+                const int n = 1 << tb.log2n, n2 = 2 * n, nref = 4 * n + 1;
+                const int x0 = tb.x - ux0, y0 = tb.y - uy0;
+                const uint32_t ord_cur = tb.order;
+                uint32_t cur_info = 0;
+                if (need_info) {
+                    int lx = tb.x << sw, ly = tb.y << sh;
+                    cur_info = L.inf[((ly >> 2) - gy0) * kWinW + ((lx >> 2) - gx0)];
+                }
+                // gather raw references with availability
+                // (src/decode.rs:4651 luma_avail, :4669 chroma_avail, :4693 gather)
+                for (int i = lane; i < nref; i += 64) {
+                    int rx, ry;
+                    if (i < n2) { rx = x0 - 1; ry = y0 + n2 - 1 - i; }
+                    else if (i == n2) { rx = x0 - 1; ry = y0 - 1; }
+                    else { rx = x0 + (i - n2 - 1); ry = y0 - 1; }
+                    int px = ux0 + rx, py = uy0 + ry;
+                    int32_t v = -1;
+                    if (px >= 0 && py >= 0 && px < pw && py < ph) {
+                        int lx = px << sw, ly = py << sh;
+                        int wx = (lx >> 2) - gx0, wy = (ly >> 2) - gy0;
+                        if (lx < P.w && ly < P.h && wy < kWinH && wx < kWinW && rx < (ry < 0 ? 2 * ctbw : ctbw)) {
+                            int c = wy * kWinW + wx;
+                            bool ok = L.ord[c] < ord_cur;
+                            if (ok && need_info) {
+                                uint32_t inf = L.inf[c];
+                                if ((P.flags & HPVCD_PF_TILES) && HPVCD_CELL_TILE(inf) != HPVCD_CELL_TILE(cur_info)) ok = false;
+                                if (P.num_slices > 1 && HPVCD_CELL_SLICE(inf) != HPVCD_CELL_SLICE(cur_info)) ok = false;
+                                if ((P.flags & HPVCD_PF_CONSTRAINED_INTRA) && !(inf & HPVCD_CELL_INTRA)) ok = false;
+                            }
+                            if (ok) {
+                                if (ry == -1) v = L.top[rx + 1];
+                                else if (rx == -1) v = L.left[ry];
+                                else v = L.body[ry * kMaxCtb + rx];
+                            }
+                        }
+                    }
+                    L.raw[wave][i] = v;
+                }
+                wave_sync();
+                // substitution (src/intra.rs:84): nearest available at or
+                // below each index, else the first available, else neutral
+                uint64_t mask[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    int i = lane + 64 * k;
+                    mask[k] = __ballot(i < nref && L.raw[wave][i] >= 0);
+                }
+                int first = -1;
+#pragma unroll
+                for (int k = 2; k >= 0; k--)
+                    if (mask[k]) first = 64 * k + __builtin_ctzll(mask[k]);
+                const int32_t neutral = 1 << (bd - 1);
+                int32_t sv[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    int i = lane + 64 * k;
+                    sv[k] = 0;
+                    if (i >= nref) continue;
+                    int j = -1;
+                    uint64_t m = mask[k] & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+                    if (m) j = 64 * k + 63 - __builtin_clzll(m);
+                    else {
+                        for (int kk = k - 1; kk >= 0; kk--)
+                            if (mask[kk]) { j = 64 * kk + 63 - __builtin_clzll(mask[kk]); break; }
+                    }
+                    if (j < 0) j = first;
+                    sv[k] = j >= 0 ? L.raw[wave][j] : neutral;
+                }
+                wave_sync();
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    int i = lane + 64 * k;
+                    if (i < nref) L.raw[wave][i] = sv[k];
+                }
+                wave_sync();
+                // reference filtering (src/intra.rs:151)
+                const int mode = tb.mode;
+                const bool filt_luma = plane == 0 || P.chroma_idc == 3;
+                const bool smoothing = !(P.flags & HPVCD_PF_INTRA_SMOOTHING_DISABLED);
+                bool do_filter = false;
+                if (smoothing && filt_luma && n != 4 && mode != 1) {
+                    int d26 = mode > 26 ? mode - 26 : 26 - mode, d10 = mode > 10 ? mode - 10 : 10 - mode;
+                    int min_dist = d26 < d10 ? d26 : d10;
+                    int thresh = n == 8 ? 7 : (n == 16 ? 1 : (n == 32 ? 0 : 8));
+                    do_filter = mode == 0 || min_dist > thresh;
+                }
+                const int32_t* S = L.raw[wave];
+                int32_t* F = L.raw[wave];
+                if (do_filter) {
+                    bool strong = false;
+                    const int32_t bl = S[0], tl = S[n2], tr = S[4 * n];
+                    if (plane == 0 && (P.flags & HPVCD_PF_STRONG_INTRA_SMOOTHING) && n == 32) {
+                        int thr = 1 << (bd - 5);
+                        int a = bl + tl - 2 * S[n], b = tr + tl - 2 * S[3 * n];
+                        strong = (a < 0 ? -a : a) < thr && (b < 0 ? -b : b) < thr;
+                    }
+                    for (int i = lane; i < nref; i += 64) {
+                        int32_t v;
+                        if (i == 0 || i == 4 * n) v = S[i];
+                        else if (strong) {
+                            if (i == n2) v = S[i];
+                            else if (i > n2) { int k = i - n2; v = ((n2 - k) * tl + k * tr + n) >> 6; }
+                            else { int k = n2 - i; v = ((n2 - k) * tl + k * bl + n) >> 6; }
+                        } else v = (S[i - 1] + 2 * S[i] + S[i + 1] + 2) >> 2;
+                        L.ref[wave][i] = v;
+                    }
+                    wave_sync();
+                    F = L.ref[wave];
+                }
+                // prediction (src/intra.rs:257)
+                const bool edge = plane == 0 && !(P.flags & HPVCD_PF_INTRA_BOUNDARY_FILTER_OFF);
+                const int log2n = tb.log2n;
+                int32_t dc = 0;
+                if (mode == 1) {
+                    int32_t s = 0;
+                    for (int k = lane; k < n; k += 64) s += F[n2 + 1 + k] + F[n2 - 1 - k];
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+                    dc = (s + n) >> (log2n + 1);
+                }
+                const int angle = mode >= 2 ? kIntraAngle[mode - 2] : 0;
+                const int inv = (mode >= 11 && mode <= 25) ? kInvAngle[mode - 11] : 0;
+                const bool vertical = mode >= 18;
+                const bool has_res = (tb.flags & TBX_HAS_RES) != 0;
+                for (int p = lane; p < n * n; p += 64) {
+                    const int x = p & (n - 1), y = p >> log2n;
+                    int32_t v;
+                    if (mode == 0) {
+                        int32_t h = (n - 1 - x) * F[n2 - 1 - y] + (x + 1) * F[n2 + n + 1];
+                        int32_t vv = (n - 1 - y) * F[n2 + 1 + x] + (y + 1) * F[n2 - n - 1];
+                        v = (h + vv + n) >> (log2n + 1);
+                    } else if (mode == 1) {
+                        v = dc;
+                        if (edge && n < 32) {
+                            if (x == 0 && y == 0) v = (F[n2 - 1] + 2 * dc + F[n2 + 1] + 2) >> 2;
+                            else if (y == 0) v = (F[n2 + 1 + x] + 3 * dc + 2) >> 2;
+                            else if (x == 0) v = (F[n2 - 1 - y] + 3 * dc + 2) >> 2;
+                        }
+                    } else {
+                        // main reference index j >= 0 reads main[j]; j < 0
+                        // reads the projected side reference
+                        const int a = vertical ? y : x, c = vertical ? x : y;
+                        const int pos = (a + 1) * angle, ii = pos >> 5, frac = pos & 31;
+                        auto refv = [&](int j) -> int32_t {
+                            if (j >= 0) return vertical ? F[n2 + j] : F[n2 - j];
+                            int idx = (j * inv + 128) >> 8;
+                            idx = idx > n2 ? n2 : (idx < 0 ? 0 : idx);
+                            return vertical ? F[n2 - idx] : F[n2 + idx];
+                        };
+                        int32_t r0 = refv(c + ii + 1);
+                        v = frac == 0 ? r0 : ((32 - frac) * r0 + frac * refv(c + ii + 2) + 16) >> 5;
+                        v = clampi(v, 0, maxv);
+                        if (edge && n < 32) {
+                            if (mode == 26 && x == 0) v = clampi(F[n2 + 1] + ((F[n2 - 1 - y] - F[n2]) >> 1), 0, maxv);
+                            else if (mode == 10 && y == 0) v = clampi(F[n2 - 1] + ((F[n2 + 1 + x] - F[n2]) >> 1), 0, maxv);
+                        }
+                    }
+                    const int bx = x0 + x, by = y0 + y;
+                    if (ux0 + bx < pw && uy0 + by < ph) {
+                        if (has_res) v = clampi(wadd(v, (int32_t)resb[by * kMaxCtb + bx]), 0, maxv);
+                        L.body[by * kMaxCtb + bx] = (uint16_t)v;
+                    }
+                }
+                wave_sync();
+            }
+            __syncthreads();
+        }
+        // publish the unit
+        for (int e = tid; e < ctbw * ctbh; e += 256) {
+            int ry = e / ctbw, rx = e - ry * ctbw;
+            int px = ux0 + rx, py = uy0 + ry;
+            if (px < pw && py < ph) pl[(size_t)py * pw + px] = (T)L.body[ry * kMaxCtb + rx];
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(&done[ui], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K3: deblocking (src/decode.rs:1774 serial order: luma V, luma H, chroma V,
+// chroma H; exempt TQB/PCM cells keep their input values)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t cell_of(const DevPic& P, const uint32_t* info, int x, int y) {
+    return info[P.cell_begin + (size_t)(y >> 2) * P.gw + (x >> 2)];
+}
+
+__device__ __forceinline__ bool suppressed(const DevPic& P, uint32_t inf) {
+    return (inf & HPVCD_CELL_TQB) || ((P.flags & HPVCD_PF_PCM_LF_DISABLED) && (inf & HPVCD_CELL_PCM));
+}
+
+__device__ __forceinline__ hpvcd_slice_params slice_params(const DevPic& P, uint32_t inf) {
+    int s = HPVCD_CELL_SLICE(inf);
+    if (s < P.num_slices) return P.slices[s];
+    hpvcd_slice_params d;
+    d.disabled = (P.flags & HPVCD_PF_DEBLOCK_DISABLED) ? 1 : 0;
+    d.beta_offset_div2 = 0;
+    d.tc_offset_div2 = 0;
+    d.lf_across = 1;
+    return d;
+}
+
+// bS of the edge between P sample (xp, yp) and Q sample (xq, yq), with the
+// slice / tile gate (src/decode.rs:1673, :1692, :1712)
+__device__ __forceinline__ int edge_bs(const DevPic& P, const uint32_t* info, bool vertical, int xq, int yq) {
+    if (xq >= P.w || yq >= P.h) return 0;
+    if (vertical ? xq == 0 : yq == 0) return 0;
+    uint32_t iq = cell_of(P, info, xq, yq);
+    int bs = vertical ? HPVCD_CELL_BSV(iq) : HPVCD_CELL_BSH(iq);
+    if (!bs) return 0;
+    int xp = vertical ? xq - 1 : xq, yp = vertical ? yq : yq - 1;
+    uint32_t ip = cell_of(P, info, xp, yp);
+    int sq = HPVCD_CELL_SLICE(iq);
+    bool q_across = sq < P.num_slices ? P.slices[sq].lf_across != 0 : (P.flags & HPVCD_PF_LF_ACROSS_SLICES) != 0;
+    if (!q_across && HPVCD_CELL_SLICE(ip) != (uint32_t)sq) return 0;
+    if ((P.flags & HPVCD_PF_TILES) && !(P.flags & HPVCD_PF_LF_ACROSS_TILES) && HPVCD_CELL_TILE(ip) != HPVCD_CELL_TILE(iq))
+        return 0;
+    return bs;
+}
+
+__device__ __forceinline__ int qp_at(const DevPic& P, const uint32_t* info, int px, int py) {
+    int gx = px >> 2, gy = py >> 2;
+    if (gx > P.gw - 1) gx = P.gw - 1;
+    if (gy > P.gh - 1) gy = P.gh - 1;
+    return HPVCD_CELL_QP(info[P.cell_begin + (size_t)gy * P.gw + gx]);
+}
+
+template <typename T>
+__device__ void luma_segment(T* base, ptrdiff_t step_tap, ptrdiff_t step_line, int beta, int tc, int maxv,
+                             bool wp, bool wq) {
+    // src/deblock.rs:216 luma_decision + :241 deblock_luma_segment
+    int32_t s[4][8];
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+#pragma unroll
+        for (int t = 0; t < 8; t++) s[l][t] = base[l * step_line + (t - 4) * step_tap];
+    auto dpf = [&](int l) { int v = s[l][1] - 2 * s[l][2] + s[l][3]; return v < 0 ? -v : v; };
+    auto dqf = [&](int l) { int v = s[l][6] - 2 * s[l][5] + s[l][4]; return v < 0 ? -v : v; };
+    int dp0 = dpf(0), dp3 = dpf(3), dq0 = dqf(0), dq3 = dqf(3);
+    if (dp0 + dp3 + dq0 + dq3 >= beta) return;
+    auto dsam = [&](int l, int dpq) {
+        int a = s[l][0] - s[l][3], b = s[l][4] - s[l][7], c = s[l][3] - s[l][4];
+        a = a < 0 ? -a : a; b = b < 0 ? -b : b; c = c < 0 ? -c : c;
+        return 2 * dpq < (beta >> 2) && a + b < (beta >> 3) && c < ((5 * tc + 1) >> 1);
+    };
+    const bool strong = dsam(0, dp0 + dq0) && dsam(3, dp3 + dq3);
+    const int side_thr = (beta + (beta >> 1)) >> 3;
+    const bool do_p1 = dp0 + dp3 < side_thr, do_q1 = dq0 + dq3 < side_thr;
+    const int tc2 = tc >> 1;
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+        const int32_t p3 = s[l][0], p2 = s[l][1], p1 = s[l][2], p0 = s[l][3];
+        const int32_t q0 = s[l][4], q1 = s[l][5], q2 = s[l][6], q3 = s[l][7];
+        T* b = base + l * step_line;
+        if (strong) {
+            auto sc = [&](int32_t orig, int32_t v) { return clampi(clampi(v, orig - 2 * tc, orig + 2 * tc), 0, maxv); };
+            if (wp) {
+                b[-1 * step_tap] = (T)sc(p0, (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3);
+                b[-2 * step_tap] = (T)sc(p1, (p2 + p1 + p0 + q0 + 2) >> 2);
+                b[-3 * step_tap] = (T)sc(p2, (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3);
+            }
+            if (wq) {
+                b[0] = (T)sc(q0, (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3);
+                b[1 * step_tap] = (T)sc(q1, (p0 + q0 + q1 + q2 + 2) >> 2);
+                b[2 * step_tap] = (T)sc(q2, (p0 + q0 + q1 + 3 * q2 + 2 * q3 + 4) >> 3);
+            }
+        } else {
+            const int32_t delta0 = (9 * (q0 - p0) - 3 * (q1 - p1) + 8) >> 4;
+            if ((delta0 < 0 ? -delta0 : delta0) < tc * 10) {
+                const int32_t delta = clampi(delta0, -tc, tc);
+                if (wp) b[-1 * step_tap] = (T)clampi(p0 + delta, 0, maxv);
+                if (wq) b[0] = (T)clampi(q0 - delta, 0, maxv);
+                if (do_p1 && wp) {
+                    int32_t d = clampi((((p2 + p0 + 1) >> 1) - p1 + delta) >> 1, -tc2, tc2);
+                    b[-2 * step_tap] = (T)clampi(p1 + d, 0, maxv);
+                }
+                if (do_q1 && wq) {
+                    int32_t d = clampi((((q2 + q0 + 1) >> 1) - q1 - delta) >> 1, -tc2, tc2);
+                    b[1 * step_tap] = (T)clampi(q1 + d, 0, maxv);
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_deblock_luma(const DevPic* __restrict__ pics, const uint32_t* __restrict__ info,
+                                                      T* __restrict__ samples, int vertical, uint32_t pic_base) {
+    const DevPic& P = pics[pic_base + blockIdx.y];
+    if (!P.deblock_on) return;
+    const int edge_max = vertical ? P.w : P.h, scan_max = vertical ? P.h : P.w;
+    const int n_e = edge_max >= 12 ? (edge_max - 4) / 8 : 0;
+    const int n_s = scan_max / 4;
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint32_t)(n_e * n_s)) return;
+    int e, s;
+    if (vertical) { s = idx / n_e; e = idx - s * n_e; } else { e = idx / n_s; s = idx - e * n_s; }
+    const int edge = 8 * (e + 1), scan = 4 * s, mid = scan + 1;
+    const int xq = vertical ? edge : mid, yq = vertical ? mid : edge;
+    const int bs = edge_bs(P, info, vertical, xq, yq);
+    if (!bs) return;
+    const uint32_t iq = cell_of(P, info, xq, yq);
+    const hpvcd_slice_params sd = slice_params(P, iq);
+    if (sd.disabled) return;
+    const int qp_p = vertical ? qp_at(P, info, edge - 1, mid) : qp_at(P, info, mid, edge - 1);
+    const int qp_q = HPVCD_CELL_QP(iq);
+    const int avg = (qp_p + qp_q + 1) >> 1;
+    const int bi = clampi(avg + sd.beta_offset_div2 * 2, 0, 51);
+    const int ti = clampi(avg + 2 * (bs - 1) + sd.tc_offset_div2 * 2, 0, 53);
+    const int beta = kBeta[bi] << (P.bd - 8), tc = kTc[ti] << (P.bd - 8);
+    if (!tc) return;
+    T* pl = samples + P.plane_off[0];
+    bool wp = true, wq = true;
+    if (P.suppress_any) {
+        const int xp = vertical ? edge - 1 : scan, yp = vertical ? scan : edge - 1;
+        wp = !suppressed(P, cell_of(P, info, xp, yp));
+        wq = !suppressed(P, cell_of(P, info, vertical ? edge : scan, vertical ? scan : edge));
+    }
+    if (vertical) luma_segment<T>(pl + (size_t)scan * P.w + edge, 1, P.w, beta, tc, (1 << P.bd) - 1, wp, wq);
+    else luma_segment<T>(pl + (size_t)edge * P.w + scan, P.w, 1, beta, tc, (1 << P.bd) - 1, wp, wq);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_deblock_chroma(const DevPic* __restrict__ pics,
+                                                        const uint32_t* __restrict__ info, T* __restrict__ samples,
+                                                        int vertical, uint32_t pic_base) {
+    const DevPic& P = pics[pic_base + blockIdx.y];
+    if (!P.deblock_on || P.chroma_idc == 0) return;
+    const int cw = P.cw, ch = P.ch;
+    const int edge_max = vertical ? cw : ch, scan_max = vertical ? ch : cw;
+    const int n_e = edge_max >= 10 ? (edge_max - 2) / 8 : 0;
+    const int n_s = scan_max / 4;
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint32_t)(n_e * n_s)) return;
+    int e, s;
+    if (vertical) { s = idx / n_e; e = idx - s * n_e; } else { e = idx / n_s; s = idx - e * n_s; }
+    const int edge = 8 * (e + 1), scan = 4 * s, mid = scan + 1;
+    const int sw = P.sub_w_log2, sh = P.sub_h_log2;
+    const int lex = vertical ? edge << sw : mid << sw, ley = vertical ? mid << sh : edge << sh;
+    const int bs = edge_bs(P, info, vertical, lex, ley);
+    if (bs < 2) return;
+    const int lqx = lex < P.w - 1 ? lex : P.w - 1, lqy = ley < P.h - 1 ? ley : P.h - 1;
+    const hpvcd_slice_params sd = slice_params(P, cell_of(P, info, lqx, lqy));
+    if (sd.disabled) return;
+    int ppx, ppy;
+    if (vertical) { ppx = lex > 0 ? lex - 1 : 0; if (ppx > P.w - 1) ppx = P.w - 1; ppy = lqy; }
+    else { ppx = lqx; ppy = ley > 0 ? ley - 1 : 0; if (ppy > P.h - 1) ppy = P.h - 1; }
+    const int avg = (qp_at(P, info, ppx, ppy) + qp_at(P, info, lqx, lqy) + 1) >> 1;
+    const int maxv = (1 << P.bd_c) - 1;
+    for (int pln = 0; pln < 2; pln++) {
+        const int off = pln == 0 ? P.pps_cb_qp_offset : P.pps_cr_qp_offset;
+        const int qpc = qpc_of(avg + off, P.chroma_idc, P.bd_c);
+        const int tci = clampi(qpc + 2 + sd.tc_offset_div2 * 2, 0, 53);
+        const int tcc = kTc[tci] << (P.bd_c - 8);
+        if (!tcc) continue;
+        T* pl = samples + P.plane_off[1 + pln];
+        for (int l = scan; l < scan + 4; l++) {
+            T *pp, *qq;
+            int32_t p0, p1, q0, q1;
+            if (vertical) {
+                T* r = pl + (size_t)l * cw + edge;
+                pp = r - 1; qq = r;
+                p0 = r[-1]; p1 = r[-2]; q0 = r[0]; q1 = r[1];
+            } else {
+                T* r = pl + (size_t)edge * cw + l;
+                pp = r - cw; qq = r;
+                p0 = r[-cw]; p1 = r[-2 * cw]; q0 = r[0]; q1 = r[cw];
+            }
+            const int32_t delta = clampi(((q0 - p0) * 4 + p1 - q1 + 4) >> 3, -tcc, tcc);
+            if (!delta) continue;
+            bool wp = true, wq = true;
+            if (P.suppress_any) {
+                int cpx = vertical ? edge - 1 : l, cpy = vertical ? l : edge - 1;
+                int cqx = vertical ? edge : l, cqy = vertical ? l : edge;
+                wp = !suppressed(P, cell_of(P, info, cpx << sw, cpy << sh));
+                wq = !suppressed(P, cell_of(P, info, cqx << sw, cqy << sh));
+            }
+            if (wp) *pp = (T)clampi(p0 + delta, 0, maxv);
+            if (wq) *qq = (T)clampi(q0 - delta, 0, maxv);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K4: SAO, out of place (src/decode.rs:2258; gated EO src/sao.rs:433)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, const uint32_t* __restrict__ info,
+                                             const hpvcd_sao* __restrict__ sao, const T* __restrict__ src_pool,
+                                             T* __restrict__ dst_pool, uint32_t pic_base) {
+    const DevPic& P = pics[pic_base + blockIdx.y];
+    const int c = blockIdx.z;
+    if (c > 0 && P.chroma_idc == 0) return;
+    const int pw = c ? P.cw : P.w, ph = c ? P.ch : P.h;
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint32_t)(pw * ph)) return;
+    const int y = idx / pw, x = idx - y * pw;
+    const T* src = src_pool + P.plane_off[c];
+    T* dst = dst_pool + P.plane_off[c];
+    const int32_t s = src[idx];
+    int32_t out = s;
+    const bool on = c == 0 ? (P.flags & HPVCD_PF_SAO_LUMA) != 0 : ((P.flags & HPVCD_PF_SAO_CHROMA) != 0 && P.cw && P.ch);
+    const int sw = c ? P.sub_w_log2 : 0, sh = c ? P.sub_h_log2 : 0;
+    if (on) {
+        const int rx = (x << sw) >> P.ctb_log2, ry = (y << sh) >> P.ctb_log2;
+        const hpvcd_sao& S = sao[P.ctb_begin + ry * P.ctb_cols + rx];
+        const int type = S.type_idx[c];
+        const int bd = c ? P.bd_c : P.bd;
+        const int32_t maxv = (1 << bd) - 1;
+        const uint32_t ic = cell_of(P, info, x << sw, y << sh);
+        if (type == 1) {
+            uint8_t rel = (uint8_t)((uint8_t)(s >> (bd - 5)) - S.band_pos[c]);
+            if (rel < 4) out = clampi(s + S.offsets[c][rel], 0, maxv);
+        } else if (type == 2) {
+            const int ec = S.eo_class[c];
+            const int dx = ec == 1 ? 0 : 1, dy = ec == 0 ? 0 : (ec == 3 ? -1 : 1);
+            const int x1 = x + dx, y1 = y + dy, x2 = x - dx, y2 = y - dy;
+            bool ok = x1 >= 0 && y1 >= 0 && x1 < pw && y1 < ph && x2 >= 0 && y2 >= 0 && x2 < pw && y2 < ph;
+            if (ok) {
+                // src/sao.rs:378 luma_neighbor_ok (center exemption, slice and tile gates)
+                if (suppressed(P, ic)) ok = false;
+                const bool across_slices = P.sao_across_slices != 0;
+                const bool across_tiles = !(P.flags & HPVCD_PF_TILES) || (P.flags & HPVCD_PF_LF_ACROSS_TILES);
+                if (ok && (!across_slices || !across_tiles)) {
+                    uint32_t i1 = cell_of(P, info, x1 << sw, y1 << sh), i2 = cell_of(P, info, x2 << sw, y2 << sh);
+                    if (!across_slices && (HPVCD_CELL_SLICE(i1) != HPVCD_CELL_SLICE(ic) || HPVCD_CELL_SLICE(i2) != HPVCD_CELL_SLICE(ic))) ok = false;
+                    if (!across_tiles && (HPVCD_CELL_TILE(i1) != HPVCD_CELL_TILE(ic) || HPVCD_CELL_TILE(i2) != HPVCD_CELL_TILE(ic))) ok = false;
+                }
+            }
+            if (ok) {
+                const int32_t n1 = src[y1 * pw + x1], n2 = src[y2 * pw + x2];
+                const int ei = ((s > n1) - (s < n1)) + ((s > n2) - (s < n2)) + 2;
+                const int32_t o = ei == 0 ? S.offsets[c][0] : ei == 1 ? S.offsets[c][1] : ei == 3 ? S.offsets[c][2] : ei == 4 ? S.offsets[c][3] : 0;
+                if (o) out = clampi(s + o, 0, maxv);
+            }
+        }
+        if (P.suppress_any && suppressed(P, ic)) out = s;
+    }
+    dst[idx] = (T)out;
+}
+
+// ---------------------------------------------------------------------------
+// K5: YUV -> RGB with crop window and grid placement (src/yuv.rs:187-396)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_rgb(const DevPic* __restrict__ pics, const T* __restrict__ pool,
+                                             uint8_t* __restrict__ out_pool, uint32_t pic_base) {
+    const DevPic& P = pics[pic_base + blockIdx.y];
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint32_t)(P.copy_w * P.copy_h)) return;
+    const int yy = idx / P.copy_w, xx = idx - yy * P.copy_w;
+    const int w = P.w, h = P.h;
+    const int x0 = P.crop_left < w - 1 ? P.crop_left : w - 1, y0 = P.crop_top < h - 1 ? P.crop_top : h - 1;
+    const int src_y = y0 + yy, src_x = x0 + xx;
+    const int y_row = src_y < h - 1 ? src_y : h - 1, x_col = src_x < w - 1 ? src_x : w - 1;
+    const int bd = P.bd;
+    const int64_t scale = (int64_t)1 << (bd - 8);
+    const bool full = (P.flags & HPVCD_PF_FULL_RANGE) != 0;
+    const int64_t ky = full ? 8192 : 9539, kc = full ? 8192 : 9326;
+    const int64_t y_black = full ? 0 : 16 * scale, neutral = 128 * scale;
+    const int64_t L = pool[P.plane_off[0] + (size_t)y_row * w + x_col];
+    const bool mono = P.chroma_idc == 0;
+    int64_t v[3];
+    if (mono) {
+        v[0] = (ky * (L - y_black) + 4096) >> 13;
+    } else {
+        const int cw = P.cw, ch = P.ch;
+        const int cr = (src_y >> P.sub_h_log2) < ch - 1 ? (src_y >> P.sub_h_log2) : ch - 1;
+        const int cc = (src_x >> P.sub_w_log2) < cw - 1 ? (src_x >> P.sub_w_log2) : cw - 1;
+        const int64_t U = pool[P.plane_off[1] + (size_t)cr * cw + cc], V = pool[P.plane_off[2] + (size_t)cr * cw + cc];
+        if (P.matrix == 0) {
+            if (full) { v[0] = V; v[1] = L; v[2] = U; }
+            else {
+                v[0] = (kc * (V - y_black) + 4096) >> 13;
+                v[1] = (ky * (L - y_black) + 4096) >> 13;
+                v[2] = (kc * (U - y_black) + 4096) >> 13;
+            }
+        } else if (P.matrix == 8) {
+            const int64_t cg = U - neutral, co = V - neutral, t = L - cg;
+            v[0] = t + co; v[1] = L + cg; v[2] = t - co;
+        } else {
+            int64_t m0, m1, m2, m3;
+            if (P.matrix == 5 || P.matrix == 6) { m0 = 11485; m1 = -2819; m2 = -5851; m3 = 14516; }
+            else if (P.matrix == 9) { m0 = 12080; m1 = -1348; m2 = -4681; m3 = 17546; }
+            else { m0 = 12901; m1 = -1534; m2 = -3835; m3 = 15201; }
+            const int64_t crr = (m0 * kc + 4096) >> 13, cbg = (m1 * kc + 4096) >> 13;
+            const int64_t crg = (m2 * kc + 4096) >> 13, cbb = (m3 * kc + 4096) >> 13;
+            const int64_t yt = ky * (L - y_black), cbc = U - neutral, crc = V - neutral;
+            v[0] = (yt + crr * crc + 4096) >> 13;
+            v[1] = (yt + cbg * cbc + crg * crc + 4096) >> 13;
+            v[2] = (yt + cbb * cbc + 4096) >> 13;
+        }
+    }
+    const int chn = mono ? 1 : 3;
+    const int64_t cmax = bd == 8 ? 255 : (1 << bd) - 1;
+    const size_t o = ((size_t)(P.dst_y + yy) * P.out_width + (P.dst_x + xx)) * chn;
+    if (P.out_kind == HPVCD_OUT_RGB16 || P.out_kind == HPVCD_OUT_LUMA16) {
+        uint16_t* d = reinterpret_cast<uint16_t*>(out_pool + P.out_off);
+        for (int k = 0; k < chn; k++) d[o + k] = (uint16_t)clampl(v[k], 0, cmax);
+    } else {
+        uint8_t* d = out_pool + P.out_off;
+        const int sh = bd == 8 ? 0 : bd - 8;
+        for (int k = 0; k < chn; k++) d[o + k] = (uint8_t)((uint16_t)clampl(v[k], 0, cmax) >> sh);
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// explicit instantiations (sample type x residual type)
+// ---------------------------------------------------------------------------
+#define HPVCD_INST_RES(R)                                                                                        \
+    template __global__ void k_residual<R, 2>(const DevPic*, const hpvcd_tb*, const uint32_t*, const uint32_t*,  \
+                                              uint32_t, const uint32_t*, const uint8_t*, R*, int);               \
+    template __global__ void k_residual<R, 3>(const DevPic*, const hpvcd_tb*, const uint32_t*, const uint32_t*,  \
+                                              uint32_t, const uint32_t*, const uint8_t*, R*, int);               \
+    template __global__ void k_residual<R, 4>(const DevPic*, const hpvcd_tb*, const uint32_t*, const uint32_t*,  \
+                                              uint32_t, const uint32_t*, const uint8_t*, R*, int);               \
+    template __global__ void k_residual<R, 5>(const DevPic*, const hpvcd_tb*, const uint32_t*, const uint32_t*,  \
+                                              uint32_t, const uint32_t*, const uint8_t*, R*, int);
+HPVCD_INST_RES(int16_t)
+HPVCD_INST_RES(int32_t)
+
+#define HPVCD_INST_INTRA(T, R)                                                                                   \
+    template __global__ void k_intra<T, R>(const DevPic*, const hpvcd_tb*, const DevUnit*, const uint32_t*,      \
+                                           uint32_t, const uint32_t*, const uint16_t*, const uint32_t*,           \
+                                           const uint32_t*, T*, const R*, uint32_t*, uint32_t*, uint32_t*);
+HPVCD_INST_INTRA(uint8_t, int16_t)
+HPVCD_INST_INTRA(uint8_t, int32_t)
+HPVCD_INST_INTRA(uint16_t, int16_t)
+HPVCD_INST_INTRA(uint16_t, int32_t)
+
+#define HPVCD_INST_T(T)                                                                                          \
+    template __global__ void k_deblock_luma<T>(const DevPic*, const uint32_t*, T*, int, uint32_t);               \
+    template __global__ void k_deblock_chroma<T>(const DevPic*, const uint32_t*, T*, int, uint32_t);             \
+    template __global__ void k_sao<T>(const DevPic*, const uint32_t*, const hpvcd_sao*, const T*, T*, uint32_t); \
+    template __global__ void k_rgb<T>(const DevPic*, const T*, uint8_t*, uint32_t);
+HPVCD_INST_T(uint8_t)
+HPVCD_INST_T(uint16_t)
+
+}  // namespace hpvcd

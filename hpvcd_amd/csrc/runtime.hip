@@ -1,0 +1,857 @@
+// runtime.hip — host runtime of the reconstruction back end: record
+// validation and preparation (availability stamps, wavefront units and
+// dependency levels, residual work lists), device pools, the per-stage
+// launch chain (batched, plain per-picture, or replayed from a hipGraph) and
+// the C ABI of include/hpvcd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.hip"
+
+namespace hpvcd {
+
+static thread_local std::string g_last_error;
+
+hpvcd_status set_error(hpvcd_status st, const char* msg) {
+    g_last_error = msg ? msg : "";
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            std::string m_ = std::string(#expr) + ": " + hipGetErrorString(e_);       \
+            return set_error(HPVCD_E_DEVICE, m_.c_str());                              \
+        }                                                                              \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// validation: every index a kernel dereferences is checked here
+// ---------------------------------------------------------------------------
+hpvcd_status validate_batch(const hpvcd_batch& b) {
+    if (b.num_pictures && !b.pictures) return set_error(HPVCD_E_INVALID, "pictures is null");
+    if (b.num_tbs && !b.tbs) return set_error(HPVCD_E_INVALID, "tbs is null");
+    if (b.num_coeffs && !b.coeffs) return set_error(HPVCD_E_INVALID, "coeffs is null");
+    if (b.num_ctbs && !b.sao) return set_error(HPVCD_E_INVALID, "sao is null");
+    if (b.num_cells && !b.cell_info) return set_error(HPVCD_E_INVALID, "cell_info is null");
+    if (b.num_outputs && !b.outputs) return set_error(HPVCD_E_INVALID, "outputs is null");
+    if (b.num_scaling && !b.scaling) return set_error(HPVCD_E_INVALID, "scaling is null");
+    for (uint32_t o = 0; o < b.num_outputs; o++) {
+        const hpvcd_output& O = b.outputs[o];
+        if (O.width <= 0 || O.height <= 0 || O.width > 65536 || O.height > 65536)
+            return set_error(HPVCD_E_BAD_DIMENSIONS, "output image size");
+        if (O.kind > HPVCD_OUT_LUMA16) return set_error(HPVCD_E_INVALID, "output kind");
+    }
+    for (uint32_t p = 0; p < b.num_pictures; p++) {
+        const hpvcd_picture& P = b.pictures[p];
+        if (P.width <= 0 || P.height <= 0 || P.width > 16384 || P.height > 16384 || (P.width & 7) || (P.height & 7))
+            return set_error(HPVCD_E_BAD_DIMENSIONS, "picture size must be a positive multiple of 8");
+        if (P.chroma_idc > 3) return set_error(HPVCD_E_UNSUPPORTED_CHROMA, "chroma_idc");
+        if (P.bit_depth_luma < 8 || P.bit_depth_luma > 12 || P.bit_depth_chroma < 8 || P.bit_depth_chroma > 12)
+            return set_error(HPVCD_E_UNSUPPORTED_BITDEPTH, "bit depth must be 8..12");
+        if ((P.bit_depth_luma > 8) != (P.bit_depth_chroma > 8))
+            return set_error(HPVCD_E_UNSUPPORTED_BITDEPTH, "luma and chroma storage classes differ");
+        if (P.log2_ctb < 4 || P.log2_ctb > 6) return set_error(HPVCD_E_PARAMSET, "log2_ctb must be 4..6");
+        if (P.num_slices > HPVCD_MAX_SLICES) return set_error(HPVCD_E_INVALID, "num_slices");
+        if ((uint64_t)P.tb_begin + P.tb_count > b.num_tbs) return set_error(HPVCD_E_INVALID, "tb range");
+        if ((uint64_t)P.coeff_begin + P.coeff_count > b.num_coeffs) return set_error(HPVCD_E_INVALID, "coeff range");
+        int ctb = 1 << P.log2_ctb;
+        uint64_t nctb = (uint64_t)((P.width + ctb - 1) / ctb) * ((P.height + ctb - 1) / ctb);
+        if ((uint64_t)P.ctb_begin + nctb > b.num_ctbs) return set_error(HPVCD_E_INVALID, "sao range");
+        uint64_t ncell = (uint64_t)((P.width + 3) / 4) * ((P.height + 3) / 4);
+        if ((uint64_t)P.cell_begin + ncell > b.num_cells) return set_error(HPVCD_E_INVALID, "cell range");
+        if (P.scaling_index >= 0 && (uint32_t)P.scaling_index >= b.num_scaling)
+            return set_error(HPVCD_E_INVALID, "scaling_index");
+        if (P.out_index >= b.num_outputs) return set_error(HPVCD_E_INVALID, "out_index");
+        const hpvcd_output& O = b.outputs[P.out_index];
+        if (P.copy_w < 0 || P.copy_h < 0 || P.copy_w > P.win_w || P.copy_h > P.win_h || P.dst_x < 0 || P.dst_y < 0 ||
+            (int64_t)P.dst_x + P.copy_w > O.width || (int64_t)P.dst_y + P.copy_h > O.height)
+            return set_error(HPVCD_E_INVALID, "picture window placement outside its output image");
+        if ((P.chroma_idc == 0) != (O.channels == 1)) return set_error(HPVCD_E_INVALID, "output channels");
+        bool wide = O.kind == HPVCD_OUT_RGB16 || O.kind == HPVCD_OUT_LUMA16;
+        if (wide && P.bit_depth_luma == 8) return set_error(HPVCD_E_INVALID, "16-bit output of an 8-bit picture");
+        int sw = sub_w_of(P.chroma_idc), sh = sub_h_of(P.chroma_idc);
+        int cw = chroma_w(P), ch = chroma_h(P);
+        int64_t last_ctb[3] = {-1, -1, -1};
+        int ctb_cols = (P.width + ctb - 1) / ctb;
+        for (uint32_t t = 0; t < P.tb_count; t++) {
+            const hpvcd_tb& T = b.tbs[P.tb_begin + t];
+            if (T.log2n < 2 || T.log2n > 5) return set_error(HPVCD_E_INVALID, "tb log2 size");
+            if (T.comp <= 2) {
+                int lx = T.comp ? T.x * sub_w_of(P.chroma_idc) : T.x, ly = T.comp ? T.y * sub_h_of(P.chroma_idc) : T.y;
+                int64_t r = (int64_t)(ly >> P.log2_ctb) * ctb_cols + (lx >> P.log2_ctb);
+                if (r < last_ctb[T.comp]) return set_error(HPVCD_E_INVALID, "tbs not in CTB raster order");
+                last_ctb[T.comp] = r;
+            }
+            if (T.comp > 2 || (T.comp && !P.chroma_idc)) return set_error(HPVCD_E_INVALID, "tb component");
+            int n = 1 << T.log2n;
+            int pw = T.comp ? cw : P.width, ph = T.comp ? ch : P.height;
+            int ctbw = T.comp ? ctb / sw : ctb, ctbh = T.comp ? ctb / sh : ctb;
+            if (n > ctbw || n > ctbh || (T.x % ctbw) + n > ctbw || (T.y % ctbh) + n > ctbh)
+                return set_error(HPVCD_E_INVALID, "tb crosses a CTB boundary");
+            if (T.x >= pw || T.y >= ph) return set_error(HPVCD_E_INVALID, "tb outside its plane");
+            if ((T.x & 3) || (T.y & 3)) return set_error(HPVCD_E_INVALID, "tb not 4-aligned");
+            if (T.mode > 34) return set_error(HPVCD_E_INVALID, "intra mode");
+            if (T.res_scale && (T.comp == 0 || P.chroma_idc != 3)) return set_error(HPVCD_E_INVALID, "res_scale");
+            if (T.flags & HPVCD_TB_CBF) {
+                if ((uint64_t)T.coeff_off + T.coeff_cnt > P.coeff_count) return set_error(HPVCD_E_INVALID, "tb coeff range");
+                for (uint32_t k = 0; k < T.coeff_cnt; k++) {
+                    uint32_t w = b.coeffs[P.coeff_begin + T.coeff_off + k];
+                    if (HPVCD_COEFF_POS(w) >= (uint32_t)(n * n)) return set_error(HPVCD_E_INVALID, "coefficient position");
+                }
+            }
+        }
+    }
+    return HPVCD_OK;
+}
+
+}  // namespace hpvcd
+
+using namespace hpvcd;
+
+// ---------------------------------------------------------------------------
+// device context
+// ---------------------------------------------------------------------------
+struct hpvcd_gpu {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    int occ_intra[4] = {0, 0, 0, 0};   // blocks per CU for k_intra<T,R> (T16*2 + R32)
+};
+
+namespace {
+
+struct DevBuf {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+    template <typename X>
+    X* take(size_t count) {
+        size_t off = (used + 255) & ~(size_t)255;
+        used = off + count * sizeof(X);
+        return reinterpret_cast<X*>(base + off);
+    }
+};
+
+template <typename X>
+size_t bytes_of(size_t n) { return ((n * sizeof(X)) + 255) & ~(size_t)255; }
+
+struct ClassJob {
+    bool samp16 = false, res32 = false;
+    std::vector<uint32_t> batch_pics;     // batch picture index per class-local picture
+    // host-prepared arrays
+    std::vector<DevPic> pics;
+    std::vector<hpvcd_tb> tbs;
+    std::vector<uint32_t> tb_pic, coeffs, cells, order;
+    std::vector<hpvcd_sao> sao;
+    std::vector<DevUnit> units;
+    std::vector<uint32_t> tickets, plain_tickets, unit_tbs;
+    std::vector<uint16_t> level_off;
+    std::vector<uint32_t> res_list[2][4];            // [pass][log2-2], batched order
+    std::vector<uint32_t> pic_units_begin;            // plain_tickets segment per picture (npics+1)
+    std::vector<uint32_t> pic_res_begin[2][4];        // per picture list segments (npics+1)
+    size_t plane_elems = 0, res_elems = 0;
+    uint32_t max_luma_v = 0, max_luma_h = 0, max_chroma_v = 0, max_chroma_h = 0, max_plane = 0, max_copy = 0;
+    // device pointers
+    DevPic* d_pics = nullptr;
+    hpvcd_tb* d_tbs = nullptr;
+    uint32_t *d_tb_pic = nullptr, *d_coeffs = nullptr, *d_cells = nullptr, *d_order = nullptr;
+    hpvcd_sao* d_sao = nullptr;
+    DevUnit* d_units = nullptr;
+    uint32_t *d_tickets = nullptr, *d_plain_tickets = nullptr, *d_unit_tbs = nullptr;
+    uint16_t* d_level_off = nullptr;
+    uint32_t* d_res_list[2][4] = {};
+    void *d_planes_a = nullptr, *d_planes_b = nullptr, *d_res = nullptr;
+    uint32_t* d_sync = nullptr;   // [0] err, [1..npics+1] counters, then done flags
+    uint32_t sync_words = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_stop = -1;
+};
+
+}  // namespace
+
+struct hpvcd_job {
+    hpvcd_gpu* g = nullptr;
+    std::vector<ClassJob> classes;
+    std::vector<hpvcd_output> outputs;
+    std::vector<size_t> out_off;
+    DevBuf mem;
+    uint8_t* d_out = nullptr;
+    uint8_t* d_scaling = nullptr;
+    size_t out_bytes = 0;
+    int last_stop = HPVCD_STAGE_RGB;
+    std::vector<uint32_t> pic_class, pic_local;   // batch picture -> (class, local index)
+};
+
+namespace {
+
+// Availability stamps, units, levels and residual lists for one class.
+hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<size_t>& out_off) {
+    const uint32_t np = (uint32_t)C.batch_pics.size();
+    C.pics.resize(np);
+    size_t plane_elems = 0;
+    uint32_t unit_base = 0;
+    C.pic_units_begin.assign(np + 1, 0);
+    for (int pass = 0; pass < 2; pass++)
+        for (int s = 0; s < 4; s++) C.pic_res_begin[pass][s].assign(np + 1, 0);
+    std::vector<std::vector<uint32_t>> per_pic_res[2][4];
+    for (int pass = 0; pass < 2; pass++)
+        for (int s = 0; s < 4; s++) per_pic_res[pass][s].resize(np);
+    struct UnitKey { uint32_t t, pic, plane, idx; };
+    std::vector<UnitKey> keys;
+
+    for (uint32_t lp = 0; lp < np; lp++) {
+        const hpvcd_picture& P = b.pictures[C.batch_pics[lp]];
+        DevPic& D = C.pics[lp];
+        std::memset(&D, 0, sizeof(D));
+        D.w = P.width; D.h = P.height;
+        D.cw = chroma_w(P); D.ch = chroma_h(P);
+        D.gw = (P.width + 3) / 4; D.gh = (P.height + 3) / 4;
+        D.sub_w_log2 = sub_w_of(P.chroma_idc) == 2 ? 1 : 0;
+        D.sub_h_log2 = sub_h_of(P.chroma_idc) == 2 ? 1 : 0;
+        D.ctb_log2 = P.log2_ctb;
+        int ctb = 1 << P.log2_ctb;
+        D.ctb_cols = (P.width + ctb - 1) / ctb; D.ctb_rows = (P.height + ctb - 1) / ctb;
+        D.bd = P.bit_depth_luma; D.bd_c = P.bit_depth_chroma;
+        D.flags = P.flags;
+        D.num_slices = P.num_slices;
+        std::memcpy(D.slices, P.slices, sizeof(D.slices));
+        D.pps_cb_qp_offset = P.pps_cb_qp_offset; D.pps_cr_qp_offset = P.pps_cr_qp_offset;
+        D.chroma_idc = P.chroma_idc; D.matrix = P.matrix;
+        D.crop_left = P.crop_left; D.crop_top = P.crop_top; D.win_w = P.win_w; D.win_h = P.win_h;
+        D.dst_x = P.dst_x; D.dst_y = P.dst_y; D.copy_w = P.copy_w; D.copy_h = P.copy_h;
+        D.out_kind = b.outputs[P.out_index].kind;
+        D.out_width = b.outputs[P.out_index].width;
+        D.out_off = out_off[P.out_index];
+        size_t ny = (size_t)D.w * D.h, nc = (size_t)D.cw * D.ch;
+        D.plane_off[0] = plane_elems;
+        D.plane_off[1] = plane_elems + ny;
+        D.plane_off[2] = plane_elems + ny + nc;
+        D.res_off[0] = D.plane_off[0]; D.res_off[1] = D.plane_off[1]; D.res_off[2] = D.plane_off[2];
+        plane_elems += ny + 2 * nc;
+        D.tb_begin = (uint32_t)C.tbs.size();
+        D.coeff_begin = (uint32_t)C.coeffs.size();
+        D.ctb_begin = (uint32_t)C.sao.size();
+        D.cell_begin = (uint32_t)C.cells.size();
+        D.scaling_index = P.scaling_index;
+        bool all_across = true, deb = false;
+        for (int s = 0; s < std::max<int>(1, P.num_slices); s++) {
+            if (P.num_slices) { all_across &= P.slices[s].lf_across != 0; deb |= !P.slices[s].disabled; }
+            else deb = !(P.flags & HPVCD_PF_DEBLOCK_DISABLED);
+        }
+        D.sao_across_slices = (P.flags & HPVCD_PF_LF_ACROSS_SLICES) && all_across;
+        D.deblock_on = deb;
+        C.max_luma_v = std::max<uint32_t>(C.max_luma_v, (D.w >= 12 ? (D.w - 4) / 8 : 0) * (D.h / 4));
+        C.max_luma_h = std::max<uint32_t>(C.max_luma_h, (D.h >= 12 ? (D.h - 4) / 8 : 0) * (D.w / 4));
+        C.max_chroma_v = std::max<uint32_t>(C.max_chroma_v, (D.cw >= 10 ? (D.cw - 2) / 8 : 0) * (D.ch / 4));
+        C.max_chroma_h = std::max<uint32_t>(C.max_chroma_h, (D.ch >= 10 ? (D.ch - 2) / 8 : 0) * (D.cw / 4));
+        C.max_plane = std::max<uint32_t>(C.max_plane, (uint32_t)ny);
+        C.max_copy = std::max<uint32_t>(C.max_copy, (uint32_t)(P.copy_w * P.copy_h));
+
+        // copy records
+        const hpvcd_tb* src_tb = b.tbs + P.tb_begin;
+        C.coeffs.insert(C.coeffs.end(), b.coeffs + P.coeff_begin, b.coeffs + P.coeff_begin + P.coeff_count);
+        uint32_t nctb = (uint32_t)(D.ctb_cols * D.ctb_rows);
+        C.sao.insert(C.sao.end(), b.sao + P.ctb_begin, b.sao + P.ctb_begin + nctb);
+        uint32_t ncell = (uint32_t)(D.gw * D.gh);
+        C.cells.insert(C.cells.end(), b.cell_info + P.cell_begin, b.cell_info + P.cell_begin + ncell);
+        uint32_t suppress = 0;
+        for (uint32_t g = 0; g < ncell; g++) {
+            uint32_t inf = b.cell_info[P.cell_begin + g];
+            if ((inf & HPVCD_CELL_TQB) || ((P.flags & HPVCD_PF_PCM_LF_DISABLED) && (inf & HPVCD_CELL_PCM))) suppress = 1;
+        }
+        D.suppress_any = suppress;
+
+        // availability stamps: luma TB k gets stamp k (>=1) painted over its
+        // cells; a chroma TB sees every luma TB decoded before it.
+        size_t cell0 = C.order.size();
+        C.order.resize(cell0 + ncell, 0xffffffffu);
+        uint32_t stamp = 0;
+        uint32_t tb0 = (uint32_t)C.tbs.size();
+        for (uint32_t t = 0; t < P.tb_count; t++) {
+            hpvcd_tb T = src_tb[t];
+            T.flags &= 0xffu;
+            if (T.comp == 0) {
+                stamp++;
+                T.order = stamp;
+                int n = 1 << T.log2n;
+                for (int yy = T.y; yy < std::min(T.y + n, D.h); yy += 4)
+                    for (int xx = T.x; xx < std::min(T.x + n, D.w); xx += 4)
+                        C.order[cell0 + (size_t)(yy / 4) * D.gw + xx / 4] = stamp;
+            } else {
+                T.order = stamp + 1;
+            }
+            if ((T.flags & HPVCD_TB_CBF) || T.res_scale != 0) T.flags |= TBX_HAS_RES;
+            C.tbs.push_back(T);
+            C.tb_pic.push_back(lp);
+        }
+
+        // wavefront units: (plane, CTB) in decode order of their TBs
+        int nplanes = P.chroma_idc ? 3 : 1;
+        uint32_t nunits = (uint32_t)(nplanes * D.ctb_cols * D.ctb_rows);
+        std::vector<std::vector<uint32_t>> utbs(nunits);
+        for (uint32_t t = 0; t < P.tb_count; t++) {
+            const hpvcd_tb& T = C.tbs[tb0 + t];
+            int sw = T.comp ? D.sub_w_log2 : 0, sh = T.comp ? D.sub_h_log2 : 0;
+            int cx = (T.x << sw) >> P.log2_ctb, cy = (T.y << sh) >> P.log2_ctb;
+            uint32_t u = (uint32_t)((T.comp * D.ctb_rows + cy) * D.ctb_cols + cx);
+            utbs[u].push_back(tb0 + t);
+            if (T.flags & TBX_HAS_RES) {
+                int pass = T.res_scale != 0 ? 1 : 0;
+                per_pic_res[pass][T.log2n - 2][lp].push_back(tb0 + t);
+            }
+        }
+        C.pic_units_begin[lp] = unit_base;
+        for (int pl = 0; pl < nplanes; pl++)
+            for (int cy = 0; cy < D.ctb_rows; cy++)
+                for (int cx = 0; cx < D.ctb_cols; cx++) {
+                    uint32_t u = (uint32_t)((pl * D.ctb_rows + cy) * D.ctb_cols + cx);
+                    DevUnit U;
+                    std::memset(&U, 0, sizeof(U));
+                    U.pic = lp; U.cx = (uint16_t)cx; U.cy = (uint16_t)cy; U.plane = (uint8_t)pl;
+                    U.dep0 = cx > 0 ? (int32_t)(unit_base + u - 1) : -1;
+                    if (cy > 0) U.dep1 = (int32_t)(unit_base + u - D.ctb_cols + (cx + 1 < D.ctb_cols ? 1 : 0));
+                    else U.dep1 = -1;
+                    // dependency levels inside the unit
+                    int sw = pl ? D.sub_w_log2 : 0, sh = pl ? D.sub_h_log2 : 0;
+                    int ctbw = ctb >> sw, ctbh = ctb >> sh, cgw = ctbw / 4, cgh = ctbh / 4;
+                    int ux0 = cx * ctbw, uy0 = cy * ctbh;
+                    std::vector<uint16_t> lmap((size_t)cgw * cgh, 0);
+                    std::vector<std::pair<uint16_t, uint32_t>> lt;  // (level, tb)
+                    for (uint32_t tbi : utbs[u]) {
+                        const hpvcd_tb& T = C.tbs[tbi];
+                        int n = 1 << T.log2n, x0 = T.x - ux0, y0 = T.y - uy0;
+                        int lvl = 0;
+                        auto probe = [&](int rx, int ry) {
+                            if (rx < 0 || ry < 0 || rx >= ctbw || ry >= ctbh) return;
+                            lvl = std::max<int>(lvl, lmap[(ry / 4) * cgw + rx / 4]);
+                        };
+                        for (int k = -1; k < 2 * n; k += 4) { probe(x0 - 1, y0 + k); probe(x0 + k, y0 - 1); }
+                        probe(x0 - 1, y0 + 2 * n - 1);
+                        probe(x0 + 2 * n - 1, y0 - 1);
+                        for (int yy = y0; yy < y0 + n; yy += 4)
+                            for (int xx = x0; xx < x0 + n; xx += 4) lmap[(yy / 4) * cgw + xx / 4] = (uint16_t)(lvl + 1);
+                        lt.push_back({(uint16_t)lvl, tbi});
+                    }
+                    std::stable_sort(lt.begin(), lt.end(),
+                                     [](const std::pair<uint16_t, uint32_t>& a, const std::pair<uint16_t, uint32_t>& b) {
+                                         return a.first < b.first;
+                                     });
+                    int nlev = lt.empty() ? 0 : lt.back().first + 1;
+                    if (nlev > 255) return set_error(HPVCD_E_INVALID, "too many dependency levels in one CTB");
+                    U.nlevels = (uint8_t)nlev;
+                    U.ntb = (uint16_t)lt.size();
+                    U.tb_list = (uint32_t)C.unit_tbs.size();
+                    U.level_list = (uint32_t)C.level_off.size();
+                    size_t k = 0;
+                    for (int l = 0; l <= nlev; l++) {
+                        while (k < lt.size() && lt[k].first < l) k++;
+                        C.level_off.push_back((uint16_t)k);
+                    }
+                    for (auto& e : lt) C.unit_tbs.push_back(e.second);
+                    C.units.push_back(U);
+                    keys.push_back({(uint32_t)(cx + 2 * cy), lp, (uint32_t)pl, unit_base + u});
+                }
+        unit_base += nunits;
+    }
+    C.pic_units_begin[np] = unit_base;
+    // batched tickets: topological (t = cx + 2 cy) across all pictures
+    std::vector<UnitKey> sorted = keys;
+    std::stable_sort(sorted.begin(), sorted.end(), [](const UnitKey& a, const UnitKey& b) {
+        if (a.t != b.t) return a.t < b.t;
+        if (a.pic != b.pic) return a.pic < b.pic;
+        return a.plane < b.plane;
+    });
+    C.tickets.clear();
+    for (auto& k : sorted) C.tickets.push_back(k.idx);
+    // plain tickets: per picture, same order restricted to the picture
+    C.plain_tickets.clear();
+    for (uint32_t lp = 0; lp < np; lp++)
+        for (auto& k : sorted)
+            if (k.pic == lp) C.plain_tickets.push_back(k.idx);
+    // residual lists
+    for (int pass = 0; pass < 2; pass++)
+        for (int s = 0; s < 4; s++) {
+            C.res_list[pass][s].clear();
+            for (uint32_t lp = 0; lp < np; lp++) {
+                C.pic_res_begin[pass][s][lp] = (uint32_t)C.res_list[pass][s].size();
+                auto& v = per_pic_res[pass][s][lp];
+                C.res_list[pass][s].insert(C.res_list[pass][s].end(), v.begin(), v.end());
+            }
+            C.pic_res_begin[pass][s][np] = (uint32_t)C.res_list[pass][s].size();
+        }
+    C.plane_elems = plane_elems;
+    C.res_elems = plane_elems;
+    C.sync_words = 1 + (np + 1) + (uint32_t)C.units.size();
+    return HPVCD_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Timer {
+    hipEvent_t ev[2 * HPVCD_KERNEL_COUNT * 4];
+    int used = 0;
+    bool on = false;
+    int kinds[2 * HPVCD_KERNEL_COUNT * 4];
+};
+
+inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+template <typename R>
+void launch_residual(hipStream_t s, ClassJob& C, const uint8_t* sc, int pass, int log2, const uint32_t* list,
+                     uint32_t count) {
+    if (!count) return;
+    R* res = reinterpret_cast<R*>(C.d_res);
+    switch (log2) {
+        case 2: k_residual<R, 2><<<cdiv(count, 16), 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, list, count, C.d_coeffs, sc, res, pass); break;
+        case 3: k_residual<R, 3><<<cdiv(count, 4), 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, list, count, C.d_coeffs, sc, res, pass); break;
+        case 4: k_residual<R, 4><<<count, 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, list, count, C.d_coeffs, sc, res, pass); break;
+        default: k_residual<R, 5><<<count, 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, list, count, C.d_coeffs, sc, res, pass); break;
+    }
+}
+
+template <typename T, typename R>
+void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* d_out, int mode, int stop,
+                  std::vector<std::pair<int, hipEvent_t>>* marks) {
+    hipStream_t s = g->stream;
+    const uint32_t np = (uint32_t)C.pics.size();
+    T* pa = reinterpret_cast<T*>(C.d_planes_a);
+    T* pb = reinterpret_cast<T*>(C.d_planes_b);
+    const R* res = reinterpret_cast<const R*>(C.d_res);
+    auto mark = [&](int kind) {
+        if (!marks) return;
+        hipEvent_t e;
+        hipEventCreate(&e);
+        hipEventRecord(e, s);
+        marks->push_back({kind, e});
+    };
+    int occ = g->occ_intra[(sizeof(T) == 2 ? 2 : 0) + (sizeof(R) == 4 ? 1 : 0)];
+    if (occ < 1) occ = 1;
+    const uint32_t max_blocks = (uint32_t)(g->num_cus * occ);
+    k_fill_u32<<<cdiv(C.sync_words, 256), 256, 0, s>>>(C.d_sync, C.sync_words, 0u);
+    uint32_t* err = C.d_sync;
+    uint32_t* counters = C.d_sync + 1;
+    uint32_t* done = C.d_sync + 1 + (np + 1);
+    const uint32_t nunits = (uint32_t)C.units.size();
+    const int nsteps = mode == HPVCD_MODE_PLAIN ? (int)np : 1;
+    for (int step = 0; step < nsteps; step++) {
+        const bool plain = mode == HPVCD_MODE_PLAIN;
+        const uint32_t p0 = plain ? (uint32_t)step : 0, pn = plain ? 1 : np;
+        mark(HPVCD_KERNEL_RESIDUAL);
+        for (int pass = 0; pass < 2; pass++)
+            for (int sz = 0; sz < 4; sz++) {
+                uint32_t b0 = plain ? C.pic_res_begin[pass][sz][p0] : 0;
+                uint32_t b1 = plain ? C.pic_res_begin[pass][sz][p0 + 1] : (uint32_t)C.res_list[pass][sz].size();
+                launch_residual<R>(s, C, d_scaling, pass, sz + 2, C.d_res_list[pass][sz] + b0, b1 - b0);
+            }
+        mark(HPVCD_KERNEL_INTRA);
+        {
+            const uint32_t* tick = plain ? C.d_plain_tickets + C.pic_units_begin[p0] : C.d_tickets;
+            uint32_t nu = plain ? C.pic_units_begin[p0 + 1] - C.pic_units_begin[p0] : nunits;
+            uint32_t* ctr = counters + (plain ? 1 + p0 : 0);
+            uint32_t blocks = std::min<uint32_t>(nu, max_blocks);
+            if (blocks)
+                k_intra<T, R><<<blocks, 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_units, tick, nu, C.d_unit_tbs, C.d_level_off,
+                                                     C.d_order, C.d_cells, pa, res, ctr, done, err);
+        }
+        mark(-1);
+        if (stop >= HPVCD_STAGE_DEBLOCK) {
+            mark(HPVCD_KERNEL_DEBLOCK);
+            uint32_t lv = plain ? (uint32_t)((C.pics[p0].w >= 12 ? (C.pics[p0].w - 4) / 8 : 0) * (C.pics[p0].h / 4)) : C.max_luma_v;
+            uint32_t lh = plain ? (uint32_t)((C.pics[p0].h >= 12 ? (C.pics[p0].h - 4) / 8 : 0) * (C.pics[p0].w / 4)) : C.max_luma_h;
+            uint32_t cv = plain ? (uint32_t)((C.pics[p0].cw >= 10 ? (C.pics[p0].cw - 2) / 8 : 0) * (C.pics[p0].ch / 4)) : C.max_chroma_v;
+            uint32_t ch = plain ? (uint32_t)((C.pics[p0].ch >= 10 ? (C.pics[p0].ch - 2) / 8 : 0) * (C.pics[p0].cw / 4)) : C.max_chroma_h;
+            if (lv) k_deblock_luma<T><<<dim3(cdiv(lv, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 1, p0);
+            if (lh) k_deblock_luma<T><<<dim3(cdiv(lh, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 0, p0);
+            if (cv) k_deblock_chroma<T><<<dim3(cdiv(cv, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 1, p0);
+            if (ch) k_deblock_chroma<T><<<dim3(cdiv(ch, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 0, p0);
+            mark(-1);
+        }
+        if (stop >= HPVCD_STAGE_SAO) {
+            mark(HPVCD_KERNEL_SAO);
+            uint32_t mp = plain ? (uint32_t)(C.pics[p0].w * C.pics[p0].h) : C.max_plane;
+            k_sao<T><<<dim3(cdiv(mp, 256), pn, 3), 256, 0, s>>>(C.d_pics, C.d_cells, C.d_sao, pa, pb, p0);
+            mark(-1);
+        }
+        if (stop >= HPVCD_STAGE_RGB) {
+            mark(HPVCD_KERNEL_RGB);
+            uint32_t mc = plain ? (uint32_t)(C.pics[p0].copy_w * C.pics[p0].copy_h) : C.max_copy;
+            if (mc) k_rgb<T><<<dim3(cdiv(mc, 256), pn), 256, 0, s>>>(C.d_pics, pb, d_out, p0);
+            mark(-1);
+        }
+    }
+}
+
+void launch_job(hpvcd_job* j, int mode, int stop, std::vector<std::pair<int, hipEvent_t>>* marks) {
+    for (auto& C : j->classes) {
+        if (C.samp16) {
+            if (C.res32) launch_class<uint16_t, int32_t>(j->g, C, j->d_scaling, j->d_out, mode, stop, marks);
+            else launch_class<uint16_t, int16_t>(j->g, C, j->d_scaling, j->d_out, mode, stop, marks);
+        } else {
+            if (C.res32) launch_class<uint8_t, int32_t>(j->g, C, j->d_scaling, j->d_out, mode, stop, marks);
+            else launch_class<uint8_t, int16_t>(j->g, C, j->d_scaling, j->d_out, mode, stop, marks);
+        }
+    }
+}
+
+hpvcd_status check_err(hpvcd_job* j) {
+    for (auto& C : j->classes) {
+        uint32_t e = 0;
+        HIP_TRY(hipMemcpy(&e, C.d_sync, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (e) return set_error(HPVCD_E_DEVICE, "intra wavefront: a dependency wait exceeded its spin limit");
+    }
+    return HPVCD_OK;
+}
+
+template <typename X>
+hpvcd_status upload(X*& dst, DevBuf& mem, const std::vector<X>& v) {
+    dst = mem.take<X>(std::max<size_t>(v.size(), 1));
+    if (!v.empty()) HIP_TRY(hipMemcpy(dst, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
+    return HPVCD_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* hpvcd_last_error(void) { return g_last_error.c_str(); }
+const char* hpvcd_version(void) { return "hpvcd-mi355x 0.1.0 (gfx950)"; }
+
+size_t hpvcd_output_bytes(const hpvcd_output* o) {
+    if (!o) return 0;
+    size_t bps = (o->kind == HPVCD_OUT_RGB16 || o->kind == HPVCD_OUT_LUMA16) ? 2 : 1;
+    size_t ch = (o->kind == HPVCD_OUT_LUMA8 || o->kind == HPVCD_OUT_LUMA16) ? 1 : 3;
+    return (size_t)o->width * o->height * ch * bps;
+}
+
+void hpvcd_records_view(const hpvcd_records* r, hpvcd_batch* out) {
+    std::memset(out, 0, sizeof(*out));
+    if (!r) return;
+    out->num_pictures = (uint32_t)r->pictures.size(); out->pictures = r->pictures.data();
+    out->num_tbs = (uint32_t)r->tbs.size(); out->tbs = r->tbs.data();
+    out->num_coeffs = (uint32_t)r->coeffs.size(); out->coeffs = r->coeffs.data();
+    out->num_ctbs = (uint32_t)r->sao.size(); out->sao = r->sao.data();
+    out->num_cells = (uint32_t)r->cells.size(); out->cell_info = r->cells.data();
+    out->num_scaling = (uint32_t)(r->scaling.size() / HPVCD_SCALING_SET_BYTES);
+    out->scaling = r->scaling.empty() ? nullptr : r->scaling.data();
+    out->num_outputs = (uint32_t)r->outputs.size(); out->outputs = r->outputs.data();
+}
+
+void hpvcd_records_free(hpvcd_records* r) { delete r; }
+
+hpvcd_status hpvcd_records_concat(hpvcd_records* const* sets, uint32_t count, hpvcd_records** out) {
+    if (!out || (count && !sets)) return set_error(HPVCD_E_INVALID, "null argument");
+    hpvcd_records* r = new (std::nothrow) hpvcd_records();
+    if (!r) return set_error(HPVCD_E_ALLOC, "out of memory");
+    for (uint32_t i = 0; i < count; i++) {
+        const hpvcd_records* s = sets[i];
+        if (!s) { delete r; return set_error(HPVCD_E_INVALID, "null record set"); }
+        uint32_t tb0 = (uint32_t)r->tbs.size(), co0 = (uint32_t)r->coeffs.size(), ctb0 = (uint32_t)r->sao.size();
+        uint32_t cell0 = (uint32_t)r->cells.size(), out0 = (uint32_t)r->outputs.size();
+        int32_t sc0 = (int32_t)(r->scaling.size() / HPVCD_SCALING_SET_BYTES);
+        for (hpvcd_picture p : s->pictures) {
+            p.tb_begin += tb0; p.coeff_begin += co0; p.ctb_begin += ctb0; p.cell_begin += cell0;
+            p.out_index += out0;
+            if (p.scaling_index >= 0) p.scaling_index += sc0;
+            r->pictures.push_back(p);
+        }
+        r->tbs.insert(r->tbs.end(), s->tbs.begin(), s->tbs.end());
+        r->coeffs.insert(r->coeffs.end(), s->coeffs.begin(), s->coeffs.end());
+        r->sao.insert(r->sao.end(), s->sao.begin(), s->sao.end());
+        r->cells.insert(r->cells.end(), s->cells.begin(), s->cells.end());
+        r->scaling.insert(r->scaling.end(), s->scaling.begin(), s->scaling.end());
+        r->outputs.insert(r->outputs.end(), s->outputs.begin(), s->outputs.end());
+    }
+    *out = r;
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_gpu_open(int device, hpvcd_gpu** out) {
+    if (!out) return set_error(HPVCD_E_INVALID, "null argument");
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_error(HPVCD_E_DEVICE, "no such GPU");
+    HIP_TRY(hipSetDevice(device));
+    hpvcd_gpu* g = new (std::nothrow) hpvcd_gpu();
+    if (!g) return set_error(HPVCD_E_ALLOC, "out of memory");
+    g->device = device;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    g->num_cus = prop.multiProcessorCount;
+    HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[0], k_intra<uint8_t, int16_t>, 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[1], k_intra<uint8_t, int32_t>, 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[2], k_intra<uint16_t, int16_t>, 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[3], k_intra<uint16_t, int32_t>, 256, 0));
+    *out = g;
+    return HPVCD_OK;
+}
+
+void hpvcd_gpu_close(hpvcd_gpu* g) {
+    if (!g) return;
+    hipSetDevice(g->device);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
+}
+
+hpvcd_status hpvcd_gpu_mem_info(hpvcd_gpu* g, size_t* free_b, size_t* total_b) {
+    if (!g || !free_b || !total_b) return set_error(HPVCD_E_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(g->device));
+    HIP_TRY(hipMemGetInfo(free_b, total_b));
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_gpu_sync(hpvcd_gpu* g) {
+    if (!g) return set_error(HPVCD_E_INVALID, "null argument");
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return HPVCD_OK;
+}
+
+void hpvcd_job_free(hpvcd_job* j) {
+    if (!j) return;
+    hipSetDevice(j->g->device);
+    for (auto& C : j->classes)
+        if (C.graph) hipGraphExecDestroy(C.graph);
+    if (j->mem.base) hipFree(j->mem.base);
+    delete j;
+}
+
+hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out) {
+    if (!g || !b || !out) return set_error(HPVCD_E_INVALID, "null argument");
+    *out = nullptr;
+    hpvcd_status st = validate_batch(*b);
+    if (st != HPVCD_OK) return st;
+    HIP_TRY(hipSetDevice(g->device));
+    hpvcd_job* j = new (std::nothrow) hpvcd_job();
+    if (!j) return set_error(HPVCD_E_ALLOC, "out of memory");
+    j->g = g;
+    j->outputs.assign(b->outputs, b->outputs + b->num_outputs);
+    size_t ob = 0;
+    for (auto& o : j->outputs) {
+        j->out_off.push_back(ob);
+        ob += (hpvcd_output_bytes(&o) + 255) & ~(size_t)255;
+    }
+    j->out_bytes = ob;
+    // format classes: (16-bit samples, 32-bit residuals)
+    j->pic_class.resize(b->num_pictures);
+    j->pic_local.resize(b->num_pictures);
+    for (uint32_t p = 0; p < b->num_pictures; p++) {
+        const hpvcd_picture& P = b->pictures[p];
+        bool s16 = P.bit_depth_luma > 8;
+        bool r32 = s16 || (P.flags & HPVCD_PF_EXTENDED_PRECISION) || P.chroma_idc == 3;
+        int ci = -1;
+        for (size_t c = 0; c < j->classes.size(); c++)
+            if (j->classes[c].samp16 == s16 && j->classes[c].res32 == r32) ci = (int)c;
+        if (ci < 0) {
+            j->classes.emplace_back();
+            j->classes.back().samp16 = s16;
+            j->classes.back().res32 = r32;
+            ci = (int)j->classes.size() - 1;
+        }
+        j->pic_class[p] = (uint32_t)ci;
+        j->pic_local[p] = (uint32_t)j->classes[ci].batch_pics.size();
+        j->classes[ci].batch_pics.push_back(p);
+    }
+    size_t sc_bytes = (size_t)b->num_scaling * HPVCD_SCALING_SET_BYTES;
+    size_t total = ob + 256 + ((sc_bytes + 256 + 255) & ~(size_t)255);
+    for (auto& C : j->classes) {
+        st = prepare_class(*b, C, j->out_off);
+        if (st != HPVCD_OK) { delete j; return st; }
+        size_t sb = C.samp16 ? 2 : 1, rb = C.res32 ? 4 : 2;
+        total += bytes_of<DevPic>(C.pics.size()) + bytes_of<hpvcd_tb>(C.tbs.size()) + bytes_of<uint32_t>(C.tb_pic.size()) +
+                 bytes_of<uint32_t>(C.coeffs.size() + 1) + bytes_of<uint32_t>(C.cells.size()) * 2 +
+                 bytes_of<hpvcd_sao>(C.sao.size()) + bytes_of<DevUnit>(C.units.size()) +
+                 bytes_of<uint32_t>(C.tickets.size()) * 2 + bytes_of<uint32_t>(C.unit_tbs.size() + 1) +
+                 bytes_of<uint16_t>(C.level_off.size() + 1) + bytes_of<uint32_t>(C.sync_words) +
+                 2 * ((C.plane_elems * sb + 255) & ~(size_t)255) + ((C.res_elems * rb + 255) & ~(size_t)255) + 8 * 256;
+        for (int pass = 0; pass < 2; pass++)
+            for (int s = 0; s < 4; s++) total += bytes_of<uint32_t>(C.res_list[pass][s].size() + 1);
+    }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total > free_b) {
+        delete j;
+        return set_error(HPVCD_E_DEVICE, "not enough free HBM for the batch");
+    }
+    if (hipMalloc(&j->mem.base, total) != hipSuccess) {
+        delete j;
+        return set_error(HPVCD_E_DEVICE, "hipMalloc of the job pool failed");
+    }
+    j->mem.size = total;
+    j->d_out = j->mem.take<uint8_t>(ob + 1);
+    if (hipMemset(j->d_out, 0, ob + 1) != hipSuccess) { hpvcd_job_free(j); return set_error(HPVCD_E_DEVICE, "hipMemset"); }
+    j->d_scaling = j->mem.take<uint8_t>(sc_bytes + 1);
+    if (sc_bytes && hipMemcpy(j->d_scaling, b->scaling, sc_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        hpvcd_job_free(j);
+        return set_error(HPVCD_E_DEVICE, "scaling upload");
+    }
+    for (auto& C : j->classes) {
+        size_t sb = C.samp16 ? 2 : 1, rb = C.res32 ? 4 : 2;
+        if ((st = upload(C.d_pics, j->mem, C.pics)) || (st = upload(C.d_tbs, j->mem, C.tbs)) ||
+            (st = upload(C.d_tb_pic, j->mem, C.tb_pic)) || (st = upload(C.d_coeffs, j->mem, C.coeffs)) ||
+            (st = upload(C.d_cells, j->mem, C.cells)) || (st = upload(C.d_order, j->mem, C.order)) ||
+            (st = upload(C.d_sao, j->mem, C.sao)) || (st = upload(C.d_units, j->mem, C.units)) ||
+            (st = upload(C.d_tickets, j->mem, C.tickets)) || (st = upload(C.d_plain_tickets, j->mem, C.plain_tickets)) ||
+            (st = upload(C.d_unit_tbs, j->mem, C.unit_tbs)) || (st = upload(C.d_level_off, j->mem, C.level_off))) {
+            hpvcd_job_free(j);
+            return st;
+        }
+        for (int pass = 0; pass < 2; pass++)
+            for (int s = 0; s < 4; s++)
+                if ((st = upload(C.d_res_list[pass][s], j->mem, C.res_list[pass][s]))) { hpvcd_job_free(j); return st; }
+        C.d_sync = j->mem.take<uint32_t>(C.sync_words);
+        C.d_planes_a = j->mem.take<char>(C.plane_elems * sb);
+        C.d_planes_b = j->mem.take<char>(C.plane_elems * sb);
+        C.d_res = j->mem.take<char>(C.res_elems * rb);
+        if (j->mem.used > j->mem.size) { hpvcd_job_free(j); return set_error(HPVCD_E_DEVICE, "pool overflow"); }
+    }
+    *out = j;
+    return HPVCD_OK;
+}
+
+size_t hpvcd_job_device_bytes(const hpvcd_job* j) { return j ? j->mem.size : 0; }
+
+hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop) {
+    if (!j) return set_error(HPVCD_E_INVALID, "null argument");
+    if (stop < HPVCD_STAGE_RECON || stop > HPVCD_STAGE_RGB) return set_error(HPVCD_E_INVALID, "stop stage");
+    HIP_TRY(hipSetDevice(j->g->device));
+    j->last_stop = stop;
+    if (mode == HPVCD_MODE_GRAPH) {
+        for (auto& C : j->classes) {
+            if (!C.graph || C.graph_stop != stop) {
+                if (C.graph) { hipGraphExecDestroy(C.graph); C.graph = nullptr; }
+                hipGraph_t gr;
+                HIP_TRY(hipStreamBeginCapture(j->g->stream, hipStreamCaptureModeThreadLocal));
+                if (C.samp16) {
+                    if (C.res32) launch_class<uint16_t, int32_t>(j->g, C, j->d_scaling, j->d_out, HPVCD_MODE_BATCHED, stop, nullptr);
+                    else launch_class<uint16_t, int16_t>(j->g, C, j->d_scaling, j->d_out, HPVCD_MODE_BATCHED, stop, nullptr);
+                } else {
+                    if (C.res32) launch_class<uint8_t, int32_t>(j->g, C, j->d_scaling, j->d_out, HPVCD_MODE_BATCHED, stop, nullptr);
+                    else launch_class<uint8_t, int16_t>(j->g, C, j->d_scaling, j->d_out, HPVCD_MODE_BATCHED, stop, nullptr);
+                }
+                HIP_TRY(hipStreamEndCapture(j->g->stream, &gr));
+                hipError_t e = hipGraphInstantiate(&C.graph, gr, nullptr, nullptr, 0);
+                hipGraphDestroy(gr);
+                if (e != hipSuccess) return set_error(HPVCD_E_DEVICE, "hipGraphInstantiate failed");
+                C.graph_stop = stop;
+            }
+            HIP_TRY(hipGraphLaunch(C.graph, j->g->stream));
+        }
+    } else if (mode == HPVCD_MODE_BATCHED || mode == HPVCD_MODE_PLAIN) {
+        launch_job(j, mode, stop, nullptr);
+        HIP_TRY(hipGetLastError());
+    } else {
+        return set_error(HPVCD_E_INVALID, "mode");
+    }
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_job_download(hpvcd_job* j, void** outputs) {
+    if (!j || !outputs) return set_error(HPVCD_E_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(j->g->device));
+    HIP_TRY(hipStreamSynchronize(j->g->stream));
+    hpvcd_status st = check_err(j);
+    if (st) return st;
+    for (size_t o = 0; o < j->outputs.size(); o++) {
+        if (!outputs[o]) continue;
+        HIP_TRY(hipMemcpy(outputs[o], j->d_out + j->out_off[o], hpvcd_output_bytes(&j->outputs[o]), hipMemcpyDeviceToHost));
+    }
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_job_download_planes(hpvcd_job* j, uint32_t picture, uint16_t* y, uint16_t* cb, uint16_t* cr) {
+    if (!j || !y) return set_error(HPVCD_E_INVALID, "null argument");
+    if (picture >= j->pic_class.size()) return set_error(HPVCD_E_INVALID, "picture index");
+    HIP_TRY(hipSetDevice(j->g->device));
+    HIP_TRY(hipStreamSynchronize(j->g->stream));
+    hpvcd_status st = check_err(j);
+    if (st) return st;
+    ClassJob& C = j->classes[j->pic_class[picture]];
+    const DevPic& D = C.pics[j->pic_local[picture]];
+    const char* pool = (const char*)(j->last_stop >= HPVCD_STAGE_SAO ? C.d_planes_b : C.d_planes_a);
+    size_t sb = C.samp16 ? 2 : 1;
+    uint16_t* dst[3] = {y, cb, cr};
+    size_t cnt[3] = {(size_t)D.w * D.h, (size_t)D.cw * D.ch, (size_t)D.cw * D.ch};
+    std::vector<uint8_t> tmp;
+    for (int c = 0; c < (D.chroma_idc ? 3 : 1); c++) {
+        if (!dst[c]) continue;
+        tmp.resize(cnt[c] * sb);
+        HIP_TRY(hipMemcpy(tmp.data(), pool + D.plane_off[c] * sb, cnt[c] * sb, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < cnt[c]; i++)
+            dst[c][i] = sb == 2 ? ((const uint16_t*)tmp.data())[i] : tmp[i];
+    }
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_job_time(hpvcd_job* j, int mode, int warmup, int iters, double* ms_per_run, double* kernel_ms) {
+    if (!j || !ms_per_run || iters < 1) return set_error(HPVCD_E_INVALID, "bad arguments");
+    HIP_TRY(hipSetDevice(j->g->device));
+    hpvcd_status st;
+    for (int i = 0; i < warmup; i++)
+        if ((st = hpvcd_job_run(j, mode, HPVCD_STAGE_RGB))) return st;
+    HIP_TRY(hipStreamSynchronize(j->g->stream));
+    if ((st = check_err(j))) return st;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, j->g->stream));
+    for (int i = 0; i < iters; i++)
+        if ((st = hpvcd_job_run(j, mode, HPVCD_STAGE_RGB))) return st;
+    HIP_TRY(hipEventRecord(e1, j->g->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_run = ms / iters;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if ((st = check_err(j))) return st;
+    if (kernel_ms) {
+        for (int k = 0; k < HPVCD_KERNEL_COUNT; k++) kernel_ms[k] = 0.0;
+        // per-kernel breakdown from event marks around each stage (batched
+        // or plain launch sequence; a graph replays the batched sequence)
+        int kmode = mode == HPVCD_MODE_GRAPH ? HPVCD_MODE_BATCHED : mode;
+        for (int i = 0; i < iters; i++) {
+            std::vector<std::pair<int, hipEvent_t>> marks;
+            j->last_stop = HPVCD_STAGE_RGB;
+            launch_job(j, kmode, HPVCD_STAGE_RGB, &marks);
+            HIP_TRY(hipStreamSynchronize(j->g->stream));
+            for (size_t m = 0; m + 1 < marks.size(); m++) {
+                if (marks[m].first < 0) continue;
+                float t = 0.f;
+                hipEventElapsedTime(&t, marks[m].second, marks[m + 1].second);
+                kernel_ms[marks[m].first] += t / iters;
+            }
+            for (auto& m : marks) hipEventDestroy(m.second);
+        }
+        if ((st = check_err(j))) return st;
+    }
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_gpu_decode_batch(hpvcd_gpu* g, const hpvcd_batch* b, void** outputs) {
+    hpvcd_job* j = nullptr;
+    hpvcd_status st = hpvcd_job_create(g, b, &j);
+    if (st) return st;
+    st = hpvcd_job_run(j, HPVCD_MODE_BATCHED, HPVCD_STAGE_RGB);
+    if (!st) st = hpvcd_job_download(j, outputs);
+    hpvcd_job_free(j);
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+/*
+ * hpvcd.h — C ABI of the MI355X reconstruction back end.
+ *
+ * The reference (awxkee/hpvcd v0.3.1) is a Rust crate whose public API is
+ * decode_heic(&[u8]) / Decoder::{decode, decode_rgb8, decode_yuv}
+ * (src/lib.rs:1476-1497, src/decoder.rs:54-297).  Its reconstruction half
+ * runs per TU through the ExecContext function table (src/exec.rs:33-130).
+ * This library replaces that half: the host parse emits records
+ * (hpvcd_records.h) and the entry points below reconstruct whole batches of
+ * pictures on one GPU.  Every entry point returns an hpvcd_status; no
+ * exception or abort crosses the ABI.
+ */
+#ifndef HPVCD_H
+#define HPVCD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "hpvcd_records.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  1..13 follow DecodeError's variants in order
+ * (src/error.rs:30-58); the last two are new to the GPU back end. */
+typedef enum hpvcd_status {
+    HPVCD_OK = 0,
+    HPVCD_E_NOT_HEIF = 1,
+    HPVCD_E_TRUNCATED_BOX = 2,
+    HPVCD_E_MISSING_BOX = 3,
+    HPVCD_E_UNSUPPORTED_ITEM = 4,
+    HPVCD_E_BITSTREAM = 5,
+    HPVCD_E_CABAC_DESYNC = 6,
+    HPVCD_E_UNSUPPORTED_CHROMA = 7,
+    HPVCD_E_UNSUPPORTED_BITDEPTH = 8,
+    HPVCD_E_BAD_DIMENSIONS = 9,
+    HPVCD_E_PARAMSET = 10,
+    HPVCD_E_UNSUPPORTED = 11,
+    HPVCD_E_LIMIT = 12,
+    HPVCD_E_ALLOC = 13,
+    HPVCD_E_DEVICE = 14,      /* HIP error or GPU/HBM admission failure */
+    HPVCD_E_INVALID = 15      /* malformed records or arguments */
+} hpvcd_status;
+
+/* Thread-local message for the last non-OK status of this thread
+ * (the Display text of DecodeError, src/error.rs:62-90). */
+const char* hpvcd_last_error(void);
+const char* hpvcd_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Seeded syntax-level generator: stands in for the CABAC parse and emits   */
+/* conformant reconstruction records for synthetic pictures.                */
+/* ------------------------------------------------------------------------ */
+typedef struct hpvcd_gen_config {
+    int32_t  tile_width, tile_height;   /* coded size of each picture */
+    int32_t  grid_cols, grid_rows;      /* pictures per output image */
+    int32_t  out_width, out_height;     /* output image size (grid output) */
+    uint8_t  chroma_idc, bit_depth, log2_ctb, log2_min_cb;
+    uint8_t  log2_min_tb, log2_max_tb, max_transform_depth_intra, qp;
+    uint8_t  log2_max_ts;               /* log2_max_transform_skip_block_size */
+    uint8_t  transform_skip, implicit_rdpcm, cross_component, extended_precision;
+    uint8_t  strong_intra_smoothing, sao, deblock, ts_rotation;
+    uint8_t  matrix, full_range, out_kind, num_outputs;
+    uint8_t  log2_res_scale_max;        /* CCP log2_res_scale drawn from [-max, max] */
+    uint8_t  _pad[3];
+    float    p_split_cu, p_nxn, p_split_tu, p_cbf, p_sig, p_level_geom, p_ts, p_tqb;
+    uint32_t config_id;
+    uint64_t base_seed;
+} hpvcd_gen_config;
+
+typedef struct hpvcd_records hpvcd_records;   /* owned record set */
+
+/* Fill *cfg with the distributions shared by every workload (SAO, deblock,
+ * split/cbf/significance probabilities) for a tile of the given size. */
+void hpvcd_gen_default_config(hpvcd_gen_config* cfg);
+hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_records** out);
+/* Concatenate record sets (outputs renumbered); the inputs stay owned. */
+hpvcd_status hpvcd_records_concat(hpvcd_records* const* sets, uint32_t count, hpvcd_records** out);
+void hpvcd_records_view(const hpvcd_records* r, hpvcd_batch* out);
+void hpvcd_records_free(hpvcd_records* r);
+/* Bytes each output image needs (kind and size from the batch). */
+size_t hpvcd_output_bytes(const hpvcd_output* o);
+
+/* ------------------------------------------------------------------------ */
+/* GPU reconstruction back end                                              */
+/* ------------------------------------------------------------------------ */
+typedef struct hpvcd_gpu hpvcd_gpu;           /* one device, one stream */
+typedef struct hpvcd_job hpvcd_job;           /* a batch resident in HBM */
+
+#define HPVCD_MODE_BATCHED 0   /* one launch per stage for the whole batch */
+#define HPVCD_MODE_PLAIN   1   /* one launch per stage per picture */
+#define HPVCD_MODE_GRAPH   2   /* batched stage chain replayed from a hipGraph */
+
+#define HPVCD_STAGE_RECON   0  /* stop after intra reconstruction */
+#define HPVCD_STAGE_DEBLOCK 1  /* stop after deblocking */
+#define HPVCD_STAGE_SAO     2  /* stop after SAO */
+#define HPVCD_STAGE_RGB     3  /* full pipeline (default) */
+
+hpvcd_status hpvcd_gpu_open(int device, hpvcd_gpu** out);
+void         hpvcd_gpu_close(hpvcd_gpu* g);
+/* Free device memory on the GPU (admission control). */
+hpvcd_status hpvcd_gpu_mem_info(hpvcd_gpu* g, size_t* free_bytes, size_t* total_bytes);
+
+/* Upload a batch (host records) and prepare it for reconstruction.  The
+ * records are copied; the caller may free them after return. */
+hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out);
+void         hpvcd_job_free(hpvcd_job* j);
+/* Device bytes the job holds. */
+size_t       hpvcd_job_device_bytes(const hpvcd_job* j);
+/* Enqueue the reconstruction on the GPU's stream (asynchronous). */
+hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop_stage);
+/* Wait for the stream. */
+hpvcd_status hpvcd_gpu_sync(hpvcd_gpu* g);
+/* Copy output images to host buffers (outputs[i] sized by hpvcd_output_bytes). */
+hpvcd_status hpvcd_job_download(hpvcd_job* j, void** outputs);
+/* Copy one picture's planes (u16 samples, coded size) at the stage the last
+ * run stopped at; cb/cr may be NULL for 4:0:0. */
+hpvcd_status hpvcd_job_download_planes(hpvcd_job* j, uint32_t picture, uint16_t* y, uint16_t* cb, uint16_t* cr);
+
+/* Timed replay: `iters` runs of the given mode after `warmup` runs; writes
+ * the mean milliseconds per run measured by HIP events on the job's stream
+ * and, per kernel kind, the mean milliseconds per run
+ * (kernel_ms[HPVCD_KERNEL_COUNT]). */
+#define HPVCD_KERNEL_RESIDUAL 0
+#define HPVCD_KERNEL_INTRA    1
+#define HPVCD_KERNEL_DEBLOCK  2
+#define HPVCD_KERNEL_SAO      3
+#define HPVCD_KERNEL_RGB      4
+#define HPVCD_KERNEL_COUNT    5
+hpvcd_status hpvcd_job_time(hpvcd_job* j, int mode, int warmup, int iters, double* ms_per_run,
+                            double* kernel_ms);
+
+/* One-call convenience: upload, reconstruct (batched graph), download. */
+hpvcd_status hpvcd_gpu_decode_batch(hpvcd_gpu* g, const hpvcd_batch* b, void** outputs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HPVCD_H */

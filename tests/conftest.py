@@ -1,0 +1,21 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP back end)")
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    import hpvcd_amd as H
+
+    g = H.Gpu(0)
+    yield g
+    g.close()
