@@ -272,11 +272,24 @@ __global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pic
 
 // ---------------------------------------------------------------------------
 // K2: intra prediction + reconstruction, persistent CTU wavefront
+//
+// Work unit = one (picture, plane, CTB).  Units are handed out by an atomic
+// ticket in an order where every unit a ticket reads from has a smaller
+// ticket, so every wait ends (the producer already runs).  A unit needs only
+// the bottom row of the three CTBs above it and the right column of the CTB
+// to its left; each unit publishes exactly those samples as 32-bit words
+// (run tag << 16 | sample) with relaxed agent-scope atomics, and consumers
+// poll the words they need until the tag matches.  Per-word coherence is all
+// that is required, so no fences (and no L2 write-back / invalidate) are
+// needed.  Inside a unit the TBs run level by level (dependency levels
+// precomputed on the host); waves take the TBs of a level round-robin.
 // ---------------------------------------------------------------------------
 constexpr int kMaxCtb = 64;
 constexpr int kWinW = 33;   // luma cell window: 1 + 2 * 64 / 4 columns
 constexpr int kWinH = 17;   // 1 + 64 / 4 rows
-constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr int kMaxUnitTbs = 256;
+constexpr int kBorderWords = 2 * kMaxCtb;  // bottom row + right column per unit
+constexpr uint32_t kSpinLimit = 1u << 19;
 
 struct IntraLds {
     uint16_t body[kMaxCtb * kMaxCtb];
@@ -286,8 +299,33 @@ struct IntraLds {
     uint32_t inf[kWinH * kWinW];
     int32_t raw[4][132];
     int32_t ref[4][132];
+    hpvcd_tb tb[kMaxUnitTbs];
+    uint16_t lev[kMaxUnitTbs + 1];
     uint32_t unit;
 };
+
+__device__ __forceinline__ uint32_t border_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void border_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits for a tagged border word and returns its sample.
+__device__ __forceinline__ uint16_t border_wait(const uint32_t* p, uint32_t tag, uint32_t* err) {
+    uint32_t spins = 0;
+    uint32_t v = border_load(p);
+    while ((v >> 16) != tag) {
+        ++spins;
+        if (spins > kSpinLimit || ((spins & 1023u) == 0 && border_load(err) != 0)) {
+            atomicOr(err, 1u);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = border_load(p);
+    }
+    return (uint16_t)(v & 0xffffu);
+}
 
 template <typename T, typename R>
 __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, const hpvcd_tb* __restrict__ tbs,
@@ -297,11 +335,12 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
                                                const uint16_t* __restrict__ level_off,
                                                const uint32_t* __restrict__ cell_order,
                                                const uint32_t* __restrict__ cell_info, T* __restrict__ samples,
-                                               const R* __restrict__ res_pool, uint32_t* counter, uint32_t* done,
-                                               uint32_t* err) {
+                                               const R* __restrict__ res_pool, uint32_t* __restrict__ border,
+                                               uint32_t* counter, const uint32_t* gen, uint32_t* err) {
     __shared__ IntraLds L;
     __shared__ R resb[kMaxCtb * kMaxCtb];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint32_t tag = *gen & 0xffffu;
 
     for (;;) {
         if (tid == 0) L.unit = atomicAdd(counter, 1u);
@@ -325,8 +364,12 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
         T* pl = samples + P.plane_off[plane];
         const R* rp = res_pool + P.res_off[plane];
         const bool need_info = (P.flags & (HPVCD_PF_TILES | HPVCD_PF_CONSTRAINED_INTRA)) || P.num_slices > 1;
+        const int ntb = U.ntb, nlev = U.nlevels;
 
-        // prefetch what does not depend on neighbours: residuals and cells
+        // stage what does not depend on neighbours: TB records, levels,
+        // residuals and the cell maps
+        for (int e = tid; e < ntb; e += 256) L.tb[e] = tbs[unit_tbs[U.tb_list + e]];
+        for (int e = tid; e <= nlev; e += 256) L.lev[e] = level_off[U.level_list + e];
         for (int e = tid; e < ctbw * ctbh; e += 256) {
             int ry = e / ctbw, rx = e - ry * ctbw;
             int px = ux0 + rx, py = uy0 + ry;
@@ -340,41 +383,30 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
             L.ord[e] = in ? cell_order[g] : 0xffffffffu;
             L.inf[e] = (in && need_info) ? cell_info[g] : 0u;
         }
-        // wait for the left and above-right (or above) units
-        if (tid == 0) {
-            int deps[2] = {U.dep0, U.dep1};
-            for (int d = 0; d < 2; d++) {
-                if (deps[d] < 0) continue;
-                uint32_t spins = 0;
-                while (__hip_atomic_load(&done[deps[d]], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-                    if (++spins > kSpinLimit) {
-                        atomicOr(err, 1u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
-                }
-            }
-        }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        // neighbour samples: the row above (2 units wide + corner) and the
-        // column to the left
+        // neighbour samples from the published borders: the row above
+        // (corner + two unit widths) and the column to the left
         for (int e = tid; e < 2 * ctbw + 1; e += 256) {
             int px = ux0 - 1 + e, py = uy0 - 1;
-            L.top[e] = (py >= 0 && px >= 0 && px < pw) ? (uint16_t)pl[(size_t)py * pw + px] : 0;
+            uint16_t v = 0;
+            if (py >= 0 && px >= 0 && px < pw) {
+                int ncx = px / ctbw;
+                uint32_t nu = ui - (uint32_t)U.cx + (uint32_t)ncx - (uint32_t)P.ctb_cols;  // unit (ncx, cy - 1)
+                v = border_wait(border + (size_t)nu * kBorderWords + (px - ncx * ctbw), tag, err);
+            }
+            L.top[e] = v;
         }
         for (int e = tid; e < ctbh; e += 256) {
             int px = ux0 - 1, py = uy0 + e;
-            L.left[e] = (px >= 0 && py < ph) ? (uint16_t)pl[(size_t)py * pw + px] : 0;
+            uint16_t v = 0;
+            if (px >= 0 && py < ph) v = border_wait(border + (size_t)(ui - 1) * kBorderWords + kMaxCtb + e, tag, err);
+            L.left[e] = v;
         }
         __syncthreads();
 
-        const uint32_t* tlist = unit_tbs + U.tb_list;
-        const uint16_t* lev = level_off + U.level_list;
-        for (int lv = 0; lv < U.nlevels; lv++) {
-            const int lo = lev[lv], hi = lev[lv + 1];
+        for (int lv = 0; lv < nlev; lv++) {
+            const int lo = L.lev[lv], hi = L.lev[lv + 1];
             for (int j = lo + wave; j < hi; j += 4) {
-                const hpvcd_tb tb = tbs[tlist[j]];
+                const hpvcd_tb tb = L.tb[j];
                 const int n = 1 << tb.log2n, n2 = 2 * n, nref = 4 * n + 1;
                 const int x0 = tb.x - ux0, y0 = tb.y - uy0;
                 const uint32_t ord_cur = tb.order;
@@ -433,15 +465,15 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
                     int i = lane + 64 * k;
                     sv[k] = 0;
                     if (i >= nref) continue;
-                    int j = -1;
+                    int j2 = -1;
                     uint64_t m = mask[k] & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
-                    if (m) j = 64 * k + 63 - __builtin_clzll(m);
+                    if (m) j2 = 64 * k + 63 - __builtin_clzll(m);
                     else {
                         for (int kk = k - 1; kk >= 0; kk--)
-                            if (mask[kk]) { j = 64 * kk + 63 - __builtin_clzll(mask[kk]); break; }
+                            if (mask[kk]) { j2 = 64 * kk + 63 - __builtin_clzll(mask[kk]); break; }
                     }
-                    if (j < 0) j = first;
-                    sv[k] = j >= 0 ? L.raw[wave][j] : neutral;
+                    if (j2 < 0) j2 = first;
+                    sv[k] = j2 >= 0 ? L.raw[wave][j2] : neutral;
                 }
                 wave_sync();
 #pragma unroll
@@ -542,16 +574,30 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
             }
             __syncthreads();
         }
-        // publish the unit
+        // publish: the whole CTB to the plane, bottom row and right column
+        // as tagged border words
         for (int e = tid; e < ctbw * ctbh; e += 256) {
             int ry = e / ctbw, rx = e - ry * ctbw;
             int px = ux0 + rx, py = uy0 + ry;
             if (px < pw && py < ph) pl[(size_t)py * pw + px] = (T)L.body[ry * kMaxCtb + rx];
         }
-        __threadfence();
+        uint32_t* bw = border + (size_t)ui * kBorderWords;
+        for (int e = tid; e < ctbw; e += 256) border_store(bw + e, (tag << 16) | L.body[(ctbh - 1) * kMaxCtb + e]);
+        for (int e = tid; e < ctbh; e += 256) border_store(bw + kMaxCtb + e, (tag << 16) | L.body[e * kMaxCtb + ctbw - 1]);
         __syncthreads();
-        if (tid == 0) __hip_atomic_store(&done[ui], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// Starts a run: zeroes the error word and the ticket counters and advances
+// the border tag (never 0; border words start zeroed).
+__global__ void k_begin_run(uint32_t* sync, uint32_t ncounters) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        uint32_t g = (sync[1] + 1) & 0xffffu;
+        sync[1] = g ? g : 1;
+        sync[0] = 0;
+    }
+    if (i < ncounters) sync[2 + i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -904,7 +950,8 @@ HPVCD_INST_RES(int32_t)
 #define HPVCD_INST_INTRA(T, R)                                                                                   \
     template __global__ void k_intra<T, R>(const DevPic*, const hpvcd_tb*, const DevUnit*, const uint32_t*,      \
                                            uint32_t, const uint32_t*, const uint16_t*, const uint32_t*,           \
-                                           const uint32_t*, T*, const R*, uint32_t*, uint32_t*, uint32_t*);
+                                           const uint32_t*, T*, const R*, uint32_t*, uint32_t*, const uint32_t*,  \
+                                           uint32_t*);
 HPVCD_INST_INTRA(uint8_t, int16_t)
 HPVCD_INST_INTRA(uint8_t, int32_t)
 HPVCD_INST_INTRA(uint16_t, int16_t)

@@ -170,8 +170,9 @@ struct ClassJob {
     uint16_t* d_level_off = nullptr;
     uint32_t* d_res_list[2][4] = {};
     void *d_planes_a = nullptr, *d_planes_b = nullptr, *d_res = nullptr;
-    uint32_t* d_sync = nullptr;   // [0] err, [1..npics+1] counters, then done flags
+    uint32_t* d_sync = nullptr;   // [0] err, [1] border tag, [2..] ticket counters (batched + one per picture)
     uint32_t sync_words = 0;
+    uint32_t* d_border = nullptr; // kBorderWords per unit
     hipGraphExec_t graph = nullptr;
     int graph_stop = -1;
 };
@@ -348,6 +349,7 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                     int nlev = lt.empty() ? 0 : lt.back().first + 1;
                     if (nlev > 255) return set_error(HPVCD_E_INVALID, "too many dependency levels in one CTB");
                     U.nlevels = (uint8_t)nlev;
+                    if (lt.size() > (size_t)kMaxUnitTbs) return set_error(HPVCD_E_INVALID, "too many TBs in one CTB");
                     U.ntb = (uint16_t)lt.size();
                     U.tb_list = (uint32_t)C.unit_tbs.size();
                     U.level_list = (uint32_t)C.level_off.size();
@@ -390,7 +392,7 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         }
     C.plane_elems = plane_elems;
     C.res_elems = plane_elems;
-    C.sync_words = 1 + (np + 1) + (uint32_t)C.units.size();
+    C.sync_words = 2 + (np + 1);
     return HPVCD_OK;
 }
 
@@ -434,17 +436,17 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
     auto mark = [&](int kind) {
         if (!marks) return;
         hipEvent_t e;
-        hipEventCreate(&e);
-        hipEventRecord(e, s);
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s);
         marks->push_back({kind, e});
     };
     int occ = g->occ_intra[(sizeof(T) == 2 ? 2 : 0) + (sizeof(R) == 4 ? 1 : 0)];
     if (occ < 1) occ = 1;
     const uint32_t max_blocks = (uint32_t)(g->num_cus * occ);
-    k_fill_u32<<<cdiv(C.sync_words, 256), 256, 0, s>>>(C.d_sync, C.sync_words, 0u);
+    k_begin_run<<<cdiv(np + 1, 256), 256, 0, s>>>(C.d_sync, np + 1);
     uint32_t* err = C.d_sync;
-    uint32_t* counters = C.d_sync + 1;
-    uint32_t* done = C.d_sync + 1 + (np + 1);
+    const uint32_t* gen = C.d_sync + 1;
+    uint32_t* counters = C.d_sync + 2;
     const uint32_t nunits = (uint32_t)C.units.size();
     const int nsteps = mode == HPVCD_MODE_PLAIN ? (int)np : 1;
     for (int step = 0; step < nsteps; step++) {
@@ -465,7 +467,7 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
             uint32_t blocks = std::min<uint32_t>(nu, max_blocks);
             if (blocks)
                 k_intra<T, R><<<blocks, 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_units, tick, nu, C.d_unit_tbs, C.d_level_off,
-                                                     C.d_order, C.d_cells, pa, res, ctr, done, err);
+                                                     C.d_order, C.d_cells, pa, res, C.d_border, ctr, gen, err);
         }
         mark(-1);
         if (stop >= HPVCD_STAGE_DEBLOCK) {
@@ -680,6 +682,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
                  bytes_of<hpvcd_sao>(C.sao.size()) + bytes_of<DevUnit>(C.units.size()) +
                  bytes_of<uint32_t>(C.tickets.size()) * 2 + bytes_of<uint32_t>(C.unit_tbs.size() + 1) +
                  bytes_of<uint16_t>(C.level_off.size() + 1) + bytes_of<uint32_t>(C.sync_words) +
+                 bytes_of<uint32_t>(C.units.size() * kBorderWords + 1) +
                  2 * ((C.plane_elems * sb + 255) & ~(size_t)255) + ((C.res_elems * rb + 255) & ~(size_t)255) + 8 * 256;
         for (int pass = 0; pass < 2; pass++)
             for (int s = 0; s < 4; s++) total += bytes_of<uint32_t>(C.res_list[pass][s].size() + 1);
@@ -716,6 +719,12 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
             for (int s = 0; s < 4; s++)
                 if ((st = upload(C.d_res_list[pass][s], j->mem, C.res_list[pass][s]))) { hpvcd_job_free(j); return st; }
         C.d_sync = j->mem.take<uint32_t>(C.sync_words);
+        C.d_border = j->mem.take<uint32_t>(C.units.size() * kBorderWords + 1);
+        if (hipMemset(C.d_sync, 0, C.sync_words * sizeof(uint32_t)) != hipSuccess ||
+            hipMemset(C.d_border, 0, (C.units.size() * kBorderWords + 1) * sizeof(uint32_t)) != hipSuccess) {
+            hpvcd_job_free(j);
+            return set_error(HPVCD_E_DEVICE, "hipMemset of the sync words");
+        }
         C.d_planes_a = j->mem.take<char>(C.plane_elems * sb);
         C.d_planes_b = j->mem.take<char>(C.plane_elems * sb);
         C.d_res = j->mem.take<char>(C.res_elems * rb);
