@@ -155,6 +155,7 @@ SIGNATURES = {
     "hpvcd_records_view": (None, [_P, C.POINTER(Batch)]),
     "hpvcd_records_free": (None, [_P]),
     "hpvcd_output_bytes": (C.c_size_t, [C.POINTER(Output)]),
+    "hpvcd_validate_batch": (C.c_int, [C.POINTER(Batch)]),
     "hpvcd_gpu_open": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "hpvcd_gpu_close": (None, [_P]),
     "hpvcd_gpu_mem_info": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),

@@ -535,6 +535,11 @@ extern "C" {
 const char* hpvcd_last_error(void) { return g_last_error.c_str(); }
 const char* hpvcd_version(void) { return "hpvcd-mi355x 0.1.0 (gfx950)"; }
 
+hpvcd_status hpvcd_validate_batch(const hpvcd_batch* b) {
+    if (!b) return set_error(HPVCD_E_INVALID, "null argument");
+    return validate_batch(*b);
+}
+
 size_t hpvcd_output_bytes(const hpvcd_output* o) {
     if (!o) return 0;
     size_t bps = (o->kind == HPVCD_OUT_RGB16 || o->kind == HPVCD_OUT_LUMA16) ? 2 : 1;

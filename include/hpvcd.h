@@ -78,6 +78,9 @@ hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_records** out)
 hpvcd_status hpvcd_records_concat(hpvcd_records* const* sets, uint32_t count, hpvcd_records** out);
 void hpvcd_records_view(const hpvcd_records* r, hpvcd_batch* out);
 void hpvcd_records_free(hpvcd_records* r);
+/* Checks every range, size and cross reference of a batch on the host (the
+ * same checks hpvcd_job_create runs before touching the GPU). */
+hpvcd_status hpvcd_validate_batch(const hpvcd_batch* b);
 /* Bytes each output image needs (kind and size from the batch). */
 size_t hpvcd_output_bytes(const hpvcd_output* o);
 

