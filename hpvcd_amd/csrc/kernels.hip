@@ -284,309 +284,7 @@ __global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pic
 // needed.  Inside a unit the TBs run level by level (dependency levels
 // precomputed on the host); waves take the TBs of a level round-robin.
 // ---------------------------------------------------------------------------
-constexpr int kMaxCtb = 64;
-constexpr int kWinW = 33;   // luma cell window: 1 + 2 * 64 / 4 columns
-constexpr int kWinH = 17;   // 1 + 64 / 4 rows
-constexpr int kMaxUnitTbs = 256;
-constexpr int kBorderWords = 2 * kMaxCtb;  // bottom row + right column per unit
-constexpr uint32_t kSpinLimit = 1u << 19;
-
-struct IntraLds {
-    uint16_t body[kMaxCtb * kMaxCtb];
-    uint16_t top[2 * kMaxCtb + 1];
-    uint16_t left[kMaxCtb];
-    uint32_t ord[kWinH * kWinW];
-    uint32_t inf[kWinH * kWinW];
-    int32_t raw[4][132];
-    int32_t ref[4][132];
-    hpvcd_tb tb[kMaxUnitTbs];
-    uint16_t lev[kMaxUnitTbs + 1];
-    uint32_t unit;
-};
-
-__device__ __forceinline__ uint32_t border_load(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void border_store(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Waits for a tagged border word and returns its sample.
-__device__ __forceinline__ uint16_t border_wait(const uint32_t* p, uint32_t tag, uint32_t* err) {
-    uint32_t spins = 0;
-    uint32_t v = border_load(p);
-    while ((v >> 16) != tag) {
-        ++spins;
-        if (spins > kSpinLimit || ((spins & 1023u) == 0 && border_load(err) != 0)) {
-            atomicOr(err, 1u);
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        v = border_load(p);
-    }
-    return (uint16_t)(v & 0xffffu);
-}
-
-template <typename T, typename R>
-__global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, const hpvcd_tb* __restrict__ tbs,
-                                               const DevUnit* __restrict__ units,
-                                               const uint32_t* __restrict__ ticket_units, uint32_t n_units,
-                                               const uint32_t* __restrict__ unit_tbs,
-                                               const uint16_t* __restrict__ level_off,
-                                               const uint32_t* __restrict__ cell_order,
-                                               const uint32_t* __restrict__ cell_info, T* __restrict__ samples,
-                                               const R* __restrict__ res_pool, uint32_t* __restrict__ border,
-                                               uint32_t* counter, const uint32_t* gen, uint32_t* err) {
-    __shared__ IntraLds L;
-    __shared__ R resb[kMaxCtb * kMaxCtb];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const uint32_t tag = *gen & 0xffffu;
-
-    for (;;) {
-        if (tid == 0) L.unit = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t ticket = L.unit;
-        __syncthreads();
-        if (ticket >= n_units) return;
-        const uint32_t ui = ticket_units[ticket];
-        const DevUnit U = units[ui];
-        const DevPic& P = pics[U.pic];
-        const int plane = U.plane;
-        const int sw = plane ? P.sub_w_log2 : 0, sh = plane ? P.sub_h_log2 : 0;
-        const int ctb = 1 << P.ctb_log2;
-        const int ctbw = ctb >> sw, ctbh = ctb >> sh;       // unit size in plane samples
-        const int pw = plane ? P.cw : P.w, ph = plane ? P.ch : P.h;
-        const int ux0 = U.cx * ctbw, uy0 = U.cy * ctbh;     // unit origin (plane samples)
-        const int X0 = U.cx << P.ctb_log2, Y0 = U.cy << P.ctb_log2;  // luma origin
-        const int gx0 = (X0 >> 2) - 1, gy0 = (Y0 >> 2) - 1;          // cell window origin
-        const int bd = plane ? P.bd_c : P.bd;
-        const int32_t maxv = (1 << bd) - 1;
-        T* pl = samples + P.plane_off[plane];
-        const R* rp = res_pool + P.res_off[plane];
-        const bool need_info = (P.flags & (HPVCD_PF_TILES | HPVCD_PF_CONSTRAINED_INTRA)) || P.num_slices > 1;
-        const int ntb = U.ntb, nlev = U.nlevels;
-
-        // stage what does not depend on neighbours: TB records, levels,
-        // residuals and the cell maps
-        for (int e = tid; e < ntb; e += 256) L.tb[e] = tbs[unit_tbs[U.tb_list + e]];
-        for (int e = tid; e <= nlev; e += 256) L.lev[e] = level_off[U.level_list + e];
-        for (int e = tid; e < ctbw * ctbh; e += 256) {
-            int ry = e / ctbw, rx = e - ry * ctbw;
-            int px = ux0 + rx, py = uy0 + ry;
-            resb[ry * kMaxCtb + rx] = (px < pw && py < ph) ? rp[(size_t)py * pw + px] : (R)0;
-        }
-        for (int e = tid; e < kWinH * kWinW; e += 256) {
-            int wy = e / kWinW, wx = e - wy * kWinW;
-            int gx = gx0 + wx, gy = gy0 + wy;
-            bool in = gx >= 0 && gy >= 0 && gx < P.gw && gy < P.gh;
-            size_t g = (size_t)P.cell_begin + (size_t)gy * P.gw + gx;
-            L.ord[e] = in ? cell_order[g] : 0xffffffffu;
-            L.inf[e] = (in && need_info) ? cell_info[g] : 0u;
-        }
-        // neighbour samples from the published borders: the row above
-        // (corner + two unit widths) and the column to the left
-        for (int e = tid; e < 2 * ctbw + 1; e += 256) {
-            int px = ux0 - 1 + e, py = uy0 - 1;
-            uint16_t v = 0;
-            if (py >= 0 && px >= 0 && px < pw) {
-                int ncx = px / ctbw;
-                uint32_t nu = ui - (uint32_t)U.cx + (uint32_t)ncx - (uint32_t)P.ctb_cols;  // unit (ncx, cy - 1)
-                v = border_wait(border + (size_t)nu * kBorderWords + (px - ncx * ctbw), tag, err);
-            }
-            L.top[e] = v;
-        }
-        for (int e = tid; e < ctbh; e += 256) {
-            int px = ux0 - 1, py = uy0 + e;
-            uint16_t v = 0;
-            if (px >= 0 && py < ph) v = border_wait(border + (size_t)(ui - 1) * kBorderWords + kMaxCtb + e, tag, err);
-            L.left[e] = v;
-        }
-        __syncthreads();
-
-        for (int lv = 0; lv < nlev; lv++) {
-            const int lo = L.lev[lv], hi = L.lev[lv + 1];
-            for (int j = lo + wave; j < hi; j += 4) {
-                const hpvcd_tb tb = L.tb[j];
-                const int n = 1 << tb.log2n, n2 = 2 * n, nref = 4 * n + 1;
-                const int x0 = tb.x - ux0, y0 = tb.y - uy0;
-                const uint32_t ord_cur = tb.order;
-                uint32_t cur_info = 0;
-                if (need_info) {
-                    int lx = tb.x << sw, ly = tb.y << sh;
-                    cur_info = L.inf[((ly >> 2) - gy0) * kWinW + ((lx >> 2) - gx0)];
-                }
-                // gather raw references with availability
-                // (src/decode.rs:4651 luma_avail, :4669 chroma_avail, :4693 gather)
-                for (int i = lane; i < nref; i += 64) {
-                    int rx, ry;
-                    if (i < n2) { rx = x0 - 1; ry = y0 + n2 - 1 - i; }
-                    else if (i == n2) { rx = x0 - 1; ry = y0 - 1; }
-                    else { rx = x0 + (i - n2 - 1); ry = y0 - 1; }
-                    int px = ux0 + rx, py = uy0 + ry;
-                    int32_t v = -1;
-                    if (px >= 0 && py >= 0 && px < pw && py < ph) {
-                        int lx = px << sw, ly = py << sh;
-                        int wx = (lx >> 2) - gx0, wy = (ly >> 2) - gy0;
-                        if (lx < P.w && ly < P.h && wy < kWinH && wx < kWinW && rx < (ry < 0 ? 2 * ctbw : ctbw)) {
-                            int c = wy * kWinW + wx;
-                            bool ok = L.ord[c] < ord_cur;
-                            if (ok && need_info) {
-                                uint32_t inf = L.inf[c];
-                                if ((P.flags & HPVCD_PF_TILES) && HPVCD_CELL_TILE(inf) != HPVCD_CELL_TILE(cur_info)) ok = false;
-                                if (P.num_slices > 1 && HPVCD_CELL_SLICE(inf) != HPVCD_CELL_SLICE(cur_info)) ok = false;
-                                if ((P.flags & HPVCD_PF_CONSTRAINED_INTRA) && !(inf & HPVCD_CELL_INTRA)) ok = false;
-                            }
-                            if (ok) {
-                                if (ry == -1) v = L.top[rx + 1];
-                                else if (rx == -1) v = L.left[ry];
-                                else v = L.body[ry * kMaxCtb + rx];
-                            }
-                        }
-                    }
-                    L.raw[wave][i] = v;
-                }
-                wave_sync();
-                // substitution (src/intra.rs:84): nearest available at or
-                // below each index, else the first available, else neutral
-                uint64_t mask[3];
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    int i = lane + 64 * k;
-                    mask[k] = __ballot(i < nref && L.raw[wave][i] >= 0);
-                }
-                int first = -1;
-#pragma unroll
-                for (int k = 2; k >= 0; k--)
-                    if (mask[k]) first = 64 * k + __builtin_ctzll(mask[k]);
-                const int32_t neutral = 1 << (bd - 1);
-                int32_t sv[3];
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    int i = lane + 64 * k;
-                    sv[k] = 0;
-                    if (i >= nref) continue;
-                    int j2 = -1;
-                    uint64_t m = mask[k] & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
-                    if (m) j2 = 64 * k + 63 - __builtin_clzll(m);
-                    else {
-                        for (int kk = k - 1; kk >= 0; kk--)
-                            if (mask[kk]) { j2 = 64 * kk + 63 - __builtin_clzll(mask[kk]); break; }
-                    }
-                    if (j2 < 0) j2 = first;
-                    sv[k] = j2 >= 0 ? L.raw[wave][j2] : neutral;
-                }
-                wave_sync();
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    int i = lane + 64 * k;
-                    if (i < nref) L.raw[wave][i] = sv[k];
-                }
-                wave_sync();
-                // reference filtering (src/intra.rs:151)
-                const int mode = tb.mode;
-                const bool filt_luma = plane == 0 || P.chroma_idc == 3;
-                const bool smoothing = !(P.flags & HPVCD_PF_INTRA_SMOOTHING_DISABLED);
-                bool do_filter = false;
-                if (smoothing && filt_luma && n != 4 && mode != 1) {
-                    int d26 = mode > 26 ? mode - 26 : 26 - mode, d10 = mode > 10 ? mode - 10 : 10 - mode;
-                    int min_dist = d26 < d10 ? d26 : d10;
-                    int thresh = n == 8 ? 7 : (n == 16 ? 1 : (n == 32 ? 0 : 8));
-                    do_filter = mode == 0 || min_dist > thresh;
-                }
-                const int32_t* S = L.raw[wave];
-                int32_t* F = L.raw[wave];
-                if (do_filter) {
-                    bool strong = false;
-                    const int32_t bl = S[0], tl = S[n2], tr = S[4 * n];
-                    if (plane == 0 && (P.flags & HPVCD_PF_STRONG_INTRA_SMOOTHING) && n == 32) {
-                        int thr = 1 << (bd - 5);
-                        int a = bl + tl - 2 * S[n], b = tr + tl - 2 * S[3 * n];
-                        strong = (a < 0 ? -a : a) < thr && (b < 0 ? -b : b) < thr;
-                    }
-                    for (int i = lane; i < nref; i += 64) {
-                        int32_t v;
-                        if (i == 0 || i == 4 * n) v = S[i];
-                        else if (strong) {
-                            if (i == n2) v = S[i];
-                            else if (i > n2) { int k = i - n2; v = ((n2 - k) * tl + k * tr + n) >> 6; }
-                            else { int k = n2 - i; v = ((n2 - k) * tl + k * bl + n) >> 6; }
-                        } else v = (S[i - 1] + 2 * S[i] + S[i + 1] + 2) >> 2;
-                        L.ref[wave][i] = v;
-                    }
-                    wave_sync();
-                    F = L.ref[wave];
-                }
-                // prediction (src/intra.rs:257)
-                const bool edge = plane == 0 && !(P.flags & HPVCD_PF_INTRA_BOUNDARY_FILTER_OFF);
-                const int log2n = tb.log2n;
-                int32_t dc = 0;
-                if (mode == 1) {
-                    int32_t s = 0;
-                    for (int k = lane; k < n; k += 64) s += F[n2 + 1 + k] + F[n2 - 1 - k];
-#pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-                    dc = (s + n) >> (log2n + 1);
-                }
-                const int angle = mode >= 2 ? kIntraAngle[mode - 2] : 0;
-                const int inv = (mode >= 11 && mode <= 25) ? kInvAngle[mode - 11] : 0;
-                const bool vertical = mode >= 18;
-                const bool has_res = (tb.flags & TBX_HAS_RES) != 0;
-                for (int p = lane; p < n * n; p += 64) {
-                    const int x = p & (n - 1), y = p >> log2n;
-                    int32_t v;
-                    if (mode == 0) {
-                        int32_t h = (n - 1 - x) * F[n2 - 1 - y] + (x + 1) * F[n2 + n + 1];
-                        int32_t vv = (n - 1 - y) * F[n2 + 1 + x] + (y + 1) * F[n2 - n - 1];
-                        v = (h + vv + n) >> (log2n + 1);
-                    } else if (mode == 1) {
-                        v = dc;
-                        if (edge && n < 32) {
-                            if (x == 0 && y == 0) v = (F[n2 - 1] + 2 * dc + F[n2 + 1] + 2) >> 2;
-                            else if (y == 0) v = (F[n2 + 1 + x] + 3 * dc + 2) >> 2;
-                            else if (x == 0) v = (F[n2 - 1 - y] + 3 * dc + 2) >> 2;
-                        }
-                    } else {
-                        // main reference index j >= 0 reads main[j]; j < 0
-                        // reads the projected side reference
-                        const int a = vertical ? y : x, c = vertical ? x : y;
-                        const int pos = (a + 1) * angle, ii = pos >> 5, frac = pos & 31;
-                        auto refv = [&](int j) -> int32_t {
-                            if (j >= 0) return vertical ? F[n2 + j] : F[n2 - j];
-                            int idx = (j * inv + 128) >> 8;
-                            idx = idx > n2 ? n2 : (idx < 0 ? 0 : idx);
-                            return vertical ? F[n2 - idx] : F[n2 + idx];
-                        };
-                        int32_t r0 = refv(c + ii + 1);
-                        v = frac == 0 ? r0 : ((32 - frac) * r0 + frac * refv(c + ii + 2) + 16) >> 5;
-                        v = clampi(v, 0, maxv);
-                        if (edge && n < 32) {
-                            if (mode == 26 && x == 0) v = clampi(F[n2 + 1] + ((F[n2 - 1 - y] - F[n2]) >> 1), 0, maxv);
-                            else if (mode == 10 && y == 0) v = clampi(F[n2 - 1] + ((F[n2 + 1 + x] - F[n2]) >> 1), 0, maxv);
-                        }
-                    }
-                    const int bx = x0 + x, by = y0 + y;
-                    if (ux0 + bx < pw && uy0 + by < ph) {
-                        if (has_res) v = clampi(wadd(v, (int32_t)resb[by * kMaxCtb + bx]), 0, maxv);
-                        L.body[by * kMaxCtb + bx] = (uint16_t)v;
-                    }
-                }
-                wave_sync();
-            }
-            __syncthreads();
-        }
-        // publish: the whole CTB to the plane, bottom row and right column
-        // as tagged border words
-        for (int e = tid; e < ctbw * ctbh; e += 256) {
-            int ry = e / ctbw, rx = e - ry * ctbw;
-            int px = ux0 + rx, py = uy0 + ry;
-            if (px < pw && py < ph) pl[(size_t)py * pw + px] = (T)L.body[ry * kMaxCtb + rx];
-        }
-        uint32_t* bw = border + (size_t)ui * kBorderWords;
-        for (int e = tid; e < ctbw; e += 256) border_store(bw + e, (tag << 16) | L.body[(ctbh - 1) * kMaxCtb + e]);
-        for (int e = tid; e < ctbh; e += 256) border_store(bw + kMaxCtb + e, (tag << 16) | L.body[e * kMaxCtb + ctbw - 1]);
-        __syncthreads();
-    }
-}
+#include "intra.hip"
 
 // Starts a run: zeroes the error word and the ticket counters and advances
 // the border tag (never 0; border words start zeroed).
@@ -948,10 +646,10 @@ HPVCD_INST_RES(int16_t)
 HPVCD_INST_RES(int32_t)
 
 #define HPVCD_INST_INTRA(T, R)                                                                                   \
-    template __global__ void k_intra<T, R>(const DevPic*, const hpvcd_tb*, const DevUnit*, const uint32_t*,      \
-                                           uint32_t, const uint32_t*, const uint16_t*, const uint32_t*,           \
-                                           const uint32_t*, T*, const R*, uint32_t*, uint32_t*, const uint32_t*,  \
-                                           uint32_t*);
+    template __global__ void k_intra<T, R>(const DevPic*, const hpvcd_tb*, const uint64_t*, const DevUnit*,      \
+                                           const uint32_t*, uint32_t, const uint32_t*, const uint16_t*,           \
+                                           const uint16_t*, T*,                                                   \
+                                           const R*, uint32_t*, uint32_t*, const uint32_t*, uint32_t*);
 HPVCD_INST_INTRA(uint8_t, int16_t)
 HPVCD_INST_INTRA(uint8_t, int32_t)
 HPVCD_INST_INTRA(uint16_t, int16_t)

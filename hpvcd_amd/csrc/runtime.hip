@@ -155,6 +155,7 @@ struct ClassJob {
     std::vector<DevUnit> units;
     std::vector<uint32_t> tickets, plain_tickets, unit_tbs;
     std::vector<uint16_t> level_off;
+    std::vector<uint16_t> unit_refoff;             // per unit TB entry: offset of its reference map
     std::vector<uint32_t> res_list[2][4];            // [pass][log2-2], batched order
     std::vector<uint32_t> pic_units_begin;            // plain_tickets segment per picture (npics+1)
     std::vector<uint32_t> pic_res_begin[2][4];        // per picture list segments (npics+1)
@@ -168,11 +169,13 @@ struct ClassJob {
     DevUnit* d_units = nullptr;
     uint32_t *d_tickets = nullptr, *d_plain_tickets = nullptr, *d_unit_tbs = nullptr;
     uint16_t* d_level_off = nullptr;
+    uint16_t* d_unit_refoff = nullptr;
     uint32_t* d_res_list[2][4] = {};
     void *d_planes_a = nullptr, *d_planes_b = nullptr, *d_res = nullptr;
     uint32_t* d_sync = nullptr;   // [0] err, [1] border tag, [2..] ticket counters (batched + one per picture)
     uint32_t sync_words = 0;
     uint32_t* d_border = nullptr; // kBorderWords per unit
+    uint64_t* d_masks = nullptr;  // reference availability mask per TB
     hipGraphExec_t graph = nullptr;
     int graph_stop = -1;
 };
@@ -358,7 +361,13 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                         while (k < lt.size() && lt[k].first < l) k++;
                         C.level_off.push_back((uint16_t)k);
                     }
-                    for (auto& e : lt) C.unit_tbs.push_back(e.second);
+                    uint32_t roff = 0;
+                    for (auto& e : lt) {
+                        C.unit_tbs.push_back(e.second);
+                        C.unit_refoff.push_back((uint16_t)roff);
+                        roff += 4u * (1u << C.tbs[e.second].log2n) + 1u;
+                    }
+                    if (roff > (uint32_t)kMaxUnitRefs) return set_error(HPVCD_E_INVALID, "too many reference samples in one CTB");
                     C.units.push_back(U);
                     keys.push_back({(uint32_t)(cx + 2 * cy), lp, (uint32_t)pl, unit_base + u});
                 }
@@ -453,6 +462,9 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
         const bool plain = mode == HPVCD_MODE_PLAIN;
         const uint32_t p0 = plain ? (uint32_t)step : 0, pn = plain ? 1 : np;
         mark(HPVCD_KERNEL_RESIDUAL);
+        if (step == 0 && !C.tbs.empty())
+            k_avail<<<cdiv((uint32_t)C.tbs.size(), 256), 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, (uint32_t)C.tbs.size(),
+                                                                     C.d_order, C.d_cells, C.d_masks);
         for (int pass = 0; pass < 2; pass++)
             for (int sz = 0; sz < 4; sz++) {
                 uint32_t b0 = plain ? C.pic_res_begin[pass][sz][p0] : 0;
@@ -466,8 +478,9 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
             uint32_t* ctr = counters + (plain ? 1 + p0 : 0);
             uint32_t blocks = std::min<uint32_t>(nu, max_blocks);
             if (blocks)
-                k_intra<T, R><<<blocks, 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_units, tick, nu, C.d_unit_tbs, C.d_level_off,
-                                                     C.d_order, C.d_cells, pa, res, C.d_border, ctr, gen, err);
+                k_intra<T, R><<<blocks, 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_masks, C.d_units, tick, nu, C.d_unit_tbs,
+                                                     C.d_level_off, C.d_unit_refoff, pa, res, C.d_border, ctr, gen,
+                                                     err);
         }
         mark(-1);
         if (stop >= HPVCD_STAGE_DEBLOCK) {
@@ -687,7 +700,8 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
                  bytes_of<hpvcd_sao>(C.sao.size()) + bytes_of<DevUnit>(C.units.size()) +
                  bytes_of<uint32_t>(C.tickets.size()) * 2 + bytes_of<uint32_t>(C.unit_tbs.size() + 1) +
                  bytes_of<uint16_t>(C.level_off.size() + 1) + bytes_of<uint32_t>(C.sync_words) +
-                 bytes_of<uint32_t>(C.units.size() * kBorderWords + 1) +
+                 bytes_of<uint16_t>(C.unit_refoff.size() + 1) +
+                 bytes_of<uint32_t>(C.units.size() * kBorderWords + 1) + bytes_of<uint64_t>(C.tbs.size() + 1) +
                  2 * ((C.plane_elems * sb + 255) & ~(size_t)255) + ((C.res_elems * rb + 255) & ~(size_t)255) + 8 * 256;
         for (int pass = 0; pass < 2; pass++)
             for (int s = 0; s < 4; s++) total += bytes_of<uint32_t>(C.res_list[pass][s].size() + 1);
@@ -716,7 +730,8 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
             (st = upload(C.d_cells, j->mem, C.cells)) || (st = upload(C.d_order, j->mem, C.order)) ||
             (st = upload(C.d_sao, j->mem, C.sao)) || (st = upload(C.d_units, j->mem, C.units)) ||
             (st = upload(C.d_tickets, j->mem, C.tickets)) || (st = upload(C.d_plain_tickets, j->mem, C.plain_tickets)) ||
-            (st = upload(C.d_unit_tbs, j->mem, C.unit_tbs)) || (st = upload(C.d_level_off, j->mem, C.level_off))) {
+            (st = upload(C.d_unit_tbs, j->mem, C.unit_tbs)) || (st = upload(C.d_level_off, j->mem, C.level_off)) ||
+            (st = upload(C.d_unit_refoff, j->mem, C.unit_refoff))) {
             hpvcd_job_free(j);
             return st;
         }
@@ -725,6 +740,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
                 if ((st = upload(C.d_res_list[pass][s], j->mem, C.res_list[pass][s]))) { hpvcd_job_free(j); return st; }
         C.d_sync = j->mem.take<uint32_t>(C.sync_words);
         C.d_border = j->mem.take<uint32_t>(C.units.size() * kBorderWords + 1);
+        C.d_masks = j->mem.take<uint64_t>(C.tbs.size() + 1);
         if (hipMemset(C.d_sync, 0, C.sync_words * sizeof(uint32_t)) != hipSuccess ||
             hipMemset(C.d_border, 0, (C.units.size() * kBorderWords + 1) * sizeof(uint32_t)) != hipSuccess) {
             hpvcd_job_free(j);
