@@ -1,6 +1,22 @@
 // intra.hip — K2: availability masks and the intra prediction wavefront
 // (included by kernels.hip inside namespace hpvcd).
 
+// Optional cycle profiling of the wavefront (build with -DHPVCD_INTRA_PROFILE):
+// per unit and wave, shader-clock cycles spent in each phase.  Only records
+// timestamps; the computation is unchanged.
+#ifdef HPVCD_INTRA_PROFILE
+__device__ uint64_t* g_intra_prof = nullptr;
+#define HPVCD_PROF_DECL uint64_t prof_[8] = {0, 0, 0, 0, 0, 0, 0, 0}; uint64_t prof_t_ = clock64();
+#define HPVCD_PROF(k) { const uint64_t c_ = clock64(); prof_[k] += c_ - prof_t_; prof_t_ = c_; }
+#define HPVCD_PROF_FLUSH(u)                                                                                 \
+    if (lane == 0 && g_intra_prof)                                                                          \
+        for (int k_ = 0; k_ < 8; k_++) g_intra_prof[((size_t)(u) * 4 + wave) * 8 + k_] = prof_[k_];
+#else
+#define HPVCD_PROF_DECL
+#define HPVCD_PROF(k)
+#define HPVCD_PROF_FLUSH(u)
+#endif
+
 constexpr int kMaxCtb = 64;
 constexpr int kTopW = 2 * kMaxCtb + 1;     // row above the unit: x0-1 .. x0+2*ctb-1
 constexpr int kRowW = kMaxCtb + 1;         // body rows: left neighbour + the unit's samples
@@ -138,6 +154,7 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
         T* pl = samples + P.plane_off[plane];
         const R* rp = res_pool + P.res_off[plane];
         const int ntb = U.ntb, nlev = U.nlevels;
+        HPVCD_PROF_DECL
 
         // stage what does not depend on neighbours: TB records, masks,
         // levels and residuals
@@ -194,6 +211,7 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
                 }
             }
         }
+        HPVCD_PROF(0)
         // neighbour samples from the published borders: the row above
         // (corner + two unit widths) and the column to the left
         for (int e = tid; e < 2 * ctbw + 1; e += 256) {
@@ -214,6 +232,7 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
         }
         __syncthreads();
 
+        HPVCD_PROF(1)
         const uint32_t pflags = uni(P.flags);
         const bool filt_luma = plane == 0 || uni(P.chroma_idc) == 3;
         const bool smoothing = !(pflags & HPVCD_PF_INTRA_SMOOTHING_DISABLED);
@@ -236,6 +255,7 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
                     const int a = L.refmap[roff + k];
                     return a == 0xffff ? neutral : (int32_t)L.tile[a];
                 };
+                HPVCD_PROF(5)
                 // reference filtering decision (src/intra.rs:151)
                 bool do_filter = false;
                 if (smoothing && filt_luma && n != 4 && mode != 1) {
@@ -302,6 +322,7 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
                     }
                     wave_sync();
                 }
+                HPVCD_PROF(2)
                 // prediction (src/intra.rs:257); four outputs per lane per
                 // pass so their LDS reads overlap
                 int32_t dc = 0;
@@ -357,9 +378,12 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
                     }
                 }
                 wave_sync();
+                HPVCD_PROF(3)
             }
             __syncthreads();
+            HPVCD_PROF(4)
         }
+        HPVCD_PROF_FLUSH(ui)
         // publish: the whole CTB to the plane, bottom row and right column
         // as tagged border words
         for (int e = tid; e < ctbw * ctbh; e += 256) {

@@ -548,6 +548,19 @@ extern "C" {
 const char* hpvcd_last_error(void) { return g_last_error.c_str(); }
 const char* hpvcd_version(void) { return "hpvcd-mi355x 0.1.0 (gfx950)"; }
 
+#ifdef HPVCD_INTRA_PROFILE
+// profiling builds only: device buffer of units * 4 waves * 8 cycle counters
+int hpvcd_debug_intra_profile(void* device_buffer) {
+    return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_intra_prof), &device_buffer, sizeof(device_buffer));
+}
+int hpvcd_debug_unit_info(hpvcd_job* j, uint32_t u, uint32_t* out) {
+    if (j->classes.empty() || u >= j->classes[0].units.size()) return -1;
+    const auto& U = j->classes[0].units[u];
+    out[0] = U.pic; out[1] = U.cx; out[2] = U.cy; out[3] = U.plane; out[4] = U.ntb; out[5] = U.nlevels;
+    return (int)j->classes[0].units.size();
+}
+#endif
+
 hpvcd_status hpvcd_validate_batch(const hpvcd_batch* b) {
     if (!b) return set_error(HPVCD_E_INVALID, "null argument");
     return validate_batch(*b);
