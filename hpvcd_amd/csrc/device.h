@@ -45,7 +45,10 @@ struct DevUnit {
     uint16_t ntb;
     uint32_t tb_list;       // offset into unit_tbs
     uint32_t level_list;    // offset into level_off (nlevels + 1 entries)
-    int32_t dep0, dep1;     // unit indices that must finish first (-1 = none)
+    uint32_t sbase;         // first per-sample reference word of the unit
+    uint32_t nsamp;         // reference words of the unit (sum of its TB areas)
+    uint32_t rbase;         // first reference map entry of the unit
+    uint32_t nref;          // reference map entries (sum of 4n + 1 over its TBs)
 };
 
 // TB flag bits added by the runtime (upper byte of hpvcd_tb.flags).

@@ -6,85 +6,39 @@
 // timestamps; the computation is unchanged.
 #ifdef HPVCD_INTRA_PROFILE
 __device__ uint64_t* g_intra_prof = nullptr;
+__device__ uint64_t* g_intra_tl = nullptr;  // per unit: 260 wall-clock stamps (start, ready, levels..., end)
+#define HPVCD_TL(u, k) if (tid == 0 && g_intra_tl) g_intra_tl[(size_t)(u) * 260 + (k)] = wall_clock64();
 #define HPVCD_PROF_DECL uint64_t prof_[8] = {0, 0, 0, 0, 0, 0, 0, 0}; uint64_t prof_t_ = clock64();
 #define HPVCD_PROF(k) { const uint64_t c_ = clock64(); prof_[k] += c_ - prof_t_; prof_t_ = c_; }
 #define HPVCD_PROF_FLUSH(u)                                                                                 \
     if (lane == 0 && g_intra_prof)                                                                          \
-        for (int k_ = 0; k_ < 8; k_++) g_intra_prof[((size_t)(u) * 4 + wave) * 8 + k_] = prof_[k_];
+        for (int k_ = 0; k_ < 8; k_++) g_intra_prof[((size_t)(u) * 2 + wave) * 8 + k_] = prof_[k_];
 #else
 #define HPVCD_PROF_DECL
 #define HPVCD_PROF(k)
 #define HPVCD_PROF_FLUSH(u)
+#define HPVCD_TL(u, k)
 #endif
 
 constexpr int kMaxCtb = 64;
-constexpr int kTopW = 2 * kMaxCtb + 1;     // row above the unit: x0-1 .. x0+2*ctb-1
-constexpr int kRowW = kMaxCtb + 1;         // body rows: left neighbour + the unit's samples
+constexpr int kTopW = 2 * kMaxCtb + 1;            // row above the unit: x0-1 .. x0+2*ctb-1
+constexpr int kRowW = kMaxCtb + 1;                // body rows: left neighbour + the unit's samples
+constexpr int kNeutral = kTopW + kMaxCtb * kRowW; // tile slot holding 1 << (bitDepth - 1)
+constexpr int kTileWords = kNeutral + 1;
+constexpr int kFStride = 132;                     // filtered references of one TB (4 * 32 + 1)
+constexpr int kFBase = kTileWords;                // the filtered reference row follows the tile
 constexpr int kMaxUnitTbs = 256;
-constexpr int kMaxUnitRefs = 256 * 17;    // reference entries of one unit (all 4x4 TBs is the maximum)
-constexpr int kBorderWords = 2 * kMaxCtb;  // bottom row + right column per unit
+constexpr int kMaxUnitRefs = 256 * 17;            // reference entries of one unit (all 4x4 TBs is the maximum)
+constexpr int kBorderWords = 2 * kMaxCtb;         // bottom row + right column per unit
+constexpr int kIntraThreads = 128;               // wave 0 predicts, wave 1 fetches neighbour border samples
 constexpr uint32_t kSpinLimit = 1u << 19;
+constexpr int kCopyBatch = 16;                   // global loads in flight per thread when staging a unit
+constexpr int kFetchAhead = 8;                   // TBs ahead of the prediction wave whose neighbour words are polled
 
-// Reference availability of one TB as a bit mask over 4-sample groups
-// (src/decode.rs:4651 luma_avail, :4669 chroma_avail): bits [0, ngl) are the
-// left groups from the bottom up, bit ngl the corner, then the above groups
-// left to right.  Availability is constant within a group because a group
-// never spans two 4x4 luma cells.  Computed for every TB in parallel before
-// the wavefront so that the wavefront does no availability work.
-__global__ void __launch_bounds__(256) k_avail(const DevPic* __restrict__ pics, const hpvcd_tb* __restrict__ tbs,
-                                               const uint32_t* __restrict__ tb_pic, uint32_t ntbs,
-                                               const uint32_t* __restrict__ cell_order,
-                                               const uint32_t* __restrict__ cell_info, uint64_t* __restrict__ masks) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntbs) return;
-    const hpvcd_tb tb = tbs[t];
-    const DevPic& P = pics[tb_pic[t]];
-    const int comp = tb.comp;
-    const int sw = comp ? P.sub_w_log2 : 0, sh = comp ? P.sub_h_log2 : 0;
-    const int pw = comp ? P.cw : P.w, ph = comp ? P.ch : P.h;
-    const int n = 1 << tb.log2n, n2 = 2 * n;
-    const int gv = 4 >> sh, gh = 4 >> sw, ngl = n2 / gv, nga = n2 / gh;
-    const int x0 = tb.x, y0 = tb.y;
-    const bool need_info = (P.flags & (HPVCD_PF_TILES | HPVCD_PF_CONSTRAINED_INTRA)) || P.num_slices > 1;
-    const uint32_t* ord = cell_order + P.cell_begin;
-    const uint32_t* inf = cell_info + P.cell_begin;
-    const uint32_t cur = need_info ? inf[((y0 << sh) >> 2) * P.gw + ((x0 << sw) >> 2)] : 0u;
-    auto avail = [&](int px, int py) -> bool {
-        if (px < 0 || py < 0 || px >= pw || py >= ph) return false;
-        const int lx = px << sw, ly = py << sh;
-        if (lx >= P.w || ly >= P.h) return false;
-        const int c = (ly >> 2) * P.gw + (lx >> 2);
-        if (ord[c] >= tb.order) return false;
-        if (need_info) {
-            const uint32_t i = inf[c];
-            if ((P.flags & HPVCD_PF_TILES) && HPVCD_CELL_TILE(i) != HPVCD_CELL_TILE(cur)) return false;
-            if (P.num_slices > 1 && HPVCD_CELL_SLICE(i) != HPVCD_CELL_SLICE(cur)) return false;
-            if ((P.flags & HPVCD_PF_CONSTRAINED_INTRA) && !(i & HPVCD_CELL_INTRA)) return false;
-        }
-        return true;
-    };
-    uint64_t m = 0;
-    for (int g = 0; g < ngl; g++)
-        if (avail(x0 - 1, y0 + n2 - (g + 1) * gv)) m |= 1ull << g;
-    if (avail(x0 - 1, y0 - 1)) m |= 1ull << ngl;
-    for (int g = 0; g < nga; g++)
-        if (avail(x0 + g * gh, y0 - 1)) m |= 1ull << (ngl + 1 + g);
-    masks[t] = m;
-}
+constexpr int kLdsSamples = kFBase + kFStride;
 
-struct IntraLds {
-    uint16_t tile[kTopW + kMaxCtb * kRowW];
-    int32_t raw[4][132];
-    int32_t ref[4][132];
-    hpvcd_tb tb[kMaxUnitTbs];
-    uint64_t mask[kMaxUnitTbs];
-    uint16_t lev[kMaxUnitTbs + 1];
-    uint16_t refoff[kMaxUnitTbs];
-    uint16_t refmap[kMaxUnitRefs];
-    int16_t angle[33];
-    int16_t inv_angle[15];
-    uint32_t unit;
-};
+// prediction classes of the TB descriptor
+enum { kClsAngular = 0, kClsHv = 1, kClsPlanar = 2, kClsDc = 3 };
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
@@ -116,21 +70,431 @@ __device__ __forceinline__ uint16_t border_wait(const uint32_t* p, uint32_t tag,
     return (uint16_t)(v & 0xffffu);
 }
 
+// Sum over the 64 lanes of a wave with DPP row shifts and row broadcasts
+// (no LDS traffic); the result is returned to every lane.
+__device__ __forceinline__ int32_t wave_sum(int32_t v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ int32_t mul24(int32_t a, int32_t b) { return __mul24(a, b); }
+
+
+
+
+// Inclusive prefix maximum over the 64 lanes of a wave (DPP row shifts and
+// row broadcasts).
+__device__ __forceinline__ uint32_t wave_prefix_max(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true));   // row_shr:1
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true));   // row_shr:2
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true));   // row_shr:4
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true));   // row_shr:8
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return v;
+}
+
+// Static shape of one TB's prediction: everything that depends on the
+// records only (mode, size, picture flags), src/intra.rs:151 for the filter
+// decision.
+struct TbShape {
+    int n, log2n, mode, cls, angle, inv;
+    bool filt, edge_on, vertical;
+};
+
+__device__ __forceinline__ TbShape tb_shape(int log2n, int mode, int plane, uint32_t pflags, int chroma_idc) {
+    TbShape t;
+    t.log2n = log2n;
+    t.n = 1 << log2n;
+    t.mode = mode;
+    t.cls = mode == 0 ? kClsPlanar : mode == 1 ? kClsDc : (mode == 10 || mode == 26) ? kClsHv : kClsAngular;
+    t.angle = mode >= 2 ? kIntraAngle[mode - 2] : 0;
+    t.inv = (mode >= 11 && mode <= 25) ? kInvAngle[mode - 11] : 0;
+    t.vertical = mode >= 18;
+    const bool smoothing = !(pflags & HPVCD_PF_INTRA_SMOOTHING_DISABLED);
+    const bool filt_plane = plane == 0 || chroma_idc == 3;
+    t.filt = false;
+    if (smoothing && filt_plane && t.n != 4 && mode != 1) {
+        const int d26 = mode > 26 ? mode - 26 : 26 - mode, d10 = mode > 10 ? mode - 10 : 10 - mode;
+        const int min_dist = d26 < d10 ? d26 : d10;
+        const int thresh = t.n == 8 ? 7 : (t.n == 16 ? 1 : (t.n == 32 ? 0 : 8));
+        t.filt = mode == 0 || min_dist > thresh;
+    }
+    t.edge_on = plane == 0 && !(pflags & HPVCD_PF_INTRA_BOUNDARY_FILTER_OFF) && t.n < 32;
+    return t;
+}
+
+// Tile address of the substituted reference at s-index k (src/intra.rs:84):
+// s 0..2n-1 runs up the left column, 2n is the corner, 2n+1..4n runs along the
+// row above; an unavailable sample takes the nearest available one below it
+// in s order, or the first available one, or the neutral value.
+__device__ __forceinline__ int ref_tile_addr(int k, uint64_t amask, int n, int x0, int y0, int lgv, int lgh) {
+    const int n2 = 2 * n, ngl = n2 >> lgv;
+    const int b = k < n2 ? k >> lgv : (k == n2 ? ngl : ngl + 1 + ((k - n2 - 1) >> lgh));
+    int src = k;
+    if (!((amask >> b) & 1ull)) {
+        const uint64_t below = amask & ((1ull << b) - 1ull);
+        if (below) {
+            const int bb = 63 - __builtin_clzll(below);
+            src = bb < ngl ? ((bb + 1) << lgv) - 1 : (bb == ngl ? n2 : n2 + ((bb - ngl) << lgh));
+        } else if (amask) {
+            const int fb = __builtin_ctzll(amask);
+            src = fb < ngl ? fb << lgv : (fb == ngl ? n2 : n2 + 1 + ((fb - ngl - 1) << lgh));
+        } else {
+            return kNeutral;
+        }
+    }
+    if (src < n2) return kTopW + (y0 + n2 - 1 - src) * kRowW + x0;
+    return y0 == 0 ? x0 + src - n2 : kTopW + (y0 - 1) * kRowW + x0 + src - n2;
+}
+
+// s-index of the extended main reference ref[k], k in [-n, 2n], of an
+// angular mode (negative k projects the side reference with invAngle).
+__device__ __forceinline__ int ext_sidx(int k, int n, int inv, bool vertical) {
+    int off = k;
+    if (k < 0) {
+        const int idx = (k * inv + 128) >> 8;
+        off = -(idx > 2 * n ? 2 * n : (idx < 0 ? 0 : idx));
+    }
+    return vertical ? 2 * n + off : 2 * n - off;
+}
+
+// Per-sample reference word: a0 [0,13), a1 [13,26), [26,31) the angular
+// fraction or the DC edge kind, bit 31 the H/V edge flag.  a0/a1 are tile
+// addresses of substituted references (M, by s-index), or s-indices into the
+// TB's filtered reference row when the references are filtered (M null).
+__device__ __forceinline__ uint32_t sample_word(const TbShape& t, int x, int y, const uint16_t* M) {
+    const int n = t.n, n2 = 2 * n;
+    auto A = [&](int s) -> uint32_t { return M ? (uint32_t)M[s] : (uint32_t)s; };
+    if (t.cls == kClsAngular) {
+        const int a = t.vertical ? y : x, c = t.vertical ? x : y;
+        const int pos = (a + 1) * t.angle, ii = pos >> 5, frac = pos & 31;
+        const int k0 = c + ii + 1;
+        const uint32_t a0 = A(ext_sidx(k0, n, t.inv, t.vertical));
+        const uint32_t a1 = frac ? A(ext_sidx(k0 + 1, n, t.inv, t.vertical)) : a0;
+        return a0 | (a1 << 13) | ((uint32_t)frac << 26);
+    }
+    if (t.cls == kClsHv) {
+        const bool ver = t.mode == 26;
+        const int s0 = ver ? n2 + 1 + x : n2 - 1 - y, s1 = ver ? n2 - 1 - y : n2 + 1 + x;
+        const bool e = t.edge_on && (ver ? x == 0 : y == 0);
+        return A(s0) | (A(s1) << 13) | (e ? 1u << 31 : 0u);
+    }
+    if (t.cls == kClsPlanar) return A(n2 - 1 - y) | (A(n2 + 1 + x) << 13);
+    const uint32_t kind = !t.edge_on ? 0u : (x == 0 && y == 0 ? 2u : ((x == 0 || y == 0) ? 1u : 0u));
+    const int s0 = kind == 2 ? n2 - 1 : (y == 0 ? n2 + 1 + x : n2 - 1 - y);
+    return A(s0) | (A(n2 + 1) << 13) | (kind << 26);
+}
+
+// Static preparation of every TB for the wavefront, one wave per TB in unit
+// order (entry g indexes unit_tbs).  Everything here depends on the records
+// only, so it runs fully parallel before the wavefront:
+//  * reference availability (src/decode.rs:4651 luma_avail, :4669
+//    chroma_avail) as a mask over 4-sample groups - bits [0, ngl) the left
+//    groups from the bottom up, bit ngl the corner, then the above groups;
+//    availability is constant within a group because a group never spans
+//    two 4x4 luma cells;
+//  * the substituted reference map (tile address per s-index, src/intra.rs:84);
+//  * the per-sample reference words;
+//  * the TB descriptor, with the neighbour samples the TB reads.
+__global__ void __launch_bounds__(256) k_prep(const DevPic* __restrict__ pics, const hpvcd_tb* __restrict__ tbs,
+                                              const uint32_t* __restrict__ tb_pic,
+                                              const uint32_t* __restrict__ cell_order,
+                                              const uint32_t* __restrict__ cell_info, uint32_t n_entries,
+                                              const uint32_t* __restrict__ unit_tbs,
+                                              const uint32_t* __restrict__ unit_refoff,
+                                              const uint32_t* __restrict__ ent_smap,
+                                              const uint32_t* __restrict__ ent_rmap, uint4* __restrict__ desc_g,
+                                              uint16_t* __restrict__ rmap_g, uint32_t* __restrict__ smap_g) {
+    __shared__ uint16_t mref[4][kFStride];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t g = blockIdx.x * 4 + wv;
+    if (g >= n_entries) return;
+    const uint32_t t = uni(unit_tbs[g]);
+    const hpvcd_tb tb = tbs[t];
+    const DevPic& P = pics[uni(tb_pic[t])];
+    const int plane = tb.comp;
+    const int sw = plane ? P.sub_w_log2 : 0, sh = plane ? P.sub_h_log2 : 0;
+    const int pw = plane ? P.cw : P.w, ph = plane ? P.ch : P.h;
+    const int ctbw = (1 << P.ctb_log2) >> sw, ctbh = (1 << P.ctb_log2) >> sh;
+    const int n = 1 << tb.log2n, n2 = 2 * n;
+    const int gv = 4 >> sh, gh = 4 >> sw, ngl = n2 / gv, nga = n2 / gh;
+    const int x = tb.x, y = tb.y, x0 = x & (ctbw - 1), y0 = y & (ctbh - 1);
+    const bool need_info = (P.flags & (HPVCD_PF_TILES | HPVCD_PF_CONSTRAINED_INTRA)) || P.num_slices > 1;
+    const uint32_t* ord = cell_order + P.cell_begin;
+    const uint32_t* inf = cell_info + P.cell_begin;
+    const uint32_t cur = need_info ? inf[((y << sh) >> 2) * P.gw + ((x << sw) >> 2)] : 0u;
+    auto avail = [&](int px, int py) -> bool {
+        if (px < 0 || py < 0 || px >= pw || py >= ph) return false;
+        const int lx = px << sw, ly = py << sh;
+        if (lx >= P.w || ly >= P.h) return false;
+        const int c = (ly >> 2) * P.gw + (lx >> 2);
+        if (ord[c] >= tb.order) return false;
+        if (need_info) {
+            const uint32_t i = inf[c];
+            if ((P.flags & HPVCD_PF_TILES) && HPVCD_CELL_TILE(i) != HPVCD_CELL_TILE(cur)) return false;
+            if (P.num_slices > 1 && HPVCD_CELL_SLICE(i) != HPVCD_CELL_SLICE(cur)) return false;
+            if ((P.flags & HPVCD_PF_CONSTRAINED_INTRA) && !(i & HPVCD_CELL_INTRA)) return false;
+        }
+        return true;
+    };
+    bool av = false;
+    if (lane < ngl) av = avail(x - 1, y + n2 - (lane + 1) * gv);
+    else if (lane == ngl) av = avail(x - 1, y - 1);
+    else if (lane < ngl + 1 + nga) av = avail(x + (lane - ngl - 1) * gh, y - 1);
+    const uint64_t amask = __ballot(av);
+
+    // substituted reference map and the neighbour samples it reads
+    const uint32_t ro = unit_refoff[g];
+    const int lgv = 2 - sh, lgh = 2 - sw;
+    uint16_t* M = mref[wv];
+    uint16_t* rdst = rmap_g + ent_rmap[g];
+    uint32_t top_need = 0, left_need = 0;
+    for (int k = lane; k <= 2 * n2; k += 64) {
+        const int addr = ref_tile_addr(k, amask, n, x0, y0, lgv, lgh);
+        if (addr < kTopW) {
+            top_need = max(top_need, (uint32_t)addr + 1u);
+        } else if (addr < kNeutral) {
+            const int r = (addr - kTopW) / kRowW;
+            if (addr - kTopW - r * kRowW == 0) left_need = max(left_need, (uint32_t)r + 1u);
+        }
+        M[k] = (uint16_t)addr;
+        rdst[k] = (uint16_t)addr;
+    }
+    top_need = (uint32_t)__builtin_amdgcn_readlane((int)wave_prefix_max(top_need), 63);
+    left_need = (uint32_t)__builtin_amdgcn_readlane((int)wave_prefix_max(left_need), 63);
+    wave_sync();
+
+    const TbShape s = tb_shape(tb.log2n, tb.mode, plane, P.flags, P.chroma_idc);
+    const uint16_t* Ms = s.filt ? nullptr : M;
+    if (lane == 0) {
+        auto A = [&](int s_) -> uint32_t { return Ms ? (uint32_t)Ms[s_] : (uint32_t)s_; };
+        uint32_t u0 = 0, u1 = 0;
+        if (s.cls == kClsHv) {
+            u0 = A(s.mode == 26 ? n2 + 1 : n2 - 1);
+            u1 = A(n2);
+        } else if (s.cls == kClsPlanar) {
+            u0 = A(n2 + 1 + n);
+            u1 = A(n2 - 1 - n);
+        }
+        const bool strong_on = plane == 0 && (P.flags & HPVCD_PF_STRONG_INTRA_SMOOTHING);
+        uint4 d;
+        d.x = (uint32_t)x0 | ((uint32_t)y0 << 6) | ((uint32_t)(tb.log2n - 2) << 12) | ((uint32_t)s.cls << 14) |
+              ((tb.flags & TBX_HAS_RES) ? 1u << 16 : 0u) | (s.filt ? 1u << 17 : 0u) |
+              ((s.filt && strong_on && n == 32) ? 1u << 18 : 0u) | (y0 + n == ctbh ? 1u << 19 : 0u) |
+              (x0 + n == ctbw ? 1u << 20 : 0u);
+        d.y = (ro & 0x1fffu) | (((ro >> 13) & 0xffu) << 17);  // sample word offset = 16 * field
+        d.z = u0 | (u1 << 13);
+        d.w = top_need | (left_need << 8);
+        desc_g[g] = d;
+    }
+    uint32_t* sdst = smap_g + ent_smap[g];
+    for (int p = lane; p < n * n; p += 64) sdst[p] = sample_word(s, p & (n - 1), p >> tb.log2n, Ms);
+}
+
+// Per-TB descriptor, decoded once per TB with scalar bit-field extracts.
+//   x: [0,6) x0  [6,12) y0 (unit relative)  [12,14) log2n-2  [14,16) class
+//      16 residual  17 filtered references  18 strong-filter candidate
+//      19 writes the unit's bottom row  20 writes its right column
+//   y: [0,13) reference map offset  [13,25) sample word offset
+//   z: uniform references of the class: [0,13) u0  [13,26) u1
+struct IntraLds {
+    uint16_t tile[kLdsSamples];
+    uint32_t smap[kMaxCtb * kMaxCtb];
+    uint4 desc[kMaxUnitTbs];
+    uint16_t refmap[kMaxUnitRefs];
+    uint32_t pneed[kMaxUnitTbs];      // prefix maxima over the TB order of the neighbour samples read:
+                                      // row-above words | left-column rows << 16
+    uint32_t bready;                  // TBs whose neighbour samples are all in the tile
+    uint32_t unit;
+};
+
+// Prediction of one n x n TB (n = 1 << LOG2N) from its per-sample reference
+// words (src/intra.rs:257), residual add and write into the unit tile.  Lane l
+// predicts samples l, l + 64, ... in raster order.  S indexes the tile and the
+// filtered reference rows (fbase selects the row of this wave); u0/u1 are the
+// class's uniform references.  For n <= 8 the word and residual of the lane's
+// sample may come prefetched (pf).
+template <int LOG2N, int CLS, typename R>
+__device__ __forceinline__ void intra_predict(const uint16_t* __restrict__ S, const uint32_t* __restrict__ W,
+                                              uint16_t* __restrict__ dst, const R* __restrict__ rs, bool has_res,
+                                              bool pf, uint32_t w_pf, int32_t r_pf, int lane, int fbase, int u0,
+                                              int u1, int32_t dc, int32_t maxv, uint32_t* pub_bottom,
+                                              uint32_t* pub_right, uint32_t tagw) {
+    constexpr int n = 1 << LOG2N, nn = n * n;
+    constexpr int NQ = nn < 64 ? 1 : nn / 64;
+    constexpr int CH = NQ < 4 ? NQ : 4;  // samples per lane whose LDS reads overlap
+    const int32_t ru0 = (CLS == kClsHv || CLS == kClsPlanar) ? (int32_t)S[u0] : 0;
+    const int32_t ru1 = (CLS == kClsHv || CLS == kClsPlanar) ? (int32_t)S[u1] : 0;
+#pragma unroll 1
+    for (int q0 = 0; q0 < NQ; q0 += CH) {
+        uint32_t w[CH];
+        int32_t res[CH], v[CH];
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            const int p = ((q0 + c) * 64 + lane) & (nn - 1);
+            if (NQ == 1 && pf) {
+                w[c] = w_pf;
+                res[c] = r_pf;
+            } else {
+                w[c] = W[p];
+                res[c] = has_res ? (int32_t)rs[(p >> LOG2N) * kMaxCtb + (p & (n - 1))] : 0;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            const int p = ((q0 + c) * 64 + lane) & (nn - 1);
+            const int x = p & (n - 1), y = p >> LOG2N;
+            const int32_t r0 = S[(w[c] & 0x1fffu) + fbase], r1 = S[((w[c] >> 13) & 0x1fffu) + fbase];
+            const int k = (w[c] >> 26) & 31;
+            int32_t r;
+            if (CLS == kClsAngular) {
+                r = (mul24(32 - k, r0) + mul24(k, r1) + 16) >> 5;
+            } else if (CLS == kClsHv) {
+                r = (int32_t)(w[c] >> 31) ? clampi(ru0 + ((r1 - ru1) >> 1), 0, maxv) : r0;
+            } else if (CLS == kClsPlanar) {
+                r = (mul24(n - 1 - x, r0) + mul24(x + 1, ru0) + mul24(n - 1 - y, r1) + mul24(y + 1, ru1) + n) >>
+                    (LOG2N + 1);
+            } else {
+                r = k == 0 ? dc : (k == 1 ? (r0 + 3 * dc + 2) >> 2 : (r0 + 2 * dc + r1 + 2) >> 2);
+            }
+            v[c] = clampi(wadd(r, res[c]), 0, maxv);
+        }
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            const int p = ((q0 + c) * 64 + lane) & (nn - 1);
+            if (nn >= 64 || lane < nn) {
+                const int x = p & (n - 1), y = p >> LOG2N;
+                dst[y * kRowW + x] = (uint16_t)v[c];
+                // the unit's bottom row and right column go out at once as
+                // tagged words for the units below and to the right
+                if (pub_bottom && y == n - 1) border_store(pub_bottom + x, tagw | (uint32_t)v[c]);
+                if (pub_right && x == n - 1) border_store(pub_right + y, tagw | (uint32_t)v[c]);
+            }
+        }
+    }
+}
+
+// One TB step of the wavefront for TBs of size n = 1 << LOG2N: filtered
+// references ([1 2 1] or strong, src/intra.rs:151) into this wave's row when
+// the TB filters, the DC value, then prediction.
+template <int LOG2N, typename R>
+__device__ __forceinline__ void intra_tb(IntraLds& L, const R* __restrict__ resb, uint32_t d0, uint32_t d1, uint32_t d2,
+                                         bool pf, uint32_t w_pf, int32_t r_pf, int lane, int bd, int32_t maxv,
+                                         uint32_t* unit_border, uint32_t tagw) {
+    constexpr int n = 1 << LOG2N, n2 = 2 * n, n4 = 4 * n;
+    constexpr int NIT = (n4 + 64) / 64;  // passes over the 4n + 1 reference slots
+    uint16_t* S = L.tile;
+    const int x0 = d0 & 63, y0 = (d0 >> 6) & 63, cls = (d0 >> 14) & 3;
+    const bool has_res = (d0 >> 16) & 1, filt = (d0 >> 17) & 1, scand = (d0 >> 18) & 1;
+    const uint16_t* M = L.refmap + (d1 & 0x1fff);
+    const uint32_t* W = L.smap + ((d1 >> 13) & 0xfff);
+    int fbase = 0;
+    if (LOG2N > 2 && filt) {
+        uint16_t* F = S + kFBase;
+        bool strong = false;
+        int32_t bl = 0, tl = 0, tr = 0;
+        if (LOG2N == 5 && scand) {
+            bl = S[M[0]];
+            tl = S[M[n2]];
+            tr = S[M[n4]];
+            const int thr = 1 << (bd - 5);
+            const int a = bl + tl - 2 * (int32_t)S[M[n]], b = tr + tl - 2 * (int32_t)S[M[3 * n]];
+            strong = uni((int)((a < 0 ? -a : a) < thr && (b < 0 ? -b : b) < thr)) != 0;
+        }
+        if (LOG2N == 5 && strong) {
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                const int i = it * 64 + lane;
+                if (i <= n4)
+                    F[i] = (uint16_t)(i < n2 ? (i * tl + (n2 - i) * bl + n) >> 6 : ((n4 - i) * tl + (i - n2) * tr + n) >> 6);
+            }
+        } else {
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                const int i = it * 64 + lane;
+                if (i <= n4) {
+                    const bool endp = i == 0 || i == n4;  // end samples stay unfiltered
+                    const int i0 = endp ? i : i - 1, i2 = endp ? i : i + 1;
+                    F[i] = (uint16_t)(((int32_t)S[M[i0]] + 2 * (int32_t)S[M[i]] + (int32_t)S[M[i2]] + 2) >> 2);
+                }
+            }
+        }
+        wave_sync();
+        fbase = kFBase;
+    }
+    int32_t dc = 0;
+    if (cls == kClsDc) {  // DC never filters: sum of the n left and n above references
+        int32_t v = 0;
+        if (lane < n2) v = S[M[lane < n ? n2 - n + lane : n2 + 1 + lane - n]];
+        dc = (wave_sum(v) + n) >> (LOG2N + 1);
+    }
+    const int u0 = (int)(d2 & 0x1fff) + fbase, u1 = (int)((d2 >> 13) & 0x1fff) + fbase;
+    uint16_t* dst = S + kTopW + y0 * kRowW + x0 + 1;
+    const R* rs = resb + y0 * kMaxCtb + x0;
+    uint32_t* pb = ((d0 >> 19) & 1) ? unit_border + x0 : nullptr;
+    uint32_t* pr = ((d0 >> 20) & 1) ? unit_border + kMaxCtb + y0 : nullptr;
+    switch (cls) {
+        case kClsAngular:
+            intra_predict<LOG2N, kClsAngular, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, pb, pr, tagw);
+            break;
+        case kClsHv:
+            intra_predict<LOG2N, kClsHv, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, pb, pr, tagw);
+            break;
+        case kClsPlanar:
+            intra_predict<LOG2N, kClsPlanar, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, pb, pr, tagw);
+            break;
+        default:
+            intra_predict<LOG2N, kClsDc, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, pb, pr, tagw);
+    }
+    wave_sync();
+}
+
+// Global -> LDS copy with kCopyBatch loads in flight per thread.
+template <typename V>
+__device__ __forceinline__ void batch_copy(V* __restrict__ dst, const V* __restrict__ src, int count, int tid) {
+    for (int e0 = tid; e0 < count; e0 += kCopyBatch * kIntraThreads) {
+        V v[kCopyBatch];
+#pragma unroll
+        for (int b = 0; b < kCopyBatch; b++) {
+            const int e = e0 + b * kIntraThreads;
+            if (e < count) v[b] = src[e];
+        }
+#pragma unroll
+        for (int b = 0; b < kCopyBatch; b++) {
+            const int e = e0 + b * kIntraThreads;
+            if (e < count) dst[e] = v[b];
+        }
+    }
+}
+
+__device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
+__device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
+
+// The intra wavefront.  A persistent grid takes (picture, plane, CTB) units
+// in ticket order (a topological order of the CTB dependencies).  Per unit,
+// wave 0 reconstructs the TBs one after the other in an order where every TB
+// follows the TBs it references, and publishes the unit's bottom row and
+// right column as tagged border words the moment they are written; wave 1
+// polls the neighbours' border words the unit reads and moves them into the
+// unit tile, counting in L.bready how many TBs can start.  The two waves meet
+// only at the start and end of a unit.
 template <typename T, typename R>
-__global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, const hpvcd_tb* __restrict__ tbs,
-                                               const uint64_t* __restrict__ masks, const DevUnit* __restrict__ units,
-                                               const uint32_t* __restrict__ ticket_units, uint32_t n_units,
-                                               const uint32_t* __restrict__ unit_tbs,
-                                               const uint16_t* __restrict__ level_off,
-                                               const uint16_t* __restrict__ unit_refoff, T* __restrict__ samples,
-                                               const R* __restrict__ res_pool, uint32_t* __restrict__ border,
-                                               uint32_t* counter, const uint32_t* gen, uint32_t* err) {
+__global__ void __launch_bounds__(kIntraThreads)
+    k_intra(const DevPic* __restrict__ pics, const DevUnit* __restrict__ units,
+            const uint32_t* __restrict__ ticket_units, uint32_t n_units, const uint4* __restrict__ desc_g,
+            const uint16_t* __restrict__ rmap_g, const uint32_t* __restrict__ smap_g, T* __restrict__ samples,
+            const R* __restrict__ res_pool, uint32_t* __restrict__ border, uint32_t* counter, const uint32_t* gen,
+            uint32_t* err) {
     __shared__ IntraLds L;
     __shared__ R resb[kMaxCtb * kMaxCtb];
     const int tid = threadIdx.x, wave = uni(tid >> 6), lane = tid & 63;
     const uint32_t tag = *gen & 0xffffu;
-    if (tid < 33) L.angle[tid] = kIntraAngle[tid];
-    if (tid < 15) L.inv_angle[tid] = kInvAngle[tid];
 
     for (;;) {
         if (tid == 0) L.unit = atomicAdd(counter, 1u);
@@ -147,256 +511,220 @@ __global__ void __launch_bounds__(256) k_intra(const DevPic* __restrict__ pics, 
         const int sw = plane ? P.sub_w_log2 : 0, sh = plane ? P.sub_h_log2 : 0;
         const int ctb = 1 << P.ctb_log2;
         const int ctbw = ctb >> sw, ctbh = ctb >> sh;       // unit size in plane samples
+        const int ctbw_log2 = P.ctb_log2 - sw;
         const int pw = plane ? P.cw : P.w, ph = plane ? P.ch : P.h;
         const int ux0 = U.cx * ctbw, uy0 = U.cy * ctbh;     // unit origin (plane samples)
         const int bd = plane ? P.bd_c : P.bd;
         const int32_t maxv = (1 << bd) - 1;
         T* pl = samples + P.plane_off[plane];
         const R* rp = res_pool + P.res_off[plane];
-        const int ntb = U.ntb, nlev = U.nlevels;
+        const int ntb = U.ntb;
         HPVCD_PROF_DECL
+        HPVCD_TL(ui, 0)
 
-        // stage what does not depend on neighbours: TB records, masks,
-        // levels and residuals
-        for (int e = tid; e < ntb; e += 256) {
-            const uint32_t t = unit_tbs[U.tb_list + e];
-            L.tb[e] = tbs[t];
-            L.mask[e] = masks[t];
-            L.refoff[e] = unit_refoff[U.tb_list + e];
-        }
-        for (int e = tid; e <= nlev; e += 256) L.lev[e] = level_off[U.level_list + e];
-        for (int e = tid; e < ctbw * ctbh; e += 256) {
-            int ry = e / ctbw, rx = e - ry * ctbw;
-            int px = ux0 + rx, py = uy0 + ry;
-            resb[ry * kMaxCtb + rx] = (px < pw && py < ph) ? rp[(size_t)py * pw + px] : (R)0;
-        }
-        __syncthreads();
-        // Reference address maps of every TB of the unit: tile address of the
-        // substituted reference for each s-index (src/intra.rs:84), 0xffff for
-        // the neutral value.  Availability does not depend on sample values,
-        // so this runs before the neighbours are ready and is off the
-        // dependency chain.  s-index 0..2n-1 runs up the left column, 2n is
-        // the corner, 2n+1..4n runs along the row above.
+        // stage the unit: TB descriptors, reference maps and words (all
+        // prepared before the wavefront) and residuals; every thread keeps
+        // kCopyBatch loads in flight
         {
-            const int lgv = 2 - sh, lgh = 2 - sw;
-            for (int t = wave; t < ntb; t += 4) {
-                const hpvcd_tb tb = L.tb[t];
-                const uint64_t amask = uni64(L.mask[t]);
-                const int n = 1 << uni((int)tb.log2n), n2 = 2 * n, nref = 4 * n + 1;
-                const int x0 = uni((int)tb.x) - ux0, y0 = uni((int)tb.y) - uy0;
-                const int ngl = n2 >> lgv;
-                const int roff = uni((int)L.refoff[t]);
-                const int first_b = amask ? __builtin_ctzll(amask) : -1;
-                const int first = first_b < 0 ? -1
-                                  : (first_b < ngl ? first_b << lgv
-                                                   : (first_b == ngl ? n2 : n2 + 1 + ((first_b - ngl - 1) << lgh)));
-                for (int k = lane; k < nref; k += 64) {
-                    const int b = k < n2 ? k >> lgv : (k == n2 ? ngl : ngl + 1 + ((k - n2 - 1) >> lgh));
-                    int src = k;
-                    if (!((amask >> b) & 1ull)) {
-                        const uint64_t below = amask & ((1ull << b) - 1ull);
-                        if (below) {
-                            const int bb = 63 - __builtin_clzll(below);
-                            src = bb < ngl ? ((bb + 1) << lgv) - 1 : (bb == ngl ? n2 : n2 + ((bb - ngl) << lgh));
-                        } else {
-                            src = first;
-                        }
-                    }
-                    int addr = 0xffff;
-                    if (src >= 0)
-                        addr = src < n2 ? kTopW + (y0 + n2 - 1 - src) * kRowW + x0
-                                        : (src == n2 ? (y0 == 0 ? x0 : kTopW + (y0 - 1) * kRowW + x0)
-                                                     : (y0 == 0 ? x0 + src - n2 : kTopW + (y0 - 1) * kRowW + x0 + src - n2));
-                    L.refmap[roff + k] = (uint16_t)addr;
+            batch_copy(reinterpret_cast<uint32_t*>(L.desc), reinterpret_cast<const uint32_t*>(desc_g + U.tb_list), 4 * ntb, tid);
+            batch_copy(L.refmap, rmap_g + U.rbase, (int)U.nref, tid);
+            batch_copy(L.smap, smap_g + U.sbase, (int)U.nsamp, tid);
+            const int nres = ctbw * ctbh;
+            for (int e0 = tid; e0 < nres; e0 += kCopyBatch * kIntraThreads) {
+                R v[kCopyBatch];
+#pragma unroll
+                for (int b = 0; b < kCopyBatch; b++) {
+                    const int e = e0 + b * kIntraThreads;
+                    const int ry = e >> ctbw_log2, rx = e - (ry << ctbw_log2);
+                    const int px = ux0 + rx, py = uy0 + ry;
+                    v[b] = (e < nres && px < pw && py < ph) ? rp[(size_t)py * pw + px] : (R)0;
+                }
+#pragma unroll
+                for (int b = 0; b < kCopyBatch; b++) {
+                    const int e = e0 + b * kIntraThreads;
+                    if (e < nres) resb[(e >> ctbw_log2) * kMaxCtb + (e & (ctbw - 1))] = v[b];
                 }
             }
         }
-        HPVCD_PROF(0)
-        // neighbour samples from the published borders: the row above
-        // (corner + two unit widths) and the column to the left
-        for (int e = tid; e < 2 * ctbw + 1; e += 256) {
-            int px = ux0 - 1 + e, py = uy0 - 1;
-            uint16_t v = 0;
-            if (py >= 0 && px >= 0 && px < pw) {
-                int ncx = px / ctbw;
-                uint32_t nu = ui - (uint32_t)U.cx + (uint32_t)ncx - (uint32_t)P.ctb_cols;  // unit (ncx, cy - 1)
-                v = border_wait(border + (size_t)nu * kBorderWords + (px - ncx * ctbw), tag, err);
-            }
-            L.tile[e] = v;
-        }
-        for (int e = tid; e < ctbh; e += 256) {
-            int px = ux0 - 1, py = uy0 + e;
-            uint16_t v = 0;
-            if (px >= 0 && py < ph) v = border_wait(border + (size_t)(ui - 1) * kBorderWords + kMaxCtb + e, tag, err);
-            L.tile[kTopW + e * kRowW] = v;
+        if (tid == 0) {
+            L.bready = 0;
+            L.tile[kNeutral] = (uint16_t)(1 << (bd - 1));
         }
         __syncthreads();
+        HPVCD_PROF(7)
+        HPVCD_TL(ui, 1)
 
-        HPVCD_PROF(1)
-        const uint32_t pflags = uni(P.flags);
-        const bool filt_luma = plane == 0 || uni(P.chroma_idc) == 3;
-        const bool smoothing = !(pflags & HPVCD_PF_INTRA_SMOOTHING_DISABLED);
-        const bool strong_on = plane == 0 && (pflags & HPVCD_PF_STRONG_INTRA_SMOOTHING);
-        const bool edge = plane == 0 && !(pflags & HPVCD_PF_INTRA_BOUNDARY_FILTER_OFF);
-        const int32_t neutral = 1 << (bd - 1);
-        const int lgv = 2 - sh, lgh = 2 - sw;                  // log2 of the availability group sizes
-        for (int lv = 0; lv < nlev; lv++) {
-            const int lo = uni((int)L.lev[lv]), hi = uni((int)L.lev[lv + 1]);
-            for (int j = lo + wave; j < hi; j += 4) {
-                const hpvcd_tb tb = L.tb[j];
-                // TB fields are the same in every lane: keep them scalar
-                const int log2n = uni((int)tb.log2n), mode = uni((int)tb.mode);
-                const int x0 = uni((int)tb.x) - ux0, y0 = uni((int)tb.y) - uy0;
-                const bool has_res = (uni((int)tb.flags) & TBX_HAS_RES) != 0;
-                const int n = 1 << log2n, n2 = 2 * n, nref = 4 * n + 1;
-                const int roff = uni((int)L.refoff[j]);
-                // substituted reference at s-index k (map built in the prologue)
-                auto sval = [&](int k) -> int32_t {
-                    const int a = L.refmap[roff + k];
-                    return a == 0xffff ? neutral : (int32_t)L.tile[a];
-                };
-                HPVCD_PROF(5)
-                // reference filtering decision (src/intra.rs:151)
-                bool do_filter = false;
-                if (smoothing && filt_luma && n != 4 && mode != 1) {
-                    int d26 = mode > 26 ? mode - 26 : 26 - mode, d10 = mode > 10 ? mode - 10 : 10 - mode;
-                    int min_dist = d26 < d10 ? d26 : d10;
-                    int thresh = n == 8 ? 7 : (n == 16 ? 1 : (n == 32 ? 0 : 8));
-                    do_filter = mode == 0 || min_dist > thresh;
-                }
-                // angular modes other than pure H/V read an extended main
-                // reference ref[k], k in [-n, 2n] (stored at k + n):
-                // main[k] for k >= 0, the projected side reference below
-                const bool ext = mode >= 2 && mode != 10 && mode != 26;
-                const bool vertical = mode >= 18;
-                const int angle = mode >= 2 ? L.angle[mode - 2] : 0;
-                const int inv = (mode >= 11 && mode <= 25) ? L.inv_angle[mode - 11] : 0;
-                auto ext_sidx = [&](int k) -> int {
-                    int off = k;
-                    if (k < 0) {
-                        int idx = (k * inv + 128) >> 8;
-                        off = -(idx > n2 ? n2 : (idx < 0 ? 0 : idx));
-                    }
-                    return vertical ? n2 + off : n2 - off;
-                };
-                int32_t* S = L.raw[wave];
-                int32_t* F = L.ref[wave];
-                int32_t dcsum = 0;
-                if (!do_filter) {
-                    // one pass: substituted references straight into the
-                    // layout the predictor reads
-                    if (ext) {
-                        for (int k = lane; k <= 3 * n; k += 64) F[k] = sval(ext_sidx(k - n));
-                    } else {
-                        for (int i = lane; i < nref; i += 64) {
-                            const int32_t v = sval(i);
-                            F[i] = v;
-                            if ((i >= n2 - n && i < n2) || (i > n2 && i <= n2 + n)) dcsum += v;
-                        }
-                    }
-                    wave_sync();
-                } else {
-                    for (int i = lane; i < nref; i += 64) S[i] = sval(i);
-                    wave_sync();
-                    bool strong = false;
-                    const int32_t bl = S[0], tl = S[n2], tr = S[4 * n];
-                    if (strong_on && n == 32) {
-                        const int thr = 1 << (bd - 5);
-                        const int a = bl + tl - 2 * S[n], b = tr + tl - 2 * S[3 * n];
-                        strong = (a < 0 ? -a : a) < thr && (b < 0 ? -b : b) < thr;
-                    }
-                    auto fval = [&](int i) -> int32_t {
-                        if (i == 0 || i == 4 * n) return S[i];
-                        if (strong) {
-                            if (i == n2) return tl;
-                            if (i > n2) { const int k = i - n2; return ((n2 - k) * tl + k * tr + n) >> 6; }
-                            const int k = n2 - i;
-                            return ((n2 - k) * tl + k * bl + n) >> 6;
-                        }
-                        return (S[i - 1] + 2 * S[i] + S[i + 1] + 2) >> 2;
-                    };
-                    if (ext) {
-                        for (int k = lane; k <= 3 * n; k += 64) F[k] = fval(ext_sidx(k - n));
-                    } else {
-                        for (int i = lane; i < nref; i += 64) F[i] = fval(i);
-                    }
-                    wave_sync();
-                }
-                HPVCD_PROF(2)
-                // prediction (src/intra.rs:257); four outputs per lane per
-                // pass so their LDS reads overlap
-                int32_t dc = 0;
-                if (mode == 1) {
-#pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) dcsum += __shfl_xor(dcsum, off);
-                    dc = (dcsum + n) >> (log2n + 1);
-                }
-                const int nn = n * n;
-                for (int base = 0; base < nn; base += 256) {
-                    int32_t vals[4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        vals[q] = 0;
-                        if (base + q * 64 >= nn) continue;  // wave-uniform: small TBs use one pass
-                        const int p = base + q * 64 + lane;
-                        const int x = p & (n - 1), y = (p >> log2n) & (n - 1);
-                        int32_t v;
-                        if (ext) {
-                            const int a = vertical ? y : x, c = vertical ? x : y;
-                            const int pos = (a + 1) * angle, ii = pos >> 5, frac = pos & 31;
-                            const int32_t r0 = F[c + ii + 1 + n];
-                            v = frac == 0 ? r0 : ((32 - frac) * r0 + frac * F[c + ii + 2 + n] + 16) >> 5;
-                            v = clampi(v, 0, maxv);
-                        } else if (mode == 26) {
-                            v = F[n2 + 1 + x];
-                            if (edge && n < 32 && x == 0) v = clampi(F[n2 + 1] + ((F[n2 - 1 - y] - F[n2]) >> 1), 0, maxv);
-                        } else if (mode == 10) {
-                            v = F[n2 - 1 - y];
-                            if (edge && n < 32 && y == 0) v = clampi(F[n2 - 1] + ((F[n2 + 1 + x] - F[n2]) >> 1), 0, maxv);
-                        } else if (mode == 0) {
-                            const int32_t h = (n - 1 - x) * F[n2 - 1 - y] + (x + 1) * F[n2 + n + 1];
-                            const int32_t vv = (n - 1 - y) * F[n2 + 1 + x] + (y + 1) * F[n2 - n - 1];
-                            v = (h + vv + n) >> (log2n + 1);
-                        } else {
-                            v = dc;
-                            if (edge && n < 32) {
-                                if (x == 0 && y == 0) v = (F[n2 - 1] + 2 * dc + F[n2 + 1] + 2) >> 2;
-                                else if (y == 0) v = (F[n2 + 1 + x] + 3 * dc + 2) >> 2;
-                                else if (x == 0) v = (F[n2 - 1 - y] + 3 * dc + 2) >> 2;
-                            }
-                        }
-                        if (has_res) v = clampi(wadd(v, (int32_t)resb[(y0 + y) * kMaxCtb + x0 + x]), 0, maxv);
-                        vals[q] = v;
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        if (base + q * 64 >= nn) continue;
-                        const int p = base + q * 64 + lane;
-                        const int x = p & (n - 1), y = (p >> log2n) & (n - 1);
-                        if (ux0 + x0 + x < pw && uy0 + y0 + y < ph)
-                            L.tile[kTopW + (y0 + y) * kRowW + x0 + x + 1] = (uint16_t)vals[q];
-                    }
-                }
-                wave_sync();
-                HPVCD_PROF(3)
+        uint32_t* const bw = border + (size_t)ui * kBorderWords;
+        const uint32_t tagw = tag << 16;
+        if (wave == 1) {
+            // ---- neighbour samples ----
+            // prefix maxima of the per-TB needs over the TB order
+            uint32_t carry_t = 0, carry_l = 0;
+            for (int c = 0; c < ntb; c += 64) {
+                const int i = c + lane;
+                const uint32_t w = i < ntb ? L.desc[i].w : 0u;
+                const uint32_t vt = max(wave_prefix_max(w & 0xffu), carry_t);
+                const uint32_t vl = max(wave_prefix_max((w >> 8) & 0xffu), carry_l);
+                if (i < ntb) L.pneed[i] = vt | (vl << 16);
+                carry_t = (uint32_t)__builtin_amdgcn_readlane((int)vt, 63);
+                carry_l = (uint32_t)__builtin_amdgcn_readlane((int)vl, 63);
             }
-            __syncthreads();
+            // lane l owns row-above words l, l + 64, l + 128 and left row l;
+            // tile[e] of the row above is sample (ux0 - 1 + e, uy0 - 1), the
+            // bottom row of unit (ncx, cy - 1); left rows come from the right
+            // column of unit (cx - 1, cy)
+            const int all_top = (int)carry_t, all_left = (int)carry_l;
+            uint32_t valid = 0, done = 0;
+#pragma unroll
+            for (int w = 0; w < 3; w++)
+                if (lane + 64 * w < all_top && ux0 - 1 + lane + 64 * w >= 0) valid |= 1u << w;
+            if (lane < all_left) valid |= 8u;
+            auto wptr = [&](int w) -> const uint32_t* {
+                if (w == 3) return border + (size_t)(ui - 1) * kBorderWords + kMaxCtb + lane;
+                const int px = ux0 - 1 + lane + 64 * w, ncx = px >> ctbw_log2;
+                const uint32_t nu = ui - (uint32_t)U.cx + (uint32_t)ncx - (uint32_t)P.ctb_cols;
+                return border + (size_t)nu * kBorderWords + (px - (ncx << ctbw_log2));
+            };
+            int ready = 0;
+            uint32_t spins = 0;
+            HPVCD_PROF(3)
+            for (;;) {
+#ifdef HPVCD_INTRA_PROFILE
+                prof_[5]++;
+#endif
+                // poll only the missing words that the next few TBs read
+                const int horizon = min(ntb, ready + kFetchAhead) - 1;
+                const uint32_t hn = L.pneed[horizon];
+                const int want_top = (int)(hn & 0xffffu), want_left = (int)(hn >> 16);
+                uint32_t want = 0;
+#pragma unroll
+                for (int w = 0; w < 3; w++)
+                    if (lane + 64 * w < want_top) want |= 1u << w;
+                if (lane < want_left) want |= 8u;
+                const uint32_t poll = valid & ~done & want;
+                uint32_t hv[4];
+#pragma unroll
+                for (int w = 0; w < 4; w++) hv[w] = ((poll >> w) & 1u) ? border_load(wptr(w)) : 0u;
+#pragma unroll
+                for (int w = 0; w < 4; w++)
+                    if (((poll >> w) & 1u) && (hv[w] >> 16) == tag) {
+                        L.tile[w == 3 ? kTopW + lane * kRowW : lane + 64 * w] = (uint16_t)(hv[w] & 0xffffu);
+                        done |= 1u << w;
+                    }
+                // contiguous prefixes of the row above / left column now present
+                const uint32_t miss = valid & ~done;
+                int top_have = all_top, left_have = all_left;
+#pragma unroll
+                for (int w = 0; w < 3; w++) {
+                    const uint64_t m = __ballot((miss >> w) & 1u);
+                    if (m) top_have = min(top_have, 64 * w + (int)__builtin_ctzll(m));
+                }
+                {
+                    const uint64_t m = __ballot((miss >> 3) & 1u);
+                    if (m) left_have = min(left_have, (int)__builtin_ctzll(m));
+                }
+                // TBs whose reads are all covered
+                int nr = ready;
+                for (;;) {
+                    const int i = nr + lane;
+                    const uint32_t pn = i < ntb ? L.pneed[i] : 0u;
+                    const bool ok = i < ntb && (int)(pn & 0xffffu) <= top_have && (int)(pn >> 16) <= left_have;
+                    const uint64_t okm = __ballot(ok);
+                    const int cnt = okm == ~0ull ? 64 : (int)__builtin_ctzll(~okm);
+                    nr += cnt;
+                    if (cnt < 64 || nr >= ntb) break;
+                }
+                const bool progress = nr > ready;
+                if (progress) {
+                    ready = nr;
+                    lds_release();
+                    if (lane == 0) __hip_atomic_store(&L.bready, (uint32_t)ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                if (ready >= ntb) break;
+                if (progress) continue;
+                if (++spins > kSpinLimit || ((spins & 1023u) == 0 && border_load(err) != 0)) {
+                    // give up: release the prediction wave (the error flag fails the run)
+                    atomicOr(err, 1u);
+                    lds_release();
+                    if (lane == 0) __hip_atomic_store(&L.bready, (uint32_t)ntb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);  // back off: the words come from units still running
+            }
             HPVCD_PROF(4)
+        } else {
+            // ---- prediction ----
+            // the descriptor of the next TB and, for TBs of up to 8 x 8, its
+            // reference words and residuals are loaded while the current TB runs
+            uint32_t cd0 = 0, cd1 = 0, cd2 = 0, cw = 0;
+            int32_t cr = 0;
+            uint4 nraw = make_uint4(0, 0, 0, 0);
+            auto prefetch = [&](uint32_t d0, uint32_t d1, uint32_t& w, int32_t& r) {
+                if (((d0 >> 12) & 3) <= 1) {
+                    const int log2n = ((d0 >> 12) & 3) + 2, n = 1 << log2n;
+                    const int p = lane & (n * n - 1);
+                    w = L.smap[((d1 >> 13) & 0xfff) + p];
+                    r = ((d0 >> 16) & 1) ? (int32_t)resb[(((d0 >> 6) & 63) + (p >> log2n)) * kMaxCtb + (d0 & 63) + (p & (n - 1))]
+                                         : 0;
+                }
+            };
+            if (ntb > 0) {
+                const uint4 d = L.desc[0];
+                cd0 = uni(d.x);
+                cd1 = uni(d.y);
+                cd2 = uni(d.z);
+                prefetch(cd0, cd1, cw, cr);
+                nraw = L.desc[min(1, kMaxUnitTbs - 1)];
+            }
+            int ready = 0;
+            HPVCD_PROF(0)
+            for (int i = 0; i < ntb; i++) {
+                if (i >= ready) {
+                    HPVCD_PROF(2)
+                    uint32_t spins = 0;
+                    for (;;) {
+                        ready = (int)uni(__hip_atomic_load(&L.bready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+                        if (ready > i) break;
+                        if (++spins > (kSpinLimit << 4)) {
+                            atomicOr(err, 1u);
+                            ready = ntb;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    lds_acquire();
+                    HPVCD_PROF(1)
+                }
+                const uint32_t nd0 = uni(nraw.x), nd1 = uni(nraw.y), nd2 = uni(nraw.z);
+                uint32_t nw = 0;
+                int32_t nr = 0;
+                if (i + 1 < ntb) prefetch(nd0, nd1, nw, nr);
+                const uint4 nnraw = L.desc[min(i + 2, kMaxUnitTbs - 1)];
+                HPVCD_PROF(2)
+                switch ((cd0 >> 12) & 3) {
+                    case 0: intra_tb<2, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); HPVCD_PROF(3) break;
+                    case 1: intra_tb<3, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); HPVCD_PROF(4) break;
+                    case 2: intra_tb<4, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); HPVCD_PROF(5) break;
+                    default: intra_tb<5, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); HPVCD_PROF(6) break;
+                }
+                cd0 = nd0;
+                cd1 = nd1;
+                cd2 = nd2;
+                cw = nw;
+                cr = nr;
+                nraw = nnraw;
+            }
+            HPVCD_PROF(2)
         }
+        __syncthreads();
         HPVCD_PROF_FLUSH(ui)
-        // publish: the whole CTB to the plane, bottom row and right column
-        // as tagged border words
-        for (int e = tid; e < ctbw * ctbh; e += 256) {
-            int ry = e / ctbw, rx = e - ry * ctbw;
+        HPVCD_TL(ui, 259)
+        // the whole CTB to the plane
+        for (int e = tid; e < ctbw * ctbh; e += kIntraThreads) {
+            int ry = e >> ctbw_log2, rx = e - (ry << ctbw_log2);
             int px = ux0 + rx, py = uy0 + ry;
             if (px < pw && py < ph) pl[(size_t)py * pw + px] = (T)L.tile[kTopW + ry * kRowW + rx + 1];
         }
-        uint32_t* bw = border + (size_t)ui * kBorderWords;
-        for (int e = tid; e < ctbw; e += 256)
-            border_store(bw + e, (tag << 16) | L.tile[kTopW + (ctbh - 1) * kRowW + e + 1]);
-        for (int e = tid; e < ctbh; e += 256)
-            border_store(bw + kMaxCtb + e, (tag << 16) | L.tile[kTopW + e * kRowW + ctbw]);
         __syncthreads();
     }
 }
-

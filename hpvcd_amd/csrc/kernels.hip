@@ -646,10 +646,9 @@ HPVCD_INST_RES(int16_t)
 HPVCD_INST_RES(int32_t)
 
 #define HPVCD_INST_INTRA(T, R)                                                                                   \
-    template __global__ void k_intra<T, R>(const DevPic*, const hpvcd_tb*, const uint64_t*, const DevUnit*,      \
-                                           const uint32_t*, uint32_t, const uint32_t*, const uint16_t*,           \
-                                           const uint16_t*, T*,                                                   \
-                                           const R*, uint32_t*, uint32_t*, const uint32_t*, uint32_t*);
+    template __global__ void k_intra<T, R>(const DevPic*, const DevUnit*, const uint32_t*, uint32_t, const uint4*,  \
+                                           const uint16_t*, const uint32_t*, T*, const R*, uint32_t*, uint32_t*,   \
+                                           const uint32_t*, uint32_t*);
 HPVCD_INST_INTRA(uint8_t, int16_t)
 HPVCD_INST_INTRA(uint8_t, int32_t)
 HPVCD_INST_INTRA(uint16_t, int16_t)

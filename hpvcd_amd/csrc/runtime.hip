@@ -155,7 +155,9 @@ struct ClassJob {
     std::vector<DevUnit> units;
     std::vector<uint32_t> tickets, plain_tickets, unit_tbs;
     std::vector<uint16_t> level_off;
-    std::vector<uint16_t> unit_refoff;             // per unit TB entry: offset of its reference map
+    std::vector<uint32_t> unit_refoff;             // per unit TB entry: map offset | sample offset / 16 << 13 | level << 21
+    std::vector<uint32_t> ent_smap, ent_rmap;      // per unit TB entry: first sample word / reference map entry
+    size_t smap_words = 0, rmap_words = 0;
     std::vector<uint32_t> res_list[2][4];            // [pass][log2-2], batched order
     std::vector<uint32_t> pic_units_begin;            // plain_tickets segment per picture (npics+1)
     std::vector<uint32_t> pic_res_begin[2][4];        // per picture list segments (npics+1)
@@ -169,13 +171,16 @@ struct ClassJob {
     DevUnit* d_units = nullptr;
     uint32_t *d_tickets = nullptr, *d_plain_tickets = nullptr, *d_unit_tbs = nullptr;
     uint16_t* d_level_off = nullptr;
-    uint16_t* d_unit_refoff = nullptr;
+    uint32_t* d_unit_refoff = nullptr;
+    uint32_t *d_ent_smap = nullptr, *d_ent_rmap = nullptr;
+    uint32_t* d_smap = nullptr;   // per-sample reference words
+    uint16_t* d_rmap = nullptr;   // substituted reference maps
+    uint4* d_desc = nullptr;      // wavefront TB descriptors (per unit TB entry)
     uint32_t* d_res_list[2][4] = {};
     void *d_planes_a = nullptr, *d_planes_b = nullptr, *d_res = nullptr;
     uint32_t* d_sync = nullptr;   // [0] err, [1] border tag, [2..] ticket counters (batched + one per picture)
     uint32_t sync_words = 0;
     uint32_t* d_border = nullptr; // kBorderWords per unit
-    uint64_t* d_masks = nullptr;  // reference availability mask per TB
     hipGraphExec_t graph = nullptr;
     int graph_stop = -1;
 };
@@ -321,9 +326,6 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                     DevUnit U;
                     std::memset(&U, 0, sizeof(U));
                     U.pic = lp; U.cx = (uint16_t)cx; U.cy = (uint16_t)cy; U.plane = (uint8_t)pl;
-                    U.dep0 = cx > 0 ? (int32_t)(unit_base + u - 1) : -1;
-                    if (cy > 0) U.dep1 = (int32_t)(unit_base + u - D.ctb_cols + (cx + 1 < D.ctb_cols ? 1 : 0));
-                    else U.dep1 = -1;
                     // dependency levels inside the unit
                     int sw = pl ? D.sub_w_log2 : 0, sh = pl ? D.sub_h_log2 : 0;
                     int ctbw = ctb >> sw, ctbh = ctb >> sh, cgw = ctbw / 4, cgh = ctbh / 4;
@@ -361,13 +363,26 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                         while (k < lt.size() && lt[k].first < l) k++;
                         C.level_off.push_back((uint16_t)k);
                     }
-                    uint32_t roff = 0;
+                    // per TB: reference map offset, sample word offset (a
+                    // multiple of 16) and level
+                    uint32_t roff = 0, soff = 0;
+                    U.sbase = (uint32_t)C.smap_words;
+                    U.rbase = (uint32_t)C.rmap_words;
                     for (auto& e : lt) {
+                        const uint32_t n = 1u << C.tbs[e.second].log2n;
+                        if (roff + 4 * n + 1 > (uint32_t)kMaxUnitRefs || soff + n * n > (uint32_t)(kMaxCtb * kMaxCtb))
+                            return set_error(HPVCD_E_INVALID, "too many reference samples in one CTB");
                         C.unit_tbs.push_back(e.second);
-                        C.unit_refoff.push_back((uint16_t)roff);
-                        roff += 4u * (1u << C.tbs[e.second].log2n) + 1u;
+                        C.unit_refoff.push_back(roff | ((soff >> 4) << 13) | ((uint32_t)e.first << 21));
+                        C.ent_smap.push_back(U.sbase + soff);
+                        C.ent_rmap.push_back(U.rbase + roff);
+                        roff += 4 * n + 1;
+                        soff += n * n;
                     }
-                    if (roff > (uint32_t)kMaxUnitRefs) return set_error(HPVCD_E_INVALID, "too many reference samples in one CTB");
+                    U.nsamp = soff;
+                    U.nref = roff;
+                    C.smap_words += soff;
+                    C.rmap_words += roff;
                     C.units.push_back(U);
                     keys.push_back({(uint32_t)(cx + 2 * cy), lp, (uint32_t)pl, unit_base + u});
                 }
@@ -462,9 +477,6 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
         const bool plain = mode == HPVCD_MODE_PLAIN;
         const uint32_t p0 = plain ? (uint32_t)step : 0, pn = plain ? 1 : np;
         mark(HPVCD_KERNEL_RESIDUAL);
-        if (step == 0 && !C.tbs.empty())
-            k_avail<<<cdiv((uint32_t)C.tbs.size(), 256), 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, (uint32_t)C.tbs.size(),
-                                                                     C.d_order, C.d_cells, C.d_masks);
         for (int pass = 0; pass < 2; pass++)
             for (int sz = 0; sz < 4; sz++) {
                 uint32_t b0 = plain ? C.pic_res_begin[pass][sz][p0] : 0;
@@ -472,15 +484,22 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
                 launch_residual<R>(s, C, d_scaling, pass, sz + 2, C.d_res_list[pass][sz] + b0, b1 - b0);
             }
         mark(HPVCD_KERNEL_INTRA);
+        if (step == 0 && !C.tbs.empty()) {
+            // reference availability and per-sample reference words of every
+            // TB (records only, fully parallel)
+            const uint32_t ne = (uint32_t)C.unit_tbs.size();
+            k_prep<<<cdiv(ne, 4), 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, C.d_order, C.d_cells, ne, C.d_unit_tbs,
+                                                C.d_unit_refoff, C.d_ent_smap, C.d_ent_rmap, C.d_desc, C.d_rmap,
+                                                C.d_smap);
+        }
         {
             const uint32_t* tick = plain ? C.d_plain_tickets + C.pic_units_begin[p0] : C.d_tickets;
             uint32_t nu = plain ? C.pic_units_begin[p0 + 1] - C.pic_units_begin[p0] : nunits;
             uint32_t* ctr = counters + (plain ? 1 + p0 : 0);
             uint32_t blocks = std::min<uint32_t>(nu, max_blocks);
             if (blocks)
-                k_intra<T, R><<<blocks, 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_masks, C.d_units, tick, nu, C.d_unit_tbs,
-                                                     C.d_level_off, C.d_unit_refoff, pa, res, C.d_border, ctr, gen,
-                                                     err);
+                k_intra<T, R><<<blocks, kIntraThreads, 0, s>>>(C.d_pics, C.d_units, tick, nu, C.d_desc, C.d_rmap, C.d_smap, pa,
+                                                     res, C.d_border, ctr, gen, err);
         }
         mark(-1);
         if (stop >= HPVCD_STAGE_DEBLOCK) {
@@ -549,9 +568,13 @@ const char* hpvcd_last_error(void) { return g_last_error.c_str(); }
 const char* hpvcd_version(void) { return "hpvcd-mi355x 0.1.0 (gfx950)"; }
 
 #ifdef HPVCD_INTRA_PROFILE
-// profiling builds only: device buffer of units * 4 waves * 8 cycle counters
+// profiling builds only: device buffer of units * (kIntraWaves + 1) waves * 8 cycle counters
 int hpvcd_debug_intra_profile(void* device_buffer) {
     return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_intra_prof), &device_buffer, sizeof(device_buffer));
+}
+// profiling builds only: device buffer of units * 260 wall-clock stamps
+int hpvcd_debug_intra_timeline(void* device_buffer) {
+    return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_intra_tl), &device_buffer, sizeof(device_buffer));
 }
 int hpvcd_debug_unit_info(hpvcd_job* j, uint32_t u, uint32_t* out) {
     if (j->classes.empty() || u >= j->classes[0].units.size()) return -1;
@@ -629,10 +652,10 @@ hpvcd_status hpvcd_gpu_open(int device, hpvcd_gpu** out) {
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     g->num_cus = prop.multiProcessorCount;
     HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[0], k_intra<uint8_t, int16_t>, 256, 0));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[1], k_intra<uint8_t, int32_t>, 256, 0));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[2], k_intra<uint16_t, int16_t>, 256, 0));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[3], k_intra<uint16_t, int32_t>, 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[0], k_intra<uint8_t, int16_t>, kIntraThreads, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[1], k_intra<uint8_t, int32_t>, kIntraThreads, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[2], k_intra<uint16_t, int16_t>, kIntraThreads, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[3], k_intra<uint16_t, int32_t>, kIntraThreads, 0));
     *out = g;
     return HPVCD_OK;
 }
@@ -713,7 +736,9 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
                  bytes_of<hpvcd_sao>(C.sao.size()) + bytes_of<DevUnit>(C.units.size()) +
                  bytes_of<uint32_t>(C.tickets.size()) * 2 + bytes_of<uint32_t>(C.unit_tbs.size() + 1) +
                  bytes_of<uint16_t>(C.level_off.size() + 1) + bytes_of<uint32_t>(C.sync_words) +
-                 bytes_of<uint16_t>(C.unit_refoff.size() + 1) +
+                 bytes_of<uint32_t>(C.unit_refoff.size() + 1) + 2 * bytes_of<uint32_t>(C.ent_smap.size() + 1) +
+                 bytes_of<uint32_t>(C.smap_words + 1) + bytes_of<uint16_t>(C.rmap_words + 1) +
+                 bytes_of<uint4>(C.unit_tbs.size() + 1) +
                  bytes_of<uint32_t>(C.units.size() * kBorderWords + 1) + bytes_of<uint64_t>(C.tbs.size() + 1) +
                  2 * ((C.plane_elems * sb + 255) & ~(size_t)255) + ((C.res_elems * rb + 255) & ~(size_t)255) + 8 * 256;
         for (int pass = 0; pass < 2; pass++)
@@ -744,7 +769,8 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
             (st = upload(C.d_sao, j->mem, C.sao)) || (st = upload(C.d_units, j->mem, C.units)) ||
             (st = upload(C.d_tickets, j->mem, C.tickets)) || (st = upload(C.d_plain_tickets, j->mem, C.plain_tickets)) ||
             (st = upload(C.d_unit_tbs, j->mem, C.unit_tbs)) || (st = upload(C.d_level_off, j->mem, C.level_off)) ||
-            (st = upload(C.d_unit_refoff, j->mem, C.unit_refoff))) {
+            (st = upload(C.d_unit_refoff, j->mem, C.unit_refoff)) || (st = upload(C.d_ent_smap, j->mem, C.ent_smap)) ||
+            (st = upload(C.d_ent_rmap, j->mem, C.ent_rmap))) {
             hpvcd_job_free(j);
             return st;
         }
@@ -753,7 +779,9 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
                 if ((st = upload(C.d_res_list[pass][s], j->mem, C.res_list[pass][s]))) { hpvcd_job_free(j); return st; }
         C.d_sync = j->mem.take<uint32_t>(C.sync_words);
         C.d_border = j->mem.take<uint32_t>(C.units.size() * kBorderWords + 1);
-        C.d_masks = j->mem.take<uint64_t>(C.tbs.size() + 1);
+        C.d_smap = j->mem.take<uint32_t>(C.smap_words + 1);
+        C.d_rmap = j->mem.take<uint16_t>(C.rmap_words + 1);
+        C.d_desc = j->mem.take<uint4>(C.unit_tbs.size() + 1);
         if (hipMemset(C.d_sync, 0, C.sync_words * sizeof(uint32_t)) != hipSuccess ||
             hipMemset(C.d_border, 0, (C.units.size() * kBorderWords + 1) * sizeof(uint32_t)) != hipSuccess) {
             hpvcd_job_free(j);
