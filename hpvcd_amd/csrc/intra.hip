@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(256) k_prep(const DevPic* __restrict__ pics, c
         d.x = (uint32_t)x0 | ((uint32_t)y0 << 6) | ((uint32_t)(tb.log2n - 2) << 12) | ((uint32_t)s.cls << 14) |
               ((tb.flags & TBX_HAS_RES) ? 1u << 16 : 0u) | (s.filt ? 1u << 17 : 0u) |
               ((s.filt && strong_on && n == 32) ? 1u << 18 : 0u) | (y0 + n == ctbh ? 1u << 19 : 0u) |
-              (x0 + n == ctbw ? 1u << 20 : 0u);
+              (x0 + n == ctbw ? 1u << 20 : 0u) | ((n <= 8 && !s.filt && s.cls != kClsDc) ? 1u << 21 : 0u);
         d.y = (ro & 0x1fffu) | (((ro >> 13) & 0xffu) << 17);  // sample word offset = 16 * field
         d.z = u0 | (u1 << 13);
         d.w = top_need | (left_need << 8);
@@ -301,6 +301,7 @@ __global__ void __launch_bounds__(256) k_prep(const DevPic* __restrict__ pics, c
 //   x: [0,6) x0  [6,12) y0 (unit relative)  [12,14) log2n-2  [14,16) class
 //      16 residual  17 filtered references  18 strong-filter candidate
 //      19 writes the unit's bottom row  20 writes its right column
+//      21 fast path: at most 8 x 8, references unfiltered, not DC
 //   y: [0,13) reference map offset  [13,25) sample word offset
 //   z: uniform references of the class: [0,13) u0  [13,26) u1
 struct IntraLds {
@@ -324,8 +325,8 @@ template <int LOG2N, int CLS, typename R>
 __device__ __forceinline__ void intra_predict(const uint16_t* __restrict__ S, const uint32_t* __restrict__ W,
                                               uint16_t* __restrict__ dst, const R* __restrict__ rs, bool has_res,
                                               bool pf, uint32_t w_pf, int32_t r_pf, int lane, int fbase, int u0,
-                                              int u1, int32_t dc, int32_t maxv, uint32_t* pub_bottom,
-                                              uint32_t* pub_right, uint32_t tagw) {
+                                              int u1, int32_t dc, int32_t maxv, uint32_t* __restrict__ bw,
+                                              int pub_b, int pub_r, uint32_t tagw) {
     constexpr int n = 1 << LOG2N, nn = n * n;
     constexpr int NQ = nn < 64 ? 1 : nn / 64;
     constexpr int CH = NQ < 4 ? NQ : 4;  // samples per lane whose LDS reads overlap
@@ -361,7 +362,8 @@ __device__ __forceinline__ void intra_predict(const uint16_t* __restrict__ S, co
                 r = (mul24(n - 1 - x, r0) + mul24(x + 1, ru0) + mul24(n - 1 - y, r1) + mul24(y + 1, ru1) + n) >>
                     (LOG2N + 1);
             } else {
-                r = k == 0 ? dc : (k == 1 ? (r0 + 3 * dc + 2) >> 2 : (r0 + 2 * dc + r1 + 2) >> 2);
+                // kind 0: dc; 1: (r0 + 3 dc + 2) >> 2; 2: (r0 + 2 dc + r1 + 2) >> 2
+                r = ((k ? r0 : 0) + (4 - k) * dc + ((k >> 1) ? r1 : 0) + 2) >> 2;
             }
             v[c] = clampi(wadd(r, res[c]), 0, maxv);
         }
@@ -373,8 +375,8 @@ __device__ __forceinline__ void intra_predict(const uint16_t* __restrict__ S, co
                 dst[y * kRowW + x] = (uint16_t)v[c];
                 // the unit's bottom row and right column go out at once as
                 // tagged words for the units below and to the right
-                if (pub_bottom && y == n - 1) border_store(pub_bottom + x, tagw | (uint32_t)v[c]);
-                if (pub_right && x == n - 1) border_store(pub_right + y, tagw | (uint32_t)v[c]);
+                if (pub_b >= 0 && y == n - 1) border_store(bw + pub_b + x, tagw | (uint32_t)v[c]);
+                if (pub_r >= 0 && x == n - 1) border_store(bw + pub_r + y, tagw | (uint32_t)v[c]);
             }
         }
     }
@@ -437,20 +439,20 @@ __device__ __forceinline__ void intra_tb(IntraLds& L, const R* __restrict__ resb
     const int u0 = (int)(d2 & 0x1fff) + fbase, u1 = (int)((d2 >> 13) & 0x1fff) + fbase;
     uint16_t* dst = S + kTopW + y0 * kRowW + x0 + 1;
     const R* rs = resb + y0 * kMaxCtb + x0;
-    uint32_t* pb = ((d0 >> 19) & 1) ? unit_border + x0 : nullptr;
-    uint32_t* pr = ((d0 >> 20) & 1) ? unit_border + kMaxCtb + y0 : nullptr;
+    const int pb = ((d0 >> 19) & 1) ? x0 : -1;            // bottom-row word of column x0
+    const int pr = ((d0 >> 20) & 1) ? kMaxCtb + y0 : -1;  // right-column word of row y0
     switch (cls) {
         case kClsAngular:
-            intra_predict<LOG2N, kClsAngular, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, pb, pr, tagw);
+            intra_predict<LOG2N, kClsAngular, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
             break;
         case kClsHv:
-            intra_predict<LOG2N, kClsHv, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, pb, pr, tagw);
+            intra_predict<LOG2N, kClsHv, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
             break;
         case kClsPlanar:
-            intra_predict<LOG2N, kClsPlanar, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, pb, pr, tagw);
+            intra_predict<LOG2N, kClsPlanar, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
             break;
         default:
-            intra_predict<LOG2N, kClsDc, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, pb, pr, tagw);
+            intra_predict<LOG2N, kClsDc, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
     }
     wave_sync();
 }
@@ -471,6 +473,37 @@ __device__ __forceinline__ void batch_copy(V* __restrict__ dst, const V* __restr
             if (e < count) dst[e] = v[b];
         }
     }
+}
+
+// The common TB step: at most 8 x 8, unfiltered references, not DC, one
+// sample per lane.  w/rv are the lane's prefetched reference word and
+// residual, loff the lane's sample offset in the tile (y * kRowW + x).
+__device__ __forceinline__ void intra_tb_fast(uint16_t* __restrict__ S, uint32_t d0, uint32_t d2, uint32_t w,
+                                              int32_t rv, int lane, int loff, int32_t maxv,
+                                              uint32_t* __restrict__ bw, uint32_t tagw) {
+    const int log2n = ((d0 >> 12) & 3) + 2, n = 1 << log2n, nn = n * n;
+    const int x0 = d0 & 63, y0 = (d0 >> 6) & 63, cls = (d0 >> 14) & 3;
+    const int32_t r0 = S[w & 0x1fffu], r1 = S[(w >> 13) & 0x1fffu];
+    const int k = (w >> 26) & 31;
+    const int p = lane & (nn - 1), x = p & (n - 1), y = p >> log2n;
+    int32_t r;
+    if (cls == kClsAngular) {
+        r = (mul24(32 - k, r0) + mul24(k, r1) + 16) >> 5;
+    } else {
+        const int32_t ru0 = S[d2 & 0x1fffu], ru1 = S[(d2 >> 13) & 0x1fffu];
+        if (cls == kClsHv)
+            r = (int32_t)(w >> 31) ? clampi(ru0 + ((r1 - ru1) >> 1), 0, maxv) : r0;
+        else  // planar
+            r = (mul24(n - 1 - x, r0) + mul24(x + 1, ru0) + mul24(n - 1 - y, r1) + mul24(y + 1, ru1) + n) >>
+                (log2n + 1);
+    }
+    const int32_t v = clampi(wadd(r, rv), 0, maxv);
+    if (lane < nn) {
+        S[kTopW + y0 * kRowW + x0 + 1 + loff] = (uint16_t)v;
+        if (((d0 >> 19) & 1) && y == n - 1) border_store(bw + x0 + x, tagw | (uint32_t)v);
+        if (((d0 >> 20) & 1) && x == n - 1) border_store(bw + kMaxCtb + y0 + y, tagw | (uint32_t)v);
+    }
+    wave_sync();
 }
 
 __device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
@@ -659,13 +692,15 @@ __global__ void __launch_bounds__(kIntraThreads)
             uint32_t cd0 = 0, cd1 = 0, cd2 = 0, cw = 0;
             int32_t cr = 0;
             uint4 nraw = make_uint4(0, 0, 0, 0);
+            // the lane's first sample of a 4 x 4 and an 8 x 8 TB: offsets in
+            // the tile and in the residual block
+            const int lt4 = ((lane & 15) >> 2) * kRowW + (lane & 3), lr4 = ((lane & 15) >> 2) * kMaxCtb + (lane & 3);
+            const int lt8 = (lane >> 3) * kRowW + (lane & 7), lr8 = (lane >> 3) * kMaxCtb + (lane & 7);
             auto prefetch = [&](uint32_t d0, uint32_t d1, uint32_t& w, int32_t& r) {
                 if (((d0 >> 12) & 3) <= 1) {
-                    const int log2n = ((d0 >> 12) & 3) + 2, n = 1 << log2n;
-                    const int p = lane & (n * n - 1);
-                    w = L.smap[((d1 >> 13) & 0xfff) + p];
-                    r = ((d0 >> 16) & 1) ? (int32_t)resb[(((d0 >> 6) & 63) + (p >> log2n)) * kMaxCtb + (d0 & 63) + (p & (n - 1))]
-                                         : 0;
+                    const bool small = ((d0 >> 12) & 3) == 0;
+                    w = L.smap[((d1 >> 13) & 0xfff) + (small ? (lane & 15) : lane)];
+                    r = ((d0 >> 16) & 1) ? (int32_t)resb[((d0 >> 6) & 63) * kMaxCtb + (d0 & 63) + (small ? lr4 : lr8)] : 0;
                 }
             };
             if (ntb > 0) {
@@ -701,7 +736,10 @@ __global__ void __launch_bounds__(kIntraThreads)
                 if (i + 1 < ntb) prefetch(nd0, nd1, nw, nr);
                 const uint4 nnraw = L.desc[min(i + 2, kMaxUnitTbs - 1)];
                 HPVCD_PROF(2)
-                switch ((cd0 >> 12) & 3) {
+                if ((cd0 >> 21) & 1) {
+                    intra_tb_fast(L.tile, cd0, cd2, cw, cr, lane, ((cd0 >> 12) & 3) ? lt8 : lt4, maxv, bw, tagw);
+                    HPVCD_PROF(3)
+                } else switch ((cd0 >> 12) & 3) {
                     case 0: intra_tb<2, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); HPVCD_PROF(3) break;
                     case 1: intra_tb<3, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); HPVCD_PROF(4) break;
                     case 2: intra_tb<4, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); HPVCD_PROF(5) break;
