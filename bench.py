@@ -45,6 +45,38 @@ def kernel_alg_bytes(stats: dict, sample_b: int, res_b: int, out_b: int) -> dict
     }
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a copy-kernel calibration), or
+    None when no summary exists for this workload."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else int(k["hbm_bytes_per_launch"])
+
+
+def copy_calibration(nbytes: int) -> None:
+    """Three device-to-device copies of `nbytes` (the HIP runtime's copy
+    kernel: nbytes read, nbytes written), through the same HIP runtime the
+    library uses."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so", mode=C.RTLD_GLOBAL)
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc for the copy calibration failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    for _ in range(3):
+        hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +86,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (read + write) three times, for calibrating "
+                         "the FETCH_SIZE/WRITE_SIZE counters of a profiling run")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -76,6 +111,8 @@ def main() -> int:
     mpix = st["out_pixels"] / 1e6
 
     with H.Gpu(0) as g:
+        if args.copy_calibration > 0:
+            copy_calibration(args.copy_calibration << 20)
         job = g.job(rs)
         # warmup (also builds the hipGraph)
         for _ in range(max(1, args.warmup)):
@@ -107,7 +144,7 @@ def main() -> int:
 
     value = mpix / (wall_ms / 1e3)
     alg = kernel_alg_bytes(st, sample_b, res_b, out_b)
-    dominant = max(kms, key=lambda k: kms[k])
+    dominant = max((k for k in kms if k in alg), key=lambda k: kms[k])
     dom_ms = kms[dominant]
     achieved = alg[dominant] / (dom_ms / 1e3) / 1e9
     total_alg = 4 * st["nnz"] + 6 * st["samples"] * sample_b + 3 * st["out_pixels"] * out_b
@@ -182,7 +219,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(args.workload, dominant),
             "alg_bytes_per_launch": alg[dominant],
         },
         "cpu_baseline": cpu,

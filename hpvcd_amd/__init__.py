@@ -235,6 +235,6 @@ class Job:
 
     def time(self, mode: int = N.MODE_GRAPH, warmup: int = 2, iters: int = 10, breakdown: bool = True):
         ms = C.c_double()
-        km = (C.c_double * 5)()
+        km = (C.c_double * len(N.KERNEL_NAMES))()
         check(N.lib().hpvcd_job_time(self._h, mode, warmup, iters, C.byref(ms), km if breakdown else None))
-        return ms.value, {N.KERNEL_NAMES[i]: km[i] for i in range(5)} if breakdown else None
+        return ms.value, {N.KERNEL_NAMES[i]: km[i] for i in range(len(N.KERNEL_NAMES))} if breakdown else None

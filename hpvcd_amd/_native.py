@@ -58,7 +58,7 @@ STAGE_DEBLOCK = 1
 STAGE_SAO = 2
 STAGE_RGB = 3
 
-KERNEL_NAMES = ("residual", "intra", "deblock", "sao", "rgb")
+KERNEL_NAMES = ("residual", "intra", "deblock", "sao", "rgb", "prep")
 
 
 class SliceParams(C.Structure):

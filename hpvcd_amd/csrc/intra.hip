@@ -13,7 +13,10 @@ __device__ uint64_t* g_intra_tl = nullptr;  // per unit: 260 wall-clock stamps (
 #define HPVCD_PROF_FLUSH(u)                                                                                 \
     if (lane == 0 && g_intra_prof)                                                                          \
         for (int k_ = 0; k_ < 8; k_++) g_intra_prof[((size_t)(u) * 2 + wave) * 8 + k_] = prof_[k_];
+__device__ uint64_t* g_prep_prof = nullptr;  // per unit: cycles of the k_prep phases
+#define HPVCD_PREP_T(k) if (tid == 0 && g_prep_prof) g_prep_prof[(size_t)blockIdx.x * 8 + (k)] = clock64();
 #else
+#define HPVCD_PREP_T(k)
 #define HPVCD_PROF_DECL
 #define HPVCD_PROF(k)
 #define HPVCD_PROF_FLUSH(u)
@@ -107,14 +110,15 @@ struct TbShape {
     bool filt, edge_on, vertical;
 };
 
-__device__ __forceinline__ TbShape tb_shape(int log2n, int mode, int plane, uint32_t pflags, int chroma_idc) {
+__device__ __forceinline__ TbShape tb_shape(int log2n, int mode, int plane, uint32_t pflags, int chroma_idc,
+                                            const int16_t* angles, const int16_t* inv_angles) {
     TbShape t;
     t.log2n = log2n;
     t.n = 1 << log2n;
     t.mode = mode;
     t.cls = mode == 0 ? kClsPlanar : mode == 1 ? kClsDc : (mode == 10 || mode == 26) ? kClsHv : kClsAngular;
-    t.angle = mode >= 2 ? kIntraAngle[mode - 2] : 0;
-    t.inv = (mode >= 11 && mode <= 25) ? kInvAngle[mode - 11] : 0;
+    t.angle = mode >= 2 ? angles[mode - 2] : 0;
+    t.inv = (mode >= 11 && mode <= 25) ? inv_angles[mode - 11] : 0;
     t.vertical = mode >= 18;
     const bool smoothing = !(pflags & HPVCD_PF_INTRA_SMOOTHING_DISABLED);
     const bool filt_plane = plane == 0 || chroma_idc == 3;
@@ -191,9 +195,9 @@ __device__ __forceinline__ uint32_t sample_word(const TbShape& t, int x, int y, 
     return A(s0) | (A(n2 + 1) << 13) | (kind << 26);
 }
 
-// Static preparation of every TB for the wavefront, one wave per TB in unit
-// order (entry g indexes unit_tbs).  Everything here depends on the records
-// only, so it runs fully parallel before the wavefront:
+// Static preparation of every TB for the wavefront: one block per unit.
+// Everything here depends on the records only, so it runs fully parallel
+// before the wavefront:
 //  * reference availability (src/decode.rs:4651 luma_avail, :4669
 //    chroma_avail) as a mask over 4-sample groups - bits [0, ngl) the left
 //    groups from the bottom up, bit ngl the corner, then the above groups;
@@ -202,99 +206,144 @@ __device__ __forceinline__ uint32_t sample_word(const TbShape& t, int x, int y, 
 //  * the substituted reference map (tile address per s-index, src/intra.rs:84);
 //  * the per-sample reference words;
 //  * the TB descriptor, with the neighbour samples the TB reads.
-__global__ void __launch_bounds__(256) k_prep(const DevPic* __restrict__ pics, const hpvcd_tb* __restrict__ tbs,
-                                              const uint32_t* __restrict__ tb_pic,
-                                              const uint32_t* __restrict__ cell_order,
-                                              const uint32_t* __restrict__ cell_info, uint32_t n_entries,
-                                              const uint32_t* __restrict__ unit_tbs,
-                                              const uint32_t* __restrict__ unit_refoff,
-                                              const uint32_t* __restrict__ ent_smap,
-                                              const uint32_t* __restrict__ ent_rmap, uint4* __restrict__ desc_g,
-                                              uint16_t* __restrict__ rmap_g, uint32_t* __restrict__ smap_g) {
-    __shared__ uint16_t mref[4][kFStride];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t g = blockIdx.x * 4 + wv;
-    if (g >= n_entries) return;
-    const uint32_t t = uni(unit_tbs[g]);
-    const hpvcd_tb tb = tbs[t];
-    const DevPic& P = pics[uni(tb_pic[t])];
-    const int plane = tb.comp;
-    const int sw = plane ? P.sub_w_log2 : 0, sh = plane ? P.sub_h_log2 : 0;
-    const int pw = plane ? P.cw : P.w, ph = plane ? P.ch : P.h;
-    const int ctbw = (1 << P.ctb_log2) >> sw, ctbh = (1 << P.ctb_log2) >> sh;
-    const int n = 1 << tb.log2n, n2 = 2 * n;
-    const int gv = 4 >> sh, gh = 4 >> sw, ngl = n2 / gv, nga = n2 / gh;
-    const int x = tb.x, y = tb.y, x0 = x & (ctbw - 1), y0 = y & (ctbh - 1);
-    const bool need_info = (P.flags & (HPVCD_PF_TILES | HPVCD_PF_CONSTRAINED_INTRA)) || P.num_slices > 1;
-    const uint32_t* ord = cell_order + P.cell_begin;
-    const uint32_t* inf = cell_info + P.cell_begin;
-    const uint32_t cur = need_info ? inf[((y << sh) >> 2) * P.gw + ((x << sw) >> 2)] : 0u;
-    auto avail = [&](int px, int py) -> bool {
-        if (px < 0 || py < 0 || px >= pw || py >= ph) return false;
-        const int lx = px << sw, ly = py << sh;
-        if (lx >= P.w || ly >= P.h) return false;
-        const int c = (ly >> 2) * P.gw + (lx >> 2);
-        if (ord[c] >= tb.order) return false;
-        if (need_info) {
-            const uint32_t i = inf[c];
-            if ((P.flags & HPVCD_PF_TILES) && HPVCD_CELL_TILE(i) != HPVCD_CELL_TILE(cur)) return false;
-            if (P.num_slices > 1 && HPVCD_CELL_SLICE(i) != HPVCD_CELL_SLICE(cur)) return false;
-            if ((P.flags & HPVCD_PF_CONSTRAINED_INTRA) && !(i & HPVCD_CELL_INTRA)) return false;
-        }
-        return true;
-    };
-    bool av = false;
-    if (lane < ngl) av = avail(x - 1, y + n2 - (lane + 1) * gv);
-    else if (lane == ngl) av = avail(x - 1, y - 1);
-    else if (lane < ngl + 1 + nga) av = avail(x + (lane - ngl - 1) * gh, y - 1);
-    const uint64_t amask = __ballot(av);
+// The unit's window of 4x4 luma cells (one cell left/above, two CTB widths
+// right/below) is staged in LDS first, so availability reads no global memory.
+constexpr int kWin = 2 * kMaxCtb / 4 + 1;
 
-    // substituted reference map and the neighbour samples it reads
-    const uint32_t ro = unit_refoff[g];
-    const int lgv = 2 - sh, lgh = 2 - sw;
-    uint16_t* M = mref[wv];
-    uint16_t* rdst = rmap_g + ent_rmap[g];
-    uint32_t top_need = 0, left_need = 0;
-    for (int k = lane; k <= 2 * n2; k += 64) {
-        const int addr = ref_tile_addr(k, amask, n, x0, y0, lgv, lgh);
-        if (addr < kTopW) {
-            top_need = max(top_need, (uint32_t)addr + 1u);
-        } else if (addr < kNeutral) {
-            const int r = (addr - kTopW) / kRowW;
-            if (addr - kTopW - r * kRowW == 0) left_need = max(left_need, (uint32_t)r + 1u);
-        }
-        M[k] = (uint16_t)addr;
-        rdst[k] = (uint16_t)addr;
+struct PrepLds {
+    uint32_t ord[kWin * kWin];
+    uint32_t inf[kWin * kWin];
+    hpvcd_tb tb[kMaxUnitTbs];
+    uint32_t ro[kMaxUnitTbs];
+    uint16_t rmap[kMaxUnitRefs];
+};
+
+constexpr int kPrepThreads = 256;
+
+__global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict__ pics, const DevUnit* __restrict__ units,
+                                                       const hpvcd_tb* __restrict__ tbs,
+                                                       const uint32_t* __restrict__ unit_tbs,
+                                                       const uint32_t* __restrict__ unit_refoff,
+                                                       const uint32_t* __restrict__ cell_order,
+                                                       const uint32_t* __restrict__ cell_info, uint4* __restrict__ desc_g,
+                                                       uint16_t* __restrict__ rmap_g, uint32_t* __restrict__ smap_g) {
+    __shared__ PrepLds L;
+    __shared__ int16_t angles[33], inv_angles[15];
+    const int tid = threadIdx.x, wave = uni(tid >> 6), lane = tid & 63;
+    constexpr int kWaves = kPrepThreads / 64;
+    const DevUnit U = units[blockIdx.x];
+    const DevPic* Pp = pics + uni(U.pic);
+    // picture fields the loops use, held in registers
+    const int plane = uni((int)U.plane);
+    const int ctb_log2 = Pp->ctb_log2, chroma_idc = Pp->chroma_idc, num_slices = Pp->num_slices;
+    const int sw = plane ? Pp->sub_w_log2 : 0, sh = plane ? Pp->sub_h_log2 : 0;
+    const int lw = Pp->w, lh = Pp->h, gw = Pp->gw, gh_cells = Pp->gh;
+    const int pw = plane ? Pp->cw : lw, ph = plane ? Pp->ch : lh;
+    const uint32_t pflags = Pp->flags, cell_begin = Pp->cell_begin;
+    const int ctb = 1 << ctb_log2, ctbw = ctb >> sw, ctbh = ctb >> sh;
+    const int ux0 = U.cx * ctbw, uy0 = U.cy * ctbh;
+    const int gx0 = ((int)U.cx << (ctb_log2 - 2)) - 1, gy0 = ((int)U.cy << (ctb_log2 - 2)) - 1;
+    const int ntb = U.ntb;
+    const uint32_t rbase = U.rbase, sbase = U.sbase, tb_list = U.tb_list;
+    const bool need_info = (pflags & (HPVCD_PF_TILES | HPVCD_PF_CONSTRAINED_INTRA)) || num_slices > 1;
+    const uint32_t* ord_g = cell_order + cell_begin;
+    const uint32_t* inf_g = cell_info + cell_begin;
+    HPVCD_PREP_T(0)
+    if (tid < 33) angles[tid] = kIntraAngle[tid];
+    if (tid < 15) inv_angles[tid] = kInvAngle[tid];
+    for (int c = tid; c < kWin * kWin; c += kPrepThreads) {
+        const int gx = gx0 + c % kWin, gy = gy0 + c / kWin;
+        const bool in = gx >= 0 && gy >= 0 && gx < gw && gy < gh_cells;
+        L.ord[c] = in ? ord_g[gy * gw + gx] : 0xffffffffu;
+        L.inf[c] = (in && need_info) ? inf_g[gy * gw + gx] : 0u;
     }
-    top_need = (uint32_t)__builtin_amdgcn_readlane((int)wave_prefix_max(top_need), 63);
-    left_need = (uint32_t)__builtin_amdgcn_readlane((int)wave_prefix_max(left_need), 63);
-    wave_sync();
-
-    const TbShape s = tb_shape(tb.log2n, tb.mode, plane, P.flags, P.chroma_idc);
-    const uint16_t* Ms = s.filt ? nullptr : M;
-    if (lane == 0) {
+    for (int e = tid; e < ntb; e += kPrepThreads) {
+        L.tb[e] = tbs[unit_tbs[tb_list + e]];
+        L.ro[e] = unit_refoff[tb_list + e];
+    }
+    __syncthreads();
+    HPVCD_PREP_T(1)
+    const int lgv = 2 - sh, lgh = 2 - sw;
+    const int gv = 4 >> sh, gh = 4 >> sw;
+    const bool strong_on = plane == 0 && (pflags & HPVCD_PF_STRONG_INTRA_SMOOTHING);
+    // one wave per TB: availability mask (one lane per 4-sample group),
+    // substituted reference map, descriptor, per-sample reference words
+    for (int e = wave; e < ntb; e += kWaves) {
+        const hpvcd_tb tb = L.tb[e];
+        const int log2n = uni((int)tb.log2n), mode = uni((int)tb.mode), n = 1 << log2n, n2 = 2 * n;
+        const int x = uni((int)tb.x), y = uni((int)tb.y), x0 = x - ux0, y0 = y - uy0;
+        const uint32_t order = uni(tb.order), ro = uni(L.ro[e]);
+        const int roff = ro & 0x1fff, soff = ((ro >> 13) & 0xff) << 4;
+        const int ngl = n2 / gv, nga = n2 / gh;
+        auto cell = [&](int px, int py) -> int {
+            return (((py << sh) >> 2) - gy0) * kWin + (((px << sw) >> 2) - gx0);
+        };
+        const uint32_t cur = L.inf[cell(x, y)];
+        int px = x - 1, py = y - 1;
+        if (lane < ngl) py = y + n2 - (lane + 1) * gv;
+        else if (lane > ngl) { px = x + (lane - ngl - 1) * gh; }
+        bool av = lane < ngl + 1 + nga && px >= 0 && py >= 0 && px < pw && py < ph && (px << sw) < lw && (py << sh) < lh;
+        if (av) {
+            const int c = cell(px, py);
+            av = L.ord[c] < order;
+            if (av && need_info) {
+                const uint32_t i = L.inf[c];
+                if ((pflags & HPVCD_PF_TILES) && HPVCD_CELL_TILE(i) != HPVCD_CELL_TILE(cur)) av = false;
+                if (num_slices > 1 && HPVCD_CELL_SLICE(i) != HPVCD_CELL_SLICE(cur)) av = false;
+                if ((pflags & HPVCD_PF_CONSTRAINED_INTRA) && !(i & HPVCD_CELL_INTRA)) av = false;
+            }
+        }
+        const uint64_t m = __ballot(av);
+        // substituted reference map
+        uint16_t* M = L.rmap + roff;
+        for (int k = lane; k <= 2 * n2; k += 64) {
+            const uint16_t addr = (uint16_t)ref_tile_addr(k, m, n, x0, y0, lgv, lgh);
+            M[k] = addr;
+            rmap_g[rbase + roff + k] = addr;
+        }
+        wave_sync();
+        // border samples read (substitution only ever reads available
+        // samples): the row above for a TB on the unit's top edge, the left
+        // column for one on its left edge
+        uint32_t top_need = 0, left_need = 0;
+        const uint64_t left_bits = m & ((1ull << ngl) - 1ull);
+        const uint64_t above_bits = (m >> (ngl + 1)) & ((1ull << nga) - 1ull);
+        const bool corner = (m >> ngl) & 1ull;
+        if (y0 == 0) {
+            if (above_bits) top_need = (uint32_t)(x0 + ((64 - __builtin_clzll(above_bits)) << lgh) + 1);
+            else if (corner) top_need = (uint32_t)(x0 + 1);
+        }
+        if (x0 == 0) {
+            if (left_bits) left_need = (uint32_t)(y0 + n2 - ((int)__builtin_ctzll(left_bits) << lgv));
+            if (corner && y0 > 0) left_need = max(left_need, (uint32_t)y0);
+        }
+        const TbShape s = tb_shape(log2n, mode, plane, pflags, chroma_idc, angles, inv_angles);
+        const uint16_t* Ms = s.filt ? nullptr : M;
         auto A = [&](int s_) -> uint32_t { return Ms ? (uint32_t)Ms[s_] : (uint32_t)s_; };
         uint32_t u0 = 0, u1 = 0;
         if (s.cls == kClsHv) {
-            u0 = A(s.mode == 26 ? n2 + 1 : n2 - 1);
+            u0 = A(mode == 26 ? n2 + 1 : n2 - 1);
             u1 = A(n2);
         } else if (s.cls == kClsPlanar) {
             u0 = A(n2 + 1 + n);
             u1 = A(n2 - 1 - n);
         }
-        const bool strong_on = plane == 0 && (P.flags & HPVCD_PF_STRONG_INTRA_SMOOTHING);
-        uint4 d;
-        d.x = (uint32_t)x0 | ((uint32_t)y0 << 6) | ((uint32_t)(tb.log2n - 2) << 12) | ((uint32_t)s.cls << 14) |
-              ((tb.flags & TBX_HAS_RES) ? 1u << 16 : 0u) | (s.filt ? 1u << 17 : 0u) |
-              ((s.filt && strong_on && n == 32) ? 1u << 18 : 0u) | (y0 + n == ctbh ? 1u << 19 : 0u) |
-              (x0 + n == ctbw ? 1u << 20 : 0u) | ((n <= 8 && !s.filt && s.cls != kClsDc) ? 1u << 21 : 0u);
-        d.y = (ro & 0x1fffu) | (((ro >> 13) & 0xffu) << 17);  // sample word offset = 16 * field
-        d.z = u0 | (u1 << 13);
-        d.w = top_need | (left_need << 8);
-        desc_g[g] = d;
+        if (lane == 0) {
+            uint4 d;
+            d.x = (uint32_t)x0 | ((uint32_t)y0 << 6) | ((uint32_t)(log2n - 2) << 12) | ((uint32_t)s.cls << 14) |
+                  ((tb.flags & TBX_HAS_RES) ? 1u << 16 : 0u) | (s.filt ? 1u << 17 : 0u) |
+                  ((s.filt && strong_on && n == 32) ? 1u << 18 : 0u) | (y0 + n == ctbh ? 1u << 19 : 0u) |
+                  (x0 + n == ctbw ? 1u << 20 : 0u) | ((n <= 8 && !s.filt && s.cls != kClsDc) ? 1u << 21 : 0u);
+            d.y = (ro & 0x1fffu) | ((uint32_t)soff << 13);
+            d.z = u0 | (u1 << 13);
+            d.w = top_need | (left_need << 8);
+            desc_g[tb_list + e] = d;
+        }
+        uint32_t* dst = smap_g + sbase + soff;
+        for (int p = lane; p < n * n; p += 64) dst[p] = sample_word(s, p & (n - 1), p >> log2n, Ms);
     }
-    uint32_t* sdst = smap_g + ent_smap[g];
-    for (int p = lane; p < n * n; p += 64) sdst[p] = sample_word(s, p & (n - 1), p >> tb.log2n, Ms);
+    __syncthreads();
+    HPVCD_PREP_T(5)
 }
 
 // Per-TB descriptor, decoded once per TB with scalar bit-field extracts.

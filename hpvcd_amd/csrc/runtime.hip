@@ -34,6 +34,15 @@ hpvcd_status set_error(hpvcd_status st, const char* msg) {
         }                                                                              \
     } while (0)
 
+// Launch-time HIP calls inside launch_class (stream forks and joins) record
+// their first failure here; the entry points check it after launching.
+thread_local hipError_t t_launch_err = hipSuccess;
+#define HIP_TRY_V(expr)                                                                \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess && t_launch_err == hipSuccess) t_launch_err = e_;        \
+    } while (0)
+
 // ---------------------------------------------------------------------------
 // validation: every index a kernel dereferences is checked here
 // ---------------------------------------------------------------------------
@@ -124,6 +133,8 @@ using namespace hpvcd;
 struct hpvcd_gpu {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;                   // per-TB preparation, concurrent with the residual stage
+    hipEvent_t fork = nullptr, join = nullptr;
     int num_cus = 0;
     int occ_intra[4] = {0, 0, 0, 0};   // blocks per CU for k_intra<T,R> (T16*2 + R32)
 };
@@ -156,7 +167,6 @@ struct ClassJob {
     std::vector<uint32_t> tickets, plain_tickets, unit_tbs;
     std::vector<uint16_t> level_off;
     std::vector<uint32_t> unit_refoff;             // per unit TB entry: map offset | sample offset / 16 << 13 | level << 21
-    std::vector<uint32_t> ent_smap, ent_rmap;      // per unit TB entry: first sample word / reference map entry
     size_t smap_words = 0, rmap_words = 0;
     std::vector<uint32_t> res_list[2][4];            // [pass][log2-2], batched order
     std::vector<uint32_t> pic_units_begin;            // plain_tickets segment per picture (npics+1)
@@ -172,7 +182,6 @@ struct ClassJob {
     uint32_t *d_tickets = nullptr, *d_plain_tickets = nullptr, *d_unit_tbs = nullptr;
     uint16_t* d_level_off = nullptr;
     uint32_t* d_unit_refoff = nullptr;
-    uint32_t *d_ent_smap = nullptr, *d_ent_rmap = nullptr;
     uint32_t* d_smap = nullptr;   // per-sample reference words
     uint16_t* d_rmap = nullptr;   // substituted reference maps
     uint4* d_desc = nullptr;      // wavefront TB descriptors (per unit TB entry)
@@ -374,8 +383,6 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                             return set_error(HPVCD_E_INVALID, "too many reference samples in one CTB");
                         C.unit_tbs.push_back(e.second);
                         C.unit_refoff.push_back(roff | ((soff >> 4) << 13) | ((uint32_t)e.first << 21));
-                        C.ent_smap.push_back(U.sbase + soff);
-                        C.ent_rmap.push_back(U.rbase + roff);
                         roff += 4 * n + 1;
                         soff += n * n;
                     }
@@ -476,6 +483,17 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
     for (int step = 0; step < nsteps; step++) {
         const bool plain = mode == HPVCD_MODE_PLAIN;
         const uint32_t p0 = plain ? (uint32_t)step : 0, pn = plain ? 1 : np;
+        const bool prep = step == 0 && !C.tbs.empty();
+        if (prep) {
+            // reference availability, reference maps, per-sample reference
+            // words and descriptors of every TB (records only) on the side
+            // stream, concurrently with the residual stage
+            HIP_TRY_V(hipEventRecord(g->fork, s));
+            HIP_TRY_V(hipStreamWaitEvent(g->side, g->fork, 0));
+            k_prep<<<nunits, kPrepThreads, 0, g->side>>>(C.d_pics, C.d_units, C.d_tbs, C.d_unit_tbs, C.d_unit_refoff,
+                                                         C.d_order, C.d_cells, C.d_desc, C.d_rmap, C.d_smap);
+            HIP_TRY_V(hipEventRecord(g->join, g->side));
+        }
         mark(HPVCD_KERNEL_RESIDUAL);
         for (int pass = 0; pass < 2; pass++)
             for (int sz = 0; sz < 4; sz++) {
@@ -483,15 +501,8 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
                 uint32_t b1 = plain ? C.pic_res_begin[pass][sz][p0 + 1] : (uint32_t)C.res_list[pass][sz].size();
                 launch_residual<R>(s, C, d_scaling, pass, sz + 2, C.d_res_list[pass][sz] + b0, b1 - b0);
             }
+        if (prep) HIP_TRY_V(hipStreamWaitEvent(s, g->join, 0));
         mark(HPVCD_KERNEL_INTRA);
-        if (step == 0 && !C.tbs.empty()) {
-            // reference availability and per-sample reference words of every
-            // TB (records only, fully parallel)
-            const uint32_t ne = (uint32_t)C.unit_tbs.size();
-            k_prep<<<cdiv(ne, 4), 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, C.d_order, C.d_cells, ne, C.d_unit_tbs,
-                                                C.d_unit_refoff, C.d_ent_smap, C.d_ent_rmap, C.d_desc, C.d_rmap,
-                                                C.d_smap);
-        }
         {
             const uint32_t* tick = plain ? C.d_plain_tickets + C.pic_units_begin[p0] : C.d_tickets;
             uint32_t nu = plain ? C.pic_units_begin[p0 + 1] - C.pic_units_begin[p0] : nunits;
@@ -576,6 +587,9 @@ int hpvcd_debug_intra_profile(void* device_buffer) {
 int hpvcd_debug_intra_timeline(void* device_buffer) {
     return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_intra_tl), &device_buffer, sizeof(device_buffer));
 }
+int hpvcd_debug_prep_profile(void* device_buffer) {
+    return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_prep_prof), &device_buffer, sizeof(device_buffer));
+}
 int hpvcd_debug_unit_info(hpvcd_job* j, uint32_t u, uint32_t* out) {
     if (j->classes.empty() || u >= j->classes[0].units.size()) return -1;
     const auto& U = j->classes[0].units[u];
@@ -652,6 +666,9 @@ hpvcd_status hpvcd_gpu_open(int device, hpvcd_gpu** out) {
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     g->num_cus = prop.multiProcessorCount;
     HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&g->join, hipEventDisableTiming));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[0], k_intra<uint8_t, int16_t>, kIntraThreads, 0));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[1], k_intra<uint8_t, int32_t>, kIntraThreads, 0));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[2], k_intra<uint16_t, int16_t>, kIntraThreads, 0));
@@ -664,6 +681,9 @@ void hpvcd_gpu_close(hpvcd_gpu* g) {
     if (!g) return;
     hipSetDevice(g->device);
     if (g->stream) hipStreamDestroy(g->stream);
+    if (g->side) hipStreamDestroy(g->side);
+    if (g->fork) hipEventDestroy(g->fork);
+    if (g->join) hipEventDestroy(g->join);
     delete g;
 }
 
@@ -736,7 +756,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
                  bytes_of<hpvcd_sao>(C.sao.size()) + bytes_of<DevUnit>(C.units.size()) +
                  bytes_of<uint32_t>(C.tickets.size()) * 2 + bytes_of<uint32_t>(C.unit_tbs.size() + 1) +
                  bytes_of<uint16_t>(C.level_off.size() + 1) + bytes_of<uint32_t>(C.sync_words) +
-                 bytes_of<uint32_t>(C.unit_refoff.size() + 1) + 2 * bytes_of<uint32_t>(C.ent_smap.size() + 1) +
+                 bytes_of<uint32_t>(C.unit_refoff.size() + 1) +
                  bytes_of<uint32_t>(C.smap_words + 1) + bytes_of<uint16_t>(C.rmap_words + 1) +
                  bytes_of<uint4>(C.unit_tbs.size() + 1) +
                  bytes_of<uint32_t>(C.units.size() * kBorderWords + 1) + bytes_of<uint64_t>(C.tbs.size() + 1) +
@@ -769,8 +789,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
             (st = upload(C.d_sao, j->mem, C.sao)) || (st = upload(C.d_units, j->mem, C.units)) ||
             (st = upload(C.d_tickets, j->mem, C.tickets)) || (st = upload(C.d_plain_tickets, j->mem, C.plain_tickets)) ||
             (st = upload(C.d_unit_tbs, j->mem, C.unit_tbs)) || (st = upload(C.d_level_off, j->mem, C.level_off)) ||
-            (st = upload(C.d_unit_refoff, j->mem, C.unit_refoff)) || (st = upload(C.d_ent_smap, j->mem, C.ent_smap)) ||
-            (st = upload(C.d_ent_rmap, j->mem, C.ent_rmap))) {
+            (st = upload(C.d_unit_refoff, j->mem, C.unit_refoff))) {
             hpvcd_job_free(j);
             return st;
         }
@@ -803,6 +822,7 @@ hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop) {
     if (stop < HPVCD_STAGE_RECON || stop > HPVCD_STAGE_RGB) return set_error(HPVCD_E_INVALID, "stop stage");
     HIP_TRY(hipSetDevice(j->g->device));
     j->last_stop = stop;
+    t_launch_err = hipSuccess;
     if (mode == HPVCD_MODE_GRAPH) {
         for (auto& C : j->classes) {
             if (!C.graph || C.graph_stop != stop) {
@@ -817,6 +837,10 @@ hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop) {
                     else launch_class<uint8_t, int16_t>(j->g, C, j->d_scaling, j->d_out, HPVCD_MODE_BATCHED, stop, nullptr);
                 }
                 HIP_TRY(hipStreamEndCapture(j->g->stream, &gr));
+                if (t_launch_err != hipSuccess) {
+                    hipGraphDestroy(gr);
+                    return set_error(HPVCD_E_DEVICE, hipGetErrorString(t_launch_err));
+                }
                 hipError_t e = hipGraphInstantiate(&C.graph, gr, nullptr, nullptr, 0);
                 hipGraphDestroy(gr);
                 if (e != hipSuccess) return set_error(HPVCD_E_DEVICE, "hipGraphInstantiate failed");
@@ -826,6 +850,7 @@ hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop) {
         }
     } else if (mode == HPVCD_MODE_BATCHED || mode == HPVCD_MODE_PLAIN) {
         launch_job(j, mode, stop, nullptr);
+        HIP_TRY(t_launch_err);
         HIP_TRY(hipGetLastError());
     } else {
         return set_error(HPVCD_E_INVALID, "mode");
@@ -900,7 +925,9 @@ hpvcd_status hpvcd_job_time(hpvcd_job* j, int mode, int warmup, int iters, doubl
         for (int i = 0; i < iters; i++) {
             std::vector<std::pair<int, hipEvent_t>> marks;
             j->last_stop = HPVCD_STAGE_RGB;
+            t_launch_err = hipSuccess;
             launch_job(j, kmode, HPVCD_STAGE_RGB, &marks);
+            HIP_TRY(t_launch_err);
             HIP_TRY(hipStreamSynchronize(j->g->stream));
             for (size_t m = 0; m + 1 < marks.size(); m++) {
                 if (marks[m].first < 0) continue;

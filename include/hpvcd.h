@@ -129,7 +129,8 @@ hpvcd_status hpvcd_job_download_planes(hpvcd_job* j, uint32_t picture, uint16_t*
 #define HPVCD_KERNEL_DEBLOCK  2
 #define HPVCD_KERNEL_SAO      3
 #define HPVCD_KERNEL_RGB      4
-#define HPVCD_KERNEL_COUNT    5
+#define HPVCD_KERNEL_PREP     5   /* static per-TB preparation for the intra wavefront */
+#define HPVCD_KERNEL_COUNT    6
 hpvcd_status hpvcd_job_time(hpvcd_job* j, int mode, int warmup, int iters, double* ms_per_run,
                             double* kernel_ms);
 
