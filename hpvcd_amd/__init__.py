@@ -74,6 +74,41 @@ class RecordSet:
         return cls(h.value)
 
     @classmethod
+    def from_arrays(cls, pictures: Sequence[N.Picture], tbs: np.ndarray, coeffs: np.ndarray, cells: np.ndarray,
+                    outputs: Sequence[N.Output], sao: Optional[Sequence[N.Sao]] = None,
+                    scaling: Optional[np.ndarray] = None) -> "RecordSet":
+        """A batch over caller-built record arrays (what a syntax parser emits):
+        ``tbs`` in the hpvcd_tb layout, ``coeffs`` and ``cells`` as uint32 words,
+        SAO parameters per CTB (all off when omitted).  The arrays are copied
+        and kept alive by the returned set."""
+        self = cls.__new__(cls)
+        self._h = C.c_void_p()
+        tbs = np.ascontiguousarray(np.asarray(tbs, dtype=_TB_DTYPE))
+        coeffs = np.ascontiguousarray(np.asarray(coeffs, dtype=np.uint32))
+        cells = np.ascontiguousarray(np.asarray(cells, dtype=np.uint32))
+        pics = (N.Picture * len(pictures))(*pictures)
+        outs = (N.Output * len(outputs))(*outputs)
+        nctb = 0
+        for p in pictures:
+            ctb = 1 << p.log2_ctb
+            nctb = max(nctb, p.ctb_begin + ((p.width + ctb - 1) // ctb) * ((p.height + ctb - 1) // ctb))
+        sao_arr = (N.Sao * max(1, nctb))(*(sao or []))
+        b = N.Batch()
+        b.num_pictures, b.pictures = len(pictures), C.cast(pics, C.POINTER(N.Picture))
+        b.num_tbs, b.tbs = len(tbs), tbs.ctypes.data_as(C.POINTER(N.TB))
+        b.num_coeffs, b.coeffs = len(coeffs), coeffs.ctypes.data_as(C.POINTER(C.c_uint32))
+        b.num_ctbs, b.sao = nctb, C.cast(sao_arr, C.POINTER(N.Sao))
+        b.num_cells, b.cell_info = len(cells), cells.ctypes.data_as(C.POINTER(C.c_uint32))
+        if scaling is not None:
+            scaling = np.ascontiguousarray(np.asarray(scaling, dtype=np.uint8))
+            b.num_scaling = len(scaling) // N.SCALING_SET_BYTES
+            b.scaling = scaling.ctypes.data_as(C.POINTER(C.c_uint8))
+        b.num_outputs, b.outputs = len(outputs), C.cast(outs, C.POINTER(N.Output))
+        self.batch = b
+        self._keep = (tbs, coeffs, cells, pics, outs, sao_arr, scaling)
+        return self
+
+    @classmethod
     def concat(cls, sets: Sequence["RecordSet"]) -> "RecordSet":
         arr = (C.c_void_p * len(sets))(*[s._h.value for s in sets])
         h = C.c_void_p()

@@ -35,6 +35,8 @@ TB_TQB = 1 << 2
 TB_INTER = 1 << 3
 TB_RDPCM_H = 1 << 4
 TB_RDPCM_V = 1 << 5
+TB_ACT = 1 << 6
+TB_PCM = 1 << 7
 
 CELL_TQB = 1 << 4
 CELL_PCM = 1 << 5
@@ -136,9 +138,11 @@ class GenConfig(C.Structure):
         ("ts_rotation", C.c_uint8),
         ("matrix", C.c_uint8), ("full_range", C.c_uint8), ("out_kind", C.c_uint8), ("num_outputs", C.c_uint8),
         ("log2_res_scale_max", C.c_uint8),
-        ("_pad", C.c_uint8 * 3),
+        ("act", C.c_uint8), ("pcm", C.c_uint8), ("pcm_bit_depth", C.c_uint8), ("pcm_lf_disabled", C.c_uint8),
+        ("_pad", C.c_uint8 * 2),
         ("p_split_cu", C.c_float), ("p_nxn", C.c_float), ("p_split_tu", C.c_float), ("p_cbf", C.c_float),
         ("p_sig", C.c_float), ("p_level_geom", C.c_float), ("p_ts", C.c_float), ("p_tqb", C.c_float),
+        ("p_act", C.c_float), ("p_pcm", C.c_float),
         ("config_id", C.c_uint32),
         ("base_seed", C.c_uint64),
     ]

@@ -161,7 +161,16 @@ struct PicGen {
         return T[qpi - 30];
     }
 
-    hpvcd_tb make_tb(int x, int y, int log2n, int comp, uint8_t mode, bool cbf) {
+    // Qp' of a TB.  Under ACT the component offsets -5/-5/-3 apply to QpY
+    // before the chroma mapping and no PPS chroma offsets (src/decode.rs:3788).
+    int tb_qp(int comp, bool act) {
+        if (!act) return comp == 0 ? qp_prime(qp_y, bd) : qp_prime(qpc_table(qp_y, cfg.chroma_idc, bd), bd);
+        int qpi = qp_y + (comp == 2 ? -3 : -5);
+        if (comp == 0) return qp_prime(std::clamp(qpi, -6 * (bd - 8), 51), bd);
+        return qp_prime(qpc_table(qpi, cfg.chroma_idc, bd), bd);
+    }
+
+    hpvcd_tb make_tb(int x, int y, int log2n, int comp, uint8_t mode, bool cbf, bool act = false) {
         hpvcd_tb tb;
         std::memset(&tb, 0, sizeof(tb));
         tb.x = (uint16_t)x;
@@ -169,9 +178,8 @@ struct PicGen {
         tb.log2n = (uint8_t)log2n;
         tb.comp = (uint8_t)comp;
         tb.mode = mode;
-        int bdc = bd;
-        int qp = comp == 0 ? qp_prime(qp_y, bd) : qp_prime(qpc_table(qp_y, cfg.chroma_idc, bdc), bdc);
-        tb.qp = (uint8_t)qp;
+        tb.qp = (uint8_t)tb_qp(comp, act);
+        if (act) tb.flags |= HPVCD_TB_ACT;
         if (cu_tqb) tb.flags |= HPVCD_TB_TQB;
         if (cbf) {
             tb.flags |= HPVCD_TB_CBF;
@@ -211,11 +219,32 @@ struct PicGen {
         }
     }
 
+    // A co-located luma/Cb/Cr TB triple whose residuals go through the
+    // inverse colour transform (src/decode.rs:3619 transform_unit_act).
+    void act_unit(int x0, int y0, int log2, uint8_t luma_mode, bool cbf_luma, bool cbf_cb, bool cbf_cr) {
+        bool cross_comp = cfg.cross_component && cbf_luma;
+        tbs.push_back(make_tb(x0, y0, log2, 0, luma_mode, cbf_luma, true));
+        uint8_t mode = chroma_mode_at(x0, y0);
+        for (int comp = 1; comp <= 2; comp++) {
+            int scale = cross_comp ? draw_res_scale() : 0;
+            hpvcd_tb tb = make_tb(x0, y0, log2, comp, mode, comp == 1 ? cbf_cb : cbf_cr, true);
+            tb.res_scale = (int8_t)scale;
+            tbs.push_back(tb);
+        }
+    }
+
     void transform_unit(int x0, int y0, int xbase, int ybase, int log2, int blk, bool cbf_cb, bool cbf_cr) {
         bool cbf_luma = rng.bern(cfg.p_cbf);
         int size = 1 << log2;
         mark_block_edges(x0, y0, size, 2);
         uint8_t luma_mode = mode_y[(y0 / 4) * gw + x0 / 4];
+        // tu_residual_act_flag is present for a TU with residual when every
+        // chroma mode of the CU is the derived mode (src/decode.rs:4229)
+        if (cfg.act && cfg.chroma_idc == 3 && (cbf_luma || cbf_cb || cbf_cr) && cu_chroma_dm[0] && cu_chroma_dm[1] &&
+            cu_chroma_dm[2] && cu_chroma_dm[3] && rng.bern(cfg.p_act)) {
+            act_unit(x0, y0, log2, luma_mode, cbf_luma, cbf_cb, cbf_cr);
+            return;
+        }
         bool cross_comp = cfg.cross_component && cbf_luma && chroma_dm_at(x0, y0);
         tbs.push_back(make_tb(x0, y0, log2, 0, luma_mode, cbf_luma));
         if (cfg.chroma_idc == 0) return;
@@ -243,11 +272,48 @@ struct PicGen {
         }
     }
 
+    // I_PCM CU (src/decode.rs:2768 decode_pcm_cu): luma then Cb then Cr,
+    // every sample a raw PcmBitDepth-bit value, scaled by the TB's shift.
+    void pcm_unit(int x0, int y0, int log2) {
+        int size = 1 << log2;
+        int shift = bd - cfg.pcm_bit_depth;
+        uint32_t vmax = 1u << cfg.pcm_bit_depth;
+        fill_cells(x0, y0, size, HPVCD_CELL_PCM);
+        set_mode(x0, y0, size, MODE_DC);
+        mark_block_edges(x0, y0, size, 2);
+        int ncomp = cfg.chroma_idc ? 3 : 1;
+        for (int comp = 0; comp < ncomp; comp++) {
+            int l2 = comp == 0 || cfg.chroma_idc == 3 ? log2 : log2 - 1;
+            int n = 1 << l2;
+            hpvcd_tb tb;
+            std::memset(&tb, 0, sizeof(tb));
+            tb.x = (uint16_t)(comp ? x0 / sub_w : x0);
+            tb.y = (uint16_t)(comp ? y0 / sub_h : y0);
+            tb.log2n = (uint8_t)l2;
+            tb.comp = (uint8_t)comp;
+            tb.mode = MODE_DC;
+            tb.qp = (uint8_t)shift;
+            tb.flags = HPVCD_TB_PCM | HPVCD_TB_CBF;
+            tb.coeff_off = (uint32_t)coeffs.size() - coeff_base;
+            for (int pos = 0; pos < n * n; pos++) coeffs.push_back(HPVCD_COEFF_PACK(pos, (int)rng.uni(vmax)));
+            tb.coeff_cnt = (uint32_t)(n * n);
+            tb.nx = (uint8_t)n;
+            tbs.push_back(tb);
+        }
+        for (int yy = y0; yy < std::min(y0 + size, h); yy += 4)
+            for (int xx = x0; xx < std::min(x0 + size, w); xx += 4) cell_qp[(yy / 4) * gw + xx / 4] = (int8_t)qp_y;
+    }
+
     void coding_unit(int x0, int y0, int log2) {
         int size = 1 << log2;
         cu_tqb = cfg.p_tqb > 0.f && rng.bern(cfg.p_tqb);
         if (cu_tqb) fill_cells(x0, y0, size, HPVCD_CELL_TQB);
         bool nxn = log2 == log2_min_cb && log2 > 2 && rng.bern(cfg.p_nxn);
+        // pcm_flag follows part_mode for 2Nx2N CUs of 8x8..32x32 (src/decode.rs:2647)
+        if (cfg.pcm && !nxn && log2 >= 3 && log2 <= 5 && rng.bern(cfg.p_pcm)) {
+            pcm_unit(x0, y0, log2);
+            return;
+        }
         int npu = nxn ? 2 : 1, pu = size / npu;
         uint8_t luma_modes[4] = {MODE_DC, MODE_DC, MODE_DC, MODE_DC};
         for (int i = 0; i < npu * npu; i++) {
@@ -390,6 +456,8 @@ extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_rec
         return set_error(HPVCD_E_BAD_DIMENSIONS, "generator: tile size must be a positive multiple of the min CU");
     if (c.chroma_idc == 2) return set_error(HPVCD_E_UNSUPPORTED_CHROMA, "generator: 4:2:2 not generated");
     if (c.bit_depth < 8 || c.bit_depth > 12) return set_error(HPVCD_E_UNSUPPORTED_BITDEPTH, "generator: bit depth");
+    if (c.pcm && (c.pcm_bit_depth < 1 || c.pcm_bit_depth > c.bit_depth))
+        return set_error(HPVCD_E_PARAMSET, "generator: PCM bit depth must be 1..bit_depth");
     if (c.log2_ctb < 4 || c.log2_ctb > 6 || c.log2_min_cb < 3 || c.log2_min_cb > c.log2_ctb || c.log2_min_tb != 2 ||
         c.log2_max_tb > 5 || c.log2_max_tb < 3)
         return set_error(HPVCD_E_PARAMSET, "generator: unsupported block size limits");
@@ -436,6 +504,7 @@ extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_rec
         if (c.sao) f |= HPVCD_PF_SAO_LUMA | (c.chroma_idc ? HPVCD_PF_SAO_CHROMA : 0u);
         if (c.full_range) f |= HPVCD_PF_FULL_RANGE;
         if (!c.deblock) f |= HPVCD_PF_DEBLOCK_DISABLED;
+        if (c.pcm && c.pcm_lf_disabled) f |= HPVCD_PF_PCM_LF_DISABLED;
         p.flags = f;
         p.matrix = c.matrix;
         p.num_slices = 1;

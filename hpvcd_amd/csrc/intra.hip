@@ -27,7 +27,8 @@ constexpr int kMaxCtb = 64;
 constexpr int kTopW = 2 * kMaxCtb + 1;            // row above the unit: x0-1 .. x0+2*ctb-1
 constexpr int kRowW = kMaxCtb + 1;                // body rows: left neighbour + the unit's samples
 constexpr int kNeutral = kTopW + kMaxCtb * kRowW; // tile slot holding 1 << (bitDepth - 1)
-constexpr int kTileWords = kNeutral + 1;
+constexpr int kZero = kNeutral + 1;               // tile slot holding 0: the I_PCM predictor
+constexpr int kTileWords = kNeutral + 2;
 constexpr int kFStride = 132;                     // filtered references of one TB (4 * 32 + 1)
 constexpr int kFBase = kTileWords;                // the filtered reference row follows the tile
 constexpr int kMaxUnitTbs = 256;
@@ -317,7 +318,16 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             if (left_bits) left_need = (uint32_t)(y0 + n2 - ((int)__builtin_ctzll(left_bits) << lgv));
             if (corner && y0 > 0) left_need = max(left_need, (uint32_t)y0);
         }
-        const TbShape s = tb_shape(log2n, mode, plane, pflags, chroma_idc, angles, inv_angles);
+        // an I_PCM block reads no neighbours: its words all name the zero
+        // slot through the angular form with fraction 0, so prediction is 0
+        // and the samples arrive as its residual
+        const bool pcm = (tb.flags & HPVCD_TB_PCM) != 0;
+        if (pcm) top_need = left_need = 0;
+        TbShape s = tb_shape(log2n, mode, plane, pflags, chroma_idc, angles, inv_angles);
+        if (pcm) {
+            s.cls = kClsAngular;
+            s.filt = false;
+        }
         const uint16_t* Ms = s.filt ? nullptr : M;
         auto A = [&](int s_) -> uint32_t { return Ms ? (uint32_t)Ms[s_] : (uint32_t)s_; };
         uint32_t u0 = 0, u1 = 0;
@@ -340,7 +350,8 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             desc_g[tb_list + e] = d;
         }
         uint32_t* dst = smap_g + sbase + soff;
-        for (int p = lane; p < n * n; p += 64) dst[p] = sample_word(s, p & (n - 1), p >> log2n, Ms);
+        for (int p = lane; p < n * n; p += 64)
+            dst[p] = pcm ? (uint32_t)kZero | ((uint32_t)kZero << 13) : sample_word(s, p & (n - 1), p >> log2n, Ms);
     }
     __syncthreads();
     HPVCD_PREP_T(5)
@@ -631,6 +642,7 @@ __global__ void __launch_bounds__(kIntraThreads)
         if (tid == 0) {
             L.bready = 0;
             L.tile[kNeutral] = (uint16_t)(1 << (bd - 1));
+            L.tile[kZero] = 0;
         }
         __syncthreads();
         HPVCD_PROF(7)

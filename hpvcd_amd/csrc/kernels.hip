@@ -118,9 +118,11 @@ __global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pic
     const bool ts = (tb.flags & HPVCD_TB_TS) != 0;
     const bool tqb = (tb.flags & HPVCD_TB_TQB) != 0;
     const bool cbf = (tb.flags & HPVCD_TB_CBF) != 0;
-    const bool i16 = bd <= 8 && !ep && (comp == 0 || tb.res_scale == 0);
+    const bool pcm = (tb.flags & HPVCD_TB_PCM) != 0;
+    // the ACT path keeps its residuals in i32 (src/decode.rs:3660)
+    const bool i16 = bd <= 8 && !ep && (comp == 0 || tb.res_scale == 0) && !(tb.flags & HPVCD_TB_ACT);
     const int max_range = max_range_of(bd, ep);
-    const bool transform = cbf && !ts && !tqb;
+    const bool transform = cbf && !ts && !tqb && !pcm;
     const bool use_dst = transform && comp == 0 && LOG2N == 2 && !inter;
     // RExt residual ops (src/decode.rs:4356 rext_post_ops)
     bool rotate = false;
@@ -164,6 +166,8 @@ __global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pic
             int32_t v;
             if (tqb) {
                 v = i16 ? clampi(level, -32768, 32767) : level;
+            } else if (pcm) {
+                v = level << tb.qp;  // pcm_sample << (BitDepth - PcmBitDepth), src/decode.rs:2788
             } else {
                 int64_t f = factor;
                 if (sc) {
@@ -271,6 +275,39 @@ __global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pic
 }
 
 // ---------------------------------------------------------------------------
+// K1b: inverse adaptive colour transform (src/act.rs:43, YCgCo-R lifting) of
+// the co-located 4:4:4 residual triples of ACT TUs, in place in the residual
+// pool after the cross-component pass.  One wave per TU; list holds the luma
+// TB of each triple.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) k_act(const DevPic* __restrict__ pics, const hpvcd_tb* __restrict__ tbs,
+                                             const uint32_t* __restrict__ tb_pic, const uint32_t* __restrict__ list,
+                                             uint32_t count, R* __restrict__ res_pool) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= count) return;
+    const uint32_t tbi = list[i];
+    const hpvcd_tb tb = tbs[tbi];
+    const DevPic& P = pics[tb_pic[tbi]];
+    const int log2n = tb.log2n, n = 1 << log2n;
+    R* r0 = res_pool + P.res_off[0];
+    R* r1 = res_pool + P.res_off[1];
+    R* r2 = res_pool + P.res_off[2];
+    for (int e = lane; e < n * n; e += 64) {
+        const int px = tb.x + (e & (n - 1)), py = tb.y + (e >> log2n);
+        if (px >= P.w || py >= P.h) continue;
+        const size_t o = (size_t)py * P.w + px;  // 4:4:4: the three planes share one geometry
+        const int32_t y = r0[o], cg = r1[o], co = r2[o];
+        const int32_t t = (int32_t)((uint32_t)y - (uint32_t)(cg >> 1));
+        const int32_t b = (int32_t)((uint32_t)t - (uint32_t)(co >> 1));
+        r0[o] = (R)wadd(cg, t);
+        r1[o] = (R)b;
+        r2[o] = (R)wadd(b, co);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // K2: intra prediction + reconstruction, persistent CTU wavefront
 //
 // Work unit = one (picture, plane, CTB).  Units are handed out by an atomic
@@ -281,8 +318,11 @@ __global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pic
 // (run tag << 16 | sample) with relaxed agent-scope atomics, and consumers
 // poll the words they need until the tag matches.  Per-word coherence is all
 // that is required, so no fences (and no L2 write-back / invalidate) are
-// needed.  Inside a unit the TBs run level by level (dependency levels
-// precomputed on the host); waves take the TBs of a level round-robin.
+// needed.  Inside a unit one wave predicts the TBs one after another in
+// dependency order while a second wave polls the neighbour words the next TBs
+// read into the LDS tile; the static part of every TB (availability,
+// substituted reference addresses, per-sample reference words) comes from
+// k_prep, which runs before the wavefront on the records alone.
 // ---------------------------------------------------------------------------
 #include "intra.hip"
 

@@ -111,11 +111,32 @@ hpvcd_status validate_batch(const hpvcd_batch& b) {
             if ((T.x & 3) || (T.y & 3)) return set_error(HPVCD_E_INVALID, "tb not 4-aligned");
             if (T.mode > 34) return set_error(HPVCD_E_INVALID, "intra mode");
             if (T.res_scale && (T.comp == 0 || P.chroma_idc != 3)) return set_error(HPVCD_E_INVALID, "res_scale");
-            if (T.flags & HPVCD_TB_CBF) {
+            const bool pcm = (T.flags & HPVCD_TB_PCM) != 0;
+            const int bd = T.comp ? P.bit_depth_chroma : P.bit_depth_luma;
+            if (pcm) {
+                if (T.flags & (HPVCD_TB_TS | HPVCD_TB_TQB | HPVCD_TB_INTER | HPVCD_TB_RDPCM_H | HPVCD_TB_RDPCM_V |
+                               HPVCD_TB_ACT) || T.res_scale)
+                    return set_error(HPVCD_E_INVALID, "PCM TB with residual-coding flags");
+                if (T.qp >= bd) return set_error(HPVCD_E_INVALID, "PCM sample shift must be below the bit depth");
+            }
+            if (T.flags & HPVCD_TB_ACT) {
+                // a luma TB opens the triple; its Cb and Cr TBs follow at once
+                if (P.chroma_idc != 3 || T.comp > 2 || t < T.comp || t - T.comp + 2 >= P.tb_count)
+                    return set_error(HPVCD_E_INVALID, "ACT TB outside a 4:4:4 luma/Cb/Cr triple");
+                const uint32_t t0 = t - T.comp;
+                for (int k = 0; k <= 2; k++) {
+                    const hpvcd_tb& U = b.tbs[P.tb_begin + t0 + k];
+                    if (!(U.flags & HPVCD_TB_ACT) || U.comp != k || U.x != T.x || U.y != T.y || U.log2n != T.log2n)
+                        return set_error(HPVCD_E_INVALID, "ACT triple not co-located luma, Cb, Cr of one size");
+                }
+            }
+            if (T.flags & (HPVCD_TB_CBF | HPVCD_TB_PCM)) {
                 if ((uint64_t)T.coeff_off + T.coeff_cnt > P.coeff_count) return set_error(HPVCD_E_INVALID, "tb coeff range");
                 for (uint32_t k = 0; k < T.coeff_cnt; k++) {
                     uint32_t w = b.coeffs[P.coeff_begin + T.coeff_off + k];
                     if (HPVCD_COEFF_POS(w) >= (uint32_t)(n * n)) return set_error(HPVCD_E_INVALID, "coefficient position");
+                    if (pcm && (HPVCD_COEFF_LEVEL(w) < 0 || HPVCD_COEFF_LEVEL(w) >= (1 << (bd - T.qp))))
+                        return set_error(HPVCD_E_INVALID, "PCM sample outside PcmBitDepth");
                 }
             }
         }
@@ -171,6 +192,7 @@ struct ClassJob {
     std::vector<uint32_t> res_list[2][4];            // [pass][log2-2], batched order
     std::vector<uint32_t> pic_units_begin;            // plain_tickets segment per picture (npics+1)
     std::vector<uint32_t> pic_res_begin[2][4];        // per picture list segments (npics+1)
+    std::vector<uint32_t> act_list, pic_act_begin;    // luma TB of each ACT triple; per picture segments
     size_t plane_elems = 0, res_elems = 0;
     uint32_t max_luma_v = 0, max_luma_h = 0, max_chroma_v = 0, max_chroma_h = 0, max_plane = 0, max_copy = 0;
     // device pointers
@@ -186,6 +208,7 @@ struct ClassJob {
     uint16_t* d_rmap = nullptr;   // substituted reference maps
     uint4* d_desc = nullptr;      // wavefront TB descriptors (per unit TB entry)
     uint32_t* d_res_list[2][4] = {};
+    uint32_t* d_act_list = nullptr;
     void *d_planes_a = nullptr, *d_planes_b = nullptr, *d_res = nullptr;
     uint32_t* d_sync = nullptr;   // [0] err, [1] border tag, [2..] ticket counters (batched + one per picture)
     uint32_t sync_words = 0;
@@ -223,6 +246,7 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
     std::vector<std::vector<uint32_t>> per_pic_res[2][4];
     for (int pass = 0; pass < 2; pass++)
         for (int s = 0; s < 4; s++) per_pic_res[pass][s].resize(np);
+    std::vector<std::vector<uint32_t>> per_pic_act(np);
     struct UnitKey { uint32_t t, pic, plane, idx; };
     std::vector<UnitKey> keys;
 
@@ -307,7 +331,9 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
             } else {
                 T.order = stamp + 1;
             }
-            if ((T.flags & HPVCD_TB_CBF) || T.res_scale != 0) T.flags |= TBX_HAS_RES;
+            if (T.flags & HPVCD_TB_PCM) T.flags |= HPVCD_TB_CBF;  // the samples travel as levels
+            // ACT mixes all three residuals: each TB of a triple gets one
+            if ((T.flags & (HPVCD_TB_CBF | HPVCD_TB_ACT)) || T.res_scale != 0) T.flags |= TBX_HAS_RES;
             C.tbs.push_back(T);
             C.tb_pic.push_back(lp);
         }
@@ -326,6 +352,7 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                 int pass = T.res_scale != 0 ? 1 : 0;
                 per_pic_res[pass][T.log2n - 2][lp].push_back(tb0 + t);
             }
+            if ((T.flags & HPVCD_TB_ACT) && T.comp == 0) per_pic_act[lp].push_back(tb0 + t);
         }
         C.pic_units_begin[lp] = unit_base;
         for (int pl = 0; pl < nplanes; pl++)
@@ -421,6 +448,13 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
             }
             C.pic_res_begin[pass][s][np] = (uint32_t)C.res_list[pass][s].size();
         }
+    C.act_list.clear();
+    C.pic_act_begin.assign(np + 1, 0);
+    for (uint32_t lp = 0; lp < np; lp++) {
+        C.pic_act_begin[lp] = (uint32_t)C.act_list.size();
+        C.act_list.insert(C.act_list.end(), per_pic_act[lp].begin(), per_pic_act[lp].end());
+    }
+    C.pic_act_begin[np] = (uint32_t)C.act_list.size();
     C.plane_elems = plane_elems;
     C.res_elems = plane_elems;
     C.sync_words = 2 + (np + 1);
@@ -501,6 +535,13 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
                 uint32_t b1 = plain ? C.pic_res_begin[pass][sz][p0 + 1] : (uint32_t)C.res_list[pass][sz].size();
                 launch_residual<R>(s, C, d_scaling, pass, sz + 2, C.d_res_list[pass][sz] + b0, b1 - b0);
             }
+        {
+            uint32_t b0 = plain ? C.pic_act_begin[p0] : 0;
+            uint32_t b1 = plain ? C.pic_act_begin[p0 + 1] : (uint32_t)C.act_list.size();
+            if (b1 > b0)
+                k_act<R><<<cdiv(b1 - b0, 4), 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, C.d_act_list + b0, b1 - b0,
+                                                          reinterpret_cast<R*>(C.d_res));
+        }
         if (prep) HIP_TRY_V(hipStreamWaitEvent(s, g->join, 0));
         mark(HPVCD_KERNEL_INTRA);
         {
@@ -763,6 +804,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
                  2 * ((C.plane_elems * sb + 255) & ~(size_t)255) + ((C.res_elems * rb + 255) & ~(size_t)255) + 8 * 256;
         for (int pass = 0; pass < 2; pass++)
             for (int s = 0; s < 4; s++) total += bytes_of<uint32_t>(C.res_list[pass][s].size() + 1);
+        total += bytes_of<uint32_t>(C.act_list.size() + 1);
     }
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total > free_b) {
@@ -796,6 +838,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
         for (int pass = 0; pass < 2; pass++)
             for (int s = 0; s < 4; s++)
                 if ((st = upload(C.d_res_list[pass][s], j->mem, C.res_list[pass][s]))) { hpvcd_job_free(j); return st; }
+        if ((st = upload(C.d_act_list, j->mem, C.act_list))) { hpvcd_job_free(j); return st; }
         C.d_sync = j->mem.take<uint32_t>(C.sync_words);
         C.d_border = j->mem.take<uint32_t>(C.units.size() * kBorderWords + 1);
         C.d_smap = j->mem.take<uint32_t>(C.smap_words + 1);

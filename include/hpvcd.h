@@ -62,8 +62,14 @@ typedef struct hpvcd_gen_config {
     uint8_t  strong_intra_smoothing, sao, deblock, ts_rotation;
     uint8_t  matrix, full_range, out_kind, num_outputs;
     uint8_t  log2_res_scale_max;        /* CCP log2_res_scale drawn from [-max, max] */
-    uint8_t  _pad[3];
+    uint8_t  act;                       /* residual_adaptive_colour_transform_enabled_flag (4:4:4) */
+    uint8_t  pcm;                       /* pcm_enabled_flag: 8x8..32x32 2Nx2N intra CUs may be I_PCM */
+    uint8_t  pcm_bit_depth;             /* PcmBitDepth (luma and chroma), 1..bit_depth */
+    uint8_t  pcm_lf_disabled;           /* pcm_loop_filter_disabled_flag */
+    uint8_t  _pad[2];
     float    p_split_cu, p_nxn, p_split_tu, p_cbf, p_sig, p_level_geom, p_ts, p_tqb;
+    float    p_act;                     /* tu_residual_act_flag where the syntax is present */
+    float    p_pcm;                     /* pcm_flag where the syntax is present */
     uint32_t config_id;
     uint64_t base_seed;
 } hpvcd_gen_config;

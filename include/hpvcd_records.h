@@ -47,6 +47,17 @@ extern "C" {
 #define HPVCD_TB_INTER        (1u << 3)   /* CU is inter: no intra prediction, MC result is the predictor */
 #define HPVCD_TB_RDPCM_H      (1u << 4)   /* explicit RDPCM, horizontal (inter, parsed) */
 #define HPVCD_TB_RDPCM_V      (1u << 5)   /* explicit RDPCM, vertical (inter, parsed) */
+/* tu_residual_act_flag (4:4:4 only, src/decode.rs:3619 transform_unit_act):
+ * set on the luma TB and on the Cb and Cr TBs that follow it directly, all
+ * three co-located and of one size.  Their residuals are computed in 32-bit
+ * arithmetic, Cb/Cr get cross-component prediction from the luma residual,
+ * then the inverse YCgCo-R (src/act.rs:43) mixes the triple before each is
+ * added to its prediction.  qp carries the ACT-adjusted Qp'. */
+#define HPVCD_TB_ACT          (1u << 6)
+/* I_PCM block (src/decode.rs:2768 decode_pcm_cu): no prediction; the
+ * coefficient words carry the raw pcm_sample values (position, value) and qp
+ * carries the shift BitDepth - PcmBitDepth, so each sample is value << qp. */
+#define HPVCD_TB_PCM          (1u << 7)
 
 /* ---- per-4x4 luma cell info word (cell_info[]) --------------------------
  * bits 0-1  bS of the vertical edge at the cell's left side (0 when the

@@ -462,9 +462,10 @@ int oracle_weighted_bi(int a, int b, int w0, int w1, int o0, int o1, int log2_de
 
 void oracle_act_inverse(int32_t* y, int32_t* cg, int32_t* co) {
     /* src/act.rs:43 inverse_sample */
-    int32_t t = *y - (*cg >> 1);
-    int32_t g = *cg + t;
-    int32_t b = t - (*co >> 1);
-    int32_t r = b + *co;
+    /* two's-complement wrap-around like the reference's release build */
+    int32_t t = (int32_t)((uint32_t)*y - (uint32_t)(*cg >> 1));
+    int32_t g = (int32_t)((uint32_t)*cg + (uint32_t)t);
+    int32_t b = (int32_t)((uint32_t)t - (uint32_t)(*co >> 1));
+    int32_t r = (int32_t)((uint32_t)b + (uint32_t)*co);
     *y = g; *cg = b; *co = r;
 }

@@ -151,9 +151,15 @@ static int tb_residual(pic_ctx* c, const hpvcd_tb* t, int32_t* res) {
             rdpcm = t->mode == 26;
         }
     }
+    if (t->flags & HPVCD_TB_PCM) {
+        /* src/decode.rs:2788 pcm_sample << (BitDepth - PcmBitDepth) */
+        for (int i = 0; i < n * n; i++) res[i] = (int32_t)((uint32_t)levels[i] << t->qp);
+        return 0;
+    }
     /* i16 storage: luma at bd<=8 without EP (src/decode.rs:4755); chroma
-     * additionally needs res_scale == 0 (src/decode.rs:5315). */
-    int i16 = bd <= 8 && !c->ep && (comp == 0 || t->res_scale == 0);
+     * additionally needs res_scale == 0 (src/decode.rs:5315).  The ACT path
+     * keeps all three residuals in i32 (src/decode.rs:3660). */
+    int i16 = bd <= 8 && !c->ep && (comp == 0 || t->res_scale == 0) && !(t->flags & HPVCD_TB_ACT);
     int max_range = oracle_max_range(bd, c->ep);
     if (tqb) {
         for (int i = 0; i < n * n; i++)
@@ -191,27 +197,70 @@ static void mark_decoded(pic_ctx* c, int x0, int y0, int n) {
         for (int x = x0; x < x0 + n && x < c->w; x += 4) c->decoded[cell_at(c, x, y)] = 1;
 }
 
-static void recon_tb(pic_ctx* c, const hpvcd_tb* t) {
+/* Prediction of one TB: intra, or the MC result already in the plane for an
+ * inter TB (load_plane_block_clipped, src/decode.rs:4744), or zero for I_PCM
+ * (its samples are the whole reconstruction). */
+static void tb_prediction(pic_ctx* c, const hpvcd_tb* t, uint16_t* pred) {
     int n = 1 << t->log2n;
     int comp = t->comp;
-    int inter = (t->flags & HPVCD_TB_INTER) != 0;
-    int cbf = (t->flags & HPVCD_TB_CBF) != 0;
     uint16_t* plane = comp == 0 ? c->y : (comp == 1 ? c->cb : c->cr);
     int stride = comp ? c->cw : c->w, pw = comp ? c->cw : c->w, ph = comp ? c->ch : c->h;
-    int bd = comp ? c->bd_c : c->bd;
-    uint16_t pred[32 * 32];
-    int32_t res[32 * 32];
-
-    if (!inter) {
+    if (t->flags & HPVCD_TB_PCM) {
+        memset(pred, 0, sizeof(uint16_t) * n * n);
+    } else if (!(t->flags & HPVCD_TB_INTER)) {
         predict_block(c, comp, t->x, t->y, n, t->mode, pred);
     } else {
-        /* load_plane_block_clipped (src/decode.rs:4744): MC result in the plane */
         for (int y = 0; y < n; y++)
             for (int x = 0; x < n; x++) {
                 int px = t->x + x, py = t->y + y;
                 pred[y * n + x] = (px < pw && py < ph) ? plane[py * stride + px] : 0;
             }
     }
+}
+
+/* One ACT triple (src/decode.rs:3619 transform_unit_act): the three i32
+ * residuals (zero where cbf is 0), cross-component prediction of Cb and Cr
+ * from the luma residual, the inverse colour transform per sample, then
+ * prediction and residual add in the order luma, Cb, Cr. */
+static void recon_act(pic_ctx* c, const hpvcd_tb* t) {
+    int n = 1 << t->log2n;
+    static __thread int32_t res[3][32 * 32];
+    uint16_t pred[32 * 32];
+    for (int k = 0; k < 3; k++) {
+        if (t[k].flags & HPVCD_TB_CBF) tb_residual(c, &t[k], res[k]);
+        else memset(res[k], 0, sizeof(int32_t) * n * n);
+    }
+    for (int k = 1; k < 3; k++) {
+        if (!t[k].res_scale) continue;
+        for (int i = 0; i < n * n; i++) { /* src/decode.rs:5488 apply_cross_comp_residual */
+            int64_t sy = ((int64_t)res[0][i] << c->bd_c) >> c->bd;
+            int64_t delta = ((int64_t)t[k].res_scale * sy) >> 3;
+            res[k][i] = (int32_t)clamp64((int64_t)res[k][i] + delta, INT32_MIN, INT32_MAX);
+        }
+    }
+    for (int i = 0; i < n * n; i++) oracle_act_inverse(&res[0][i], &res[1][i], &res[2][i]);
+    for (int k = 0; k < 3; k++) {
+        int comp = t[k].comp;
+        uint16_t* plane = comp == 0 ? c->y : (comp == 1 ? c->cb : c->cr);
+        int stride = comp ? c->cw : c->w, ph = comp ? c->ch : c->h;
+        tb_prediction(c, &t[k], pred);
+        add_residual(plane, stride, stride, ph, t[k].x, t[k].y, n, pred, res[k], comp ? c->bd_c : c->bd);
+        if (comp == 0) mark_decoded(c, t[k].x, t[k].y, n);
+    }
+}
+
+static void recon_tb(pic_ctx* c, const hpvcd_tb* t) {
+    int n = 1 << t->log2n;
+    int comp = t->comp;
+    int inter = (t->flags & HPVCD_TB_INTER) != 0;
+    int cbf = (t->flags & (HPVCD_TB_CBF | HPVCD_TB_PCM)) != 0; /* PCM samples travel as levels */
+    uint16_t* plane = comp == 0 ? c->y : (comp == 1 ? c->cb : c->cr);
+    int stride = comp ? c->cw : c->w, pw = comp ? c->cw : c->w, ph = comp ? c->ch : c->h;
+    int bd = comp ? c->bd_c : c->bd;
+    uint16_t pred[32 * 32];
+    int32_t res[32 * 32];
+
+    tb_prediction(c, t, pred);
     if (comp == 0) {
         if (cbf) {
             tb_residual(c, t, res);
@@ -603,7 +652,15 @@ int oracle_picture_planes(const hpvcd_batch* b, uint32_t p, int stage, uint16_t*
     c->decoded = calloc((size_t)c->gw * c->gh, 1);
     memset(y, 0, sizeof(uint16_t) * c->w * c->h);
     if (c->cw) { memset(cb, 0, sizeof(uint16_t) * c->cw * c->ch); memset(cr, 0, sizeof(uint16_t) * c->cw * c->ch); }
-    for (uint32_t i = 0; i < P->tb_count; i++) recon_tb(c, &b->tbs[P->tb_begin + i]);
+    for (uint32_t i = 0; i < P->tb_count; i++) {
+        const hpvcd_tb* t = &b->tbs[P->tb_begin + i];
+        if ((t->flags & HPVCD_TB_ACT) && t->comp == 0 && i + 2 < P->tb_count) {
+            recon_act(c, t);
+            i += 2;
+        } else {
+            recon_tb(c, t);
+        }
+    }
     int deblock_any = 0;
     for (int s = 0; s < (P->num_slices ? P->num_slices : 1); s++)
         deblock_any |= P->num_slices ? !P->slices[s].disabled : !(P->flags & HPVCD_PF_DEBLOCK_DISABLED);

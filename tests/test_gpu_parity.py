@@ -7,6 +7,7 @@ tolerance).  Runs through the C ABI (libhpvcd.so).
 import numpy as np
 import pytest
 
+import handmade as M
 import hpvcd_amd as H
 from hpvcd_amd import _native as N
 from oracle import oracle as O
@@ -37,6 +38,17 @@ SMALL = {
     "ctb32": dict(workload="single512", log2_ctb=5, base_seed=19),
     "ctb16": dict(workload="single512", log2_ctb=4, log2_max_tb=4, base_seed=23, tile_width=256, tile_height=128,
                   out_width=256, out_height=128),
+    # adaptive colour transform on 4:4:4 10-bit (with TS / RDPCM / CCP)
+    "act444": dict(workload="rext4096", grid_cols=1, grid_rows=1, out_width=512, out_height=512, act=1, p_act=0.6,
+                   base_seed=41),
+    # ACT at 8 bits (i32 residuals where the plain path wraps to i16), lossless CUs, TS up to 8x8
+    "act444_8bit": dict(workload="rext4096", bit_depth=8, out_kind=N.OUT_RGB8, grid_cols=1, grid_rows=1,
+                        out_width=512, out_height=512, log2_max_ts=3, act=1, p_act=0.6, p_tqb=0.15, base_seed=45),
+    # I_PCM CUs, exempt from the loop filters
+    "pcm420": dict(workload="single512", pcm=1, p_pcm=0.15, pcm_bit_depth=6, pcm_lf_disabled=1, base_seed=47),
+    # I_PCM CUs in 4:4:4 10-bit, loop-filtered
+    "pcm444_10": dict(workload="rext4096", grid_cols=1, grid_rows=1, out_width=512, out_height=512, pcm=1,
+                      p_pcm=0.15, pcm_bit_depth=7, base_seed=49),
 }
 
 
@@ -83,6 +95,41 @@ def test_modes_identical(gpu, mode):
             got = job.download()
             for a, b in zip(ref, got):
                 assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", [N.MODE_PLAIN, N.MODE_GRAPH])
+def test_modes_act_pcm(gpu, mode):
+    """Per-picture ACT lists (plain mode) and the replayed graph."""
+    rs = H.RecordSet.generate("rext4096", grid_cols=2, grid_rows=2, out_width=1024, out_height=1024, act=1,
+                              p_act=0.6, pcm=1, p_pcm=0.1, pcm_bit_depth=8, pcm_lf_disabled=1, base_seed=53)
+    want = rs.output_buffers()
+    O.decode_batch(rs.batch, want, threads=8)
+    with gpu.job(rs) as job:
+        for _ in range(2):
+            job.run(mode)
+            for g, w in zip(job.download(), want):
+                assert np.array_equal(g, w), f"{int(np.count_nonzero(g != w))} samples differ"
+
+
+@pytest.mark.parametrize("n,scale_cb,scale_cr", [(16, 0, 0), (8, 2, -1), (4, -8, 4), (32, 1, 0)])
+def test_act_lossless_closed_form(gpu, n, scale_cb, scale_cr):
+    rs, r = M.act_lossless_case(seed=n, n=n, scale_cb=scale_cb, scale_cr=scale_cr)
+    with gpu.job(rs) as job:
+        job.run(N.MODE_BATCHED, N.STAGE_RECON)
+        got = job.download_planes(0)
+    for g, t, w in zip(got, r, O.picture_planes(rs.batch, 0, N.STAGE_RECON)):
+        assert np.array_equal(g[:n, :n], 128 + t) and np.array_equal(g, w)
+
+
+@pytest.mark.parametrize("chroma_idc,log2_cu,bd,bits", [(1, 3, 8, 5), (1, 5, 8, 8), (3, 4, 10, 7), (0, 4, 12, 1)])
+def test_pcm_closed_form(gpu, chroma_idc, log2_cu, bd, bits):
+    rs, want = M.pcm_case(seed=log2_cu, chroma_idc=chroma_idc, bd=bd, pcm_bits=bits, log2_cu=log2_cu, size=64)
+    with gpu.job(rs) as job:
+        job.run(N.MODE_BATCHED, N.STAGE_RECON)
+        got = job.download_planes(0)
+    for g, w in zip(got, want):
+        if w is not None:
+            assert np.array_equal(g, w)
 
 
 def test_mixed_formats_one_batch(gpu):

@@ -41,7 +41,7 @@ def test_struct_layouts_match_c():
     assert C.sizeof(N.Output) == 12
     assert C.sizeof(N.Picture) == 124
     assert C.sizeof(N.Batch) == 112
-    assert C.sizeof(N.GenConfig) == 96
+    assert C.sizeof(N.GenConfig) == 104
 
 
 def test_generator_is_deterministic_and_seeded():
@@ -144,6 +144,35 @@ def test_rext_records_carry_ts_and_ccp():
     assert np.any(t["res_scale"] != 0)
 
 
+def test_act_records_are_colocated_triples_with_act_qps():
+    rs = H.RecordSet.generate("rext4096", grid_cols=1, grid_rows=1, out_width=512, out_height=512, act=1,
+                              p_act=0.5, base_seed=41)
+    t = rs.tbs()
+    idx = np.flatnonzero((t["flags"] & N.TB_ACT) != 0)
+    assert len(idx) >= 30 and len(idx) % 3 == 0
+    for i in idx[t["comp"][idx] == 0]:
+        tri = t[i:i + 3]
+        assert list(tri["comp"]) == [0, 1, 2] and np.all(tri["flags"] & N.TB_ACT)
+        assert len(set(zip(tri["x"], tri["y"], tri["log2n"]))) == 1
+        # QP 22, 10-bit: Y qp' = 17 + 12, Cb = qpc(17) + 12, Cr = qpc(19) + 12 (src/decode.rs:3788)
+        assert list(tri["qp"]) == [29, 29, 31]
+        assert tri["mode"][1] == tri["mode"][2] == tri["mode"][0]
+
+
+def test_pcm_records_carry_raw_samples():
+    rs = H.RecordSet.generate("single512", pcm=1, p_pcm=0.2, pcm_bit_depth=5, base_seed=43)
+    t, co = rs.tbs(), rs.coeffs()
+    pcm = t[(t["flags"] & N.TB_PCM) != 0]
+    assert len(pcm) > 30 and np.all(pcm["qp"] == 3)
+    luma = pcm[pcm["comp"] == 0]
+    assert set(np.unique(luma["log2n"])) <= {3, 4, 5} and len(pcm) == 3 * len(luma)
+    for tb in pcm[:40]:
+        n = 1 << int(tb["log2n"])
+        w = co[int(tb["coeff_off"]):int(tb["coeff_off"]) + int(tb["coeff_cnt"])]
+        assert tb["coeff_cnt"] == n * n and np.array_equal(w & 0x3ff, np.arange(n * n))
+        assert np.all((w.view(np.int32) >> 10) >= 0) and np.all((w >> 10) < 32)
+
+
 def _status(rs):
     return N.lib().hpvcd_validate_batch(C.byref(rs.batch))
 
@@ -187,6 +216,20 @@ def test_validation_accepts_generated_and_rejects_malformed():
     pics[0].tb_count += 1
     expect(15, "tb range")
     pics[0].tb_count -= 1
+    # ACT outside 4:4:4; PCM shift and sample range
+    f, q = tbs[i].flags, tbs[i].qp
+    tbs[i].flags = f | N.TB_ACT
+    expect(15, "ACT")
+    tbs[i].flags = N.TB_PCM
+    tbs[i].qp = 8
+    expect(15, "PCM sample shift")
+    tbs[i].qp = 7
+    w = pics[0].coeff_begin + tbs[i].coeff_off
+    old = coeffs[w]
+    coeffs[w] = (old & 0x3ff) | (5 << 10)
+    expect(15, "PCM sample outside")
+    coeffs[w] = old
+    tbs[i].flags, tbs[i].qp = f, q
     assert _status(rs) == 0
 
 

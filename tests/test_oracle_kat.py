@@ -5,6 +5,7 @@ Three kinds of checks (none of them retypes the oracle's own code):
     from the reference source by tests/golden/make_tables_fixture.py);
   * the reference's own known-answer tests, restated: src/decode.rs:8247-8313
     (QP), src/mc.rs:737-758 (weighted MC), src/act.rs:95-148 (ACT),
+    src/decode.rs:2768 (I_PCM samples),
     src/yuv.rs:670-756 (YUV->RGB), src/intra.rs:374-448 (smoothing off),
     src/transform.rs:1388-1439 (extended precision);
   * closed forms worked out by hand from the H.265 equations and invariants
@@ -359,6 +360,51 @@ def test_act_inverse_is_inverse_of_forward_lifting():
                 a, b, c = C.c_int32(y), C.c_int32(cg), C.c_int32(co)
                 O.lib().oracle_act_inverse(C.byref(a), C.byref(b), C.byref(c))
                 assert (a.value, b.value, c.value) == (r0, r1, r2)
+
+
+@pytest.mark.parametrize("n,scale_cb,scale_cr", [(16, 0, 0), (8, 2, -1), (4, -8, 4), (32, 1, 0)])
+def test_act_lossless_picture_closed_form(n, scale_cb, scale_cr):
+    """A lossless ACT TU with neutral prediction reconstructs 128 + the target
+    residuals that its levels encode through CCP and the forward lifting: pins
+    the component order (G->Y, B->Cb, R->Cr), CCP before the colour transform
+    and the unclipped i32 residual path of src/decode.rs:3619."""
+    import handmade as M
+    from hpvcd_amd import _native as N
+
+    rs, r = M.act_lossless_case(seed=n, n=n, scale_cb=scale_cb, scale_cr=scale_cr)
+    got = O.picture_planes(rs.batch, 0, N.STAGE_RECON)
+    for g, t in zip(got, r):
+        assert np.array_equal(g[:n, :n], 128 + t)
+
+
+@pytest.mark.parametrize("chroma_idc,log2_cu,bd,bits", [(1, 3, 8, 5), (1, 5, 8, 8), (3, 4, 10, 7), (0, 4, 12, 1)])
+def test_pcm_picture_closed_form(chroma_idc, log2_cu, bd, bits):
+    """I_PCM CUs reconstruct to pcm_sample << (BitDepth - PcmBitDepth)
+    (src/decode.rs:2788), whatever their neighbours."""
+    import handmade as M
+    from hpvcd_amd import _native as N
+
+    rs, want = M.pcm_case(seed=log2_cu, chroma_idc=chroma_idc, bd=bd, pcm_bits=bits, log2_cu=log2_cu, size=64)
+    got = O.picture_planes(rs.batch, 0, N.STAGE_RECON)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+def test_pcm_loop_filter_disabled_exempts_pcm_samples():
+    """With pcm_loop_filter_disabled_flag the samples of I_PCM CUs leave
+    deblocking and SAO untouched (src/decode.rs:1833, :2303) while the rest
+    of the picture is filtered."""
+    import hpvcd_amd as H
+    from hpvcd_amd import _native as N
+
+    rs = H.RecordSet.generate("single512", pcm=1, p_pcm=0.2, pcm_bit_depth=5, pcm_lf_disabled=1, base_seed=43)
+    recon = O.picture_planes(rs.batch, 0, N.STAGE_RECON)
+    final = O.picture_planes(rs.batch, 0, N.STAGE_SAO)
+    pcm = ((rs.cells() & N.CELL_PCM) != 0).reshape(128, 128)
+    for k, (a, b) in enumerate(zip(recon, final)):
+        m = np.kron(pcm, np.ones((4 if k == 0 else 2,) * 2, bool))
+        assert m.any() and np.array_equal(a[m], b[m])
+        assert np.count_nonzero(a[~m] != b[~m]) > 1000
 
 
 # --------------------------------------------------------------------------------------
