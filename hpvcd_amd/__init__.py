@@ -76,7 +76,8 @@ class RecordSet:
     @classmethod
     def from_arrays(cls, pictures: Sequence[N.Picture], tbs: np.ndarray, coeffs: np.ndarray, cells: np.ndarray,
                     outputs: Sequence[N.Output], sao: Optional[Sequence[N.Sao]] = None,
-                    scaling: Optional[np.ndarray] = None) -> "RecordSet":
+                    scaling: Optional[np.ndarray] = None, pus: Optional[Sequence[N.PU]] = None,
+                    refs: Optional[Sequence[N.Ref]] = None) -> "RecordSet":
         """A batch over caller-built record arrays (what a syntax parser emits):
         ``tbs`` in the hpvcd_tb layout, ``coeffs`` and ``cells`` as uint32 words,
         SAO parameters per CTB (all off when omitted).  The arrays are copied
@@ -104,8 +105,12 @@ class RecordSet:
             b.num_scaling = len(scaling) // N.SCALING_SET_BYTES
             b.scaling = scaling.ctypes.data_as(C.POINTER(C.c_uint8))
         b.num_outputs, b.outputs = len(outputs), C.cast(outs, C.POINTER(N.Output))
+        pu_arr = (N.PU * max(1, len(pus or [])))(*(pus or []))
+        ref_arr = (N.Ref * max(1, len(refs or [])))(*(refs or []))
+        b.num_pus, b.pus = len(pus or []), C.cast(pu_arr, C.POINTER(N.PU))
+        b.num_refs, b.refs = len(refs or []), C.cast(ref_arr, C.POINTER(N.Ref))
         self.batch = b
-        self._keep = (tbs, coeffs, cells, pics, outs, sao_arr, scaling)
+        self._keep = (tbs, coeffs, cells, pics, outs, sao_arr, scaling, pu_arr, ref_arr)
         return self
 
     @classmethod

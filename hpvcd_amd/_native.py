@@ -37,6 +37,9 @@ TB_RDPCM_H = 1 << 4
 TB_RDPCM_V = 1 << 5
 TB_ACT = 1 << 6
 TB_PCM = 1 << 7
+PU_L0 = 1 << 0
+PU_L1 = 1 << 1
+REF_MISSING = 0xFFFFFFFF
 
 CELL_TQB = 1 << 4
 CELL_PCM = 1 << 5
@@ -60,7 +63,7 @@ STAGE_DEBLOCK = 1
 STAGE_SAO = 2
 STAGE_RGB = 3
 
-KERNEL_NAMES = ("residual", "intra", "deblock", "sao", "rgb", "prep")
+KERNEL_NAMES = ("residual", "intra", "deblock", "sao", "rgb", "prep", "mc")
 
 
 class SliceParams(C.Structure):
@@ -88,6 +91,23 @@ class Picture(C.Structure):
         ("coeff_begin", C.c_uint32), ("coeff_count", C.c_uint32),
         ("ctb_begin", C.c_uint32),
         ("cell_begin", C.c_uint32),
+        ("pu_begin", C.c_uint32), ("pu_count", C.c_uint32),
+        ("ref_begin", C.c_uint32), ("ref_count", C.c_uint32),
+    ]
+
+
+class PU(C.Structure):
+    _fields_ = [
+        ("x", C.c_uint16), ("y", C.c_uint16), ("w", C.c_uint8), ("h", C.c_uint8),
+        ("flags", C.c_uint8), ("_pad", C.c_uint8),
+        ("ref", C.c_uint16 * 2), ("mv", (C.c_int16 * 2) * 2),
+    ]
+
+
+class Ref(C.Structure):
+    _fields_ = [
+        ("picture", C.c_uint32), ("weighted", C.c_uint8), ("log2_denom", C.c_uint8 * 2), ("_pad", C.c_uint8),
+        ("weight", C.c_int16 * 3), ("offset", C.c_int16 * 3),
     ]
 
 
@@ -120,6 +140,8 @@ class Batch(C.Structure):
         ("num_cells", C.c_uint32), ("cell_info", C.POINTER(C.c_uint32)),
         ("num_scaling", C.c_uint32), ("scaling", C.POINTER(C.c_uint8)),
         ("num_outputs", C.c_uint32), ("outputs", C.POINTER(Output)),
+        ("num_pus", C.c_uint32), ("pus", C.POINTER(PU)),
+        ("num_refs", C.c_uint32), ("refs", C.POINTER(Ref)),
     ]
 
 
@@ -139,10 +161,12 @@ class GenConfig(C.Structure):
         ("matrix", C.c_uint8), ("full_range", C.c_uint8), ("out_kind", C.c_uint8), ("num_outputs", C.c_uint8),
         ("log2_res_scale_max", C.c_uint8),
         ("act", C.c_uint8), ("pcm", C.c_uint8), ("pcm_bit_depth", C.c_uint8), ("pcm_lf_disabled", C.c_uint8),
+        ("inter", C.c_uint8), ("num_refs", C.c_uint8), ("weighted", C.c_uint8), ("amp", C.c_uint8),
         ("_pad", C.c_uint8 * 2),
         ("p_split_cu", C.c_float), ("p_nxn", C.c_float), ("p_split_tu", C.c_float), ("p_cbf", C.c_float),
         ("p_sig", C.c_float), ("p_level_geom", C.c_float), ("p_ts", C.c_float), ("p_tqb", C.c_float),
         ("p_act", C.c_float), ("p_pcm", C.c_float),
+        ("p_intra_cu", C.c_float), ("p_skip", C.c_float), ("p_bi", C.c_float), ("mv_sigma", C.c_float),
         ("config_id", C.c_uint32),
         ("base_seed", C.c_uint64),
     ]

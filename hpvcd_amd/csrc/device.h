@@ -34,6 +34,28 @@ struct DevPic {
     uint32_t suppress_any;          // some cell is TQB or PCM-with-loop-filter-disabled
     uint32_t sao_across_slices;     // pps flag && every slice's flag (SAO neighbour gate)
     uint32_t deblock_on;            // some slice of the picture enables deblocking
+    uint32_t inter;                 // the picture has prediction units (MC output in the plane)
+};
+
+// One prediction unit as the MC kernel sees it: references resolved to
+// class-local pictures (lists whose reference is missing are dropped from
+// flags) and the weight-table row of each list.
+struct DevPu {
+    int32_t x, y, w, h;             // luma geometry
+    uint32_t flags;                 // usable lists: bit 0 list 0, bit 1 list 1
+    uint32_t weighted;              // explicit weighted prediction
+    int32_t log2_denom[2];          // luma, chroma
+    int32_t mv[2][2];               // [list][x, y], quarter luma samples
+    uint32_t ref[2];                // class-local reference picture per list
+    int32_t weight[2][3], offset[2][3];
+};
+
+// MC work item: a block of at most 16 x 16 luma samples of one PU (and the
+// co-located chroma), offset (ox, oy) from the PU origin.
+struct DevMcTask {
+    uint32_t pu;
+    uint16_t pic;
+    uint8_t ox, oy;
 };
 
 // One (picture, plane, CTB) work unit of the intra reconstruction wavefront.

@@ -11,7 +11,11 @@
 //   SAO offset signs and scaling               src/decode.rs:1507-1614 parse_sao
 //   CCP ResScaleVal                            src/decode.rs:4257 (log2_res_scale_abs_plus1, sign)
 //   grid placement / crop window               src/lib.rs:1610-1653, :1754-1817
+//   inter CU, part_mode, PU edges, skip        src/decode.rs:6380 decode_inter_cu, :6462
+//   inter bS (intra side, coefficients, motion) src/decode.rs:2955 finalize_coeff_bs, :620
+//   pred_weight_table rows                     src/inter.rs:141 PredWeightTable
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <thread>
@@ -54,6 +58,21 @@ struct PicGen {
     int qp_y, bd;
     std::vector<hpvcd_tb>& tbs;
     std::vector<uint32_t>& coeffs;
+    std::vector<hpvcd_pu>& pus;
+    std::vector<hpvcd_ref>& refs;
+    // inter pictures: reference entries (list 0 first), per-cell motion and
+    // the transform / prediction edges for the boundary strengths
+    bool inter_pic = false;
+    int nref[2] = {0, 0};
+    std::vector<uint32_t> ref_pic;
+    struct Motion {
+        uint8_t intra = 1, flags = 0;
+        int32_t pic[2] = {-1, -1};
+        int16_t mv[2][2] = {{0, 0}, {0, 0}};
+    };
+    std::vector<Motion> motion;
+    std::vector<uint8_t> tu_v, tu_h, pu_v, pu_h, nz;
+    bool cu_inter = false;
     std::vector<uint32_t> cell;      // info word without QP (QP packed at the end)
     std::vector<int8_t> cell_qp;
     std::vector<uint8_t> mode_y;     // luma intra mode per 4x4 (src/decode.rs mode_y)
@@ -64,8 +83,9 @@ struct PicGen {
     int cu_x0, cu_y0, cu_pu;
     bool cu_tqb;
 
-    PicGen(const hpvcd_gen_config& c, uint64_t seed, std::vector<hpvcd_tb>& t, std::vector<uint32_t>& co)
-        : cfg(c), rng{seed}, tbs(t), coeffs(co) {
+    PicGen(const hpvcd_gen_config& c, uint64_t seed, std::vector<hpvcd_tb>& t, std::vector<uint32_t>& co,
+           std::vector<hpvcd_pu>& pu_out, std::vector<hpvcd_ref>& ref_out, uint32_t pic_index)
+        : cfg(c), rng{seed}, tbs(t), coeffs(co), pus(pu_out), refs(ref_out) {
         w = c.tile_width;
         h = c.tile_height;
         gw = (w + 3) / 4;
@@ -81,6 +101,185 @@ struct PicGen {
         cell_qp.assign((size_t)gw * gh, (int8_t)qp_y);
         mode_y.assign((size_t)gw * gh, MODE_DC);
         coeff_base = (uint32_t)co.size();
+        inter_pic = c.inter && pic_index > 0;
+        if (inter_pic) setup_refs(pic_index);
+    }
+
+    // Low-delay reference lists: list 0 holds the nearest earlier pictures,
+    // list 1 (B pictures) the same pictures farthest first.  Each entry is a
+    // pred_weight_table row (src/inter.rs:141): weights around 1 << denom,
+    // offsets in 8-bit units scaled to the bit depth.
+    void setup_refs(uint32_t pic_index) {
+        int n = std::max(1, std::min<int>(cfg.num_refs ? cfg.num_refs : 1, (int)pic_index));
+        nref[0] = n;
+        nref[1] = cfg.p_bi > 0.f ? n : 0;
+        for (int l = 0; l < 2; l++)
+            for (int i = 0; i < nref[l]; i++) {
+                hpvcd_ref e;
+                std::memset(&e, 0, sizeof(e));
+                e.picture = pic_index - 1 - (uint32_t)(l == 0 ? i : n - 1 - i);
+                e.weighted = cfg.weighted ? 1 : 0;
+                e.log2_denom[0] = 6;
+                e.log2_denom[1] = 6;
+                for (int k = 0; k < 3; k++) {
+                    bool signalled = cfg.weighted && rng.bern(0.8f);
+                    e.weight[k] = (int16_t)(signalled ? rng.sint(32, 96) : 64);
+                    e.offset[k] = (int16_t)(signalled ? rng.sint(-16, 16) * (1 << (bd - 8)) : 0);
+                }
+                refs.push_back(e);
+                ref_pic.push_back(e.picture);
+            }
+        size_t ncell = (size_t)gw * gh;
+        motion.assign(ncell, Motion());
+        tu_v.assign(ncell, 0); tu_h.assign(ncell, 0); pu_v.assign(ncell, 0); pu_h.assign(ncell, 0); nz.assign(ncell, 0);
+    }
+
+    // transform-block boundary of an inter CU or TU (edge flags only; the
+    // strength is derived once the picture is complete)
+    void mark_tu_edges(int x0, int y0, int size) {
+        if (x0 > 0 && x0 < w)
+            for (int yy = y0; yy < std::min(y0 + size, h); yy += 4) tu_v[(yy / 4) * gw + x0 / 4] = 1;
+        if (y0 > 0 && y0 < h)
+            for (int xx = x0; xx < std::min(x0 + size, w); xx += 4) tu_h[(y0 / 4) * gw + xx / 4] = 1;
+    }
+
+    static bool motion_differs(const Motion& p, const Motion& q) {
+        // src/decode.rs:620 inter_motion_differs (reference pictures compared by identity)
+        int32_t pr[2] = {(p.flags & 1) ? p.pic[0] : -1, (p.flags & 2) ? p.pic[1] : -1};
+        int32_t qr[2] = {(q.flags & 1) ? q.pic[0] : -1, (q.flags & 2) ? q.pic[1] : -1};
+        bool direct = pr[0] == qr[0] && pr[1] == qr[1];
+        bool crossed = pr[0] == qr[1] && pr[1] == qr[0];
+        if (!direct && !crossed) return true;
+        auto mv = [](const Motion& m, int l, int c) -> int { return (m.flags >> l) & 1 ? m.mv[l][c] : 0; };
+        auto far = [&](int pl, int ql) {
+            return std::abs(mv(p, pl, 0) - mv(q, ql, 0)) >= 4 || std::abs(mv(p, pl, 1) - mv(q, ql, 1)) >= 4;
+        };
+        if (pr[0] != pr[1]) return direct ? (far(0, 0) || far(1, 1)) : (far(0, 1) || far(1, 0));
+        return (far(0, 0) || far(1, 1)) && (far(0, 1) || far(1, 0));
+    }
+
+    // src/decode.rs:2955 finalize_coeff_bs over the whole inter picture
+    void finalize_bs() {
+        for (int gy = 0; gy < gh; gy++)
+            for (int gx = 0; gx < gw; gx++) {
+                size_t g = (size_t)gy * gw + gx;
+                uint32_t& c = cell[g];
+                if (gx > 0 && (tu_v[g] || pu_v[g])) {
+                    uint32_t bs = c & 3u;
+                    if (bs < 2 && (motion[g].intra || motion[g - 1].intra)) bs = 2;
+                    else if (bs < 1 && tu_v[g] && (nz[g] || nz[g - 1])) bs = 1;
+                    else if (bs < 1 && motion_differs(motion[g], motion[g - 1])) bs = 1;
+                    c = (c & ~3u) | bs;
+                }
+                if (gy > 0 && (tu_h[g] || pu_h[g])) {
+                    uint32_t bs = (c >> 2) & 3u;
+                    if (bs < 2 && (motion[g].intra || motion[g - gw].intra)) bs = 2;
+                    else if (bs < 1 && tu_h[g] && (nz[g] || nz[g - gw])) bs = 1;
+                    else if (bs < 1 && motion_differs(motion[g], motion[g - gw])) bs = 1;
+                    c = (c & ~0xcu) | (bs << 2);
+                }
+            }
+    }
+
+    double gauss() {
+        double u1 = ((double)(rng.next() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        double u2 = (double)(rng.next() >> 11) * (1.0 / 9007199254740992.0);
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+
+    // One PU: prediction lists (no bi for 8x4 / 4x8), reference entries and
+    // quarter-sample motion vectors ~ N(0, mv_sigma) keeping the block within
+    // 64 samples of the picture.
+    void prediction_unit(int px, int py, int pw, int ph) {
+        hpvcd_pu pu;
+        std::memset(&pu, 0, sizeof(pu));
+        pu.x = (uint16_t)px; pu.y = (uint16_t)py; pu.w = (uint8_t)pw; pu.h = (uint8_t)ph;
+        if (nref[1] > 0) {
+            if (pw + ph != 12 && rng.bern(cfg.p_bi)) pu.flags = HPVCD_PU_L0 | HPVCD_PU_L1;
+            else pu.flags = (rng.next() & 1) ? HPVCD_PU_L0 : HPVCD_PU_L1;
+        } else {
+            pu.flags = HPVCD_PU_L0;
+        }
+        Motion m;
+        m.intra = 0;
+        m.flags = pu.flags;
+        for (int l = 0; l < 2; l++) {
+            if (!((pu.flags >> l) & 1)) continue;
+            int idx = (int)rng.uni((uint32_t)nref[l]);
+            pu.ref[l] = (uint16_t)(l == 0 ? idx : nref[0] + idx);
+            double sig = cfg.mv_sigma * 4.0;
+            int mvx = (int)std::lround(gauss() * sig), mvy = (int)std::lround(gauss() * sig);
+            mvx = std::clamp(mvx, (-64 - px - pw) * 4, (w + 64 - px) * 4);
+            mvy = std::clamp(mvy, (-64 - py - ph) * 4, (h + 64 - py) * 4);
+            pu.mv[l][0] = (int16_t)mvx;
+            pu.mv[l][1] = (int16_t)mvy;
+            m.pic[l] = (int32_t)ref_pic[pu.ref[l]];
+            m.mv[l][0] = pu.mv[l][0];
+            m.mv[l][1] = pu.mv[l][1];
+        }
+        pus.push_back(pu);
+        for (int yy = py; yy < std::min(py + ph, h); yy += 4)
+            for (int xx = px; xx < std::min(px + pw, w); xx += 4) motion[(yy / 4) * gw + xx / 4] = m;
+    }
+
+    // TBs without residual tiling an inter CU that codes none (skip, or
+    // rqt_root_cbf = 0): the MC prediction is the reconstruction.
+    void inter_tiles(int x0, int y0, int log2) {
+        if (log2 > log2_max_tb) {
+            int half = 1 << (log2 - 1);
+            for (int k = 0; k < 4; k++) inter_tiles(x0 + (k & 1) * half, y0 + (k >> 1) * half, log2 - 1);
+            return;
+        }
+        tbs.push_back(make_tb(x0, y0, log2, 0, MODE_DC, false));
+        if (!cfg.chroma_idc) return;
+        int cl = cfg.chroma_idc == 3 ? log2 : log2 - 1;
+        for (int comp = 1; comp <= 2; comp++) tbs.push_back(make_tb(x0 / sub_w, y0 / sub_h, cl, comp, MODE_DC, false));
+    }
+
+    // src/decode.rs:6380 decode_inter_cu: part_mode, PUs in coding order,
+    // then the residual tree or none
+    void inter_cu(int x0, int y0, int log2) {
+        int cb = 1 << log2;
+        cu_inter = true;
+        for (int yy = y0; yy < std::min(y0 + cb, h); yy += 4)
+            for (int xx = x0; xx < std::min(x0 + cb, w); xx += 4) cell[(yy / 4) * gw + xx / 4] &= ~HPVCD_CELL_INTRA;
+        mark_tu_edges(x0, y0, cb);
+        bool skip = rng.bern(cfg.p_skip);
+        int part = 0;  // 2Nx2N, 2NxN, Nx2N, NxN, 2NxnU, 2NxnD, nLx2N, nRx2N
+        if (!skip) {
+            int opts[8], n = 0;
+            opts[n++] = 0; opts[n++] = 1; opts[n++] = 2;
+            if (log2 == log2_min_cb && log2 > 3) opts[n++] = 3;
+            if (cfg.amp && log2 > log2_min_cb) for (int k = 4; k < 8; k++) opts[n++] = k;
+            part = opts[rng.uni((uint32_t)n)];
+        }
+        int q = cb / 4, hf = cb / 2;
+        int rects[4][4], nr = 0;
+        auto add = [&](int x, int y, int rw, int rh) { rects[nr][0] = x; rects[nr][1] = y; rects[nr][2] = rw; rects[nr][3] = rh; nr++; };
+        switch (part) {
+            case 0: add(x0, y0, cb, cb); break;
+            case 1: add(x0, y0, cb, hf); add(x0, y0 + hf, cb, hf); break;
+            case 2: add(x0, y0, hf, cb); add(x0 + hf, y0, hf, cb); break;
+            case 3: add(x0, y0, hf, hf); add(x0 + hf, y0, hf, hf); add(x0, y0 + hf, hf, hf); add(x0 + hf, y0 + hf, hf, hf); break;
+            case 4: add(x0, y0, cb, q); add(x0, y0 + q, cb, cb - q); break;
+            case 5: add(x0, y0, cb, cb - q); add(x0, y0 + cb - q, cb, q); break;
+            case 6: add(x0, y0, q, cb); add(x0 + q, y0, cb - q, cb); break;
+            default: add(x0, y0, cb - q, cb); add(x0 + cb - q, y0, q, cb); break;
+        }
+        for (int i = 0; i < nr; i++) {
+            int px = rects[i][0], py = rects[i][1], pw = rects[i][2], ph = rects[i][3];
+            if (px > x0)
+                for (int yy = py; yy < std::min(py + ph, h); yy += 4) pu_v[(yy / 4) * gw + px / 4] = 1;
+            if (py > y0)
+                for (int xx = px; xx < std::min(px + pw, w); xx += 4) pu_h[(py / 4) * gw + xx / 4] = 1;
+            prediction_unit(px, py, pw, ph);
+        }
+        bool root_cbf = !skip && rng.bern(0.85f);
+        if (root_cbf) transform_tree(x0, y0, x0, y0, log2, 0, 0, false, cfg.max_transform_depth_intra, false, false);
+        else inter_tiles(x0, y0, log2);
+        for (int yy = y0; yy < std::min(y0 + cb, h); yy += 4)
+            for (int xx = x0; xx < std::min(x0 + cb, w); xx += 4) cell_qp[(yy / 4) * gw + xx / 4] = (int8_t)qp_y;
+        cu_inter = false;
     }
 
     // src/decode.rs:3018 mark_block_edges (bS kept as the max of the marks)
@@ -181,9 +380,17 @@ struct PicGen {
         tb.qp = (uint8_t)tb_qp(comp, act);
         if (act) tb.flags |= HPVCD_TB_ACT;
         if (cu_tqb) tb.flags |= HPVCD_TB_TQB;
+        if (cu_inter) tb.flags |= HPVCD_TB_INTER;
         if (cbf) {
             tb.flags |= HPVCD_TB_CBF;
             if (cfg.transform_skip && !cu_tqb && log2n <= cfg.log2_max_ts && rng.bern(cfg.p_ts)) tb.flags |= HPVCD_TB_TS;
+            // explicit_rdpcm_flag / explicit_rdpcm_dir_flag of an inter TB with
+            // transform skip or bypass (RExt, src/decode.rs:4356)
+            if (cu_inter && cfg.implicit_rdpcm && (tb.flags & (HPVCD_TB_TS | HPVCD_TB_TQB))) {
+                uint32_t r = rng.uni(3);
+                if (r == 1) tb.flags |= HPVCD_TB_RDPCM_H;
+                else if (r == 2) tb.flags |= HPVCD_TB_RDPCM_V;
+            }
             emit_levels(tb);
         }
         return tb;
@@ -207,7 +414,7 @@ struct PicGen {
     }
 
     void do_chroma(int lx, int ly, int luma_log2, bool cbf_cb, bool cbf_cr, bool cross_comp) {
-        uint8_t mode = chroma_mode_at(lx, ly);
+        uint8_t mode = cu_inter ? MODE_DC : chroma_mode_at(lx, ly);
         int clog2 = cfg.chroma_idc == 3 ? luma_log2 : luma_log2 - 1;
         int cx0 = lx / sub_w, cy0 = ly / sub_h;
         for (int comp = 1; comp <= 2; comp++) {
@@ -224,7 +431,7 @@ struct PicGen {
     void act_unit(int x0, int y0, int log2, uint8_t luma_mode, bool cbf_luma, bool cbf_cb, bool cbf_cr) {
         bool cross_comp = cfg.cross_component && cbf_luma;
         tbs.push_back(make_tb(x0, y0, log2, 0, luma_mode, cbf_luma, true));
-        uint8_t mode = chroma_mode_at(x0, y0);
+        uint8_t mode = cu_inter ? MODE_DC : chroma_mode_at(x0, y0);
         for (int comp = 1; comp <= 2; comp++) {
             int scale = cross_comp ? draw_res_scale() : 0;
             hpvcd_tb tb = make_tb(x0, y0, log2, comp, mode, comp == 1 ? cbf_cb : cbf_cr, true);
@@ -236,16 +443,25 @@ struct PicGen {
     void transform_unit(int x0, int y0, int xbase, int ybase, int log2, int blk, bool cbf_cb, bool cbf_cr) {
         bool cbf_luma = rng.bern(cfg.p_cbf);
         int size = 1 << log2;
-        mark_block_edges(x0, y0, size, 2);
-        uint8_t luma_mode = mode_y[(y0 / 4) * gw + x0 / 4];
-        // tu_residual_act_flag is present for a TU with residual when every
-        // chroma mode of the CU is the derived mode (src/decode.rs:4229)
-        if (cfg.act && cfg.chroma_idc == 3 && (cbf_luma || cbf_cb || cbf_cr) && cu_chroma_dm[0] && cu_chroma_dm[1] &&
-            cu_chroma_dm[2] && cu_chroma_dm[3] && rng.bern(cfg.p_act)) {
+        if (cu_inter) {
+            mark_tu_edges(x0, y0, size);
+            if (cbf_luma)
+                for (int yy = y0; yy < std::min(y0 + size, h); yy += 4)
+                    for (int xx = x0; xx < std::min(x0 + size, w); xx += 4) nz[(yy / 4) * gw + xx / 4] = 1;
+        } else {
+            mark_block_edges(x0, y0, size, 2);
+        }
+        uint8_t luma_mode = cu_inter ? MODE_DC : mode_y[(y0 / 4) * gw + x0 / 4];
+        // tu_residual_act_flag is present for a TU with residual of an inter
+        // CU, or of an intra CU whose chroma modes are all the derived mode
+        // (src/decode.rs:4229)
+        bool dm_all = cu_chroma_dm[0] && cu_chroma_dm[1] && cu_chroma_dm[2] && cu_chroma_dm[3];
+        if (cfg.act && cfg.chroma_idc == 3 && (cbf_luma || cbf_cb || cbf_cr) && (cu_inter || dm_all) &&
+            rng.bern(cfg.p_act)) {
             act_unit(x0, y0, log2, luma_mode, cbf_luma, cbf_cb, cbf_cr);
             return;
         }
-        bool cross_comp = cfg.cross_component && cbf_luma && chroma_dm_at(x0, y0);
+        bool cross_comp = cfg.cross_component && cbf_luma && (cu_inter || chroma_dm_at(x0, y0));
         tbs.push_back(make_tb(x0, y0, log2, 0, luma_mode, cbf_luma));
         if (cfg.chroma_idc == 0) return;
         if (log2 > 2 || cfg.chroma_idc == 3) do_chroma(x0, y0, log2, cbf_cb, cbf_cr, cross_comp);
@@ -308,6 +524,11 @@ struct PicGen {
         int size = 1 << log2;
         cu_tqb = cfg.p_tqb > 0.f && rng.bern(cfg.p_tqb);
         if (cu_tqb) fill_cells(x0, y0, size, HPVCD_CELL_TQB);
+        // cu_skip_flag / pred_mode_flag of an inter picture
+        if (inter_pic && !rng.bern(cfg.p_intra_cu)) {
+            inter_cu(x0, y0, log2);
+            return;
+        }
         bool nxn = log2 == log2_min_cb && log2 > 2 && rng.bern(cfg.p_nxn);
         // pcm_flag follows part_mode for 2Nx2N CUs of 8x8..32x32 (src/decode.rs:2647)
         if (cfg.pcm && !nxn && log2 >= 3 && log2 <= 5 && rng.bern(cfg.p_pcm)) {
@@ -387,10 +608,12 @@ struct TileOut {
     std::vector<hpvcd_tb> tbs;
     std::vector<uint32_t> coeffs, cells;
     std::vector<hpvcd_sao> sao;
+    std::vector<hpvcd_pu> pus;
+    std::vector<hpvcd_ref> refs;
 };
 
 void gen_tile(const hpvcd_gen_config& cfg, uint32_t tile, TileOut& out) {
-    PicGen g(cfg, tile_seed(cfg.base_seed, cfg.config_id, tile), out.tbs, out.coeffs);
+    PicGen g(cfg, tile_seed(cfg.base_seed, cfg.config_id, tile), out.tbs, out.coeffs, out.pus, out.refs, tile);
     int ctb = 1 << cfg.log2_ctb;
     int cols = (g.w + ctb - 1) / ctb, rows = (g.h + ctb - 1) / ctb;
     out.sao.resize((size_t)cols * rows);
@@ -399,6 +622,7 @@ void gen_tile(const hpvcd_gen_config& cfg, uint32_t tile, TileOut& out) {
             g.sao_params(out.sao[(size_t)ry * cols + rx]);
             g.coding_quadtree(rx * ctb, ry * ctb, cfg.log2_ctb);
         }
+    if (g.inter_pic) g.finalize_bs();
     out.cells.resize(g.cell.size());
     for (size_t i = 0; i < g.cell.size(); i++)
         out.cells[i] = (g.cell[i] & 0xffu) | ((uint32_t)(uint8_t)g.cell_qp[i] << 8);
@@ -442,6 +666,12 @@ extern "C" void hpvcd_gen_default_config(hpvcd_gen_config* c) {
     c->p_level_geom = 0.5f;
     c->p_ts = 0.2f;
     c->p_tqb = 0.0f;
+    c->num_refs = 4;
+    c->amp = 1;
+    c->p_intra_cu = 0.1f;
+    c->p_skip = 0.3f;
+    c->p_bi = 0.5f;
+    c->mv_sigma = 16.0f;
     c->config_id = 0;
     c->base_seed = 0x48505643ull;
 }
@@ -517,8 +747,9 @@ extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_rec
         p.crop_top = 0;
         p.win_w = c.tile_width;
         p.win_h = c.tile_height;
-        p.dst_x = (int32_t)col * c.tile_width;
-        p.dst_y = (int32_t)row * c.tile_height;
+        // one output per picture (frames, independent images) or one grid
+        p.dst_x = nout > 1 ? 0 : (int32_t)col * c.tile_width;
+        p.dst_y = nout > 1 ? 0 : (int32_t)row * c.tile_height;
         p.copy_w = std::max(0, std::min(c.tile_width, c.out_width - p.dst_x));
         p.copy_h = std::max(0, std::min(c.tile_height, c.out_height - p.dst_y));
         p.out_index = (uint32_t)(i % (uint32_t)nout == 0 ? 0 : 0);
@@ -532,7 +763,13 @@ extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_rec
         p.coeff_count = (uint32_t)t.coeffs.size();
         p.ctb_begin = (uint32_t)r->sao.size();
         p.cell_begin = (uint32_t)r->cells.size();
+        p.pu_begin = (uint32_t)r->pus.size();
+        p.pu_count = (uint32_t)t.pus.size();
+        p.ref_begin = (uint32_t)r->refs.size();
+        p.ref_count = (uint32_t)t.refs.size();
         r->pictures.push_back(p);
+        r->pus.insert(r->pus.end(), t.pus.begin(), t.pus.end());
+        r->refs.insert(r->refs.end(), t.refs.begin(), t.refs.end());
         r->tbs.insert(r->tbs.end(), t.tbs.begin(), t.tbs.end());
         r->coeffs.insert(r->coeffs.end(), t.coeffs.begin(), t.coeffs.end());
         r->sao.insert(r->sao.end(), t.sao.begin(), t.sao.end());

@@ -14,6 +14,8 @@ struct hpvcd_records {
     std::vector<uint32_t> cells;
     std::vector<uint8_t> scaling;
     std::vector<hpvcd_output> outputs;
+    std::vector<hpvcd_pu> pus;
+    std::vector<hpvcd_ref> refs;
 };
 
 namespace hpvcd {

@@ -318,13 +318,14 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             if (left_bits) left_need = (uint32_t)(y0 + n2 - ((int)__builtin_ctzll(left_bits) << lgv));
             if (corner && y0 > 0) left_need = max(left_need, (uint32_t)y0);
         }
-        // an I_PCM block reads no neighbours: its words all name the zero
-        // slot through the angular form with fraction 0, so prediction is 0
-        // and the samples arrive as its residual
-        const bool pcm = (tb.flags & HPVCD_TB_PCM) != 0;
-        if (pcm) top_need = left_need = 0;
+        // I_PCM and inter blocks read no neighbours: their words name, through
+        // the angular form with fraction 0, the zero slot (PCM: the samples
+        // arrive as the residual) or the sample's own tile slot, which holds
+        // the motion-compensated prediction (inter: pred + residual)
+        const bool pcm = (tb.flags & HPVCD_TB_PCM) != 0, inter = (tb.flags & HPVCD_TB_INTER) != 0;
+        if (pcm || inter) top_need = left_need = 0;
         TbShape s = tb_shape(log2n, mode, plane, pflags, chroma_idc, angles, inv_angles);
-        if (pcm) {
+        if (pcm || inter) {
             s.cls = kClsAngular;
             s.filt = false;
         }
@@ -350,8 +351,11 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             desc_g[tb_list + e] = d;
         }
         uint32_t* dst = smap_g + sbase + soff;
-        for (int p = lane; p < n * n; p += 64)
-            dst[p] = pcm ? (uint32_t)kZero | ((uint32_t)kZero << 13) : sample_word(s, p & (n - 1), p >> log2n, Ms);
+        for (int p = lane; p < n * n; p += 64) {
+            const uint32_t own = (uint32_t)(kTopW + (y0 + (p >> log2n)) * kRowW + x0 + (p & (n - 1)) + 1);
+            dst[p] = pcm ? (uint32_t)kZero | ((uint32_t)kZero << 13)
+                         : inter ? own | (own << 13) : sample_word(s, p & (n - 1), p >> log2n, Ms);
+        }
     }
     __syncthreads();
     HPVCD_PREP_T(5)
@@ -636,6 +640,24 @@ __global__ void __launch_bounds__(kIntraThreads)
                 for (int b = 0; b < kCopyBatch; b++) {
                     const int e = e0 + b * kIntraThreads;
                     if (e < nres) resb[(e >> ctbw_log2) * kMaxCtb + (e & (ctbw - 1))] = v[b];
+                }
+            }
+            if (P.inter) {
+                // motion-compensated prediction of the unit, the predictor of its inter TBs
+                for (int e0 = tid; e0 < nres; e0 += kCopyBatch * kIntraThreads) {
+                    T v[kCopyBatch];
+#pragma unroll
+                    for (int b = 0; b < kCopyBatch; b++) {
+                        const int e = e0 + b * kIntraThreads;
+                        const int ry = e >> ctbw_log2, rx = e - (ry << ctbw_log2);
+                        const int px = ux0 + rx, py = uy0 + ry;
+                        v[b] = (e < nres && px < pw && py < ph) ? pl[(size_t)py * pw + px] : (T)0;
+                    }
+#pragma unroll
+                    for (int b = 0; b < kCopyBatch; b++) {
+                        const int e = e0 + b * kIntraThreads;
+                        if (e < nres) L.tile[kTopW + (e >> ctbw_log2) * kRowW + (e & (ctbw - 1)) + 1] = (uint16_t)v[b];
+                    }
                 }
             }
         }

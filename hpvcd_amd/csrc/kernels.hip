@@ -308,6 +308,122 @@ __global__ void __launch_bounds__(256) k_act(const DevPic* __restrict__ pics, co
 }
 
 // ---------------------------------------------------------------------------
+// K1c: motion compensation (src/decode.rs:7000 motion_compensate, src/mc.rs)
+//
+// One workgroup per block of at most 16 x 16 luma samples of a PU; for each
+// plane and list the clamped reference window ((bh + taps - 1) x (bw + taps
+// - 1), src/mc.rs:310) is staged in LDS, filtered horizontally into an LDS
+// row buffer (>> BitDepth - 8) and vertically (>> 6), and narrowed to i16 as
+// the reference's 14-bit intermediates are.  The separable form equals the
+// reference's full-sample / one-direction special cases exactly (the
+// identity tap 64 is a shift by 6 on either side).  The lists then combine
+// (uni / bi / explicit weighted, src/mc.rs:538-733) into the current
+// picture's plane; reference samples are the final planes of earlier
+// pictures of the job, read in place.
+// ---------------------------------------------------------------------------
+__constant__ int8_t kLumaFilter[4][8] = {{0, 0, 0, 64, 0, 0, 0, 0},
+                                         {-1, 4, -10, 58, 17, -5, 1, 0},
+                                         {-1, 4, -11, 40, 40, -11, 4, -1},
+                                         {0, 1, -5, 17, 58, -10, 4, -1}};
+__constant__ int8_t kChromaFilter[8][4] = {{0, 64, 0, 0},    {-2, 58, 10, -2}, {-4, 54, 16, -2}, {-6, 46, 28, -4},
+                                           {-4, 36, 36, -4}, {-4, 28, 46, -6}, {-2, 16, 54, -4}, {-2, 10, 58, -2}};
+
+constexpr int kMcBlock = 16;
+constexpr int kMcWin = kMcBlock + 7;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_mc(const DevPic* __restrict__ pics, const DevPu* __restrict__ pus,
+                                            const DevMcTask* __restrict__ tasks, T* __restrict__ cur,
+                                            const T* __restrict__ refs) {
+    __shared__ int32_t win[kMcWin * kMcWin];
+    __shared__ int32_t rows[kMcWin * kMcBlock];
+    const int tid = threadIdx.x;
+    const DevMcTask tk = tasks[blockIdx.x];
+    const DevPu& U = pus[tk.pu];
+    const DevPic& P = pics[tk.pic];
+    const int flags = (int)U.flags;
+    const int bw_l = min(kMcBlock, U.w - (int)tk.ox), bh_l = min(kMcBlock, U.h - (int)tk.oy);
+    const int nplanes = P.chroma_idc ? 3 : 1;
+    for (int pl = 0; pl < nplanes; pl++) {
+        const int sw = pl ? P.sub_w_log2 : 0, sh = pl ? P.sub_h_log2 : 0;
+        const int bw = bw_l >> sw, bh = bh_l >> sh;
+        const int bx = (U.x >> sw) + ((int)tk.ox >> sw), by = (U.y >> sh) + ((int)tk.oy >> sh);
+        const int pw = pl ? P.cw : P.w, ph = pl ? P.ch : P.h;
+        const int bd = pl ? P.bd_c : P.bd, s1 = bd - 8;
+        const int taps = pl ? 4 : 8, half = taps / 2 - 1;
+        const int ww = bw + taps - 1, wh = bh + taps - 1;
+        const bool mine = tid < bw * bh;
+        const int sx = mine ? tid % bw : 0, sy = mine ? tid / bw : 0;
+        int32_t pred[2] = {0, 0};
+        for (int l = 0; l < 2; l++) {
+            if (!((flags >> l) & 1)) continue;
+            const T* rp = refs + pics[U.ref[l]].plane_off[pl];
+            int mx = U.mv[l][0], my = U.mv[l][1], fx, fy, ix, iy;
+            if (pl == 0) {
+                ix = bx + (mx >> 2); fx = mx & 3;
+                iy = by + (my >> 2); fy = my & 3;
+            } else {
+                mx = mx * 2 / (1 << sw);   // chroma vector, src/decode.rs:7293
+                my = my * 2 / (1 << sh);
+                ix = bx + (mx >> 3); fx = mx & 7;
+                iy = by + (my >> 3); fy = my & 7;
+            }
+            // clamped reference window
+            for (int e = tid; e < ww * wh; e += 256) {
+                const int r = e / ww, c = e - r * ww;
+                const int x = min(max(ix - half + c, 0), pw - 1), y = min(max(iy - half + r, 0), ph - 1);
+                win[e] = (int32_t)rp[(size_t)y * pw + x];
+            }
+            __syncthreads();
+            const int8_t* hf = pl ? kChromaFilter[fx] : kLumaFilter[fx];
+            const int8_t* vf = pl ? kChromaFilter[fy] : kLumaFilter[fy];
+            for (int e = tid; e < wh * bw; e += 256) {
+                const int r = e / bw, c = e - r * bw;
+                int32_t acc = 0;
+                for (int t = 0; t < taps; t++) acc += (int32_t)hf[t] * win[r * ww + c + t];
+                rows[e] = acc >> s1;
+            }
+            __syncthreads();
+            if (mine) {
+                int32_t acc = 0;
+                for (int t = 0; t < taps; t++) acc += (int32_t)vf[t] * rows[(sy + t) * bw + sx];
+                pred[l] = wrap16(acc >> 6);
+            }
+            __syncthreads();
+        }
+        const int px = bx + sx, py = by + sy;
+        if (!mine || px >= pw || py >= ph) continue;
+        const int32_t maxv = (1 << bd) - 1;
+        const int comp = pl, dn = pl ? 1 : 0;
+        int32_t v;
+        if ((flags & 3) == 3) {
+            if (U.weighted) {
+                const int log2wd = U.log2_denom[dn] + 14 - bd;
+                const int64_t rnd = ((int64_t)U.offset[0][comp] + U.offset[1][comp] + 1) << log2wd;
+                v = (int32_t)(((int64_t)pred[0] * U.weight[0][comp] + (int64_t)pred[1] * U.weight[1][comp] + rnd) >>
+                              (log2wd + 1));
+            } else {
+                const int shift = 15 - bd;
+                v = (pred[0] + pred[1] + (1 << (shift - 1))) >> shift;
+            }
+        } else if (flags & 3) {
+            const int l = (flags & 1) ? 0 : 1;
+            if (U.weighted) {
+                const int log2wd = U.log2_denom[dn] + 14 - bd;
+                v = log2wd >= 1 ? ((pred[l] * U.weight[l][comp] + (1 << (log2wd - 1))) >> log2wd) + U.offset[l][comp]
+                                : pred[l] * U.weight[l][comp] + U.offset[l][comp];
+            } else {
+                const int shift = 14 - bd;
+                v = (pred[l] + (shift > 0 ? 1 << (shift - 1) : 0)) >> shift;
+            }
+        } else {
+            v = 1 << (bd - 1);  // no usable reference: mid-grey (src/decode.rs:7206)
+        }
+        cur[P.plane_off[pl] + (size_t)py * pw + px] = (T)clampi(v, 0, maxv);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // K2: intra prediction + reconstruction, persistent CTU wavefront
 //
 // Work unit = one (picture, plane, CTB).  Units are handed out by an atomic

@@ -46,6 +46,63 @@ thread_local hipError_t t_launch_err = hipSuccess;
 // ---------------------------------------------------------------------------
 // validation: every index a kernel dereferences is checked here
 // ---------------------------------------------------------------------------
+// Prediction units, reference entries and the TBs of inter CUs of picture p.
+hpvcd_status validate_inter(const hpvcd_batch& b, uint32_t p) {
+    const hpvcd_picture& P = b.pictures[p];
+    if ((uint64_t)P.pu_begin + P.pu_count > b.num_pus) return set_error(HPVCD_E_INVALID, "pu range");
+    if ((uint64_t)P.ref_begin + P.ref_count > b.num_refs) return set_error(HPVCD_E_INVALID, "ref range");
+    for (uint32_t k = 0; k < P.ref_count; k++) {
+        const hpvcd_ref& E = b.refs[P.ref_begin + k];
+        if (E.log2_denom[0] > 7 || E.log2_denom[1] > 7) return set_error(HPVCD_E_INVALID, "weight denominator");
+        if (E.picture == HPVCD_REF_MISSING) continue;
+        if (E.picture >= p) return set_error(HPVCD_E_INVALID, "reference picture must come earlier in the batch");
+        const hpvcd_picture& Q = b.pictures[E.picture];
+        if (Q.width != P.width || Q.height != P.height || Q.chroma_idc != P.chroma_idc ||
+            Q.bit_depth_luma != P.bit_depth_luma || Q.bit_depth_chroma != P.bit_depth_chroma ||
+            ((Q.flags ^ P.flags) & HPVCD_PF_EXTENDED_PRECISION))
+            return set_error(HPVCD_E_INVALID, "reference picture of another size or format");
+    }
+    if (!P.pu_count) {
+        for (uint32_t t = 0; t < P.tb_count; t++)
+            if (b.tbs[P.tb_begin + t].flags & HPVCD_TB_INTER)
+                return set_error(HPVCD_E_INVALID, "inter TB outside every prediction unit");
+        return HPVCD_OK;
+    }
+    const int gw = (P.width + 3) / 4, gh = (P.height + 3) / 4;
+    std::vector<uint8_t> cover((size_t)gw * gh, 0);
+    for (uint32_t i = 0; i < P.pu_count; i++) {
+        const hpvcd_pu& U = b.pus[P.pu_begin + i];
+        if (U.w < 4 || U.h < 4 || U.w > 64 || U.h > 64 || (U.w & 3) || (U.h & 3) || (U.x & 3) || (U.y & 3) ||
+            U.x + U.w > P.width || U.y + U.h > P.height)
+            return set_error(HPVCD_E_INVALID, "prediction unit geometry");
+        if (U.flags & ~(HPVCD_PU_L0 | HPVCD_PU_L1)) return set_error(HPVCD_E_INVALID, "prediction unit flags");
+        for (int l = 0; l < 2; l++)
+            if (((U.flags >> l) & 1) && U.ref[l] >= P.ref_count)
+                return set_error(HPVCD_E_INVALID, "prediction unit reference entry");
+        if ((U.flags & 3) == 3) {
+            const hpvcd_ref& E0 = b.refs[P.ref_begin + U.ref[0]];
+            const hpvcd_ref& E1 = b.refs[P.ref_begin + U.ref[1]];
+            if (E0.weighted != E1.weighted || E0.log2_denom[0] != E1.log2_denom[0] || E0.log2_denom[1] != E1.log2_denom[1])
+                return set_error(HPVCD_E_INVALID, "bi-prediction from entries of different weight tables");
+        }
+        for (int yy = U.y; yy < U.y + U.h; yy += 4)
+            for (int xx = U.x; xx < U.x + U.w; xx += 4) cover[(size_t)(yy / 4) * gw + xx / 4] = 1;
+    }
+    const int sw = sub_w_of(P.chroma_idc), sh = sub_h_of(P.chroma_idc);
+    for (uint32_t t = 0; t < P.tb_count; t++) {
+        const hpvcd_tb& T = b.tbs[P.tb_begin + t];
+        if (!(T.flags & HPVCD_TB_INTER)) continue;
+        if (T.flags & HPVCD_TB_PCM) return set_error(HPVCD_E_INVALID, "PCM TB with residual-coding flags");
+        int n = 1 << T.log2n, x0 = T.comp ? T.x * sw : T.x, y0 = T.comp ? T.y * sh : T.y;
+        int nx = T.comp ? n * sw : n, ny = T.comp ? n * sh : n;
+        for (int yy = y0; yy < std::min(y0 + ny, P.height); yy += 4)
+            for (int xx = x0; xx < std::min(x0 + nx, P.width); xx += 4)
+                if (!cover[(size_t)(yy / 4) * gw + xx / 4])
+                    return set_error(HPVCD_E_INVALID, "inter TB outside every prediction unit");
+    }
+    return HPVCD_OK;
+}
+
 hpvcd_status validate_batch(const hpvcd_batch& b) {
     if (b.num_pictures && !b.pictures) return set_error(HPVCD_E_INVALID, "pictures is null");
     if (b.num_tbs && !b.tbs) return set_error(HPVCD_E_INVALID, "tbs is null");
@@ -54,6 +111,8 @@ hpvcd_status validate_batch(const hpvcd_batch& b) {
     if (b.num_cells && !b.cell_info) return set_error(HPVCD_E_INVALID, "cell_info is null");
     if (b.num_outputs && !b.outputs) return set_error(HPVCD_E_INVALID, "outputs is null");
     if (b.num_scaling && !b.scaling) return set_error(HPVCD_E_INVALID, "scaling is null");
+    if (b.num_pus && !b.pus) return set_error(HPVCD_E_INVALID, "pus is null");
+    if (b.num_refs && !b.refs) return set_error(HPVCD_E_INVALID, "refs is null");
     for (uint32_t o = 0; o < b.num_outputs; o++) {
         const hpvcd_output& O = b.outputs[o];
         if (O.width <= 0 || O.height <= 0 || O.width > 65536 || O.height > 65536)
@@ -140,6 +199,8 @@ hpvcd_status validate_batch(const hpvcd_batch& b) {
                 }
             }
         }
+        hpvcd_status st = validate_inter(b, p);
+        if (st != HPVCD_OK) return st;
     }
     return HPVCD_OK;
 }
@@ -193,6 +254,14 @@ struct ClassJob {
     std::vector<uint32_t> pic_units_begin;            // plain_tickets segment per picture (npics+1)
     std::vector<uint32_t> pic_res_begin[2][4];        // per picture list segments (npics+1)
     std::vector<uint32_t> act_list, pic_act_begin;    // luma TB of each ACT triple; per picture segments
+    std::vector<DevPu> pus;
+    std::vector<DevMcTask> mc_tasks;
+    std::vector<uint32_t> pic_mc_begin;               // MC task segment per picture (npics+1)
+    // dependency waves: pictures are sorted by reference depth; wave w is the
+    // picture range [wave_begin[w], wave_begin[w+1]) and its wavefront units
+    // are tickets [wave_tick_begin[w], wave_tick_begin[w+1])
+    std::vector<uint32_t> wave_begin, wave_tick_begin;
+    std::vector<uint8_t> pic_is_ref;                  // referenced by a later picture: always completes SAO
     size_t plane_elems = 0, res_elems = 0;
     uint32_t max_luma_v = 0, max_luma_h = 0, max_chroma_v = 0, max_chroma_h = 0, max_plane = 0, max_copy = 0;
     // device pointers
@@ -209,6 +278,8 @@ struct ClassJob {
     uint4* d_desc = nullptr;      // wavefront TB descriptors (per unit TB entry)
     uint32_t* d_res_list[2][4] = {};
     uint32_t* d_act_list = nullptr;
+    DevPu* d_pus = nullptr;
+    DevMcTask* d_mc_tasks = nullptr;
     void *d_planes_a = nullptr, *d_planes_b = nullptr, *d_res = nullptr;
     uint32_t* d_sync = nullptr;   // [0] err, [1] border tag, [2..] ticket counters (batched + one per picture)
     uint32_t sync_words = 0;
@@ -235,7 +306,9 @@ struct hpvcd_job {
 namespace {
 
 // Availability stamps, units, levels and residual lists for one class.
-hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<size_t>& out_off) {
+hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<size_t>& out_off,
+                           const std::vector<uint32_t>& depth, const std::vector<uint32_t>& pic_local,
+                           const std::vector<uint8_t>& is_ref) {
     const uint32_t np = (uint32_t)C.batch_pics.size();
     C.pics.resize(np);
     size_t plane_elems = 0;
@@ -291,6 +364,7 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         }
         D.sao_across_slices = (P.flags & HPVCD_PF_LF_ACROSS_SLICES) && all_across;
         D.deblock_on = deb;
+        D.inter = P.pu_count > 0;
         C.max_luma_v = std::max<uint32_t>(C.max_luma_v, (D.w >= 12 ? (D.w - 4) / 8 : 0) * (D.h / 4));
         C.max_luma_h = std::max<uint32_t>(C.max_luma_h, (D.h >= 12 ? (D.h - 4) / 8 : 0) * (D.w / 4));
         C.max_chroma_v = std::max<uint32_t>(C.max_chroma_v, (D.cw >= 10 ? (D.cw - 2) / 8 : 0) * (D.ch / 4));
@@ -430,8 +504,6 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         if (a.pic != b.pic) return a.pic < b.pic;
         return a.plane < b.plane;
     });
-    C.tickets.clear();
-    for (auto& k : sorted) C.tickets.push_back(k.idx);
     // plain tickets: per picture, same order restricted to the picture
     C.plain_tickets.clear();
     for (uint32_t lp = 0; lp < np; lp++)
@@ -455,9 +527,67 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         C.act_list.insert(C.act_list.end(), per_pic_act[lp].begin(), per_pic_act[lp].end());
     }
     C.pic_act_begin[np] = (uint32_t)C.act_list.size();
+    // prediction units and MC blocks (16 x 16 luma at most)
+    C.pus.clear();
+    C.mc_tasks.clear();
+    C.pic_mc_begin.assign(np + 1, 0);
+    C.pic_is_ref.assign(np, 0);
+    for (uint32_t lp = 0; lp < np; lp++) {
+        const uint32_t bp = C.batch_pics[lp];
+        const hpvcd_picture& P = b.pictures[bp];
+        C.pic_is_ref[lp] = is_ref[bp];
+        C.pic_mc_begin[lp] = (uint32_t)C.mc_tasks.size();
+        for (uint32_t i = 0; i < P.pu_count; i++) {
+            const hpvcd_pu& S = b.pus[P.pu_begin + i];
+            DevPu U;
+            std::memset(&U, 0, sizeof(U));
+            U.x = S.x; U.y = S.y; U.w = S.w; U.h = S.h;
+            const hpvcd_ref* e[2] = {nullptr, nullptr};
+            for (int l = 0; l < 2; l++) {
+                if (!((S.flags >> l) & 1)) continue;
+                e[l] = &b.refs[P.ref_begin + S.ref[l]];
+                if (e[l]->picture == HPVCD_REF_MISSING) continue;  // that list reads nothing
+                U.flags |= 1u << l;
+                U.ref[l] = pic_local[e[l]->picture];
+                U.mv[l][0] = S.mv[l][0];
+                U.mv[l][1] = S.mv[l][1];
+                for (int c = 0; c < 3; c++) { U.weight[l][c] = e[l]->weight[c]; U.offset[l][c] = e[l]->offset[c]; }
+            }
+            const hpvcd_ref* w = (U.flags & 1) ? e[0] : ((U.flags & 2) ? e[1] : nullptr);
+            if (w) {
+                U.weighted = w->weighted;
+                U.log2_denom[0] = w->log2_denom[0];
+                U.log2_denom[1] = w->log2_denom[1];
+            }
+            const uint32_t ui = (uint32_t)C.pus.size();
+            C.pus.push_back(U);
+            for (int oy = 0; oy < S.h; oy += kMcBlock)
+                for (int ox = 0; ox < S.w; ox += kMcBlock) C.mc_tasks.push_back({ui, (uint16_t)lp, (uint8_t)ox, (uint8_t)oy});
+        }
+    }
+    C.pic_mc_begin[np] = (uint32_t)C.mc_tasks.size();
+    if (np > 0xffffu && !C.mc_tasks.empty()) return set_error(HPVCD_E_LIMIT, "too many pictures in one format class");
+    // dependency waves (pictures arrive sorted by depth) and their tickets
+    C.wave_begin.clear();
+    std::vector<uint32_t> wave_of(np, 0);
+    for (uint32_t lp = 0; lp < np; lp++) {
+        if (lp == 0 || depth[C.batch_pics[lp]] != depth[C.batch_pics[lp - 1]]) C.wave_begin.push_back(lp);
+        wave_of[lp] = (uint32_t)C.wave_begin.size() - 1;
+    }
+    C.wave_begin.push_back(np);
+    const uint32_t nwaves = (uint32_t)C.wave_begin.size() - 1;
+    C.tickets.clear();
+    C.wave_tick_begin.assign(nwaves + 1, 0);
+    for (uint32_t w = 0; w < nwaves; w++) {
+        C.wave_tick_begin[w] = (uint32_t)C.tickets.size();
+        for (auto& k : sorted)
+            if (wave_of[k.pic] == w) C.tickets.push_back(k.idx);
+    }
+    C.wave_tick_begin[nwaves] = (uint32_t)C.tickets.size();
     C.plane_elems = plane_elems;
     C.res_elems = plane_elems;
-    C.sync_words = 2 + (np + 1);
+    // [0] error, [1] border tag, then one ticket counter per wave and per picture
+    C.sync_words = 2 + nwaves + np;
     return HPVCD_OK;
 }
 
@@ -495,6 +625,7 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
                   std::vector<std::pair<int, hipEvent_t>>* marks) {
     hipStream_t s = g->stream;
     const uint32_t np = (uint32_t)C.pics.size();
+    const uint32_t nwaves = (uint32_t)C.wave_begin.size() - 1;
     T* pa = reinterpret_cast<T*>(C.d_planes_a);
     T* pb = reinterpret_cast<T*>(C.d_planes_b);
     const R* res = reinterpret_cast<const R*>(C.d_res);
@@ -508,15 +639,25 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
     int occ = g->occ_intra[(sizeof(T) == 2 ? 2 : 0) + (sizeof(R) == 4 ? 1 : 0)];
     if (occ < 1) occ = 1;
     const uint32_t max_blocks = (uint32_t)(g->num_cus * occ);
-    k_begin_run<<<cdiv(np + 1, 256), 256, 0, s>>>(C.d_sync, np + 1);
+    const uint32_t ncounters = nwaves + np;  // one ticket counter per wave, then one per picture
+    k_begin_run<<<cdiv(ncounters, 256), 256, 0, s>>>(C.d_sync, ncounters);
     uint32_t* err = C.d_sync;
     const uint32_t* gen = C.d_sync + 1;
     uint32_t* counters = C.d_sync + 2;
     const uint32_t nunits = (uint32_t)C.units.size();
-    const int nsteps = mode == HPVCD_MODE_PLAIN ? (int)np : 1;
+    const bool plain = mode == HPVCD_MODE_PLAIN;
+    // batched: one step per dependency wave (all-intra batches have one);
+    // plain: one step per picture, references first
+    const int nsteps = plain ? (int)np : (int)nwaves;
     for (int step = 0; step < nsteps; step++) {
-        const bool plain = mode == HPVCD_MODE_PLAIN;
-        const uint32_t p0 = plain ? (uint32_t)step : 0, pn = plain ? 1 : np;
+        const uint32_t p0 = plain ? (uint32_t)step : C.wave_begin[step];
+        const uint32_t p1 = plain ? (uint32_t)step + 1 : C.wave_begin[step + 1];
+        const uint32_t pn = p1 - p0;
+        // a step holding a picture that later pictures predict from completes
+        // the loop filters whatever the stop stage: MC reads final samples
+        bool referenced = false;
+        for (uint32_t p = p0; p < p1; p++) referenced |= C.pic_is_ref[p] != 0;
+        const int st = referenced ? std::max(stop, (int)HPVCD_STAGE_SAO) : stop;
         const bool prep = step == 0 && !C.tbs.empty();
         if (prep) {
             // reference availability, reference maps, per-sample reference
@@ -528,33 +669,46 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
                                                          C.d_order, C.d_cells, C.d_desc, C.d_rmap, C.d_smap);
             HIP_TRY_V(hipEventRecord(g->join, g->side));
         }
-        mark(HPVCD_KERNEL_RESIDUAL);
-        for (int pass = 0; pass < 2; pass++)
-            for (int sz = 0; sz < 4; sz++) {
-                uint32_t b0 = plain ? C.pic_res_begin[pass][sz][p0] : 0;
-                uint32_t b1 = plain ? C.pic_res_begin[pass][sz][p0 + 1] : (uint32_t)C.res_list[pass][sz].size();
-                launch_residual<R>(s, C, d_scaling, pass, sz + 2, C.d_res_list[pass][sz] + b0, b1 - b0);
-            }
-        {
-            uint32_t b0 = plain ? C.pic_act_begin[p0] : 0;
-            uint32_t b1 = plain ? C.pic_act_begin[p0 + 1] : (uint32_t)C.act_list.size();
-            if (b1 > b0)
-                k_act<R><<<cdiv(b1 - b0, 4), 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, C.d_act_list + b0, b1 - b0,
+        if (plain || step == 0) {
+            // residuals depend on the records only: the step's picture in
+            // plain mode, every picture of the class at once otherwise
+            mark(HPVCD_KERNEL_RESIDUAL);
+            for (int pass = 0; pass < 2; pass++)
+                for (int sz = 0; sz < 4; sz++) {
+                    uint32_t b0 = plain ? C.pic_res_begin[pass][sz][p0] : 0;
+                    uint32_t b1 = plain ? C.pic_res_begin[pass][sz][p0 + 1] : (uint32_t)C.res_list[pass][sz].size();
+                    launch_residual<R>(s, C, d_scaling, pass, sz + 2, C.d_res_list[pass][sz] + b0, b1 - b0);
+                }
+            uint32_t a0 = plain ? C.pic_act_begin[p0] : 0;
+            uint32_t a1 = plain ? C.pic_act_begin[p0 + 1] : (uint32_t)C.act_list.size();
+            if (a1 > a0)
+                k_act<R><<<cdiv(a1 - a0, 4), 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, C.d_act_list + a0, a1 - a0,
                                                           reinterpret_cast<R*>(C.d_res));
         }
         if (prep) HIP_TRY_V(hipStreamWaitEvent(s, g->join, 0));
+        {
+            // motion compensation of the step's pictures from the final
+            // planes of earlier steps
+            const uint32_t t0 = C.pic_mc_begin[p0], t1 = C.pic_mc_begin[p1];
+            if (t1 > t0) {
+                mark(HPVCD_KERNEL_MC);
+                k_mc<T><<<t1 - t0, 256, 0, s>>>(C.d_pics, C.d_pus, C.d_mc_tasks + t0, pa, pb);
+            }
+        }
         mark(HPVCD_KERNEL_INTRA);
         {
-            const uint32_t* tick = plain ? C.d_plain_tickets + C.pic_units_begin[p0] : C.d_tickets;
-            uint32_t nu = plain ? C.pic_units_begin[p0 + 1] - C.pic_units_begin[p0] : nunits;
-            uint32_t* ctr = counters + (plain ? 1 + p0 : 0);
+            const uint32_t* tick = plain ? C.d_plain_tickets + C.pic_units_begin[p0] : C.d_tickets + C.wave_tick_begin[step];
+            uint32_t nu = plain ? C.pic_units_begin[p0 + 1] - C.pic_units_begin[p0]
+                                : C.wave_tick_begin[step + 1] - C.wave_tick_begin[step];
+            uint32_t* ctr = counters + (plain ? nwaves + p0 : (uint32_t)step);
             uint32_t blocks = std::min<uint32_t>(nu, max_blocks);
             if (blocks)
                 k_intra<T, R><<<blocks, kIntraThreads, 0, s>>>(C.d_pics, C.d_units, tick, nu, C.d_desc, C.d_rmap, C.d_smap, pa,
                                                      res, C.d_border, ctr, gen, err);
         }
         mark(-1);
-        if (stop >= HPVCD_STAGE_DEBLOCK) {
+        (void)nunits;
+        if (st >= HPVCD_STAGE_DEBLOCK) {
             mark(HPVCD_KERNEL_DEBLOCK);
             uint32_t lv = plain ? (uint32_t)((C.pics[p0].w >= 12 ? (C.pics[p0].w - 4) / 8 : 0) * (C.pics[p0].h / 4)) : C.max_luma_v;
             uint32_t lh = plain ? (uint32_t)((C.pics[p0].h >= 12 ? (C.pics[p0].h - 4) / 8 : 0) * (C.pics[p0].w / 4)) : C.max_luma_h;
@@ -566,13 +720,13 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
             if (ch) k_deblock_chroma<T><<<dim3(cdiv(ch, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 0, p0);
             mark(-1);
         }
-        if (stop >= HPVCD_STAGE_SAO) {
+        if (st >= HPVCD_STAGE_SAO) {
             mark(HPVCD_KERNEL_SAO);
             uint32_t mp = plain ? (uint32_t)(C.pics[p0].w * C.pics[p0].h) : C.max_plane;
             k_sao<T><<<dim3(cdiv(mp, 256), pn, 3), 256, 0, s>>>(C.d_pics, C.d_cells, C.d_sao, pa, pb, p0);
             mark(-1);
         }
-        if (stop >= HPVCD_STAGE_RGB) {
+        if (st >= HPVCD_STAGE_RGB) {
             mark(HPVCD_KERNEL_RGB);
             uint32_t mc = plain ? (uint32_t)(C.pics[p0].copy_w * C.pics[p0].copy_h) : C.max_copy;
             if (mc) k_rgb<T><<<dim3(cdiv(mc, 256), pn), 256, 0, s>>>(C.d_pics, pb, d_out, p0);
@@ -662,6 +816,8 @@ void hpvcd_records_view(const hpvcd_records* r, hpvcd_batch* out) {
     out->num_scaling = (uint32_t)(r->scaling.size() / HPVCD_SCALING_SET_BYTES);
     out->scaling = r->scaling.empty() ? nullptr : r->scaling.data();
     out->num_outputs = (uint32_t)r->outputs.size(); out->outputs = r->outputs.data();
+    out->num_pus = (uint32_t)r->pus.size(); out->pus = r->pus.data();
+    out->num_refs = (uint32_t)r->refs.size(); out->refs = r->refs.data();
 }
 
 void hpvcd_records_free(hpvcd_records* r) { delete r; }
@@ -676,8 +832,10 @@ hpvcd_status hpvcd_records_concat(hpvcd_records* const* sets, uint32_t count, hp
         uint32_t tb0 = (uint32_t)r->tbs.size(), co0 = (uint32_t)r->coeffs.size(), ctb0 = (uint32_t)r->sao.size();
         uint32_t cell0 = (uint32_t)r->cells.size(), out0 = (uint32_t)r->outputs.size();
         int32_t sc0 = (int32_t)(r->scaling.size() / HPVCD_SCALING_SET_BYTES);
+        uint32_t pic0 = (uint32_t)r->pictures.size(), pu0 = (uint32_t)r->pus.size(), ref0 = (uint32_t)r->refs.size();
         for (hpvcd_picture p : s->pictures) {
             p.tb_begin += tb0; p.coeff_begin += co0; p.ctb_begin += ctb0; p.cell_begin += cell0;
+            p.pu_begin += pu0; p.ref_begin += ref0;
             p.out_index += out0;
             if (p.scaling_index >= 0) p.scaling_index += sc0;
             r->pictures.push_back(p);
@@ -688,6 +846,11 @@ hpvcd_status hpvcd_records_concat(hpvcd_records* const* sets, uint32_t count, hp
         r->cells.insert(r->cells.end(), s->cells.begin(), s->cells.end());
         r->scaling.insert(r->scaling.end(), s->scaling.begin(), s->scaling.end());
         r->outputs.insert(r->outputs.end(), s->outputs.begin(), s->outputs.end());
+        r->pus.insert(r->pus.end(), s->pus.begin(), s->pus.end());
+        for (hpvcd_ref e : s->refs) {
+            if (e.picture != HPVCD_REF_MISSING) e.picture += pic0;
+            r->refs.push_back(e);
+        }
     }
     *out = r;
     return HPVCD_OK;
@@ -783,13 +946,32 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
             ci = (int)j->classes.size() - 1;
         }
         j->pic_class[p] = (uint32_t)ci;
-        j->pic_local[p] = (uint32_t)j->classes[ci].batch_pics.size();
         j->classes[ci].batch_pics.push_back(p);
+    }
+    // reference depth of every picture (references come earlier in the
+    // batch and share its format class); each class runs its pictures in
+    // order of depth, one wave per depth
+    std::vector<uint32_t> depth(b->num_pictures, 0);
+    std::vector<uint8_t> is_ref(b->num_pictures, 0);
+    for (uint32_t p = 0; p < b->num_pictures; p++) {
+        const hpvcd_picture& P = b->pictures[p];
+        if (!P.pu_count) continue;
+        for (uint32_t k = 0; k < P.ref_count; k++) {
+            uint32_t r = b->refs[P.ref_begin + k].picture;
+            if (r == HPVCD_REF_MISSING) continue;
+            is_ref[r] = 1;
+            depth[p] = std::max(depth[p], depth[r] + 1);
+        }
+    }
+    for (auto& C : j->classes) {
+        std::stable_sort(C.batch_pics.begin(), C.batch_pics.end(),
+                         [&](uint32_t a, uint32_t c) { return depth[a] < depth[c]; });
+        for (uint32_t i = 0; i < C.batch_pics.size(); i++) j->pic_local[C.batch_pics[i]] = i;
     }
     size_t sc_bytes = (size_t)b->num_scaling * HPVCD_SCALING_SET_BYTES;
     size_t total = ob + 256 + ((sc_bytes + 256 + 255) & ~(size_t)255);
     for (auto& C : j->classes) {
-        st = prepare_class(*b, C, j->out_off);
+        st = prepare_class(*b, C, j->out_off, depth, j->pic_local, is_ref);
         if (st != HPVCD_OK) { delete j; return st; }
         size_t sb = C.samp16 ? 2 : 1, rb = C.res32 ? 4 : 2;
         total += bytes_of<DevPic>(C.pics.size()) + bytes_of<hpvcd_tb>(C.tbs.size()) + bytes_of<uint32_t>(C.tb_pic.size()) +
@@ -804,7 +986,8 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
                  2 * ((C.plane_elems * sb + 255) & ~(size_t)255) + ((C.res_elems * rb + 255) & ~(size_t)255) + 8 * 256;
         for (int pass = 0; pass < 2; pass++)
             for (int s = 0; s < 4; s++) total += bytes_of<uint32_t>(C.res_list[pass][s].size() + 1);
-        total += bytes_of<uint32_t>(C.act_list.size() + 1);
+        total += bytes_of<uint32_t>(C.act_list.size() + 1) + bytes_of<DevPu>(C.pus.size() + 1) +
+                 bytes_of<DevMcTask>(C.mc_tasks.size() + 1);
     }
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total > free_b) {
@@ -838,7 +1021,11 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
         for (int pass = 0; pass < 2; pass++)
             for (int s = 0; s < 4; s++)
                 if ((st = upload(C.d_res_list[pass][s], j->mem, C.res_list[pass][s]))) { hpvcd_job_free(j); return st; }
-        if ((st = upload(C.d_act_list, j->mem, C.act_list))) { hpvcd_job_free(j); return st; }
+        if ((st = upload(C.d_act_list, j->mem, C.act_list)) || (st = upload(C.d_pus, j->mem, C.pus)) ||
+            (st = upload(C.d_mc_tasks, j->mem, C.mc_tasks))) {
+            hpvcd_job_free(j);
+            return st;
+        }
         C.d_sync = j->mem.take<uint32_t>(C.sync_words);
         C.d_border = j->mem.take<uint32_t>(C.units.size() * kBorderWords + 1);
         C.d_smap = j->mem.take<uint32_t>(C.smap_words + 1);
@@ -923,7 +1110,9 @@ hpvcd_status hpvcd_job_download_planes(hpvcd_job* j, uint32_t picture, uint16_t*
     if (st) return st;
     ClassJob& C = j->classes[j->pic_class[picture]];
     const DevPic& D = C.pics[j->pic_local[picture]];
-    const char* pool = (const char*)(j->last_stop >= HPVCD_STAGE_SAO ? C.d_planes_b : C.d_planes_a);
+    const uint32_t lp = j->pic_local[picture];
+    const bool final_planes = j->last_stop >= HPVCD_STAGE_SAO || C.pic_is_ref[lp];
+    const char* pool = (const char*)(final_planes ? C.d_planes_b : C.d_planes_a);
     size_t sb = C.samp16 ? 2 : 1;
     uint16_t* dst[3] = {y, cb, cr};
     size_t cnt[3] = {(size_t)D.w * D.h, (size_t)D.cw * D.ch, (size_t)D.cw * D.ch};

@@ -66,10 +66,18 @@ typedef struct hpvcd_gen_config {
     uint8_t  pcm;                       /* pcm_enabled_flag: 8x8..32x32 2Nx2N intra CUs may be I_PCM */
     uint8_t  pcm_bit_depth;             /* PcmBitDepth (luma and chroma), 1..bit_depth */
     uint8_t  pcm_lf_disabled;           /* pcm_loop_filter_disabled_flag */
+    uint8_t  inter;                     /* pictures after the first are inter pictures of one sequence */
+    uint8_t  num_refs;                  /* references per list (1..4): the nearest earlier pictures */
+    uint8_t  weighted;                  /* explicit weighted prediction */
+    uint8_t  amp;                       /* asymmetric motion partitions */
     uint8_t  _pad[2];
     float    p_split_cu, p_nxn, p_split_tu, p_cbf, p_sig, p_level_geom, p_ts, p_tqb;
     float    p_act;                     /* tu_residual_act_flag where the syntax is present */
     float    p_pcm;                     /* pcm_flag where the syntax is present */
+    float    p_intra_cu;                /* intra CUs in inter pictures */
+    float    p_skip;                    /* cu_skip_flag */
+    float    p_bi;                      /* bi-prediction (0: P pictures, list 0 only) */
+    float    mv_sigma;                  /* motion vector standard deviation in luma samples */
     uint32_t config_id;
     uint64_t base_seed;
 } hpvcd_gen_config;
@@ -123,7 +131,9 @@ hpvcd_status hpvcd_gpu_sync(hpvcd_gpu* g);
 /* Copy output images to host buffers (outputs[i] sized by hpvcd_output_bytes). */
 hpvcd_status hpvcd_job_download(hpvcd_job* j, void** outputs);
 /* Copy one picture's planes (u16 samples, coded size) at the stage the last
- * run stopped at; cb/cr may be NULL for 4:0:0. */
+ * run stopped at; cb/cr may be NULL for 4:0:0.  A picture that a later
+ * picture of the batch predicts from always completes SAO (its final samples
+ * are the reference), so its planes are those after SAO whatever the stop. */
 hpvcd_status hpvcd_job_download_planes(hpvcd_job* j, uint32_t picture, uint16_t* y, uint16_t* cb, uint16_t* cr);
 
 /* Timed replay: `iters` runs of the given mode after `warmup` runs; writes
@@ -136,7 +146,8 @@ hpvcd_status hpvcd_job_download_planes(hpvcd_job* j, uint32_t picture, uint16_t*
 #define HPVCD_KERNEL_SAO      3
 #define HPVCD_KERNEL_RGB      4
 #define HPVCD_KERNEL_PREP     5   /* static per-TB preparation for the intra wavefront */
-#define HPVCD_KERNEL_COUNT    6
+#define HPVCD_KERNEL_MC       6   /* motion compensation of inter pictures */
+#define HPVCD_KERNEL_COUNT    7
 hpvcd_status hpvcd_job_time(hpvcd_job* j, int mode, int warmup, int iters, double* ms_per_run,
                             double* kernel_ms);
 

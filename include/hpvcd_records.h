@@ -59,6 +59,15 @@ extern "C" {
  * carries the shift BitDepth - PcmBitDepth, so each sample is value << qp. */
 #define HPVCD_TB_PCM          (1u << 7)
 
+/* ---- prediction-unit flags (hpvcd_pu.flags) ----------------------------- */
+#define HPVCD_PU_L0           (1u << 0)   /* predFlagL0 */
+#define HPVCD_PU_L1           (1u << 1)   /* predFlagL1 */
+
+/* hpvcd_ref.picture of a reference that is not available (the PU reads
+ * nothing from that list; with no list left the prediction is mid-grey,
+ * src/decode.rs:7206) */
+#define HPVCD_REF_MISSING     0xffffffffu
+
 /* ---- per-4x4 luma cell info word (cell_info[]) --------------------------
  * bits 0-1  bS of the vertical edge at the cell's left side (0 when the
  *           cell's left side is not a TU/PU edge: edge_v folded in)
@@ -127,6 +136,10 @@ typedef struct hpvcd_picture {
     uint32_t coeff_begin, coeff_count;
     uint32_t ctb_begin;                /* ceil(w/ctb) * ceil(h/ctb) SAO entries */
     uint32_t cell_begin;               /* ceil(w/4) * ceil(h/4) cell entries */
+    /* inter prediction: the picture's prediction units and the reference
+     * entries they name (all slices' lists, in any order) */
+    uint32_t pu_begin, pu_count;
+    uint32_t ref_begin, ref_count;
 } hpvcd_picture;
 
 /* One transform block of one colour component, in decode order. */
@@ -143,6 +156,33 @@ typedef struct hpvcd_tb {
     uint32_t coeff_cnt;   /* number of nonzero levels */
     uint32_t order;       /* reference-availability stamp */
 } hpvcd_tb;
+
+/* One inter prediction unit (src/decode.rs:7000 motion_compensate): luma
+ * position and size, the lists it predicts from and a quarter-sample luma
+ * motion vector per list.  Its samples of every plane are predicted before
+ * the TBs of its CU add their residuals (those TBs carry HPVCD_TB_INTER);
+ * the TB records still tile every inter CU, skipped ones included. */
+typedef struct hpvcd_pu {
+    uint16_t x, y;        /* luma position in the picture */
+    uint8_t  w, h;        /* luma size: multiples of 4, at most 64 */
+    uint8_t  flags;       /* HPVCD_PU_L0 | HPVCD_PU_L1 */
+    uint8_t  _pad;
+    uint16_t ref[2];      /* reference entry per list, relative to the picture's ref_begin */
+    int16_t  mv[2][2];    /* [list][x, y] in quarter luma samples */
+} hpvcd_pu;
+
+/* One reference-list entry (RefPicList[X][i] with its pred_weight_table
+ * row, src/inter.rs:141).  The picture is an earlier picture of the same
+ * batch with the same size and format: its final (deblocked, SAO) samples
+ * are read in place on the device. */
+typedef struct hpvcd_ref {
+    uint32_t picture;      /* batch picture index, or HPVCD_REF_MISSING */
+    uint8_t  weighted;     /* explicit weighted prediction (weighted_pred_flag / weighted_bipred_flag) */
+    uint8_t  log2_denom[2];/* luma_log2_weight_denom, ChromaLog2WeightDenom */
+    uint8_t  _pad;
+    int16_t  weight[3];    /* Y, Cb, Cr weights */
+    int16_t  offset[3];    /* Y, Cb, Cr offsets in reconstructed-sample units */
+} hpvcd_ref;
 
 /* SAO parameters of one CTB, components [Y, Cb, Cr]; offsets already carry
  * the edge-offset sign convention and log2_sao_offset_scale. */
@@ -183,6 +223,8 @@ typedef struct hpvcd_batch {
     uint32_t num_cells;      const uint32_t*      cell_info;
     uint32_t num_scaling;    const uint8_t*       scaling;   /* num_scaling * HPVCD_SCALING_SET_BYTES */
     uint32_t num_outputs;    const hpvcd_output*  outputs;
+    uint32_t num_pus;        const hpvcd_pu*      pus;
+    uint32_t num_refs;       const hpvcd_ref*     refs;
 } hpvcd_batch;
 
 #ifdef __cplusplus

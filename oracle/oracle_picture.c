@@ -7,6 +7,9 @@
  *       :5007-5504 (do_chroma, reconstruct_chroma, cross-component prediction),
  *       :4356 rext_post_ops, :8418 apply_rext_residual_ops,
  *       src/reconstruct.rs:125,226 (narrow, add_residual_generic)
+ *   motion compensation (before the TBs)   src/decode.rs:7000-7485 motion_compensate,
+ *       src/mc.rs:310 RefPlane::sample, :351 / :442 luma / chroma interpolation,
+ *       :538-733 uni / bi / weighted sample prediction
  *   serial deblocking                      src/decode.rs:1673-2143
  *   serial SAO                             src/decode.rs:2192-2493, src/sao.rs:179-470
  *   YUV -> RGB                             src/yuv.rs:187-668
@@ -32,6 +35,7 @@ typedef struct {
     uint8_t* decoded;        /* per 4x4 luma cell: reconstructed so far */
     const uint32_t* cell;    /* cell info words */
     int32_t ccl[32 * 32];    /* cross_comp_luma (src/decode.rs:279) */
+    uint16_t* const* refp;   /* final planes of earlier pictures: [3 * picture + plane], NULL if absent */
 } pic_ctx;
 
 static int sub_w_of(int idc) { return (idc == 1 || idc == 2) ? 2 : 1; }
@@ -290,6 +294,148 @@ static void recon_tb(pic_ctx* c, const hpvcd_tb* t) {
         }
     }
     add_residual(plane, stride, pw, ph, t->x, t->y, n, pred, res, bd);
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* motion compensation (src/mc.rs, src/decode.rs:7000-7485)                  */
+/* ------------------------------------------------------------------------ */
+/* HEVC luma (quarter-sample) and chroma (eighth-sample) interpolation
+ * filters, src/mc.rs:30-46 (Tables 8-11/8-12 of the standard). */
+static const int LUMA_FILTER[4][8] = {{0, 0, 0, 64, 0, 0, 0, 0},
+                                      {-1, 4, -10, 58, 17, -5, 1, 0},
+                                      {-1, 4, -11, 40, 40, -11, 4, -1},
+                                      {0, 1, -5, 17, 58, -10, 4, -1}};
+static const int CHROMA_FILTER[8][4] = {{0, 64, 0, 0},    {-2, 58, 10, -2}, {-4, 54, 16, -2}, {-6, 46, 28, -4},
+                                        {-4, 36, 36, -4}, {-4, 28, 46, -6}, {-2, 16, 54, -4}, {-2, 10, 58, -2}};
+
+/* src/mc.rs:310 RefPlane::sample: reads clamp to the plane */
+static int ref_sample(const uint16_t* d, int w, int h, int x, int y) {
+    x = x < 0 ? 0 : (x >= w ? w - 1 : x);
+    y = y < 0 ? 0 : (y >= h ? h - 1 : y);
+    return d[(size_t)y * w + x];
+}
+
+/* src/mc.rs:351 luma_interp_scalar_scratch (taps 8, quarter fractions) and
+ * :442 chroma_interp_scalar_scratch (taps 4, eighth fractions): the four
+ * cases of the reference, 14-bit intermediates narrowed to i16. */
+static void mc_interp(const uint16_t* d, int pw, int ph, int x0, int y0, int fx, int fy, int w, int h, int bd,
+                      int taps, const int* hf, const int* vf, int16_t* dst) {
+    int s1 = bd - 8, half = taps / 2 - 1;
+    if (fx == 0 && fy == 0) {
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) dst[y * w + x] = (int16_t)((ref_sample(d, pw, ph, x0 + x, y0 + y) << 6) >> s1);
+        return;
+    }
+    if (fy == 0) {
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                int acc = 0;
+                for (int t = 0; t < taps; t++) acc += hf[t] * ref_sample(d, pw, ph, x0 + x - half + t, y0 + y);
+                dst[y * w + x] = (int16_t)(acc >> s1);
+            }
+        return;
+    }
+    if (fx == 0) {
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                int acc = 0;
+                for (int t = 0; t < taps; t++) acc += vf[t] * ref_sample(d, pw, ph, x0 + x, y0 + y - half + t);
+                dst[y * w + x] = (int16_t)(acc >> s1);
+            }
+        return;
+    }
+    int th = h + taps - 1;
+    int32_t* tmp = malloc(sizeof(int32_t) * (size_t)w * th);
+    for (int ty = 0; ty < th; ty++)
+        for (int x = 0; x < w; x++) {
+            int acc = 0;
+            for (int t = 0; t < taps; t++) acc += hf[t] * ref_sample(d, pw, ph, x0 + x - half + t, y0 + ty - half);
+            tmp[ty * w + x] = acc >> s1;
+        }
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int acc = 0;
+            for (int t = 0; t < taps; t++) acc += vf[t] * tmp[(y + t) * w + x];
+            dst[y * w + x] = (int16_t)(acc >> 6);
+        }
+    free(tmp);
+}
+
+/* src/mc.rs:538-733: uni / bi / weighted uni / weighted bi into the valid
+ * region of the plane; neither list usable gives mid-grey (src/decode.rs:7206) */
+static void mc_write(uint16_t* plane, int stride, int vw, int vh, int w, const int16_t* p0, const int16_t* p1,
+                     int use0, int use1, const hpvcd_ref* e0, const hpvcd_ref* e1, int comp, int bd) {
+    int max = (1 << bd) - 1;
+    int dn = comp ? 1 : 0;
+    for (int y = 0; y < vh; y++)
+        for (int x = 0; x < vw; x++) {
+            int i = y * w + x, v;
+            if (use0 && use1) {
+                if (e0->weighted)
+                    v = oracle_weighted_bi(p0[i], p1[i], e0->weight[comp], e1->weight[comp], e0->offset[comp],
+                                           e1->offset[comp], e0->log2_denom[dn], bd);
+                else {
+                    int sh = 15 - bd;
+                    v = clamp32((p0[i] + p1[i] + (1 << (sh - 1))) >> sh, 0, max);
+                }
+            } else if (use0 || use1) {
+                const hpvcd_ref* e = use0 ? e0 : e1;
+                int s = use0 ? p0[i] : p1[i];
+                if (e->weighted)
+                    v = oracle_weighted_uni(s, e->weight[comp], e->offset[comp], e->log2_denom[dn], bd);
+                else {
+                    int sh = 14 - bd;
+                    v = clamp32((s + (sh > 0 ? 1 << (sh - 1) : 0)) >> sh, 0, max);
+                }
+            } else {
+                v = 1 << (bd - 1);
+            }
+            plane[(size_t)y * stride + x] = (uint16_t)v;
+        }
+}
+
+/* src/decode.rs:7000 motion_compensate of one PU: luma, then Cb and Cr with
+ * the chroma vector mv * 2 / SubWidthC (eighth samples at 4:2:0). */
+static void mc_pu(pic_ctx* c, const hpvcd_pu* pu) {
+    const hpvcd_ref* e[2] = {NULL, NULL};
+    int use[2] = {0, 0};
+    for (int l = 0; l < 2; l++) {
+        if (!((pu->flags >> l) & 1)) continue;
+        e[l] = &c->b->refs[c->P->ref_begin + pu->ref[l]];
+        use[l] = e[l]->picture != HPVCD_REF_MISSING && c->refp && c->refp[3 * e[l]->picture] != NULL;
+    }
+    int w = pu->w, h = pu->h;
+    int16_t* pred[2] = {malloc(sizeof(int16_t) * w * h), malloc(sizeof(int16_t) * w * h)};
+    for (int l = 0; l < 2; l++)
+        if (use[l]) {
+            int mx = pu->mv[l][0], my = pu->mv[l][1];
+            mc_interp(c->refp[3 * e[l]->picture], c->w, c->h, pu->x + (mx >> 2), pu->y + (my >> 2), mx & 3, my & 3, w,
+                      h, c->bd, 8, LUMA_FILTER[mx & 3], LUMA_FILTER[my & 3], pred[l]);
+        }
+    if (pu->x < c->w && pu->y < c->h) {
+        int vw = c->w - pu->x < w ? c->w - pu->x : w, vh = c->h - pu->y < h ? c->h - pu->y : h;
+        mc_write(c->y + (size_t)pu->y * c->w + pu->x, c->w, vw, vh, w, pred[0], pred[1], use[0], use[1], e[0], e[1],
+                 0, c->bd);
+    }
+    if (c->P->chroma_idc) {
+        int cw = w / c->sub_w, ch = h / c->sub_h, cx = pu->x / c->sub_w, cy = pu->y / c->sub_h;
+        for (int pl = 1; pl <= 2; pl++) {
+            for (int l = 0; l < 2; l++)
+                if (use[l]) {
+                    int mx = pu->mv[l][0] * 2 / c->sub_w, my = pu->mv[l][1] * 2 / c->sub_h;
+                    mc_interp(c->refp[3 * e[l]->picture + pl], c->cw, c->ch, cx + (mx >> 3), cy + (my >> 3), mx & 7,
+                              my & 7, cw, ch, c->bd_c, 4, CHROMA_FILTER[mx & 7], CHROMA_FILTER[my & 7], pred[l]);
+                }
+            if (cx >= c->cw || cy >= c->ch) continue;
+            int vw = c->cw - cx < cw ? c->cw - cx : cw, vh = c->ch - cy < ch ? c->ch - cy : ch;
+            uint16_t* plane = pl == 1 ? c->cb : c->cr;
+            mc_write(plane + (size_t)cy * c->cw + cx, c->cw, vw, vh, cw, pred[0], pred[1], use[0], use[1], e[0], e[1],
+                     pl, c->bd_c);
+        }
+    }
+    free(pred[0]);
+    free(pred[1]);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -635,7 +781,9 @@ static void apply_sao(pic_ctx* c) {
 /* ------------------------------------------------------------------------ */
 /* picture driver                                                           */
 /* ------------------------------------------------------------------------ */
-int oracle_picture_planes(const hpvcd_batch* b, uint32_t p, int stage, uint16_t* y, uint16_t* cb, uint16_t* cr) {
+/* Decode picture p with the final planes of its references in refp. */
+static int decode_planes(const hpvcd_batch* b, uint32_t p, int stage, uint16_t* y, uint16_t* cb, uint16_t* cr,
+                         uint16_t* const* refp) {
     if (p >= b->num_pictures) return -1;
     pic_ctx* c = calloc(1, sizeof(pic_ctx));
     const hpvcd_picture* P = &b->pictures[p];
@@ -649,9 +797,13 @@ int oracle_picture_planes(const hpvcd_batch* b, uint32_t p, int stage, uint16_t*
     c->ep = (P->flags & HPVCD_PF_EXTENDED_PRECISION) != 0;
     c->y = y; c->cb = cb; c->cr = cr;
     c->cell = b->cell_info + P->cell_begin;
+    c->refp = refp;
     c->decoded = calloc((size_t)c->gw * c->gh, 1);
     memset(y, 0, sizeof(uint16_t) * c->w * c->h);
     if (c->cw) { memset(cb, 0, sizeof(uint16_t) * c->cw * c->ch); memset(cr, 0, sizeof(uint16_t) * c->cw * c->ch); }
+    /* inter prediction reads only other pictures, so every PU is predicted
+     * before the TBs that add residuals on top */
+    for (uint32_t i = 0; i < P->pu_count; i++) mc_pu(c, &b->pus[P->pu_begin + i]);
     for (uint32_t i = 0; i < P->tb_count; i++) {
         const hpvcd_tb* t = &b->tbs[P->tb_begin + i];
         if ((t->flags & HPVCD_TB_ACT) && t->comp == 0 && i + 2 < P->tb_count) {
@@ -669,6 +821,44 @@ int oracle_picture_planes(const hpvcd_batch* b, uint32_t p, int stage, uint16_t*
     free(c->decoded);
     free(c);
     return 0;
+}
+
+static size_t chroma_elems(const hpvcd_picture* P) {
+    int sw = sub_w_of(P->chroma_idc), sh = sub_h_of(P->chroma_idc);
+    return P->chroma_idc ? (size_t)((P->width + sw - 1) / sw) * ((P->height + sh - 1) / sh) : 0;
+}
+
+static void mark_refs(const hpvcd_batch* b, uint32_t p, uint8_t* need) {
+    const hpvcd_picture* P = &b->pictures[p];
+    for (uint32_t k = 0; k < P->ref_count; k++) {
+        uint32_t r = b->refs[P->ref_begin + k].picture;
+        if (r != HPVCD_REF_MISSING && r < p) need[r] = 1;
+    }
+}
+
+int oracle_picture_planes(const hpvcd_batch* b, uint32_t p, int stage, uint16_t* y, uint16_t* cb, uint16_t* cr) {
+    if (p >= b->num_pictures) return -1;
+    /* the references' own references, transitively; decoded in batch order */
+    uint8_t* need = calloc(p + 1, 1);
+    mark_refs(b, p, need);
+    for (uint32_t q = p; q-- > 0;)
+        if (need[q]) mark_refs(b, q, need);
+    uint16_t** refp = calloc(3 * (size_t)b->num_pictures + 3, sizeof(uint16_t*));
+    int rc = 0;
+    for (uint32_t q = 0; q < p && !rc; q++) {
+        if (!need[q]) continue;
+        const hpvcd_picture* Q = &b->pictures[q];
+        size_t nc = chroma_elems(Q);
+        refp[3 * q] = malloc(sizeof(uint16_t) * Q->width * Q->height);
+        refp[3 * q + 1] = malloc(sizeof(uint16_t) * nc + 2);
+        refp[3 * q + 2] = malloc(sizeof(uint16_t) * nc + 2);
+        rc = decode_planes(b, q, ORACLE_STAGE_SAO, refp[3 * q], refp[3 * q + 1], refp[3 * q + 2], refp);
+    }
+    if (!rc) rc = decode_planes(b, p, stage, y, cb, cr, refp);
+    for (size_t k = 0; k < 3 * (size_t)b->num_pictures; k++) free(refp[k]);
+    free(refp);
+    free(need);
+    return rc;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -766,22 +956,23 @@ typedef struct {
     int njobs;
     int* job_first;      /* pictures per job: [job_first[j], job_first[j+1]) in job_pics order */
     uint32_t* job_pics;
+    uint16_t** refp;     /* final planes kept for later pictures: [3 * picture + plane] */
+    const uint8_t* keep; /* pictures referenced by a later picture */
     int err;
 } batch_job;
 
-static int decode_one(const hpvcd_batch* b, uint32_t p, void** outputs) {
+static int decode_one(const hpvcd_batch* b, uint32_t p, void** outputs, uint16_t** refp, int keep) {
     const hpvcd_picture* P = &b->pictures[p];
     const hpvcd_output* O = &b->outputs[P->out_index];
-    int sw = sub_w_of(P->chroma_idc), sh = sub_h_of(P->chroma_idc);
     size_t ny = (size_t)P->width * P->height;
-    size_t nc = P->chroma_idc ? (size_t)((P->width + sw - 1) / sw) * ((P->height + sh - 1) / sh) : 0;
+    size_t nc = chroma_elems(P);
     uint16_t* y = malloc(ny * 2);
     uint16_t* cb = malloc(nc * 2 + 2);
     uint16_t* cr = malloc(nc * 2 + 2);
     int chn = P->chroma_idc ? 3 : 1;
     int bps = (O->kind == HPVCD_OUT_RGB16 || O->kind == HPVCD_OUT_LUMA16) ? 2 : 1;
     uint8_t* win = malloc((size_t)P->win_w * P->win_h * chn * bps + 1);
-    int rc = oracle_picture_planes(b, p, ORACLE_STAGE_SAO, y, cb, cr);
+    int rc = decode_planes(b, p, ORACLE_STAGE_SAO, y, cb, cr, refp);
     if (!rc) rc = oracle_picture_rgb(b, p, y, cb, cr, win);
     if (!rc) {
         uint8_t* dst = (uint8_t*)outputs[P->out_index];
@@ -790,7 +981,12 @@ static int decode_one(const hpvcd_batch* b, uint32_t p, void** outputs) {
             memcpy(dst + (((size_t)(P->dst_y + r) * O->width + P->dst_x) * chn) * bps,
                    win + (size_t)r * P->win_w * chn * bps, row);
     }
-    free(y); free(cb); free(cr); free(win);
+    if (keep && !rc) {
+        refp[3 * p] = y; refp[3 * p + 1] = cb; refp[3 * p + 2] = cr;
+    } else {
+        free(y); free(cb); free(cr);
+    }
+    free(win);
     return rc;
 }
 
@@ -801,15 +997,19 @@ static void* batch_worker(void* arg) {
         int j = J->next++;
         pthread_mutex_unlock(&J->mu);
         if (j >= J->njobs) break;
-        for (int k = J->job_first[j]; k < J->job_first[j + 1]; k++)
-            if (decode_one(J->b, J->job_pics[k], J->outputs)) J->err = 1;
+        for (int k = J->job_first[j]; k < J->job_first[j + 1]; k++) {
+            uint32_t p = J->job_pics[k];
+            if (decode_one(J->b, p, J->outputs, J->refp, J->keep[p])) J->err = 1;
+        }
     }
     return NULL;
 }
 
 int oracle_decode_batch(const hpvcd_batch* b, void** outputs, int threads, int rows_parallel) {
     /* src/lib.rs:1680 fill_grid_bands: one job per grid row (pictures sharing
-     * dst_y of one output), tiles of a row serial; or one job per picture. */
+     * dst_y of one output), tiles of a row serial; or one job per picture.
+     * Inter pictures wait for their references: the batch runs in waves of
+     * equal reference depth, each wave's pictures in parallel. */
     batch_job J;
     memset(&J, 0, sizeof(J));
     J.b = b; J.outputs = outputs;
@@ -817,24 +1017,43 @@ int oracle_decode_batch(const hpvcd_batch* b, void** outputs, int threads, int r
     uint32_t n = b->num_pictures;
     J.job_pics = malloc(sizeof(uint32_t) * (n + 1));
     J.job_first = malloc(sizeof(int) * (n + 2));
-    int nj = 0, k = 0;
-    uint8_t* used = calloc(n + 1, 1);
+    J.refp = calloc(3 * (size_t)n + 3, sizeof(uint16_t*));
+    uint8_t* keep = calloc(n + 1, 1);
+    uint32_t* depth = calloc(n + 1, sizeof(uint32_t));
+    uint32_t max_depth = 0;
     for (uint32_t p = 0; p < n; p++) {
-        if (used[p]) continue;
-        J.job_first[nj++] = k;
-        J.job_pics[k++] = p; used[p] = 1;
-        if (rows_parallel)
-            for (uint32_t q = p + 1; q < n; q++)
-                if (!used[q] && b->pictures[q].out_index == b->pictures[p].out_index &&
-                    b->pictures[q].dst_y == b->pictures[p].dst_y) { J.job_pics[k++] = q; used[q] = 1; }
+        const hpvcd_picture* P = &b->pictures[p];
+        for (uint32_t k = 0; k < P->ref_count; k++) {
+            uint32_t r = b->refs[P->ref_begin + k].picture;
+            if (r == HPVCD_REF_MISSING || r >= p) continue;
+            keep[r] = 1;
+            if (depth[r] + 1 > depth[p]) depth[p] = depth[r] + 1;
+        }
+        if (depth[p] > max_depth) max_depth = depth[p];
     }
-    J.job_first[nj] = k;
-    J.njobs = nj;
+    J.keep = keep;
     if (threads < 1) threads = 1;
     pthread_t* th = malloc(sizeof(pthread_t) * threads);
-    for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, batch_worker, &J);
-    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
-    free(th); free(used); free(J.job_pics); free(J.job_first);
+    uint8_t* used = calloc(n + 1, 1);
+    for (uint32_t d = 0; d <= max_depth; d++) {
+        int nj = 0, k = 0;
+        for (uint32_t p = 0; p < n; p++) {
+            if (used[p] || depth[p] != d) continue;
+            J.job_first[nj++] = k;
+            J.job_pics[k++] = p; used[p] = 1;
+            if (rows_parallel)
+                for (uint32_t q = p + 1; q < n; q++)
+                    if (!used[q] && depth[q] == d && b->pictures[q].out_index == b->pictures[p].out_index &&
+                        b->pictures[q].dst_y == b->pictures[p].dst_y) { J.job_pics[k++] = q; used[q] = 1; }
+        }
+        J.job_first[nj] = k;
+        J.njobs = nj;
+        J.next = 0;
+        for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, batch_worker, &J);
+        for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    }
+    for (size_t q = 0; q < 3 * (size_t)n; q++) free(J.refp[q]);
+    free(th); free(used); free(J.job_pics); free(J.job_first); free(J.refp); free(keep); free(depth);
     pthread_mutex_destroy(&J.mu);
     return J.err ? -1 : 0;
 }

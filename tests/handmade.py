@@ -122,3 +122,75 @@ def pcm_case(seed=0, chroma_idc=1, bd=8, pcm_bits=5, log2_cu=4, size=32):
                                  cells(size, size, 30, N.CELL_PCM),
                                  [output(size, size, chroma_idc, N.OUT_RGB16 if bd > 8 else N.OUT_RGB8)])
     return rs, planes
+
+
+def inter_case(ref_luma, ref_cb=None, ref_cr=None, mvs=((0, 0),), bi=False, weights=None, pu=16):
+    """Two 8-bit pictures: picture 0 is I_PCM carrying the given planes
+    exactly (PcmBitDepth 8); picture 1 is tiled by pu x pu inter PUs (the i-th
+    takes mvs[i % len(mvs)], list 0, and list 1 too when bi) whose CUs code
+    no residual, with the loop filters off, so its reconstruction is the
+    motion-compensated prediction.  weights: None, or ((w, o) per Y/Cb/Cr for
+    list 0, the same for list 1) with log2 denominators 6.
+    Returns the record set (outputs: one image per picture)."""
+    ref_luma = np.asarray(ref_luma)
+    h, w = ref_luma.shape
+    chroma = ref_cb is not None
+    cidc = 1 if chroma else 0
+    tbs, words, off = [], [], 0
+    order = [(bx + x, by + y) for by in range(0, h, 16) for bx in range(0, w, 16)
+             for y in range(0, 16, 16) for x in range(0, 16, 16)]
+    for cx, cy in order:  # picture 0: 16x16 PCM CUs
+        for comp, plane in enumerate([ref_luma] + ([ref_cb, ref_cr] if chroma else [])):
+            m = 16 if comp == 0 else 8
+            x, y = (cx, cy) if comp == 0 else (cx // 2, cy // 2)
+            words.append(dense_levels(np.asarray(plane)[y:y + m, x:x + m]))
+            tbs.append(tb(x, y, 4 if comp == 0 else 3, comp, 1, 0, N.TB_PCM, off, m * m))
+            off += m * m
+    p0 = picture(w, h, cidc, 8)
+    p0.tb_count, p0.coeff_count = len(tbs), off
+    t1 = []
+    for cx, cy in order:  # picture 1: inter TBs without residual
+        t1.append(tb(cx, cy, 4, 0, 1, 30, N.TB_INTER))
+        if chroma:
+            for comp in (1, 2):
+                t1.append(tb(cx // 2, cy // 2, 3, comp, 1, 30, N.TB_INTER))
+    pus = []
+    k = 0
+    for y in range(0, h, pu):
+        for x in range(0, w, pu):
+            mv = mvs[k % len(mvs)]
+            k += 1
+            u = N.PU(x=x, y=y, w=pu, h=pu, flags=N.PU_L0 | (N.PU_L1 if bi else 0))
+            u.ref[0], u.ref[1] = 0, 1 if bi else 0
+            for l in range(2):
+                u.mv[l][0], u.mv[l][1] = mv
+            pus.append(u)
+    refs = []
+    for l in range(2 if bi else 1):
+        e = N.Ref(picture=0, weighted=1 if weights else 0)
+        e.log2_denom[0] = e.log2_denom[1] = 6
+        for c in range(3):
+            e.weight[c], e.offset[c] = weights[l][c] if weights else (64, 0)
+        refs.append(e)
+    p1 = picture(w, h, cidc, 8)
+    p1.tb_begin, p1.tb_count = len(tbs), len(t1)
+    p1.coeff_begin, p1.coeff_count = off, 0
+    p1.ctb_begin = ((w + 15) // 16) * ((h + 15) // 16)
+    p1.cell_begin = ((w + 3) // 4) * ((h + 3) // 4)
+    p1.pu_begin, p1.pu_count, p1.ref_begin, p1.ref_count = 0, len(pus), 0, len(refs)
+    p1.out_index = 1
+    c0 = cells(w, h, 30, N.CELL_PCM)
+    c1 = np.full_like(c0, (30 & 0xff) << 8)
+    rs = H.RecordSet.from_arrays([p0, p1], np.array(tbs + t1, dtype=H._TB_DTYPE), np.concatenate(words),
+                                 np.concatenate([c0, c1]), [output(w, h, cidc), output(w, h, cidc)],
+                                 pus=pus, refs=refs)
+    return rs
+
+
+def shifted(plane, dx, dy):
+    """plane sampled at (x + dx, y + dy) with edge clamping (src/mc.rs:310)."""
+    plane = np.asarray(plane)
+    h, w = plane.shape
+    ys = np.clip(np.arange(h) + dy, 0, h - 1)
+    xs = np.clip(np.arange(w) + dx, 0, w - 1)
+    return plane[np.ix_(ys, xs)]

@@ -4,6 +4,8 @@ Every Y/Cb/Cr sample after each stage (reconstruction, deblocking, SAO) and
 every output RGB sample must be bit-identical (integer pipeline; no
 tolerance).  Runs through the C ABI (libhpvcd.so).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -49,6 +51,17 @@ SMALL = {
     # I_PCM CUs in 4:4:4 10-bit, loop-filtered
     "pcm444_10": dict(workload="rext4096", grid_cols=1, grid_rows=1, out_width=512, out_height=512, pcm=1,
                       p_pcm=0.15, pcm_bit_depth=7, base_seed=49),
+    # inter pictures: 1 I + 4 low-delay B, 4 references, explicit weighted prediction, AMP
+    "inter420_wp": dict(workload="single512", tile_width=256, tile_height=128, grid_cols=1, grid_rows=5,
+                        num_outputs=5, out_width=256, out_height=128, inter=1, weighted=1, base_seed=61),
+    # P pictures without weights, larger motion
+    "inter420_p": dict(workload="single512", tile_width=192, tile_height=136, grid_cols=1, grid_rows=4,
+                       num_outputs=4, out_width=192, out_height=136, inter=1, p_bi=0.0, mv_sigma=40.0,
+                       base_seed=63),
+    # inter 4:4:4 10-bit: TS, explicit RDPCM, CCP and ACT on inter TUs
+    "inter444_10": dict(workload="rext4096", tile_width=256, tile_height=128, grid_cols=1, grid_rows=3,
+                        num_outputs=3, out_width=256, out_height=128, inter=1, weighted=1, act=1, p_act=0.4,
+                        base_seed=65),
 }
 
 
@@ -57,9 +70,22 @@ def _first_diff(a, b):
     return tuple(idx[0]) if len(idx) else None
 
 
-def _check_planes(job, rs, stage):
+def _referenced(rs):
+    refs = set()
     for p in range(rs.num_pictures):
-        want = O.picture_planes(rs.batch, p, stage)
+        P = rs.picture(p)
+        for k in range(P.ref_count):
+            r = rs.batch.refs[P.ref_begin + k].picture
+            if r != N.REF_MISSING:
+                refs.add(r)
+    return refs
+
+
+def _check_planes(job, rs, stage):
+    # a picture that later pictures predict from always completes SAO
+    refd = _referenced(rs)
+    for p in range(rs.num_pictures):
+        want = O.picture_planes(rs.batch, p, N.STAGE_SAO if p in refd else stage)
         got = job.download_planes(p)
         for name, w, g in zip("YUV", want, got):
             if w is None:
@@ -72,6 +98,7 @@ def _check_planes(job, rs, stage):
 @pytest.mark.parametrize("case", sorted(SMALL))
 def test_stages_bit_exact(gpu, case):
     rs = H.RecordSet.generate(**SMALL[case])
+    print(f"case {case}: {rs.num_pictures} pictures, {rs.batch.num_tbs} TBs, {rs.batch.num_pus} PUs", flush=True)
     with gpu.job(rs) as job:
         for stage in (N.STAGE_RECON, N.STAGE_DEBLOCK, N.STAGE_SAO):
             job.run(N.MODE_BATCHED, stage)
@@ -130,6 +157,66 @@ def test_pcm_closed_form(gpu, chroma_idc, log2_cu, bd, bits):
     for g, w in zip(got, want):
         if w is not None:
             assert np.array_equal(g, w)
+
+
+def _mc_case_planes(gpu, rs):
+    with gpu.job(rs) as job:
+        job.run(N.MODE_BATCHED, N.STAGE_RECON)
+        return job.download_planes(1)
+
+
+FULL_MVS = [(8, 16), (-24, 8), (40, -16), (-200, 96), (0, 0), (336, -400), (16, 8), (-8, -8)]
+
+
+@pytest.mark.parametrize("bi", [False, True])
+def test_mc_full_sample_copy(gpu, bi):
+    import test_oracle_kat as K
+
+    rs = M.inter_case(K.MC_Y, K.MC_CB, K.MC_CR, mvs=FULL_MVS, bi=bi)
+    for g, w in zip(_mc_case_planes(gpu, rs), K.mc_expected_full(K.MC_Y, K.MC_CB, K.MC_CR, FULL_MVS)):
+        assert np.array_equal(g, w)
+
+
+def test_mc_fractional_against_oracle(gpu):
+    """Random fractional vectors (uni, bi, weighted) on random references, 8x8 PUs."""
+    rng = np.random.default_rng(5)
+    y, cb, cr = rng.integers(0, 256, (32, 64)), rng.integers(0, 256, (16, 32)), rng.integers(0, 256, (16, 32))
+    mvs = [(int(a), int(b)) for a, b in rng.integers(-300, 300, (32, 2))]
+    wts = (((40, 9), (90, -20), (64, 100)), ((80, -4), (33, 7), (64, 30)))
+    for bi in (False, True):
+        for weights in (None, wts):
+            rs = M.inter_case(y, cb, cr, mvs=mvs, bi=bi, weights=weights, pu=8)
+            got = _mc_case_planes(gpu, rs)
+            want = O.picture_planes(rs.batch, 1, N.STAGE_RECON)
+            for g, w in zip(got, want):
+                assert np.array_equal(g, w), f"bi={bi} weighted={weights is not None}"
+
+
+def test_inter_missing_references(gpu):
+    """Entries without a picture: a bi PU falls back to its other list, a PU
+    with no usable list predicts mid-grey (src/decode.rs:7206)."""
+    rs = H.RecordSet.generate(**dict(SMALL["inter420_wp"], grid_rows=3, num_outputs=3))
+    refs = (N.Ref * rs.batch.num_refs).from_address(C.addressof(rs.batch.refs.contents))
+    for k in range(0, rs.batch.num_refs, 3):
+        refs[k].picture = N.REF_MISSING
+    assert N.lib().hpvcd_validate_batch(C.byref(rs.batch)) == 0
+    want = rs.output_buffers()
+    O.decode_batch(rs.batch, want, threads=4)
+    got = gpu.decode(rs)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w), f"{int(np.count_nonzero(g != w))} samples differ"
+
+
+@pytest.mark.parametrize("mode", [N.MODE_PLAIN, N.MODE_GRAPH])
+def test_modes_inter(gpu, mode):
+    rs = H.RecordSet.generate(**dict(SMALL["inter420_wp"], base_seed=67))
+    want = rs.output_buffers()
+    O.decode_batch(rs.batch, want, threads=8)
+    with gpu.job(rs) as job:
+        for _ in range(2):
+            job.run(mode)
+            for g, w in zip(job.download(), want):
+                assert np.array_equal(g, w), f"{int(np.count_nonzero(g != w))} samples differ"
 
 
 def test_mixed_formats_one_batch(gpu):

@@ -39,9 +39,11 @@ def test_struct_layouts_match_c():
     assert C.sizeof(N.TB) == 24
     assert C.sizeof(N.Sao) == 36
     assert C.sizeof(N.Output) == 12
-    assert C.sizeof(N.Picture) == 124
-    assert C.sizeof(N.Batch) == 112
-    assert C.sizeof(N.GenConfig) == 104
+    assert C.sizeof(N.Picture) == 140
+    assert C.sizeof(N.Batch) == 144
+    assert C.sizeof(N.PU) == 20
+    assert C.sizeof(N.Ref) == 20
+    assert C.sizeof(N.GenConfig) == 128
 
 
 def test_generator_is_deterministic_and_seeded():

@@ -5,7 +5,8 @@ Three kinds of checks (none of them retypes the oracle's own code):
     from the reference source by tests/golden/make_tables_fixture.py);
   * the reference's own known-answer tests, restated: src/decode.rs:8247-8313
     (QP), src/mc.rs:737-758 (weighted MC), src/act.rs:95-148 (ACT),
-    src/decode.rs:2768 (I_PCM samples),
+    src/decode.rs:2768 (I_PCM samples), src/mc.rs:310-733 (motion compensation:
+    full-sample copies, flat references, half-sample ramps, identity weights),
     src/yuv.rs:670-756 (YUV->RGB), src/intra.rs:374-448 (smoothing off),
     src/transform.rs:1388-1439 (extended precision);
   * closed forms worked out by hand from the H.265 equations and invariants
@@ -405,6 +406,113 @@ def test_pcm_loop_filter_disabled_exempts_pcm_samples():
         m = np.kron(pcm, np.ones((4 if k == 0 else 2,) * 2, bool))
         assert m.any() and np.array_equal(a[m], b[m])
         assert np.count_nonzero(a[~m] != b[~m]) > 1000
+
+
+# --------------------------------------------------------------------------------------
+# motion compensation closed forms (src/mc.rs:310-733, src/decode.rs:7000-7485)
+# --------------------------------------------------------------------------------------
+MC_RNG = np.random.default_rng(77)
+MC_Y = MC_RNG.integers(0, 256, (32, 64))
+MC_CB = MC_RNG.integers(0, 256, (16, 32))
+MC_CR = MC_RNG.integers(0, 256, (16, 32))
+FULL_MVS = [(8, 16), (-24, 8), (40, -16), (-200, 96), (0, 0), (336, -400), (16, 8), (-8, -8)]
+
+
+def mc_expected_full(y, cb, cr, mvs, pu=16):
+    """Full-sample vectors (multiples of 8 quarter samples): every PU copies
+    its reference block displaced by mv/4 (luma) and mv/8 (4:2:0 chroma),
+    reads clamped to the picture."""
+    import handmade as M
+
+    outs = [np.zeros_like(y), np.zeros_like(cb), np.zeros_like(cr)]
+    k = 0
+    for py in range(0, y.shape[0], pu):
+        for px in range(0, y.shape[1], pu):
+            mx, my = mvs[k % len(mvs)]
+            k += 1
+            outs[0][py:py + pu, px:px + pu] = M.shifted(y, mx // 4, my // 4)[py:py + pu, px:px + pu]
+            for o, src in ((outs[1], cb), (outs[2], cr)):
+                o[py // 2:(py + pu) // 2, px // 2:(px + pu) // 2] = \
+                    M.shifted(src, mx // 8, my // 8)[py // 2:(py + pu) // 2, px // 2:(px + pu) // 2]
+    return outs
+
+
+@pytest.mark.parametrize("bi", [False, True])
+def test_mc_full_sample_vectors_copy_the_reference(bi):
+    import handmade as M
+    from hpvcd_amd import _native as N
+
+    rs = M.inter_case(MC_Y, MC_CB, MC_CR, mvs=FULL_MVS, bi=bi)
+    got = O.picture_planes(rs.batch, 1, N.STAGE_RECON)
+    for g, w in zip(got, mc_expected_full(MC_Y, MC_CB, MC_CR, FULL_MVS)):
+        assert np.array_equal(g, w)
+
+
+def test_mc_constant_reference_any_fraction():
+    """Every interpolation filter sums to 64: a flat reference stays flat for
+    any fractional vector, uni or bi."""
+    import handmade as M
+    from hpvcd_amd import _native as N
+
+    mvs = [(int(a), int(b)) for a, b in MC_RNG.integers(-300, 300, (8, 2))]
+    for bi in (False, True):
+        rs = M.inter_case(np.full((32, 64), 97), np.full((16, 32), 31), np.full((16, 32), 200), mvs=mvs, bi=bi, pu=8)
+        got = O.picture_planes(rs.batch, 1, N.STAGE_RECON)
+        assert np.all(got[0] == 97) and np.all(got[1] == 31) and np.all(got[2] == 200)
+
+
+def test_mc_half_sample_of_a_ramp_is_exact():
+    """The symmetric half-sample filter reproduces a linear ramp exactly:
+    with Y = 2x + 20 the half-sample vector (2 + 4k, 0) gives 2(x + k) + 21
+    away from the clamped border (pins the tap alignment, offset -3)."""
+    import handmade as M
+    from hpvcd_amd import _native as N
+
+    ramp = np.tile(2 * np.arange(64) + 20, (32, 1))
+    for k in (-3, 0, 2):
+        rs = M.inter_case(ramp, mvs=[(2 + 4 * k, 0)])
+        got = O.picture_planes(rs.batch, 1, N.STAGE_RECON)[0]
+        x = np.arange(64)
+        inner = (x + k - 3 >= 0) & (x + k + 4 < 64)
+        assert np.array_equal(got[:, inner], np.tile(2 * (x + k) + 21, (32, 1))[:, inner])
+        rs = M.inter_case(np.tile((2 * np.arange(32) + 20)[:, None], (1, 64)), mvs=[(0, 2 + 4 * k)])
+        got = O.picture_planes(rs.batch, 1, N.STAGE_RECON)[0]
+        y = np.arange(32)
+        inner = (y + k - 3 >= 0) & (y + k + 4 < 32)
+        assert np.array_equal(got[inner, :], np.tile((2 * (y + k) + 21)[:, None], (1, 64))[inner, :])
+
+
+def test_mc_identity_weights_add_the_offsets():
+    """Explicit weights 1 << denom are the identity: uni adds the list's
+    offset, bi the rounded mean of both offsets, then clip (src/mc.rs:631, 682)."""
+    import handmade as M
+    from hpvcd_amd import _native as N
+
+    wts = (((64, 9), (64, -20), (64, 100)), ((64, -4), (64, 7), (64, 30)))
+    full = mc_expected_full(MC_Y, MC_CB, MC_CR, FULL_MVS)
+    rs = M.inter_case(MC_Y, MC_CB, MC_CR, mvs=FULL_MVS, weights=wts)
+    got = O.picture_planes(rs.batch, 1, N.STAGE_RECON)
+    for c in range(3):
+        assert np.array_equal(got[c], np.clip(full[c] + wts[0][c][1], 0, 255))
+    rs = M.inter_case(MC_Y, MC_CB, MC_CR, mvs=FULL_MVS, bi=True, weights=wts)
+    got = O.picture_planes(rs.batch, 1, N.STAGE_RECON)
+    for c in range(3):
+        assert np.array_equal(got[c], np.clip(full[c] + ((wts[0][c][1] + wts[1][c][1] + 1) >> 1), 0, 255))
+
+
+def test_oracle_batch_waves_match_per_picture_decode():
+    """The wave-scheduled batch decode equals picture-by-picture decodes that
+    rebuild every reference themselves."""
+    import hpvcd_amd as H
+    from hpvcd_amd import _native as N
+
+    rs = H.RecordSet.generate("single512", tile_width=128, tile_height=64, grid_cols=1, grid_rows=4, num_outputs=4,
+                              out_width=128, out_height=64, inter=1, weighted=1, base_seed=5)
+    outs = rs.output_buffers()
+    O.decode_batch(rs.batch, outs, threads=4)
+    for p in range(4):
+        y, cb, cr = O.picture_planes(rs.batch, p, N.STAGE_SAO)
+        assert np.array_equal(O.picture_rgb(rs.batch, p, y, cb, cr), outs[p])
 
 
 # --------------------------------------------------------------------------------------
