@@ -34,15 +34,19 @@ def kernel_alg_bytes(stats: dict, sample_b: int, res_b: int, out_b: int) -> dict
       deblock:  2*S*sample_b (read + write of the planes)
       sao:      2*S*sample_b (read deblocked, write filtered)
       rgb:      S*sample_b (read) + 3*P*out_b (write)
+      mc:       reference samples read per list + predicted samples written, summed over PUs
     """
     S, nnz, tbs, P = stats["samples"], stats["nnz"], stats["tbs"], stats["out_pixels"]
-    return {
+    alg = {
         "residual": 4 * nnz + 24 * tbs + S * res_b,
         "intra": S * res_b + S * sample_b + 24 * tbs + 4 * S // 16,
         "deblock": 2 * S * sample_b,
         "sao": 2 * S * sample_b,
         "rgb": S * sample_b + 3 * P * out_b,
     }
+    if stats.get("pus"):
+        alg["mc"] = stats["mc_read_bytes"] + stats["mc_write_bytes"]
+    return alg
 
 
 def measured_traffic(workload: str, kernel: str):
@@ -147,7 +151,7 @@ def main() -> int:
     dominant = max((k for k in kms if k in alg), key=lambda k: kms[k])
     dom_ms = kms[dominant]
     achieved = alg[dominant] / (dom_ms / 1e3) / 1e9
-    total_alg = 4 * st["nnz"] + 6 * st["samples"] * sample_b + 3 * st["out_pixels"] * out_b
+    total_alg = 4 * st["nnz"] + 6 * st["samples"] * sample_b + 3 * st["out_pixels"] * out_b + st["mc_read_bytes"]
 
     cpu = None
     bit_exact = None
@@ -201,6 +205,7 @@ def main() -> int:
             "workload": args.workload,
             "image": f"{rs.outputs()[0].width}x{rs.outputs()[0].height}",
             "pictures": st["pictures"], "tbs": st["tbs"], "nnz": st["nnz"], "samples": st["samples"],
+            **({"pus": st["pus"]} if st["pus"] else {}),
             "mode": "batched, one launch per stage, hipGraph replay",
         },
         "bit_exact": bit_exact,

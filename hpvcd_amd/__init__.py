@@ -41,6 +41,11 @@ WORKLOADS: Dict[str, dict] = {
                      chroma_idc=3, bit_depth=10, qp=22, config_id=2, out_kind=N.OUT_RGB16,
                      transform_skip=1, log2_max_ts=2, p_ts=0.2, implicit_rdpcm=1, cross_component=1,
                      log2_res_scale_max=3),
+    # 1920x1080 4:2:0 8-bit sequence of 32 pictures: 1 I + 31 low-delay B with 4 references,
+    # bi-prediction p=0.5, explicit weighted prediction, MV ~ N(0, 16 px), QP 30 -> one RGB8 image per picture
+    "video1080": dict(tile_width=1920, tile_height=1080, grid_cols=1, grid_rows=32, out_width=1920, out_height=1080,
+                      num_outputs=32, chroma_idc=1, bit_depth=8, qp=30, config_id=3, out_kind=N.OUT_RGB8,
+                      inter=1, num_refs=4, weighted=1, p_bi=0.5, mv_sigma=16.0),
 }
 
 
@@ -179,8 +184,23 @@ class RecordSet:
             sample_bytes += n * (1 if p.bit_depth_luma == 8 else 2)
         out_px = sum(o.width * o.height for o in self.outputs())
         out_bytes = sum(N.lib().hpvcd_output_bytes(C.byref(o)) for o in self.outputs())
+        # motion compensation: reference samples read per list and predicted samples written
+        mc_read = mc_write = 0
+        for i in range(self.num_pictures):
+            p = self.picture(i)
+            if not p.pu_count:
+                continue
+            pus = (N.PU * p.pu_count).from_address(C.addressof(self.batch.pus.contents) + p.pu_begin * C.sizeof(N.PU))
+            sw = 2 if p.chroma_idc in (1, 2) else 1
+            sh = 2 if p.chroma_idc == 1 else 1
+            b = 1 if p.bit_depth_luma == 8 else 2
+            for u in pus:
+                n = u.w * u.h + (2 * (u.w // sw) * (u.h // sh) if p.chroma_idc else 0)
+                mc_read += n * b * bin(u.flags & 3).count("1")
+                mc_write += n * b
         return dict(samples=s, sample_bytes=sample_bytes, nnz=int(self.batch.num_coeffs), out_pixels=out_px,
-                    out_bytes=int(out_bytes), pictures=self.num_pictures, tbs=int(self.batch.num_tbs))
+                    out_bytes=int(out_bytes), pictures=self.num_pictures, tbs=int(self.batch.num_tbs),
+                    pus=int(self.batch.num_pus), mc_read_bytes=mc_read, mc_write_bytes=mc_write)
 
 
 _TB_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2n", "u1"), ("comp", "u1"), ("mode", "u1"), ("qp", "u1"),

@@ -38,6 +38,8 @@ KERNEL_SYMBOLS = {
     "deblock": "k_deblock_",
     "sao": "k_sao<",
     "rgb": "k_rgb<",
+    "mc": "k_mc<",
+    "act": "k_act<",
 }
 
 
@@ -92,6 +94,17 @@ def calibration(values: list[tuple[str, float]], copy_mib: int) -> dict:
             "bytes_per_kib": (copy_mib << 20) / reading_kib}
 
 
+def write_kernel_stats(workload: str, stats: list[dict], command: str, out_dir: str) -> None:
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, f"kernel_stats_{workload}.md"), "w") as f:
+        f.write(f"# Kernel durations, {workload}\n\nCommand:\n\n    {command}\n\n")
+        f.write("| kernel | calls | avg us | min us | max us | % of GPU time |\n|---|---|---|---|---|---|\n")
+        for r in stats[:16]:
+            name = r["name"].split("(")[0].replace("void ", "")
+            f.write(f"| `{name}` | {r['calls']} | {r['avg_ns'] / 1e3:.2f} | {r['min_ns'] / 1e3:.2f} | "
+                    f"{r['max_ns'] / 1e3:.2f} | {r['percent']:.1f} |\n")
+
+
 def summarize(workload: str, copy_mib: int, trace_dir: str, fetch_dir: str, write_dir: str, command: str,
               out_dir: str) -> dict:
     stats = kernel_stats(trace_dir)
@@ -143,27 +156,24 @@ def summarize(workload: str, copy_mib: int, trace_dir: str, fetch_dir: str, writ
         for kind, k in kernels.items():
             f.write(f"| {kind} | {k['fetch_bytes_per_launch'] / 1e6:.2f} | {k['write_bytes_per_launch'] / 1e6:.2f} | "
                     f"{k['hbm_bytes_per_launch'] / 1e6:.2f} |\n")
-    with open(os.path.join(out_dir, f"kernel_stats_{workload}.md"), "w") as f:
-        f.write(f"# Kernel durations, {workload}\n\nCommand:\n\n    {summary['command_trace']}\n\n")
-        f.write("| kernel | calls | avg us | min us | max us | % of GPU time |\n|---|---|---|---|---|---|\n")
-        for r in stats[:16]:
-            name = r["name"].split("(")[0].replace("void ", "")
-            f.write(f"| `{name}` | {r['calls']} | {r['avg_ns'] / 1e3:.2f} | {r['min_ns'] / 1e3:.2f} | "
-                    f"{r['max_ns'] / 1e3:.2f} | {r['percent']:.1f} |\n")
+    write_kernel_stats(workload, stats, summary["command_trace"], out_dir)
     return summary
 
 
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--workload", required=True)
-    ap.add_argument("--copy-mib", type=int, required=True)
+    ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--trace", required=True)
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
+    ap.add_argument("--fetch", help="FETCH_SIZE run (omit with --write for a durations-only summary)")
+    ap.add_argument("--write", help="WRITE_SIZE run")
     ap.add_argument("--command", required=True,
                     help="the profiled command with {run} in place of the rocprofv3 mode options")
     ap.add_argument("--out", default="profiles")
     a = ap.parse_args()
+    if not a.fetch or not a.write:
+        write_kernel_stats(a.workload, kernel_stats(a.trace), a.command.format(run="--kernel-trace --stats"), a.out)
+        return
     s = summarize(a.workload, a.copy_mib, a.trace, a.fetch, a.write, a.command, a.out)
     print(json.dumps(s["kernels"], indent=1))
 
