@@ -217,6 +217,8 @@ struct PrepLds {
     hpvcd_tb tb[kMaxUnitTbs];
     uint32_t ro[kMaxUnitTbs];
     uint16_t rmap[kMaxUnitRefs];
+    uint16_t job[kMaxUnitTbs];       // 4x4 TBs first, then the others
+    uint32_t nsmall, nbig;
 };
 
 constexpr int kPrepThreads = 256;
@@ -258,22 +260,39 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
         L.ord[c] = in ? ord_g[gy * gw + gx] : 0xffffffffu;
         L.inf[c] = (in && need_info) ? inf_g[gy * gw + gx] : 0u;
     }
-    for (int e = tid; e < ntb; e += kPrepThreads) {
-        L.tb[e] = tbs[unit_tbs[tb_list + e]];
-        L.ro[e] = unit_refoff[tb_list + e];
+    if (tid == 0) {
+        L.nsmall = 0;
+        L.nbig = 0;
     }
+    __syncthreads();
+    for (int e = tid; e < ntb; e += kPrepThreads) {
+        const hpvcd_tb t = tbs[unit_tbs[tb_list + e]];
+        L.tb[e] = t;
+        L.ro[e] = unit_refoff[tb_list + e];
+        // 4x4 TBs go four to a wave (16 lanes each), the others one to a wave
+        if (t.log2n == 2) L.job[atomicAdd(&L.nsmall, 1u)] = (uint16_t)e;
+    }
+    __syncthreads();
+    for (int e = tid; e < ntb; e += kPrepThreads)
+        if (L.tb[e].log2n != 2) L.job[L.nsmall + atomicAdd(&L.nbig, 1u)] = (uint16_t)e;
     __syncthreads();
     HPVCD_PREP_T(1)
     const int lgv = 2 - sh, lgh = 2 - sw;
     const int gv = 4 >> sh, gh = 4 >> sw;
     const bool strong_on = plane == 0 && (pflags & HPVCD_PF_STRONG_INTRA_SMOOTHING);
-    // one wave per TB: availability mask (one lane per 4-sample group),
-    // substituted reference map, descriptor, per-sample reference words
-    for (int e = wave; e < ntb; e += kWaves) {
+    const int nsmall = (int)L.nsmall, small_jobs = (nsmall + 3) >> 2, njobs = small_jobs + (int)L.nbig;
+    // per job: availability mask (one lane per 4-sample group), substituted
+    // reference map, descriptor, per-sample reference words of each TB
+    for (int j = wave; j < njobs; j += kWaves) {
+        const bool small = j < small_jobs;
+        const int G = small ? 16 : 64, gl = small ? (lane & 15) : lane, gbase = small ? (lane & 48) : 0;
+        const int slot = small ? 4 * j + (lane >> 4) : nsmall + (j - small_jobs);
+        const bool has = small ? slot < nsmall : true;
+        const int e = L.job[has ? slot : 4 * j];
         const hpvcd_tb tb = L.tb[e];
-        const int log2n = uni((int)tb.log2n), mode = uni((int)tb.mode), n = 1 << log2n, n2 = 2 * n;
-        const int x = uni((int)tb.x), y = uni((int)tb.y), x0 = x - ux0, y0 = y - uy0;
-        const uint32_t order = uni(tb.order), ro = uni(L.ro[e]);
+        const int log2n = tb.log2n, mode = tb.mode, n = 1 << log2n, n2 = 2 * n;
+        const int x = tb.x, y = tb.y, x0 = x - ux0, y0 = y - uy0;
+        const uint32_t order = tb.order, ro = L.ro[e];
         const int roff = ro & 0x1fff, soff = ((ro >> 13) & 0xff) << 4;
         const int ngl = n2 / gv, nga = n2 / gh;
         auto cell = [&](int px, int py) -> int {
@@ -281,9 +300,10 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
         };
         const uint32_t cur = L.inf[cell(x, y)];
         int px = x - 1, py = y - 1;
-        if (lane < ngl) py = y + n2 - (lane + 1) * gv;
-        else if (lane > ngl) { px = x + (lane - ngl - 1) * gh; }
-        bool av = lane < ngl + 1 + nga && px >= 0 && py >= 0 && px < pw && py < ph && (px << sw) < lw && (py << sh) < lh;
+        if (gl < ngl) py = y + n2 - (gl + 1) * gv;
+        else if (gl > ngl) { px = x + (gl - ngl - 1) * gh; }
+        bool av = has && gl < ngl + 1 + nga && px >= 0 && py >= 0 && px < pw && py < ph && (px << sw) < lw &&
+                  (py << sh) < lh;
         if (av) {
             const int c = cell(px, py);
             av = L.ord[c] < order;
@@ -294,15 +314,18 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
                 if ((pflags & HPVCD_PF_CONSTRAINED_INTRA) && !(i & HPVCD_CELL_INTRA)) av = false;
             }
         }
-        const uint64_t m = __ballot(av);
+        const uint64_t mall = __ballot(av);
+        const uint64_t m = small ? (mall >> gbase) & 0xffffull : mall;
         // substituted reference map
         uint16_t* M = L.rmap + roff;
-        for (int k = lane; k <= 2 * n2; k += 64) {
-            const uint16_t addr = (uint16_t)ref_tile_addr(k, m, n, x0, y0, lgv, lgh);
-            M[k] = addr;
-            rmap_g[rbase + roff + k] = addr;
-        }
+        if (has)
+            for (int k = gl; k <= 2 * n2; k += G) {
+                const uint16_t addr = (uint16_t)ref_tile_addr(k, m, n, x0, y0, lgv, lgh);
+                M[k] = addr;
+                rmap_g[rbase + roff + k] = addr;
+            }
         wave_sync();
+        if (!has) continue;
         // border samples read (substitution only ever reads available
         // samples): the row above for a TB on the unit's top edge, the left
         // column for one on its left edge
@@ -339,7 +362,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             u0 = A(n2 + 1 + n);
             u1 = A(n2 - 1 - n);
         }
-        if (lane == 0) {
+        if (gl == 0) {
             uint4 d;
             d.x = (uint32_t)x0 | ((uint32_t)y0 << 6) | ((uint32_t)(log2n - 2) << 12) | ((uint32_t)s.cls << 14) |
                   ((tb.flags & TBX_HAS_RES) ? 1u << 16 : 0u) | (s.filt ? 1u << 17 : 0u) |
@@ -351,7 +374,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             desc_g[tb_list + e] = d;
         }
         uint32_t* dst = smap_g + sbase + soff;
-        for (int p = lane; p < n * n; p += 64) {
+        for (int p = gl; p < n * n; p += G) {
             const uint32_t own = (uint32_t)(kTopW + (y0 + (p >> log2n)) * kRowW + x0 + (p & (n - 1)) + 1);
             dst[p] = pcm ? (uint32_t)kZero | ((uint32_t)kZero << 13)
                          : inter ? own | (own << 13) : sample_word(s, p & (n - 1), p >> log2n, Ms);
