@@ -217,8 +217,8 @@ struct PrepLds {
     hpvcd_tb tb[kMaxUnitTbs];
     uint32_t ro[kMaxUnitTbs];
     uint16_t rmap[kMaxUnitRefs];
-    uint16_t job[kMaxUnitTbs];       // 4x4 TBs first, then the others
-    uint32_t nsmall, nbig;
+    uint16_t job[kMaxUnitTbs];       // 4x4 TBs, then 8x8 TBs, then the others
+    uint32_t n4, n8, nbig;
 };
 
 constexpr int kPrepThreads = 256;
@@ -261,34 +261,45 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
         L.inf[c] = (in && need_info) ? inf_g[gy * gw + gx] : 0u;
     }
     if (tid == 0) {
-        L.nsmall = 0;
+        L.n4 = 0;
+        L.n8 = 0;
         L.nbig = 0;
     }
     __syncthreads();
+    // 4x4 TBs go four to a wave (16 lanes each), 8x8 TBs two (32 lanes), the
+    // others one
     for (int e = tid; e < ntb; e += kPrepThreads) {
         const hpvcd_tb t = tbs[unit_tbs[tb_list + e]];
         L.tb[e] = t;
         L.ro[e] = unit_refoff[tb_list + e];
-        // 4x4 TBs go four to a wave (16 lanes each), the others one to a wave
-        if (t.log2n == 2) L.job[atomicAdd(&L.nsmall, 1u)] = (uint16_t)e;
+        if (t.log2n == 2) L.job[atomicAdd(&L.n4, 1u)] = (uint16_t)e;
     }
     __syncthreads();
     for (int e = tid; e < ntb; e += kPrepThreads)
-        if (L.tb[e].log2n != 2) L.job[L.nsmall + atomicAdd(&L.nbig, 1u)] = (uint16_t)e;
+        if (L.tb[e].log2n == 3) L.job[L.n4 + atomicAdd(&L.n8, 1u)] = (uint16_t)e;
+    __syncthreads();
+    for (int e = tid; e < ntb; e += kPrepThreads)
+        if (L.tb[e].log2n > 3) L.job[L.n4 + L.n8 + atomicAdd(&L.nbig, 1u)] = (uint16_t)e;
     __syncthreads();
     HPVCD_PREP_T(1)
     const int lgv = 2 - sh, lgh = 2 - sw;
     const int gv = 4 >> sh, gh = 4 >> sw;
     const bool strong_on = plane == 0 && (pflags & HPVCD_PF_STRONG_INTRA_SMOOTHING);
-    const int nsmall = (int)L.nsmall, small_jobs = (nsmall + 3) >> 2, njobs = small_jobs + (int)L.nbig;
+    const int n4 = (int)L.n4, n8 = (int)L.n8;
+    const int j4 = (n4 + 3) >> 2, j8 = (n8 + 1) >> 1, njobs = j4 + j8 + (int)L.nbig;
     // per job: availability mask (one lane per 4-sample group), substituted
     // reference map, descriptor, per-sample reference words of each TB
     for (int j = wave; j < njobs; j += kWaves) {
-        const bool small = j < small_jobs;
-        const int G = small ? 16 : 64, gl = small ? (lane & 15) : lane, gbase = small ? (lane & 48) : 0;
-        const int slot = small ? 4 * j + (lane >> 4) : nsmall + (j - small_jobs);
-        const bool has = small ? slot < nsmall : true;
-        const int e = L.job[has ? slot : 4 * j];
+        const int lg = j < j4 ? 4 : (j < j4 + j8 ? 5 : 6);  // log2 of the lanes per TB
+        const int G = 1 << lg, gl = lane & (G - 1), gbase = lane & ~(G - 1);
+        int first, end;  // the job's first TB slot (always present) and the end of its size class
+        if (lg == 4) { first = 4 * j; end = n4; }
+        else if (lg == 5) { first = n4 + 2 * (j - j4); end = n4 + n8; }
+        else { first = n4 + n8 + (j - j4 - j8); end = ntb; }
+        const int slot = first + (lane >> lg);
+        const bool has = slot < end;
+        const int e = L.job[has ? slot : first];
+        const bool small = lg < 6;
         const hpvcd_tb tb = L.tb[e];
         const int log2n = tb.log2n, mode = tb.mode, n = 1 << log2n, n2 = 2 * n;
         const int x = tb.x, y = tb.y, x0 = x - ux0, y0 = y - uy0;
@@ -315,7 +326,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             }
         }
         const uint64_t mall = __ballot(av);
-        const uint64_t m = small ? (mall >> gbase) & 0xffffull : mall;
+        const uint64_t m = small ? (mall >> gbase) & ((1ull << G) - 1ull) : mall;
         // substituted reference map
         uint16_t* M = L.rmap + roff;
         if (has)
