@@ -75,5 +75,9 @@ struct DevUnit {
 
 // TB flag bits added by the runtime (upper byte of hpvcd_tb.flags).
 constexpr uint16_t TBX_HAS_RES = 1u << 8;  // residual samples exist in the residual pool
+// bits 9-11: in the wavefront's TB order of its unit, the TB leads a group of
+// this many consecutive 4 x 4 fast-path TBs of one dependency level (they do
+// not read each other's samples and run in one step); 0 inside a group
+constexpr int TBX_GROUP_SHIFT = 9;
 
 }  // namespace hpvcd

@@ -378,7 +378,8 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             d.x = (uint32_t)x0 | ((uint32_t)y0 << 6) | ((uint32_t)(log2n - 2) << 12) | ((uint32_t)s.cls << 14) |
                   ((tb.flags & TBX_HAS_RES) ? 1u << 16 : 0u) | (s.filt ? 1u << 17 : 0u) |
                   ((s.filt && strong_on && n == 32) ? 1u << 18 : 0u) | (y0 + n == ctbh ? 1u << 19 : 0u) |
-                  (x0 + n == ctbw ? 1u << 20 : 0u) | ((n <= 8 && !s.filt && s.cls != kClsDc) ? 1u << 21 : 0u);
+                  (x0 + n == ctbw ? 1u << 20 : 0u) | ((n <= 8 && !s.filt && s.cls != kClsDc) ? 1u << 21 : 0u) |
+                  ((uint32_t)((tb.flags >> TBX_GROUP_SHIFT) & 7u) << 22);
             d.y = (ro & 0x1fffu) | ((uint32_t)soff << 13);
             d.z = u0 | (u1 << 13);
             d.w = top_need | (left_need << 8);
@@ -400,6 +401,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
 //      16 residual  17 filtered references  18 strong-filter candidate
 //      19 writes the unit's bottom row  20 writes its right column
 //      21 fast path: at most 8 x 8, references unfiltered, not DC
+//      [22,25) size of the group of independent 4 x 4 TBs this TB leads
 //   y: [0,13) reference map offset  [13,25) sample word offset
 //   z: uniform references of the class: [0,13) u0  [13,26) u1
 struct IntraLds {
@@ -604,6 +606,65 @@ __device__ __forceinline__ void intra_tb_fast(uint16_t* __restrict__ S, uint32_t
     wave_sync();
 }
 
+// Up to four independent 4 x 4 TBs of one dependency level in one step:
+// lane group k = lane / 16 predicts TB k of the group; d0/d2 are the lane's
+// TB descriptor words, w/rv its prefetched reference word and residual.
+__device__ __forceinline__ void intra_tb_group4(uint16_t* __restrict__ S, uint32_t d0, uint32_t d2, uint32_t w,
+                                                int32_t rv, int lane, int g, int32_t maxv,
+                                                uint32_t* __restrict__ bw, uint32_t tagw) {
+    const int x0 = d0 & 63, y0 = (d0 >> 6) & 63, cls = (d0 >> 14) & 3;
+    const int32_t r0 = S[w & 0x1fffu], r1 = S[(w >> 13) & 0x1fffu];
+    const int k = (w >> 26) & 31;
+    const int p = lane & 15, x = p & 3, y = p >> 2;
+    int32_t r;
+    if (cls == kClsAngular) {
+        r = (mul24(32 - k, r0) + mul24(k, r1) + 16) >> 5;
+    } else {
+        const int32_t ru0 = S[d2 & 0x1fffu], ru1 = S[(d2 >> 13) & 0x1fffu];
+        if (cls == kClsHv)
+            r = (int32_t)(w >> 31) ? clampi(ru0 + ((r1 - ru1) >> 1), 0, maxv) : r0;
+        else  // planar
+            r = (mul24(3 - x, r0) + mul24(x + 1, ru0) + mul24(3 - y, r1) + mul24(y + 1, ru1) + 4) >> 3;
+    }
+    const int32_t v = clampi(wadd(r, rv), 0, maxv);
+    if (lane < 16 * g) {
+        S[kTopW + (y0 + y) * kRowW + x0 + x + 1] = (uint16_t)v;
+        if (((d0 >> 19) & 1) && y == 3) border_store(bw + x0 + x, tagw | (uint32_t)v);
+        if (((d0 >> 20) & 1) && x == 3) border_store(bw + kMaxCtb + y0 + y, tagw | (uint32_t)v);
+    }
+    wave_sync();
+}
+
+// TBs of one step of the prediction wave: a group of up to four independent
+// 4 x 4 TBs (descriptor bits 22-24 of its first TB), else one TB.
+__device__ __forceinline__ int step_size(uint32_t d0) { return max(1, (int)((d0 >> 22) & 7)); }
+
+// The lane's descriptor words, reference word and residual of the step that
+// starts at TB `base` (steps of TBs up to 8 x 8; a group's words are
+// contiguous, 16 per TB).
+template <typename R>
+__device__ __forceinline__ void step_prefetch(const IntraLds& L, const R* __restrict__ resb, int base, uint32_t d0,
+                                              uint32_t d1, int lane, int lr4, int lr8, uint32_t& w, int32_t& r,
+                                              uint32_t& l0, uint32_t& l2) {
+    const int g = step_size(d0);
+    if (g > 1) {
+        const uint4 ld = L.desc[base + min(lane >> 4, g - 1)];
+        l0 = ld.x;
+        l2 = ld.z;
+        w = L.smap[((d1 >> 13) & 0xfff) + min(lane, 16 * g - 1)];
+        r = ((l0 >> 16) & 1) ? (int32_t)resb[((l0 >> 6) & 63) * kMaxCtb + (l0 & 63) + lr4] : 0;
+    } else if (((d0 >> 12) & 3) <= 1) {
+        const bool small = ((d0 >> 12) & 3) == 0;
+        int lr = lr8, li = lane;
+        if (small) {
+            lr = lr4;
+            li = lane & 15;
+        }
+        w = L.smap[((d1 >> 13) & 0xfff) + li];
+        r = ((d0 >> 16) & 1) ? (int32_t)resb[((d0 >> 6) & 63) * kMaxCtb + (d0 & 63) + lr] : 0;
+    }
+}
+
 __device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
 __device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
 
@@ -806,37 +867,33 @@ __global__ void __launch_bounds__(kIntraThreads)
             // ---- prediction ----
             // the descriptor of the next TB and, for TBs of up to 8 x 8, its
             // reference words and residuals are loaded while the current TB runs
-            uint32_t cd0 = 0, cd1 = 0, cd2 = 0, cw = 0;
+            uint32_t cd0 = 0, cd1 = 0, cd2 = 0, cw = 0, cl0 = 0, cl2 = 0;
             int32_t cr = 0;
             uint4 nraw = make_uint4(0, 0, 0, 0);
             // the lane's first sample of a 4 x 4 and an 8 x 8 TB: offsets in
             // the tile and in the residual block
             const int lt4 = ((lane & 15) >> 2) * kRowW + (lane & 3), lr4 = ((lane & 15) >> 2) * kMaxCtb + (lane & 3);
             const int lt8 = (lane >> 3) * kRowW + (lane & 7), lr8 = (lane >> 3) * kMaxCtb + (lane & 7);
-            auto prefetch = [&](uint32_t d0, uint32_t d1, uint32_t& w, int32_t& r) {
-                if (((d0 >> 12) & 3) <= 1) {
-                    const bool small = ((d0 >> 12) & 3) == 0;
-                    w = L.smap[((d1 >> 13) & 0xfff) + (small ? (lane & 15) : lane)];
-                    r = ((d0 >> 16) & 1) ? (int32_t)resb[((d0 >> 6) & 63) * kMaxCtb + (d0 & 63) + (small ? lr4 : lr8)] : 0;
-                }
-            };
+            // TBs run in steps: a group of up to four independent 4 x 4 TBs
+            // (size in the leader's descriptor bits 22-24), or one TB
             if (ntb > 0) {
                 const uint4 d = L.desc[0];
                 cd0 = uni(d.x);
                 cd1 = uni(d.y);
                 cd2 = uni(d.z);
-                prefetch(cd0, cd1, cw, cr);
-                nraw = L.desc[min(1, kMaxUnitTbs - 1)];
+                step_prefetch(L, resb, 0, cd0, cd1, lane, lr4, lr8, cw, cr, cl0, cl2);
+                nraw = L.desc[min(step_size(cd0), kMaxUnitTbs - 1)];
             }
             int ready = 0;
             HPVCD_PROF(0)
-            for (int i = 0; i < ntb; i++) {
-                if (i >= ready) {
+            for (int i = 0; i < ntb;) {
+                const int g = step_size(cd0);
+                if (i + g > ready) {
                     HPVCD_PROF(2)
                     uint32_t spins = 0;
                     for (;;) {
                         ready = (int)uni(__hip_atomic_load(&L.bready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
-                        if (ready > i) break;
+                        if (ready >= i + g) break;
                         if (++spins > (kSpinLimit << 4)) {
                             atomicOr(err, 1u);
                             ready = ntb;
@@ -847,13 +904,20 @@ __global__ void __launch_bounds__(kIntraThreads)
                     lds_acquire();
                     HPVCD_PROF(1)
                 }
+                const int next = i + g;
                 const uint32_t nd0 = uni(nraw.x), nd1 = uni(nraw.y), nd2 = uni(nraw.z);
-                uint32_t nw = 0;
+                uint32_t nw = 0, nl0 = 0, nl2 = 0;
                 int32_t nr = 0;
-                if (i + 1 < ntb) prefetch(nd0, nd1, nw, nr);
-                const uint4 nnraw = L.desc[min(i + 2, kMaxUnitTbs - 1)];
+                uint4 nnraw = nraw;
+                if (next < ntb) {
+                    step_prefetch(L, resb, next, nd0, nd1, lane, lr4, lr8, nw, nr, nl0, nl2);
+                    nnraw = L.desc[min(next + step_size(nd0), kMaxUnitTbs - 1)];
+                }
                 HPVCD_PROF(2)
-                if ((cd0 >> 21) & 1) {
+                if (g > 1) {
+                    intra_tb_group4(L.tile, cl0, cl2, cw, cr, lane, g, maxv, bw, tagw);
+                    HPVCD_PROF(3)
+                } else if ((cd0 >> 21) & 1) {
                     intra_tb_fast(L.tile, cd0, cd2, cw, cr, lane, ((cd0 >> 12) & 3) ? lt8 : lt4, maxv, bw, tagw);
                     HPVCD_PROF(3)
                 } else switch ((cd0 >> 12) & 3) {
@@ -862,11 +926,14 @@ __global__ void __launch_bounds__(kIntraThreads)
                     case 2: intra_tb<4, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); HPVCD_PROF(5) break;
                     default: intra_tb<5, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); HPVCD_PROF(6) break;
                 }
+                i = next;
                 cd0 = nd0;
                 cd1 = nd1;
                 cd2 = nd2;
                 cw = nw;
                 cr = nr;
+                cl0 = nl0;
+                cl2 = nl2;
                 nraw = nnraw;
             }
             HPVCD_PROF(2)

@@ -466,6 +466,20 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                     U.nlevels = (uint8_t)nlev;
                     if (lt.size() > (size_t)kMaxUnitTbs) return set_error(HPVCD_E_INVALID, "too many TBs in one CTB");
                     U.ntb = (uint16_t)lt.size();
+                    // groups of up to four consecutive independent (same
+                    // level) 4x4 TBs that take the fast path: not DC, or a
+                    // copy (PCM / inter) TB
+                    auto groupable = [&](uint32_t tbi) {
+                        const hpvcd_tb& T = C.tbs[tbi];
+                        return T.log2n == 2 && (T.mode != 1 || (T.flags & (HPVCD_TB_PCM | HPVCD_TB_INTER)));
+                    };
+                    for (size_t a = 0; a < lt.size();) {
+                        size_t e = a + 1;
+                        if (groupable(lt[a].second))
+                            while (e < lt.size() && e - a < 4 && lt[e].first == lt[a].first && groupable(lt[e].second)) e++;
+                        C.tbs[lt[a].second].flags |= (uint16_t)((e - a) << TBX_GROUP_SHIFT);
+                        a = e;
+                    }
                     U.tb_list = (uint32_t)C.unit_tbs.size();
                     U.level_list = (uint32_t)C.level_off.size();
                     size_t k = 0;
