@@ -660,7 +660,16 @@ __global__ void __launch_bounds__(256) k_deblock_chroma(const DevPic* __restrict
 
 // ---------------------------------------------------------------------------
 // K4: SAO, out of place (src/decode.rs:2258; gated EO src/sao.rs:433)
+//
+// One workgroup per (CTB, plane) of a picture, so the CTB's SAO parameters
+// are uniform; each thread takes quads of 4 horizontally adjacent samples
+// (plane widths are multiples of 4 and planes start 16-sample aligned, so a
+// quad is one aligned vector load/store).
 // ---------------------------------------------------------------------------
+template <typename T> struct Vec4;
+template <> struct Vec4<uint8_t> { using type = uchar4; };
+template <> struct Vec4<uint16_t> { using type = ushort4; };
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, const uint32_t* __restrict__ info,
                                              const hpvcd_sao* __restrict__ sao, const T* __restrict__ src_pool,
@@ -668,116 +677,196 @@ __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, co
     const DevPic& P = pics[pic_base + blockIdx.y];
     const int c = blockIdx.z;
     if (c > 0 && P.chroma_idc == 0) return;
+    const int ctb_i = blockIdx.x;
+    if (ctb_i >= P.ctb_cols * P.ctb_rows) return;
+    const int ry = ctb_i / P.ctb_cols, rx = ctb_i - ry * P.ctb_cols;
     const int pw = c ? P.cw : P.w, ph = c ? P.ch : P.h;
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint32_t)(pw * ph)) return;
-    const int y = idx / pw, x = idx - y * pw;
+    const int sw = c ? P.sub_w_log2 : 0, sh = c ? P.sub_h_log2 : 0;
+    const int ctb = 1 << P.ctb_log2;
+    const int xs = (rx * ctb) >> sw, ys = (ry * ctb) >> sh;
+    const int xe = min(((rx + 1) * ctb) >> sw, pw), ye = min(((ry + 1) * ctb) >> sh, ph);
+    const int qw = (xe - xs) >> 2, nq = qw * (ye - ys);
     const T* src = src_pool + P.plane_off[c];
     T* dst = dst_pool + P.plane_off[c];
-    const int32_t s = src[idx];
-    int32_t out = s;
+    using V = typename Vec4<T>::type;
     const bool on = c == 0 ? (P.flags & HPVCD_PF_SAO_LUMA) != 0 : ((P.flags & HPVCD_PF_SAO_CHROMA) != 0 && P.cw && P.ch);
-    const int sw = c ? P.sub_w_log2 : 0, sh = c ? P.sub_h_log2 : 0;
-    if (on) {
-        const int rx = (x << sw) >> P.ctb_log2, ry = (y << sh) >> P.ctb_log2;
-        const hpvcd_sao& S = sao[P.ctb_begin + ry * P.ctb_cols + rx];
-        const int type = S.type_idx[c];
-        const int bd = c ? P.bd_c : P.bd;
-        const int32_t maxv = (1 << bd) - 1;
-        const uint32_t ic = cell_of(P, info, x << sw, y << sh);
+    const hpvcd_sao& S = sao[P.ctb_begin + ctb_i];
+    const int type = on ? S.type_idx[c] : 0;
+    if (type == 0) {
+        for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+            const int y = ys + q / qw, x = xs + (q % qw) * 4;
+            *reinterpret_cast<V*>(dst + (size_t)y * pw + x) = *reinterpret_cast<const V*>(src + (size_t)y * pw + x);
+        }
+        return;
+    }
+    const int bd = c ? P.bd_c : P.bd;
+    const int32_t maxv = (1 << bd) - 1;
+    const int32_t o0 = S.offsets[c][0], o1 = S.offsets[c][1], o2 = S.offsets[c][2], o3 = S.offsets[c][3];
+    const int bp = S.band_pos[c];
+    const int ec = S.eo_class[c];
+    const int dx = ec == 1 ? 0 : 1, dy = ec == 0 ? 0 : (ec == 3 ? -1 : 1);
+    const bool across_slices = P.sao_across_slices != 0;
+    const bool across_tiles = !(P.flags & HPVCD_PF_TILES) || (P.flags & HPVCD_PF_LF_ACROSS_TILES);
+    const bool gated = P.suppress_any || !across_slices || !across_tiles;
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+        const int y = ys + q / qw, x = xs + (q % qw) * 4;
+        const V sv = *reinterpret_cast<const V*>(src + (size_t)y * pw + x);
+        const int32_t s[4] = {sv.x, sv.y, sv.z, sv.w};
+        int32_t out[4] = {s[0], s[1], s[2], s[3]};
         if (type == 1) {
-            uint8_t rel = (uint8_t)((uint8_t)(s >> (bd - 5)) - S.band_pos[c]);
-            if (rel < 4) out = clampi(s + S.offsets[c][rel], 0, maxv);
-        } else if (type == 2) {
-            const int ec = S.eo_class[c];
-            const int dx = ec == 1 ? 0 : 1, dy = ec == 0 ? 0 : (ec == 3 ? -1 : 1);
-            const int x1 = x + dx, y1 = y + dy, x2 = x - dx, y2 = y - dy;
-            bool ok = x1 >= 0 && y1 >= 0 && x1 < pw && y1 < ph && x2 >= 0 && y2 >= 0 && x2 < pw && y2 < ph;
-            if (ok) {
-                // src/sao.rs:378 luma_neighbor_ok (center exemption, slice and tile gates)
-                if (suppressed(P, ic)) ok = false;
-                const bool across_slices = P.sao_across_slices != 0;
-                const bool across_tiles = !(P.flags & HPVCD_PF_TILES) || (P.flags & HPVCD_PF_LF_ACROSS_TILES);
-                if (ok && (!across_slices || !across_tiles)) {
-                    uint32_t i1 = cell_of(P, info, x1 << sw, y1 << sh), i2 = cell_of(P, info, x2 << sw, y2 << sh);
-                    if (!across_slices && (HPVCD_CELL_SLICE(i1) != HPVCD_CELL_SLICE(ic) || HPVCD_CELL_SLICE(i2) != HPVCD_CELL_SLICE(ic))) ok = false;
-                    if (!across_tiles && (HPVCD_CELL_TILE(i1) != HPVCD_CELL_TILE(ic) || HPVCD_CELL_TILE(i2) != HPVCD_CELL_TILE(ic))) ok = false;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint8_t rel = (uint8_t)((uint8_t)(s[k] >> (bd - 5)) - bp);
+                const int32_t o = rel == 0 ? o0 : rel == 1 ? o1 : rel == 2 ? o2 : o3;
+                if (rel < 4) out[k] = clampi(s[k] + o, 0, maxv);
+            }
+        } else {
+            const int y1 = y + dy, y2 = y - dy;
+            const bool rows_ok = y1 >= 0 && y1 < ph && y2 >= 0 && y2 < ph;
+            const T* r1 = src + (size_t)(rows_ok ? y1 : y) * pw;
+            const T* r2 = src + (size_t)(rows_ok ? y2 : y) * pw;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int xk = x + k, x1 = xk + dx, x2 = xk - dx;
+                bool ok = rows_ok && x1 >= 0 && x1 < pw && x2 >= 0 && x2 < pw;
+                if (ok && gated) {
+                    // src/sao.rs:378 luma_neighbor_ok (center exemption, slice and tile gates)
+                    const uint32_t ic = cell_of(P, info, xk << sw, y << sh);
+                    if (suppressed(P, ic)) ok = false;
+                    if (ok && (!across_slices || !across_tiles)) {
+                        const uint32_t i1 = cell_of(P, info, x1 << sw, y1 << sh), i2 = cell_of(P, info, x2 << sw, y2 << sh);
+                        if (!across_slices && (HPVCD_CELL_SLICE(i1) != HPVCD_CELL_SLICE(ic) || HPVCD_CELL_SLICE(i2) != HPVCD_CELL_SLICE(ic))) ok = false;
+                        if (!across_tiles && (HPVCD_CELL_TILE(i1) != HPVCD_CELL_TILE(ic) || HPVCD_CELL_TILE(i2) != HPVCD_CELL_TILE(ic))) ok = false;
+                    }
+                }
+                if (ok) {
+                    const int32_t n1 = r1[x1], n2 = r2[x2];
+                    const int ei = ((s[k] > n1) - (s[k] < n1)) + ((s[k] > n2) - (s[k] < n2)) + 2;
+                    const int32_t o = ei == 0 ? o0 : ei == 1 ? o1 : ei == 3 ? o2 : ei == 4 ? o3 : 0;
+                    if (o) out[k] = clampi(s[k] + o, 0, maxv);
                 }
             }
-            if (ok) {
-                const int32_t n1 = src[y1 * pw + x1], n2 = src[y2 * pw + x2];
-                const int ei = ((s > n1) - (s < n1)) + ((s > n2) - (s < n2)) + 2;
-                const int32_t o = ei == 0 ? S.offsets[c][0] : ei == 1 ? S.offsets[c][1] : ei == 3 ? S.offsets[c][2] : ei == 4 ? S.offsets[c][3] : 0;
-                if (o) out = clampi(s + o, 0, maxv);
-            }
         }
-        if (P.suppress_any && suppressed(P, ic)) out = s;
+        if (P.suppress_any) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (suppressed(P, cell_of(P, info, (x + k) << sw, y << sh))) out[k] = s[k];
+        }
+        V ov;
+        ov.x = (T)out[0];
+        ov.y = (T)out[1];
+        ov.z = (T)out[2];
+        ov.w = (T)out[3];
+        *reinterpret_cast<V*>(dst + (size_t)y * pw + x) = ov;
     }
-    dst[idx] = (T)out;
 }
 
 // ---------------------------------------------------------------------------
 // K5: YUV -> RGB with crop window and grid placement (src/yuv.rs:187-396)
+//
+// One thread per 4 horizontally adjacent output pixels.  The Q13 products
+// fit 32 bits for bit depths up to 12 (|coefficient| < 2^15, sample < 2^12).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_rgb(const DevPic* __restrict__ pics, const T* __restrict__ pool,
                                              uint8_t* __restrict__ out_pool, uint32_t pic_base) {
     const DevPic& P = pics[pic_base + blockIdx.y];
+    const int qx = (P.copy_w + 3) >> 2;
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint32_t)(P.copy_w * P.copy_h)) return;
-    const int yy = idx / P.copy_w, xx = idx - yy * P.copy_w;
+    if (idx >= (uint32_t)(qx * P.copy_h)) return;
+    const int yy = (int)(idx / (uint32_t)qx), xx0 = ((int)idx - yy * qx) * 4;
+    const int npx = min(4, P.copy_w - xx0);
     const int w = P.w, h = P.h;
     const int x0 = P.crop_left < w - 1 ? P.crop_left : w - 1, y0 = P.crop_top < h - 1 ? P.crop_top : h - 1;
-    const int src_y = y0 + yy, src_x = x0 + xx;
-    const int y_row = src_y < h - 1 ? src_y : h - 1, x_col = src_x < w - 1 ? src_x : w - 1;
+    const int src_y = y0 + yy;
+    const int y_row = src_y < h - 1 ? src_y : h - 1;
     const int bd = P.bd;
-    const int64_t scale = (int64_t)1 << (bd - 8);
+    const int32_t scale = 1 << (bd - 8);
     const bool full = (P.flags & HPVCD_PF_FULL_RANGE) != 0;
-    const int64_t ky = full ? 8192 : 9539, kc = full ? 8192 : 9326;
-    const int64_t y_black = full ? 0 : 16 * scale, neutral = 128 * scale;
-    const int64_t L = pool[P.plane_off[0] + (size_t)y_row * w + x_col];
+    const int32_t ky = full ? 8192 : 9539, kc = full ? 8192 : 9326;
+    const int32_t y_black = full ? 0 : 16 * scale, neutral = 128 * scale;
     const bool mono = P.chroma_idc == 0;
-    int64_t v[3];
-    if (mono) {
-        v[0] = (ky * (L - y_black) + 4096) >> 13;
-    } else {
-        const int cw = P.cw, ch = P.ch;
-        const int cr = (src_y >> P.sub_h_log2) < ch - 1 ? (src_y >> P.sub_h_log2) : ch - 1;
+    const int chn = mono ? 1 : 3;
+    const int32_t cmax = bd == 8 ? 255 : (1 << bd) - 1;
+    const T* Yp = pool + P.plane_off[0] + (size_t)y_row * w;
+    const int cw = P.cw, ch = P.ch;
+    const int crow = (src_y >> P.sub_h_log2) < ch - 1 ? (src_y >> P.sub_h_log2) : ch - 1;
+    const T* Up = pool + P.plane_off[1] + (size_t)crow * cw;
+    const T* Vp = pool + P.plane_off[2] + (size_t)crow * cw;
+    int32_t crr = 0, cbg = 0, crg = 0, cbb = 0;
+    if (!mono && P.matrix != 0 && P.matrix != 8) {
+        int32_t m0, m1, m2, m3;
+        if (P.matrix == 5 || P.matrix == 6) { m0 = 11485; m1 = -2819; m2 = -5851; m3 = 14516; }
+        else if (P.matrix == 9) { m0 = 12080; m1 = -1348; m2 = -4681; m3 = 17546; }
+        else { m0 = 12901; m1 = -1534; m2 = -3835; m3 = 15201; }
+        crr = (m0 * kc + 4096) >> 13; cbg = (m1 * kc + 4096) >> 13;
+        crg = (m2 * kc + 4096) >> 13; cbb = (m3 * kc + 4096) >> 13;
+    }
+    int32_t v[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int src_x = x0 + xx0 + min(k, npx - 1);
+        const int x_col = src_x < w - 1 ? src_x : w - 1;
+        const int32_t L = Yp[x_col];
+        if (mono) {
+            v[k][0] = v[k][1] = v[k][2] = (ky * (L - y_black) + 4096) >> 13;
+            continue;
+        }
         const int cc = (src_x >> P.sub_w_log2) < cw - 1 ? (src_x >> P.sub_w_log2) : cw - 1;
-        const int64_t U = pool[P.plane_off[1] + (size_t)cr * cw + cc], V = pool[P.plane_off[2] + (size_t)cr * cw + cc];
+        const int32_t U = Up[cc], Vv = Vp[cc];
         if (P.matrix == 0) {
-            if (full) { v[0] = V; v[1] = L; v[2] = U; }
+            if (full) { v[k][0] = Vv; v[k][1] = L; v[k][2] = U; }
             else {
-                v[0] = (kc * (V - y_black) + 4096) >> 13;
-                v[1] = (ky * (L - y_black) + 4096) >> 13;
-                v[2] = (kc * (U - y_black) + 4096) >> 13;
+                v[k][0] = (kc * (Vv - y_black) + 4096) >> 13;
+                v[k][1] = (ky * (L - y_black) + 4096) >> 13;
+                v[k][2] = (kc * (U - y_black) + 4096) >> 13;
             }
         } else if (P.matrix == 8) {
-            const int64_t cg = U - neutral, co = V - neutral, t = L - cg;
-            v[0] = t + co; v[1] = L + cg; v[2] = t - co;
+            const int32_t cg = U - neutral, co = Vv - neutral, t = L - cg;
+            v[k][0] = t + co; v[k][1] = L + cg; v[k][2] = t - co;
         } else {
-            int64_t m0, m1, m2, m3;
-            if (P.matrix == 5 || P.matrix == 6) { m0 = 11485; m1 = -2819; m2 = -5851; m3 = 14516; }
-            else if (P.matrix == 9) { m0 = 12080; m1 = -1348; m2 = -4681; m3 = 17546; }
-            else { m0 = 12901; m1 = -1534; m2 = -3835; m3 = 15201; }
-            const int64_t crr = (m0 * kc + 4096) >> 13, cbg = (m1 * kc + 4096) >> 13;
-            const int64_t crg = (m2 * kc + 4096) >> 13, cbb = (m3 * kc + 4096) >> 13;
-            const int64_t yt = ky * (L - y_black), cbc = U - neutral, crc = V - neutral;
-            v[0] = (yt + crr * crc + 4096) >> 13;
-            v[1] = (yt + cbg * cbc + crg * crc + 4096) >> 13;
-            v[2] = (yt + cbb * cbc + 4096) >> 13;
+            const int32_t yt = ky * (L - y_black), cbc = U - neutral, crc = Vv - neutral;
+            v[k][0] = (yt + crr * crc + 4096) >> 13;
+            v[k][1] = (yt + cbg * cbc + crg * crc + 4096) >> 13;
+            v[k][2] = (yt + cbb * cbc + 4096) >> 13;
         }
     }
-    const int chn = mono ? 1 : 3;
-    const int64_t cmax = bd == 8 ? 255 : (1 << bd) - 1;
-    const size_t o = ((size_t)(P.dst_y + yy) * P.out_width + (P.dst_x + xx)) * chn;
+    const size_t o = ((size_t)(P.dst_y + yy) * P.out_width + (P.dst_x + xx0)) * chn;
     if (P.out_kind == HPVCD_OUT_RGB16 || P.out_kind == HPVCD_OUT_LUMA16) {
-        uint16_t* d = reinterpret_cast<uint16_t*>(out_pool + P.out_off);
-        for (int k = 0; k < chn; k++) d[o + k] = (uint16_t)clampl(v[k], 0, cmax);
+        uint16_t* d = reinterpret_cast<uint16_t*>(out_pool + P.out_off) + o;
+        uint16_t b[12];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) b[k * 3 + j] = (uint16_t)clampi(v[k][j], 0, cmax);
+        if (chn == 3 && npx == 4 && (((uintptr_t)d) & 7) == 0) {
+            uint2* d64 = reinterpret_cast<uint2*>(d);
+#pragma unroll
+            for (int m = 0; m < 3; m++)
+                d64[m] = make_uint2((uint32_t)b[4 * m] | ((uint32_t)b[4 * m + 1] << 16),
+                                    (uint32_t)b[4 * m + 2] | ((uint32_t)b[4 * m + 3] << 16));
+        } else {
+            for (int k = 0; k < npx; k++)
+                for (int j = 0; j < chn; j++) d[k * chn + j] = b[k * 3 + j];
+        }
     } else {
-        uint8_t* d = out_pool + P.out_off;
+        uint8_t* d = out_pool + P.out_off + o;
         const int sh = bd == 8 ? 0 : bd - 8;
-        for (int k = 0; k < chn; k++) d[o + k] = (uint8_t)((uint16_t)clampl(v[k], 0, cmax) >> sh);
+        uint8_t b[12];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) b[k * 3 + j] = (uint8_t)((uint16_t)clampi(v[k][j], 0, cmax) >> sh);
+        if (chn == 3 && npx == 4 && (((uintptr_t)d) & 3) == 0) {
+            uint32_t* d32 = reinterpret_cast<uint32_t*>(d);
+#pragma unroll
+            for (int m = 0; m < 3; m++)
+                d32[m] = (uint32_t)b[4 * m] | ((uint32_t)b[4 * m + 1] << 8) | ((uint32_t)b[4 * m + 2] << 16) |
+                         ((uint32_t)b[4 * m + 3] << 24);
+        } else {
+            for (int k = 0; k < npx; k++)
+                for (int j = 0; j < chn; j++) d[k * chn + j] = b[k * 3 + j];
+        }
     }
 }
 

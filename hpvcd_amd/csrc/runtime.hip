@@ -264,6 +264,7 @@ struct ClassJob {
     std::vector<uint8_t> pic_is_ref;                  // referenced by a later picture: always completes SAO
     size_t plane_elems = 0, res_elems = 0;
     uint32_t max_luma_v = 0, max_luma_h = 0, max_chroma_v = 0, max_chroma_h = 0, max_plane = 0, max_copy = 0;
+    uint32_t max_ctbs = 0, max_rgb_quads = 0;          // K4 blocks / K5 threads of the largest picture
     // device pointers
     DevPic* d_pics = nullptr;
     hpvcd_tb* d_tbs = nullptr;
@@ -304,6 +305,9 @@ struct hpvcd_job {
 };
 
 namespace {
+
+// Threads of K5 (4 output pixels each).
+uint32_t rgb_quads(const hpvcd_picture& P) { return (uint32_t)((P.copy_w + 3) / 4) * P.copy_h; }
 
 // Availability stamps, units, levels and residual lists for one class.
 hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<size_t>& out_off,
@@ -371,6 +375,8 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         C.max_chroma_h = std::max<uint32_t>(C.max_chroma_h, (D.ch >= 10 ? (D.ch - 2) / 8 : 0) * (D.cw / 4));
         C.max_plane = std::max<uint32_t>(C.max_plane, (uint32_t)ny);
         C.max_copy = std::max<uint32_t>(C.max_copy, (uint32_t)(P.copy_w * P.copy_h));
+        C.max_ctbs = std::max<uint32_t>(C.max_ctbs, (uint32_t)(D.ctb_cols * D.ctb_rows));
+        C.max_rgb_quads = std::max<uint32_t>(C.max_rgb_quads, rgb_quads(P));
 
         // copy records
         const hpvcd_tb* src_tb = b.tbs + P.tb_begin;
@@ -736,13 +742,13 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
         }
         if (st >= HPVCD_STAGE_SAO) {
             mark(HPVCD_KERNEL_SAO);
-            uint32_t mp = plain ? (uint32_t)(C.pics[p0].w * C.pics[p0].h) : C.max_plane;
-            k_sao<T><<<dim3(cdiv(mp, 256), pn, 3), 256, 0, s>>>(C.d_pics, C.d_cells, C.d_sao, pa, pb, p0);
+            uint32_t nctb = plain ? (uint32_t)(C.pics[p0].ctb_cols * C.pics[p0].ctb_rows) : C.max_ctbs;
+            k_sao<T><<<dim3(nctb, pn, 3), 256, 0, s>>>(C.d_pics, C.d_cells, C.d_sao, pa, pb, p0);
             mark(-1);
         }
         if (st >= HPVCD_STAGE_RGB) {
             mark(HPVCD_KERNEL_RGB);
-            uint32_t mc = plain ? (uint32_t)(C.pics[p0].copy_w * C.pics[p0].copy_h) : C.max_copy;
+            uint32_t mc = plain ? (uint32_t)(((C.pics[p0].copy_w + 3) / 4) * C.pics[p0].copy_h) : C.max_rgb_quads;
             if (mc) k_rgb<T><<<dim3(cdiv(mc, 256), pn), 256, 0, s>>>(C.d_pics, pb, d_out, p0);
             mark(-1);
         }
