@@ -639,18 +639,15 @@ __device__ __forceinline__ void intra_tb_group4(uint16_t* __restrict__ S, uint32
 // 4 x 4 TBs (descriptor bits 22-24 of its first TB), else one TB.
 __device__ __forceinline__ int step_size(uint32_t d0) { return max(1, (int)((d0 >> 22) & 7)); }
 
-// The lane's descriptor words, reference word and residual of the step that
-// starts at TB `base` (steps of TBs up to 8 x 8; a group's words are
-// contiguous, 16 per TB).
+// The lane's reference word and residual of a step (steps of TBs up to 8 x 8;
+// a group's words are contiguous, 16 per TB).  d0/d1 are the first TB's
+// descriptor words, l0 the lane's own TB's (loaded a step earlier).
 template <typename R>
-__device__ __forceinline__ void step_prefetch(const IntraLds& L, const R* __restrict__ resb, int base, uint32_t d0,
-                                              uint32_t d1, int lane, int lr4, int lr8, uint32_t& w, int32_t& r,
-                                              uint32_t& l0, uint32_t& l2) {
+__device__ __forceinline__ void step_prefetch(const IntraLds& L, const R* __restrict__ resb, uint32_t d0,
+                                              uint32_t d1, uint32_t l0, int lane, int lr4, int lr8, uint32_t& w,
+                                              int32_t& r) {
     const int g = step_size(d0);
     if (g > 1) {
-        const uint4 ld = L.desc[base + min(lane >> 4, g - 1)];
-        l0 = ld.x;
-        l2 = ld.z;
         w = L.smap[((d1 >> 13) & 0xfff) + min(lane, 16 * g - 1)];
         r = ((l0 >> 16) & 1) ? (int32_t)resb[((l0 >> 6) & 63) * kMaxCtb + (l0 & 63) + lr4] : 0;
     } else if (((d0 >> 12) & 3) <= 1) {
@@ -876,13 +873,21 @@ __global__ void __launch_bounds__(kIntraThreads)
             const int lt8 = (lane >> 3) * kRowW + (lane & 7), lr8 = (lane >> 3) * kMaxCtb + (lane & 7);
             // TBs run in steps: a group of up to four independent 4 x 4 TBs
             // (size in the leader's descriptor bits 22-24), or one TB
+            // per-lane descriptor of the lane's TB in the next step (a group
+            // member's), loaded one step ahead of its use
+            uint4 nlane = make_uint4(0, 0, 0, 0);
             if (ntb > 0) {
                 const uint4 d = L.desc[0];
                 cd0 = uni(d.x);
                 cd1 = uni(d.y);
                 cd2 = uni(d.z);
-                step_prefetch(L, resb, 0, cd0, cd1, lane, lr4, lr8, cw, cr, cl0, cl2);
-                nraw = L.desc[min(step_size(cd0), kMaxUnitTbs - 1)];
+                const uint4 cl = L.desc[min(lane >> 4, ntb - 1)];
+                cl0 = cl.x;
+                cl2 = cl.z;
+                step_prefetch(L, resb, cd0, cd1, cl0, lane, lr4, lr8, cw, cr);
+                const int g0 = step_size(cd0);
+                nraw = L.desc[min(g0, kMaxUnitTbs - 1)];
+                nlane = L.desc[min(g0 + (lane >> 4), ntb - 1)];
             }
             int ready = 0;
             HPVCD_PROF(0)
@@ -906,12 +911,15 @@ __global__ void __launch_bounds__(kIntraThreads)
                 }
                 const int next = i + g;
                 const uint32_t nd0 = uni(nraw.x), nd1 = uni(nraw.y), nd2 = uni(nraw.z);
-                uint32_t nw = 0, nl0 = 0, nl2 = 0;
+                const uint32_t nl0 = nlane.x, nl2 = nlane.z;
+                uint32_t nw = 0;
                 int32_t nr = 0;
-                uint4 nnraw = nraw;
+                uint4 nnraw = nraw, nnlane = nlane;
                 if (next < ntb) {
-                    step_prefetch(L, resb, next, nd0, nd1, lane, lr4, lr8, nw, nr, nl0, nl2);
-                    nnraw = L.desc[min(next + step_size(nd0), kMaxUnitTbs - 1)];
+                    step_prefetch(L, resb, nd0, nd1, nl0, lane, lr4, lr8, nw, nr);
+                    const int nn = next + step_size(nd0);
+                    nnraw = L.desc[min(nn, kMaxUnitTbs - 1)];
+                    nnlane = L.desc[min(nn + (lane >> 4), ntb - 1)];
                 }
                 HPVCD_PROF(2)
                 if (g > 1) {
@@ -935,6 +943,7 @@ __global__ void __launch_bounds__(kIntraThreads)
                 cl0 = nl0;
                 cl2 = nl2;
                 nraw = nnraw;
+                nlane = nnlane;
             }
             HPVCD_PROF(2)
         }
