@@ -429,7 +429,7 @@ __device__ __forceinline__ void intra_predict(const uint16_t* __restrict__ S, co
                                               int pub_b, int pub_r, uint32_t tagw) {
     constexpr int n = 1 << LOG2N, nn = n * n;
     constexpr int NQ = nn < 64 ? 1 : nn / 64;
-    constexpr int CH = NQ < 4 ? NQ : 4;  // samples per lane whose LDS reads overlap
+    constexpr int CH = NQ < 16 ? NQ : 16;  // samples per lane whose LDS reads overlap
     const int32_t ru0 = (CLS == kClsHv || CLS == kClsPlanar) ? (int32_t)S[u0] : 0;
     const int32_t ru1 = (CLS == kClsHv || CLS == kClsPlanar) ? (int32_t)S[u1] : 0;
 #pragma unroll 1
