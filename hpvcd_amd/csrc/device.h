@@ -58,6 +58,40 @@ struct DevMcTask {
     uint8_t ox, oy;
 };
 
+// One residual TB as K1 sees it, packed by the host in launch order: the
+// kernel's only dependent load before the coefficient words.  Everything the
+// TB's flags, its picture and the scaling lists decide is folded in here.
+struct DevResJob {
+    uint64_t out;            // residual-pool element of the TB's sample (x, y)
+    uint32_t luma_delta;     // out minus the co-located luma residual element (CCP)
+    uint32_t coeff;          // first coefficient word in the class's coefficient pool
+    int32_t scaling;         // byte offset of the TB's scaling entry, -1 = flat
+    uint16_t coeff_cnt;
+    uint16_t stride;         // residual plane row length
+    uint16_t flags;          // RJ_*
+    uint8_t qp;              // Qp' clamped to the bit depth's range; PCM: the sample shift
+    uint8_t nx;              // columns that hold nonzero levels
+    int8_t res_scale;        // ResScaleVal (CCP)
+    uint8_t bds;             // luma bit depth - 8 | (chroma bit depth - 8) << 4
+    uint8_t vw, vh;          // TB samples inside the plane
+};
+static_assert(sizeof(DevResJob) == 32, "DevResJob is 32 bytes");
+
+// DevResJob.flags
+constexpr uint16_t RJ_CBF = 1u << 0;
+constexpr uint16_t RJ_TS = 1u << 1;
+constexpr uint16_t RJ_TQB = 1u << 2;
+constexpr uint16_t RJ_PCM = 1u << 3;
+constexpr uint16_t RJ_TRANSFORM = 1u << 4;  // CBF without TS, TQB or PCM
+constexpr uint16_t RJ_DST = 1u << 5;        // 4x4 intra luma DST-VII
+constexpr uint16_t RJ_I16 = 1u << 6;        // residual arithmetic wraps to 16 bits
+constexpr uint16_t RJ_ROTATE = 1u << 7;     // TS rotation of a 4x4 intra TB
+constexpr uint16_t RJ_RDPCM = 1u << 8;      // residual DPCM on ...
+constexpr uint16_t RJ_RDPCM_V = 1u << 9;    // ... vertical (else horizontal)
+constexpr uint16_t RJ_EP = 1u << 10;        // extended precision processing
+constexpr uint16_t RJ_CHROMA = 1u << 11;    // component 1 or 2
+constexpr uint16_t RJ_CCP = 1u << 12;       // cross-component prediction from luma (res_scale != 0)
+
 // One (picture, plane, CTB) work unit of the intra reconstruction wavefront.
 struct DevUnit {
     uint32_t pic;

@@ -85,12 +85,9 @@ __device__ __forceinline__ int qpc_of(int qpi, int chroma_idc, int bd) {
 // K1: residual (dequant + inverse transform / TS / bypass + RExt ops + CCP)
 // ---------------------------------------------------------------------------
 template <typename R, int LOG2N>
-__global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pics, const hpvcd_tb* __restrict__ tbs,
-                                                  const uint32_t* __restrict__ tb_pic,
-                                                  const uint32_t* __restrict__ list, uint32_t count,
+__global__ void __launch_bounds__(256) k_residual(const DevResJob* __restrict__ jobs, uint32_t count,
                                                   const uint32_t* __restrict__ coeffs,
-                                                  const uint8_t* __restrict__ scaling, R* __restrict__ res_pool,
-                                                  int ccp_pass) {
+                                                  const uint8_t* __restrict__ scaling, R* __restrict__ res_pool) {
     constexpr int N = 1 << LOG2N;
     constexpr int NN = N * N;
     constexpr int TPB = NN < 256 ? NN : 256;   // threads per TB
@@ -105,61 +102,42 @@ __global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pic
     const uint32_t i = blockIdx.x * TBS + slot;
     const bool valid = i < count;
 
+    // the job is the only load the coefficient addresses wait for
+    DevResJob J = jobs[valid ? i : 0];
+    if (!valid) J.flags = 0;
     for (int e = tid; e < NN; e += 256) mat[e / N][e % N] = (int8_t)dct_coef32((e / N) * (32 / N), e % N);
     for (int e = 0; e < EPT; e++) blk[slot][lt + e * TPB] = 0;
 
-    uint32_t tbi = valid ? list[i] : 0;
-    hpvcd_tb tb = tbs[tbi];
-    const DevPic& P = pics[tb_pic[tbi]];
-    const int comp = tb.comp;
-    const int bd = comp ? P.bd_c : P.bd;
-    const bool ep = (P.flags & HPVCD_PF_EXTENDED_PRECISION) != 0;
-    const bool inter = (tb.flags & HPVCD_TB_INTER) != 0;
-    const bool ts = (tb.flags & HPVCD_TB_TS) != 0;
-    const bool tqb = (tb.flags & HPVCD_TB_TQB) != 0;
-    const bool cbf = (tb.flags & HPVCD_TB_CBF) != 0;
-    const bool pcm = (tb.flags & HPVCD_TB_PCM) != 0;
-    // the ACT path keeps its residuals in i32 (src/decode.rs:3660)
-    const bool i16 = bd <= 8 && !ep && (comp == 0 || tb.res_scale == 0) && !(tb.flags & HPVCD_TB_ACT);
+    const uint32_t jf = J.flags;
+    const int bd_l = 8 + (J.bds & 15), bd_c = 8 + (J.bds >> 4);
+    const int bd = (jf & RJ_CHROMA) ? bd_c : bd_l;
+    const bool ep = (jf & RJ_EP) != 0;
+    const bool ts = (jf & RJ_TS) != 0;
+    const bool tqb = (jf & RJ_TQB) != 0;
+    const bool cbf = (jf & RJ_CBF) != 0;
+    const bool pcm = (jf & RJ_PCM) != 0;
+    const bool i16 = (jf & RJ_I16) != 0;
     const int max_range = max_range_of(bd, ep);
-    const bool transform = cbf && !ts && !tqb && !pcm;
-    const bool use_dst = transform && comp == 0 && LOG2N == 2 && !inter;
-    // RExt residual ops (src/decode.rs:4356 rext_post_ops)
-    bool rotate = false;
-    int rdpcm = -1;
-    if (cbf && (ts || tqb)) {
-        rotate = ts && !tqb && (P.flags & HPVCD_PF_TS_ROTATION) && N == 4 && !inter;
-        if (inter) {
-            if (tb.flags & HPVCD_TB_RDPCM_V) rdpcm = 1;
-            else if (tb.flags & HPVCD_TB_RDPCM_H) rdpcm = 0;
-        } else if ((P.flags & HPVCD_PF_IMPLICIT_RDPCM) && (tb.mode == 10 || tb.mode == 26)) {
-            rdpcm = tb.mode == 26 ? 1 : 0;
-        }
-    }
+    const bool transform = (jf & RJ_TRANSFORM) != 0;
+    const bool use_dst = (jf & RJ_DST) != 0;
+    const bool rotate = (jf & RJ_ROTATE) != 0;
+    const int rdpcm = (jf & RJ_RDPCM) ? ((jf & RJ_RDPCM_V) ? 1 : 0) : -1;
     __syncthreads();
 
     // scatter + dequantise the nonzero levels (src/transform.rs:764-910)
     if (valid && cbf) {
         const int log2n = LOG2N;
         const int64_t bd_shift = bd + log2n + 10 - max_range;
-        const int max_qp = 51 + 6 * (bd - 8);
-        const int qp = clampi(tb.qp, 0, max_qp);
+        const int qp = J.qp;
         const int64_t factor = (int64_t)kLevelScale[qp % 6] * ((int64_t)1 << (qp / 6)) * 16;
         const int64_t add = (int64_t)1 << (bd_shift - 1);
         const int64_t cmin = -((int64_t)1 << max_range), cmax = ((int64_t)1 << max_range) - 1;
         int tr_shift = max_range - bd - log2n;
         if (ep && tr_shift < 0) tr_shift = 0;
         const int32_t tr_add = tr_shift > 0 ? 1 << (tr_shift - 1) : 0;
-        const uint8_t* sc = nullptr;
-        if (!tqb && P.scaling_index >= 0 && !(ts && N != 4)) {
-            const int size_id = LOG2N - 2;
-            const int matrix_id = (comp < 2 ? comp : 2) + (inter ? 3 : 0);
-            sc = scaling + (size_t)P.scaling_index * HPVCD_SCALING_SET_BYTES +
-                 (size_id * 6 + matrix_id) * HPVCD_SCALING_ENTRY_BYTES;
-            if (sc[65]) sc = nullptr;  // flat-16 list == flat path
-        }
-        const uint32_t* co = coeffs + P.coeff_begin + tb.coeff_off;
-        for (uint32_t k = lt; k < tb.coeff_cnt; k += TPB) {
+        const uint8_t* sc = J.scaling >= 0 ? scaling + J.scaling : nullptr;
+        const uint32_t* co = coeffs + J.coeff;
+        for (uint32_t k = lt; k < J.coeff_cnt; k += TPB) {
             uint32_t word = co[k];
             int pos = HPVCD_COEFF_POS(word);
             int32_t level = HPVCD_COEFF_LEVEL(word);
@@ -167,7 +145,7 @@ __global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pic
             if (tqb) {
                 v = i16 ? clampi(level, -32768, 32767) : level;
             } else if (pcm) {
-                v = level << tb.qp;  // pcm_sample << (BitDepth - PcmBitDepth), src/decode.rs:2788
+                v = level << qp;  // pcm_sample << (BitDepth - PcmBitDepth), src/decode.rs:2788
             } else {
                 int64_t f = factor;
                 if (sc) {
@@ -201,7 +179,7 @@ __global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pic
         const int m = o / N, c = o % N;  // tmp[m][c]
         int32_t v = 0;
         if (transform) {
-            if (c < tb.nx) {
+            if (c < J.nx) {
                 int32_t acc = 0;
                 if (use_dst) {
 #pragma unroll
@@ -238,9 +216,9 @@ __global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pic
 
     const int shift2 = max_range + 5 - bd;
     const int32_t add2 = 1 << (shift2 - 1);
-    const int pw = comp ? P.cw : P.w, ph = comp ? P.ch : P.h;
-    R* out = res_pool + P.res_off[comp];
-    const R* lres = res_pool + P.res_off[0];
+    R* out = res_pool + J.out;
+    const R* lres = out - J.luma_delta;
+    const bool ccp = (jf & RJ_CCP) != 0;
     for (int e = 0; e < EPT; e++) {
         const int o = lt + e * TPB;
         const int r = o / N, m = o % N;
@@ -261,16 +239,16 @@ __global__ void __launch_bounds__(256) k_residual(const DevPic* __restrict__ pic
         } else {
             v = 0;  // cross-component only
         }
-        const int px = tb.x + m, py = tb.y + r;
-        if (px >= pw || py >= ph) continue;
-        if (ccp_pass && tb.res_scale != 0) {
+        if (m >= J.vw || r >= J.vh) continue;
+        const size_t at = (size_t)r * J.stride + m;
+        if (ccp) {
             // src/decode.rs:5488 apply_cross_comp_residual
-            int64_t ly = (int64_t)lres[(size_t)py * P.w + px];
-            int64_t sy = (ly << P.bd_c) >> P.bd;
-            int64_t delta = ((int64_t)tb.res_scale * sy) >> 3;
+            int64_t ly = (int64_t)lres[at];
+            int64_t sy = (ly << bd_c) >> bd_l;
+            int64_t delta = ((int64_t)J.res_scale * sy) >> 3;
             v = (int32_t)clampl((int64_t)v + delta, INT32_MIN, INT32_MAX);
         }
-        out[(size_t)py * pw + px] = (R)v;
+        out[at] = (R)v;
     }
 }
 
@@ -878,15 +856,11 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
 // ---------------------------------------------------------------------------
 // explicit instantiations (sample type x residual type)
 // ---------------------------------------------------------------------------
-#define HPVCD_INST_RES(R)                                                                                        \
-    template __global__ void k_residual<R, 2>(const DevPic*, const hpvcd_tb*, const uint32_t*, const uint32_t*,  \
-                                              uint32_t, const uint32_t*, const uint8_t*, R*, int);               \
-    template __global__ void k_residual<R, 3>(const DevPic*, const hpvcd_tb*, const uint32_t*, const uint32_t*,  \
-                                              uint32_t, const uint32_t*, const uint8_t*, R*, int);               \
-    template __global__ void k_residual<R, 4>(const DevPic*, const hpvcd_tb*, const uint32_t*, const uint32_t*,  \
-                                              uint32_t, const uint32_t*, const uint8_t*, R*, int);               \
-    template __global__ void k_residual<R, 5>(const DevPic*, const hpvcd_tb*, const uint32_t*, const uint32_t*,  \
-                                              uint32_t, const uint32_t*, const uint8_t*, R*, int);
+#define HPVCD_INST_RES(R)                                                                                 \
+    template __global__ void k_residual<R, 2>(const DevResJob*, uint32_t, const uint32_t*, const uint8_t*, R*); \
+    template __global__ void k_residual<R, 3>(const DevResJob*, uint32_t, const uint32_t*, const uint8_t*, R*); \
+    template __global__ void k_residual<R, 4>(const DevResJob*, uint32_t, const uint32_t*, const uint8_t*, R*); \
+    template __global__ void k_residual<R, 5>(const DevResJob*, uint32_t, const uint32_t*, const uint8_t*, R*);
 HPVCD_INST_RES(int16_t)
 HPVCD_INST_RES(int32_t)
 

@@ -191,6 +191,7 @@ hpvcd_status validate_batch(const hpvcd_batch& b) {
             }
             if (T.flags & (HPVCD_TB_CBF | HPVCD_TB_PCM)) {
                 if ((uint64_t)T.coeff_off + T.coeff_cnt > P.coeff_count) return set_error(HPVCD_E_INVALID, "tb coeff range");
+                if (T.coeff_cnt > (uint32_t)(n * n)) return set_error(HPVCD_E_INVALID, "more coefficient words than TB positions");
                 for (uint32_t k = 0; k < T.coeff_cnt; k++) {
                     uint32_t w = b.coeffs[P.coeff_begin + T.coeff_off + k];
                     if (HPVCD_COEFF_POS(w) >= (uint32_t)(n * n)) return set_error(HPVCD_E_INVALID, "coefficient position");
@@ -251,6 +252,7 @@ struct ClassJob {
     std::vector<uint32_t> unit_refoff;             // per unit TB entry: map offset | sample offset / 16 << 13 | level << 21
     size_t smap_words = 0, rmap_words = 0;
     std::vector<uint32_t> res_list[2][4];            // [pass][log2-2], batched order
+    std::vector<DevResJob> res_jobs[2][4];           // K1 job of each res_list entry
     std::vector<uint32_t> pic_units_begin;            // plain_tickets segment per picture (npics+1)
     std::vector<uint32_t> pic_res_begin[2][4];        // per picture list segments (npics+1)
     std::vector<uint32_t> act_list, pic_act_begin;    // luma TB of each ACT triple; per picture segments
@@ -277,7 +279,7 @@ struct ClassJob {
     uint32_t* d_smap = nullptr;   // per-sample reference words
     uint16_t* d_rmap = nullptr;   // substituted reference maps
     uint4* d_desc = nullptr;      // wavefront TB descriptors (per unit TB entry)
-    uint32_t* d_res_list[2][4] = {};
+    DevResJob* d_res_jobs[2][4] = {};
     uint32_t* d_act_list = nullptr;
     DevPu* d_pus = nullptr;
     DevMcTask* d_mc_tasks = nullptr;
@@ -308,6 +310,67 @@ namespace {
 
 // Threads of K5 (4 output pixels each).
 uint32_t rgb_quads(const hpvcd_picture& P) { return (uint32_t)((P.copy_w + 3) / 4) * P.copy_h; }
+
+// K1 job of one residual TB: the flag logic of src/decode.rs:4356
+// (rext_post_ops), :3660 (ACT keeps 32-bit residuals) and
+// src/transform.rs:764-910 (scaling-list choice) decided once on the host.
+DevResJob make_res_job(const hpvcd_batch& b, const DevPic& D, const hpvcd_tb& T, uint32_t coeff_begin) {
+    DevResJob J;
+    std::memset(&J, 0, sizeof(J));
+    const int comp = T.comp, n = 1 << T.log2n;
+    const int bd = comp ? D.bd_c : D.bd;
+    const bool ep = (D.flags & HPVCD_PF_EXTENDED_PRECISION) != 0;
+    const bool inter = (T.flags & HPVCD_TB_INTER) != 0;
+    const bool ts = (T.flags & HPVCD_TB_TS) != 0;
+    const bool tqb = (T.flags & HPVCD_TB_TQB) != 0;
+    const bool cbf = (T.flags & HPVCD_TB_CBF) != 0;
+    const bool pcm = (T.flags & HPVCD_TB_PCM) != 0;
+    const bool i16 = bd <= 8 && !ep && (comp == 0 || T.res_scale == 0) && !(T.flags & HPVCD_TB_ACT);
+    const bool transform = cbf && !ts && !tqb && !pcm;
+    uint16_t f = 0;
+    if (cbf) f |= RJ_CBF;
+    if (ts) f |= RJ_TS;
+    if (tqb) f |= RJ_TQB;
+    if (pcm) f |= RJ_PCM;
+    if (transform) f |= RJ_TRANSFORM;
+    if (transform && comp == 0 && n == 4 && !inter) f |= RJ_DST;
+    if (i16) f |= RJ_I16;
+    if (ep) f |= RJ_EP;
+    if (comp) f |= RJ_CHROMA;
+    if (T.res_scale != 0) f |= RJ_CCP;
+    if (cbf && (ts || tqb)) {
+        if (ts && !tqb && (D.flags & HPVCD_PF_TS_ROTATION) && n == 4 && !inter) f |= RJ_ROTATE;
+        if (inter) {
+            if (T.flags & HPVCD_TB_RDPCM_V) f |= RJ_RDPCM | RJ_RDPCM_V;
+            else if (T.flags & HPVCD_TB_RDPCM_H) f |= RJ_RDPCM;
+        } else if ((D.flags & HPVCD_PF_IMPLICIT_RDPCM) && (T.mode == 10 || T.mode == 26)) {
+            f |= RJ_RDPCM | (T.mode == 26 ? RJ_RDPCM_V : 0);
+        }
+    }
+    J.flags = f;
+    const int max_qp = 51 + 6 * (bd - 8);
+    J.qp = (uint8_t)std::min<int>(T.qp, max_qp);
+    J.nx = T.nx;
+    J.res_scale = T.res_scale;
+    J.bds = (uint8_t)((D.bd - 8) | ((D.bd_c - 8) << 4));
+    const int pw = comp ? D.cw : D.w, ph = comp ? D.ch : D.h;
+    J.stride = (uint16_t)pw;
+    J.vw = (uint8_t)std::max(0, std::min(n, pw - (int)T.x));
+    J.vh = (uint8_t)std::max(0, std::min(n, ph - (int)T.y));
+    J.out = D.res_off[comp] + (uint64_t)T.y * pw + T.x;
+    J.luma_delta = (uint32_t)(D.res_off[comp] - D.res_off[0]);
+    J.coeff = coeff_begin + T.coeff_off;
+    J.coeff_cnt = (uint16_t)T.coeff_cnt;
+    J.scaling = -1;
+    if (!tqb && D.scaling_index >= 0 && !(ts && n != 4)) {
+        const int size_id = T.log2n - 2;
+        const int matrix_id = (comp < 2 ? comp : 2) + (inter ? 3 : 0);
+        const size_t off = (size_t)D.scaling_index * HPVCD_SCALING_SET_BYTES +
+                           (size_t)(size_id * 6 + matrix_id) * HPVCD_SCALING_ENTRY_BYTES;
+        if (!b.scaling[off + 65]) J.scaling = (int32_t)off;  // a flat-16 list is the flat path
+    }
+    return J;
+}
 
 // Availability stamps, units, levels and residual lists for one class.
 hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<size_t>& out_off,
@@ -533,10 +596,12 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
     for (int pass = 0; pass < 2; pass++)
         for (int s = 0; s < 4; s++) {
             C.res_list[pass][s].clear();
+            C.res_jobs[pass][s].clear();
             for (uint32_t lp = 0; lp < np; lp++) {
                 C.pic_res_begin[pass][s][lp] = (uint32_t)C.res_list[pass][s].size();
                 auto& v = per_pic_res[pass][s][lp];
                 C.res_list[pass][s].insert(C.res_list[pass][s].end(), v.begin(), v.end());
+                for (uint32_t t : v) C.res_jobs[pass][s].push_back(make_res_job(b, C.pics[lp], C.tbs[t], C.pics[lp].coeff_begin));
             }
             C.pic_res_begin[pass][s][np] = (uint32_t)C.res_list[pass][s].size();
         }
@@ -628,15 +693,14 @@ struct Timer {
 inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 template <typename R>
-void launch_residual(hipStream_t s, ClassJob& C, const uint8_t* sc, int pass, int log2, const uint32_t* list,
-                     uint32_t count) {
+void launch_residual(hipStream_t s, ClassJob& C, const uint8_t* sc, int log2, const DevResJob* jobs, uint32_t count) {
     if (!count) return;
     R* res = reinterpret_cast<R*>(C.d_res);
     switch (log2) {
-        case 2: k_residual<R, 2><<<cdiv(count, 16), 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, list, count, C.d_coeffs, sc, res, pass); break;
-        case 3: k_residual<R, 3><<<cdiv(count, 4), 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, list, count, C.d_coeffs, sc, res, pass); break;
-        case 4: k_residual<R, 4><<<count, 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, list, count, C.d_coeffs, sc, res, pass); break;
-        default: k_residual<R, 5><<<count, 256, 0, s>>>(C.d_pics, C.d_tbs, C.d_tb_pic, list, count, C.d_coeffs, sc, res, pass); break;
+        case 2: k_residual<R, 2><<<cdiv(count, 16), 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
+        case 3: k_residual<R, 3><<<cdiv(count, 4), 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
+        case 4: k_residual<R, 4><<<count, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
+        default: k_residual<R, 5><<<count, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
     }
 }
 
@@ -697,7 +761,7 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
                 for (int sz = 0; sz < 4; sz++) {
                     uint32_t b0 = plain ? C.pic_res_begin[pass][sz][p0] : 0;
                     uint32_t b1 = plain ? C.pic_res_begin[pass][sz][p0 + 1] : (uint32_t)C.res_list[pass][sz].size();
-                    launch_residual<R>(s, C, d_scaling, pass, sz + 2, C.d_res_list[pass][sz] + b0, b1 - b0);
+                    launch_residual<R>(s, C, d_scaling, sz + 2, C.d_res_jobs[pass][sz] + b0, b1 - b0);
                 }
             uint32_t a0 = plain ? C.pic_act_begin[p0] : 0;
             uint32_t a1 = plain ? C.pic_act_begin[p0 + 1] : (uint32_t)C.act_list.size();
@@ -1005,7 +1069,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
                  bytes_of<uint32_t>(C.units.size() * kBorderWords + 1) + bytes_of<uint64_t>(C.tbs.size() + 1) +
                  2 * ((C.plane_elems * sb + 255) & ~(size_t)255) + ((C.res_elems * rb + 255) & ~(size_t)255) + 8 * 256;
         for (int pass = 0; pass < 2; pass++)
-            for (int s = 0; s < 4; s++) total += bytes_of<uint32_t>(C.res_list[pass][s].size() + 1);
+            for (int s = 0; s < 4; s++) total += bytes_of<DevResJob>(C.res_jobs[pass][s].size() + 1);
         total += bytes_of<uint32_t>(C.act_list.size() + 1) + bytes_of<DevPu>(C.pus.size() + 1) +
                  bytes_of<DevMcTask>(C.mc_tasks.size() + 1);
     }
@@ -1040,7 +1104,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
         }
         for (int pass = 0; pass < 2; pass++)
             for (int s = 0; s < 4; s++)
-                if ((st = upload(C.d_res_list[pass][s], j->mem, C.res_list[pass][s]))) { hpvcd_job_free(j); return st; }
+                if ((st = upload(C.d_res_jobs[pass][s], j->mem, C.res_jobs[pass][s]))) { hpvcd_job_free(j); return st; }
         if ((st = upload(C.d_act_list, j->mem, C.act_list)) || (st = upload(C.d_pus, j->mem, C.pus)) ||
             (st = upload(C.d_mc_tasks, j->mem, C.mc_tasks))) {
             hpvcd_job_free(j);
