@@ -8,11 +8,17 @@
 __device__ uint64_t* g_intra_prof = nullptr;
 __device__ uint64_t* g_intra_tl = nullptr;  // per unit: 260 wall-clock stamps (start, ready, levels..., end)
 #define HPVCD_TL(u, k) if (tid == 0 && g_intra_tl) g_intra_tl[(size_t)(u) * 260 + (k)] = wall_clock64();
+#ifdef HPVCD_TL_ONLY  // wall-clock stamps per unit and per prediction step only
+#define HPVCD_PROF_DECL
+#define HPVCD_PROF(k)
+#define HPVCD_PROF_FLUSH(u)
+#else
 #define HPVCD_PROF_DECL uint64_t prof_[8] = {0, 0, 0, 0, 0, 0, 0, 0}; uint64_t prof_t_ = clock64();
 #define HPVCD_PROF(k) { const uint64_t c_ = clock64(); prof_[k] += c_ - prof_t_; prof_t_ = c_; }
 #define HPVCD_PROF_FLUSH(u)                                                                                 \
     if (lane == 0 && g_intra_prof)                                                                          \
         for (int k_ = 0; k_ < 8; k_++) g_intra_prof[((size_t)(u) * 2 + wave) * 8 + k_] = prof_[k_];
+#endif
 __device__ uint64_t* g_prep_prof = nullptr;  // per unit: cycles of the k_prep phases
 #define HPVCD_PREP_T(k) if (tid == 0 && g_prep_prof) g_prep_prof[(size_t)blockIdx.x * 8 + (k)] = clock64();
 #else
@@ -28,7 +34,8 @@ constexpr int kTopW = 2 * kMaxCtb + 1;            // row above the unit: x0-1 ..
 constexpr int kRowW = kMaxCtb + 1;                // body rows: left neighbour + the unit's samples
 constexpr int kNeutral = kTopW + kMaxCtb * kRowW; // tile slot holding 1 << (bitDepth - 1)
 constexpr int kZero = kNeutral + 1;               // tile slot holding 0: the I_PCM predictor
-constexpr int kTileWords = kNeutral + 2;
+constexpr int kSink = kNeutral + 2;               // write-only slot of the lanes of a step that hold no sample
+constexpr int kTileWords = kNeutral + 3;
 constexpr int kFStride = 132;                     // filtered references of one TB (4 * 32 + 1)
 constexpr int kFBase = kTileWords;                // the filtered reference row follows the tile
 constexpr int kMaxUnitTbs = 256;
@@ -374,12 +381,26 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             u1 = A(n2 - 1 - n);
         }
         if (gl == 0) {
+            // a step's first TB also carries the size of the step after it
+            // and whether any TB of its step writes the unit's bottom row or
+            // right column
+            const int gsz = max(1, (int)((tb.flags >> TBX_GROUP_SHIFT) & 7u));
+            const int gnext = e + gsz < ntb ? max(1, (int)((L.tb[e + gsz].flags >> TBX_GROUP_SHIFT) & 7u)) : 1;
+            bool pub = false, plain = false;
+            for (int m = 0; m < gsz && e + m < ntb; m++) {
+                const hpvcd_tb tm = L.tb[e + m];
+                const int nm = 1 << tm.log2n;
+                pub |= (tm.y - uy0) + nm == ctbh || (tm.x - ux0) + nm == ctbw;
+                const bool copy = (tm.flags & (HPVCD_TB_PCM | HPVCD_TB_INTER)) != 0;
+                plain |= !copy && (tm.mode == 0 || tm.mode == 10 || tm.mode == 26);
+            }
             uint4 d;
             d.x = (uint32_t)x0 | ((uint32_t)y0 << 6) | ((uint32_t)(log2n - 2) << 12) | ((uint32_t)s.cls << 14) |
                   ((tb.flags & TBX_HAS_RES) ? 1u << 16 : 0u) | (s.filt ? 1u << 17 : 0u) |
                   ((s.filt && strong_on && n == 32) ? 1u << 18 : 0u) | (y0 + n == ctbh ? 1u << 19 : 0u) |
                   (x0 + n == ctbw ? 1u << 20 : 0u) | ((n <= 8 && !s.filt && s.cls != kClsDc) ? 1u << 21 : 0u) |
-                  ((uint32_t)((tb.flags >> TBX_GROUP_SHIFT) & 7u) << 22);
+                  ((uint32_t)((tb.flags >> TBX_GROUP_SHIFT) & 7u) << 22) | ((uint32_t)gnext << 25) |
+                  (pub ? 1u << 28 : 0u) | (plain ? 1u << 29 : 0u);
             d.y = (ro & 0x1fffu) | ((uint32_t)soff << 13);
             d.z = u0 | (u1 << 13);
             d.w = top_need | (left_need << 8);
@@ -402,6 +423,9 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
 //      19 writes the unit's bottom row  20 writes its right column
 //      21 fast path: at most 8 x 8, references unfiltered, not DC
 //      [22,25) size of the group of independent 4 x 4 TBs this TB leads
+//      (step leaders only:) [25,28) size of the next step  28 some TB of
+//      the step writes the unit's bottom row or right column  29 some TB of
+//      the step is planar or horizontal / vertical
 //   y: [0,13) reference map offset  [13,25) sample word offset
 //   z: uniform references of the class: [0,13) u0  [13,26) u1
 struct IntraLds {
@@ -575,91 +599,76 @@ __device__ __forceinline__ void batch_copy(V* __restrict__ dst, const V* __restr
     }
 }
 
-// The common TB step: at most 8 x 8, unfiltered references, not DC, one
-// sample per lane.  w/rv are the lane's prefetched reference word and
-// residual, loff the lane's sample offset in the tile (y * kRowW + x).
-__device__ __forceinline__ void intra_tb_fast(uint16_t* __restrict__ S, uint32_t d0, uint32_t d2, uint32_t w,
-                                              int32_t rv, int lane, int loff, int32_t maxv,
-                                              uint32_t* __restrict__ bw, uint32_t tagw) {
-    const int log2n = ((d0 >> 12) & 3) + 2, n = 1 << log2n, nn = n * n;
-    const int x0 = d0 & 63, y0 = (d0 >> 6) & 63, cls = (d0 >> 14) & 3;
+// The common step: one TB of at most 8 x 8 or a group of up to four
+// independent 4 x 4 TBs of one dependency level, references unfiltered, not
+// DC, one sample per lane.  Lane l takes sample l of the step (a group's
+// samples run TB after TB, 16 each); l0/l2 are the descriptor words of its
+// TB, w/rv its prefetched reference word and residual, (x, y) its sample in
+// the TB and loff the sample's tile offset from the TB origin.  Lanes past
+// the step's samples write to the sink slot.  Planar and H/V need two more
+// reference slots (descriptor word z); `mixed` says some lane needs them.
+__device__ __forceinline__ void intra_step_fast(uint16_t* __restrict__ S, uint32_t l0, uint32_t l2, uint32_t w,
+                                                int32_t rv, bool b8, bool mixed, int x, int y, int loff,
+                                                bool active, int32_t maxv) {
     const int32_t r0 = S[w & 0x1fffu], r1 = S[(w >> 13) & 0x1fffu];
     const int k = (w >> 26) & 31;
-    const int p = lane & (nn - 1), x = p & (n - 1), y = p >> log2n;
-    int32_t r;
-    if (cls == kClsAngular) {
-        r = (mul24(32 - k, r0) + mul24(k, r1) + 16) >> 5;
-    } else {
-        const int32_t ru0 = S[d2 & 0x1fffu], ru1 = S[(d2 >> 13) & 0x1fffu];
-        if (cls == kClsHv)
-            r = (int32_t)(w >> 31) ? clampi(ru0 + ((r1 - ru1) >> 1), 0, maxv) : r0;
-        else  // planar
-            r = (mul24(n - 1 - x, r0) + mul24(x + 1, ru0) + mul24(n - 1 - y, r1) + mul24(y + 1, ru1) + n) >>
-                (log2n + 1);
+    int32_t r = (mul24(32 - k, r0) + mul24(k, r1) + 16) >> 5;
+    if (mixed) {
+        const int n = b8 ? 8 : 4, sh = b8 ? 4 : 3;
+        const int cls = (l0 >> 14) & 3;
+        const int32_t ru0 = S[l2 & 0x1fffu], ru1 = S[(l2 >> 13) & 0x1fffu];
+        const int32_t hv = (int32_t)(w >> 31) ? max(0, min(ru0 + ((r1 - ru1) >> 1), maxv)) : r0;
+        const int32_t pl = (mul24(n - 1 - x, r0) + mul24(x + 1, ru0) + mul24(n - 1 - y, r1) + mul24(y + 1, ru1) + n) >> sh;
+        r = cls == kClsAngular ? r : (cls == kClsHv ? hv : pl);
     }
-    const int32_t v = clampi(wadd(r, rv), 0, maxv);
-    if (lane < nn) {
-        S[kTopW + y0 * kRowW + x0 + 1 + loff] = (uint16_t)v;
-        if (((d0 >> 19) & 1) && y == n - 1) border_store(bw + x0 + x, tagw | (uint32_t)v);
-        if (((d0 >> 20) & 1) && x == n - 1) border_store(bw + kMaxCtb + y0 + y, tagw | (uint32_t)v);
-    }
-    wave_sync();
+    const int32_t v = max(0, min((int32_t)wadd(r, rv), maxv));
+    const int wi = active ? kTopW + 1 + mul24((l0 >> 6) & 63, kRowW) + (l0 & 63) + loff : kSink;
+    S[wi] = (uint16_t)v;
 }
 
-// Up to four independent 4 x 4 TBs of one dependency level in one step:
-// lane group k = lane / 16 predicts TB k of the group; d0/d2 are the lane's
-// TB descriptor words, w/rv its prefetched reference word and residual.
-__device__ __forceinline__ void intra_tb_group4(uint16_t* __restrict__ S, uint32_t d0, uint32_t d2, uint32_t w,
-                                                int32_t rv, int lane, int g, int32_t maxv,
-                                                uint32_t* __restrict__ bw, uint32_t tagw) {
-    const int x0 = d0 & 63, y0 = (d0 >> 6) & 63, cls = (d0 >> 14) & 3;
-    const int32_t r0 = S[w & 0x1fffu], r1 = S[(w >> 13) & 0x1fffu];
-    const int k = (w >> 26) & 31;
-    const int p = lane & 15, x = p & 3, y = p >> 2;
-    int32_t r;
-    if (cls == kClsAngular) {
-        r = (mul24(32 - k, r0) + mul24(k, r1) + 16) >> 5;
-    } else {
-        const int32_t ru0 = S[d2 & 0x1fffu], ru1 = S[(d2 >> 13) & 0x1fffu];
-        if (cls == kClsHv)
-            r = (int32_t)(w >> 31) ? clampi(ru0 + ((r1 - ru1) >> 1), 0, maxv) : r0;
-        else  // planar
-            r = (mul24(3 - x, r0) + mul24(x + 1, ru0) + mul24(3 - y, r1) + mul24(y + 1, ru1) + 4) >> 3;
-    }
-    const int32_t v = clampi(wadd(r, rv), 0, maxv);
-    if (lane < 16 * g) {
-        S[kTopW + (y0 + y) * kRowW + x0 + x + 1] = (uint16_t)v;
-        if (((d0 >> 19) & 1) && y == 3) border_store(bw + x0 + x, tagw | (uint32_t)v);
-        if (((d0 >> 20) & 1) && x == 3) border_store(bw + kMaxCtb + y0 + y, tagw | (uint32_t)v);
-    }
-    wave_sync();
+// The border samples of a fast step: the unit's bottom row and right column
+// go out at once as tagged words for the units below and to the right.
+__device__ __forceinline__ void intra_step_publish(const uint16_t* __restrict__ S, uint32_t l0, bool b8, int x, int y,
+                                                   int loff, bool active, uint32_t* __restrict__ bw, uint32_t tagw) {
+    const int n = b8 ? 8 : 4, x0 = l0 & 63, y0 = (l0 >> 6) & 63;
+    if (!active) return;
+    const uint32_t v = S[kTopW + 1 + y0 * kRowW + x0 + loff];
+    if (((l0 >> 19) & 1) && y == n - 1) border_store(bw + x0 + x, tagw | v);
+    if (((l0 >> 20) & 1) && x == n - 1) border_store(bw + kMaxCtb + y0 + y, tagw | v);
 }
 
 // TBs of one step of the prediction wave: a group of up to four independent
 // 4 x 4 TBs (descriptor bits 22-24 of its first TB), else one TB.
 __device__ __forceinline__ int step_size(uint32_t d0) { return max(1, (int)((d0 >> 22) & 7)); }
+// Size of the step after the one d0 leads (descriptor bits 25-27).
+__device__ __forceinline__ int next_step_size(uint32_t d0) { return max(1, (int)((d0 >> 25) & 7)); }
+// Lanes holding a sample in a step of TBs of at most 8 x 8.
+__device__ __forceinline__ int step_lanes(uint32_t d0) {
+    const int g = step_size(d0);
+    return g > 1 ? 16 * g : (((d0 >> 12) & 3) ? 64 : 16);
+}
 
-// The lane's reference word and residual of a step (steps of TBs up to 8 x 8;
-// a group's words are contiguous, 16 per TB).  d0/d1 are the first TB's
-// descriptor words, l0 the lane's own TB's (loaded a step earlier).
+// The lane's reference word and residual of a step of TBs of at most 8 x 8:
+// lane l takes word l of the step (a group's words are contiguous, 16 per
+// TB).  d0/d1 are the first TB's descriptor words, l0 the lane's own TB's;
+// lr4/lr8 are the lane's residual offsets in a 4 x 4 / 8 x 8 TB.  The
+// residual block holds zeros where a TB has no residual.
 template <typename R>
 __device__ __forceinline__ void step_prefetch(const IntraLds& L, const R* __restrict__ resb, uint32_t d0,
                                               uint32_t d1, uint32_t l0, int lane, int lr4, int lr8, uint32_t& w,
                                               int32_t& r) {
-    const int g = step_size(d0);
-    if (g > 1) {
-        w = L.smap[((d1 >> 13) & 0xfff) + min(lane, 16 * g - 1)];
-        r = ((l0 >> 16) & 1) ? (int32_t)resb[((l0 >> 6) & 63) * kMaxCtb + (l0 & 63) + lr4] : 0;
-    } else if (((d0 >> 12) & 3) <= 1) {
-        const bool small = ((d0 >> 12) & 3) == 0;
-        int lr = lr8, li = lane;
-        if (small) {
-            lr = lr4;
-            li = lane & 15;
-        }
-        w = L.smap[((d1 >> 13) & 0xfff) + li];
-        r = ((d0 >> 16) & 1) ? (int32_t)resb[((d0 >> 6) & 63) * kMaxCtb + (d0 & 63) + lr] : 0;
+    const int sz = (d0 >> 12) & 3;
+    if (step_size(d0) > 1 || sz <= 1) {
+        w = L.smap[min((int)((d1 >> 13) & 0xfff) + lane, kMaxCtb * kMaxCtb - 1)];
+        r = (int32_t)resb[(l0 & 0xfffu) + (sz == 1 ? lr8 : lr4)];
     }
+}
+
+// Index of the descriptor of the lane's TB in a step that starts at TB i and
+// holds g TBs (lanes past a group read a later TB; their results go to the
+// sink slot).
+__device__ __forceinline__ int lane_desc(int i, int g, int lane16, int ntb) {
+    return min(i + (g > 1 ? lane16 : 0), ntb - 1);
 }
 
 __device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
@@ -797,7 +806,7 @@ __global__ void __launch_bounds__(kIntraThreads)
             uint32_t spins = 0;
             HPVCD_PROF(3)
             for (;;) {
-#ifdef HPVCD_INTRA_PROFILE
+#if defined(HPVCD_INTRA_PROFILE) && !defined(HPVCD_TL_ONLY)
                 prof_[5]++;
 #endif
                 // poll only the missing words that the next few TBs read
@@ -866,28 +875,36 @@ __global__ void __launch_bounds__(kIntraThreads)
             // reference words and residuals are loaded while the current TB runs
             uint32_t cd0 = 0, cd1 = 0, cd2 = 0, cw = 0, cl0 = 0, cl2 = 0;
             int32_t cr = 0;
-            uint4 nraw = make_uint4(0, 0, 0, 0);
-            // the lane's first sample of a 4 x 4 and an 8 x 8 TB: offsets in
-            // the tile and in the residual block
-            const int lt4 = ((lane & 15) >> 2) * kRowW + (lane & 3), lr4 = ((lane & 15) >> 2) * kMaxCtb + (lane & 3);
-            const int lt8 = (lane >> 3) * kRowW + (lane & 7), lr8 = (lane >> 3) * kMaxCtb + (lane & 7);
+            // the lane's sample in a 4 x 4 TB (lane % 16) and an 8 x 8 TB,
+            // its tile and residual-block offsets from the TB origin
+            const int lane16 = lane >> 4;
+            const int x4 = lane & 3, y4 = (lane >> 2) & 3, x8 = lane & 7, y8 = lane >> 3;
+            const int lt4 = y4 * kRowW + x4, lt8 = y8 * kRowW + x8;
+            const int lr4 = y4 * kMaxCtb + x4, lr8 = y8 * kMaxCtb + x8;
             // TBs run in steps: a group of up to four independent 4 x 4 TBs
-            // (size in the leader's descriptor bits 22-24), or one TB
-            // per-lane descriptor of the lane's TB in the next step (a group
-            // member's), loaded one step ahead of its use
-            uint4 nlane = make_uint4(0, 0, 0, 0);
+            // (size in the leader's descriptor bits 22-24), or one TB.  The
+            // first TB's descriptor of the next step (nr*) and the descriptor
+            // of the lane's TB in it (nl*) are loaded one step ahead of their
+            // use, its reference words and residuals while the current step
+            // runs.  Only the descriptor words used are loaded: the register
+            // of an unused word would hold up the next load into it.
+            uint32_t nr0 = 0, nr1 = 0, nr2 = 0, nl0 = 0, nl2 = 0;
+            const uint32_t* dw = reinterpret_cast<const uint32_t*>(L.desc);
             if (ntb > 0) {
-                const uint4 d = L.desc[0];
-                cd0 = uni(d.x);
-                cd1 = uni(d.y);
-                cd2 = uni(d.z);
-                const uint4 cl = L.desc[min(lane >> 4, ntb - 1)];
-                cl0 = cl.x;
-                cl2 = cl.z;
-                step_prefetch(L, resb, cd0, cd1, cl0, lane, lr4, lr8, cw, cr);
+                cd0 = uni(dw[0]);
+                cd1 = uni(dw[1]);
+                cd2 = uni(dw[2]);
                 const int g0 = step_size(cd0);
-                nraw = L.desc[min(g0, kMaxUnitTbs - 1)];
-                nlane = L.desc[min(g0 + (lane >> 4), ntb - 1)];
+                const int li = lane_desc(0, g0, lane16, ntb);
+                cl0 = dw[4 * li];
+                cl2 = dw[4 * li + 2];
+                step_prefetch(L, resb, cd0, cd1, cl0, lane, lr4, lr8, cw, cr);
+                const int ni = min(g0, kMaxUnitTbs - 1), nli = lane_desc(g0, next_step_size(cd0), lane16, ntb);
+                nr0 = dw[4 * ni];
+                nr1 = dw[4 * ni + 1];
+                nr2 = dw[4 * ni + 2];
+                nl0 = dw[4 * nli];
+                nl2 = dw[4 * nli + 2];
             }
             int ready = 0;
             HPVCD_PROF(0)
@@ -909,41 +926,81 @@ __global__ void __launch_bounds__(kIntraThreads)
                     lds_acquire();
                     HPVCD_PROF(1)
                 }
-                const int next = i + g;
-                const uint32_t nd0 = uni(nraw.x), nd1 = uni(nraw.y), nd2 = uni(nraw.z);
-                const uint32_t nl0 = nlane.x, nl2 = nlane.z;
-                uint32_t nw = 0;
-                int32_t nr = 0;
-                uint4 nnraw = nraw, nnlane = nlane;
-                if (next < ntb) {
-                    step_prefetch(L, resb, nd0, nd1, nl0, lane, lr4, lr8, nw, nr);
-                    const int nn = next + step_size(nd0);
-                    nnraw = L.desc[min(nn, kMaxUnitTbs - 1)];
-                    nnlane = L.desc[min(nn + (lane >> 4), ntb - 1)];
+                if ((cd0 >> 21) & 1) {
+                    // a run of fast steps, each ready and followed by a fast one
+                    for (;;) {
+                        const int next = i + step_size(cd0);
+                        const uint32_t nd0 = uni(nr0), nd1 = uni(nr1), nd2 = uni(nr2);
+                        uint32_t nw = 0, mr0 = nr0, mr1 = nr1, mr2 = nr2, ml0 = nl0, ml2 = nl2;
+                        int32_t nrr = 0;
+                        if (next < ntb) {
+                            step_prefetch(L, resb, nd0, nd1, nl0, lane, lr4, lr8, nw, nrr);
+                            const int nn = next + step_size(nd0);
+                            const int ni = min(nn, kMaxUnitTbs - 1), nli = lane_desc(nn, next_step_size(nd0), lane16, ntb);
+                            mr0 = dw[4 * ni];
+                            mr1 = dw[4 * ni + 1];
+                            mr2 = dw[4 * ni + 2];
+                            ml0 = dw[4 * nli];
+                            ml2 = dw[4 * nli + 2];
+                        }
+                        const bool b8 = (cd0 >> 12) & 1;
+                        const int x = b8 ? x8 : x4, y = b8 ? y8 : y4, loff = b8 ? lt8 : lt4;
+                        const bool active = lane < step_lanes(cd0);
+                        intra_step_fast(L.tile, cl0, cl2, cw, cr, b8, (cd0 >> 29) & 1, x, y, loff, active, maxv);
+                        if ((cd0 >> 28) & 1) intra_step_publish(L.tile, cl0, b8, x, y, loff, active, bw, tagw);
+                        HPVCD_PROF(3)
+                        HPVCD_TL(ui, 2 + i)
+                        i = next;
+                        cd0 = nd0;
+                        cd1 = nd1;
+                        cd2 = nd2;
+                        cw = nw;
+                        cr = nrr;
+                        cl0 = nl0;
+                        cl2 = nl2;
+                        nr0 = mr0;
+                        nr1 = mr1;
+                        nr2 = mr2;
+                        nl0 = ml0;
+                        nl2 = ml2;
+                        if (i >= ntb || !((cd0 >> 21) & 1) || i + step_size(cd0) > ready) break;
+                    }
+                } else {
+                    const int next = i + step_size(cd0);
+                    const uint32_t nd0 = uni(nr0), nd1 = uni(nr1), nd2 = uni(nr2);
+                    uint32_t nw = 0, mr0 = nr0, mr1 = nr1, mr2 = nr2, ml0 = nl0, ml2 = nl2;
+                    int32_t nrr = 0;
+                    if (next < ntb) {
+                        step_prefetch(L, resb, nd0, nd1, nl0, lane, lr4, lr8, nw, nrr);
+                        const int nn = next + step_size(nd0);
+                        const int ni = min(nn, kMaxUnitTbs - 1), nli = lane_desc(nn, next_step_size(nd0), lane16, ntb);
+                        mr0 = dw[4 * ni];
+                        mr1 = dw[4 * ni + 1];
+                        mr2 = dw[4 * ni + 2];
+                        ml0 = dw[4 * nli];
+                        ml2 = dw[4 * nli + 2];
+                    }
+                    switch ((cd0 >> 12) & 3) {
+                        case 0: intra_tb<2, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); HPVCD_PROF(3) break;
+                        case 1: intra_tb<3, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); HPVCD_PROF(4) break;
+                        case 2: intra_tb<4, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); HPVCD_PROF(5) break;
+                        default: intra_tb<5, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); HPVCD_PROF(6) break;
+                    }
+                    HPVCD_TL(ui, 2 + i)
+                    i = next;
+                    cd0 = nd0;
+                    cd1 = nd1;
+                    cd2 = nd2;
+                    cw = nw;
+                    cr = nrr;
+                    cl0 = nl0;
+                    cl2 = nl2;
+                    nr0 = mr0;
+                    nr1 = mr1;
+                    nr2 = mr2;
+                    nl0 = ml0;
+                    nl2 = ml2;
                 }
-                HPVCD_PROF(2)
-                if (g > 1) {
-                    intra_tb_group4(L.tile, cl0, cl2, cw, cr, lane, g, maxv, bw, tagw);
-                    HPVCD_PROF(3)
-                } else if ((cd0 >> 21) & 1) {
-                    intra_tb_fast(L.tile, cd0, cd2, cw, cr, lane, ((cd0 >> 12) & 3) ? lt8 : lt4, maxv, bw, tagw);
-                    HPVCD_PROF(3)
-                } else switch ((cd0 >> 12) & 3) {
-                    case 0: intra_tb<2, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); HPVCD_PROF(3) break;
-                    case 1: intra_tb<3, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); HPVCD_PROF(4) break;
-                    case 2: intra_tb<4, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); HPVCD_PROF(5) break;
-                    default: intra_tb<5, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); HPVCD_PROF(6) break;
-                }
-                i = next;
-                cd0 = nd0;
-                cd1 = nd1;
-                cd2 = nd2;
-                cw = nw;
-                cr = nr;
-                cl0 = nl0;
-                cl2 = nl2;
-                nraw = nnraw;
-                nlane = nnlane;
             }
             HPVCD_PROF(2)
         }

@@ -1124,7 +1124,15 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
         C.d_planes_b = j->mem.take<char>(C.plane_elems * sb);
         C.d_res = j->mem.take<char>(C.res_elems * rb);
         if (j->mem.used > j->mem.size) { hpvcd_job_free(j); return set_error(HPVCD_E_DEVICE, "pool overflow"); }
+        // samples of TBs without a residual are never written: they stay zero
+        if (hipMemset(C.d_res, 0, C.res_elems * rb) != hipSuccess) {
+            hpvcd_job_free(j);
+            return set_error(HPVCD_E_DEVICE, "hipMemset of the residual pool");
+        }
     }
+    // the uploads and clears above run on the null stream; the job's kernels
+    // run on the device's own non-blocking stream
+    if (hipDeviceSynchronize() != hipSuccess) { hpvcd_job_free(j); return set_error(HPVCD_E_DEVICE, "hipDeviceSynchronize"); }
     *out = j;
     return HPVCD_OK;
 }
