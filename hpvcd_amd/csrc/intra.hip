@@ -400,7 +400,8 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
                   ((s.filt && strong_on && n == 32) ? 1u << 18 : 0u) | (y0 + n == ctbh ? 1u << 19 : 0u) |
                   (x0 + n == ctbw ? 1u << 20 : 0u) | ((n <= 8 && !s.filt && s.cls != kClsDc) ? 1u << 21 : 0u) |
                   ((uint32_t)((tb.flags >> TBX_GROUP_SHIFT) & 7u) << 22) | ((uint32_t)gnext << 25) |
-                  (pub ? 1u << 28 : 0u) | (plain ? 1u << 29 : 0u);
+                  (pub ? 1u << 28 : 0u) | (plain ? 1u << 29 : 0u) |
+                  ((uint32_t)((gsz > 1 ? gsz : (n == 8 ? 4 : 1)) - 1) << 30);
             d.y = (ro & 0x1fffu) | ((uint32_t)soff << 13);
             d.z = u0 | (u1 << 13);
             d.w = top_need | (left_need << 8);
@@ -425,7 +426,8 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
 //      [22,25) size of the group of independent 4 x 4 TBs this TB leads
 //      (step leaders only:) [25,28) size of the next step  28 some TB of
 //      the step writes the unit's bottom row or right column  29 some TB of
-//      the step is planar or horizontal / vertical
+//      the step is planar or horizontal / vertical  [30,32) sixteens of
+//      samples in the step minus one (TBs of at most 8 x 8)
 //   y: [0,13) reference map offset  [13,25) sample word offset
 //   z: uniform references of the class: [0,13) u0  [13,26) u1
 struct IntraLds {
@@ -642,11 +644,9 @@ __device__ __forceinline__ void intra_step_publish(const uint16_t* __restrict__ 
 __device__ __forceinline__ int step_size(uint32_t d0) { return max(1, (int)((d0 >> 22) & 7)); }
 // Size of the step after the one d0 leads (descriptor bits 25-27).
 __device__ __forceinline__ int next_step_size(uint32_t d0) { return max(1, (int)((d0 >> 25) & 7)); }
-// Lanes holding a sample in a step of TBs of at most 8 x 8.
-__device__ __forceinline__ int step_lanes(uint32_t d0) {
-    const int g = step_size(d0);
-    return g > 1 ? 16 * g : (((d0 >> 12) & 3) ? 64 : 16);
-}
+// Whether lane l (lane16 = l / 16) holds a sample of a step of TBs of at
+// most 8 x 8.
+__device__ __forceinline__ bool step_active(uint32_t d0, int lane16) { return lane16 <= (int)(d0 >> 30); }
 
 // The lane's reference word and residual of a step of TBs of at most 8 x 8:
 // lane l takes word l of the step (a group's words are contiguous, 16 per
@@ -657,10 +657,10 @@ template <typename R>
 __device__ __forceinline__ void step_prefetch(const IntraLds& L, const R* __restrict__ resb, uint32_t d0,
                                               uint32_t d1, uint32_t l0, int lane, int lr4, int lr8, uint32_t& w,
                                               int32_t& r) {
-    const int sz = (d0 >> 12) & 3;
-    if (step_size(d0) > 1 || sz <= 1) {
+    // (a group's TBs are 4 x 4)
+    if (!((d0 >> 13) & 1)) {
         w = L.smap[min((int)((d1 >> 13) & 0xfff) + lane, kMaxCtb * kMaxCtb - 1)];
-        r = (int32_t)resb[(l0 & 0xfffu) + (sz == 1 ? lr8 : lr4)];
+        r = (int32_t)resb[(l0 & 0xfffu) + (((d0 >> 12) & 1) ? lr8 : lr4)];
     }
 }
 
@@ -945,7 +945,7 @@ __global__ void __launch_bounds__(kIntraThreads)
                         }
                         const bool b8 = (cd0 >> 12) & 1;
                         const int x = b8 ? x8 : x4, y = b8 ? y8 : y4, loff = b8 ? lt8 : lt4;
-                        const bool active = lane < step_lanes(cd0);
+                        const bool active = step_active(cd0, lane16);
                         intra_step_fast(L.tile, cl0, cl2, cw, cr, b8, (cd0 >> 29) & 1, x, y, loff, active, maxv);
                         if ((cd0 >> 28) & 1) intra_step_publish(L.tile, cl0, b8, x, y, loff, active, bw, tagw);
                         HPVCD_PROF(3)
