@@ -113,5 +113,11 @@ constexpr uint16_t TBX_HAS_RES = 1u << 8;  // residual samples exist in the resi
 // this many consecutive 4 x 4 fast-path TBs of one dependency level (they do
 // not read each other's samples and run in one step); 0 inside a group
 constexpr int TBX_GROUP_SHIFT = 9;
+// (step leaders only) some TB of the step writes the unit's bottom row or
+// right column; some TB of the step is planar or horizontal / vertical; bits
+// 14-15: size of the next step minus one
+constexpr uint16_t TBX_STEP_PUB = 1u << 12;
+constexpr uint16_t TBX_STEP_PLAIN = 1u << 13;
+constexpr int TBX_NEXT_SHIFT = 14;
 
 }  // namespace hpvcd

@@ -384,22 +384,16 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             // a step's first TB also carries the size of the step after it
             // and whether any TB of its step writes the unit's bottom row or
             // right column
+            // (step bits marked by the host on the step's first TB)
             const int gsz = max(1, (int)((tb.flags >> TBX_GROUP_SHIFT) & 7u));
-            const int gnext = e + gsz < ntb ? max(1, (int)((L.tb[e + gsz].flags >> TBX_GROUP_SHIFT) & 7u)) : 1;
-            bool pub = false, plain = false;
-            for (int m = 0; m < gsz && e + m < ntb; m++) {
-                const hpvcd_tb tm = L.tb[e + m];
-                const int nm = 1 << tm.log2n;
-                pub |= (tm.y - uy0) + nm == ctbh || (tm.x - ux0) + nm == ctbw;
-                const bool copy = (tm.flags & (HPVCD_TB_PCM | HPVCD_TB_INTER)) != 0;
-                plain |= !copy && (tm.mode == 0 || tm.mode == 10 || tm.mode == 26);
-            }
+            const uint32_t gnext = ((tb.flags >> TBX_NEXT_SHIFT) & 3u) + 1;
+            const bool pub = (tb.flags & TBX_STEP_PUB) != 0, plain = (tb.flags & TBX_STEP_PLAIN) != 0;
             uint4 d;
             d.x = (uint32_t)x0 | ((uint32_t)y0 << 6) | ((uint32_t)(log2n - 2) << 12) | ((uint32_t)s.cls << 14) |
                   ((tb.flags & TBX_HAS_RES) ? 1u << 16 : 0u) | (s.filt ? 1u << 17 : 0u) |
                   ((s.filt && strong_on && n == 32) ? 1u << 18 : 0u) | (y0 + n == ctbh ? 1u << 19 : 0u) |
                   (x0 + n == ctbw ? 1u << 20 : 0u) | ((n <= 8 && !s.filt && s.cls != kClsDc) ? 1u << 21 : 0u) |
-                  ((uint32_t)((tb.flags >> TBX_GROUP_SHIFT) & 7u) << 22) | ((uint32_t)gnext << 25) |
+                  ((uint32_t)((tb.flags >> TBX_GROUP_SHIFT) & 7u) << 22) | (gnext << 25) |
                   (pub ? 1u << 28 : 0u) | (plain ? 1u << 29 : 0u) |
                   ((uint32_t)((gsz > 1 ? gsz : (n == 8 ? 4 : 1)) - 1) << 30);
             d.y = (ro & 0x1fffu) | ((uint32_t)soff << 13);

@@ -542,12 +542,29 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                         const hpvcd_tb& T = C.tbs[tbi];
                         return T.log2n == 2 && (T.mode != 1 || (T.flags & (HPVCD_TB_PCM | HPVCD_TB_INTER)));
                     };
+                    // per step leader also: whether a TB of the step writes
+                    // the unit's bottom row or right column, whether one is
+                    // planar or H/V (not a copy), and the next step's size
+                    std::vector<size_t> leaders;
                     for (size_t a = 0; a < lt.size();) {
                         size_t e = a + 1;
                         if (groupable(lt[a].second))
                             while (e < lt.size() && e - a < 4 && lt[e].first == lt[a].first && groupable(lt[e].second)) e++;
-                        C.tbs[lt[a].second].flags |= (uint16_t)((e - a) << TBX_GROUP_SHIFT);
+                        uint16_t f = (uint16_t)((e - a) << TBX_GROUP_SHIFT);
+                        for (size_t m = a; m < e; m++) {
+                            const hpvcd_tb& M = C.tbs[lt[m].second];
+                            const int nm = 1 << M.log2n;
+                            if ((M.y - uy0) + nm == ctbh || (M.x - ux0) + nm == ctbw) f |= TBX_STEP_PUB;
+                            if (!(M.flags & (HPVCD_TB_PCM | HPVCD_TB_INTER)) && (M.mode == 0 || M.mode == 10 || M.mode == 26))
+                                f |= TBX_STEP_PLAIN;
+                        }
+                        C.tbs[lt[a].second].flags |= f;
+                        leaders.push_back(a);
                         a = e;
+                    }
+                    for (size_t k = 0; k + 1 < leaders.size(); k++) {
+                        const size_t gnext = (k + 2 < leaders.size() ? leaders[k + 2] : lt.size()) - leaders[k + 1];
+                        C.tbs[lt[leaders[k]].second].flags |= (uint16_t)((gnext - 1) << TBX_NEXT_SHIFT);
                     }
                     U.tb_list = (uint32_t)C.unit_tbs.size();
                     U.level_list = (uint32_t)C.level_off.size();
