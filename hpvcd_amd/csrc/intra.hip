@@ -426,9 +426,9 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
 //   z: uniform references of the class: [0,13) u0  [13,26) u1
 struct IntraLds {
     uint16_t tile[kLdsSamples];
-    uint32_t smap[kMaxCtb * kMaxCtb];
+    alignas(16) uint32_t smap[kMaxCtb * kMaxCtb];
     uint4 desc[kMaxUnitTbs];
-    uint16_t refmap[kMaxUnitRefs];
+    alignas(16) uint16_t refmap[kMaxUnitRefs];
     uint32_t pneed[kMaxUnitTbs];      // prefix maxima over the TB order of the neighbour samples read:
                                       // row-above words | left-column rows << 16
     uint32_t bready;                  // TBs whose neighbour samples are all in the tile
@@ -577,23 +577,28 @@ __device__ __forceinline__ void intra_tb(IntraLds& L, const R* __restrict__ resb
     wave_sync();
 }
 
-// Global -> LDS copy with kCopyBatch loads in flight per thread.
-template <typename V>
-__device__ __forceinline__ void batch_copy(V* __restrict__ dst, const V* __restrict__ src, int count, int tid) {
-    for (int e0 = tid; e0 < count; e0 += kCopyBatch * kIntraThreads) {
-        V v[kCopyBatch];
+// Global -> LDS copy of nvec 16-byte vectors with kCopyBatch loads in flight
+// per thread.
+__device__ __forceinline__ void copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, int nvec, int tid) {
+    for (int e0 = tid; e0 < nvec; e0 += kCopyBatch * kIntraThreads) {
+        uint4 v[kCopyBatch];
+#pragma unroll
+        for (int b = 0; b < kCopyBatch; b++) v[b] = src[min(e0 + b * kIntraThreads, nvec - 1)];
 #pragma unroll
         for (int b = 0; b < kCopyBatch; b++) {
             const int e = e0 + b * kIntraThreads;
-            if (e < count) v[b] = src[e];
-        }
-#pragma unroll
-        for (int b = 0; b < kCopyBatch; b++) {
-            const int e = e0 + b * kIntraThreads;
-            if (e < count) dst[e] = v[b];
+            if (e < nvec) dst[e] = v[b];
         }
     }
 }
+
+// Four residual samples as one load: 8 bytes of int16, 16 of int32.
+template <typename R> struct Quad;
+template <> struct Quad<int16_t> { using V = uint2; };
+template <> struct Quad<int32_t> { using V = uint4; };
+template <typename Q> __device__ __forceinline__ Q quad_zero();
+template <> __device__ __forceinline__ uint2 quad_zero<uint2>() { return make_uint2(0, 0); }
+template <> __device__ __forceinline__ uint4 quad_zero<uint4>() { return make_uint4(0, 0, 0, 0); }
 
 // The common step: one TB of at most 8 x 8 or a group of up to four
 // independent 4 x 4 TBs of one dependency level, references unfiltered, not
@@ -684,7 +689,7 @@ __global__ void __launch_bounds__(kIntraThreads)
             const R* __restrict__ res_pool, uint32_t* __restrict__ border, uint32_t* counter, const uint32_t* gen,
             uint32_t* err) {
     __shared__ IntraLds L;
-    __shared__ R resb[kMaxCtb * kMaxCtb];
+    __shared__ __align__(16) R resb[kMaxCtb * kMaxCtb];
     const int tid = threadIdx.x, wave = uni(tid >> 6), lane = tid & 63;
     const uint32_t tag = *gen & 0xffffu;
 
@@ -715,26 +720,33 @@ __global__ void __launch_bounds__(kIntraThreads)
         HPVCD_TL(ui, 0)
 
         // stage the unit: TB descriptors, reference maps and words (all
-        // prepared before the wavefront) and residuals; every thread keeps
-        // kCopyBatch loads in flight
+        // prepared before the wavefront) and residuals, in 16-byte (words,
+        // maps) and 4-sample (residual) vectors; every thread keeps
+        // kCopyBatch loads in flight.  Unit map offsets are multiples of 8
+        // entries, sample word offsets of 16; residual rows of a plane start
+        // at multiples of 4 samples and its width is a multiple of 4.
         {
-            batch_copy(reinterpret_cast<uint32_t*>(L.desc), reinterpret_cast<const uint32_t*>(desc_g + U.tb_list), 4 * ntb, tid);
-            batch_copy(L.refmap, rmap_g + U.rbase, (int)U.nref, tid);
-            batch_copy(L.smap, smap_g + U.sbase, (int)U.nsamp, tid);
-            const int nres = ctbw * ctbh;
-            for (int e0 = tid; e0 < nres; e0 += kCopyBatch * kIntraThreads) {
-                R v[kCopyBatch];
+            copy16(L.desc, desc_g + U.tb_list, ntb, tid);
+            copy16(reinterpret_cast<uint4*>(L.refmap), reinterpret_cast<const uint4*>(rmap_g + U.rbase),
+                   (int)(U.nref + 7) >> 3, tid);
+            copy16(reinterpret_cast<uint4*>(L.smap), reinterpret_cast<const uint4*>(smap_g + U.sbase), (int)U.nsamp >> 2,
+                   tid);
+            using Q = typename Quad<R>::V;
+            const int nres = ctbw * ctbh, nq = nres >> 2;
+            for (int e0 = tid; e0 < nq; e0 += kCopyBatch * kIntraThreads) {
+                Q v[kCopyBatch];
 #pragma unroll
                 for (int b = 0; b < kCopyBatch; b++) {
-                    const int e = e0 + b * kIntraThreads;
+                    const int e = 4 * (e0 + b * kIntraThreads);
                     const int ry = e >> ctbw_log2, rx = e - (ry << ctbw_log2);
                     const int px = ux0 + rx, py = uy0 + ry;
-                    v[b] = (e < nres && px < pw && py < ph) ? rp[(size_t)py * pw + px] : (R)0;
+                    const Q q = *reinterpret_cast<const Q*>(rp + (e < nres && px < pw && py < ph ? (size_t)py * pw + px : 0));
+                    v[b] = (e < nres && px < pw && py < ph) ? q : quad_zero<Q>();
                 }
 #pragma unroll
                 for (int b = 0; b < kCopyBatch; b++) {
-                    const int e = e0 + b * kIntraThreads;
-                    if (e < nres) resb[(e >> ctbw_log2) * kMaxCtb + (e & (ctbw - 1))] = v[b];
+                    const int e = 4 * (e0 + b * kIntraThreads);
+                    if (e < nres) *reinterpret_cast<Q*>(resb + (e >> ctbw_log2) * kMaxCtb + (e & (ctbw - 1))) = v[b];
                 }
             }
             if (P.inter) {

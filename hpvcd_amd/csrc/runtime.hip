@@ -576,6 +576,9 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                     // per TB: reference map offset, sample word offset (a
                     // multiple of 16) and level
                     uint32_t roff = 0, soff = 0;
+                    // (map offsets of a unit are multiples of 8 entries: the
+                    // wavefront stages them in 16-byte vectors)
+                    C.rmap_words = (C.rmap_words + 7) & ~(size_t)7;
                     U.sbase = (uint32_t)C.smap_words;
                     U.rbase = (uint32_t)C.rmap_words;
                     for (auto& e : lt) {
@@ -1081,7 +1084,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
                  bytes_of<uint32_t>(C.tickets.size()) * 2 + bytes_of<uint32_t>(C.unit_tbs.size() + 1) +
                  bytes_of<uint16_t>(C.level_off.size() + 1) + bytes_of<uint32_t>(C.sync_words) +
                  bytes_of<uint32_t>(C.unit_refoff.size() + 1) +
-                 bytes_of<uint32_t>(C.smap_words + 1) + bytes_of<uint16_t>(C.rmap_words + 1) +
+                 bytes_of<uint32_t>(C.smap_words + 1) + bytes_of<uint16_t>(C.rmap_words + 8) +
                  bytes_of<uint4>(C.unit_tbs.size() + 1) +
                  bytes_of<uint32_t>(C.units.size() * kBorderWords + 1) + bytes_of<uint64_t>(C.tbs.size() + 1) +
                  2 * ((C.plane_elems * sb + 255) & ~(size_t)255) + ((C.res_elems * rb + 255) & ~(size_t)255) + 8 * 256;
@@ -1130,7 +1133,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
         C.d_sync = j->mem.take<uint32_t>(C.sync_words);
         C.d_border = j->mem.take<uint32_t>(C.units.size() * kBorderWords + 1);
         C.d_smap = j->mem.take<uint32_t>(C.smap_words + 1);
-        C.d_rmap = j->mem.take<uint16_t>(C.rmap_words + 1);
+        C.d_rmap = j->mem.take<uint16_t>(C.rmap_words + 8);
         C.d_desc = j->mem.take<uint4>(C.unit_tbs.size() + 1);
         if (hipMemset(C.d_sync, 0, C.sync_words * sizeof(uint32_t)) != hipSuccess ||
             hipMemset(C.d_border, 0, (C.units.size() * kBorderWords + 1) * sizeof(uint32_t)) != hipSuccess) {
