@@ -577,20 +577,11 @@ __device__ __forceinline__ void intra_tb(IntraLds& L, const R* __restrict__ resb
     wave_sync();
 }
 
-// Global -> LDS copy of nvec 16-byte vectors with kCopyBatch loads in flight
-// per thread.
-__device__ __forceinline__ void copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, int nvec, int tid) {
-    for (int e0 = tid; e0 < nvec; e0 += kCopyBatch * kIntraThreads) {
-        uint4 v[kCopyBatch];
-#pragma unroll
-        for (int b = 0; b < kCopyBatch; b++) v[b] = src[min(e0 + b * kIntraThreads, nvec - 1)];
-#pragma unroll
-        for (int b = 0; b < kCopyBatch; b++) {
-            const int e = e0 + b * kIntraThreads;
-            if (e < nvec) dst[e] = v[b];
-        }
-    }
-}
+// Staging of a unit: the 16-byte vectors of its descriptors, reference map
+// and reference words (at most 256 + 4352 / 8 + 4096 / 4), and its residual
+// block in 4-sample vectors, all loads of a thread in flight at once.
+constexpr int kStageVec = (kMaxUnitTbs + kMaxUnitRefs / 8 + kMaxCtb * kMaxCtb / 4 + kIntraThreads - 1) / kIntraThreads;
+constexpr int kStageRes = kMaxCtb * kMaxCtb / 4 / kIntraThreads;
 
 // Four residual samples as one load: 8 bytes of int16, 16 of int32.
 template <typename R> struct Quad;
@@ -721,33 +712,44 @@ __global__ void __launch_bounds__(kIntraThreads)
 
         // stage the unit: TB descriptors, reference maps and words (all
         // prepared before the wavefront) and residuals, in 16-byte (words,
-        // maps) and 4-sample (residual) vectors; every thread keeps
-        // kCopyBatch loads in flight.  Unit map offsets are multiples of 8
-        // entries, sample word offsets of 16; residual rows of a plane start
-        // at multiples of 4 samples and its width is a multiple of 4.
+        // maps) and 4-sample (residual) vectors.  Unit map offsets are
+        // multiples of 8 entries, sample word offsets of 16; residual rows of
+        // a plane start at multiples of 4 samples and its width is a
+        // multiple of 4.
         {
-            copy16(L.desc, desc_g + U.tb_list, ntb, tid);
-            copy16(reinterpret_cast<uint4*>(L.refmap), reinterpret_cast<const uint4*>(rmap_g + U.rbase),
-                   (int)(U.nref + 7) >> 3, tid);
-            copy16(reinterpret_cast<uint4*>(L.smap), reinterpret_cast<const uint4*>(smap_g + U.sbase), (int)U.nsamp >> 2,
-                   tid);
+            // one batch: every thread issues all its loads, then stores
+            const int v1 = ntb, v2 = v1 + ((int)(U.nref + 7) >> 3), v3 = v2 + ((int)U.nsamp >> 2);
+            const uint4* rm16 = reinterpret_cast<const uint4*>(rmap_g + U.rbase);
+            const uint4* sm16 = reinterpret_cast<const uint4*>(smap_g + U.sbase);
             using Q = typename Quad<R>::V;
-            const int nres = ctbw * ctbh, nq = nres >> 2;
-            for (int e0 = tid; e0 < nq; e0 += kCopyBatch * kIntraThreads) {
-                Q v[kCopyBatch];
+            const int nres = ctbw * ctbh;
+            uint4 va[kStageVec];
+            Q vr[kStageRes];
 #pragma unroll
-                for (int b = 0; b < kCopyBatch; b++) {
-                    const int e = 4 * (e0 + b * kIntraThreads);
-                    const int ry = e >> ctbw_log2, rx = e - (ry << ctbw_log2);
-                    const int px = ux0 + rx, py = uy0 + ry;
-                    const Q q = *reinterpret_cast<const Q*>(rp + (e < nres && px < pw && py < ph ? (size_t)py * pw + px : 0));
-                    v[b] = (e < nres && px < pw && py < ph) ? q : quad_zero<Q>();
-                }
+            for (int b = 0; b < kStageVec; b++) {
+                const int e = min(tid + b * kIntraThreads, v3 - 1);
+                va[b] = e < v1 ? desc_g[U.tb_list + e] : (e < v2 ? rm16[e - v1] : sm16[e - v2]);
+            }
 #pragma unroll
-                for (int b = 0; b < kCopyBatch; b++) {
-                    const int e = 4 * (e0 + b * kIntraThreads);
-                    if (e < nres) *reinterpret_cast<Q*>(resb + (e >> ctbw_log2) * kMaxCtb + (e & (ctbw - 1))) = v[b];
-                }
+            for (int b = 0; b < kStageRes; b++) {
+                const int e = 4 * (tid + b * kIntraThreads);
+                const int ry = e >> ctbw_log2, rx = e - (ry << ctbw_log2);
+                const int px = ux0 + rx, py = uy0 + ry;
+                const bool in = e < nres && px < pw && py < ph;
+                const Q q = *reinterpret_cast<const Q*>(rp + (in ? (size_t)py * pw + px : 0));
+                vr[b] = in ? q : quad_zero<Q>();
+            }
+#pragma unroll
+            for (int b = 0; b < kStageVec; b++) {
+                const int e = tid + b * kIntraThreads;
+                if (e < v1) L.desc[e] = va[b];
+                else if (e < v2) reinterpret_cast<uint4*>(L.refmap)[e - v1] = va[b];
+                else if (e < v3) reinterpret_cast<uint4*>(L.smap)[e - v2] = va[b];
+            }
+#pragma unroll
+            for (int b = 0; b < kStageRes; b++) {
+                const int e = 4 * (tid + b * kIntraThreads);
+                if (e < nres) *reinterpret_cast<Q*>(resb + (e >> ctbw_log2) * kMaxCtb + (e & (ctbw - 1))) = vr[b];
             }
             if (P.inter) {
                 // motion-compensated prediction of the unit, the predictor of its inter TBs
