@@ -92,7 +92,8 @@ constexpr uint16_t RJ_EP = 1u << 10;        // extended precision processing
 constexpr uint16_t RJ_CHROMA = 1u << 11;    // component 1 or 2
 constexpr uint16_t RJ_CCP = 1u << 12;       // cross-component prediction from luma (res_scale != 0)
 
-// One (picture, plane, CTB) work unit of the intra reconstruction wavefront.
+// One (picture, plane, CTB) work unit of the intra reconstruction wavefront
+// (its intra and I_PCM TBs; inter TBs arrive reconstructed).
 struct DevUnit {
     uint32_t pic;
     uint16_t cx, cy;        // CTB coordinates
@@ -105,6 +106,8 @@ struct DevUnit {
     uint32_t nsamp;         // reference words of the unit (sum of its TB areas)
     uint32_t rbase;         // first reference map entry of the unit
     uint32_t nref;          // reference map entries (sum of 4n + 1 over its TBs)
+    uint64_t inter_bottom;  // bottom-row samples of inter TBs (bit = column)
+    uint64_t inter_right;   // right-column samples of inter TBs (bit = row)
 };
 
 // TB flag bits added by the runtime (upper byte of hpvcd_tb.flags).

@@ -359,14 +359,13 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             if (left_bits) left_need = (uint32_t)(y0 + n2 - ((int)__builtin_ctzll(left_bits) << lgv));
             if (corner && y0 > 0) left_need = max(left_need, (uint32_t)y0);
         }
-        // I_PCM and inter blocks read no neighbours: their words name, through
-        // the angular form with fraction 0, the zero slot (PCM: the samples
-        // arrive as the residual) or the sample's own tile slot, which holds
-        // the motion-compensated prediction (inter: pred + residual)
-        const bool pcm = (tb.flags & HPVCD_TB_PCM) != 0, inter = (tb.flags & HPVCD_TB_INTER) != 0;
-        if (pcm || inter) top_need = left_need = 0;
+        // I_PCM blocks read no neighbours: their words name, through the
+        // angular form with fraction 0, the zero slot (the samples arrive as
+        // the residual)
+        const bool pcm = (tb.flags & HPVCD_TB_PCM) != 0;
+        if (pcm) top_need = left_need = 0;
         TbShape s = tb_shape(log2n, mode, plane, pflags, chroma_idc, angles, inv_angles);
-        if (pcm || inter) {
+        if (pcm) {
             s.cls = kClsAngular;
             s.filt = false;
         }
@@ -402,11 +401,8 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             desc_g[tb_list + e] = d;
         }
         uint32_t* dst = smap_g + sbase + soff;
-        for (int p = gl; p < n * n; p += G) {
-            const uint32_t own = (uint32_t)(kTopW + (y0 + (p >> log2n)) * kRowW + x0 + (p & (n - 1)) + 1);
-            dst[p] = pcm ? (uint32_t)kZero | ((uint32_t)kZero << 13)
-                         : inter ? own | (own << 13) : sample_word(s, p & (n - 1), p >> log2n, Ms);
-        }
+        for (int p = gl; p < n * n; p += G)
+            dst[p] = pcm ? (uint32_t)kZero | ((uint32_t)kZero << 13) : sample_word(s, p & (n - 1), p >> log2n, Ms);
     }
     __syncthreads();
     HPVCD_PREP_T(5)
@@ -879,6 +875,14 @@ __global__ void __launch_bounds__(kIntraThreads)
             HPVCD_PROF(4)
         } else {
             // ---- prediction ----
+            // inter samples on the unit's bottom row and right column are
+            // final already: publish them before any intra TB
+            if (U.inter_bottom | U.inter_right) {
+                if ((U.inter_bottom >> lane) & 1ull)
+                    border_store(bw + lane, tagw | L.tile[kTopW + (ctbh - 1) * kRowW + lane + 1]);
+                if ((U.inter_right >> lane) & 1ull)
+                    border_store(bw + kMaxCtb + lane, tagw | L.tile[kTopW + lane * kRowW + ctbw]);
+            }
             // the descriptor of the next TB and, for TBs of up to 8 x 8, its
             // reference words and residuals are loaded while the current TB runs
             uint32_t cd0 = 0, cd1 = 0, cd2 = 0, cw = 0, cl0 = 0, cl2 = 0;

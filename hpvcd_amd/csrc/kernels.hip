@@ -295,9 +295,11 @@ __global__ void __launch_bounds__(256) k_act(const DevPic* __restrict__ pics, co
 // the reference's 14-bit intermediates are.  The separable form equals the
 // reference's full-sample / one-direction special cases exactly (the
 // identity tap 64 is a shift by 6 on either side).  The lists then combine
-// (uni / bi / explicit weighted, src/mc.rs:538-733) into the current
-// picture's plane; reference samples are the final planes of earlier
-// pictures of the job, read in place.
+// (uni / bi / explicit weighted, src/mc.rs:538-733) and the TB's residual
+// is added (src/decode.rs:4724: Clip(pred + res); the residual pool is zero
+// where a TB has none), so an inter CU leaves this kernel reconstructed and
+// the wavefront only publishes its border samples.  Reference samples are
+// the final planes of earlier pictures of the job, read in place.
 // ---------------------------------------------------------------------------
 __constant__ int8_t kLumaFilter[4][8] = {{0, 0, 0, 64, 0, 0, 0, 0},
                                          {-1, 4, -10, 58, 17, -5, 1, 0},
@@ -309,10 +311,10 @@ __constant__ int8_t kChromaFilter[8][4] = {{0, 64, 0, 0},    {-2, 58, 10, -2}, {
 constexpr int kMcBlock = 16;
 constexpr int kMcWin = kMcBlock + 7;
 
-template <typename T>
+template <typename T, typename R>
 __global__ void __launch_bounds__(256) k_mc(const DevPic* __restrict__ pics, const DevPu* __restrict__ pus,
                                             const DevMcTask* __restrict__ tasks, T* __restrict__ cur,
-                                            const T* __restrict__ refs) {
+                                            const T* __restrict__ refs, const R* __restrict__ res) {
     __shared__ int32_t win[kMcWin * kMcWin];
     __shared__ int32_t rows[kMcWin * kMcBlock];
     const int tid = threadIdx.x;
@@ -397,7 +399,9 @@ __global__ void __launch_bounds__(256) k_mc(const DevPic* __restrict__ pics, con
         } else {
             v = 1 << (bd - 1);  // no usable reference: mid-grey (src/decode.rs:7206)
         }
-        cur[P.plane_off[pl] + (size_t)py * pw + px] = (T)clampi(v, 0, maxv);
+        const int32_t pv = clampi(v, 0, maxv);
+        const int32_t rv = (int32_t)res[P.res_off[pl] + (size_t)py * pw + px];
+        cur[P.plane_off[pl] + (size_t)py * pw + px] = (T)clampi(wadd(pv, rv), 0, maxv);
     }
 }
 

@@ -485,12 +485,23 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         int nplanes = P.chroma_idc ? 3 : 1;
         uint32_t nunits = (uint32_t)(nplanes * D.ctb_cols * D.ctb_rows);
         std::vector<std::vector<uint32_t>> utbs(nunits);
+        // inter TBs leave K1c reconstructed: the wavefront only publishes
+        // their samples on the unit's bottom row and right column
+        std::vector<uint64_t> inter_bottom(nunits, 0), inter_right(nunits, 0);
         for (uint32_t t = 0; t < P.tb_count; t++) {
             const hpvcd_tb& T = C.tbs[tb0 + t];
             int sw = T.comp ? D.sub_w_log2 : 0, sh = T.comp ? D.sub_h_log2 : 0;
             int cx = (T.x << sw) >> P.log2_ctb, cy = (T.y << sh) >> P.log2_ctb;
             uint32_t u = (uint32_t)((T.comp * D.ctb_rows + cy) * D.ctb_cols + cx);
-            utbs[u].push_back(tb0 + t);
+            if (T.flags & HPVCD_TB_INTER) {
+                const int ctbw = (1 << P.log2_ctb) >> sw, ctbh = (1 << P.log2_ctb) >> sh, n = 1 << T.log2n;
+                const int x0 = T.x - cx * ctbw, y0 = T.y - cy * ctbh;
+                const uint64_t run = (n >= 64 ? ~0ull : ((1ull << n) - 1ull));
+                if (y0 + n == ctbh) inter_bottom[u] |= run << x0;
+                if (x0 + n == ctbw) inter_right[u] |= run << y0;
+            } else {
+                utbs[u].push_back(tb0 + t);
+            }
             if (T.flags & TBX_HAS_RES) {
                 int pass = T.res_scale != 0 ? 1 : 0;
                 per_pic_res[pass][T.log2n - 2][lp].push_back(tb0 + t);
@@ -505,6 +516,8 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                     DevUnit U;
                     std::memset(&U, 0, sizeof(U));
                     U.pic = lp; U.cx = (uint16_t)cx; U.cy = (uint16_t)cy; U.plane = (uint8_t)pl;
+                    U.inter_bottom = inter_bottom[u];
+                    U.inter_right = inter_right[u];
                     // dependency levels inside the unit
                     int sw = pl ? D.sub_w_log2 : 0, sh = pl ? D.sub_h_log2 : 0;
                     int ctbw = ctb >> sw, ctbh = ctb >> sh, cgw = ctbw / 4, cgh = ctbh / 4;
@@ -536,15 +549,14 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                     if (lt.size() > (size_t)kMaxUnitTbs) return set_error(HPVCD_E_INVALID, "too many TBs in one CTB");
                     U.ntb = (uint16_t)lt.size();
                     // groups of up to four consecutive independent (same
-                    // level) 4x4 TBs that take the fast path: not DC, or a
-                    // copy (PCM / inter) TB
+                    // level) 4x4 TBs that take the fast path: not DC, or I_PCM
                     auto groupable = [&](uint32_t tbi) {
                         const hpvcd_tb& T = C.tbs[tbi];
-                        return T.log2n == 2 && (T.mode != 1 || (T.flags & (HPVCD_TB_PCM | HPVCD_TB_INTER)));
+                        return T.log2n == 2 && (T.mode != 1 || (T.flags & HPVCD_TB_PCM));
                     };
                     // per step leader also: whether a TB of the step writes
                     // the unit's bottom row or right column, whether one is
-                    // planar or H/V (not a copy), and the next step's size
+                    // planar or H/V (not I_PCM), and the next step's size
                     std::vector<size_t> leaders;
                     for (size_t a = 0; a < lt.size();) {
                         size_t e = a + 1;
@@ -555,7 +567,7 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                             const hpvcd_tb& M = C.tbs[lt[m].second];
                             const int nm = 1 << M.log2n;
                             if ((M.y - uy0) + nm == ctbh || (M.x - ux0) + nm == ctbw) f |= TBX_STEP_PUB;
-                            if (!(M.flags & (HPVCD_TB_PCM | HPVCD_TB_INTER)) && (M.mode == 0 || M.mode == 10 || M.mode == 26))
+                            if (!(M.flags & HPVCD_TB_PCM) && (M.mode == 0 || M.mode == 10 || M.mode == 26))
                                 f |= TBX_STEP_PLAIN;
                         }
                         C.tbs[lt[a].second].flags |= f;
@@ -796,7 +808,7 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
             const uint32_t t0 = C.pic_mc_begin[p0], t1 = C.pic_mc_begin[p1];
             if (t1 > t0) {
                 mark(HPVCD_KERNEL_MC);
-                k_mc<T><<<t1 - t0, 256, 0, s>>>(C.d_pics, C.d_pus, C.d_mc_tasks + t0, pa, pb);
+                k_mc<T, R><<<t1 - t0, 256, 0, s>>>(C.d_pics, C.d_pus, C.d_mc_tasks + t0, pa, pb, res);
             }
         }
         mark(HPVCD_KERNEL_INTRA);
