@@ -9,9 +9,9 @@
 //                         CTU wavefront; Y, Cb and Cr planes are independent
 //                         units (reference: src/intra.rs:84-374,
 //                         src/decode.rs:4641-4990, 5170-5280)
-//   k_deblock_*<T>        vertical then horizontal edge passes, luma then
-//                         chroma (reference: src/decode.rs:1774-2143,
-//                         src/deblock.rs:216-330)
+//   k_deblock<T>          vertical then horizontal edge passes, each over
+//                         luma and chroma together (reference:
+//                         src/decode.rs:1774-2143, src/deblock.rs:216-330)
 //   k_sao<T>              band / edge offset, out of place from the
 //                         deblocked planes (reference: src/decode.rs:2258,
 //                         src/sao.rs:179-470)
@@ -543,14 +543,14 @@ __device__ void luma_segment(T* base, ptrdiff_t step_tap, ptrdiff_t step_line, i
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_deblock_luma(const DevPic* __restrict__ pics, const uint32_t* __restrict__ info,
-                                                      T* __restrict__ samples, int vertical, uint32_t pic_base) {
+__device__ __forceinline__ void deblock_luma(const DevPic* __restrict__ pics, const uint32_t* __restrict__ info,
+                                             T* __restrict__ samples, int vertical, uint32_t pic_base, uint32_t bx) {
     const DevPic& P = pics[pic_base + blockIdx.y];
     if (!P.deblock_on) return;
     const int edge_max = vertical ? P.w : P.h, scan_max = vertical ? P.h : P.w;
     const int n_e = edge_max >= 12 ? (edge_max - 4) / 8 : 0;
     const int n_s = scan_max / 4;
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t idx = bx * blockDim.x + threadIdx.x;
     if (idx >= (uint32_t)(n_e * n_s)) return;
     int e, s;
     if (vertical) { s = idx / n_e; e = idx - s * n_e; } else { e = idx / n_s; s = idx - e * n_s; }
@@ -580,16 +580,15 @@ __global__ void __launch_bounds__(256) k_deblock_luma(const DevPic* __restrict__
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_deblock_chroma(const DevPic* __restrict__ pics,
-                                                        const uint32_t* __restrict__ info, T* __restrict__ samples,
-                                                        int vertical, uint32_t pic_base) {
+__device__ __forceinline__ void deblock_chroma(const DevPic* __restrict__ pics, const uint32_t* __restrict__ info,
+                                               T* __restrict__ samples, int vertical, uint32_t pic_base, uint32_t bx) {
     const DevPic& P = pics[pic_base + blockIdx.y];
     if (!P.deblock_on || P.chroma_idc == 0) return;
     const int cw = P.cw, ch = P.ch;
     const int edge_max = vertical ? cw : ch, scan_max = vertical ? ch : cw;
     const int n_e = edge_max >= 10 ? (edge_max - 2) / 8 : 0;
     const int n_s = scan_max / 4;
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t idx = bx * blockDim.x + threadIdx.x;
     if (idx >= (uint32_t)(n_e * n_s)) return;
     int e, s;
     if (vertical) { s = idx / n_e; e = idx - s * n_e; } else { e = idx / n_s; s = idx - e * n_s; }
@@ -638,6 +637,17 @@ __global__ void __launch_bounds__(256) k_deblock_chroma(const DevPic* __restrict
             if (wq) *qq = (T)clampi(q0 - delta, 0, maxv);
         }
     }
+}
+
+// One pass (vertical or horizontal edges) over luma and chroma together: the
+// first luma_blocks blocks take luma segments, the rest chroma (the planes'
+// edges are independent; horizontal edges run after vertical ones).
+template <typename T>
+__global__ void __launch_bounds__(256) k_deblock(const DevPic* __restrict__ pics, const uint32_t* __restrict__ info,
+                                                 T* __restrict__ samples, int vertical, uint32_t pic_base,
+                                                 uint32_t luma_blocks) {
+    if (blockIdx.x < luma_blocks) deblock_luma<T>(pics, info, samples, vertical, pic_base, blockIdx.x);
+    else deblock_chroma<T>(pics, info, samples, vertical, pic_base, blockIdx.x - luma_blocks);
 }
 
 // ---------------------------------------------------------------------------
@@ -878,8 +888,7 @@ HPVCD_INST_INTRA(uint16_t, int16_t)
 HPVCD_INST_INTRA(uint16_t, int32_t)
 
 #define HPVCD_INST_T(T)                                                                                          \
-    template __global__ void k_deblock_luma<T>(const DevPic*, const uint32_t*, T*, int, uint32_t);               \
-    template __global__ void k_deblock_chroma<T>(const DevPic*, const uint32_t*, T*, int, uint32_t);             \
+    template __global__ void k_deblock<T>(const DevPic*, const uint32_t*, T*, int, uint32_t, uint32_t);          \
     template __global__ void k_sao<T>(const DevPic*, const uint32_t*, const hpvcd_sao*, const T*, T*, uint32_t); \
     template __global__ void k_rgb<T>(const DevPic*, const T*, uint8_t*, uint32_t);
 HPVCD_INST_T(uint8_t)

@@ -830,10 +830,12 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
             uint32_t lh = plain ? (uint32_t)((C.pics[p0].h >= 12 ? (C.pics[p0].h - 4) / 8 : 0) * (C.pics[p0].w / 4)) : C.max_luma_h;
             uint32_t cv = plain ? (uint32_t)((C.pics[p0].cw >= 10 ? (C.pics[p0].cw - 2) / 8 : 0) * (C.pics[p0].ch / 4)) : C.max_chroma_v;
             uint32_t ch = plain ? (uint32_t)((C.pics[p0].ch >= 10 ? (C.pics[p0].ch - 2) / 8 : 0) * (C.pics[p0].cw / 4)) : C.max_chroma_h;
-            if (lv) k_deblock_luma<T><<<dim3(cdiv(lv, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 1, p0);
-            if (lh) k_deblock_luma<T><<<dim3(cdiv(lh, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 0, p0);
-            if (cv) k_deblock_chroma<T><<<dim3(cdiv(cv, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 1, p0);
-            if (ch) k_deblock_chroma<T><<<dim3(cdiv(ch, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 0, p0);
+            if (lv + cv)
+                k_deblock<T><<<dim3(cdiv(lv, 256) + cdiv(cv, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 1, p0,
+                                                                                 cdiv(lv, 256));
+            if (lh + ch)
+                k_deblock<T><<<dim3(cdiv(lh, 256) + cdiv(ch, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 0, p0,
+                                                                                 cdiv(lh, 256));
             mark(-1);
         }
         if (st >= HPVCD_STAGE_SAO) {

@@ -35,7 +35,7 @@ KERNEL_SYMBOLS = {
     "intra": "k_intra<",
     "prep": "k_prep(",
     "residual": "k_residual<",
-    "deblock": "k_deblock_",
+    "deblock": "k_deblock<",
     "sao": "k_sao<",
     "rgb": "k_rgb<",
     "mc": "k_mc<",
