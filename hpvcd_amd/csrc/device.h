@@ -110,6 +110,15 @@ struct DevUnit {
     uint64_t inter_right;   // right-column samples of inter TBs (bit = row)
 };
 
+// The fields of a TB record K2a reads, copied by the host in each unit's TB
+// order (one coalesced load per TB).
+struct PrepTb {
+    uint16_t x, y;
+    uint8_t log2n, mode;
+    uint16_t flags;
+    uint32_t order;
+};
+
 // TB flag bits added by the runtime (upper byte of hpvcd_tb.flags).
 constexpr uint16_t TBX_HAS_RES = 1u << 8;  // residual samples exist in the residual pool
 // bits 9-11: in the wavefront's TB order of its unit, the TB leads a group of

@@ -221,7 +221,7 @@ constexpr int kWin = 2 * kMaxCtb / 4 + 1;
 struct PrepLds {
     uint32_t ord[kWin * kWin];
     uint32_t inf[kWin * kWin];
-    hpvcd_tb tb[kMaxUnitTbs];
+    PrepTb tb[kMaxUnitTbs];
     uint32_t ro[kMaxUnitTbs];
     uint16_t rmap[kMaxUnitRefs];
     uint16_t job[kMaxUnitTbs];       // 4x4 TBs, then 8x8 TBs, then the others
@@ -231,8 +231,7 @@ struct PrepLds {
 constexpr int kPrepThreads = 256;
 
 __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict__ pics, const DevUnit* __restrict__ units,
-                                                       const hpvcd_tb* __restrict__ tbs,
-                                                       const uint32_t* __restrict__ unit_tbs,
+                                                       const PrepTb* __restrict__ unit_recs,
                                                        const uint32_t* __restrict__ unit_refoff,
                                                        const uint32_t* __restrict__ cell_order,
                                                        const uint32_t* __restrict__ cell_info, uint4* __restrict__ desc_g,
@@ -261,11 +260,30 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
     HPVCD_PREP_T(0)
     if (tid < 33) angles[tid] = kIntraAngle[tid];
     if (tid < 15) inv_angles[tid] = kInvAngle[tid];
-    for (int c = tid; c < kWin * kWin; c += kPrepThreads) {
-        const int gx = gx0 + c % kWin, gy = gy0 + c / kWin;
-        const bool in = gx >= 0 && gy >= 0 && gx < gw && gy < gh_cells;
-        L.ord[c] = in ? ord_g[gy * gw + gx] : 0xffffffffu;
-        L.inf[c] = (in && need_info) ? inf_g[gy * gw + gx] : 0u;
+    {
+        // every load of a thread in flight at once (cells outside the
+        // picture load cell 0 and are stored as never available)
+        constexpr int kPer = (kWin * kWin + kPrepThreads - 1) / kPrepThreads;
+        uint32_t vo[kPer], vi[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int c = min(tid + k * kPrepThreads, kWin * kWin - 1);
+            const int gx = gx0 + c % kWin, gy = gy0 + c / kWin;
+            const bool in = gx >= 0 && gy >= 0 && gx < gw && gy < gh_cells;
+            const int g = in ? gy * gw + gx : 0;
+            vo[k] = ord_g[g];
+            vi[k] = need_info ? inf_g[g] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+            const int c = tid + k * kPrepThreads;
+            if (c < kWin * kWin) {
+                const int gx = gx0 + c % kWin, gy = gy0 + c / kWin;
+                const bool in = gx >= 0 && gy >= 0 && gx < gw && gy < gh_cells;
+                L.ord[c] = in ? vo[k] : 0xffffffffu;
+                L.inf[c] = in ? vi[k] : 0u;
+            }
+        }
     }
     if (tid == 0) {
         L.n4 = 0;
@@ -276,7 +294,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
     // 4x4 TBs go four to a wave (16 lanes each), 8x8 TBs two (32 lanes), the
     // others one
     for (int e = tid; e < ntb; e += kPrepThreads) {
-        const hpvcd_tb t = tbs[unit_tbs[tb_list + e]];
+        const PrepTb t = unit_recs[tb_list + e];
         L.tb[e] = t;
         L.ro[e] = unit_refoff[tb_list + e];
         if (t.log2n == 2) L.job[atomicAdd(&L.n4, 1u)] = (uint16_t)e;
@@ -307,7 +325,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
         const bool has = slot < end;
         const int e = L.job[has ? slot : first];
         const bool small = lg < 6;
-        const hpvcd_tb tb = L.tb[e];
+        const PrepTb tb = L.tb[e];
         const int log2n = tb.log2n, mode = tb.mode, n = 1 << log2n, n2 = 2 * n;
         const int x = tb.x, y = tb.y, x0 = x - ux0, y0 = y - uy0;
         const uint32_t order = tb.order, ro = L.ro[e];

@@ -248,6 +248,7 @@ struct ClassJob {
     std::vector<hpvcd_sao> sao;
     std::vector<DevUnit> units;
     std::vector<uint32_t> tickets, plain_tickets, unit_tbs;
+    std::vector<PrepTb> unit_recs;                 // K2a's fields of each unit_tbs entry
     std::vector<uint16_t> level_off;
     std::vector<uint32_t> unit_refoff;             // per unit TB entry: map offset | sample offset / 16 << 13 | level << 21
     size_t smap_words = 0, rmap_words = 0;
@@ -273,7 +274,8 @@ struct ClassJob {
     uint32_t *d_tb_pic = nullptr, *d_coeffs = nullptr, *d_cells = nullptr, *d_order = nullptr;
     hpvcd_sao* d_sao = nullptr;
     DevUnit* d_units = nullptr;
-    uint32_t *d_tickets = nullptr, *d_plain_tickets = nullptr, *d_unit_tbs = nullptr;
+    uint32_t *d_tickets = nullptr, *d_plain_tickets = nullptr;
+    PrepTb* d_unit_recs = nullptr;
     uint16_t* d_level_off = nullptr;
     uint32_t* d_unit_refoff = nullptr;
     uint32_t* d_smap = nullptr;   // per-sample reference words
@@ -598,6 +600,8 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                         if (roff + 4 * n + 1 > (uint32_t)kMaxUnitRefs || soff + n * n > (uint32_t)(kMaxCtb * kMaxCtb))
                             return set_error(HPVCD_E_INVALID, "too many reference samples in one CTB");
                         C.unit_tbs.push_back(e.second);
+                        const hpvcd_tb& R = C.tbs[e.second];
+                        C.unit_recs.push_back(PrepTb{R.x, R.y, R.log2n, R.mode, R.flags, R.order});
                         C.unit_refoff.push_back(roff | ((soff >> 4) << 13) | ((uint32_t)e.first << 21));
                         roff += 4 * n + 1;
                         soff += n * n;
@@ -781,7 +785,7 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
             // stream, concurrently with the residual stage
             HIP_TRY_V(hipEventRecord(g->fork, s));
             HIP_TRY_V(hipStreamWaitEvent(g->side, g->fork, 0));
-            k_prep<<<nunits, kPrepThreads, 0, g->side>>>(C.d_pics, C.d_units, C.d_tbs, C.d_unit_tbs, C.d_unit_refoff,
+            k_prep<<<nunits, kPrepThreads, 0, g->side>>>(C.d_pics, C.d_units, C.d_unit_recs, C.d_unit_refoff,
                                                          C.d_order, C.d_cells, C.d_desc, C.d_rmap, C.d_smap);
             HIP_TRY_V(hipEventRecord(g->join, g->side));
         }
@@ -1095,7 +1099,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
         total += bytes_of<DevPic>(C.pics.size()) + bytes_of<hpvcd_tb>(C.tbs.size()) + bytes_of<uint32_t>(C.tb_pic.size()) +
                  bytes_of<uint32_t>(C.coeffs.size() + 1) + bytes_of<uint32_t>(C.cells.size()) * 2 +
                  bytes_of<hpvcd_sao>(C.sao.size()) + bytes_of<DevUnit>(C.units.size()) +
-                 bytes_of<uint32_t>(C.tickets.size()) * 2 + bytes_of<uint32_t>(C.unit_tbs.size() + 1) +
+                 bytes_of<uint32_t>(C.tickets.size()) * 2 + bytes_of<PrepTb>(C.unit_recs.size() + 1) +
                  bytes_of<uint16_t>(C.level_off.size() + 1) + bytes_of<uint32_t>(C.sync_words) +
                  bytes_of<uint32_t>(C.unit_refoff.size() + 1) +
                  bytes_of<uint32_t>(C.smap_words + 1) + bytes_of<uint16_t>(C.rmap_words + 8) +
@@ -1131,7 +1135,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
             (st = upload(C.d_cells, j->mem, C.cells)) || (st = upload(C.d_order, j->mem, C.order)) ||
             (st = upload(C.d_sao, j->mem, C.sao)) || (st = upload(C.d_units, j->mem, C.units)) ||
             (st = upload(C.d_tickets, j->mem, C.tickets)) || (st = upload(C.d_plain_tickets, j->mem, C.plain_tickets)) ||
-            (st = upload(C.d_unit_tbs, j->mem, C.unit_tbs)) || (st = upload(C.d_level_off, j->mem, C.level_off)) ||
+            (st = upload(C.d_unit_recs, j->mem, C.unit_recs)) || (st = upload(C.d_level_off, j->mem, C.level_off)) ||
             (st = upload(C.d_unit_refoff, j->mem, C.unit_refoff))) {
             hpvcd_job_free(j);
             return st;
