@@ -597,6 +597,24 @@ __device__ __forceinline__ void intra_tb(IntraLds& L, const R* __restrict__ resb
 constexpr int kStageVec = (kMaxUnitTbs + kMaxUnitRefs / 8 + kMaxCtb * kMaxCtb / 4 + kIntraThreads - 1) / kIntraThreads;
 constexpr int kStageRes = kMaxCtb * kMaxCtb / 4 / kIntraThreads;
 
+// Four consecutive plane samples as one load.
+__device__ __forceinline__ void load_quad(const uint8_t* p, uint32_t (&v)[4]) {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(p);
+    v[0] = w & 0xff; v[1] = (w >> 8) & 0xff; v[2] = (w >> 16) & 0xff; v[3] = w >> 24;
+}
+__device__ __forceinline__ void load_quad(const uint16_t* p, uint32_t (&v)[4]) {
+    const uint2 w = *reinterpret_cast<const uint2*>(p);
+    v[0] = w.x & 0xffff; v[1] = w.x >> 16; v[2] = w.y & 0xffff; v[3] = w.y >> 16;
+}
+
+// Four consecutive plane samples as one store.
+__device__ __forceinline__ void store_quad(uint8_t* p, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    *reinterpret_cast<uint32_t*>(p) = a | (b << 8) | (c << 16) | (d << 24);
+}
+__device__ __forceinline__ void store_quad(uint16_t* p, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    *reinterpret_cast<uint2*>(p) = make_uint2(a | (b << 16), c | (d << 16));
+}
+
 // Four residual samples as one load: 8 bytes of int16, 16 of int32.
 template <typename R> struct Quad;
 template <> struct Quad<int16_t> { using V = uint2; };
@@ -1037,11 +1055,15 @@ __global__ void __launch_bounds__(kIntraThreads)
         __syncthreads();
         HPVCD_PROF_FLUSH(ui)
         HPVCD_TL(ui, 259)
-        // the whole CTB to the plane
-        for (int e = tid; e < ctbw * ctbh; e += kIntraThreads) {
-            int ry = e >> ctbw_log2, rx = e - (ry << ctbw_log2);
-            int px = ux0 + rx, py = uy0 + ry;
-            if (px < pw && py < ph) pl[(size_t)py * pw + px] = (T)L.tile[kTopW + ry * kRowW + rx + 1];
+        // the whole CTB to the plane, four samples per store (plane rows
+        // start at multiples of 4 samples)
+        for (int e = 4 * tid; e < ctbw * ctbh; e += 4 * kIntraThreads) {
+            const int ry = e >> ctbw_log2, rx = e - (ry << ctbw_log2);
+            const int px = ux0 + rx, py = uy0 + ry;
+            if (px < pw && py < ph) {
+                const uint16_t* t = L.tile + kTopW + ry * kRowW + rx + 1;
+                store_quad(pl + (size_t)py * pw + px, t[0], t[1], t[2], t[3]);
+            }
         }
         __syncthreads();
     }
