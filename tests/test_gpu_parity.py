@@ -231,7 +231,7 @@ def test_mixed_formats_one_batch(gpu):
         assert np.array_equal(g, w)
 
 
-@pytest.mark.parametrize("workload", ["grid4032", "rext4096"])
+@pytest.mark.parametrize("workload", ["grid4032", "rext4096", "video1080"])
 def test_full_size_workloads(gpu, workload):
     """The full benchmark workloads, every output sample against the oracle."""
     rs = H.RecordSet.generate(workload)
