@@ -44,10 +44,12 @@ KERNEL_SYMBOLS = {
 
 
 def _one(pattern: str) -> str:
-    hits = sorted(glob.glob(pattern, recursive=True))
+    """The newest file matching pattern (a folder reused across runs keeps
+    older outputs beside the new one)."""
+    hits = glob.glob(pattern, recursive=True)
     if not hits:
         raise FileNotFoundError(pattern)
-    return hits[0]
+    return max(hits, key=os.path.getmtime)
 
 
 def kernel_stats(trace_dir: str) -> list[dict]:
