@@ -101,6 +101,7 @@ struct DevUnit {
     uint8_t nlevels;
     uint16_t ntb;
     uint32_t tb_list;       // offset into unit_tbs
+    uint32_t desc_base;     // first descriptor (tb_list, or the Cb unit's when Cr shares its preparation)
     uint32_t level_list;    // offset into level_off (nlevels + 1 entries)
     uint32_t sbase;         // first per-sample reference word of the unit
     uint32_t nsamp;         // reference words of the unit (sum of its TB areas)

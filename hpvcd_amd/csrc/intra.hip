@@ -231,6 +231,7 @@ struct PrepLds {
 constexpr int kPrepThreads = 256;
 
 __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict__ pics, const DevUnit* __restrict__ units,
+                                                       const uint32_t* __restrict__ prep_list,
                                                        const PrepTb* __restrict__ unit_recs,
                                                        const uint32_t* __restrict__ unit_refoff,
                                                        const uint32_t* __restrict__ cell_order,
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
     __shared__ int16_t angles[33], inv_angles[15];
     const int tid = threadIdx.x, wave = uni(tid >> 6), lane = tid & 63;
     constexpr int kWaves = kPrepThreads / 64;
-    const DevUnit U = units[blockIdx.x];
+    const DevUnit U = units[prep_list[blockIdx.x]];
     const DevPic* Pp = pics + uni(U.pic);
     // picture fields the loops use, held in registers
     const int plane = uni((int)U.plane);
@@ -407,7 +408,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             const bool pub = (tb.flags & TBX_STEP_PUB) != 0, plain = (tb.flags & TBX_STEP_PLAIN) != 0;
             uint4 d;
             d.x = (uint32_t)x0 | ((uint32_t)y0 << 6) | ((uint32_t)(log2n - 2) << 12) | ((uint32_t)s.cls << 14) |
-                  ((tb.flags & TBX_HAS_RES) ? 1u << 16 : 0u) | (s.filt ? 1u << 17 : 0u) |
+                  (s.filt ? 1u << 17 : 0u) |
                   ((s.filt && strong_on && n == 32) ? 1u << 18 : 0u) | (y0 + n == ctbh ? 1u << 19 : 0u) |
                   (x0 + n == ctbw ? 1u << 20 : 0u) | ((n <= 8 && !s.filt && s.cls != kClsDc) ? 1u << 21 : 0u) |
                   ((uint32_t)((tb.flags >> TBX_GROUP_SHIFT) & 7u) << 22) | (gnext << 25) |
@@ -428,7 +429,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
 
 // Per-TB descriptor, decoded once per TB with scalar bit-field extracts.
 //   x: [0,6) x0  [6,12) y0 (unit relative)  [12,14) log2n-2  [14,16) class
-//      16 residual  17 filtered references  18 strong-filter candidate
+//      17 filtered references  18 strong-filter candidate
 //      19 writes the unit's bottom row  20 writes its right column
 //      21 fast path: at most 8 x 8, references unfiltered, not DC
 //      [22,25) size of the group of independent 4 x 4 TBs this TB leads
@@ -457,7 +458,7 @@ struct IntraLds {
 // sample may come prefetched (pf).
 template <int LOG2N, int CLS, typename R>
 __device__ __forceinline__ void intra_predict(const uint16_t* __restrict__ S, const uint32_t* __restrict__ W,
-                                              uint16_t* __restrict__ dst, const R* __restrict__ rs, bool has_res,
+                                              uint16_t* __restrict__ dst, const R* __restrict__ rs,
                                               bool pf, uint32_t w_pf, int32_t r_pf, int lane, int fbase, int u0,
                                               int u1, int32_t dc, int32_t maxv, uint32_t* __restrict__ bw,
                                               int pub_b, int pub_r, uint32_t tagw) {
@@ -478,7 +479,7 @@ __device__ __forceinline__ void intra_predict(const uint16_t* __restrict__ S, co
                 res[c] = r_pf;
             } else {
                 w[c] = W[p];
-                res[c] = has_res ? (int32_t)rs[(p >> LOG2N) * kMaxCtb + (p & (n - 1))] : 0;
+                res[c] = (int32_t)rs[(p >> LOG2N) * kMaxCtb + (p & (n - 1))];  // zero where no residual
             }
         }
 #pragma unroll
@@ -527,7 +528,7 @@ __device__ __forceinline__ void intra_tb(IntraLds& L, const R* __restrict__ resb
     constexpr int NIT = (n4 + 64) / 64;  // passes over the 4n + 1 reference slots
     uint16_t* S = L.tile;
     const int x0 = d0 & 63, y0 = (d0 >> 6) & 63, cls = (d0 >> 14) & 3;
-    const bool has_res = (d0 >> 16) & 1, filt = (d0 >> 17) & 1, scand = (d0 >> 18) & 1;
+    const bool filt = (d0 >> 17) & 1, scand = (d0 >> 18) & 1;
     const uint16_t* M = L.refmap + (d1 & 0x1fff);
     const uint32_t* W = L.smap + ((d1 >> 13) & 0xfff);
     int fbase = 0;
@@ -577,16 +578,16 @@ __device__ __forceinline__ void intra_tb(IntraLds& L, const R* __restrict__ resb
     const int pr = ((d0 >> 20) & 1) ? kMaxCtb + y0 : -1;  // right-column word of row y0
     switch (cls) {
         case kClsAngular:
-            intra_predict<LOG2N, kClsAngular, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
+            intra_predict<LOG2N, kClsAngular, R>(S, W, dst, rs, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
             break;
         case kClsHv:
-            intra_predict<LOG2N, kClsHv, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
+            intra_predict<LOG2N, kClsHv, R>(S, W, dst, rs, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
             break;
         case kClsPlanar:
-            intra_predict<LOG2N, kClsPlanar, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
+            intra_predict<LOG2N, kClsPlanar, R>(S, W, dst, rs, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
             break;
         default:
-            intra_predict<LOG2N, kClsDc, R>(S, W, dst, rs, has_res, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
+            intra_predict<LOG2N, kClsDc, R>(S, W, dst, rs, pf, w_pf, r_pf, lane, fbase, u0, u1, dc, maxv, unit_border, pb, pr, tagw);
     }
     wave_sync();
 }
@@ -760,7 +761,7 @@ __global__ void __launch_bounds__(kIntraThreads)
 #pragma unroll
             for (int b = 0; b < kStageVec; b++) {
                 const int e = min(tid + b * kIntraThreads, v3 - 1);
-                va[b] = e < v1 ? desc_g[U.tb_list + e] : (e < v2 ? rm16[e - v1] : sm16[e - v2]);
+                va[b] = e < v1 ? desc_g[U.desc_base + e] : (e < v2 ? rm16[e - v1] : sm16[e - v2]);
             }
 #pragma unroll
             for (int b = 0; b < kStageRes; b++) {

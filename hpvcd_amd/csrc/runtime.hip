@@ -249,6 +249,7 @@ struct ClassJob {
     std::vector<DevUnit> units;
     std::vector<uint32_t> tickets, plain_tickets, unit_tbs;
     std::vector<PrepTb> unit_recs;                 // K2a's fields of each unit_tbs entry
+    std::vector<uint32_t> prep_list;               // units K2a prepares (Cr units sharing Cb's are not)
     std::vector<uint16_t> level_off;
     std::vector<uint32_t> unit_refoff;             // per unit TB entry: map offset | sample offset / 16 << 13 | level << 21
     size_t smap_words = 0, rmap_words = 0;
@@ -276,6 +277,7 @@ struct ClassJob {
     DevUnit* d_units = nullptr;
     uint32_t *d_tickets = nullptr, *d_plain_tickets = nullptr;
     PrepTb* d_unit_recs = nullptr;
+    uint32_t* d_prep_list = nullptr;
     uint16_t* d_level_off = nullptr;
     uint32_t* d_unit_refoff = nullptr;
     uint32_t* d_smap = nullptr;   // per-sample reference words
@@ -608,8 +610,34 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
                     }
                     U.nsamp = soff;
                     U.nref = roff;
-                    C.smap_words += soff;
-                    C.rmap_words += roff;
+                    U.desc_base = U.tb_list;
+                    // Cr repeats Cb's TB geometry, modes and availability in
+                    // every conforming stream: when the two units' TBs agree
+                    // in everything K2a reads, Cr takes Cb's descriptors,
+                    // reference maps and words (residuals are read whole from
+                    // the pool, so residual presence does not matter)
+                    bool shared = false;
+                    if (pl == 2) {
+                        const DevUnit& B = C.units[C.units.size() - (size_t)D.ctb_rows * D.ctb_cols];
+                        shared = B.ntb == U.ntb;
+                        const uint16_t keep = HPVCD_TB_PCM | 0xfe00u;  // PCM and the step bits
+                        for (uint32_t e = 0; shared && e < U.ntb; e++) {
+                            const PrepTb& a = C.unit_recs[B.tb_list + e];
+                            const PrepTb& c = C.unit_recs[U.tb_list + e];
+                            shared = a.x == c.x && a.y == c.y && a.log2n == c.log2n && a.mode == c.mode &&
+                                     a.order == c.order && (a.flags & keep) == (c.flags & keep);
+                        }
+                        if (shared) {
+                            U.sbase = B.sbase;
+                            U.rbase = B.rbase;
+                            U.desc_base = B.desc_base;
+                        }
+                    }
+                    if (!shared) {
+                        C.smap_words += soff;
+                        C.rmap_words += roff;
+                        C.prep_list.push_back(unit_base + u);
+                    }
                     C.units.push_back(U);
                     keys.push_back({(uint32_t)(cx + 2 * cy), lp, (uint32_t)pl, unit_base + u});
                 }
@@ -778,14 +806,15 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
         bool referenced = false;
         for (uint32_t p = p0; p < p1; p++) referenced |= C.pic_is_ref[p] != 0;
         const int st = referenced ? std::max(stop, (int)HPVCD_STAGE_SAO) : stop;
-        const bool prep = step == 0 && !C.tbs.empty();
+        const bool prep = step == 0 && !C.prep_list.empty();
         if (prep) {
             // reference availability, reference maps, per-sample reference
             // words and descriptors of every TB (records only) on the side
             // stream, concurrently with the residual stage
             HIP_TRY_V(hipEventRecord(g->fork, s));
             HIP_TRY_V(hipStreamWaitEvent(g->side, g->fork, 0));
-            k_prep<<<nunits, kPrepThreads, 0, g->side>>>(C.d_pics, C.d_units, C.d_unit_recs, C.d_unit_refoff,
+            k_prep<<<(uint32_t)C.prep_list.size(), kPrepThreads, 0, g->side>>>(C.d_pics, C.d_units, C.d_prep_list,
+                                                                             C.d_unit_recs, C.d_unit_refoff,
                                                          C.d_order, C.d_cells, C.d_desc, C.d_rmap, C.d_smap);
             HIP_TRY_V(hipEventRecord(g->join, g->side));
         }
@@ -1100,6 +1129,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
                  bytes_of<uint32_t>(C.coeffs.size() + 1) + bytes_of<uint32_t>(C.cells.size()) * 2 +
                  bytes_of<hpvcd_sao>(C.sao.size()) + bytes_of<DevUnit>(C.units.size()) +
                  bytes_of<uint32_t>(C.tickets.size()) * 2 + bytes_of<PrepTb>(C.unit_recs.size() + 1) +
+                 bytes_of<uint32_t>(C.prep_list.size() + 1) +
                  bytes_of<uint16_t>(C.level_off.size() + 1) + bytes_of<uint32_t>(C.sync_words) +
                  bytes_of<uint32_t>(C.unit_refoff.size() + 1) +
                  bytes_of<uint32_t>(C.smap_words + 1) + bytes_of<uint16_t>(C.rmap_words + 8) +
@@ -1136,6 +1166,7 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
             (st = upload(C.d_sao, j->mem, C.sao)) || (st = upload(C.d_units, j->mem, C.units)) ||
             (st = upload(C.d_tickets, j->mem, C.tickets)) || (st = upload(C.d_plain_tickets, j->mem, C.plain_tickets)) ||
             (st = upload(C.d_unit_recs, j->mem, C.unit_recs)) || (st = upload(C.d_level_off, j->mem, C.level_off)) ||
+            (st = upload(C.d_prep_list, j->mem, C.prep_list)) ||
             (st = upload(C.d_unit_refoff, j->mem, C.unit_refoff))) {
             hpvcd_job_free(j);
             return st;
