@@ -44,12 +44,28 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// HEVC transform matrix row k of the 32-point DCT, column m (H.265 8.6.4.2):
-// c(k*(2m+1) mod 128) with c(j) the normative integer approximation of
-// 64*sqrt(2)*cos(j*pi/64).
-__constant__ int8_t kCos64[33] = {64, 90, 90, 90, 89, 88, 87, 85, 83, 82, 80, 78, 75, 73, 70, 67, 64,
-                                  61, 57, 54, 50, 46, 43, 38, 36, 31, 25, 22, 18, 13, 9,  4,  0};
-__constant__ int8_t kDst4[4][4] = {{29, 55, 74, 84}, {74, 74, 0, -74}, {84, -29, -74, 55}, {55, -84, 74, -29}};
+// HEVC transform matrix entries as compile-time functions: after unrolling,
+// every coefficient of the inverse transforms below is an immediate operand.
+// cos64(j) is the normative integer approximation of 64*sqrt(2)*cos(j*pi/64)
+// (H.265 8.6.4.2); the n-point matrix row k, column m is the 32-point matrix
+// row k*32/n: c(k*32/n*(2m+1) mod 128), folded into 0..32 with its sign.
+__host__ __device__ constexpr int cos64(int j) {
+    constexpr int8_t t[33] = {64, 90, 90, 90, 89, 88, 87, 85, 83, 82, 80, 78, 75, 73, 70, 67, 64,
+                              61, 57, 54, 50, 46, 43, 38, 36, 31, 25, 22, 18, 13, 9,  4,  0};
+    return t[j];
+}
+__host__ __device__ constexpr int dct_coef(int n, int k, int m) {
+    const int a = (k * (32 / n) * (2 * m + 1)) & 127;
+    return a < 32 ? cos64(a) : a < 64 ? -cos64(64 - a) : a < 96 ? -cos64(a - 64) : cos64(128 - a);
+}
+// 4-point DST-VII of 4x4 intra luma, row k column m (H.265 8.6.4.2 eq. 8-315)
+__host__ __device__ constexpr int dst4_coef(int k, int m) {
+    constexpr int8_t t[16] = {29, 55, 74, 84, 74, 74, 0, -74, 84, -29, -74, 55, 55, -84, 74, -29};
+    return t[k * 4 + m];
+}
+static_assert(dct_coef(32, 1, 0) == 90 && dct_coef(32, 31, 31) == -4 && dct_coef(32, 31, 15) == -90 && dct_coef(4, 1, 1) == 36 &&
+                  dct_coef(4, 3, 0) == 36 && dct_coef(4, 2, 1) == -64 && dct_coef(8, 1, 3) == 18,
+              "HEVC DCT matrix entries");
 __constant__ int16_t kLevelScale[6] = {40, 45, 51, 57, 64, 72};
 __constant__ uint8_t kBeta[52] = {0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  6,  7,
                                   8,  9,  10, 11, 12, 13, 14, 15, 16, 17, 18, 20, 22, 24, 26, 28, 30, 32,
@@ -60,15 +76,6 @@ __constant__ int8_t kIntraAngle[33] = {32,  26,  21,  17,  13,  9,  5,  2,  0,  
                                        -26, -21, -17, -13, -9, -5, -2, 0,  2,  5,  9,  13,  17,  21,  26,  32};
 __constant__ int16_t kInvAngle[15] = {-4096, -1638, -910, -630, -482, -390, -315, -256,
                                       -315,  -390,  -482, -630, -910, -1638, -4096};
-
-__device__ __forceinline__ int dct_coef32(int k, int m) {
-    if (k == 0) return 64;
-    int a = (k * (2 * m + 1)) & 127;
-    if (a < 32) return kCos64[a];
-    if (a < 64) return -kCos64[64 - a];
-    if (a < 96) return -kCos64[a - 64];
-    return kCos64[128 - a];
-}
 
 __device__ __forceinline__ int max_range_of(int bd, bool ep) { return ep ? (bd + 6 > 15 ? bd + 6 : 15) : 15; }
 
@@ -84,29 +91,105 @@ __device__ __forceinline__ int qpc_of(int qpi, int chroma_idc, int bd) {
 // ---------------------------------------------------------------------------
 // K1: residual (dequant + inverse transform / TS / bypass + RExt ops + CCP)
 // ---------------------------------------------------------------------------
+// wrapping 32-bit sum and 24-bit multiply-add (inputs of the inverse
+// transforms are clipped to at most 23 bits; the accumulation wraps as the
+// reference's i32 arithmetic does)
+__device__ __forceinline__ int32_t mad24w(int32_t acc, int c, int32_t x) { return wadd(acc, __mul24(c, x)); }
+
+// n-point inverse DCT of one column or row (src/transform.rs:489): the even
+// outputs come from the n/2-point inverse of the even inputs, the odd half
+// from the odd basis rows, mirrored (the matrix rows are symmetric /
+// antisymmetric), which is the direct product's value exactly.
+template <int N> struct InvDct {
+    __device__ __forceinline__ static void run(const int32_t (&in)[N], int32_t (&out)[N]) {
+        constexpr int H = N / 2;
+        int32_t ev[H], e[H];
+#pragma unroll
+        for (int j = 0; j < H; j++) ev[j] = in[2 * j];
+        InvDct<H>::run(ev, e);
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            int32_t o = 0;
+#pragma unroll
+            for (int j = 1; j < N; j += 2) o = mad24w(o, dct_coef(N, j, k), in[j]);
+            out[k] = wadd(e[k], o);
+            out[N - 1 - k] = (int32_t)((uint32_t)e[k] - (uint32_t)o);
+        }
+    }
+};
+template <> struct InvDct<2> {
+    __device__ __forceinline__ static void run(const int32_t (&in)[2], int32_t (&out)[2]) {
+        out[0] = (int32_t)(64u * (uint32_t)wadd(in[0], in[1]));
+        out[1] = (int32_t)(64u * ((uint32_t)in[0] - (uint32_t)in[1]));
+    }
+};
+
+__device__ __forceinline__ void inv_dst4(const int32_t (&in)[4], int32_t (&out)[4]) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        int32_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc = mad24w(acc, dst4_coef(k, m), in[k]);
+        out[m] = acc;
+    }
+}
+
+// One row of n residual samples to the pool: 16- or 8-byte stores when the
+// row is whole and aligned, else one sample at a time.
+template <int N, typename R>
+__device__ __forceinline__ void store_res_row(R* p, const int32_t (&v)[N], int vw) {
+    constexpr int BYTES = N * (int)sizeof(R);
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    if (vw == N && BYTES >= 16 && (a & 15) == 0) {
+        constexpr int PER = 16 / (int)sizeof(R);
+#pragma unroll
+        for (int q = 0; q < N / PER; q++) {
+            uint32_t w[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                if (sizeof(R) == 2)
+                    w[t] = (uint32_t)(uint16_t)v[q * PER + 2 * t] | ((uint32_t)(uint16_t)v[q * PER + 2 * t + 1] << 16);
+                else
+                    w[t] = (uint32_t)v[q * PER + t];
+            }
+            reinterpret_cast<uint4*>(p)[q] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    } else if (vw == N && BYTES == 8 && (a & 7) == 0) {
+        reinterpret_cast<uint2*>(p)[0] =
+            make_uint2((uint32_t)(uint16_t)v[0] | ((uint32_t)(uint16_t)v[1] << 16),
+                       (uint32_t)(uint16_t)v[2] | ((uint32_t)(uint16_t)v[3] << 16));
+    } else {
+#pragma unroll
+        for (int m = 0; m < N; m++)
+            if (m < vw) p[m] = (R)v[m];
+    }
+}
+
+// N lanes of one wave per TB: a TB's samples sit in LDS rows of N + 1 words
+// (conflict-free column and row access); lane j owns column j for the
+// vertical pass and row j for the horizontal pass, so the TB needs no
+// barrier beyond its wave.
 template <typename R, int LOG2N>
 __global__ void __launch_bounds__(256) k_residual(const DevResJob* __restrict__ jobs, uint32_t count,
                                                   const uint32_t* __restrict__ coeffs,
                                                   const uint8_t* __restrict__ scaling, R* __restrict__ res_pool) {
     constexpr int N = 1 << LOG2N;
     constexpr int NN = N * N;
-    constexpr int TPB = NN < 256 ? NN : 256;   // threads per TB
-    constexpr int TBS = 256 / TPB;             // TBs per workgroup
-    constexpr int EPT = NN / TPB;              // elements per thread
-    __shared__ int32_t blk[TBS][NN];
-    __shared__ int32_t tmp[TBS][NN];
-    __shared__ int8_t mat[N][N];
+    constexpr int P = N + 1;        // LDS row pitch in words
+    constexpr int TBS = 256 / N;    // TBs per workgroup
+    __shared__ int32_t blk[TBS * N * P];
 
     const int tid = threadIdx.x;
-    const int slot = tid / TPB, lt = tid % TPB;
+    const int slot = tid >> LOG2N, j = tid & (N - 1);
     const uint32_t i = blockIdx.x * TBS + slot;
     const bool valid = i < count;
+    int32_t* B = blk + slot * N * P;
 
     // the job is the only load the coefficient addresses wait for
     DevResJob J = jobs[valid ? i : 0];
     if (!valid) J.flags = 0;
-    for (int e = tid; e < NN; e += 256) mat[e / N][e % N] = (int8_t)dct_coef32((e / N) * (32 / N), e % N);
-    for (int e = 0; e < EPT; e++) blk[slot][lt + e * TPB] = 0;
+#pragma unroll
+    for (int m = 0; m < N; m++) B[j * P + m] = 0;
 
     const uint32_t jf = J.flags;
     const int bd_l = 8 + (J.bds & 15), bd_c = 8 + (J.bds >> 4);
@@ -122,22 +205,23 @@ __global__ void __launch_bounds__(256) k_residual(const DevResJob* __restrict__ 
     const bool use_dst = (jf & RJ_DST) != 0;
     const bool rotate = (jf & RJ_ROTATE) != 0;
     const int rdpcm = (jf & RJ_RDPCM) ? ((jf & RJ_RDPCM_V) ? 1 : 0) : -1;
-    __syncthreads();
+    wave_sync();
 
-    // scatter + dequantise the nonzero levels (src/transform.rs:764-910)
-    if (valid && cbf) {
-        const int log2n = LOG2N;
-        const int64_t bd_shift = bd + log2n + 10 - max_range;
+    // scatter + dequantise the nonzero levels (src/transform.rs:764-910); a
+    // rotated block (src/decode.rs:4356) takes each level at its mirrored
+    // position
+    if (cbf) {
+        const int64_t bd_shift = bd + LOG2N + 10 - max_range;
         const int qp = J.qp;
         const int64_t factor = (int64_t)kLevelScale[qp % 6] * ((int64_t)1 << (qp / 6)) * 16;
         const int64_t add = (int64_t)1 << (bd_shift - 1);
         const int64_t cmin = -((int64_t)1 << max_range), cmax = ((int64_t)1 << max_range) - 1;
-        int tr_shift = max_range - bd - log2n;
+        int tr_shift = max_range - bd - LOG2N;
         if (ep && tr_shift < 0) tr_shift = 0;
         const int32_t tr_add = tr_shift > 0 ? 1 << (tr_shift - 1) : 0;
         const uint8_t* sc = J.scaling >= 0 ? scaling + J.scaling : nullptr;
         const uint32_t* co = coeffs + J.coeff;
-        for (uint32_t k = lt; k < J.coeff_cnt; k += TPB) {
+        for (uint32_t k = j; k < J.coeff_cnt; k += N) {
             uint32_t word = co[k];
             int pos = HPVCD_COEFF_POS(word);
             int32_t level = HPVCD_COEFF_LEVEL(word);
@@ -149,12 +233,11 @@ __global__ void __launch_bounds__(256) k_residual(const DevResJob* __restrict__ 
             } else {
                 int64_t f = factor;
                 if (sc) {
-                    int idx = pos;
                     int64_t m;
-                    if (LOG2N == 2) m = sc[idx];
-                    else if (idx == 0 && LOG2N >= 4) m = sc[64];
+                    if (LOG2N == 2) m = sc[pos];
+                    else if (pos == 0 && LOG2N >= 4) m = sc[64];
                     else {
-                        int yy = idx >> LOG2N, xx = idx & (N - 1);
+                        int yy = pos >> LOG2N, xx = pos & (N - 1);
                         int down = LOG2N > 3 ? LOG2N - 3 : 0;
                         m = sc[(yy >> down) * 8 + (xx >> down)];
                     }
@@ -167,89 +250,87 @@ __global__ void __launch_bounds__(256) k_residual(const DevResJob* __restrict__ 
                     if (i16) v = wrap16(v);
                 }
             }
-            blk[slot][pos] = v;
+            if (rotate) pos = NN - 1 - pos;
+            B[(pos >> LOG2N) * P + (pos & (N - 1))] = v;
         }
     }
-    __syncthreads();
+    wave_sync();
 
-    // stage 1 (columns) or rotation copy
+    // vertical pass, lane j = column j: first inverse transform stage (only
+    // the nx columns that hold levels), or vertical RDPCM accumulation
+    // (src/decode.rs:8418)
     const int32_t cl_min = -(1 << max_range), cl_max = (1 << max_range) - 1;
-    for (int e = 0; e < EPT; e++) {
-        const int o = lt + e * TPB;
-        const int m = o / N, c = o % N;  // tmp[m][c]
-        int32_t v = 0;
-        if (transform) {
-            if (c < J.nx) {
-                int32_t acc = 0;
-                if (use_dst) {
+    if (transform) {
+        int32_t c[N], o[N];
 #pragma unroll
-                    for (int k = 0; k < 4; k++) acc += (int32_t)kDst4[k][m] * blk[slot][k * N + c];
-                } else {
-#pragma unroll 8
-                    for (int k = 0; k < N; k++) acc += (int32_t)mat[k][m] * blk[slot][k * N + c];
-                }
-                v = clampi((acc + 64) >> 7, cl_min, cl_max);
-            }
+        for (int k = 0; k < N; k++) c[k] = B[k * P + j];
+        if constexpr (N == 4) {
+            if (use_dst) inv_dst4(c, o);
+            else InvDct<4>::run(c, o);
         } else {
-            v = blk[slot][rotate ? NN - 1 - o : o];
+            InvDct<N>::run(c, o);
         }
-        tmp[slot][o] = v;
-    }
-    __syncthreads();
-
-    // RDPCM accumulation (src/decode.rs:8418), one thread per row/column
-    if (valid && !transform && rdpcm >= 0 && lt < N) {
-        if (rdpcm == 1) {
-            for (int y = 1; y < N; y++) {
-                int32_t v = wadd(tmp[slot][y * N + lt], tmp[slot][(y - 1) * N + lt]);
-                tmp[slot][y * N + lt] = i16 ? wrap16(v) : v;
-            }
-        } else {
-            for (int x = 1; x < N; x++) {
-                int32_t v = wadd(tmp[slot][lt * N + x], tmp[slot][lt * N + x - 1]);
-                tmp[slot][lt * N + x] = i16 ? wrap16(v) : v;
-            }
+        const bool live = j < (int)J.nx;
+#pragma unroll
+        for (int m = 0; m < N; m++) B[m * P + j] = live ? clampi(wadd(o[m], 64) >> 7, cl_min, cl_max) : 0;
+    } else if (rdpcm == 1) {
+        int32_t acc = B[j];
+#pragma unroll
+        for (int y = 1; y < N; y++) {
+            acc = wadd(B[y * P + j], acc);
+            if (i16) acc = wrap16(acc);
+            B[y * P + j] = acc;
         }
     }
-    __syncthreads();
+    wave_sync();
     if (!valid) return;
 
+    // horizontal pass, lane j = row j: second inverse transform stage, or
+    // horizontal RDPCM accumulation; then cross-component prediction
     const int shift2 = max_range + 5 - bd;
     const int32_t add2 = 1 << (shift2 - 1);
-    R* out = res_pool + J.out;
-    const R* lres = out - J.luma_delta;
-    const bool ccp = (jf & RJ_CCP) != 0;
-    for (int e = 0; e < EPT; e++) {
-        const int o = lt + e * TPB;
-        const int r = o / N, m = o % N;
-        int32_t v;
-        if (transform) {
-            int32_t acc = 0;
-            if (use_dst) {
+    int32_t row[N], v[N];
 #pragma unroll
-                for (int k = 0; k < 4; k++) acc += (int32_t)kDst4[k][m] * tmp[slot][r * N + k];
-            } else {
-#pragma unroll 8
-                for (int k = 0; k < N; k++) acc += (int32_t)mat[k][m] * tmp[slot][r * N + k];
-            }
-            v = wadd(acc, add2) >> shift2;
-            if (i16) v = wrap16(v);
-        } else if (cbf) {
-            v = tmp[slot][o];
+    for (int m = 0; m < N; m++) row[m] = B[j * P + m];
+    if (transform) {
+        if constexpr (N == 4) {
+            if (use_dst) inv_dst4(row, v);
+            else InvDct<4>::run(row, v);
         } else {
-            v = 0;  // cross-component only
+            InvDct<N>::run(row, v);
         }
-        if (m >= J.vw || r >= J.vh) continue;
-        const size_t at = (size_t)r * J.stride + m;
-        if (ccp) {
-            // src/decode.rs:5488 apply_cross_comp_residual
-            int64_t ly = (int64_t)lres[at];
+#pragma unroll
+        for (int m = 0; m < N; m++) {
+            v[m] = wadd(v[m], add2) >> shift2;
+            if (i16) v[m] = wrap16(v[m]);
+        }
+    } else {
+        v[0] = row[0];
+#pragma unroll
+        for (int m = 1; m < N; m++) {
+            int32_t a = row[m];
+            if (rdpcm == 0) {
+                a = wadd(a, v[m - 1]);
+                if (i16) a = wrap16(a);
+            }
+            v[m] = a;
+        }
+    }
+    if (j >= (int)J.vh) return;
+    R* out = res_pool + J.out + (size_t)j * J.stride;
+    if (jf & RJ_CCP) {
+        // src/decode.rs:5488 apply_cross_comp_residual
+        const R* lres = out - J.luma_delta;
+#pragma unroll
+        for (int m = 0; m < N; m++) {
+            if (m >= (int)J.vw) break;
+            int64_t ly = (int64_t)lres[m];
             int64_t sy = (ly << bd_c) >> bd_l;
             int64_t delta = ((int64_t)J.res_scale * sy) >> 3;
-            v = (int32_t)clampl((int64_t)v + delta, INT32_MIN, INT32_MAX);
+            v[m] = (int32_t)clampl((int64_t)v[m] + delta, INT32_MIN, INT32_MAX);
         }
-        out[at] = (R)v;
     }
+    store_res_row<N, R>(out, v, J.vw);
 }
 
 // ---------------------------------------------------------------------------

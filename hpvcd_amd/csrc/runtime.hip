@@ -760,11 +760,13 @@ template <typename R>
 void launch_residual(hipStream_t s, ClassJob& C, const uint8_t* sc, int log2, const DevResJob* jobs, uint32_t count) {
     if (!count) return;
     R* res = reinterpret_cast<R*>(C.d_res);
+    // 256 / n TBs per workgroup (n lanes each)
+    const uint32_t grid = cdiv(count, 256u >> log2);
     switch (log2) {
-        case 2: k_residual<R, 2><<<cdiv(count, 16), 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
-        case 3: k_residual<R, 3><<<cdiv(count, 4), 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
-        case 4: k_residual<R, 4><<<count, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
-        default: k_residual<R, 5><<<count, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
+        case 2: k_residual<R, 2><<<grid, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
+        case 3: k_residual<R, 3><<<grid, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
+        case 4: k_residual<R, 4><<<grid, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
+        default: k_residual<R, 5><<<grid, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
     }
 }
 
