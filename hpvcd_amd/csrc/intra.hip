@@ -170,7 +170,7 @@ __device__ __forceinline__ int ref_tile_addr(int k, uint64_t amask, int n, int x
 __device__ __forceinline__ int ext_sidx(int k, int n, int inv, bool vertical) {
     int off = k;
     if (k < 0) {
-        const int idx = (k * inv + 128) >> 8;
+        const int idx = (mul24(k, inv) + 128) >> 8;
         off = -(idx > 2 * n ? 2 * n : (idx < 0 ? 0 : idx));
     }
     return vertical ? 2 * n + off : 2 * n - off;
@@ -185,7 +185,7 @@ __device__ __forceinline__ uint32_t sample_word(const TbShape& t, int x, int y, 
     auto A = [&](int s) -> uint32_t { return M ? (uint32_t)M[s] : (uint32_t)s; };
     if (t.cls == kClsAngular) {
         const int a = t.vertical ? y : x, c = t.vertical ? x : y;
-        const int pos = (a + 1) * t.angle, ii = pos >> 5, frac = pos & 31;
+        const int pos = mul24(a + 1, t.angle), ii = pos >> 5, frac = pos & 31;
         const int k0 = c + ii + 1;
         const uint32_t a0 = A(ext_sidx(k0, n, t.inv, t.vertical));
         const uint32_t a1 = frac ? A(ext_sidx(k0 + 1, n, t.inv, t.vertical)) : a0;
@@ -224,8 +224,8 @@ struct PrepLds {
     PrepTb tb[kMaxUnitTbs];
     uint32_t ro[kMaxUnitTbs];
     uint16_t rmap[kMaxUnitRefs];
-    uint16_t job[kMaxUnitTbs];       // 4x4 TBs, then 8x8 TBs, then the others
-    uint32_t n4, n8, nbig;
+    uint16_t job[kMaxUnitTbs];       // TBs by size class (4x4, 8x8, larger), each by prediction class
+    uint32_t cnt[12];                // TBs per (size class, prediction class)
 };
 
 constexpr int kPrepThreads = 256;
@@ -286,41 +286,50 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             }
         }
     }
-    if (tid == 0) {
-        L.n4 = 0;
-        L.n8 = 0;
-        L.nbig = 0;
+    if (tid < 12) L.cnt[tid] = 0;
+    __syncthreads();
+    // job order: TBs by size class, and within one by prediction class, so
+    // that the TBs sharing a wave mostly take one branch of the word loop
+    static_assert(kMaxUnitTbs <= kPrepThreads, "one TB per thread");
+    int key = 0, rank = 0;
+    if (tid < ntb) {
+        const PrepTb t = unit_recs[tb_list + tid];
+        L.tb[tid] = t;
+        L.ro[tid] = unit_refoff[tb_list + tid];
+        const int m = (t.flags & HPVCD_TB_PCM) ? 2 : t.mode;
+        const int pc = m == 0 ? kClsPlanar : m == 1 ? kClsDc : (m == 10 || m == 26) ? kClsHv : kClsAngular;
+        key = (t.log2n == 2 ? 0 : t.log2n == 3 ? 4 : 8) + pc;
+        rank = (int)atomicAdd(&L.cnt[key], 1u);
     }
     __syncthreads();
-    // 4x4 TBs go four to a wave (16 lanes each), 8x8 TBs two (32 lanes), the
-    // others one
-    for (int e = tid; e < ntb; e += kPrepThreads) {
-        const PrepTb t = unit_recs[tb_list + e];
-        L.tb[e] = t;
-        L.ro[e] = unit_refoff[tb_list + e];
-        if (t.log2n == 2) L.job[atomicAdd(&L.n4, 1u)] = (uint16_t)e;
+    if (tid < ntb) {
+        int base = 0;
+        for (int k = 0; k < key; k++) base += (int)L.cnt[k];
+        L.job[base + rank] = (uint16_t)tid;
     }
-    __syncthreads();
-    for (int e = tid; e < ntb; e += kPrepThreads)
-        if (L.tb[e].log2n == 3) L.job[L.n4 + atomicAdd(&L.n8, 1u)] = (uint16_t)e;
-    __syncthreads();
-    for (int e = tid; e < ntb; e += kPrepThreads)
-        if (L.tb[e].log2n > 3) L.job[L.n4 + L.n8 + atomicAdd(&L.nbig, 1u)] = (uint16_t)e;
+    const int n4 = (int)(L.cnt[0] + L.cnt[1] + L.cnt[2] + L.cnt[3]);
+    const int n8 = (int)(L.cnt[4] + L.cnt[5] + L.cnt[6] + L.cnt[7]);
     __syncthreads();
     HPVCD_PREP_T(1)
     const int lgv = 2 - sh, lgh = 2 - sw;
     const int gv = 4 >> sh, gh = 4 >> sw;
     const bool strong_on = plane == 0 && (pflags & HPVCD_PF_STRONG_INTRA_SMOOTHING);
-    const int n4 = (int)L.n4, n8 = (int)L.n8;
-    const int j4 = (n4 + 3) >> 2, j8 = (n8 + 1) >> 1, njobs = j4 + j8 + (int)L.nbig;
+    // lanes per TB: enough for the TB's reference groups (2n / 4-sample
+    // group height on the left, the corner, 2n / group width above), so 4x4
+    // TBs go eight to a wave where the groups fit 8 lanes (else four), 8x8
+    // TBs four where they fit 16 (else two), the others one
+    const int lg4 = (8 >> lgv) + 1 + (8 >> lgh) <= 8 ? 3 : 4;
+    const int lg8 = (16 >> lgv) + 1 + (16 >> lgh) <= 16 ? 4 : 5;
+    const int t4 = 64 >> lg4, t8 = 64 >> lg8;
+    const int j4 = (n4 + t4 - 1) / t4, j8 = (n8 + t8 - 1) / t8, njobs = j4 + j8 + (ntb - n4 - n8);
     // per job: availability mask (one lane per 4-sample group), substituted
     // reference map, descriptor, per-sample reference words of each TB
     for (int j = wave; j < njobs; j += kWaves) {
-        const int lg = j < j4 ? 4 : (j < j4 + j8 ? 5 : 6);  // log2 of the lanes per TB
+        const int lg = j < j4 ? lg4 : (j < j4 + j8 ? lg8 : 6);  // log2 of the lanes per TB
         const int G = 1 << lg, gl = lane & (G - 1), gbase = lane & ~(G - 1);
         int first, end;  // the job's first TB slot (always present) and the end of its size class
-        if (lg == 4) { first = 4 * j; end = n4; }
-        else if (lg == 5) { first = n4 + 2 * (j - j4); end = n4 + n8; }
+        if (j < j4) { first = t4 * j; end = n4; }
+        else if (j < j4 + j8) { first = n4 + t8 * (j - j4); end = n4 + n8; }
         else { first = n4 + n8 + (j - j4 - j8); end = ntb; }
         const int slot = first + (lane >> lg);
         const bool has = slot < end;
