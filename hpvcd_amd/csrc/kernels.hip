@@ -758,17 +758,25 @@ __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, co
     const int ctb = 1 << P.ctb_log2;
     const int xs = (rx * ctb) >> sw, ys = (ry * ctb) >> sh;
     const int xe = min(((rx + 1) * ctb) >> sw, pw), ye = min(((ry + 1) * ctb) >> sh, ph);
-    const int qw = (xe - xs) >> 2, nq = qw * (ye - ys);
+    const int qw = (xe - xs) >> 2;
+    if (qw <= 0 || ye <= ys) return;
     const T* src = src_pool + P.plane_off[c];
     T* dst = dst_pool + P.plane_off[c];
     using V = typename Vec4<T>::type;
+    // quad (qx, qy) of the thread, advanced by the block size each pass
+    // without a division per pass
+    const int rows = ye - ys, sy = (int)blockDim.x / qw, sx = (int)blockDim.x - sy * qw;
+    const int qy0 = (int)threadIdx.x / qw, qx0 = (int)threadIdx.x - qy0 * qw;
     const bool on = c == 0 ? (P.flags & HPVCD_PF_SAO_LUMA) != 0 : ((P.flags & HPVCD_PF_SAO_CHROMA) != 0 && P.cw && P.ch);
     const hpvcd_sao& S = sao[P.ctb_begin + ctb_i];
     const int type = on ? S.type_idx[c] : 0;
     if (type == 0) {
-        for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-            const int y = ys + q / qw, x = xs + (q % qw) * 4;
+        for (int qy = qy0, qx = qx0; qy < rows;) {
+            const int y = ys + qy, x = xs + qx * 4;
             *reinterpret_cast<V*>(dst + (size_t)y * pw + x) = *reinterpret_cast<const V*>(src + (size_t)y * pw + x);
+            qx += sx;
+            qy += sy;
+            if (qx >= qw) { qx -= qw; qy++; }
         }
         return;
     }
@@ -781,8 +789,11 @@ __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, co
     const bool across_slices = P.sao_across_slices != 0;
     const bool across_tiles = !(P.flags & HPVCD_PF_TILES) || (P.flags & HPVCD_PF_LF_ACROSS_TILES);
     const bool gated = P.suppress_any || !across_slices || !across_tiles;
-    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-        const int y = ys + q / qw, x = xs + (q % qw) * 4;
+    for (int qy = qy0, qx = qx0; qy < rows;) {
+        const int y = ys + qy, x = xs + qx * 4;
+        qx += sx;
+        qy += sy;
+        if (qx >= qw) { qx -= qw; qy++; }
         const V sv = *reinterpret_cast<const V*>(src + (size_t)y * pw + x);
         const int32_t s[4] = {sv.x, sv.y, sv.z, sv.w};
         int32_t out[4] = {s[0], s[1], s[2], s[3]};
@@ -837,17 +848,19 @@ __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, co
 // ---------------------------------------------------------------------------
 // K5: YUV -> RGB with crop window and grid placement (src/yuv.rs:187-396)
 //
-// One thread per 4 horizontally adjacent output pixels.  The Q13 products
-// fit 32 bits for bit depths up to 12 (|coefficient| < 2^15, sample < 2^12).
+// One thread per 4 horizontally adjacent output pixels; a workgroup covers
+// 64 such quads of 4 output rows (grid: quads of a row / 64, rows / 4,
+// pictures).  The Q13 products fit 32 bits for bit depths up to 12
+// (|coefficient| < 2^15, sample < 2^12).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_rgb(const DevPic* __restrict__ pics, const T* __restrict__ pool,
                                              uint8_t* __restrict__ out_pool, uint32_t pic_base) {
-    const DevPic& P = pics[pic_base + blockIdx.y];
+    const DevPic& P = pics[pic_base + blockIdx.z];
     const int qx = (P.copy_w + 3) >> 2;
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint32_t)(qx * P.copy_h)) return;
-    const int yy = (int)(idx / (uint32_t)qx), xx0 = ((int)idx - yy * qx) * 4;
+    const int xq = (int)(blockIdx.x * 64 + (threadIdx.x & 63)), yy = (int)(blockIdx.y * 4 + (threadIdx.x >> 6));
+    if (xq >= qx || yy >= P.copy_h) return;
+    const int xx0 = xq * 4;
     const int npx = min(4, P.copy_w - xx0);
     const int w = P.w, h = P.h;
     const int x0 = P.crop_left < w - 1 ? P.crop_left : w - 1, y0 = P.crop_top < h - 1 ? P.crop_top : h - 1;
@@ -875,18 +888,52 @@ __global__ void __launch_bounds__(256) k_rgb(const DevPic* __restrict__ pics, co
         crr = (m0 * kc + 4096) >> 13; cbg = (m1 * kc + 4096) >> 13;
         crg = (m2 * kc + 4096) >> 13; cbb = (m3 * kc + 4096) >> 13;
     }
+    // samples of the quad: aligned vector loads inside the picture (rows
+    // start at multiples of 4 samples), one sample at a time at its edges
+    using V4 = typename Vec4<T>::type;
+    const int sx0 = x0 + xx0, csw = P.sub_w_log2;
+    const bool vec = npx == 4 && (sx0 & 3) == 0 && sx0 + 3 <= w - 1;
+    int32_t Ls[4], Us[4] = {0, 0, 0, 0}, Vs[4] = {0, 0, 0, 0};
+    if (vec) {
+        const V4 q = *reinterpret_cast<const V4*>(Yp + sx0);
+        Ls[0] = q.x; Ls[1] = q.y; Ls[2] = q.z; Ls[3] = q.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int src_x = sx0 + min(k, npx - 1);
+            Ls[k] = Yp[src_x < w - 1 ? src_x : w - 1];
+        }
+    }
+    if (!mono) {
+        const int c0 = sx0 >> csw;
+        if (vec && csw == 1 && c0 + 1 <= cw - 1) {
+            Us[0] = Us[1] = Up[c0];
+            Us[2] = Us[3] = Up[c0 + 1];
+            Vs[0] = Vs[1] = Vp[c0];
+            Vs[2] = Vs[3] = Vp[c0 + 1];
+        } else if (vec && csw == 0 && c0 + 3 <= cw - 1) {
+            const V4 u = *reinterpret_cast<const V4*>(Up + c0), vv = *reinterpret_cast<const V4*>(Vp + c0);
+            Us[0] = u.x; Us[1] = u.y; Us[2] = u.z; Us[3] = u.w;
+            Vs[0] = vv.x; Vs[1] = vv.y; Vs[2] = vv.z; Vs[3] = vv.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int src_x = sx0 + min(k, npx - 1);
+                const int cc = (src_x >> csw) < cw - 1 ? (src_x >> csw) : cw - 1;
+                Us[k] = Up[cc];
+                Vs[k] = Vp[cc];
+            }
+        }
+    }
     int32_t v[4][3];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const int src_x = x0 + xx0 + min(k, npx - 1);
-        const int x_col = src_x < w - 1 ? src_x : w - 1;
-        const int32_t L = Yp[x_col];
+        const int32_t L = Ls[k];
         if (mono) {
             v[k][0] = v[k][1] = v[k][2] = (ky * (L - y_black) + 4096) >> 13;
             continue;
         }
-        const int cc = (src_x >> P.sub_w_log2) < cw - 1 ? (src_x >> P.sub_w_log2) : cw - 1;
-        const int32_t U = Up[cc], Vv = Vp[cc];
+        const int32_t U = Us[k], Vv = Vs[k];
         if (P.matrix == 0) {
             if (full) { v[k][0] = Vv; v[k][1] = L; v[k][2] = U; }
             else {

@@ -268,7 +268,7 @@ struct ClassJob {
     std::vector<uint8_t> pic_is_ref;                  // referenced by a later picture: always completes SAO
     size_t plane_elems = 0, res_elems = 0;
     uint32_t max_luma_v = 0, max_luma_h = 0, max_chroma_v = 0, max_chroma_h = 0, max_plane = 0, max_copy = 0;
-    uint32_t max_ctbs = 0, max_rgb_quads = 0;          // K4 blocks / K5 threads of the largest picture
+    uint32_t max_ctbs = 0, max_rgb_qx = 0, max_rgb_rows = 0;  // K4 blocks; K5 quads per row, rows
     // device pointers
     DevPic* d_pics = nullptr;
     hpvcd_tb* d_tbs = nullptr;
@@ -312,8 +312,6 @@ struct hpvcd_job {
 
 namespace {
 
-// Threads of K5 (4 output pixels each).
-uint32_t rgb_quads(const hpvcd_picture& P) { return (uint32_t)((P.copy_w + 3) / 4) * P.copy_h; }
 
 // K1 job of one residual TB: the flag logic of src/decode.rs:4356
 // (rext_post_ops), :3660 (ACT keeps 32-bit residuals) and
@@ -443,7 +441,8 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         C.max_plane = std::max<uint32_t>(C.max_plane, (uint32_t)ny);
         C.max_copy = std::max<uint32_t>(C.max_copy, (uint32_t)(P.copy_w * P.copy_h));
         C.max_ctbs = std::max<uint32_t>(C.max_ctbs, (uint32_t)(D.ctb_cols * D.ctb_rows));
-        C.max_rgb_quads = std::max<uint32_t>(C.max_rgb_quads, rgb_quads(P));
+        C.max_rgb_qx = std::max<uint32_t>(C.max_rgb_qx, (uint32_t)((P.copy_w + 3) / 4));
+        C.max_rgb_rows = std::max<uint32_t>(C.max_rgb_rows, (uint32_t)P.copy_h);
 
         // copy records
         const hpvcd_tb* src_tb = b.tbs + P.tb_begin;
@@ -881,8 +880,10 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
         }
         if (st >= HPVCD_STAGE_RGB) {
             mark(HPVCD_KERNEL_RGB);
-            uint32_t mc = plain ? (uint32_t)(((C.pics[p0].copy_w + 3) / 4) * C.pics[p0].copy_h) : C.max_rgb_quads;
-            if (mc) k_rgb<T><<<dim3(cdiv(mc, 256), pn), 256, 0, s>>>(C.d_pics, pb, d_out, p0);
+            // quads of 4 pixels: 64 of a row by 4 rows per workgroup
+            const uint32_t qx = plain ? (uint32_t)((C.pics[p0].copy_w + 3) / 4) : C.max_rgb_qx;
+            const uint32_t rows = plain ? (uint32_t)C.pics[p0].copy_h : C.max_rgb_rows;
+            if (qx && rows) k_rgb<T><<<dim3(cdiv(qx, 64), cdiv(rows, 4), pn), 256, 0, s>>>(C.d_pics, pb, d_out, p0);
             mark(-1);
         }
     }
