@@ -565,15 +565,37 @@ __device__ __forceinline__ int qp_at(const DevPic& P, const uint32_t* info, int 
     return HPVCD_CELL_QP(info[P.cell_begin + (size_t)gy * P.gw + gx]);
 }
 
+// Vector of 4 samples.
+template <typename T> struct Vec4;
+template <> struct Vec4<uint8_t> { using type = uchar4; };
+template <> struct Vec4<uint16_t> { using type = ushort4; };
+
+// One 4-line luma edge segment.  base points at q0 of the first line; a
+// vertical edge's lines are rows (taps along the row), a horizontal edge's
+// lines are columns (taps down the column).  Either way the 8 x 4 samples
+// p3..q3 come in as eight aligned 4-sample vectors (edges sit on multiples of
+// 8, lines on multiples of 4, rows start at multiples of 4 samples) and go
+// back the same way; a thread owns p3..q3 of its segment, so rewriting the
+// samples the filter leaves is safe.
 template <typename T>
-__device__ void luma_segment(T* base, ptrdiff_t step_tap, ptrdiff_t step_line, int beta, int tc, int maxv,
-                             bool wp, bool wq) {
+__device__ void luma_segment(T* base, bool vertical, int stride, int beta, int tc, int maxv, bool wp, bool wq) {
     // src/deblock.rs:216 luma_decision + :241 deblock_luma_segment
+    using V = typename Vec4<T>::type;
     int32_t s[4][8];
+    if (vertical) {
 #pragma unroll
-    for (int l = 0; l < 4; l++)
+        for (int l = 0; l < 4; l++) {
+            const V a = *reinterpret_cast<const V*>(base + l * stride - 4), b = *reinterpret_cast<const V*>(base + l * stride);
+            s[l][0] = a.x; s[l][1] = a.y; s[l][2] = a.z; s[l][3] = a.w;
+            s[l][4] = b.x; s[l][5] = b.y; s[l][6] = b.z; s[l][7] = b.w;
+        }
+    } else {
 #pragma unroll
-        for (int t = 0; t < 8; t++) s[l][t] = base[l * step_line + (t - 4) * step_tap];
+        for (int t = 0; t < 8; t++) {
+            const V a = *reinterpret_cast<const V*>(base + (t - 4) * stride);
+            s[0][t] = a.x; s[1][t] = a.y; s[2][t] = a.z; s[3][t] = a.w;
+        }
+    }
     auto dpf = [&](int l) { int v = s[l][1] - 2 * s[l][2] + s[l][3]; return v < 0 ? -v : v; };
     auto dqf = [&](int l) { int v = s[l][6] - 2 * s[l][5] + s[l][4]; return v < 0 ? -v : v; };
     int dp0 = dpf(0), dp3 = dpf(3), dq0 = dqf(0), dq3 = dqf(3);
@@ -587,38 +609,51 @@ __device__ void luma_segment(T* base, ptrdiff_t step_tap, ptrdiff_t step_line, i
     const int side_thr = (beta + (beta >> 1)) >> 3;
     const bool do_p1 = dp0 + dp3 < side_thr, do_q1 = dq0 + dq3 < side_thr;
     const int tc2 = tc >> 1;
+    int32_t o[4][8];
 #pragma unroll
     for (int l = 0; l < 4; l++) {
         const int32_t p3 = s[l][0], p2 = s[l][1], p1 = s[l][2], p0 = s[l][3];
         const int32_t q0 = s[l][4], q1 = s[l][5], q2 = s[l][6], q3 = s[l][7];
-        T* b = base + l * step_line;
+#pragma unroll
+        for (int t = 0; t < 8; t++) o[l][t] = s[l][t];
         if (strong) {
             auto sc = [&](int32_t orig, int32_t v) { return clampi(clampi(v, orig - 2 * tc, orig + 2 * tc), 0, maxv); };
             if (wp) {
-                b[-1 * step_tap] = (T)sc(p0, (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3);
-                b[-2 * step_tap] = (T)sc(p1, (p2 + p1 + p0 + q0 + 2) >> 2);
-                b[-3 * step_tap] = (T)sc(p2, (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3);
+                o[l][3] = sc(p0, (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3);
+                o[l][2] = sc(p1, (p2 + p1 + p0 + q0 + 2) >> 2);
+                o[l][1] = sc(p2, (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3);
             }
             if (wq) {
-                b[0] = (T)sc(q0, (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3);
-                b[1 * step_tap] = (T)sc(q1, (p0 + q0 + q1 + q2 + 2) >> 2);
-                b[2 * step_tap] = (T)sc(q2, (p0 + q0 + q1 + 3 * q2 + 2 * q3 + 4) >> 3);
+                o[l][4] = sc(q0, (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3);
+                o[l][5] = sc(q1, (p0 + q0 + q1 + q2 + 2) >> 2);
+                o[l][6] = sc(q2, (p0 + q0 + q1 + 3 * q2 + 2 * q3 + 4) >> 3);
             }
         } else {
             const int32_t delta0 = (9 * (q0 - p0) - 3 * (q1 - p1) + 8) >> 4;
             if ((delta0 < 0 ? -delta0 : delta0) < tc * 10) {
                 const int32_t delta = clampi(delta0, -tc, tc);
-                if (wp) b[-1 * step_tap] = (T)clampi(p0 + delta, 0, maxv);
-                if (wq) b[0] = (T)clampi(q0 - delta, 0, maxv);
-                if (do_p1 && wp) {
-                    int32_t d = clampi((((p2 + p0 + 1) >> 1) - p1 + delta) >> 1, -tc2, tc2);
-                    b[-2 * step_tap] = (T)clampi(p1 + d, 0, maxv);
-                }
-                if (do_q1 && wq) {
-                    int32_t d = clampi((((q2 + q0 + 1) >> 1) - q1 - delta) >> 1, -tc2, tc2);
-                    b[1 * step_tap] = (T)clampi(q1 + d, 0, maxv);
-                }
+                if (wp) o[l][3] = clampi(p0 + delta, 0, maxv);
+                if (wq) o[l][4] = clampi(q0 - delta, 0, maxv);
+                if (do_p1 && wp) o[l][2] = clampi(p1 + clampi((((p2 + p0 + 1) >> 1) - p1 + delta) >> 1, -tc2, tc2), 0, maxv);
+                if (do_q1 && wq) o[l][5] = clampi(q1 + clampi((((q2 + q0 + 1) >> 1) - q1 - delta) >> 1, -tc2, tc2), 0, maxv);
             }
+        }
+    }
+    if (vertical) {
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            V a, b;
+            a.x = (T)o[l][0]; a.y = (T)o[l][1]; a.z = (T)o[l][2]; a.w = (T)o[l][3];
+            b.x = (T)o[l][4]; b.y = (T)o[l][5]; b.z = (T)o[l][6]; b.w = (T)o[l][7];
+            *reinterpret_cast<V*>(base + l * stride - 4) = a;
+            *reinterpret_cast<V*>(base + l * stride) = b;
+        }
+    } else {
+#pragma unroll
+        for (int t = 1; t < 7; t++) {
+            V a;
+            a.x = (T)o[0][t]; a.y = (T)o[1][t]; a.z = (T)o[2][t]; a.w = (T)o[3][t];
+            *reinterpret_cast<V*>(base + (t - 4) * stride) = a;
         }
     }
 }
@@ -656,8 +691,8 @@ __device__ __forceinline__ void deblock_luma(const DevPic* __restrict__ pics, co
         wp = !suppressed(P, cell_of(P, info, xp, yp));
         wq = !suppressed(P, cell_of(P, info, vertical ? edge : scan, vertical ? scan : edge));
     }
-    if (vertical) luma_segment<T>(pl + (size_t)scan * P.w + edge, 1, P.w, beta, tc, (1 << P.bd) - 1, wp, wq);
-    else luma_segment<T>(pl + (size_t)edge * P.w + scan, P.w, 1, beta, tc, (1 << P.bd) - 1, wp, wq);
+    if (vertical) luma_segment<T>(pl + (size_t)scan * P.w + edge, true, P.w, beta, tc, (1 << P.bd) - 1, wp, wq);
+    else luma_segment<T>(pl + (size_t)edge * P.w + scan, false, P.w, beta, tc, (1 << P.bd) - 1, wp, wq);
 }
 
 template <typename T>
@@ -739,10 +774,6 @@ __global__ void __launch_bounds__(256) k_deblock(const DevPic* __restrict__ pics
 // (plane widths are multiples of 4 and planes start 16-sample aligned, so a
 // quad is one aligned vector load/store).
 // ---------------------------------------------------------------------------
-template <typename T> struct Vec4;
-template <> struct Vec4<uint8_t> { using type = uchar4; };
-template <> struct Vec4<uint16_t> { using type = ushort4; };
-
 template <typename T>
 __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, const uint32_t* __restrict__ info,
                                              const hpvcd_sao* __restrict__ sao, const T* __restrict__ src_pool,
