@@ -728,19 +728,37 @@ __device__ __forceinline__ void deblock_chroma(const DevPic* __restrict__ pics, 
         const int tcc = kTc[tci] << (P.bd_c - 8);
         if (!tcc) continue;
         T* pl = samples + P.plane_off[1 + pln];
-        for (int l = scan; l < scan + 4; l++) {
-            T *pp, *qq;
-            int32_t p0, p1, q0, q1;
-            if (vertical) {
-                T* r = pl + (size_t)l * cw + edge;
-                pp = r - 1; qq = r;
-                p0 = r[-1]; p1 = r[-2]; q0 = r[0]; q1 = r[1];
-            } else {
-                T* r = pl + (size_t)edge * cw + l;
-                pp = r - cw; qq = r;
-                p0 = r[-cw]; p1 = r[-2 * cw]; q0 = r[0]; q1 = r[cw];
+        // p1 p0 q0 q1 of the 4 lines from aligned 4-sample vectors (edges
+        // on multiples of 8, lines on multiples of 4, rows start at
+        // multiples of 4 samples); a thread owns the 8 samples around its
+        // edge on each line, so whole vectors go back
+        using V = typename Vec4<T>::type;
+        int32_t p1[4], p0[4], q0[4], q1[4];
+        V pv[4], qv[4];
+        T* base = vertical ? pl + (size_t)scan * cw + edge : pl + (size_t)edge * cw + scan;
+        if (vertical) {
+#pragma unroll
+            for (int l = 0; l < 4; l++) {
+                pv[l] = *reinterpret_cast<const V*>(base + l * cw - 4);
+                qv[l] = *reinterpret_cast<const V*>(base + l * cw);
+                p1[l] = pv[l].z; p0[l] = pv[l].w; q0[l] = qv[l].x; q1[l] = qv[l].y;
             }
-            const int32_t delta = clampi(((q0 - p0) * 4 + p1 - q1 + 4) >> 3, -tcc, tcc);
+        } else {
+            const V a = *reinterpret_cast<const V*>(base - 2 * cw), b = *reinterpret_cast<const V*>(base - cw);
+            const V c = *reinterpret_cast<const V*>(base), d = *reinterpret_cast<const V*>(base + cw);
+            p1[0] = a.x; p1[1] = a.y; p1[2] = a.z; p1[3] = a.w;
+            p0[0] = b.x; p0[1] = b.y; p0[2] = b.z; p0[3] = b.w;
+            q0[0] = c.x; q0[1] = c.y; q0[2] = c.z; q0[3] = c.w;
+            q1[0] = d.x; q1[1] = d.y; q1[2] = d.z; q1[3] = d.w;
+        }
+        int32_t np[4], nq[4];
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int l = scan + k;
+            np[k] = p0[k];
+            nq[k] = q0[k];
+            const int32_t delta = clampi(((q0[k] - p0[k]) * 4 + p1[k] - q1[k] + 4) >> 3, -tcc, tcc);
             if (!delta) continue;
             bool wp = true, wq = true;
             if (P.suppress_any) {
@@ -749,8 +767,26 @@ __device__ __forceinline__ void deblock_chroma(const DevPic* __restrict__ pics, 
                 wp = !suppressed(P, cell_of(P, info, cpx << sw, cpy << sh));
                 wq = !suppressed(P, cell_of(P, info, cqx << sw, cqy << sh));
             }
-            if (wp) *pp = (T)clampi(p0 + delta, 0, maxv);
-            if (wq) *qq = (T)clampi(q0 - delta, 0, maxv);
+            if (wp) np[k] = clampi(p0[k] + delta, 0, maxv);
+            if (wq) nq[k] = clampi(q0[k] - delta, 0, maxv);
+            any = true;
+        }
+        if (!any) continue;
+        if (vertical) {
+#pragma unroll
+            for (int l = 0; l < 4; l++) {
+                V a = pv[l], b = qv[l];
+                a.w = (T)np[l];
+                b.x = (T)nq[l];
+                *reinterpret_cast<V*>(base + l * cw - 4) = a;
+                *reinterpret_cast<V*>(base + l * cw) = b;
+            }
+        } else {
+            V a, b;
+            a.x = (T)np[0]; a.y = (T)np[1]; a.z = (T)np[2]; a.w = (T)np[3];
+            b.x = (T)nq[0]; b.y = (T)nq[1]; b.z = (T)nq[2]; b.w = (T)nq[3];
+            *reinterpret_cast<V*>(base - cw) = a;
+            *reinterpret_cast<V*>(base) = b;
         }
     }
 }
