@@ -876,6 +876,19 @@ __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, co
             const bool rows_ok = y1 >= 0 && y1 < ph && y2 >= 0 && y2 < ph;
             const T* r1 = src + (size_t)(rows_ok ? y1 : y) * pw;
             const T* r2 = src + (size_t)(rows_ok ? y2 : y) * pw;
+            if (!gated) {
+                // no gates: branch-free, neighbour columns clamped into the
+                // row (a sample whose neighbour lies outside keeps its value)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int xk = x + k, x1 = xk + dx, x2 = xk - dx;
+                    const bool ok = rows_ok && x2 >= 0 && x1 < pw;
+                    const int32_t n1 = r1[min(x1, pw - 1)], n2 = r2[max(x2, 0)];
+                    const int ei = ((s[k] > n1) - (s[k] < n1)) + ((s[k] > n2) - (s[k] < n2)) + 2;
+                    const int32_t o = ei == 0 ? o0 : ei == 1 ? o1 : ei == 3 ? o2 : ei == 4 ? o3 : 0;
+                    out[k] = ok ? clampi(s[k] + o, 0, maxv) : s[k];
+                }
+            } else
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int xk = x + k, x1 = xk + dx, x2 = xk - dx;
