@@ -805,17 +805,21 @@ __global__ void __launch_bounds__(256) k_deblock(const DevPic* __restrict__ pics
 // ---------------------------------------------------------------------------
 // K4: SAO, out of place (src/decode.rs:2258; gated EO src/sao.rs:433)
 //
-// One workgroup per (CTB, plane) of a picture, so the CTB's SAO parameters
-// are uniform; each thread takes quads of 4 horizontally adjacent samples
+// One workgroup per part of a luma CTB's rows (the CTB whole, or quarters
+// when the launch holds few CTBs) and per chroma plane of a CTB, so the CTB's
+// SAO parameters are uniform; each thread takes quads of 4 horizontally adjacent samples
 // (plane widths are multiples of 4 and planes start 16-sample aligned, so a
 // quad is one aligned vector load/store).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, const uint32_t* __restrict__ info,
                                              const hpvcd_sao* __restrict__ sao, const T* __restrict__ src_pool,
-                                             T* __restrict__ dst_pool, uint32_t pic_base) {
+                                             T* __restrict__ dst_pool, uint32_t pic_base, int luma_parts) {
     const DevPic& P = pics[pic_base + blockIdx.y];
-    const int c = blockIdx.z;
+    // z [0, luma_parts): parts of the luma CTB's rows; then Cb, Cr
+    const int z = (int)blockIdx.z;
+    const int c = z < luma_parts ? 0 : z - luma_parts + 1, part = z < luma_parts ? z : 0;
+    const int nparts = c ? 1 : luma_parts;
     if (c > 0 && P.chroma_idc == 0) return;
     const int ctb_i = blockIdx.x;
     if (ctb_i >= P.ctb_cols * P.ctb_rows) return;
@@ -823,8 +827,10 @@ __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, co
     const int pw = c ? P.cw : P.w, ph = c ? P.ch : P.h;
     const int sw = c ? P.sub_w_log2 : 0, sh = c ? P.sub_h_log2 : 0;
     const int ctb = 1 << P.ctb_log2;
-    const int xs = (rx * ctb) >> sw, ys = (ry * ctb) >> sh;
-    const int xe = min(((rx + 1) * ctb) >> sw, pw), ye = min(((ry + 1) * ctb) >> sh, ph);
+    const int xs = (rx * ctb) >> sw, ys0 = (ry * ctb) >> sh;
+    const int xe = min(((rx + 1) * ctb) >> sw, pw), ye0 = min(((ry + 1) * ctb) >> sh, ph);
+    const int prow = nparts == 4 ? (ye0 - ys0 + 3) >> 2 : ye0 - ys0;  // luma_parts is 1 or 4
+    const int ys = ys0 + part * prow, ye = min(ye0, ys + prow);
     const int qw = (xe - xs) >> 2;
     if (qw <= 0 || ye <= ys) return;
     const T* src = src_pool + P.plane_off[c];
@@ -1097,7 +1103,7 @@ HPVCD_INST_INTRA(uint16_t, int32_t)
 
 #define HPVCD_INST_T(T)                                                                                          \
     template __global__ void k_deblock<T>(const DevPic*, const uint32_t*, T*, int, uint32_t, uint32_t);          \
-    template __global__ void k_sao<T>(const DevPic*, const uint32_t*, const hpvcd_sao*, const T*, T*, uint32_t); \
+    template __global__ void k_sao<T>(const DevPic*, const uint32_t*, const hpvcd_sao*, const T*, T*, uint32_t, int); \
     template __global__ void k_rgb<T>(const DevPic*, const T*, uint8_t*, uint32_t);
 HPVCD_INST_T(uint8_t)
 HPVCD_INST_T(uint16_t)

@@ -875,7 +875,9 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
         if (st >= HPVCD_STAGE_SAO) {
             mark(HPVCD_KERNEL_SAO);
             uint32_t nctb = plain ? (uint32_t)(C.pics[p0].ctb_cols * C.pics[p0].ctb_rows) : C.max_ctbs;
-            k_sao<T><<<dim3(nctb, pn, 3), 256, 0, s>>>(C.d_pics, C.d_cells, C.d_sao, pa, pb, p0);
+            // a launch of few CTBs splits each luma CTB's rows over four workgroups
+            const int parts = nctb * pn < 4u * (uint32_t)g->num_cus ? 4 : 1;
+            k_sao<T><<<dim3(nctb, pn, parts + 2), 256, 0, s>>>(C.d_pics, C.d_cells, C.d_sao, pa, pb, p0, parts);
             mark(-1);
         }
         if (st >= HPVCD_STAGE_RGB) {
