@@ -261,28 +261,40 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
     HPVCD_PREP_T(0)
     if (tid < 33) angles[tid] = kIntraAngle[tid];
     if (tid < 15) inv_angles[tid] = kInvAngle[tid];
+    // The window's cells that can precede the unit's TBs in decode order:
+    // the row above (the CTBs above and above-right), the column to the left
+    // (the left CTB and the corner) and the CTB itself.  Every other cell
+    // belongs to a CTB decoded later (to the right, or a CTB row below) and is
+    // stored as never available without a load.
+    const int cc = ctb >> 2;  // the CTB's width in 4x4 luma cells
+    for (int c = tid; c < kWin * kWin; c += kPrepThreads) L.ord[c] = 0xffffffffu;
+    __syncthreads();
     {
-        // every load of a thread in flight at once (cells outside the
-        // picture load cell 0 and are stored as never available)
-        constexpr int kPer = (kWin * kWin + kPrepThreads - 1) / kPrepThreads;
-        uint32_t vo[kPer], vi[kPer];
+        const int nneed = kWin + cc + cc * cc;
+        constexpr int kNeedPer = (kWin + 16 + 16 * 16 + kPrepThreads - 1) / kPrepThreads;
+        const int lcc = ctb_log2 - 2;
+        uint32_t vo[kNeedPer], vi[kNeedPer];
+        int wc[kNeedPer];
+        bool win[kNeedPer];
 #pragma unroll
-        for (int k = 0; k < kPer; k++) {
-            const int c = min(tid + k * kPrepThreads, kWin * kWin - 1);
-            const int gx = gx0 + c % kWin, gy = gy0 + c / kWin;
-            const bool in = gx >= 0 && gy >= 0 && gx < gw && gy < gh_cells;
-            const int g = in ? gy * gw + gx : 0;
+        for (int k = 0; k < kNeedPer; k++) {
+            const int t = tid + k * kPrepThreads;
+            int wx, wy;
+            if (t < kWin) { wx = t; wy = 0; }
+            else if (t < kWin + cc) { wx = 0; wy = t - kWin + 1; }
+            else { const int i = t - kWin - cc; wx = 1 + (i & (cc - 1)); wy = 1 + (i >> lcc); }
+            const int gx = gx0 + wx, gy = gy0 + wy;
+            win[k] = t < nneed && gx >= 0 && gy >= 0 && gx < gw && gy < gh_cells;
+            wc[k] = wy * kWin + wx;
+            const int g = win[k] ? gy * gw + gx : 0;
             vo[k] = ord_g[g];
             vi[k] = need_info ? inf_g[g] : 0u;
         }
 #pragma unroll
-        for (int k = 0; k < kPer; k++) {
-            const int c = tid + k * kPrepThreads;
-            if (c < kWin * kWin) {
-                const int gx = gx0 + c % kWin, gy = gy0 + c / kWin;
-                const bool in = gx >= 0 && gy >= 0 && gx < gw && gy < gh_cells;
-                L.ord[c] = in ? vo[k] : 0xffffffffu;
-                L.inf[c] = in ? vi[k] : 0u;
+        for (int k = 0; k < kNeedPer; k++) {
+            if (tid + k * kPrepThreads < nneed) {
+                L.ord[wc[k]] = win[k] ? vo[k] : 0xffffffffu;
+                L.inf[wc[k]] = win[k] ? vi[k] : 0u;
             }
         }
     }
