@@ -1039,11 +1039,11 @@ hpvcd_status hpvcd_gpu_open(int device, hpvcd_gpu** out) {
 
 void hpvcd_gpu_close(hpvcd_gpu* g) {
     if (!g) return;
-    hipSetDevice(g->device);
-    if (g->stream) hipStreamDestroy(g->stream);
-    if (g->side) hipStreamDestroy(g->side);
-    if (g->fork) hipEventDestroy(g->fork);
-    if (g->join) hipEventDestroy(g->join);
+    (void)hipSetDevice(g->device);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->side) (void)hipStreamDestroy(g->side);
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    if (g->join) (void)hipEventDestroy(g->join);
     delete g;
 }
 
@@ -1062,10 +1062,10 @@ hpvcd_status hpvcd_gpu_sync(hpvcd_gpu* g) {
 
 void hpvcd_job_free(hpvcd_job* j) {
     if (!j) return;
-    hipSetDevice(j->g->device);
+    (void)hipSetDevice(j->g->device);
     for (auto& C : j->classes)
-        if (C.graph) hipGraphExecDestroy(C.graph);
-    if (j->mem.base) hipFree(j->mem.base);
+        if (C.graph) (void)hipGraphExecDestroy(C.graph);
+    if (j->mem.base) (void)hipFree(j->mem.base);
     delete j;
 }
 
@@ -1222,7 +1222,7 @@ hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop) {
     if (mode == HPVCD_MODE_GRAPH) {
         for (auto& C : j->classes) {
             if (!C.graph || C.graph_stop != stop) {
-                if (C.graph) { hipGraphExecDestroy(C.graph); C.graph = nullptr; }
+                if (C.graph) { (void)hipGraphExecDestroy(C.graph); C.graph = nullptr; }
                 hipGraph_t gr;
                 HIP_TRY(hipStreamBeginCapture(j->g->stream, hipStreamCaptureModeThreadLocal));
                 if (C.samp16) {
@@ -1234,11 +1234,11 @@ hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop) {
                 }
                 HIP_TRY(hipStreamEndCapture(j->g->stream, &gr));
                 if (t_launch_err != hipSuccess) {
-                    hipGraphDestroy(gr);
+                    (void)hipGraphDestroy(gr);
                     return set_error(HPVCD_E_DEVICE, hipGetErrorString(t_launch_err));
                 }
                 hipError_t e = hipGraphInstantiate(&C.graph, gr, nullptr, nullptr, 0);
-                hipGraphDestroy(gr);
+                (void)hipGraphDestroy(gr);
                 if (e != hipSuccess) return set_error(HPVCD_E_DEVICE, "hipGraphInstantiate failed");
                 C.graph_stop = stop;
             }
@@ -1312,8 +1312,8 @@ hpvcd_status hpvcd_job_time(hpvcd_job* j, int mode, int warmup, int iters, doubl
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     *ms_per_run = ms / iters;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if ((st = check_err(j))) return st;
     if (kernel_ms) {
         for (int k = 0; k < HPVCD_KERNEL_COUNT; k++) kernel_ms[k] = 0.0;
@@ -1330,10 +1330,10 @@ hpvcd_status hpvcd_job_time(hpvcd_job* j, int mode, int warmup, int iters, doubl
             for (size_t m = 0; m + 1 < marks.size(); m++) {
                 if (marks[m].first < 0) continue;
                 float t = 0.f;
-                hipEventElapsedTime(&t, marks[m].second, marks[m + 1].second);
+                (void)hipEventElapsedTime(&t, marks[m].second, marks[m + 1].second);
                 kernel_ms[marks[m].first] += t / iters;
             }
-            for (auto& m : marks) hipEventDestroy(m.second);
+            for (auto& m : marks) (void)hipEventDestroy(m.second);
         }
         if ((st = check_err(j))) return st;
     }
