@@ -643,6 +643,10 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         unit_base += nunits;
     }
     C.pic_units_begin[np] = unit_base;
+    // K2a takes the units with the most predicted samples first, so that the
+    // launch's tail holds the small ones
+    std::stable_sort(C.prep_list.begin(), C.prep_list.end(),
+                     [&](uint32_t a, uint32_t b) { return C.units[a].nsamp > C.units[b].nsamp; });
     // batched tickets: topological (t = cx + 2 cy) across all pictures
     std::vector<UnitKey> sorted = keys;
     std::stable_sort(sorted.begin(), sorted.end(), [](const UnitKey& a, const UnitKey& b) {
