@@ -578,10 +578,8 @@ template <> struct Vec4<uint16_t> { using type = ushort4; };
 // back the same way; a thread owns p3..q3 of its segment, so rewriting the
 // samples the filter leaves is safe.
 template <typename T>
-__device__ void luma_segment(T* base, bool vertical, int stride, int beta, int tc, int maxv, bool wp, bool wq) {
-    // src/deblock.rs:216 luma_decision + :241 deblock_luma_segment
+__device__ __forceinline__ void luma_segment_load(const T* base, bool vertical, int stride, int32_t (&s)[4][8]) {
     using V = typename Vec4<T>::type;
-    int32_t s[4][8];
     if (vertical) {
 #pragma unroll
         for (int l = 0; l < 4; l++) {
@@ -596,6 +594,13 @@ __device__ void luma_segment(T* base, bool vertical, int stride, int beta, int t
             s[0][t] = a.x; s[1][t] = a.y; s[2][t] = a.z; s[3][t] = a.w;
         }
     }
+}
+
+template <typename T>
+__device__ void luma_segment(T* base, bool vertical, int stride, const int32_t (&s)[4][8], int beta, int tc,
+                             int maxv, bool wp, bool wq) {
+    // src/deblock.rs:216 luma_decision + :241 deblock_luma_segment
+    using V = typename Vec4<T>::type;
     auto dpf = [&](int l) { int v = s[l][1] - 2 * s[l][2] + s[l][3]; return v < 0 ? -v : v; };
     auto dqf = [&](int l) { int v = s[l][6] - 2 * s[l][5] + s[l][4]; return v < 0 ? -v : v; };
     int dp0 = dpf(0), dp3 = dpf(3), dq0 = dqf(0), dq3 = dqf(3);
@@ -672,6 +677,10 @@ __device__ __forceinline__ void deblock_luma(const DevPic* __restrict__ pics, co
     if (vertical) { s = idx / n_e; e = idx - s * n_e; } else { e = idx / n_s; s = idx - e * n_s; }
     const int edge = 8 * (e + 1), scan = 4 * s, mid = scan + 1;
     const int xq = vertical ? edge : mid, yq = vertical ? mid : edge;
+    // the segment's samples load while the edge's strength and QP are decided
+    T* const seg = samples + P.plane_off[0] + (vertical ? (size_t)scan * P.w + edge : (size_t)edge * P.w + scan);
+    int32_t px[4][8];
+    luma_segment_load<T>(seg, vertical != 0, P.w, px);
     const int bs = edge_bs(P, info, vertical, xq, yq);
     if (!bs) return;
     const uint32_t iq = cell_of(P, info, xq, yq);
@@ -684,15 +693,13 @@ __device__ __forceinline__ void deblock_luma(const DevPic* __restrict__ pics, co
     const int ti = clampi(avg + 2 * (bs - 1) + sd.tc_offset_div2 * 2, 0, 53);
     const int beta = kBeta[bi] << (P.bd - 8), tc = kTc[ti] << (P.bd - 8);
     if (!tc) return;
-    T* pl = samples + P.plane_off[0];
     bool wp = true, wq = true;
     if (P.suppress_any) {
         const int xp = vertical ? edge - 1 : scan, yp = vertical ? scan : edge - 1;
         wp = !suppressed(P, cell_of(P, info, xp, yp));
         wq = !suppressed(P, cell_of(P, info, vertical ? edge : scan, vertical ? scan : edge));
     }
-    if (vertical) luma_segment<T>(pl + (size_t)scan * P.w + edge, true, P.w, beta, tc, (1 << P.bd) - 1, wp, wq);
-    else luma_segment<T>(pl + (size_t)edge * P.w + scan, false, P.w, beta, tc, (1 << P.bd) - 1, wp, wq);
+    luma_segment<T>(seg, vertical != 0, P.w, px, beta, tc, (1 << P.bd) - 1, wp, wq);
 }
 
 template <typename T>
