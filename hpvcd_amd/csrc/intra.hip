@@ -1,34 +1,6 @@
 // intra.hip — K2: availability masks and the intra prediction wavefront
 // (included by kernels.hip inside namespace hpvcd).
 
-// Optional cycle profiling of the wavefront (build with -DHPVCD_INTRA_PROFILE):
-// per unit and wave, shader-clock cycles spent in each phase.  Only records
-// timestamps; the computation is unchanged.
-#ifdef HPVCD_INTRA_PROFILE
-__device__ uint64_t* g_intra_prof = nullptr;
-__device__ uint64_t* g_intra_tl = nullptr;  // per unit: 260 wall-clock stamps (start, ready, levels..., end)
-#define HPVCD_TL(u, k) if (tid == 0 && g_intra_tl) g_intra_tl[(size_t)(u) * 260 + (k)] = wall_clock64();
-#ifdef HPVCD_TL_ONLY  // wall-clock stamps per unit and per prediction step only
-#define HPVCD_PROF_DECL
-#define HPVCD_PROF(k)
-#define HPVCD_PROF_FLUSH(u)
-#else
-#define HPVCD_PROF_DECL uint64_t prof_[8] = {0, 0, 0, 0, 0, 0, 0, 0}; uint64_t prof_t_ = clock64();
-#define HPVCD_PROF(k) { const uint64_t c_ = clock64(); prof_[k] += c_ - prof_t_; prof_t_ = c_; }
-#define HPVCD_PROF_FLUSH(u)                                                                                 \
-    if (lane == 0 && g_intra_prof)                                                                          \
-        for (int k_ = 0; k_ < 8; k_++) g_intra_prof[((size_t)(u) * 2 + wave) * 8 + k_] = prof_[k_];
-#endif
-__device__ uint64_t* g_prep_prof = nullptr;  // per unit: cycles of the k_prep phases
-#define HPVCD_PREP_T(k) if (tid == 0 && g_prep_prof) g_prep_prof[(size_t)blockIdx.x * 8 + (k)] = clock64();
-#else
-#define HPVCD_PREP_T(k)
-#define HPVCD_PROF_DECL
-#define HPVCD_PROF(k)
-#define HPVCD_PROF_FLUSH(u)
-#define HPVCD_TL(u, k)
-#endif
-
 constexpr int kMaxCtb = 64;
 constexpr int kTopW = 2 * kMaxCtb + 1;            // row above the unit: x0-1 .. x0+2*ctb-1
 constexpr int kRowW = kMaxCtb + 1;                // body rows: left neighbour + the unit's samples
@@ -258,7 +230,6 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
     const bool need_info = (pflags & (HPVCD_PF_TILES | HPVCD_PF_CONSTRAINED_INTRA)) || num_slices > 1;
     const uint32_t* ord_g = cell_order + cell_begin;
     const uint32_t* inf_g = cell_info + cell_begin;
-    HPVCD_PREP_T(0)
     if (tid < 33) angles[tid] = kIntraAngle[tid];
     if (tid < 15) inv_angles[tid] = kInvAngle[tid];
     // The window's cells that can precede the unit's TBs in decode order:
@@ -322,7 +293,6 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
     const int n4 = (int)(L.cnt[0] + L.cnt[1] + L.cnt[2] + L.cnt[3]);
     const int n8 = (int)(L.cnt[4] + L.cnt[5] + L.cnt[6] + L.cnt[7]);
     __syncthreads();
-    HPVCD_PREP_T(1)
     const int lgv = 2 - sh, lgh = 2 - sw;
     const int gv = 4 >> sh, gh = 4 >> sw;
     const bool strong_on = plane == 0 && (pflags & HPVCD_PF_STRONG_INTRA_SMOOTHING);
@@ -445,7 +415,6 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             dst[p] = pcm ? (uint32_t)kZero | ((uint32_t)kZero << 13) : sample_word(s, p & (n - 1), p >> log2n, Ms);
     }
     __syncthreads();
-    HPVCD_PREP_T(5)
 }
 
 // Per-TB descriptor, decoded once per TB with scalar bit-field extracts.
@@ -761,8 +730,6 @@ __global__ void __launch_bounds__(kIntraThreads)
         T* pl = samples + P.plane_off[plane];
         const R* rp = res_pool + P.res_off[plane];
         const int ntb = U.ntb;
-        HPVCD_PROF_DECL
-        HPVCD_TL(ui, 0)
 
         // stage the unit: TB descriptors, reference maps and words (all
         // prepared before the wavefront) and residuals, in 16-byte (words,
@@ -830,8 +797,6 @@ __global__ void __launch_bounds__(kIntraThreads)
             L.tile[kZero] = 0;
         }
         __syncthreads();
-        HPVCD_PROF(7)
-        HPVCD_TL(ui, 1)
 
         uint32_t* const bw = border + (size_t)ui * kBorderWords;
         const uint32_t tagw = tag << 16;
@@ -866,11 +831,7 @@ __global__ void __launch_bounds__(kIntraThreads)
             };
             int ready = 0;
             uint32_t spins = 0;
-            HPVCD_PROF(3)
             for (;;) {
-#if defined(HPVCD_INTRA_PROFILE) && !defined(HPVCD_TL_ONLY)
-                prof_[5]++;
-#endif
                 // poll only the missing words that the next few TBs read
                 const int horizon = min(ntb, ready + kFetchAhead) - 1;
                 const uint32_t hn = L.pneed[horizon];
@@ -930,7 +891,6 @@ __global__ void __launch_bounds__(kIntraThreads)
                 }
                 __builtin_amdgcn_s_sleep(8);  // back off: the words come from units still running
             }
-            HPVCD_PROF(4)
         } else {
             // ---- prediction ----
             // inter samples on the unit's bottom row and right column are
@@ -977,11 +937,9 @@ __global__ void __launch_bounds__(kIntraThreads)
                 nl2 = dw[4 * nli + 2];
             }
             int ready = 0;
-            HPVCD_PROF(0)
             for (int i = 0; i < ntb;) {
                 const int g = step_size(cd0);
                 if (i + g > ready) {
-                    HPVCD_PROF(2)
                     uint32_t spins = 0;
                     for (;;) {
                         ready = (int)uni(__hip_atomic_load(&L.bready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
@@ -994,7 +952,6 @@ __global__ void __launch_bounds__(kIntraThreads)
                         __builtin_amdgcn_s_sleep(1);
                     }
                     lds_acquire();
-                    HPVCD_PROF(1)
                 }
                 if ((cd0 >> 21) & 1) {
                     // a run of fast steps, each ready and followed by a fast one
@@ -1018,8 +975,6 @@ __global__ void __launch_bounds__(kIntraThreads)
                         const bool active = step_active(cd0, lane16);
                         intra_step_fast(L.tile, cl0, cl2, cw, cr, b8, (cd0 >> 29) & 1, x, y, loff, active, maxv);
                         if ((cd0 >> 28) & 1) intra_step_publish(L.tile, cl0, b8, x, y, loff, active, bw, tagw);
-                        HPVCD_PROF(3)
-                        HPVCD_TL(ui, 2 + i)
                         i = next;
                         cd0 = nd0;
                         cd1 = nd1;
@@ -1051,12 +1006,11 @@ __global__ void __launch_bounds__(kIntraThreads)
                         ml2 = dw[4 * nli + 2];
                     }
                     switch ((cd0 >> 12) & 3) {
-                        case 0: intra_tb<2, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); HPVCD_PROF(3) break;
-                        case 1: intra_tb<3, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); HPVCD_PROF(4) break;
-                        case 2: intra_tb<4, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); HPVCD_PROF(5) break;
-                        default: intra_tb<5, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); HPVCD_PROF(6) break;
+                        case 0: intra_tb<2, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); break;
+                        case 1: intra_tb<3, R>(L, resb, cd0, cd1, cd2, true, cw, cr, lane, bd, maxv, bw, tagw); break;
+                        case 2: intra_tb<4, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); break;
+                        default: intra_tb<5, R>(L, resb, cd0, cd1, cd2, false, 0, 0, lane, bd, maxv, bw, tagw); break;
                     }
-                    HPVCD_TL(ui, 2 + i)
                     i = next;
                     cd0 = nd0;
                     cd1 = nd1;
@@ -1072,11 +1026,8 @@ __global__ void __launch_bounds__(kIntraThreads)
                     nl2 = ml2;
                 }
             }
-            HPVCD_PROF(2)
         }
         __syncthreads();
-        HPVCD_PROF_FLUSH(ui)
-        HPVCD_TL(ui, 259)
         // the whole CTB to the plane, four samples per store (plane rows
         // start at multiples of 4 samples)
         for (int e = 4 * tid; e < ctbw * ctbh; e += 4 * kIntraThreads) {

@@ -124,6 +124,10 @@ hpvcd_status validate_batch(const hpvcd_batch& b) {
         if (P.width <= 0 || P.height <= 0 || P.width > 16384 || P.height > 16384 || (P.width & 7) || (P.height & 7))
             return set_error(HPVCD_E_BAD_DIMENSIONS, "picture size must be a positive multiple of 8");
         if (P.chroma_idc > 3) return set_error(HPVCD_E_UNSUPPORTED_CHROMA, "chroma_idc");
+        // 4:2:2 (two stacked chroma TBs per luma TB, the remapped chroma
+        // modes, chroma CTBs of ctb/2 x ctb) has no record producer and no
+        // parity evidence yet, so the back end refuses it
+        if (P.chroma_idc == 2) return set_error(HPVCD_E_UNSUPPORTED_CHROMA, "4:2:2 pictures are not supported");
         if (P.bit_depth_luma < 8 || P.bit_depth_luma > 12 || P.bit_depth_chroma < 8 || P.bit_depth_chroma > 12)
             return set_error(HPVCD_E_UNSUPPORTED_BITDEPTH, "bit depth must be 8..12");
         if ((P.bit_depth_luma > 8) != (P.bit_depth_chroma > 8))
@@ -806,11 +810,11 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
         const uint32_t p0 = plain ? (uint32_t)step : C.wave_begin[step];
         const uint32_t p1 = plain ? (uint32_t)step + 1 : C.wave_begin[step + 1];
         const uint32_t pn = p1 - p0;
-        // a step holding a picture that later pictures predict from completes
-        // the loop filters whatever the stop stage: MC reads final samples
-        bool referenced = false;
-        for (uint32_t p = p0; p < p1; p++) referenced |= C.pic_is_ref[p] != 0;
-        const int st = referenced ? std::max(stop, (int)HPVCD_STAGE_SAO) : stop;
+        // pictures that later pictures predict from (a prefix of the step)
+        // complete the loop filters whatever the stop stage: MC reads final
+        // samples; the others stop at the stop stage
+        uint32_t pref = p0;
+        while (pref < p1 && C.pic_is_ref[pref]) pref++;
         const bool prep = step == 0 && !C.prep_list.empty();
         if (prep) {
             // reference availability, reference maps, per-sample reference
@@ -862,29 +866,38 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
         }
         mark(-1);
         (void)nunits;
-        if (st >= HPVCD_STAGE_DEBLOCK) {
+        // the loop filters over pictures [q0, q1) of the step
+        auto filters = [&](uint32_t q0, uint32_t q1, int upto) {
+            const uint32_t qn = q1 - q0;
+            if (!qn || upto < HPVCD_STAGE_DEBLOCK) return;
             mark(HPVCD_KERNEL_DEBLOCK);
-            uint32_t lv = plain ? (uint32_t)((C.pics[p0].w >= 12 ? (C.pics[p0].w - 4) / 8 : 0) * (C.pics[p0].h / 4)) : C.max_luma_v;
-            uint32_t lh = plain ? (uint32_t)((C.pics[p0].h >= 12 ? (C.pics[p0].h - 4) / 8 : 0) * (C.pics[p0].w / 4)) : C.max_luma_h;
-            uint32_t cv = plain ? (uint32_t)((C.pics[p0].cw >= 10 ? (C.pics[p0].cw - 2) / 8 : 0) * (C.pics[p0].ch / 4)) : C.max_chroma_v;
-            uint32_t ch = plain ? (uint32_t)((C.pics[p0].ch >= 10 ? (C.pics[p0].ch - 2) / 8 : 0) * (C.pics[p0].cw / 4)) : C.max_chroma_h;
+            const DevPic& D0 = C.pics[q0];
+            uint32_t lv = plain ? (uint32_t)((D0.w >= 12 ? (D0.w - 4) / 8 : 0) * (D0.h / 4)) : C.max_luma_v;
+            uint32_t lh = plain ? (uint32_t)((D0.h >= 12 ? (D0.h - 4) / 8 : 0) * (D0.w / 4)) : C.max_luma_h;
+            uint32_t cv = plain ? (uint32_t)((D0.cw >= 10 ? (D0.cw - 2) / 8 : 0) * (D0.ch / 4)) : C.max_chroma_v;
+            uint32_t ch = plain ? (uint32_t)((D0.ch >= 10 ? (D0.ch - 2) / 8 : 0) * (D0.cw / 4)) : C.max_chroma_h;
             if (lv + cv)
-                k_deblock<T><<<dim3(cdiv(lv, 256) + cdiv(cv, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 1, p0,
+                k_deblock<T><<<dim3(cdiv(lv, 256) + cdiv(cv, 256), qn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 1, q0,
                                                                                  cdiv(lv, 256));
             if (lh + ch)
-                k_deblock<T><<<dim3(cdiv(lh, 256) + cdiv(ch, 256), pn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 0, p0,
+                k_deblock<T><<<dim3(cdiv(lh, 256) + cdiv(ch, 256), qn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 0, q0,
                                                                                  cdiv(lh, 256));
             mark(-1);
-        }
-        if (st >= HPVCD_STAGE_SAO) {
+            if (upto < HPVCD_STAGE_SAO) return;
             mark(HPVCD_KERNEL_SAO);
-            uint32_t nctb = plain ? (uint32_t)(C.pics[p0].ctb_cols * C.pics[p0].ctb_rows) : C.max_ctbs;
+            uint32_t nctb = plain ? (uint32_t)(D0.ctb_cols * D0.ctb_rows) : C.max_ctbs;
             // a launch of few CTBs splits each luma CTB's rows over four workgroups
-            const int parts = nctb * pn < 4u * (uint32_t)g->num_cus ? 4 : 1;
-            k_sao<T><<<dim3(nctb, pn, parts + 2), 256, 0, s>>>(C.d_pics, C.d_cells, C.d_sao, pa, pb, p0, parts);
+            const int parts = nctb * qn < 4u * (uint32_t)g->num_cus ? 4 : 1;
+            k_sao<T><<<dim3(nctb, qn, parts + 2), 256, 0, s>>>(C.d_pics, C.d_cells, C.d_sao, pa, pb, q0, parts);
             mark(-1);
+        };
+        if (stop >= HPVCD_STAGE_SAO || pref == p0) {
+            filters(p0, p1, stop);
+        } else {
+            filters(p0, pref, HPVCD_STAGE_SAO);
+            filters(pref, p1, stop);
         }
-        if (st >= HPVCD_STAGE_RGB) {
+        if (stop >= HPVCD_STAGE_RGB) {
             mark(HPVCD_KERNEL_RGB);
             // quads of 4 pixels: 64 of a row by 4 rows per workgroup
             const uint32_t qx = plain ? (uint32_t)((C.pics[p0].copy_w + 3) / 4) : C.max_rgb_qx;
@@ -933,25 +946,6 @@ extern "C" {
 const char* hpvcd_last_error(void) { return g_last_error.c_str(); }
 const char* hpvcd_version(void) { return "hpvcd-mi355x 0.1.0 (gfx950)"; }
 
-#ifdef HPVCD_INTRA_PROFILE
-// profiling builds only: device buffer of units * (kIntraWaves + 1) waves * 8 cycle counters
-int hpvcd_debug_intra_profile(void* device_buffer) {
-    return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_intra_prof), &device_buffer, sizeof(device_buffer));
-}
-// profiling builds only: device buffer of units * 260 wall-clock stamps
-int hpvcd_debug_intra_timeline(void* device_buffer) {
-    return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_intra_tl), &device_buffer, sizeof(device_buffer));
-}
-int hpvcd_debug_prep_profile(void* device_buffer) {
-    return (int)hipMemcpyToSymbol(HIP_SYMBOL(g_prep_prof), &device_buffer, sizeof(device_buffer));
-}
-int hpvcd_debug_unit_info(hpvcd_job* j, uint32_t u, uint32_t* out) {
-    if (j->classes.empty() || u >= j->classes[0].units.size()) return -1;
-    const auto& U = j->classes[0].units[u];
-    out[0] = U.pic; out[1] = U.cx; out[2] = U.cy; out[3] = U.plane; out[4] = U.ntb; out[5] = U.nlevels;
-    return (int)j->classes[0].units.size();
-}
-#endif
 
 hpvcd_status hpvcd_validate_batch(const hpvcd_batch* b) {
     if (!b) return set_error(HPVCD_E_INVALID, "null argument");
@@ -1123,9 +1117,14 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
             depth[p] = std::max(depth[p], depth[r] + 1);
         }
     }
+    // within a depth, pictures that later pictures reference come first: the
+    // launch chain filters that prefix to SAO whatever the stop stage, and the
+    // rest only to the stop stage
     for (auto& C : j->classes) {
-        std::stable_sort(C.batch_pics.begin(), C.batch_pics.end(),
-                         [&](uint32_t a, uint32_t c) { return depth[a] < depth[c]; });
+        std::stable_sort(C.batch_pics.begin(), C.batch_pics.end(), [&](uint32_t a, uint32_t c) {
+            if (depth[a] != depth[c]) return depth[a] < depth[c];
+            return is_ref[a] > is_ref[c];
+        });
         for (uint32_t i = 0; i < C.batch_pics.size(); i++) j->pic_local[C.batch_pics[i]] = i;
     }
     size_t sc_bytes = (size_t)b->num_scaling * HPVCD_SCALING_SET_BYTES;

@@ -11,6 +11,8 @@ oracle's and the kernels' tables against the reference's own.  Sources:
   src/intra.rs      INTRA_PRED_ANGLE (:31), INV_ANGLE (:37)
   src/deblock.rs    BETA (:35), TC (:44)
   src/decode.rs     qpc table (:7599)
+  src/mc.rs         LUMA_FILTER (:30), CHROMA_FILTER (:37)
+  src/config.rs     QUANT_INTRA_DEFAULT_8X8 (:360), QUANT_INTER_DEFAULT_8X8 (:366)
 """
 import json
 import os
@@ -36,11 +38,20 @@ def int_table(src: str, name: str):
     return [int(v) for v in re.findall(r"-?\d+", m.group(1))]
 
 
+def int_rows(src: str, name: str, width: int):
+    """A static 2-D table [[T; width]; rows] as a list of rows."""
+    m = re.search(r"static %s: \[\[[^\]]*\]; \d+\] = \[(.*?)\];" % re.escape(name), src, re.S)
+    vals = [int(v) for v in re.findall(r"-?\d+", m.group(1))]
+    return [vals[i:i + width] for i in range(0, len(vals), width)]
+
+
 def main(ref: str) -> None:
     tr = open(os.path.join(ref, "src/transform.rs")).read()
     intra = open(os.path.join(ref, "src/intra.rs")).read()
     deb = open(os.path.join(ref, "src/deblock.rs")).read()
     dec = open(os.path.join(ref, "src/decode.rs")).read()
+    mc = open(os.path.join(ref, "src/mc.rs")).read()
+    cfg = open(os.path.join(ref, "src/config.rs")).read()
     dst = re.search(r"static DST4: \[\[i32; 4\]; 4\] = \[(.*?)\];", tr, re.S).group(1)
     dst_rows = [[int(v) for v in re.findall(r"-?\d+", r)] for r in re.findall(r"\[([^\]]*)\]", dst)]
     ls = re.search(r"const DEQUANT_SCALE: \[i64; 6\] = \[(.*?)\];", tr).group(1)
@@ -56,6 +67,10 @@ def main(ref: str) -> None:
         "beta": int_table(deb, "BETA"),
         "tc": int_table(deb, "TC"),
         "qpc_30_43": [int(v) for v in re.findall(r"\d+", qpc)],
+        "luma_filter": int_rows(mc, "LUMA_FILTER", 8),
+        "chroma_filter": int_rows(mc, "CHROMA_FILTER", 4),
+        "scaling_default_intra_8x8": int_table(cfg, "QUANT_INTRA_DEFAULT_8X8"),
+        "scaling_default_inter_8x8": int_table(cfg, "QUANT_INTER_DEFAULT_8X8"),
     }
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_tables.json")
     with open(out, "w") as f:

@@ -124,9 +124,9 @@ def pcm_case(seed=0, chroma_idc=1, bd=8, pcm_bits=5, log2_cu=4, size=32):
     return rs, planes
 
 
-def inter_case(ref_luma, ref_cb=None, ref_cr=None, mvs=((0, 0),), bi=False, weights=None, pu=16):
-    """Two 8-bit pictures: picture 0 is I_PCM carrying the given planes
-    exactly (PcmBitDepth 8); picture 1 is tiled by pu x pu inter PUs (the i-th
+def inter_case(ref_luma, ref_cb=None, ref_cr=None, mvs=((0, 0),), bi=False, weights=None, pu=16, bd=8):
+    """Two pictures of bit depth bd: picture 0 is I_PCM carrying the given
+    planes exactly (PcmBitDepth = bd); picture 1 is tiled by pu x pu inter PUs (the i-th
     takes mvs[i % len(mvs)], list 0, and list 1 too when bi) whose CUs code
     no residual, with the loop filters off, so its reconstruction is the
     motion-compensated prediction.  weights: None, or ((w, o) per Y/Cb/Cr for
@@ -146,7 +146,7 @@ def inter_case(ref_luma, ref_cb=None, ref_cr=None, mvs=((0, 0),), bi=False, weig
             words.append(dense_levels(np.asarray(plane)[y:y + m, x:x + m]))
             tbs.append(tb(x, y, 4 if comp == 0 else 3, comp, 1, 0, N.TB_PCM, off, m * m))
             off += m * m
-    p0 = picture(w, h, cidc, 8)
+    p0 = picture(w, h, cidc, bd)
     p0.tb_count, p0.coeff_count = len(tbs), off
     t1 = []
     for cx, cy in order:  # picture 1: inter TBs without residual
@@ -172,7 +172,7 @@ def inter_case(ref_luma, ref_cb=None, ref_cr=None, mvs=((0, 0),), bi=False, weig
         for c in range(3):
             e.weight[c], e.offset[c] = weights[l][c] if weights else (64, 0)
         refs.append(e)
-    p1 = picture(w, h, cidc, 8)
+    p1 = picture(w, h, cidc, bd)
     p1.tb_begin, p1.tb_count = len(tbs), len(t1)
     p1.coeff_begin, p1.coeff_count = off, 0
     p1.ctb_begin = ((w + 15) // 16) * ((h + 15) // 16)
@@ -181,10 +181,82 @@ def inter_case(ref_luma, ref_cb=None, ref_cr=None, mvs=((0, 0),), bi=False, weig
     p1.out_index = 1
     c0 = cells(w, h, 30, N.CELL_PCM)
     c1 = np.full_like(c0, (30 & 0xff) << 8)
+    kind = N.OUT_RGB16 if bd > 8 else N.OUT_RGB8
+    if not cidc:
+        kind = N.OUT_LUMA16 if bd > 8 else N.OUT_LUMA8
     rs = H.RecordSet.from_arrays([p0, p1], np.array(tbs + t1, dtype=H._TB_DTYPE), np.concatenate(words),
-                                 np.concatenate([c0, c1]), [output(w, h, cidc), output(w, h, cidc)],
+                                 np.concatenate([c0, c1]), [output(w, h, cidc, kind), output(w, h, cidc, kind)],
                                  pus=pus, refs=refs)
     return rs
+
+
+def cell_word(bsv=0, bsh=0, extra=N.CELL_INTRA, qp=30, slice_idx=0, tile=0):
+    """One per-4x4 cell info word (HPVCD_CELL_PACK)."""
+    return (bsv & 3) | ((bsh & 3) << 2) | extra | ((qp & 0xff) << 8) | ((slice_idx & 0xff) << 16) | \
+        ((tile & 0xff) << 24)
+
+
+def pcm_picture(planes, chroma_idc, bd, cell_words, log2_ctb=4, cu=8, flags=0, slices=((0, 0, 0, 1),),
+                pps_cb_qp_offset=0, pps_cr_qp_offset=0, sao=None, deblock=True):
+    """One picture whose CUs are all I_PCM CUs of cu x cu carrying `planes`
+    exactly (PcmBitDepth = bit depth, loop-filtered), so the loop filters see
+    chosen samples.  cell_words: the per-4x4 cell words (bS, QP, slice, tile)
+    in raster order; slices: (disabled, beta_offset_div2, tc_offset_div2,
+    lf_across) per slice; sao: one hpvcd_sao per CTB or None (all off).
+    Returns (picture, tbs, coefficient words, cells, sao list)."""
+    y = np.asarray(planes[0])
+    h, w = y.shape
+    sw = 2 if chroma_idc in (1, 2) else 1
+    sh = 2 if chroma_idc == 1 else 1
+    ctb = 1 << log2_ctb
+    tbs, words = [], []
+    off = 0
+    for by in range(0, h, ctb):
+        for bx in range(0, w, ctb):
+            for k in range((ctb // cu) ** 2):
+                # z-order of the CUs inside the CTB
+                zx = sum(((k >> (2 * b)) & 1) << b for b in range(4))
+                zy = sum(((k >> (2 * b + 1)) & 1) << b for b in range(4))
+                cx, cy = bx + zx * cu, by + zy * cu
+                if cx >= w or cy >= h:
+                    continue
+                for comp in range(3 if chroma_idc else 1):
+                    m = cu if comp == 0 or chroma_idc == 3 else cu // 2
+                    x0, y0 = (cx, cy) if comp == 0 else (cx // sw, cy // sh)
+                    blk = np.asarray(planes[comp])[y0:y0 + m, x0:x0 + m]
+                    words.append(dense_levels(blk))
+                    tbs.append(tb(x0, y0, m.bit_length() - 1, comp, 1, 0, N.TB_PCM, off, m * m))
+                    off += m * m
+    p = picture(w, h, chroma_idc, bd, log2_ctb=log2_ctb, flags=flags, deblock=deblock)
+    p.num_slices = len(slices)
+    for i, (dis, bo, to, acr) in enumerate(slices):
+        p.slices[i].disabled = 0 if deblock and not dis else 1
+        p.slices[i].beta_offset_div2, p.slices[i].tc_offset_div2, p.slices[i].lf_across = bo, to, acr
+    p.pps_cb_qp_offset, p.pps_cr_qp_offset = pps_cb_qp_offset, pps_cr_qp_offset
+    p.tb_count, p.coeff_count = len(tbs), off
+    nctb = ((w + ctb - 1) // ctb) * ((h + ctb - 1) // ctb)
+    sao = list(sao) if sao is not None else [N.Sao() for _ in range(nctb)]
+    return p, tbs, np.concatenate(words), np.asarray(cell_words, dtype=np.uint32), sao
+
+
+def batch_of(pictures, kind=None):
+    """A record set over pcm_picture() results (one output image per picture)."""
+    pics, tbs, words, cells, sao, outs = [], [], [], [], [], []
+    tb0 = co0 = cell0 = ctb0 = 0
+    for i, (p, t, wd, c, s) in enumerate(pictures):
+        p.tb_begin, p.coeff_begin, p.cell_begin, p.ctb_begin, p.out_index = tb0, co0, cell0, ctb0, i
+        pics.append(p)
+        tbs += t
+        words.append(wd)
+        cells.append(c)
+        sao += s
+        tb0, co0, cell0, ctb0 = tb0 + len(t), co0 + len(wd), cell0 + len(c), ctb0 + len(s)
+        k = kind if kind is not None else (N.OUT_RGB16 if p.bit_depth_luma > 8 else N.OUT_RGB8)
+        if not p.chroma_idc:
+            k = N.OUT_LUMA16 if p.bit_depth_luma > 8 else N.OUT_LUMA8
+        outs.append(output(p.width, p.height, p.chroma_idc, k))
+    return H.RecordSet.from_arrays(pics, np.array(tbs, dtype=H._TB_DTYPE), np.concatenate(words),
+                                   np.concatenate(cells), outs, sao=sao)
 
 
 def shifted(plane, dx, dy):

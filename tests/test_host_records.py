@@ -258,9 +258,23 @@ def test_loop_filters_are_exercised():
 
 
 def test_lossless_cus_are_prediction_plus_levels():
-    """Transquant bypass: residual == levels, untouched by the loop filters."""
-    rs = H.RecordSet.generate("single512", p_tqb=1.0, p_sig=0.05, base_seed=19)
-    rec = O.picture_planes(rs.batch, 0, O.STAGE_RECON)
-    out = O.picture_planes(rs.batch, 0, O.STAGE_SAO)
-    for c in range(3):
-        assert np.array_equal(rec[c], out[c])
+    """Transquant bypass: residual == levels.  The first TB of each plane has
+    no available neighbour, so every intra mode predicts the neutral 1 << (bd - 1)
+    and its reconstruction is exactly 128 + its levels; the loop filters leave
+    lossless CUs untouched."""
+    for seed in (19, 23, 29):
+        rs = H.RecordSet.generate("single512", p_tqb=1.0, p_sig=0.05, base_seed=seed)
+        rec = O.picture_planes(rs.batch, 0, O.STAGE_RECON)
+        out = O.picture_planes(rs.batch, 0, O.STAGE_SAO)
+        tbs, coeffs = rs.tbs(), rs.coeffs()
+        for comp in range(3):
+            t = tbs[np.nonzero(tbs["comp"] == comp)[0][0]]
+            n = 1 << int(t["log2n"])
+            want = np.full((n, n), 128, np.int64)
+            if t["flags"] & N.TB_CBF:
+                for w in coeffs[t["coeff_off"]:t["coeff_off"] + t["coeff_cnt"]]:
+                    pos, level = int(w) & 0x3FF, int(np.int32(w)) >> 10
+                    want[pos // n, pos % n] += level
+            got = rec[comp][t["y"]:t["y"] + n, t["x"]:t["x"] + n]
+            assert np.array_equal(got, np.clip(want, 0, 255)), (seed, comp)
+            assert np.array_equal(rec[comp], out[comp])
