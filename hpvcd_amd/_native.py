@@ -51,6 +51,9 @@ OUT_RGB8_FROM16 = 2
 OUT_LUMA8 = 3
 OUT_LUMA16 = 4
 
+ORIENT_NORMAL, ORIENT_FLIP_H, ORIENT_ROTATE_180, ORIENT_FLIP_V = 1, 2, 3, 4
+ORIENT_TRANSPOSE, ORIENT_ROTATE_90, ORIENT_TRANSVERSE, ORIENT_ROTATE_270 = 5, 6, 7, 8
+
 SCALING_ENTRY_BYTES = 66
 SCALING_SET_BYTES = 4 * 6 * SCALING_ENTRY_BYTES
 
@@ -128,7 +131,7 @@ class Sao(C.Structure):
 
 class Output(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("kind", C.c_uint8), ("channels", C.c_uint8),
-                ("bytes_per_sample", C.c_uint8), ("_pad", C.c_uint8)]
+                ("bytes_per_sample", C.c_uint8), ("orientation", C.c_uint8)]
 
 
 class Batch(C.Structure):
@@ -169,6 +172,17 @@ class GenConfig(C.Structure):
         ("p_intra_cu", C.c_float), ("p_skip", C.c_float), ("p_bi", C.c_float), ("mv_sigma", C.c_float),
         ("config_id", C.c_uint32),
         ("base_seed", C.c_uint64),
+        ("scaling_list", C.c_uint8), ("cu_qp_delta", C.c_uint8), ("diff_cu_qp_delta_depth", C.c_uint8),
+        ("qp_delta_max", C.c_uint8),
+        ("pps_cb_qp_offset", C.c_int8), ("pps_cr_qp_offset", C.c_int8),
+        ("slice_cb_qp_offset", C.c_int8), ("slice_cr_qp_offset", C.c_int8),
+        ("beta_offset_div2", C.c_int8), ("tc_offset_div2", C.c_int8),
+        ("num_slices", C.c_uint8), ("tile_cols", C.c_uint8), ("tile_rows", C.c_uint8),
+        ("lf_across_tiles", C.c_uint8), ("lf_across_slices", C.c_uint8), ("random_slice_params", C.c_uint8),
+        ("orientation", C.c_uint8), ("sign_hiding", C.c_uint8),
+        ("conf_win", C.c_uint16 * 4),
+        ("p_sao_merge", C.c_float),
+        ("_pad2", C.c_uint8 * 4),
     ]
 
 

@@ -23,6 +23,7 @@ struct DevPic {
     int32_t chroma_idc, matrix;
     int32_t crop_left, crop_top, win_w, win_h, dst_x, dst_y, copy_w, copy_h;
     int32_t out_kind, out_width;    // output image kind / row length in pixels
+    int32_t out_height, orient;     // output canvas rows; HPVCD_ORIENT_* of the output image
     uint64_t out_off;               // byte offset of the output image in the output pool
     uint64_t plane_off[3];          // element offsets of Y/Cb/Cr in the sample pools
     uint64_t res_off[3];            // element offsets of Y/Cb/Cr in the residual pool

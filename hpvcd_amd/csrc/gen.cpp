@@ -82,6 +82,15 @@ struct PicGen {
     bool cu_chroma_dm[4];
     int cu_x0, cu_y0, cu_pu;
     bool cu_tqb;
+    // quantisation groups and the QpY predictor (src/decode.rs:2509 QG reset,
+    // :2590 QG entry, :3084 predict_qp); qp_y is the running QpY
+    int log2_qg = 6, log2_ctb = 6;
+    int slice_qp = 0, qp_prev = 0, qp_bd_off = 0;
+    bool qp_delta_coded = false;
+    // per raster CTB: slice index and tile id
+    std::vector<uint8_t> ctb_slice, ctb_tile;
+    int ctb_cols = 1;
+    size_t tb_first = 0;
 
     PicGen(const hpvcd_gen_config& c, uint64_t seed, std::vector<hpvcd_tb>& t, std::vector<uint32_t>& co,
            std::vector<hpvcd_pu>& pu_out, std::vector<hpvcd_ref>& ref_out, uint32_t pic_index)
@@ -95,8 +104,14 @@ struct PicGen {
         log2_min_cb = c.log2_min_cb;
         log2_min_tb = c.log2_min_tb;
         log2_max_tb = c.log2_max_tb;
-        qp_y = c.qp;
         bd = c.bit_depth;
+        qp_bd_off = 6 * (bd - 8);
+        qp_y = std::clamp<int>(c.qp, -qp_bd_off, 51);
+        slice_qp = qp_prev = qp_y;
+        log2_ctb = c.log2_ctb;
+        log2_qg = std::max<int>(c.log2_min_cb, c.log2_ctb - (c.cu_qp_delta ? c.diff_cu_qp_delta_depth : 0));
+        ctb_cols = (w + (1 << log2_ctb) - 1) >> log2_ctb;
+        tb_first = t.size();
         cell.assign((size_t)gw * gh, HPVCD_CELL_INTRA);
         cell_qp.assign((size_t)gw * gh, (int8_t)qp_y);
         mode_y.assign((size_t)gw * gh, MODE_DC);
@@ -279,6 +294,7 @@ struct PicGen {
         else inter_tiles(x0, y0, log2);
         for (int yy = y0; yy < std::min(y0 + cb, h); yy += 4)
             for (int xx = x0; xx < std::min(x0 + cb, w); xx += 4) cell_qp[(yy / 4) * gw + xx / 4] = (int8_t)qp_y;
+        qp_prev = qp_y;
         cu_inter = false;
     }
 
@@ -360,10 +376,42 @@ struct PicGen {
         return T[qpi - 30];
     }
 
+    // qPY_PRED of the quantisation group at (xqg, yqg): the left and above
+    // QpY inside the same CTB, else the QpY of the previous CU in decode
+    // order (src/decode.rs:3084 predict_qp; no WPP)
+    int predict_qp(int xqg, int yqg) {
+        const int ctb = 1 << log2_ctb;
+        const int cx = (xqg / ctb) * ctb, cy = (yqg / ctb) * ctb;
+        const int a = (xqg >= 1 && xqg - 1 >= cx) ? cell_qp[(size_t)(yqg / 4) * gw + (xqg - 1) / 4] : qp_prev;
+        const int b = (yqg >= 1 && yqg - 1 >= cy) ? cell_qp[(size_t)((yqg - 1) / 4) * gw + xqg / 4] : qp_prev;
+        return std::clamp((a + b + 1) >> 1, -qp_bd_off, 51);
+    }
+    // src/decode.rs:5768 derive_qpy_from_delta
+    int qpy_from_delta(int pred, int delta) {
+        const int m = 52 + qp_bd_off;
+        int q = (pred + delta + 52 + 2 * qp_bd_off) % m;
+        if (q < 0) q += m;
+        return std::clamp(q - qp_bd_off, -qp_bd_off, 51);
+    }
+    // cu_qp_delta of the first TU with a coded residual in a quantisation
+    // group (src/decode.rs:3528)
+    void maybe_qp_delta(bool residual) {
+        if (!cfg.cu_qp_delta || !residual || qp_delta_coded) return;
+        qp_delta_coded = true;
+        const int d = rng.sint(-(int)cfg.qp_delta_max, (int)cfg.qp_delta_max);
+        qp_y = qpy_from_delta(qp_y, d);
+    }
+
     // Qp' of a TB.  Under ACT the component offsets -5/-5/-3 apply to QpY
-    // before the chroma mapping and no PPS chroma offsets (src/decode.rs:3788).
+    // before the chroma mapping and no PPS chroma offsets (src/decode.rs:3788);
+    // otherwise chroma adds the PPS and slice offsets (src/decode.rs:5045).
     int tb_qp(int comp, bool act) {
-        if (!act) return comp == 0 ? qp_prime(qp_y, bd) : qp_prime(qpc_table(qp_y, cfg.chroma_idc, bd), bd);
+        if (!act) {
+            if (comp == 0) return qp_prime(qp_y, bd);
+            const int off = comp == 1 ? cfg.pps_cb_qp_offset + cfg.slice_cb_qp_offset
+                                      : cfg.pps_cr_qp_offset + cfg.slice_cr_qp_offset;
+            return qp_prime(qpc_table(qp_y + off, cfg.chroma_idc, bd), bd);
+        }
         int qpi = qp_y + (comp == 2 ? -3 : -5);
         if (comp == 0) return qp_prime(std::clamp(qpi, -6 * (bd - 8), 51), bd);
         return qp_prime(qpc_table(qpi, cfg.chroma_idc, bd), bd);
@@ -452,6 +500,7 @@ struct PicGen {
             mark_block_edges(x0, y0, size, 2);
         }
         uint8_t luma_mode = cu_inter ? MODE_DC : mode_y[(y0 / 4) * gw + x0 / 4];
+        maybe_qp_delta(cbf_luma || cbf_cb || cbf_cr);
         // tu_residual_act_flag is present for a TU with residual of an inter
         // CU, or of an intra CU whose chroma modes are all the derived mode
         // (src/decode.rs:4229)
@@ -518,10 +567,13 @@ struct PicGen {
         }
         for (int yy = y0; yy < std::min(y0 + size, h); yy += 4)
             for (int xx = x0; xx < std::min(x0 + size, w); xx += 4) cell_qp[(yy / 4) * gw + xx / 4] = (int8_t)qp_y;
+        qp_prev = qp_y;
     }
 
     void coding_unit(int x0, int y0, int log2) {
         int size = 1 << log2;
+        const int qgm = ~((1 << log2_qg) - 1);
+        if ((x0 & qgm) == x0 && (y0 & qgm) == y0) qp_y = predict_qp(x0, y0);
         cu_tqb = cfg.p_tqb > 0.f && rng.bern(cfg.p_tqb);
         if (cu_tqb) fill_cells(x0, y0, size, HPVCD_CELL_TQB);
         // cu_skip_flag / pred_mode_flag of an inter picture
@@ -562,6 +614,7 @@ struct PicGen {
         }
         for (int yy = y0; yy < std::min(y0 + size, h); yy += 4)
             for (int xx = x0; xx < std::min(x0 + size, w); xx += 4) cell_qp[(yy / 4) * gw + xx / 4] = (int8_t)qp_y;
+        qp_prev = qp_y;
     }
 
     void coding_quadtree(int x0, int y0, int log2) {
@@ -570,6 +623,7 @@ struct PicGen {
         bool split;
         if (x0 + size > w || y0 + size > h) split = log2 > log2_min_cb;
         else split = log2 > log2_min_cb && rng.bern(cfg.p_split_cu);
+        if (cfg.cu_qp_delta && log2 >= log2_qg) qp_delta_coded = false;
         if (split) {
             int half = size / 2;
             coding_quadtree(x0, y0, log2 - 1);
@@ -581,9 +635,19 @@ struct PicGen {
         }
     }
 
-    void sao_params(hpvcd_sao& s) {
+    // SAO parameters of CTB (rx, ry): merged from the left or above CTB
+    // where that CTB is in the same slice and tile (src/decode.rs:1507
+    // parse_sao), else drawn
+    void sao_params(std::vector<hpvcd_sao>& all, int rx, int ry, int cols) {
+        hpvcd_sao& s = all[(size_t)ry * cols + rx];
         std::memset(&s, 0, sizeof(s));
         if (!cfg.sao) return;
+        const size_t me = (size_t)ry * cols + rx;
+        auto avail = [&](size_t n) { return ctb_slice[n] == ctb_slice[me] && ctb_tile[n] == ctb_tile[me]; };
+        if (cfg.p_sao_merge > 0.f) {
+            if (rx > 0 && avail(me - 1) && rng.bern(cfg.p_sao_merge)) { s = all[me - 1]; return; }
+            if (ry > 0 && avail(me - cols) && rng.bern(cfg.p_sao_merge)) { s = all[me - cols]; return; }
+        }
         int ncomp = cfg.chroma_idc ? 3 : 1;
         for (int c = 0; c < ncomp; c++) {
             int type = c == 2 ? s.type_idx[1] : (int)rng.uni(3);
@@ -610,28 +674,126 @@ struct TileOut {
     std::vector<hpvcd_sao> sao;
     std::vector<hpvcd_pu> pus;
     std::vector<hpvcd_ref> refs;
+    int num_slices = 1, num_tiles = 1;
+};
+
+// Uniformly spaced tile boundaries in CTBs (uniform_spacing_flag, H.265
+// eq. 6-3/6-4) and the tile-scan order of the CTBs (CtbAddrTsToRs).
+struct TileLayout {
+    std::vector<int> col_bd, row_bd;
+    std::vector<uint32_t> ts_to_rs;
+    std::vector<uint8_t> tile_of;   // per raster CTB
+    TileLayout(int cols, int rows, int ntc, int ntr) {
+        ntc = std::max(1, std::min(ntc, cols));
+        ntr = std::max(1, std::min(ntr, rows));
+        for (int i = 0; i <= ntc; i++) col_bd.push_back(i * cols / ntc);
+        for (int j = 0; j <= ntr; j++) row_bd.push_back(j * rows / ntr);
+        tile_of.assign((size_t)cols * rows, 0);
+        for (int tj = 0; tj < ntr; tj++)
+            for (int ti = 0; ti < ntc; ti++)
+                for (int ry = row_bd[tj]; ry < row_bd[tj + 1]; ry++)
+                    for (int rx = col_bd[ti]; rx < col_bd[ti + 1]; rx++) {
+                        ts_to_rs.push_back((uint32_t)(ry * cols + rx));
+                        tile_of[(size_t)ry * cols + rx] = (uint8_t)(tj * ntc + ti);
+                    }
+    }
+    int num_tiles() const { return (int)(col_bd.size() - 1) * (int)(row_bd.size() - 1); }
 };
 
 void gen_tile(const hpvcd_gen_config& cfg, uint32_t tile, TileOut& out) {
     PicGen g(cfg, tile_seed(cfg.base_seed, cfg.config_id, tile), out.tbs, out.coeffs, out.pus, out.refs, tile);
     int ctb = 1 << cfg.log2_ctb;
     int cols = (g.w + ctb - 1) / ctb, rows = (g.h + ctb - 1) / ctb;
-    out.sao.resize((size_t)cols * rows);
-    for (int ry = 0; ry < rows; ry++)
-        for (int rx = 0; rx < cols; rx++) {
-            g.sao_params(out.sao[(size_t)ry * cols + rx]);
-            g.coding_quadtree(rx * ctb, ry * ctb, cfg.log2_ctb);
-        }
+    const int nctb = cols * rows;
+    out.sao.resize((size_t)nctb);
+    // CTBs in tile-scan order; slices are equal runs of that order
+    TileLayout T(cols, rows, std::max<int>(1, cfg.tile_cols), std::max<int>(1, cfg.tile_rows));
+    const int nslices = std::max(1, std::min<int>(std::max<int>(1, cfg.num_slices), std::min(nctb, HPVCD_MAX_SLICES)));
+    g.ctb_tile = T.tile_of;
+    g.ctb_slice.assign((size_t)nctb, 0);
+    for (int ts = 0; ts < nctb; ts++) g.ctb_slice[T.ts_to_rs[ts]] = (uint8_t)((int64_t)ts * nslices / nctb);
+    out.num_slices = nslices;
+    out.num_tiles = T.num_tiles();
+    for (int ts = 0; ts < nctb; ts++) {
+        const uint32_t rs = T.ts_to_rs[ts];
+        const int rx = (int)(rs % cols), ry = (int)(rs / cols);
+        const bool slice_start = ts == 0 || g.ctb_slice[rs] != g.ctb_slice[T.ts_to_rs[ts - 1]];
+        const bool tile_start = ts == 0 || g.ctb_tile[rs] != g.ctb_tile[T.ts_to_rs[ts - 1]];
+        if (slice_start || tile_start) g.qp_prev = g.slice_qp;  // src/decode.rs:776, :1262
+        if (slice_start) g.qp_y = g.slice_qp;
+        g.sao_params(out.sao, rx, ry, cols);
+        g.coding_quadtree(rx * ctb, ry * ctb, cfg.log2_ctb);
+    }
     if (g.inter_pic) g.finalize_bs();
     out.cells.resize(g.cell.size());
-    for (size_t i = 0; i < g.cell.size(); i++)
-        out.cells[i] = (g.cell[i] & 0xffu) | ((uint32_t)(uint8_t)g.cell_qp[i] << 8);
+    for (size_t i = 0; i < g.cell.size(); i++) {
+        const int gx = (int)(i % (size_t)g.gw), gy = (int)(i / (size_t)g.gw);
+        const size_t c = (size_t)((gy * 4) >> cfg.log2_ctb) * cols + ((gx * 4) >> cfg.log2_ctb);
+        out.cells[i] = (g.cell[i] & 0xffu) | ((uint32_t)(uint8_t)g.cell_qp[i] << 8) |
+                       ((uint32_t)g.ctb_slice[c] << 16) | ((uint32_t)g.ctb_tile[c] << 24);
+    }
+    // records list TBs in CTB raster order (decode order inside each CTB)
+    if (T.num_tiles() > 1) {
+        const int sw = g.sub_w, sh = g.sub_h;
+        auto ctb_of = [&](const hpvcd_tb& t) {
+            const int lx = t.comp ? t.x * sw : t.x, ly = t.comp ? t.y * sh : t.y;
+            return (ly >> cfg.log2_ctb) * cols + (lx >> cfg.log2_ctb);
+        };
+        std::stable_sort(out.tbs.begin(), out.tbs.end(),
+                         [&](const hpvcd_tb& a, const hpvcd_tb& b) { return ctb_of(a) < ctb_of(b); });
+    }
+}
+
+// Default lists (H.265 Tables 7-5 and 7-6, raster order as
+// src/config.rs:358-370 stores them).
+const uint8_t kDefaultIntra8x8[64] = {
+    16, 16, 16, 16, 17, 18, 21, 24, 16, 16, 16, 16, 17, 19, 22, 25, 16, 16, 17, 18, 20, 22, 25, 29,
+    16, 16, 18, 21, 24, 27, 31, 36, 17, 17, 20, 24, 30, 35, 41, 47, 18, 19, 22, 27, 35, 44, 54, 65,
+    21, 22, 25, 31, 41, 54, 70, 88, 24, 25, 29, 36, 47, 65, 88, 115};
+const uint8_t kDefaultInter8x8[64] = {
+    16, 16, 16, 16, 17, 18, 20, 24, 16, 16, 16, 17, 18, 20, 24, 25, 16, 16, 17, 18, 20, 24, 25, 28,
+    16, 17, 18, 20, 24, 25, 28, 33, 17, 18, 20, 24, 25, 28, 33, 41, 18, 20, 24, 25, 28, 33, 41, 54,
+    20, 24, 25, 28, 33, 41, 54, 71, 24, 25, 28, 33, 41, 54, 71, 91};
+
+// One record scaling set (sizeId 0..3 x matrixId 0..5 entries of 64 raster
+// coefficients, DC, flat flag): the default lists, or random lists of the
+// kind scaling_list_data codes explicitly.  sizeId 3 signals matrixId 0 and
+// 3 only; its chroma entries repeat sizeId 2's (src/config.rs:558-563).  The
+// flat flag follows ScalingList::refresh_stats (src/config.rs:409).
+void make_scaling_set(const hpvcd_gen_config& c, uint8_t* set) {
+    Rng rng{tile_seed(c.base_seed ^ 0x5ca1eull, c.config_id, 0)};
+    auto entry = [&](int size_id, int m) { return set + (size_t)(size_id * 6 + m) * HPVCD_SCALING_ENTRY_BYTES; };
+    for (int size_id = 0; size_id < 4; size_id++)
+        for (int m = 0; m < 6; m++) {
+            uint8_t* e = entry(size_id, m);
+            std::memset(e, 16, 64);
+            e[64] = 16;
+            if (size_id == 3 && m % 3) {
+                std::memcpy(e, entry(2, m), HPVCD_SCALING_ENTRY_BYTES);
+                continue;
+            }
+            const int used = size_id == 0 ? 16 : 64;
+            if (c.scaling_list == 1) {
+                if (size_id > 0) std::memcpy(e, m < 3 ? kDefaultIntra8x8 : kDefaultInter8x8, 64);
+            } else {
+                for (int k = 0; k < used; k++) e[k] = (uint8_t)rng.sint(4, 80);
+                if (size_id >= 2) e[64] = (uint8_t)rng.sint(4, 80);
+            }
+            bool flat = size_id < 2 || e[64] == 16;
+            for (int k = 0; k < used; k++) flat &= e[k] == 16;
+            e[65] = flat ? 1 : 0;
+        }
 }
 
 }  // namespace
 }  // namespace hpvcd
 
 using namespace hpvcd;
+
+static_assert(sizeof(hpvcd_gen_config) == 168, "hpvcd_gen_config layout");
+static_assert(sizeof(hpvcd_picture) == 140, "hpvcd_picture layout");
+static_assert(sizeof(hpvcd_tb) == 24 && sizeof(hpvcd_sao) == 36 && sizeof(hpvcd_output) == 12, "record layout");
+static_assert(sizeof(hpvcd_pu) == 20 && sizeof(hpvcd_ref) == 20, "inter record layout");
 
 extern "C" void hpvcd_gen_default_config(hpvcd_gen_config* c) {
     std::memset(c, 0, sizeof(*c));
@@ -674,6 +836,12 @@ extern "C" void hpvcd_gen_default_config(hpvcd_gen_config* c) {
     c->mv_sigma = 16.0f;
     c->config_id = 0;
     c->base_seed = 0x48505643ull;
+    c->qp_delta_max = 4;
+    c->lf_across_slices = 1;
+    c->lf_across_tiles = 1;
+    c->num_slices = 1;
+    c->tile_cols = 1;
+    c->tile_rows = 1;
 }
 
 extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_records** out) {
@@ -691,6 +859,22 @@ extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_rec
     if (c.log2_ctb < 4 || c.log2_ctb > 6 || c.log2_min_cb < 3 || c.log2_min_cb > c.log2_ctb || c.log2_min_tb != 2 ||
         c.log2_max_tb > 5 || c.log2_max_tb < 3)
         return set_error(HPVCD_E_PARAMSET, "generator: unsupported block size limits");
+    if (c.conf_win[0] + c.conf_win[1] >= c.tile_width || c.conf_win[2] + c.conf_win[3] >= c.tile_height)
+        return set_error(HPVCD_E_BAD_DIMENSIONS, "generator: conformance window leaves no picture");
+    {
+        const int cw = (c.chroma_idc == 1 || c.chroma_idc == 2) ? 2 : 1, ch = c.chroma_idc == 1 ? 2 : 1;
+        if (c.conf_win[0] % cw || c.conf_win[1] % cw || c.conf_win[2] % ch || c.conf_win[3] % ch)
+            return set_error(HPVCD_E_PARAMSET, "generator: conformance window offsets in chroma units");
+    }
+    if (c.orientation > HPVCD_ORIENT_ROTATE_270) return set_error(HPVCD_E_INVALID, "generator: orientation");
+    if (c.pps_cb_qp_offset < -12 || c.pps_cb_qp_offset > 12 || c.pps_cr_qp_offset < -12 || c.pps_cr_qp_offset > 12 ||
+        c.pps_cb_qp_offset + c.slice_cb_qp_offset < -12 || c.pps_cb_qp_offset + c.slice_cb_qp_offset > 12 ||
+        c.pps_cr_qp_offset + c.slice_cr_qp_offset < -12 || c.pps_cr_qp_offset + c.slice_cr_qp_offset > 12)
+        return set_error(HPVCD_E_PARAMSET, "generator: chroma QP offsets outside [-12, 12]");
+    if (c.beta_offset_div2 < -6 || c.beta_offset_div2 > 6 || c.tc_offset_div2 < -6 || c.tc_offset_div2 > 6)
+        return set_error(HPVCD_E_PARAMSET, "generator: deblocking offsets outside [-6, 6]");
+    if (c.cu_qp_delta && (c.diff_cu_qp_delta_depth > c.log2_ctb - c.log2_min_cb || c.qp_delta_max > 26))
+        return set_error(HPVCD_E_PARAMSET, "generator: cu_qp_delta depth or range");
     uint32_t ntiles = (uint32_t)(c.grid_cols * c.grid_rows);
     std::vector<TileOut> tiles(ntiles);
     unsigned nth = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
@@ -703,8 +887,12 @@ extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_rec
 
     hpvcd_records* r = new (std::nothrow) hpvcd_records();
     if (!r) return set_error(HPVCD_E_ALLOC, "generator: out of memory");
-    int sub_w = (c.chroma_idc == 1 || c.chroma_idc == 2) ? 2 : 1;
-    (void)sub_w;
+    if (c.scaling_list) {
+        r->scaling.resize(HPVCD_SCALING_SET_BYTES);
+        make_scaling_set(c, r->scaling.data());
+    }
+    const int win_w = c.tile_width - c.conf_win[0] - c.conf_win[1];
+    const int win_h = c.tile_height - c.conf_win[2] - c.conf_win[3];
     int nout = std::max<int>(1, c.num_outputs);
     for (int o = 0; o < nout; o++) {
         hpvcd_output ho;
@@ -714,6 +902,7 @@ extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_rec
         ho.kind = c.out_kind;
         ho.channels = c.chroma_idc ? 3 : 1;
         ho.bytes_per_sample = (c.out_kind == HPVCD_OUT_RGB16 || c.out_kind == HPVCD_OUT_LUMA16) ? 2 : 1;
+        ho.orientation = c.orientation;
         r->outputs.push_back(ho);
     }
     for (uint32_t i = 0; i < ntiles; i++) {
@@ -726,7 +915,7 @@ extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_rec
         p.bit_depth_luma = c.bit_depth;
         p.bit_depth_chroma = c.bit_depth;
         p.log2_ctb = c.log2_ctb;
-        uint32_t f = HPVCD_PF_LF_ACROSS_SLICES;
+        uint32_t f = c.lf_across_slices ? HPVCD_PF_LF_ACROSS_SLICES : 0u;
         if (c.strong_intra_smoothing) f |= HPVCD_PF_STRONG_INTRA_SMOOTHING;
         if (c.extended_precision) f |= HPVCD_PF_EXTENDED_PRECISION;
         if (c.implicit_rdpcm) f |= HPVCD_PF_IMPLICIT_RDPCM;
@@ -735,28 +924,44 @@ extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_rec
         if (c.full_range) f |= HPVCD_PF_FULL_RANGE;
         if (!c.deblock) f |= HPVCD_PF_DEBLOCK_DISABLED;
         if (c.pcm && c.pcm_lf_disabled) f |= HPVCD_PF_PCM_LF_DISABLED;
+        if (t.num_tiles > 1) f |= HPVCD_PF_TILES | (c.lf_across_tiles ? HPVCD_PF_LF_ACROSS_TILES : 0u);
         p.flags = f;
         p.matrix = c.matrix;
-        p.num_slices = 1;
-        p.slices[0].disabled = c.deblock ? 0 : 1;
-        p.slices[0].lf_across = 1;
+        p.pps_cb_qp_offset = c.pps_cb_qp_offset;
+        p.pps_cr_qp_offset = c.pps_cr_qp_offset;
+        // slice deblocking parameters: the configured offsets, or per slice a
+        // draw of disable, offsets and the across-slices flag
+        p.num_slices = (uint8_t)t.num_slices;
+        Rng prng{tile_seed(c.base_seed ^ 0x5b1cull, c.config_id, i)};
+        for (int sl = 0; sl < t.num_slices; sl++) {
+            hpvcd_slice_params& S = p.slices[sl];
+            S.disabled = c.deblock ? 0 : 1;
+            S.beta_offset_div2 = c.beta_offset_div2;
+            S.tc_offset_div2 = c.tc_offset_div2;
+            S.lf_across = c.lf_across_slices ? 1 : 0;
+            if (c.random_slice_params) {
+                if (c.deblock) S.disabled = prng.bern(0.2f) ? 1 : 0;
+                S.beta_offset_div2 = (int8_t)prng.sint(-6, 6);
+                S.tc_offset_div2 = (int8_t)prng.sint(-6, 6);
+                if (c.lf_across_slices) S.lf_across = prng.bern(0.5f) ? 1 : 0;
+            }
+        }
         p.slice_qp = c.qp;
         p.out_kind = c.out_kind;
         uint32_t col = i % (uint32_t)c.grid_cols, row = i / (uint32_t)c.grid_cols;
-        p.crop_left = 0;
-        p.crop_top = 0;
-        p.win_w = c.tile_width;
-        p.win_h = c.tile_height;
+        // conformance window: the visible tile, placed at (col, row) * window
+        // size in a grid (src/lib.rs:143 plane_window, :1754-1793)
+        p.crop_left = c.conf_win[0];
+        p.crop_top = c.conf_win[2];
+        p.win_w = win_w;
+        p.win_h = win_h;
         // one output per picture (frames, independent images) or one grid
-        p.dst_x = nout > 1 ? 0 : (int32_t)col * c.tile_width;
-        p.dst_y = nout > 1 ? 0 : (int32_t)row * c.tile_height;
-        p.copy_w = std::max(0, std::min(c.tile_width, c.out_width - p.dst_x));
-        p.copy_h = std::max(0, std::min(c.tile_height, c.out_height - p.dst_y));
-        p.out_index = (uint32_t)(i % (uint32_t)nout == 0 ? 0 : 0);
-        if (nout > 1) {  // one output per tile (serving-style independent images)
-            p.out_index = i % (uint32_t)nout;
-        }
-        p.scaling_index = -1;
+        p.dst_x = nout > 1 ? 0 : (int32_t)col * win_w;
+        p.dst_y = nout > 1 ? 0 : (int32_t)row * win_h;
+        p.copy_w = std::max(0, std::min(win_w, c.out_width - p.dst_x));
+        p.copy_h = std::max(0, std::min(win_h, c.out_height - p.dst_y));
+        p.out_index = nout > 1 ? i % (uint32_t)nout : 0u;
+        p.scaling_index = c.scaling_list ? 0 : -1;
         p.tb_begin = (uint32_t)r->tbs.size();
         p.tb_count = (uint32_t)t.tbs.size();
         p.coeff_begin = (uint32_t)r->coeffs.size();

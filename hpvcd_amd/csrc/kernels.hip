@@ -1044,8 +1044,31 @@ __global__ void __launch_bounds__(256) k_rgb(const DevPic* __restrict__ pics, co
             v[k][2] = (yt + cbb * cbc + 4096) >> 13;
         }
     }
+    const bool wide = P.out_kind == HPVCD_OUT_RGB16 || P.out_kind == HPVCD_OUT_LUMA16;
+    const int orient = P.orient;
+    if (orient >= HPVCD_ORIENT_FLIP_H && orient != HPVCD_ORIENT_TRANSPOSE && orient != HPVCD_ORIENT_TRANSVERSE) {
+        // oriented image (src/lib.rs:2065 rotate_luma, :2116 rotate_buf): each
+        // pixel of the canvas goes to its place in the oriented buffer
+        const int W = P.out_width, H = P.out_height, Y = P.dst_y + yy;
+        const int sh = (wide || bd == 8) ? 0 : bd - 8;
+        for (int k = 0; k < npx; k++) {
+            const int X = P.dst_x + xx0 + k;
+            size_t idx;
+            if (orient == HPVCD_ORIENT_FLIP_H) idx = (size_t)Y * W + (W - 1 - X);
+            else if (orient == HPVCD_ORIENT_ROTATE_180) idx = (size_t)(H - 1 - Y) * W + (W - 1 - X);
+            else if (orient == HPVCD_ORIENT_FLIP_V) idx = (size_t)(H - 1 - Y) * W + X;
+            else if (orient == HPVCD_ORIENT_ROTATE_90) idx = (size_t)X * H + (H - 1 - Y);
+            else idx = (size_t)(W - 1 - X) * H + Y;
+            for (int j = 0; j < chn; j++) {
+                const uint16_t sv = (uint16_t)clampi(v[k][j], 0, cmax);
+                if (wide) reinterpret_cast<uint16_t*>(out_pool + P.out_off)[idx * chn + j] = sv;
+                else out_pool[P.out_off + idx * chn + j] = (uint8_t)(sv >> sh);
+            }
+        }
+        return;
+    }
     const size_t o = ((size_t)(P.dst_y + yy) * P.out_width + (P.dst_x + xx0)) * chn;
-    if (P.out_kind == HPVCD_OUT_RGB16 || P.out_kind == HPVCD_OUT_LUMA16) {
+    if (wide) {
         uint16_t* d = reinterpret_cast<uint16_t*>(out_pool + P.out_off) + o;
         uint16_t b[12];
 #pragma unroll

@@ -118,6 +118,7 @@ hpvcd_status validate_batch(const hpvcd_batch& b) {
         if (O.width <= 0 || O.height <= 0 || O.width > 65536 || O.height > 65536)
             return set_error(HPVCD_E_BAD_DIMENSIONS, "output image size");
         if (O.kind > HPVCD_OUT_LUMA16) return set_error(HPVCD_E_INVALID, "output kind");
+        if (O.orientation > HPVCD_ORIENT_ROTATE_270) return set_error(HPVCD_E_INVALID, "output orientation");
     }
     for (uint32_t p = 0; p < b.num_pictures; p++) {
         const hpvcd_picture& P = b.pictures[p];
@@ -141,6 +142,11 @@ hpvcd_status validate_batch(const hpvcd_batch& b) {
         if ((uint64_t)P.ctb_begin + nctb > b.num_ctbs) return set_error(HPVCD_E_INVALID, "sao range");
         uint64_t ncell = (uint64_t)((P.width + 3) / 4) * ((P.height + 3) / 4);
         if ((uint64_t)P.cell_begin + ncell > b.num_cells) return set_error(HPVCD_E_INVALID, "cell range");
+        // the loop filters index slices[] by the cell's slice
+        const uint32_t nsl = P.num_slices ? P.num_slices : 1u;
+        for (uint64_t g = 0; g < ncell; g++)
+            if (HPVCD_CELL_SLICE(b.cell_info[P.cell_begin + g]) >= nsl)
+                return set_error(HPVCD_E_INVALID, "cell slice index beyond num_slices");
         if (P.scaling_index >= 0 && (uint32_t)P.scaling_index >= b.num_scaling)
             return set_error(HPVCD_E_INVALID, "scaling_index");
         if (P.out_index >= b.num_outputs) return set_error(HPVCD_E_INVALID, "out_index");
@@ -418,6 +424,8 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         D.dst_x = P.dst_x; D.dst_y = P.dst_y; D.copy_w = P.copy_w; D.copy_h = P.copy_h;
         D.out_kind = b.outputs[P.out_index].kind;
         D.out_width = b.outputs[P.out_index].width;
+        D.out_height = b.outputs[P.out_index].height;
+        D.orient = b.outputs[P.out_index].orientation;
         D.out_off = out_off[P.out_index];
         size_t ny = (size_t)D.w * D.h, nc = (size_t)D.cw * D.ch;
         D.plane_off[0] = plane_elems;

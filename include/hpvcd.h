@@ -80,6 +80,24 @@ typedef struct hpvcd_gen_config {
     float    mv_sigma;                  /* motion vector standard deviation in luma samples */
     uint32_t config_id;
     uint64_t base_seed;
+    /* quantisation, loop-filter and picture-structure options */
+    uint8_t  scaling_list;              /* 0 flat, 1 default lists (H.265 Tables 7-5/7-6), 2 random lists */
+    uint8_t  cu_qp_delta;               /* cu_qp_delta_enabled_flag: per-CU QpY deltas */
+    uint8_t  diff_cu_qp_delta_depth;    /* quantisation group = CTB >> depth */
+    uint8_t  qp_delta_max;              /* CuQpDeltaVal drawn from [-max, max] */
+    int8_t   pps_cb_qp_offset, pps_cr_qp_offset;
+    int8_t   slice_cb_qp_offset, slice_cr_qp_offset;
+    int8_t   beta_offset_div2, tc_offset_div2;   /* slice deblocking offsets (all slices) */
+    uint8_t  num_slices;                /* slices per picture: CTB runs in tile-scan order (0/1 = one) */
+    uint8_t  tile_cols, tile_rows;      /* uniform tile grid (0/1 = no tiles) */
+    uint8_t  lf_across_tiles;           /* loop_filter_across_tiles_enabled_flag */
+    uint8_t  lf_across_slices;          /* pps_loop_filter_across_slices_enabled_flag */
+    uint8_t  random_slice_params;       /* each slice draws its deblocking offsets, disable and across flags */
+    uint8_t  orientation;               /* EXIF orientation of the output images (0/1 = as decoded) */
+    uint8_t  sign_hiding;               /* sign_data_hiding_enabled_flag of the coded stream */
+    uint16_t conf_win[4];               /* conformance window: left, right, top, bottom (luma samples) */
+    float    p_sao_merge;               /* sao_merge_left / sao_merge_up where the neighbour CTB is available */
+    uint8_t  _pad2[4];
 } hpvcd_gen_config;
 
 typedef struct hpvcd_records hpvcd_records;   /* owned record set */

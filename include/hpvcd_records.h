@@ -206,12 +206,27 @@ typedef struct hpvcd_sao {
 #define HPVCD_OUT_LUMA8  3
 #define HPVCD_OUT_LUMA16 4
 
+/* EXIF orientation of an output image (src/metadata.rs:31 Orientation,
+ * applied after RGB conversion as src/lib.rs:2017 apply_orientation does).
+ * Pictures are placed on the width x height canvas as decoded; the output
+ * buffer holds the oriented image, whose size swaps for 5..8.  5 and 7
+ * (transpose, transverse) swap the size but keep the buffer as decoded, as
+ * the reference does. */
+#define HPVCD_ORIENT_NORMAL     1
+#define HPVCD_ORIENT_FLIP_H     2
+#define HPVCD_ORIENT_ROTATE_180 3
+#define HPVCD_ORIENT_FLIP_V     4
+#define HPVCD_ORIENT_TRANSPOSE  5
+#define HPVCD_ORIENT_ROTATE_90  6
+#define HPVCD_ORIENT_TRANSVERSE 7
+#define HPVCD_ORIENT_ROTATE_270 8
+
 typedef struct hpvcd_output {
-    int32_t  width, height;   /* output image size */
+    int32_t  width, height;   /* output canvas size (before orientation) */
     uint8_t  kind;            /* HPVCD_OUT_* */
     uint8_t  channels;        /* 3 or 1 */
     uint8_t  bytes_per_sample;
-    uint8_t  _pad;
+    uint8_t  orientation;     /* HPVCD_ORIENT_* (0 = as decoded) */
 } hpvcd_output;
 
 /* A batch of pictures.  All pointers are host pointers for submission. */

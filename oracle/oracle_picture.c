@@ -1005,6 +1005,36 @@ static void* batch_worker(void* arg) {
     return NULL;
 }
 
+/* src/lib.rs:2017 apply_orientation with :2065 rotate_luma / :2116
+ * rotate_buf on a whole output image: FlipH, Rotate180, FlipV, Rotate90 and
+ * Rotate270 move pixels; Transpose and Transverse (5, 7) keep the buffer as
+ * decoded (the reference's match arms fall through to the identity). */
+static int orient_output(const hpvcd_output* O, void* buf) {
+    int o = O->orientation;
+    if (o < HPVCD_ORIENT_FLIP_H || o == HPVCD_ORIENT_TRANSPOSE || o == HPVCD_ORIENT_TRANSVERSE) return 0;
+    size_t px = (size_t)O->width * O->height;
+    size_t chn = (O->kind == HPVCD_OUT_LUMA8 || O->kind == HPVCD_OUT_LUMA16) ? 1 : 3;
+    size_t bps = (O->kind == HPVCD_OUT_RGB16 || O->kind == HPVCD_OUT_LUMA16) ? 2 : 1;
+    size_t pb = chn * bps;
+    uint8_t* src = malloc(px * pb + 1);
+    if (!src) return -1;
+    memcpy(src, buf, px * pb);
+    uint8_t* dst = (uint8_t*)buf;
+    size_t W = (size_t)O->width, H = (size_t)O->height;
+    for (size_t y = 0; y < H; y++)
+        for (size_t x = 0; x < W; x++) {
+            size_t idx;
+            if (o == HPVCD_ORIENT_FLIP_H) idx = y * W + (W - 1 - x);
+            else if (o == HPVCD_ORIENT_ROTATE_180) idx = (H - 1 - y) * W + (W - 1 - x);
+            else if (o == HPVCD_ORIENT_FLIP_V) idx = (H - 1 - y) * W + x;
+            else if (o == HPVCD_ORIENT_ROTATE_90) idx = x * H + (H - 1 - y);
+            else idx = (W - 1 - x) * H + y;
+            memcpy(dst + idx * pb, src + (y * W + x) * pb, pb);
+        }
+    free(src);
+    return 0;
+}
+
 int oracle_decode_batch(const hpvcd_batch* b, void** outputs, int threads, int rows_parallel) {
     /* src/lib.rs:1680 fill_grid_bands: one job per grid row (pictures sharing
      * dst_y of one output), tiles of a row serial; or one job per picture.
@@ -1053,6 +1083,8 @@ int oracle_decode_batch(const hpvcd_batch* b, void** outputs, int threads, int r
         for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
     }
     for (size_t q = 0; q < 3 * (size_t)n; q++) free(J.refp[q]);
+    for (uint32_t o = 0; o < b->num_outputs && !J.err; o++)
+        if (outputs[o] && orient_output(&b->outputs[o], outputs[o])) J.err = 1;
     free(th); free(used); free(J.job_pics); free(J.job_first); free(J.refp); free(keep); free(depth);
     pthread_mutex_destroy(&J.mu);
     return J.err ? -1 : 0;

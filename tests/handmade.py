@@ -266,3 +266,46 @@ def shifted(plane, dx, dy):
     ys = np.clip(np.arange(h) + dy, 0, h - 1)
     xs = np.clip(np.arange(w) + dx, 0, w - 1)
     return plane[np.ix_(ys, xs)]
+
+
+# levels of the reference's dequantisation stress tests (src/transform.rs:1374-1382)
+SAFE_LEVELS = [0, 1, -1, 2, -2, 3, -3, 7, -7, 15, -15, 31, -31, 63, -63, 5]
+STRESS_LEVELS = [0, 1, -1, 255, -255, 4095, -4095, 32767, -32767, 100_000, -100_000, 1_000_000, -1_000_000, 17, -33,
+                 511]
+
+
+def stress_case(bd=8, ep=False, stress=True, ts=False, qp=None, seed=0, chroma_idc=3):
+    """A 64x64 intra picture (CTB 64, DC prediction) whose TBs carry every
+    position with levels from the reference's SAFE/STRESS tables, rotated per
+    TB: a 32x32, four 16x16, sixteen 8x8 and sixty-four 4x4 TBs in z-order,
+    each TU luma then (4:4:4) Cb and Cr of the same size.  ts sets transform
+    skip on the 4x4 TBs.  Exercises the clamp before the multiply, the
+    clamps after dequantisation and both transform stages at the given bit
+    depth with and without extended precision."""
+    src = STRESS_LEVELS if stress else SAFE_LEVELS
+    qp = (22 + 6 * (bd - 8)) if qp is None else qp
+    tbs, words = [], []
+    off = 0
+    rot = seed
+    tus = [(0, 0, 5)]
+    tus += [(32 + (k & 1) * 16, (k >> 1) * 16, 4) for k in range(4)]
+    tus += [(((k & 1) | ((k >> 1) & 2)) * 8, 32 + (((k >> 1) & 1) | ((k >> 2) & 2)) * 8, 3) for k in range(16)]
+    tus += [(32 + sum(((k >> (2 * b)) & 1) << b for b in range(3)) * 4,
+             32 + sum(((k >> (2 * b + 1)) & 1) << b for b in range(3)) * 4, 2) for k in range(64)]
+    for x, y, l2 in tus:
+        n = 1 << l2
+        for comp in range(3 if chroma_idc == 3 else 1):
+            lv = np.array([src[(i + rot) % len(src)] for i in range(n * n)], np.int64).reshape(n, n)
+            rot += 5
+            words.append(dense_levels(lv))
+            fl = N.TB_CBF | (N.TB_TS if ts and l2 == 2 else 0)
+            tbs.append(tb(x, y, l2, comp, 1, qp, fl, off, n * n))
+            off += n * n
+    flags = N.PF_DEBLOCK_DISABLED | (N.PF_EXTENDED_PRECISION if ep else 0)
+    p = picture(64, 64, chroma_idc, bd, log2_ctb=6, flags=flags)
+    p.tb_count, p.coeff_count = len(tbs), off
+    kind = N.OUT_RGB16 if bd > 8 else N.OUT_RGB8
+    if not chroma_idc:
+        kind = N.OUT_LUMA16 if bd > 8 else N.OUT_LUMA8
+    return H.RecordSet.from_arrays([p], np.array(tbs, dtype=H._TB_DTYPE), np.concatenate(words),
+                                   cells(64, 64, qp - 6 * (bd - 8)), [output(64, 64, chroma_idc, kind)])

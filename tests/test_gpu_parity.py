@@ -62,7 +62,57 @@ SMALL = {
     "inter444_10": dict(workload="rext4096", tile_width=256, tile_height=128, grid_cols=1, grid_rows=3,
                         num_outputs=3, out_width=256, out_height=128, inter=1, weighted=1, act=1, p_act=0.4,
                         base_seed=65),
+    # scaling lists: random lists (8-bit 4:2:0), default lists on 4:4:4 10-bit (32x32 chroma entries),
+    # random lists on inter pictures (inter matrices)
+    "scaling_random": dict(workload="single512", scaling_list=2, qp=27, base_seed=81),
+    "scaling_default444": dict(workload="rext4096", grid_cols=1, grid_rows=1, out_width=512, out_height=512,
+                               scaling_list=1, base_seed=83),
+    "scaling_inter": dict(workload="single512", tile_width=256, tile_height=128, grid_cols=1, grid_rows=3,
+                          num_outputs=3, out_width=256, out_height=128, inter=1, weighted=1, scaling_list=2,
+                          base_seed=85),
+    # extended precision at 12 bits 4:4:4 with transform skip up to 8x8 and long level tails
+    "ep12": dict(workload="rext4096", grid_cols=1, grid_rows=1, out_width=512, out_height=512, bit_depth=12,
+                 extended_precision=1, p_level_geom=0.02, log2_max_ts=3, qp=10, base_seed=87),
+    # per-CU QP deltas, PPS/slice chroma offsets and slice deblocking offsets
+    "qp_delta": dict(workload="single512", cu_qp_delta=1, diff_cu_qp_delta_depth=2, qp_delta_max=8,
+                     pps_cb_qp_offset=4, pps_cr_qp_offset=-5, slice_cb_qp_offset=-2, slice_cr_qp_offset=3,
+                     beta_offset_div2=3, tc_offset_div2=-2, p_sig=0.05, qp=32, base_seed=89),
+    "qp_delta10": dict(workload="single512", bit_depth=10, out_kind=N.OUT_RGB16, cu_qp_delta=1,
+                       diff_cu_qp_delta_depth=1, qp_delta_max=12, beta_offset_div2=-4, tc_offset_div2=5,
+                       base_seed=91),
+    # slices and tiles: gated deblocking and SAO across their boundaries, per-slice parameters
+    "slices_tiles": dict(workload="single512", num_slices=5, tile_cols=3, tile_rows=2, random_slice_params=1,
+                         lf_across_tiles=0, cu_qp_delta=1, p_sig=0.05, qp=35, base_seed=93),
+    "slices_tiles_across": dict(workload="single512", num_slices=3, tile_cols=2, tile_rows=3,
+                                random_slice_params=1, lf_across_tiles=1, p_sao_merge=0.3, base_seed=95),
+    "slices_no_across": dict(workload="rext4096", grid_cols=1, grid_rows=1, out_width=512, out_height=512,
+                             num_slices=4, lf_across_slices=0, tile_cols=2, base_seed=97),
+    # colour conversion: BT.601 full range, BT.2020, GBR and YCgCo 4:4:4, 4:0:0 to LUMA8 / LUMA16
+    "bt601_full": dict(workload="single512", matrix=5, full_range=1, base_seed=99),
+    "bt601_6_limited": dict(workload="single512", matrix=6, base_seed=101),
+    "bt2020_10": dict(workload="single512", matrix=9, bit_depth=10, out_kind=N.OUT_RGB16, base_seed=103),
+    "gbr444_full": dict(workload="rext4096", grid_cols=1, grid_rows=1, out_width=512, out_height=512, matrix=0,
+                        full_range=1, base_seed=105),
+    "gbr444_limited_rgb8": dict(workload="rext4096", grid_cols=1, grid_rows=1, out_width=512, out_height=512,
+                                matrix=0, out_kind=N.OUT_RGB8_FROM16, base_seed=107),
+    "ycgco444": dict(workload="rext4096", grid_cols=1, grid_rows=1, out_width=512, out_height=512, matrix=8,
+                     bit_depth=8, out_kind=N.OUT_RGB8, base_seed=109),
+    "mono8": dict(workload="single512", chroma_idc=0, out_kind=N.OUT_LUMA8, base_seed=111),
+    "mono12": dict(workload="single512", chroma_idc=0, bit_depth=12, out_kind=N.OUT_LUMA16, base_seed=113),
+    # conformance window crop of grid tiles, and oriented outputs
+    "crop_grid": dict(workload="grid4032", grid_cols=3, grid_rows=2, tile_width=256, tile_height=128,
+                      out_width=3 * 232, out_height=2 * 112, conf_win=(8, 16, 4, 12), base_seed=115),
+    "orient_rot90": dict(workload="grid4032", grid_cols=2, grid_rows=2, tile_width=128, tile_height=64,
+                         out_width=240, out_height=120, orientation=6, base_seed=117),
+    "orient_rot270_mono": dict(workload="grid4032", grid_cols=2, grid_rows=1, tile_width=128, tile_height=64,
+                               out_width=250, out_height=64, orientation=8, chroma_idc=0, out_kind=N.OUT_LUMA8,
+                               base_seed=119),
+    "orient_flips16": dict(workload="rext4096", grid_cols=2, grid_rows=1, out_width=1000, out_height=500,
+                           orientation=3, conf_win=(0, 8, 0, 12), base_seed=121),
 }
+
+# every other orientation on one small grid (RGB only, against the oracle)
+ORIENTATIONS = [2, 4, 5, 7]
 
 
 def _first_diff(a, b):
@@ -242,3 +292,48 @@ def test_full_size_workloads(gpu, workload):
     O.decode_batch(rs.batch, want, threads=8)
     for g, w in zip(got, want):
         assert np.array_equal(g, w), f"{int(np.count_nonzero(g != w))} samples differ"
+
+
+@pytest.mark.parametrize("orientation", ORIENTATIONS)
+def test_orientations(gpu, orientation):
+    rs = H.RecordSet.generate("grid4032", grid_cols=2, grid_rows=2, tile_width=128, tile_height=64,
+                              out_width=230, out_height=100, orientation=orientation, base_seed=123)
+    got = gpu.decode(rs)
+    want = rs.output_buffers()
+    O.decode_batch(rs.batch, want, threads=4)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w), f"orientation {orientation}: {int(np.count_nonzero(g != w))} samples differ"
+
+
+@pytest.mark.parametrize("bd", [8, 10, 12])
+@pytest.mark.parametrize("ep", [False, True])
+@pytest.mark.parametrize("stress", [False, True])
+def test_dequant_stress_levels(gpu, bd, ep, stress):
+    """The reference's SAFE/STRESS level tables (src/transform.rs:1374-1382)
+    through K1 at every position of 4..32 TBs, with and without extended
+    precision, transform skip on the 4x4 TBs in the second half."""
+    for ts in (False, True):
+        rs = M.stress_case(bd=bd, ep=ep, stress=stress, ts=ts, seed=bd)
+        print(f"stress bd {bd} ep {ep} stress {stress} ts {ts}", flush=True)
+        with gpu.job(rs) as job:
+            job.run(N.MODE_BATCHED, N.STAGE_RECON)
+            got = job.download_planes(0)
+        want = O.picture_planes(rs.batch, 0, N.STAGE_RECON)
+        for c, (g, w) in enumerate(zip(got, want)):
+            if w is not None:
+                assert np.array_equal(g, w), f"plane {c}: {int(np.count_nonzero(g != w))} samples differ"
+
+
+def test_mixed_intra_and_inter_waves_stop_at_recon(gpu):
+    """An all-intra picture and an inter sequence's I picture share wave 0;
+    at STAGE_RECON the unreferenced intra picture returns reconstructed (not
+    deblocked) samples while the referenced I picture completes SAO."""
+    a = H.RecordSet.generate("single512", tile_width=256, tile_height=128, out_width=256, out_height=128,
+                             base_seed=125)
+    b = H.RecordSet.generate(**dict(SMALL["inter420_wp"], grid_rows=3, num_outputs=3, base_seed=127))
+    rs = H.RecordSet.concat([a, b])
+    with gpu.job(rs) as job:
+        job.run(N.MODE_BATCHED, N.STAGE_RECON)
+        _check_planes(job, rs, N.STAGE_RECON)
+        job.run(N.MODE_GRAPH, N.STAGE_DEBLOCK)
+        _check_planes(job, rs, N.STAGE_DEBLOCK)

@@ -43,7 +43,7 @@ def test_struct_layouts_match_c():
     assert C.sizeof(N.Batch) == 144
     assert C.sizeof(N.PU) == 20
     assert C.sizeof(N.Ref) == 20
-    assert C.sizeof(N.GenConfig) == 128
+    assert C.sizeof(N.GenConfig) == 168
 
 
 def test_generator_is_deterministic_and_seeded():
@@ -278,3 +278,68 @@ def test_lossless_cus_are_prediction_plus_levels():
             got = rec[comp][t["y"]:t["y"] + n, t["x"]:t["x"] + n]
             assert np.array_equal(got, np.clip(want, 0, 255)), (seed, comp)
             assert np.array_equal(rec[comp], out[comp])
+
+
+# --------------------------------------------------------------------------------------
+# generator options that reach the remaining record paths
+# --------------------------------------------------------------------------------------
+def test_cu_qp_delta_varies_qp_and_chroma_offsets_reach_the_tbs():
+    rs = H.RecordSet.generate("single512", cu_qp_delta=1, diff_cu_qp_delta_depth=2, qp_delta_max=6,
+                              pps_cb_qp_offset=3, slice_cb_qp_offset=-2, pps_cr_qp_offset=-4, base_seed=71)
+    qp = ((rs.cells() >> 8) & 0xFF).astype(np.int8)
+    assert len(np.unique(qp)) > 8 and qp.min() >= 0 and qp.max() <= 51
+    t = rs.tbs()
+    # a coded luma TB's Qp' equals the QpY of its CU's cells (8-bit: no QpBdOffset)
+    luma = t[(t["comp"] == 0) & ((t["flags"] & N.TB_CBF) != 0)]
+    gw = 512 // 4
+    assert np.array_equal(luma["qp"], qp[(luma["y"].astype(int) // 4) * gw + luma["x"].astype(int) // 4])
+    P = rs.picture(0)
+    assert P.pps_cb_qp_offset == 3 and P.pps_cr_qp_offset == -4
+
+
+def test_slices_and_tiles_in_the_cells():
+    rs = H.RecordSet.generate("single512", num_slices=3, tile_cols=2, tile_rows=2, random_slice_params=1,
+                              lf_across_tiles=0, base_seed=73)
+    P = rs.picture(0)
+    c = rs.cells().reshape(128, 128)
+    tile = (c >> 24) & 0xFF
+    sl = (c >> 16) & 0xFF
+    assert P.num_slices == 3 and P.flags & N.PF_TILES and not P.flags & N.PF_LF_ACROSS_TILES
+    # tiles: 2 x 2 uniform over the 8 x 8 CTBs (CTB columns 0-3 / 4-7)
+    assert np.array_equal(np.unique(tile), [0, 1, 2, 3])
+    assert (tile[:, :64] % 2 == 0).all() and (tile[:, 64:] % 2 == 1).all()
+    assert (tile[:64] < 2).all() and (tile[64:] >= 2).all()
+    # slices: runs of the tile scan, so slice 0 lies inside tile 0
+    assert set(np.unique(sl)) == {0, 1, 2}
+    assert (sl[tile == 0] <= 1).all() and (sl[tile == 3] == 2).all()
+    assert N.lib().hpvcd_validate_batch(C.byref(rs.batch)) == 0
+    # TBs stay in CTB raster order
+    t = rs.tbs()
+    luma = t[t["comp"] == 0]
+    ctb = (luma["y"].astype(int) >> 6) * 8 + (luma["x"].astype(int) >> 6)
+    assert (np.diff(ctb) >= 0).all()
+
+
+def test_scaling_lists_in_the_records():
+    for kind in (1, 2):
+        rs = H.RecordSet.generate("single512", scaling_list=kind, base_seed=75)
+        assert rs.batch.num_scaling == 1 and rs.picture(0).scaling_index == 0
+        sc = np.ctypeslib.as_array(rs.batch.scaling, shape=(N.SCALING_SET_BYTES,)).reshape(4, 6, -1)
+        assert sc[0, :, 65].all() == (kind == 1)            # default 4x4 lists are flat
+        assert not sc[1:, :, 65].any()
+        assert np.array_equal(sc[3, 1], sc[2, 1]) and np.array_equal(sc[3, 5], sc[2, 5])
+        if kind == 1:
+            import json
+            T = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ref_tables.json")))
+            assert list(sc[1, 0, :64]) == T["scaling_default_intra_8x8"]
+            assert list(sc[2, 4, :64]) == T["scaling_default_inter_8x8"]
+
+
+def test_conformance_window_places_grid_tiles():
+    rs = H.RecordSet.generate("grid4032", grid_cols=3, grid_rows=2, tile_width=256, tile_height=128,
+                              out_width=3 * 232, out_height=2 * 112, conf_win=(8, 16, 4, 12), base_seed=77)
+    for i in range(6):
+        P = rs.picture(i)
+        assert (P.crop_left, P.crop_top, P.win_w, P.win_h) == (8, 4, 232, 112)
+        assert (P.dst_x, P.dst_y) == ((i % 3) * 232, (i // 3) * 112)
+    assert N.lib().hpvcd_validate_batch(C.byref(rs.batch)) == 0

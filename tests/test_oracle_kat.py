@@ -550,3 +550,30 @@ def test_rgb8_from_16bit_narrowing():
     b = _rgb_batch(1, 1, 3, 10, 0, True, N.OUT_RGB8_FROM16)
     rgb = O.picture_rgb(b, 0, np.array([[1023]]), np.array([[512]]), np.array([[4]]))
     assert rgb.reshape(-1).tolist() == [1, 255, 128]
+
+
+# --------------------------------------------------------------------------------------
+# orientation (src/lib.rs:2017 apply_orientation, :2065 rotate_luma, :2116 rotate_buf)
+# --------------------------------------------------------------------------------------
+@pytest.mark.parametrize("mono", [False, True])
+def test_orientation_matches_numpy_transforms(mono):
+    import hpvcd_amd as H
+    from hpvcd_amd import _native as N
+
+    kw = dict(grid_cols=2, grid_rows=1, tile_width=64, tile_height=32, out_width=120, out_height=32,
+              base_seed=79)
+    if mono:
+        kw.update(chroma_idc=0, out_kind=N.OUT_LUMA8)
+    ref = H.RecordSet.generate("grid4032", **kw)
+    base = ref.output_buffers()
+    O.decode_batch(ref.batch, base, threads=2)
+    img = base[0]
+    want = {1: img, 2: img[:, ::-1], 3: img[::-1, ::-1], 4: img[::-1], 5: img, 6: np.rot90(img, -1),
+            7: img, 8: np.rot90(img, 1)}
+    for o in range(1, 9):
+        rs = H.RecordSet.generate("grid4032", orientation=o, **kw)
+        out = rs.output_buffers()
+        O.decode_batch(rs.batch, out, threads=2)
+        w = np.ascontiguousarray(want[o])
+        # the buffer holds the oriented image; 5 and 7 keep the decoded buffer
+        assert np.array_equal(out[0].reshape(-1), w.reshape(-1)), o
