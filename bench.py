@@ -26,27 +26,42 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
 def kernel_alg_bytes(stats: dict, sample_b: int, res_b: int, out_b: int) -> dict:
-    """Algorithmic HBM bytes per launch of each kernel kind (one step).
+    """Algorithmic HBM bytes per launch of each kernel kind (one step), from
+    the design-independent pipeline figure
 
-    S = coded samples, NNZ = nonzero levels, P = output pixels.
-      residual: 4*NNZ (coefficient words) + 24*TBs (records) + S*res_b (residual write)
-      intra:    S*res_b (residual read) + S*sample_b (sample write) + 24*TBs + 4*S/16 (order map)
-      deblock:  2*S*sample_b (read + write of the planes)
-      sao:      2*S*sample_b (read deblocked, write filtered)
-      rgb:      S*sample_b (read) + 3*P*out_b (write)
-      mc:       reference samples read per list + predicted samples written, summed over PUs
+        4*NNZ + 6*S*b + 3*P*b_out [+ MC reference reads]
+
+    split over the stages that move them (S = coded samples, NNZ = nonzero
+    levels, P = output pixels, b = sample bytes, b_out = output sample bytes):
+      residual: 4*NNZ (coefficient words in)
+      intra:    S*b (reconstruction write)
+      deblock:  2*S*b (read + write)
+      sao:      2*S*b (read the deblocked planes, write the filtered ones)
+      rgb:      S*b + 3*P*b_out (read the planes, write the image)
+      mc:       reference samples read per list, summed over PUs
     """
-    S, nnz, tbs, P = stats["samples"], stats["nnz"], stats["tbs"], stats["out_pixels"]
+    S, nnz, P = stats["samples"], stats["nnz"], stats["out_pixels"]
     alg = {
-        "residual": 4 * nnz + 24 * tbs + S * res_b,
-        "intra": S * res_b + S * sample_b + 24 * tbs + 4 * S // 16,
+        "residual": 4 * nnz,
+        "intra": S * sample_b,
         "deblock": 2 * S * sample_b,
         "sao": 2 * S * sample_b,
         "rgb": S * sample_b + 3 * P * out_b,
     }
     if stats.get("pus"):
-        alg["mc"] = stats["mc_read_bytes"] + stats["mc_write_bytes"]
+        alg["mc"] = stats["mc_read_bytes"]
     return alg
+
+
+def kernel_design_bytes(stats: dict, sample_b: int, res_b: int, out_b: int) -> dict:
+    """Bytes each kernel must move in this design, including its own
+    intermediates (the residual pool hand-off, TB records, order maps):
+      residual: 4*NNZ + 24*TBs + S*res_b;  intra: S*res_b + S*b + 24*TBs + S/4."""
+    S, nnz, tbs = stats["samples"], stats["nnz"], stats["tbs"]
+    d = kernel_alg_bytes(stats, sample_b, res_b, out_b)
+    d["residual"] = 4 * nnz + 24 * tbs + S * res_b
+    d["intra"] = S * res_b + S * sample_b + 24 * tbs + 4 * S // 16
+    return d
 
 
 def measured_traffic(workload: str, kernel: str):
@@ -81,6 +96,91 @@ def copy_calibration(nbytes: int) -> None:
     hip.hipFree(dst)
 
 
+def _formats(rs, N):
+    pic0 = rs.picture(0)
+    sample_b = 1 if pic0.bit_depth_luma == 8 else 2
+    res_b = 2 if (pic0.bit_depth_luma == 8 and pic0.chroma_idc != 3) else 4
+    out_kind = rs.outputs()[0].kind
+    out_b = 2 if out_kind in (N.OUT_RGB16, N.OUT_LUMA16) else 1
+    return sample_b, res_b, out_b
+
+
+def _pipeline_bytes(st, sample_b, out_b):
+    return 4 * st["nnz"] + 6 * st["samples"] * sample_b + 3 * st["out_pixels"] * out_b + st["mc_read_bytes"]
+
+
+def _pinned_outputs(H, N, rs):
+    """Page-locked output buffers of the record set's images."""
+    bufs = []
+    for o in rs.outputs():
+        ch = 1 if o.kind in (N.OUT_LUMA8, N.OUT_LUMA16) else 3
+        dt = "u2" if o.kind in (N.OUT_RGB16, N.OUT_LUMA16) else "u1"
+        bufs.append(H.PinnedBuffer((o.height, o.width, ch), dt))
+    return bufs
+
+
+def end_to_end(H, N, g, rs, iters):
+    """Records on the host to pixels on the host, per step: job creation
+    (record preparation + one pinned host-to-device copy + pool clears),
+    the batched launch chain, and the download into page-locked buffers."""
+    import numpy as np
+
+    pins = _pinned_outputs(H, N, rs)
+    tot, prep, up, run, down = [], [], [], [], []
+    for _ in range(iters + 1):
+        t0 = time.perf_counter()
+        job = g.job(rs)
+        t1 = time.perf_counter()
+        job.run(N.MODE_BATCHED)
+        g.sync()
+        t2 = time.perf_counter()
+        job.download_into([b.array for b in pins])
+        t3 = time.perf_counter()
+        pm, um = job.host_ms()
+        job.close()
+        tot.append((t3 - t0) * 1e3)
+        prep.append(pm)
+        up.append(um)
+        run.append((t2 - t1) * 1e3)
+        down.append((t3 - t2) * 1e3)
+    # the first iteration grows the staging buffer and pages in the pools
+    sl = slice(1, None)
+    med = lambda v: float(np.median(v[sl]))
+    out = {"ms_per_step": round(med(tot), 4), "host_prep_ms": round(med(prep), 4),
+           "upload_ms": round(med(up), 4), "device_ms": round(med(run), 4), "download_ms": round(med(down), 4),
+           "p50_ms": round(float(np.percentile(tot[sl], 50)), 4), "p99_ms": round(float(np.percentile(tot[sl], 99)), 4)}
+    for b in pins:
+        b.close()
+    return out
+
+
+def config_line(H, N, g, name, steps):
+    """Device throughput of another workload (graph replay of a resident job)."""
+    rs = H.RecordSet.generate(name)
+    st = rs.stats()
+    sample_b, res_b, out_b = _formats(rs, N)
+    with g.job(rs) as job:
+        for _ in range(2):
+            job.run(N.MODE_GRAPH)
+        g.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            job.run(N.MODE_GRAPH)
+        g.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        ev_ms, kms = job.time(N.MODE_GRAPH, warmup=1, iters=3)
+        plain_ms, _ = job.time(N.MODE_PLAIN, warmup=1, iters=2, breakdown=False)
+    e2e = end_to_end(H, N, g, rs, 3)
+    alg = _pipeline_bytes(st, sample_b, out_b)
+    mpix = st["out_pixels"] / 1e6
+    return {"ms_per_step": round(ms, 4), "value": round(mpix / (ms / 1e3), 2), "unit": "Mpixel/s",
+            "pictures": st["pictures"], "out_pixels": st["out_pixels"],
+            "plain_ms_per_step": round(plain_ms, 4), "batched_speedup": round(plain_ms / ms, 2),
+            "pipeline_hbm_frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
+            "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
+            "e2e_ms_per_step": e2e["ms_per_step"], "e2e_value": round(mpix / (e2e["ms_per_step"] / 1e3), 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +190,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other workloads' lines")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (read + write) three times, for calibrating "
                          "the FETCH_SIZE/WRITE_SIZE counters of a profiling run")
@@ -107,12 +208,9 @@ def main() -> int:
 
     rs = H.RecordSet.generate(args.workload)
     st = rs.stats()
-    pic0 = rs.picture(0)
-    sample_b = 1 if pic0.bit_depth_luma == 8 else 2
-    res_b = 2 if (pic0.bit_depth_luma == 8 and pic0.chroma_idc != 3) else 4
-    out_kind = rs.outputs()[0].kind
-    out_b = 2 if out_kind in (N.OUT_RGB16, N.OUT_LUMA16) else 1
+    sample_b, res_b, out_b = _formats(rs, N)
     mpix = st["out_pixels"] / 1e6
+    others = {}
 
     with H.Gpu(0) as g:
         if args.copy_calibration > 0:
@@ -131,7 +229,7 @@ def main() -> int:
         print(f"[bench] timed {args.steps} steps: {wall_ms:.3f} ms/step", file=sys.stderr, flush=True)
         # device-side timing and per-kernel breakdown (HIP events on the job's stream)
         ev_ms, kms = job.time(N.MODE_GRAPH, warmup=1, iters=max(3, min(args.steps, 20)))
-        # per-request latency (each step synchronised)
+        # replay latency of the resident job (each step synchronised)
         lat = []
         for _ in range(max(10, min(args.steps, 50))):
             t = time.perf_counter()
@@ -145,13 +243,21 @@ def main() -> int:
             plain_ms, _ = job.time(N.MODE_PLAIN, warmup=1, iters=3, breakdown=False)
         job.close()
         print(f"[bench] events {ev_ms:.3f} ms/step, kernels {kms}, plain {plain_ms}", file=sys.stderr, flush=True)
+        e2e = end_to_end(H, N, g, rs, 5)
+        print(f"[bench] end to end {e2e}", file=sys.stderr, flush=True)
+        if not args.no_configs:
+            for name in ("single512", "rext4096", "video1080"):
+                if name != args.workload:
+                    others[name] = config_line(H, N, g, name, 10)
+                    print(f"[bench] {name}: {others[name]}", file=sys.stderr, flush=True)
 
     value = mpix / (wall_ms / 1e3)
     alg = kernel_alg_bytes(st, sample_b, res_b, out_b)
+    design = kernel_design_bytes(st, sample_b, res_b, out_b)
     dominant = max((k for k in kms if k in alg), key=lambda k: kms[k])
     dom_ms = kms[dominant]
     achieved = alg[dominant] / (dom_ms / 1e3) / 1e9
-    total_alg = 4 * st["nnz"] + 6 * st["samples"] * sample_b + 3 * st["out_pixels"] * out_b + st["mc_read_bytes"]
+    total_alg = _pipeline_bytes(st, sample_b, out_b)
 
     cpu = None
     bit_exact = None
@@ -171,9 +277,19 @@ def main() -> int:
                 O.decode_batch(rs.batch, bufs, threads=threads, rows_parallel=rows)
                 n += 1
                 el = time.perf_counter() - t0
-                if el > args.cpu_seconds / 2 or n >= 50:
+                if el > args.cpu_seconds / 3 or n >= 50:
                     break
             best[label] = (n, el / n)
+        # one thread: the whole batch, once or more within the same bound
+        n1 = 0
+        t0 = time.perf_counter()
+        while True:
+            bufs = rs.output_buffers()
+            O.decode_batch(rs.batch, bufs, threads=1, rows_parallel=False)
+            n1 += 1
+            el1 = time.perf_counter() - t0
+            if el1 > args.cpu_seconds / 3 or n1 >= 20:
+                break
         label = min(best, key=lambda k: best[k][1])
         n, sec = best[label]
         cpu = {
@@ -181,11 +297,12 @@ def main() -> int:
             "unit": "Mpixel/s",
             "cores": threads,
             "kind": "port",
-            "sample": (f"C restatement of the reference recon path (oracle/), full {args.workload} batch "
-                       f"x{n} runs, {label} over {threads} threads "
+            "single_thread": round(mpix / (el1 / n1), 2),
+            "sample": (f"C restatement of the reference recon path (oracle/, gcc -O3 -march=x86-64-v3), full "
+                       f"{args.workload} batch x{n} runs, {label} over {threads} threads "
                        f"(grid-row jobs: {mpix / best['grid-row jobs'][1]:.1f} Mpixel/s, "
-                       f"tile-parallel: {mpix / best['tile-parallel'][1]:.1f} Mpixel/s); records from the "
-                       f"generator, parse not timed"),
+                       f"tile-parallel: {mpix / best['tile-parallel'][1]:.1f} Mpixel/s, one thread x{n1}: "
+                       f"{mpix / (el1 / n1):.1f} Mpixel/s); records from the generator, parse not timed"),
         }
 
     line = {
@@ -209,7 +326,10 @@ def main() -> int:
             "mode": "batched, one launch per stage, hipGraph replay",
         },
         "bit_exact": bit_exact,
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
+                       "kind": "replay of the resident job"},
+        "e2e": {**e2e, "value": round(mpix / (e2e["ms_per_step"] / 1e3), 2), "unit": "Mpixel/s",
+                "kind": "host records -> job creation (prep + pinned H2D) -> batched launch -> D2H to pinned"},
         "plain_per_tile": None if plain_ms is None else {
             "ms_per_step": round(plain_ms, 4), "value": round(mpix / (plain_ms / 1e3), 2),
             "batched_speedup": round(plain_ms / wall_ms, 2)},
@@ -226,8 +346,11 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": measured_traffic(args.workload, dominant),
             "alg_bytes_per_launch": alg[dominant],
+            "design_bytes_per_launch": design[dominant],
+            "design_frac": round(design[dominant] / (dom_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
         },
         "cpu_baseline": cpu,
+        "configs": others,
     }
     if cpu:
         line["vs_cpu"] = round(value / cpu["value"], 1)

@@ -22,7 +22,7 @@ import numpy as np
 from . import _native as N
 from ._native import HpvcdError, check
 
-__all__ = ["RecordSet", "Gpu", "Job", "HpvcdError", "WORKLOADS", "gen_config", "N"]
+__all__ = ["RecordSet", "Gpu", "Job", "PinnedBuffer", "HpvcdError", "WORKLOADS", "gen_config", "N"]
 
 
 # --------------------------------------------------------------------------------------
@@ -209,6 +209,30 @@ _TB_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2n", "u1"), ("comp", "u1"
 assert _TB_DTYPE.itemsize == C.sizeof(N.TB)
 
 
+class PinnedBuffer:
+    """Page-locked host memory (hpvcd_host_alloc) viewed as a numpy array."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * dtype.itemsize
+        p = C.c_void_p()
+        check(N.lib().hpvcd_host_alloc(n, C.byref(p)))
+        self._p = p
+        self.array = np.ctypeslib.as_array((C.c_uint8 * max(n, 1)).from_address(p.value))[:n].view(dtype).reshape(shape)
+
+    def close(self) -> None:
+        if self._p:
+            self.array = None
+            N.lib().hpvcd_host_free(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Gpu:
     """One GPU and its stream (hpvcd_gpu_open)."""
 
@@ -270,6 +294,17 @@ class Job:
     @property
     def device_bytes(self) -> int:
         return int(N.lib().hpvcd_job_device_bytes(self._h))
+
+    def host_ms(self) -> Tuple[float, float]:
+        """(record preparation, upload) milliseconds spent in job creation."""
+        a, b = C.c_double(), C.c_double()
+        check(N.lib().hpvcd_job_host_ms(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def download_into(self, buffers: Sequence[np.ndarray]) -> None:
+        """Copy the output images into caller buffers (e.g. PinnedBuffer arrays)."""
+        ptrs = (C.c_void_p * len(buffers))(*[b.ctypes.data for b in buffers])
+        check(N.lib().hpvcd_job_download(self._h, ptrs))
 
     def run(self, mode: int = N.MODE_BATCHED, stop: int = N.STAGE_RGB) -> None:
         check(N.lib().hpvcd_job_run(self._h, mode, stop))

@@ -204,6 +204,9 @@ SIGNATURES = {
     "hpvcd_job_create": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(_P)]),
     "hpvcd_job_free": (None, [_P]),
     "hpvcd_job_device_bytes": (C.c_size_t, [_P]),
+    "hpvcd_job_host_ms": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "hpvcd_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(_P)]),
+    "hpvcd_host_free": (None, [_P]),
     "hpvcd_job_run": (C.c_int, [_P, C.c_int, C.c_int]),
     "hpvcd_gpu_sync": (C.c_int, [_P]),
     "hpvcd_job_download": (C.c_int, [_P, C.POINTER(_P)]),

@@ -6,7 +6,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
+#include <type_traits>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -230,6 +232,11 @@ struct hpvcd_gpu {
     hipEvent_t fork = nullptr, join = nullptr;
     int num_cus = 0;
     int occ_intra[4] = {0, 0, 0, 0};   // blocks per CU for k_intra<T,R> (T16*2 + R32)
+    // pinned staging of record uploads, reused by every job of this GPU
+    // (grown on demand; job creation holds `mu` while it stages and copies)
+    std::mutex mu;
+    uint8_t* pin = nullptr;
+    size_t pin_size = 0;
 };
 
 namespace {
@@ -237,16 +244,16 @@ namespace {
 struct DevBuf {
     char* base = nullptr;
     size_t size = 0, used = 0;
+    // next 256-byte aligned block of count elements (null while sizing, when
+    // base is not allocated yet)
     template <typename X>
     X* take(size_t count) {
         size_t off = (used + 255) & ~(size_t)255;
         used = off + count * sizeof(X);
-        return reinterpret_cast<X*>(base + off);
+        return base ? reinterpret_cast<X*>(base + off) : nullptr;
     }
 };
 
-template <typename X>
-size_t bytes_of(size_t n) { return ((n * sizeof(X)) + 255) & ~(size_t)255; }
 
 struct ClassJob {
     bool samp16 = false, res32 = false;
@@ -260,6 +267,7 @@ struct ClassJob {
     std::vector<uint32_t> tickets, plain_tickets, unit_tbs;
     std::vector<PrepTb> unit_recs;                 // K2a's fields of each unit_tbs entry
     std::vector<uint32_t> prep_list;               // units K2a prepares (Cr units sharing Cb's are not)
+    std::vector<uint32_t> plain_prep, pic_prep_begin; // the same units per picture (plain mode) and segments
     std::vector<uint16_t> level_off;
     std::vector<uint32_t> unit_refoff;             // per unit TB entry: map offset | sample offset / 16 << 13 | level << 21
     size_t smap_words = 0, rmap_words = 0;
@@ -288,6 +296,7 @@ struct ClassJob {
     uint32_t *d_tickets = nullptr, *d_plain_tickets = nullptr;
     PrepTb* d_unit_recs = nullptr;
     uint32_t* d_prep_list = nullptr;
+    uint32_t* d_plain_prep = nullptr;
     uint16_t* d_level_off = nullptr;
     uint32_t* d_unit_refoff = nullptr;
     uint32_t* d_smap = nullptr;   // per-sample reference words
@@ -318,6 +327,7 @@ struct hpvcd_job {
     size_t out_bytes = 0;
     int last_stop = HPVCD_STAGE_RGB;
     std::vector<uint32_t> pic_class, pic_local;   // batch picture -> (class, local index)
+    double prep_ms = 0.0, upload_ms = 0.0;        // host preparation; staging + H2D + pool clears
 };
 
 namespace {
@@ -656,9 +666,22 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
     }
     C.pic_units_begin[np] = unit_base;
     // K2a takes the units with the most predicted samples first, so that the
-    // launch's tail holds the small ones
-    std::stable_sort(C.prep_list.begin(), C.prep_list.end(),
-                     [&](uint32_t a, uint32_t b) { return C.units[a].nsamp > C.units[b].nsamp; });
+    // launch's tail holds the small ones; plain mode prepares each picture's
+    // units in its own launch
+    auto larger = [&](uint32_t a, uint32_t b) { return C.units[a].nsamp > C.units[b].nsamp; };
+    C.plain_prep = C.prep_list;  // in picture order
+    C.pic_prep_begin.assign(np + 1, 0);
+    {
+        size_t k = 0;
+        for (uint32_t lp = 0; lp < np; lp++) {
+            C.pic_prep_begin[lp] = (uint32_t)k;
+            const size_t k0 = k;
+            while (k < C.plain_prep.size() && C.units[C.plain_prep[k]].pic == lp) k++;
+            std::stable_sort(C.plain_prep.begin() + k0, C.plain_prep.begin() + k, larger);
+        }
+        C.pic_prep_begin[np] = (uint32_t)k;
+    }
+    std::stable_sort(C.prep_list.begin(), C.prep_list.end(), larger);
     // batched tickets: topological (t = cx + 2 cy) across all pictures
     std::vector<UnitKey> sorted = keys;
     std::stable_sort(sorted.begin(), sorted.end(), [](const UnitKey& a, const UnitKey& b) {
@@ -823,16 +846,20 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
         // samples; the others stop at the stop stage
         uint32_t pref = p0;
         while (pref < p1 && C.pic_is_ref[pref]) pref++;
-        const bool prep = step == 0 && !C.prep_list.empty();
+        // reference availability, reference maps, per-sample reference
+        // words and descriptors of the TBs (records only) on the side stream,
+        // concurrently with the residual stage: every picture at once in the
+        // first step (batched), the step's picture (plain)
+        const uint32_t* prep_units = plain ? C.d_plain_prep + C.pic_prep_begin[p0] : C.d_prep_list;
+        const uint32_t nprep = plain ? C.pic_prep_begin[p0 + 1] - C.pic_prep_begin[p0]
+                                     : (step == 0 ? (uint32_t)C.prep_list.size() : 0u);
+        const bool prep = nprep > 0;
         if (prep) {
-            // reference availability, reference maps, per-sample reference
-            // words and descriptors of every TB (records only) on the side
-            // stream, concurrently with the residual stage
             HIP_TRY_V(hipEventRecord(g->fork, s));
             HIP_TRY_V(hipStreamWaitEvent(g->side, g->fork, 0));
-            k_prep<<<(uint32_t)C.prep_list.size(), kPrepThreads, 0, g->side>>>(C.d_pics, C.d_units, C.d_prep_list,
-                                                                             C.d_unit_recs, C.d_unit_refoff,
-                                                         C.d_order, C.d_cells, C.d_desc, C.d_rmap, C.d_smap);
+            k_prep<<<nprep, kPrepThreads, 0, g->side>>>(C.d_pics, C.d_units, prep_units, C.d_unit_recs,
+                                                        C.d_unit_refoff, C.d_order, C.d_cells, C.d_desc, C.d_rmap,
+                                                        C.d_smap);
             HIP_TRY_V(hipEventRecord(g->join, g->side));
         }
         if (plain || step == 0) {
@@ -937,12 +964,6 @@ hpvcd_status check_err(hpvcd_job* j) {
     return HPVCD_OK;
 }
 
-template <typename X>
-hpvcd_status upload(X*& dst, DevBuf& mem, const std::vector<X>& v) {
-    dst = mem.take<X>(std::max<size_t>(v.size(), 1));
-    if (!v.empty()) HIP_TRY(hipMemcpy(dst, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
-    return HPVCD_OK;
-}
 
 }  // namespace
 
@@ -1050,6 +1071,7 @@ void hpvcd_gpu_close(hpvcd_gpu* g) {
     if (g->side) (void)hipStreamDestroy(g->side);
     if (g->fork) (void)hipEventDestroy(g->fork);
     if (g->join) (void)hipEventDestroy(g->join);
+    if (g->pin) (void)hipHostFree(g->pin);
     delete g;
 }
 
@@ -1078,6 +1100,7 @@ void hpvcd_job_free(hpvcd_job* j) {
 hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out) {
     if (!g || !b || !out) return set_error(HPVCD_E_INVALID, "null argument");
     *out = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
     hpvcd_status st = validate_batch(*b);
     if (st != HPVCD_OK) return st;
     HIP_TRY(hipSetDevice(g->device));
@@ -1135,28 +1158,60 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
         });
         for (uint32_t i = 0; i < C.batch_pics.size(); i++) j->pic_local[C.batch_pics[i]] = i;
     }
-    size_t sc_bytes = (size_t)b->num_scaling * HPVCD_SCALING_SET_BYTES;
-    size_t total = ob + 256 + ((sc_bytes + 256 + 255) & ~(size_t)255);
     for (auto& C : j->classes) {
         st = prepare_class(*b, C, j->out_off, depth, j->pic_local, is_ref);
         if (st != HPVCD_OK) { delete j; return st; }
-        size_t sb = C.samp16 ? 2 : 1, rb = C.res32 ? 4 : 2;
-        total += bytes_of<DevPic>(C.pics.size()) + bytes_of<hpvcd_tb>(C.tbs.size()) + bytes_of<uint32_t>(C.tb_pic.size()) +
-                 bytes_of<uint32_t>(C.coeffs.size() + 1) + bytes_of<uint32_t>(C.cells.size()) * 2 +
-                 bytes_of<hpvcd_sao>(C.sao.size()) + bytes_of<DevUnit>(C.units.size()) +
-                 bytes_of<uint32_t>(C.tickets.size()) * 2 + bytes_of<PrepTb>(C.unit_recs.size() + 1) +
-                 bytes_of<uint32_t>(C.prep_list.size() + 1) +
-                 bytes_of<uint16_t>(C.level_off.size() + 1) + bytes_of<uint32_t>(C.sync_words) +
-                 bytes_of<uint32_t>(C.unit_refoff.size() + 1) +
-                 bytes_of<uint32_t>(C.smap_words + 1) + bytes_of<uint16_t>(C.rmap_words + 8) +
-                 bytes_of<uint4>(C.unit_tbs.size() + 1) +
-                 bytes_of<uint32_t>(C.units.size() * kBorderWords + 1) + bytes_of<uint64_t>(C.tbs.size() + 1) +
-                 2 * ((C.plane_elems * sb + 255) & ~(size_t)255) + ((C.res_elems * rb + 255) & ~(size_t)255) + 8 * 256;
-        for (int pass = 0; pass < 2; pass++)
-            for (int s = 0; s < 4; s++) total += bytes_of<DevResJob>(C.res_jobs[pass][s].size() + 1);
-        total += bytes_of<uint32_t>(C.act_list.size() + 1) + bytes_of<DevPu>(C.pus.size() + 1) +
-                 bytes_of<DevMcTask>(C.mc_tasks.size() + 1);
     }
+    size_t sc_bytes = (size_t)b->num_scaling * HPVCD_SCALING_SET_BYTES;
+    // Device layout: first every array uploaded from the host (one
+    // contiguous region, copied once from pinned staging), then the pools
+    // the kernels write (outputs, sync words, border words, prep output,
+    // planes, residuals).  Two passes of the same take() sequence: the
+    // first sizes the regions, the second assigns pointers.
+    using Upload = std::pair<size_t, std::pair<const void*, size_t>>;
+    auto lay_out = [&](DevBuf& m, bool assign, std::vector<Upload>* ups) {
+        auto up = [&](auto*& dst, const auto& v) {
+            using X = typename std::remove_reference<decltype(v)>::type::value_type;
+            X* d = m.take<X>(std::max<size_t>(v.size(), 1));
+            if (assign) dst = d;
+            if (ups && !v.empty()) ups->push_back({(size_t)((char*)d - m.base), {v.data(), v.size() * sizeof(X)}});
+        };
+        uint8_t* dsc = m.take<uint8_t>(sc_bytes + 1);
+        if (assign) j->d_scaling = dsc;
+        if (ups && sc_bytes) ups->push_back({(size_t)((char*)dsc - m.base), {b->scaling, sc_bytes}});
+        for (auto& C : j->classes) {
+            up(C.d_pics, C.pics); up(C.d_tbs, C.tbs); up(C.d_tb_pic, C.tb_pic); up(C.d_coeffs, C.coeffs);
+            up(C.d_cells, C.cells); up(C.d_order, C.order); up(C.d_sao, C.sao); up(C.d_units, C.units);
+            up(C.d_tickets, C.tickets); up(C.d_plain_tickets, C.plain_tickets); up(C.d_unit_recs, C.unit_recs);
+            up(C.d_level_off, C.level_off); up(C.d_prep_list, C.prep_list); up(C.d_plain_prep, C.plain_prep);
+            up(C.d_unit_refoff, C.unit_refoff);
+            for (int pass = 0; pass < 2; pass++)
+                for (int sz = 0; sz < 4; sz++) up(C.d_res_jobs[pass][sz], C.res_jobs[pass][sz]);
+            up(C.d_act_list, C.act_list); up(C.d_pus, C.pus); up(C.d_mc_tasks, C.mc_tasks);
+        }
+        const size_t upload_end = m.used;
+        uint8_t* dout = m.take<uint8_t>(ob + 1);
+        if (assign) j->d_out = dout;
+        for (auto& C : j->classes) {
+            size_t sb = C.samp16 ? 2 : 1, rb = C.res32 ? 4 : 2;
+            uint32_t* sync = m.take<uint32_t>(C.sync_words);
+            uint32_t* border = m.take<uint32_t>(C.units.size() * kBorderWords + 1);
+            uint32_t* smap = m.take<uint32_t>(C.smap_words + 1);
+            uint16_t* rmap = m.take<uint16_t>(C.rmap_words + 8);
+            uint4* desc = m.take<uint4>(C.unit_tbs.size() + 1);
+            char* pa = m.take<char>(C.plane_elems * sb);
+            char* pb = m.take<char>(C.plane_elems * sb);
+            char* res = m.take<char>(C.res_elems * rb);
+            if (assign) {
+                C.d_sync = sync; C.d_border = border; C.d_smap = smap; C.d_rmap = rmap; C.d_desc = desc;
+                C.d_planes_a = pa; C.d_planes_b = pb; C.d_res = res;
+            }
+        }
+        return upload_end;
+    };
+    DevBuf sizing;
+    const size_t upload_bytes = lay_out(sizing, false, nullptr);
+    const size_t total = sizing.used + 256;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total > free_b) {
         delete j;
@@ -1167,62 +1222,70 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
         return set_error(HPVCD_E_DEVICE, "hipMalloc of the job pool failed");
     }
     j->mem.size = total;
-    j->d_out = j->mem.take<uint8_t>(ob + 1);
-    if (hipMemset(j->d_out, 0, ob + 1) != hipSuccess) { hpvcd_job_free(j); return set_error(HPVCD_E_DEVICE, "hipMemset"); }
-    j->d_scaling = j->mem.take<uint8_t>(sc_bytes + 1);
-    if (sc_bytes && hipMemcpy(j->d_scaling, b->scaling, sc_bytes, hipMemcpyHostToDevice) != hipSuccess) {
-        hpvcd_job_free(j);
-        return set_error(HPVCD_E_DEVICE, "scaling upload");
-    }
-    for (auto& C : j->classes) {
-        size_t sb = C.samp16 ? 2 : 1, rb = C.res32 ? 4 : 2;
-        if ((st = upload(C.d_pics, j->mem, C.pics)) || (st = upload(C.d_tbs, j->mem, C.tbs)) ||
-            (st = upload(C.d_tb_pic, j->mem, C.tb_pic)) || (st = upload(C.d_coeffs, j->mem, C.coeffs)) ||
-            (st = upload(C.d_cells, j->mem, C.cells)) || (st = upload(C.d_order, j->mem, C.order)) ||
-            (st = upload(C.d_sao, j->mem, C.sao)) || (st = upload(C.d_units, j->mem, C.units)) ||
-            (st = upload(C.d_tickets, j->mem, C.tickets)) || (st = upload(C.d_plain_tickets, j->mem, C.plain_tickets)) ||
-            (st = upload(C.d_unit_recs, j->mem, C.unit_recs)) || (st = upload(C.d_level_off, j->mem, C.level_off)) ||
-            (st = upload(C.d_prep_list, j->mem, C.prep_list)) ||
-            (st = upload(C.d_unit_refoff, j->mem, C.unit_refoff))) {
-            hpvcd_job_free(j);
-            return st;
+    std::vector<Upload> ups;
+    lay_out(j->mem, true, &ups);
+    if (j->mem.used > j->mem.size) { hpvcd_job_free(j); return set_error(HPVCD_E_DEVICE, "pool overflow"); }
+    const auto t1 = std::chrono::steady_clock::now();
+    j->prep_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    {
+        // stage the uploaded arrays at their device offsets in the GPU's
+        // pinned buffer, copy them in one transfer, clear the written pools
+        std::lock_guard<std::mutex> lock(g->mu);
+        if (g->pin_size < upload_bytes) {
+            if (g->pin) (void)hipHostFree(g->pin);
+            g->pin = nullptr;
+            g->pin_size = 0;
+            if (hipHostMalloc((void**)&g->pin, upload_bytes, hipHostMallocDefault) != hipSuccess) {
+                g->pin = nullptr;
+                hpvcd_job_free(j);
+                return set_error(HPVCD_E_DEVICE, "hipHostMalloc of the upload staging buffer failed");
+            }
+            g->pin_size = upload_bytes;
         }
-        for (int pass = 0; pass < 2; pass++)
-            for (int s = 0; s < 4; s++)
-                if ((st = upload(C.d_res_jobs[pass][s], j->mem, C.res_jobs[pass][s]))) { hpvcd_job_free(j); return st; }
-        if ((st = upload(C.d_act_list, j->mem, C.act_list)) || (st = upload(C.d_pus, j->mem, C.pus)) ||
-            (st = upload(C.d_mc_tasks, j->mem, C.mc_tasks))) {
-            hpvcd_job_free(j);
-            return st;
+        for (auto& u : ups) std::memcpy(g->pin + u.first, u.second.first, u.second.second);
+        hipError_t e = hipMemcpyAsync(j->mem.base, g->pin, upload_bytes, hipMemcpyHostToDevice, g->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(j->d_out, 0, ob + 1, g->stream);
+        for (auto& C : j->classes) {
+            size_t rb = C.res32 ? 4 : 2;
+            // samples of TBs without a residual are never written: they stay zero
+            if (e == hipSuccess) e = hipMemsetAsync(C.d_sync, 0, C.sync_words * sizeof(uint32_t), g->stream);
+            if (e == hipSuccess)
+                e = hipMemsetAsync(C.d_border, 0, (C.units.size() * kBorderWords + 1) * sizeof(uint32_t), g->stream);
+            if (e == hipSuccess) e = hipMemsetAsync(C.d_res, 0, C.res_elems * rb, g->stream);
         }
-        C.d_sync = j->mem.take<uint32_t>(C.sync_words);
-        C.d_border = j->mem.take<uint32_t>(C.units.size() * kBorderWords + 1);
-        C.d_smap = j->mem.take<uint32_t>(C.smap_words + 1);
-        C.d_rmap = j->mem.take<uint16_t>(C.rmap_words + 8);
-        C.d_desc = j->mem.take<uint4>(C.unit_tbs.size() + 1);
-        if (hipMemset(C.d_sync, 0, C.sync_words * sizeof(uint32_t)) != hipSuccess ||
-            hipMemset(C.d_border, 0, (C.units.size() * kBorderWords + 1) * sizeof(uint32_t)) != hipSuccess) {
+        if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+        if (e != hipSuccess) {
             hpvcd_job_free(j);
-            return set_error(HPVCD_E_DEVICE, "hipMemset of the sync words");
-        }
-        C.d_planes_a = j->mem.take<char>(C.plane_elems * sb);
-        C.d_planes_b = j->mem.take<char>(C.plane_elems * sb);
-        C.d_res = j->mem.take<char>(C.res_elems * rb);
-        if (j->mem.used > j->mem.size) { hpvcd_job_free(j); return set_error(HPVCD_E_DEVICE, "pool overflow"); }
-        // samples of TBs without a residual are never written: they stay zero
-        if (hipMemset(C.d_res, 0, C.res_elems * rb) != hipSuccess) {
-            hpvcd_job_free(j);
-            return set_error(HPVCD_E_DEVICE, "hipMemset of the residual pool");
+            return set_error(HPVCD_E_DEVICE, hipGetErrorString(e));
         }
     }
-    // the uploads and clears above run on the null stream; the job's kernels
-    // run on the device's own non-blocking stream
-    if (hipDeviceSynchronize() != hipSuccess) { hpvcd_job_free(j); return set_error(HPVCD_E_DEVICE, "hipDeviceSynchronize"); }
+    j->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
     *out = j;
     return HPVCD_OK;
 }
 
 size_t hpvcd_job_device_bytes(const hpvcd_job* j) { return j ? j->mem.size : 0; }
+
+hpvcd_status hpvcd_job_host_ms(const hpvcd_job* j, double* prep_ms, double* upload_ms) {
+    if (!j || !prep_ms || !upload_ms) return set_error(HPVCD_E_INVALID, "null argument");
+    *prep_ms = j->prep_ms;
+    *upload_ms = j->upload_ms;
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_host_alloc(size_t bytes, void** out) {
+    if (!out) return set_error(HPVCD_E_INVALID, "null argument");
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return set_error(HPVCD_E_ALLOC, "hipHostMalloc failed");
+    }
+    return HPVCD_OK;
+}
+
+void hpvcd_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
+}
 
 hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop) {
     if (!j) return set_error(HPVCD_E_INVALID, "null argument");

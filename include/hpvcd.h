@@ -142,6 +142,14 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
 void         hpvcd_job_free(hpvcd_job* j);
 /* Device bytes the job holds. */
 size_t       hpvcd_job_device_bytes(const hpvcd_job* j);
+/* Host milliseconds hpvcd_job_create spent preparing the records
+ * (validation, availability stamps, wavefront units, work lists) and
+ * uploading them (pinned staging, one host-to-device copy, pool clears). */
+hpvcd_status hpvcd_job_host_ms(const hpvcd_job* j, double* prep_ms, double* upload_ms);
+/* Page-locked host memory (hipHostMalloc): output buffers that
+ * hpvcd_job_download fills by DMA without a staging copy. */
+hpvcd_status hpvcd_host_alloc(size_t bytes, void** out);
+void         hpvcd_host_free(void* p);
 /* Enqueue the reconstruction on the GPU's stream (asynchronous). */
 hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop_stage);
 /* Wait for the stream. */
