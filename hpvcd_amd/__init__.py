@@ -22,7 +22,7 @@ import numpy as np
 from . import _native as N
 from ._native import HpvcdError, check
 
-__all__ = ["RecordSet", "Gpu", "Job", "PinnedBuffer", "HpvcdError", "WORKLOADS", "gen_config", "N"]
+__all__ = ["RecordSet", "Gpu", "Job", "PinnedBuffer", "Decoder", "HpvcdError", "WORKLOADS", "gen_config", "N"]
 
 
 # --------------------------------------------------------------------------------------
@@ -333,3 +333,51 @@ class Job:
         km = (C.c_double * len(N.KERNEL_NAMES))()
         check(N.lib().hpvcd_job_time(self._h, mode, warmup, iters, C.byref(ms), km if breakdown else None))
         return ms.value, {N.KERNEL_NAMES[i]: km[i] for i in range(len(N.KERNEL_NAMES))} if breakdown else None
+
+
+def image_array(img: N.Image) -> np.ndarray:
+    """A copy of a decoded image's pixels as (height, width, channels)."""
+    dt = np.uint16 if img.sample_bytes == 2 else np.uint8
+    if not img.pixels:
+        raise ValueError("image has no pixels")
+    raw = (C.c_uint8 * img.pixels_len).from_address(img.pixels)
+    return np.frombuffer(raw, dtype=dt).reshape(img.height, img.width, img.channels).copy()
+
+
+class Decoder:
+    """The drop-in decoder (hpvcd_decoder_new / src/decoder.rs Decoder)."""
+
+    def __init__(self, backend: int = N.BACKEND_GPU_BATCHED, device: int = 0, threads: int = 0, **limits):
+        s = N.Settings()
+        N.lib().hpvcd_settings_default(C.byref(s))
+        s.backend, s.device, s.threads = backend, device, threads
+        for k, v in limits.items():
+            if not hasattr(s, k):
+                raise KeyError(f"unknown setting {k!r}")
+            setattr(s, k, v)
+        h = C.c_void_p()
+        check(N.lib().hpvcd_decoder_new(C.byref(s), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().hpvcd_decoder_free(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def decode_records(self, records: RecordSet) -> List[np.ndarray]:
+        """Every output image of a record set (hpvcd_decode_records)."""
+        n = records.batch.num_outputs
+        imgs = (N.Image * max(n, 1))()
+        st = N.lib().hpvcd_decode_records(self._h, C.byref(records.batch), imgs)
+        try:
+            check(st)
+            return [image_array(imgs[i]) for i in range(n)]
+        finally:
+            for i in range(n):
+                N.lib().hpvcd_image_free(C.byref(imgs[i]))

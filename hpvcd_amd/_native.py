@@ -186,6 +186,27 @@ class GenConfig(C.Structure):
     ]
 
 
+class Settings(C.Structure):
+    _fields_ = [
+        ("threads", C.c_int32), ("max_dimension", C.c_uint32),
+        ("max_pixels", C.c_uint64), ("max_box_size", C.c_uint64), ("max_item_size", C.c_uint64),
+        ("max_items", C.c_uint32), ("max_tiles", C.c_uint32),
+        ("decode_alpha", C.c_uint8), ("decode_gain_map", C.c_uint8), ("_pad", C.c_uint8 * 2),
+        ("backend", C.c_int32), ("device", C.c_int32),
+    ]
+
+
+class Image(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32),
+        ("channels", C.c_uint8), ("bit_depth", C.c_uint8), ("sample_bytes", C.c_uint8), ("orientation", C.c_uint8),
+        ("pixels", C.c_void_p), ("pixels_len", C.c_size_t),
+        ("matrix", C.c_uint8), ("full_range", C.c_uint8), ("_pad", C.c_uint8 * 6),
+    ]
+
+
+BACKEND_CPU_REF, BACKEND_GPU_PLAIN, BACKEND_GPU_BATCHED = 0, 1, 2
+
 # every symbol include/hpvcd.h declares, with its ctypes signature
 _P = C.c_void_p
 SIGNATURES = {
@@ -213,6 +234,11 @@ SIGNATURES = {
     "hpvcd_job_download_planes": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
     "hpvcd_job_time": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hpvcd_gpu_decode_batch": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(_P)]),
+    "hpvcd_settings_default": (None, [C.POINTER(Settings)]),
+    "hpvcd_decoder_new": (C.c_int, [C.POINTER(Settings), C.POINTER(_P)]),
+    "hpvcd_decoder_free": (None, [_P]),
+    "hpvcd_decode_records": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(Image)]),
+    "hpvcd_image_free": (None, [C.POINTER(Image)]),
 }
 
 _lib = None

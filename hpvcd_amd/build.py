@@ -48,7 +48,8 @@ def build_library(verbose: bool = True) -> str:
     objs = []
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
-    units = [("runtime.hip", HIP_FLAGS + ["-x", "hip"]), ("gen.cpp", ["-O3", "-std=c++17", "-fPIC", "-Wall"])]
+    host = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
+    units = [("runtime.hip", HIP_FLAGS + ["-x", "hip"]), ("gen.cpp", host), ("decoder.cpp", host)]
     for src, flags in units:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         _run([HIPCC] + flags + ["-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj])

@@ -900,7 +900,7 @@ extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_rec
         ho.width = c.out_width;
         ho.height = c.out_height;
         ho.kind = c.out_kind;
-        ho.channels = c.chroma_idc ? 3 : 1;
+        ho.channels = (c.out_kind == HPVCD_OUT_LUMA8 || c.out_kind == HPVCD_OUT_LUMA16) ? 1 : 3;
         ho.bytes_per_sample = (c.out_kind == HPVCD_OUT_RGB16 || c.out_kind == HPVCD_OUT_LUMA16) ? 2 : 1;
         ho.orientation = c.orientation;
         r->outputs.push_back(ho);

@@ -965,7 +965,9 @@ __global__ void __launch_bounds__(256) k_rgb(const DevPic* __restrict__ pics, co
     const int32_t ky = full ? 8192 : 9539, kc = full ? 8192 : 9326;
     const int32_t y_black = full ? 0 : 16 * scale, neutral = 128 * scale;
     const bool mono = P.chroma_idc == 0;
-    const int chn = mono ? 1 : 3;
+    // luma outputs hold one channel; RGB outputs three (a 4:0:0 picture as gray,
+    // the decode_rgb8 expansion of src/lib.rs:1978)
+    const int chn = (P.out_kind == HPVCD_OUT_LUMA8 || P.out_kind == HPVCD_OUT_LUMA16) ? 1 : 3;
     const int32_t cmax = bd == 8 ? 255 : (1 << bd) - 1;
     const T* Yp = pool + P.plane_off[0] + (size_t)y_row * w;
     const int cw = P.cw, ch = P.ch;

@@ -156,7 +156,9 @@ hpvcd_status validate_batch(const hpvcd_batch& b) {
         if (P.copy_w < 0 || P.copy_h < 0 || P.copy_w > P.win_w || P.copy_h > P.win_h || P.dst_x < 0 || P.dst_y < 0 ||
             (int64_t)P.dst_x + P.copy_w > O.width || (int64_t)P.dst_y + P.copy_h > O.height)
             return set_error(HPVCD_E_INVALID, "picture window placement outside its output image");
-        if ((P.chroma_idc == 0) != (O.channels == 1)) return set_error(HPVCD_E_INVALID, "output channels");
+        const bool luma_out = O.kind == HPVCD_OUT_LUMA8 || O.kind == HPVCD_OUT_LUMA16;
+        if (luma_out && P.chroma_idc != 0) return set_error(HPVCD_E_INVALID, "luma output of a colour picture");
+        if (O.channels != (luma_out ? 1 : 3)) return set_error(HPVCD_E_INVALID, "output channels");
         bool wide = O.kind == HPVCD_OUT_RGB16 || O.kind == HPVCD_OUT_LUMA16;
         if (wide && P.bit_depth_luma == 8) return set_error(HPVCD_E_INVALID, "16-bit output of an 8-bit picture");
         int sw = sub_w_of(P.chroma_idc), sh = sub_h_of(P.chroma_idc);
@@ -237,6 +239,10 @@ struct hpvcd_gpu {
     std::mutex mu;
     uint8_t* pin = nullptr;
     size_t pin_size = 0;
+    // one launch chain at a time on the GPU's streams (the fork/join events
+    // and graph captures are per GPU); decode calls from several threads
+    // interleave whole chains
+    std::mutex launch_mu;
 };
 
 namespace {
@@ -1291,6 +1297,7 @@ hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop) {
     if (!j) return set_error(HPVCD_E_INVALID, "null argument");
     if (stop < HPVCD_STAGE_RECON || stop > HPVCD_STAGE_RGB) return set_error(HPVCD_E_INVALID, "stop stage");
     HIP_TRY(hipSetDevice(j->g->device));
+    std::lock_guard<std::mutex> launch_lock(j->g->launch_mu);
     j->last_stop = stop;
     t_launch_err = hipSuccess;
     if (mode == HPVCD_MODE_GRAPH) {
@@ -1398,7 +1405,10 @@ hpvcd_status hpvcd_job_time(hpvcd_job* j, int mode, int warmup, int iters, doubl
             std::vector<std::pair<int, hipEvent_t>> marks;
             j->last_stop = HPVCD_STAGE_RGB;
             t_launch_err = hipSuccess;
-            launch_job(j, kmode, HPVCD_STAGE_RGB, &marks);
+            {
+                std::lock_guard<std::mutex> launch_lock(j->g->launch_mu);
+                launch_job(j, kmode, HPVCD_STAGE_RGB, &marks);
+            }
             HIP_TRY(t_launch_err);
             HIP_TRY(hipStreamSynchronize(j->g->stream));
             for (size_t m = 0; m + 1 < marks.size(); m++) {

@@ -180,6 +180,61 @@ hpvcd_status hpvcd_job_time(hpvcd_job* j, int mode, int warmup, int iters, doubl
 /* One-call convenience: upload, reconstruct (batched graph), download. */
 hpvcd_status hpvcd_gpu_decode_batch(hpvcd_gpu* g, const hpvcd_batch* b, void** outputs);
 
+/* ------------------------------------------------------------------------ */
+/* Drop-in decoder: the shape of the reference's public API                 */
+/* (src/decoder.rs:54-234 Decoder::{from_settings, decode, decode_rgb8},    */
+/*  src/lib.rs:1476 decode_heic, :1951 decode_heic_rgb8, src/settings.rs:59 */
+/*  HeicSettings, src/limits.rs:85-102 ParseLimits).                        */
+/* ------------------------------------------------------------------------ */
+typedef struct hpvcd_decoder hpvcd_decoder;   /* one GPU, host worker threads; decode calls may overlap */
+
+#define HPVCD_BACKEND_CPU_REF     0  /* the reference's own CPU decoder: not in this library (E_UNSUPPORTED) */
+#define HPVCD_BACKEND_GPU_PLAIN   1  /* one launch per stage per picture */
+#define HPVCD_BACKEND_GPU_BATCHED 2  /* one launch per stage for every picture of the image (default) */
+
+typedef struct hpvcd_settings {
+    int32_t  threads;            /* host worker threads; 0 = available parallelism (DecodeThreads::Available) */
+    uint32_t max_dimension;      /* ParseLimits::DEFAULT_MAX_DIMENSION 16384 */
+    uint64_t max_pixels;         /* 64 Mi pixels */
+    uint64_t max_box_size;       /* 256 MiB */
+    uint64_t max_item_size;      /* 256 MiB */
+    uint32_t max_items;          /* 4096 */
+    uint32_t max_tiles;          /* 16384 */
+    uint8_t  decode_alpha, decode_gain_map;   /* HeicSettings flags; auxiliary images are not decoded here */
+    uint8_t  _pad[2];
+    int32_t  backend;            /* HPVCD_BACKEND_* */
+    int32_t  device;             /* GPU ordinal */
+} hpvcd_settings;
+
+/* A decoded image (DecodedImage / Rgb8Image, src/lib.rs:430-455): pixels
+ * are row-major, interleaved, in page-locked memory the library owns until
+ * hpvcd_image_free. */
+typedef struct hpvcd_image {
+    uint32_t width, height;      /* displayed size (after orientation) */
+    uint8_t  channels;           /* 3 (RGB) or 1 (luma) */
+    uint8_t  bit_depth;          /* bit depth of the coded samples */
+    uint8_t  sample_bytes;       /* 1 (8-bit output) or 2 */
+    uint8_t  orientation;        /* HPVCD_ORIENT_* applied (0/1: none) */
+    void*    pixels;
+    size_t   pixels_len;         /* bytes */
+    uint8_t  matrix;             /* CICP matrix_coefficients used for RGB */
+    uint8_t  full_range;
+    uint8_t  _pad[6];
+} hpvcd_image;
+
+/* HeicSettings::new() with ParseLimits::new() and backend GPU_BATCHED. */
+void         hpvcd_settings_default(hpvcd_settings* s);
+/* Decoder::from_settings: opens the GPU and the host worker threads. */
+hpvcd_status hpvcd_decoder_new(const hpvcd_settings* s, hpvcd_decoder** out);
+void         hpvcd_decoder_free(hpvcd_decoder* d);
+/* Records in (what the host parse of one HEIC image emits), one image per
+ * output of the batch out (images[b->num_outputs]), reconstructed by the
+ * decoder's backend.  Each image is filled even when the call fails part
+ * way (pixels NULL); free every one with hpvcd_image_free. */
+hpvcd_status hpvcd_decode_records(hpvcd_decoder* d, const hpvcd_batch* b, hpvcd_image* images);
+/* Release an image's pixels (the Vec ownership hand-off of the reference). */
+void         hpvcd_image_free(hpvcd_image* img);
+
 #ifdef __cplusplus
 }
 #endif

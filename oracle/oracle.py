@@ -141,7 +141,7 @@ def sao_ctb(src, x0, y0, x1, y1, sao_type, offsets, band_pos=0, eo_class=0, bd=8
 def picture_rgb(batch, p, y, cb, cr) -> np.ndarray:
     pic = batch.pictures[p]
     o = batch.outputs[pic.out_index]
-    ch = 1 if pic.chroma_idc == 0 else 3
+    ch = 1 if o.kind in (3, 4) else 3
     dt = np.uint16 if o.kind in (1, 4) else np.uint8
     out = np.zeros((pic.win_h, pic.win_w, ch), dt)
     y = np.ascontiguousarray(y, np.uint16)

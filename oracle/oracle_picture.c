@@ -897,7 +897,9 @@ int oracle_picture_rgb(const hpvcd_batch* b, uint32_t p, const uint16_t* Y, cons
     uint8_t* o8 = (uint8_t*)out;
     uint16_t* o16 = (uint16_t*)out;
     int mono = P->chroma_idc == 0;
-    int chn = mono ? 1 : 3;
+    /* luma outputs: one channel; RGB outputs of a 4:0:0 picture repeat the
+     * luma value (src/lib.rs:1978 decode_heic_rgb8_with Luma8/Luma16 arms) */
+    int chn = (O->kind == HPVCD_OUT_LUMA8 || O->kind == HPVCD_OUT_LUMA16) ? 1 : 3;
     for (int yy = 0; yy < dh; yy++) {
         int src_y = y0 + yy;
         int y_row = src_y < h - 1 ? src_y : h - 1;
@@ -910,7 +912,7 @@ int oracle_picture_rgb(const hpvcd_batch* b, uint32_t p, const uint16_t* Y, cons
             int64_t L = Y[y_row * w + x_col];
             if (mono) {
                 /* src/yuv.rs:269 mono_rows (edge columns replicate the last in-range one, same as the clamp) */
-                v[0] = (k_y * (L - y_black) + 4096) >> 13;
+                v[0] = v[1] = v[2] = (k_y * (L - y_black) + 4096) >> 13;
             } else {
                 int64_t U = Cb[c_row * cw + c_col], V = Cr[c_row * cw + c_col];
                 if (P->matrix == 0) {
@@ -969,7 +971,7 @@ static int decode_one(const hpvcd_batch* b, uint32_t p, void** outputs, uint16_t
     uint16_t* y = malloc(ny * 2);
     uint16_t* cb = malloc(nc * 2 + 2);
     uint16_t* cr = malloc(nc * 2 + 2);
-    int chn = P->chroma_idc ? 3 : 1;
+    int chn = (O->kind == HPVCD_OUT_LUMA8 || O->kind == HPVCD_OUT_LUMA16) ? 1 : 3;
     int bps = (O->kind == HPVCD_OUT_RGB16 || O->kind == HPVCD_OUT_LUMA16) ? 2 : 1;
     uint8_t* win = malloc((size_t)P->win_w * P->win_h * chn * bps + 1);
     int rc = decode_planes(b, p, ORACLE_STAGE_SAO, y, cb, cr, refp);

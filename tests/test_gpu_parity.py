@@ -337,3 +337,32 @@ def test_mixed_intra_and_inter_waves_stop_at_recon(gpu):
         _check_planes(job, rs, N.STAGE_RECON)
         job.run(N.MODE_GRAPH, N.STAGE_DEBLOCK)
         _check_planes(job, rs, N.STAGE_DEBLOCK)
+
+
+@pytest.mark.parametrize("backend", [N.BACKEND_GPU_PLAIN, N.BACKEND_GPU_BATCHED])
+def test_decoder_records_every_backend(backend):
+    """hpvcd_decode_records through the drop-in decoder object: library-owned
+    images equal the oracle's outputs for both GPU backends, oriented
+    outputs report the swapped size, 4:0:0 decodes to gray RGB8."""
+    cases = [H.RecordSet.generate("grid4032", grid_cols=2, grid_rows=2, out_width=1000, out_height=1000,
+                                  base_seed=131),
+             H.RecordSet.generate(**dict(SMALL["orient_rot90"])),
+             H.RecordSet.generate("single512", chroma_idc=0, out_kind=N.OUT_RGB8, base_seed=133),
+             H.RecordSet.generate(**dict(SMALL["inter420_wp"]))]
+    with H.Decoder(backend=backend) as d:
+        for rs in cases:
+            got = d.decode_records(rs)
+            want = rs.output_buffers()
+            O.decode_batch(rs.batch, want, threads=4)
+            for g, w, o in zip(got, want, rs.outputs()):
+                if o.orientation in (5, 6, 7, 8):
+                    assert g.shape[:2] == (o.width, o.height)
+                assert np.array_equal(g.reshape(-1), w.reshape(-1))
+
+
+def test_decoder_limits():
+    rs = H.RecordSet.generate("single512", base_seed=135)
+    with H.Decoder(max_dimension=256) as d:
+        with pytest.raises(H.HpvcdError) as e:
+            d.decode_records(rs)
+        assert e.value.status == 12  # E_LIMIT

@@ -44,6 +44,7 @@ def test_struct_layouts_match_c():
     assert C.sizeof(N.PU) == 20
     assert C.sizeof(N.Ref) == 20
     assert C.sizeof(N.GenConfig) == 168
+    assert C.sizeof(N.Settings) == 56 and C.sizeof(N.Image) == 40
 
 
 def test_generator_is_deterministic_and_seeded():
@@ -343,3 +344,19 @@ def test_conformance_window_places_grid_tiles():
         assert (P.crop_left, P.crop_top, P.win_w, P.win_h) == (8, 4, 232, 112)
         assert (P.dst_x, P.dst_y) == ((i % 3) * 232, (i // 3) * 112)
     assert N.lib().hpvcd_validate_batch(C.byref(rs.batch)) == 0
+
+
+def test_decoder_settings_defaults_and_backends():
+    """HeicSettings::new() + ParseLimits::new() (src/settings.rs:59,
+    src/limits.rs:85-102); the CPU_REF backend is the reference decoder itself
+    and is refused, as is an unknown backend, before any GPU is touched."""
+    s = N.Settings()
+    N.lib().hpvcd_settings_default(C.byref(s))
+    assert (s.threads, s.max_dimension, s.max_pixels) == (0, 16384, 64 << 20)
+    assert (s.max_box_size, s.max_item_size, s.max_items, s.max_tiles) == (256 << 20, 256 << 20, 4096, 16384)
+    assert s.decode_alpha == 1 and s.decode_gain_map == 1 and s.backend == N.BACKEND_GPU_BATCHED
+    h = C.c_void_p()
+    s.backend = N.BACKEND_CPU_REF
+    assert N.lib().hpvcd_decoder_new(C.byref(s), C.byref(h)) == 11 and not h.value   # E_UNSUPPORTED
+    s.backend = 7
+    assert N.lib().hpvcd_decoder_new(C.byref(s), C.byref(h)) == 15 and not h.value   # E_INVALID
