@@ -159,6 +159,21 @@ class RecordSet:
         n = self.batch.num_cells
         return np.ctypeslib.as_array(self.batch.cell_info, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
 
+    _RAW = {"pictures": ("num_pictures", N.Picture), "tbs": ("num_tbs", N.TB), "coeffs": ("num_coeffs", C.c_uint32),
+            "sao": ("num_ctbs", N.Sao), "cell_info": ("num_cells", C.c_uint32), "scaling": ("num_scaling", C.c_uint8),
+            "outputs": ("num_outputs", N.Output), "pus": ("num_pus", N.PU), "refs": ("num_refs", N.Ref)}
+
+    def raw(self, name: str) -> bytes:
+        """The bytes of one record array (pictures, tbs, coeffs, sao,
+        cell_info, scaling, outputs, pus, refs)."""
+        count, ctype = self._RAW[name]
+        n = getattr(self.batch, count)
+        if name == "scaling":
+            n *= N.SCALING_SET_BYTES
+        if n == 0:
+            return b""
+        return C.string_at(C.addressof(getattr(self.batch, name).contents), n * C.sizeof(ctype))
+
     def outputs(self) -> List[N.Output]:
         return [self.batch.outputs[i] for i in range(self.batch.num_outputs)]
 

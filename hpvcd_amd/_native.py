@@ -182,7 +182,8 @@ class GenConfig(C.Structure):
         ("orientation", C.c_uint8), ("sign_hiding", C.c_uint8),
         ("conf_win", C.c_uint16 * 4),
         ("p_sao_merge", C.c_float),
-        ("_pad2", C.c_uint8 * 4),
+        ("persistent_rice", C.c_uint8), ("ts_context", C.c_uint8),
+        ("bypass_alignment", C.c_uint8), ("_pad2", C.c_uint8),
     ]
 
 

@@ -9,11 +9,16 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhpvcd.so")
+
+# host C++ units: record generator, coding-tree walker, CABAC, parameter
+# sets, decoder object
+HOST_UNITS = ["gen.cpp", "walker.cpp", "cabac.cpp", "hevc_ps.cpp", "decoder.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HPVCD_OFFLOAD_ARCH", "gfx950")
@@ -49,11 +54,15 @@ def build_library(verbose: bool = True) -> str:
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
     host = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
-    units = [("runtime.hip", HIP_FLAGS + ["-x", "hip"]), ("gen.cpp", host), ("decoder.cpp", host)]
+    units = [("runtime.hip", HIP_FLAGS + ["-x", "hip"])] + [(u, host) for u in HOST_UNITS]
+    jobs = []
     for src, flags in units:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
-        _run([HIPCC] + flags + ["-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj])
+        jobs.append([HIPCC] + flags + ["-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as pool:
+        for f in [pool.submit(_run, j) for j in jobs]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)

@@ -1,0 +1,255 @@
+// cabac.h — CABAC for the HEVC syntax of the record producer.
+//
+// Context models and their initialisation (H.265 9.3.2.2; the values the
+// reference decoder uses, src/cabac/contexts.rs:108-430), the arithmetic
+// decoding engine (9.3.4.3; src/cabac/engine.rs:219 decode_bin, bypass,
+// terminate, the PCM byte alignment of :313), the arithmetic encoding engine
+// of 9.3.5 (the reference has none: it pairs with the decoder above), and
+// residual_coding in both directions (7.3.8.11, 9.3.4.2.5-7;
+// src/cabac/residual_full.rs:392).
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "bitio.h"
+
+namespace hpvcd {
+
+// ---------------------------------------------------------------------------
+// context index layout (one flat array per slice)
+// ---------------------------------------------------------------------------
+enum CtxOff : int {
+    CTX_SAO_MERGE = 0,
+    CTX_SAO_TYPE = CTX_SAO_MERGE + 1,
+    CTX_SPLIT_CU = CTX_SAO_TYPE + 1,
+    CTX_TQB = CTX_SPLIT_CU + 3,
+    CTX_SKIP = CTX_TQB + 1,
+    CTX_PRED_MODE = CTX_SKIP + 3,
+    CTX_PART_MODE = CTX_PRED_MODE + 1,
+    CTX_PREV_INTRA = CTX_PART_MODE + 4,
+    CTX_CHROMA_MODE = CTX_PREV_INTRA + 1,
+    CTX_RQT_ROOT_CBF = CTX_CHROMA_MODE + 1,
+    CTX_MERGE_FLAG = CTX_RQT_ROOT_CBF + 1,
+    CTX_MERGE_IDX = CTX_MERGE_FLAG + 1,
+    CTX_INTER_PRED = CTX_MERGE_IDX + 1,
+    CTX_REF_IDX = CTX_INTER_PRED + 5,
+    CTX_MVP_FLAG = CTX_REF_IDX + 2,
+    CTX_MVD = CTX_MVP_FLAG + 1,  // abs_mvd_greater0_flag, abs_mvd_greater1_flag
+    CTX_SPLIT_TF = CTX_MVD + 2,
+    CTX_CBF_LUMA = CTX_SPLIT_TF + 3,
+    CTX_CBF_CHROMA = CTX_CBF_LUMA + 2,
+    CTX_QP_DELTA = CTX_CBF_CHROMA + 5,
+    CTX_TS = CTX_QP_DELTA + 2,
+    CTX_LAST_X = CTX_TS + 2,
+    CTX_LAST_Y = CTX_LAST_X + 18,
+    CTX_CSBF = CTX_LAST_Y + 18,
+    CTX_SIG = CTX_CSBF + 4,
+    CTX_G1 = CTX_SIG + 44,
+    CTX_G2 = CTX_G1 + 24,
+    CTX_RDPCM_FLAG = CTX_G2 + 6,
+    CTX_RDPCM_DIR = CTX_RDPCM_FLAG + 2,
+    CTX_RES_SCALE = CTX_RDPCM_DIR + 2,
+    CTX_RES_SIGN = CTX_RES_SCALE + 8,
+    CTX_ACT = CTX_RES_SIGN + 2,
+    CTX_COUNT = CTX_ACT + 1
+};
+
+// One syntax element's contexts: name, offset, count (for the table pins).
+struct CtxElement {
+    const char* name;
+    int off, count;
+};
+extern const CtxElement kCtxElements[];
+extern const int kNumCtxElements;
+
+// initValue of every context for initType 0 (I), 1 and 2 (9.3.2.2)
+const uint8_t (&ctx_init_table())[3][CTX_COUNT];
+// rangeTabLps row of pStateIdx s (Table 9-52)
+const uint8_t* cabac_range_lps_row(int s);
+
+// packed context: bits 0-5 pStateIdx, bit 6 valMps
+struct Contexts {
+    uint8_t s[CTX_COUNT];
+    uint8_t stat_coeff[4];  // persistent Rice statistics (9.3.2.2, RExt)
+    void init(int init_type, int slice_qp);
+};
+
+// ---------------------------------------------------------------------------
+// arithmetic decoding engine (9.3.4.3)
+// ---------------------------------------------------------------------------
+struct CabacDec {
+    const uint8_t* data = nullptr;
+    size_t len = 0;       // bytes of the slice data
+    size_t pos = 0;       // bits consumed
+    uint32_t range = 510, offset = 0;
+    bool overrun = false;
+
+    uint32_t read_bit() {
+        if (pos >= len * 8) {  // past the end: stuffing ones, as the reference reads
+            pos++;
+            overrun = true;
+            return 1;
+        }
+        uint32_t b = (data[pos >> 3] >> (7 - (pos & 7))) & 1u;
+        pos++;
+        return b;
+    }
+    uint32_t read_bits(int n) {
+        uint32_t v = 0;
+        for (int i = 0; i < n; i++) v = (v << 1) | read_bit();
+        return v;
+    }
+    void start(const uint8_t* d, size_t n) {
+        data = d;
+        len = n;
+        pos = 0;
+        init_engine();
+    }
+    void init_engine() {  // 9.3.2.5
+        range = 510;
+        offset = read_bits(9);
+    }
+    int decode(uint8_t& ctx);
+    int bypass() {
+        offset = (offset << 1) | read_bit();
+        if (offset >= range) {
+            offset -= range;
+            return 1;
+        }
+        return 0;
+    }
+    uint32_t bypass_bits(int n) {
+        uint32_t v = 0;
+        for (int i = 0; i < n; i++) v = (v << 1) | (uint32_t)bypass();
+        return v;
+    }
+    int terminate() {
+        range -= 2;
+        if (offset >= range) return 1;
+        renorm();
+        return 0;
+    }
+    void renorm() {
+        while (range < 256) {
+            range <<= 1;
+            offset = (offset << 1) | read_bit();
+        }
+    }
+    void align_bypass() { range = 256; }
+    // to the next byte boundary after a terminating bin (pcm_flag, end of
+    // a sub-stream): the bits the engine holds are given back
+    void byte_align() { pos = (pos + 7) & ~(size_t)7; }
+};
+
+// ---------------------------------------------------------------------------
+// arithmetic encoding engine (9.3.5, bit-serial with outstanding bits)
+// ---------------------------------------------------------------------------
+struct CabacEnc {
+    BitWriter* w = nullptr;
+    uint32_t low = 0, range = 510;
+    uint32_t outstanding = 0;
+    bool first = true;
+
+    void start(BitWriter* out) {
+        w = out;
+        low = 0;
+        range = 510;
+        outstanding = 0;
+        first = true;
+    }
+    void put(uint32_t b) {
+        if (first) first = false;
+        else w->bit(b);
+        while (outstanding) {
+            w->bit(1 - b);
+            outstanding--;
+        }
+    }
+    void renorm() {
+        while (range < 256) {
+            if (low < 256) put(0);
+            else if (low >= 512) {
+                low -= 512;
+                put(1);
+            } else {
+                low -= 256;
+                outstanding++;
+            }
+            range <<= 1;
+            low <<= 1;
+        }
+    }
+    void encode(uint8_t& ctx, int bin);
+    void bypass(int bin) {
+        low <<= 1;
+        if (bin) low += range;
+        if (low >= 1024) {
+            put(1);
+            low -= 1024;
+        } else if (low < 512) put(0);
+        else {
+            low -= 512;
+            outstanding++;
+        }
+    }
+    void bypass_bits(uint32_t v, int n) {
+        for (int i = n - 1; i >= 0; i--) bypass((v >> i) & 1);
+    }
+    // terminating bin; a one flushes the engine, whose last written bit is
+    // the rbsp_stop_one_bit / alignment bit that follows
+    void terminate(int bin) {
+        range -= 2;
+        if (bin) {
+            low += range;
+            range = 2;
+            renorm();
+            put((low >> 9) & 1);
+            w->bits(((low >> 7) & 3) | 1, 2);
+        } else {
+            renorm();
+        }
+    }
+    void align_bypass() { range = 256; }
+};
+
+// ---------------------------------------------------------------------------
+// residual_coding
+// ---------------------------------------------------------------------------
+constexpr int SCAN_DIAG = 0, SCAN_HOR = 1, SCAN_VER = 2;
+
+struct ResidualParams {
+    int log2n = 2;
+    bool luma = true;
+    int scan = SCAN_DIAG;
+    bool ts_allowed = false;        // transform_skip_enabled && !tqb && log2n <= Log2MaxTransformSkipSize
+    bool tqb = false;
+    bool explicit_rdpcm = false;    // explicit_rdpcm_enabled_flag && inter CU
+    bool implicit_rdpcm = false;    // implicit_rdpcm_enabled_flag && intra && mode 10 / 26
+    bool sign_hiding = false;
+    bool persistent_rice = false;
+    bool ts_context = false;
+    bool bypass_alignment = false;
+    bool extended_precision = false;
+    int bit_depth = 8;
+};
+
+// decoded / encoded residual of one TB: levels n*n row-major
+struct ResidualOut {
+    bool ts = false;
+    int rdpcm = -1;  // -1 none, 0 horizontal, 1 vertical (explicit)
+};
+
+void residual_decode(CabacDec& d, Contexts& c, const ResidualParams& p, int32_t* levels, ResidualOut& r);
+void residual_encode(CabacEnc& e, Contexts& c, const ResidualParams& p, const int32_t* levels, const ResidualOut& r);
+// whether the sign of sub-block coefficients at the lowest scan position is
+// hidden; the producer makes each such sign follow the parity of the sum of
+// the sub-block's magnitudes (7.3.8.11 signHidden)
+void residual_hide_signs(const ResidualParams& p, const ResidualOut& r, int32_t* levels);
+// scan order positions (x + y * w) of a w x w grid, w in {1, 2, 4, 8}
+const uint8_t* scan_order(int log2w, int scan);
+// mode-dependent scan (7.4.9.11; src/decode.rs:4481 luma_scan, :4495 chroma_scan)
+int scan_for(int mode, int log2n, bool luma, bool chroma444);
+
+}  // namespace hpvcd

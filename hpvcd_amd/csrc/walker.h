@@ -1,0 +1,192 @@
+// walker.h — one picture's coding-tree syntax, walked once per picture and
+// turned into reconstruction records.
+//
+// The walker owns the parse-side derivations the reference decoder runs while
+// it parses (src/decode.rs): CTB and coding-quadtree order (:1319, :2495),
+// the coding unit (:2585), PCM (:2768), intra mode derivation (:3204 MPM,
+// :3248 chroma), the transform tree and unit (:3315, :3478, :3619 ACT),
+// chroma placement (:5007), QP groups and prediction (:2509, :3084, :5768),
+// deblocking edge marking (:3018) and the per-4x4 side maps, SAO merge
+// (:1507).  Every syntax decision comes from a Source: the seeded generator
+// (random draws, optionally CABAC-encoded) or the CABAC parser.  Both build
+// records through this same code, so a parse of a generated stream yields
+// the generator's records exactly when every decision is recovered.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "../../include/hpvcd.h"
+#include "cabac.h"
+
+namespace hpvcd {
+
+// Picture-level syntax parameters (from the generator's configuration or
+// from the active SPS / PPS / slice header).
+struct PicParams {
+    int w = 0, h = 0, chroma_idc = 1, bd = 8, bd_c = 8;
+    int log2_ctb = 6, log2_min_cb = 3, log2_min_tb = 2, log2_max_tb = 5;
+    int max_depth_intra = 1, max_depth_inter = 1;
+    bool tqb_enabled = false, transform_skip = false;
+    int log2_max_ts = 2;
+    bool implicit_rdpcm = false, explicit_rdpcm = false, cross_component = false, extended_precision = false;
+    bool persistent_rice = false, ts_context = false, bypass_alignment = false;
+    bool sign_hiding = false, act = false;
+    int act_qp[3] = {-5, -5, -3};
+    bool pcm = false;
+    int pcm_bd = 8, pcm_bd_c = 8, log2_min_pcm = 3, log2_max_pcm = 5;
+    bool cu_qp_delta = false;
+    int diff_cu_qp_delta_depth = 0;
+    int chroma_qp_off[2] = {0, 0};   // PPS + slice offsets of Cb, Cr (dequantisation)
+    bool sao = false, sao_luma = false, sao_chroma = false;
+    bool amp = false, constrained_intra = false;
+};
+
+constexpr uint8_t kModePlanar = 0, kModeDC = 1;
+
+inline uint64_t splitmix64_step(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct Rng {
+    uint64_t s;
+    uint64_t next() { return splitmix64_step(s); }
+    bool bern(float p) { return (double)(next() >> 11) * (1.0 / 9007199254740992.0) < (double)p; }
+    uint32_t uni(uint32_t n) { return (uint32_t)(next() % n); }
+    int sint(int lo, int hi) { return lo + (int)uni((uint32_t)(hi - lo + 1)); }
+};
+
+// Decisions of one picture's syntax.  Each is asked only where the syntax
+// element is present; `ctx` arguments are the context increments the
+// walker derived.
+struct Source {
+    virtual ~Source() = default;
+    virtual void begin_slice(int slice_qp) = 0;
+    virtual void begin_substream() = 0;   // a new tile (not the slice's first)
+    // end_of_slice_segment_flag / end_of_subset_one_bit after a CTB
+    virtual int end_ctb(bool last_of_slice, bool last_of_tile) = 0;
+    virtual void sao(hpvcd_sao& s, bool left_avail, bool up_avail, const hpvcd_sao* left, const hpvcd_sao* up) = 0;
+    virtual bool split_cu(int ctx_inc) = 0;
+    virtual bool tqb() = 0;
+    virtual bool intra_nxn() = 0;
+    virtual bool pcm_flag() = 0;
+    // PCM samples of one CU, luma then Cb then Cr: counts and bit depths
+    virtual void pcm_samples(int n_luma, int n_chroma, int bd_y, int bd_c, uint32_t* out) = 0;
+    // luma modes of npu PUs: mpm(k, list) fills PU k's candidates from the
+    // modes committed so far; commit(k, mode) records PU k's mode
+    virtual void luma_modes(int npu, const std::function<void(int, uint8_t*)>& mpm,
+                            const std::function<void(int, uint8_t)>& commit) = 0;
+    virtual int chroma_mode_syntax() = 0;   // intra_chroma_pred_mode 0..4
+    virtual bool split_tf(int log2) = 0;
+    virtual bool cbf_chroma(int depth) = 0;
+    virtual bool cbf_luma(int depth) = 0;
+    virtual bool act_flag() = 0;
+    virtual int qp_delta() = 0;             // CuQpDeltaVal
+    virtual int res_scale(int c) = 0;       // ResScaleVal of Cb (0) / Cr (1)
+    virtual void residual(const ResidualParams& p, int32_t* levels, ResidualOut& r) = 0;
+    // generator-only inter decisions (the parser refuses P/B slices)
+    virtual Rng* rng() { return nullptr; }
+};
+
+// One picture's records as the walker emits them (TBs in decode order, CTBs
+// in tile-scan order until finish() reorders them).
+struct PicRecords {
+    std::vector<hpvcd_tb> tbs;
+    std::vector<uint32_t> coeffs, cells;
+    std::vector<hpvcd_sao> sao;
+    std::vector<hpvcd_pu> pus;
+    std::vector<hpvcd_ref> refs;
+    int num_slices = 1, num_tiles = 1;
+};
+
+class PicWalker {
+public:
+    // gcfg: the generator's configuration for its inter decisions (null when parsing)
+    PicWalker(const PicParams& P, Source& src, PicRecords& out, const hpvcd_gen_config* gcfg = nullptr,
+              uint32_t pic_index = 0);
+    // CTB layout: per raster CTB its slice and tile; the walk visits CTBs in
+    // ts_to_rs order.  slice_qp per slice.
+    void set_layout(const std::vector<uint8_t>& ctb_slice, const std::vector<uint8_t>& ctb_tile);
+    // one CTB (SAO, coding quadtree, end flags); first_in_slice / first_in_tile
+    // reset the QP predictor; returns the end_of_slice_segment_flag
+    int ctb(int rx, int ry, bool first_in_slice, bool first_in_tile, bool last_in_slice, bool last_in_tile,
+            int slice_qp);
+    // cells (QP, slice, tile packed), TB order, inter bS
+    void finish();
+    bool failed() const { return fail_; }
+
+private:
+    struct Motion {
+        uint8_t intra = 1, flags = 0;
+        int32_t pic[2] = {-1, -1};
+        int16_t mv[2][2] = {{0, 0}, {0, 0}};
+    };
+    PicParams P;
+    Source& src;
+    PicRecords& out;
+    const hpvcd_gen_config* gcfg;
+    int w, h, gw, gh, sub_w, sub_h, ctb_cols, ctb_rows;
+    int qp_y, slice_qp = 0, qp_prev = 0, qp_bd_off = 0, log2_qg = 6;
+    bool qp_delta_coded = false;
+    bool fail_ = false;
+    std::vector<uint32_t> cell;
+    std::vector<int8_t> cell_qp;
+    std::vector<uint8_t> mode_y, ct_depth;
+    std::vector<uint8_t> ctb_slice, ctb_tile;
+    int cur_slice = 0, cur_tile = 0;
+    uint32_t coeff_base = 0;
+    // current CU
+    uint8_t cu_chroma_modes[4];
+    bool cu_chroma_dm[4];
+    int cu_x0 = 0, cu_y0 = 0, cu_pu = 1;
+    bool cu_tqb = false, cu_inter = false;
+    // inter pictures (generator only)
+    bool inter_pic = false;
+    int nref[2] = {0, 0};
+    std::vector<uint32_t> ref_pic;
+    std::vector<Motion> motion;
+    std::vector<uint8_t> tu_v, tu_h, pu_v, pu_h, nz;
+    std::vector<int32_t> lv;  // one TB's levels
+
+    size_t g_of(int x, int y) const { return (size_t)(y >> 2) * gw + (x >> 2); }
+    bool same_region(int x, int y) const;
+    void setup_refs(uint32_t pic_index);
+    void mark_tu_edges(int x0, int y0, int size);
+    static bool motion_differs(const Motion& p, const Motion& q);
+    void finalize_bs();
+    double gauss(Rng& r);
+    void prediction_unit(int px, int py, int pw, int ph);
+    void inter_tiles(int x0, int y0, int log2);
+    void inter_cu(int x0, int y0, int log2);
+    void mark_block_edges(int x0, int y0, int size, int bs);
+    void fill_cells(int x0, int y0, int size, uint32_t bit);
+    void set_mode(int x0, int y0, int size, uint8_t mode);
+    uint8_t neighbour_mode(int x, int y) const;
+    void mpm_list(int x0, int y0, uint8_t* mpm) const;
+    int split_ctx(int x0, int y0, int depth) const;
+    int predict_qp(int xqg, int yqg);
+    int qpy_from_delta(int pred, int delta) const;
+    void maybe_qp_delta(bool residual);
+    int qp_prime(int qp, int bitd) const;
+    static int qpc_table(int qpi, int chroma_idc, int bitd);
+    int tb_qp(int comp, bool act);
+    hpvcd_tb make_tb(int x, int y, int log2n, int comp, uint8_t mode, bool cbf, bool act = false);
+    bool chroma_dm_at(int x0, int y0) const;
+    uint8_t chroma_mode_at(int x0, int y0) const;
+    void do_chroma(int lx, int ly, int luma_log2, bool cbf_cb, bool cbf_cr, bool cross_comp);
+    void act_unit(int x0, int y0, int log2, uint8_t luma_mode, bool cbf_luma, bool cbf_cb, bool cbf_cr);
+    void transform_unit(int x0, int y0, int xbase, int ybase, int log2, int depth, int blk, bool cbf_cb, bool cbf_cr);
+    void transform_tree(int x0, int y0, int xbase, int ybase, int log2, int depth, int blk, bool intra_split,
+                        int max_depth, bool pcb, bool pcr);
+    void pcm_unit(int x0, int y0, int log2);
+    void coding_unit(int x0, int y0, int log2);
+    void coding_quadtree(int x0, int y0, int log2, int depth);
+};
+
+}  // namespace hpvcd

@@ -97,7 +97,11 @@ typedef struct hpvcd_gen_config {
     uint8_t  sign_hiding;               /* sign_data_hiding_enabled_flag of the coded stream */
     uint16_t conf_win[4];               /* conformance window: left, right, top, bottom (luma samples) */
     float    p_sao_merge;               /* sao_merge_left / sao_merge_up where the neighbour CTB is available */
-    uint8_t  _pad2[4];
+    /* entropy-coding options of the coded stream (records unchanged) */
+    uint8_t  persistent_rice;           /* persistent_rice_adaptation_enabled_flag */
+    uint8_t  ts_context;                /* transform_skip_context_enabled_flag */
+    uint8_t  bypass_alignment;          /* cabac_bypass_alignment_enabled_flag */
+    uint8_t  _pad2;
 } hpvcd_gen_config;
 
 typedef struct hpvcd_records hpvcd_records;   /* owned record set */
