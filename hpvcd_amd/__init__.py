@@ -22,7 +22,8 @@ import numpy as np
 from . import _native as N
 from ._native import HpvcdError, check
 
-__all__ = ["RecordSet", "Gpu", "Job", "PinnedBuffer", "Decoder", "HpvcdError", "WORKLOADS", "gen_config", "N"]
+__all__ = ["RecordSet", "Gpu", "Job", "PinnedBuffer", "Decoder", "HpvcdError", "WORKLOADS", "gen_config", "gen_hevc",
+           "N"]
 
 
 # --------------------------------------------------------------------------------------
@@ -76,6 +77,13 @@ class RecordSet:
         cfg = gen_config(**params)
         h = C.c_void_p()
         check(N.lib().hpvcd_gen_records(C.byref(cfg), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def parse_hevc(cls, data: bytes, out_kind: int = -1) -> "RecordSet":
+        """Records of an Annex B stream of intra pictures (hpvcd_parse_hevc)."""
+        h = C.c_void_p()
+        check(N.lib().hpvcd_parse_hevc(data, len(data), out_kind, C.byref(h)))
         return cls(h.value)
 
     @classmethod
@@ -222,6 +230,24 @@ _TB_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2n", "u1"), ("comp", "u1"
                       ("flags", "<u2"), ("nx", "u1"), ("res_scale", "i1"), ("coeff_off", "<u4"),
                       ("coeff_cnt", "<u4"), ("order", "<u4")])
 assert _TB_DTYPE.itemsize == C.sizeof(N.TB)
+
+
+def _take_bytes(p, n) -> bytes:
+    try:
+        return C.string_at(p, n.value)
+    finally:
+        N.lib().hpvcd_bytes_free(p)
+
+
+def gen_hevc(workload: Optional[str] = None, **overrides) -> bytes:
+    """The generator's pictures as an Annex B HEVC stream (hpvcd_gen_hevc)."""
+    params = dict(WORKLOADS[workload]) if workload else {}
+    params.update(overrides)
+    cfg = gen_config(**params)
+    p = C.POINTER(C.c_uint8)()
+    n = C.c_size_t()
+    check(N.lib().hpvcd_gen_hevc(C.byref(cfg), C.byref(p), C.byref(n)))
+    return _take_bytes(p, n)
 
 
 class PinnedBuffer:

@@ -54,6 +54,11 @@ OUT_LUMA16 = 4
 ORIENT_NORMAL, ORIENT_FLIP_H, ORIENT_ROTATE_180, ORIENT_FLIP_V = 1, 2, 3, 4
 ORIENT_TRANSPOSE, ORIENT_ROTATE_90, ORIENT_TRANSVERSE, ORIENT_ROTATE_270 = 5, 6, 7, 8
 
+# hpvcd_status codes (1..13 mirror DecodeError, src/error.rs:30-58)
+OK, E_NOT_HEIF, E_TRUNCATED_BOX, E_MISSING_BOX, E_UNSUPPORTED_ITEM, E_BITSTREAM, E_CABAC_DESYNC = 0, 1, 2, 3, 4, 5, 6
+E_UNSUPPORTED_CHROMA, E_UNSUPPORTED_BITDEPTH, E_BAD_DIMENSIONS, E_PARAMSET, E_UNSUPPORTED = 7, 8, 9, 10, 11
+E_LIMIT, E_ALLOC, E_DEVICE, E_INVALID = 12, 13, 14, 15
+
 SCALING_ENTRY_BYTES = 66
 SCALING_SET_BYTES = 4 * 6 * SCALING_ENTRY_BYTES
 
@@ -218,6 +223,13 @@ SIGNATURES = {
     "hpvcd_records_concat": (C.c_int, [C.POINTER(_P), C.c_uint32, C.POINTER(_P)]),
     "hpvcd_records_view": (None, [_P, C.POINTER(Batch)]),
     "hpvcd_records_free": (None, [_P]),
+    "hpvcd_gen_hevc": (C.c_int, [C.POINTER(GenConfig), C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t)]),
+    "hpvcd_bytes_free": (None, [C.POINTER(C.c_uint8)]),
+    "hpvcd_parse_hevc": (C.c_int, [C.c_char_p, C.c_size_t, C.c_int32, C.POINTER(_P)]),
+    "hpvcd_cabac_num_elements": (C.c_int32, []),
+    "hpvcd_cabac_element": (C.c_int32, [C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "hpvcd_cabac_tables": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hpvcd_cabac_init_states": (C.c_int32, [C.c_int32, C.c_int32, C.c_void_p]),
     "hpvcd_output_bytes": (C.c_size_t, [C.POINTER(Output)]),
     "hpvcd_validate_batch": (C.c_int, [C.POINTER(Batch)]),
     "hpvcd_gpu_open": (C.c_int, [C.c_int, C.POINTER(_P)]),

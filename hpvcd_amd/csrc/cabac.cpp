@@ -652,3 +652,38 @@ void residual_encode(CabacEnc& e, Contexts& c, const ResidualParams& p, const in
 }
 
 }  // namespace hpvcd
+
+// ---------------------------------------------------------------------------
+// table introspection for the C API (pins against the reference's tables)
+// ---------------------------------------------------------------------------
+extern "C" int32_t hpvcd_cabac_num_elements(void) { return hpvcd::kNumCtxElements; }
+
+extern "C" int32_t hpvcd_cabac_element(int32_t i, const char** name, int32_t* offset, int32_t* count) {
+    if (i < 0 || i >= hpvcd::kNumCtxElements || !name || !offset || !count) return -1;
+    *name = hpvcd::kCtxElements[i].name;
+    *offset = hpvcd::kCtxElements[i].off;
+    *count = hpvcd::kCtxElements[i].count;
+    return 0;
+}
+
+// initValues (3 x CTX_COUNT), rangeTabLps (64 x 4), transIdxLps / Mps (64 each);
+// returns CTX_COUNT
+extern "C" int32_t hpvcd_cabac_tables(uint8_t* init, uint8_t* range_lps, uint8_t* trans_lps, uint8_t* trans_mps) {
+    if (init) std::memcpy(init, hpvcd::ctx_init_table(), sizeof(uint8_t) * 3 * hpvcd::CTX_COUNT);
+    for (int s = 0; s < 64; s++) {
+        if (range_lps) std::memcpy(range_lps + 4 * s, hpvcd::cabac_range_lps_row(s), 4);
+        if (trans_lps) trans_lps[s] = hpvcd::cabac_trans_lps(s);
+        if (trans_mps) trans_mps[s] = hpvcd::cabac_trans_mps(s);
+    }
+    return hpvcd::CTX_COUNT;
+}
+
+// packed context states (bit 6 valMps, bits 0-5 pStateIdx) after 9.3.2.2
+// initialisation at SliceQpY `qp`
+extern "C" int32_t hpvcd_cabac_init_states(int32_t init_type, int32_t qp, uint8_t* out) {
+    if (init_type < 0 || init_type > 2 || !out) return -1;
+    hpvcd::Contexts c;
+    c.init(init_type, qp);
+    std::memcpy(out, c.s, hpvcd::CTX_COUNT);
+    return hpvcd::CTX_COUNT;
+}

@@ -66,8 +66,10 @@ extern const int kNumCtxElements;
 
 // initValue of every context for initType 0 (I), 1 and 2 (9.3.2.2)
 const uint8_t (&ctx_init_table())[3][CTX_COUNT];
-// rangeTabLps row of pStateIdx s (Table 9-52)
+// rangeTabLps row of pStateIdx s (Table 9-52), transIdxLps / transIdxMps
 const uint8_t* cabac_range_lps_row(int s);
+uint8_t cabac_trans_lps(int s);
+uint8_t cabac_trans_mps(int s);
 
 // packed context: bits 0-5 pStateIdx, bit 6 valMps
 struct Contexts {

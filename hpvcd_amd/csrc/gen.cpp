@@ -46,7 +46,7 @@ PicParams gen_params(const hpvcd_gen_config& c) {
     P.transform_skip = c.transform_skip != 0;
     P.log2_max_ts = c.log2_max_ts;
     P.implicit_rdpcm = P.explicit_rdpcm = c.implicit_rdpcm != 0;
-    P.cross_component = c.cross_component != 0;
+    P.cross_component = c.cross_component != 0 && c.chroma_idc == 3;
     P.extended_precision = c.extended_precision != 0;
     P.persistent_rice = c.persistent_rice != 0;
     P.ts_context = c.ts_context != 0;
@@ -92,6 +92,30 @@ std::vector<uint8_t> gen_slices(const hpvcd_gen_config& cfg, const TileLayout& T
     std::vector<uint8_t> ctb_slice((size_t)nctb, 0);
     for (int ts = 0; ts < nctb; ts++) ctb_slice[T.ts_to_rs[ts]] = (uint8_t)((int64_t)ts * nslices / nctb);
     return ctb_slice;
+}
+
+// Deblocking parameters of the slices of picture `index`: the configured
+// offsets, or per slice a draw of disable, offsets and the across-slices
+// flag.  Values a slice header cannot carry take what a parse infers: a
+// disabled slice has zero offsets, and the across-slices flag of a slice
+// without SAO and without deblocking is the PPS flag.
+void gen_slice_params(const hpvcd_gen_config& c, uint32_t index, int nslices, hpvcd_slice_params* out) {
+    Rng prng{tile_seed(c.base_seed ^ 0x5b1cull, c.config_id, index)};
+    for (int sl = 0; sl < nslices; sl++) {
+        hpvcd_slice_params& S = out[sl];
+        S.disabled = c.deblock ? 0 : 1;
+        S.beta_offset_div2 = c.beta_offset_div2;
+        S.tc_offset_div2 = c.tc_offset_div2;
+        S.lf_across = c.lf_across_slices ? 1 : 0;
+        if (c.random_slice_params) {
+            if (c.deblock) S.disabled = prng.bern(0.2f) ? 1 : 0;
+            S.beta_offset_div2 = (int8_t)prng.sint(-6, 6);
+            S.tc_offset_div2 = (int8_t)prng.sint(-6, 6);
+            if (c.lf_across_slices) S.lf_across = prng.bern(0.5f) ? 1 : 0;
+        }
+        if (S.disabled) S.beta_offset_div2 = S.tc_offset_div2 = 0;
+        if (!c.sao && S.disabled) S.lf_across = c.lf_across_slices ? 1 : 0;
+    }
 }
 
 namespace {
@@ -333,20 +357,7 @@ hpvcd_status gen_assemble(const hpvcd_gen_config& c, std::vector<PicRecords>& ti
         // slice deblocking parameters: the configured offsets, or per slice a
         // draw of disable, offsets and the across-slices flag
         p.num_slices = (uint8_t)t.num_slices;
-        Rng prng{tile_seed(c.base_seed ^ 0x5b1cull, c.config_id, i)};
-        for (int sl = 0; sl < t.num_slices; sl++) {
-            hpvcd_slice_params& S = p.slices[sl];
-            S.disabled = c.deblock ? 0 : 1;
-            S.beta_offset_div2 = c.beta_offset_div2;
-            S.tc_offset_div2 = c.tc_offset_div2;
-            S.lf_across = c.lf_across_slices ? 1 : 0;
-            if (c.random_slice_params) {
-                if (c.deblock) S.disabled = prng.bern(0.2f) ? 1 : 0;
-                S.beta_offset_div2 = (int8_t)prng.sint(-6, 6);
-                S.tc_offset_div2 = (int8_t)prng.sint(-6, 6);
-                if (c.lf_across_slices) S.lf_across = prng.bern(0.5f) ? 1 : 0;
-            }
-        }
+        gen_slice_params(c, i, t.num_slices, p.slices);
         p.slice_qp = c.qp;
         p.out_kind = c.out_kind;
         uint32_t col = i % (uint32_t)c.grid_cols, row = i / (uint32_t)c.grid_cols;

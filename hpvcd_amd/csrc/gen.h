@@ -23,6 +23,9 @@ PicParams gen_params(const hpvcd_gen_config& c);
 // slice index per raster CTB: equal runs of the tile scan
 std::vector<uint8_t> gen_slices(const hpvcd_gen_config& c, const TileLayout& T);
 
+// deblocking parameters of the slices of one picture
+void gen_slice_params(const hpvcd_gen_config& c, uint32_t index, int nslices, hpvcd_slice_params* out);
+
 // Turns the generator's decisions into coded syntax: wrap() returns the
 // Source the walker reads, which forwards every decision to `inner` and
 // codes it.

@@ -579,7 +579,7 @@ int parse_sps(const std::vector<uint8_t>& rbsp, Sps& s) {
             if (r.flag()) return set_error(HPVCD_E_UNSUPPORTED, "current-picture referencing");
             if (r.flag()) return set_error(HPVCD_E_UNSUPPORTED, "palette mode");
             if (r.bits(2)) return set_error(HPVCD_E_UNSUPPORTED, "motion vector resolution control");
-            if (r.flag()) return set_error(HPVCD_E_UNSUPPORTED, "intra boundary filtering disabled");
+            s.intra_boundary_filter_disabled = r.flag();
         }
     }
     if (r.overrun) return set_error(HPVCD_E_BITSTREAM, "SPS truncated");

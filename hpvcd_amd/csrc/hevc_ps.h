@@ -61,6 +61,8 @@ struct Sps {
     bool ts_rotation = false, ts_context = false, implicit_rdpcm = false, explicit_rdpcm = false;
     bool extended_precision = false, intra_smoothing_disabled = false, high_precision_offsets = false;
     bool persistent_rice = false, bypass_alignment = false;
+    // screen content extension (7.3.2.2.3)
+    bool intra_boundary_filter_disabled = false;
 };
 
 struct Pps {

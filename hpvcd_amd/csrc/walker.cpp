@@ -26,6 +26,8 @@ PicWalker::PicWalker(const PicParams& p, Source& s, PicRecords& o, const hpvcd_g
     ctb_tile.assign((size_t)ctb_cols * ctb_rows, 0);
     out.sao.assign((size_t)ctb_cols * ctb_rows, hpvcd_sao());
     coeff_base = (uint32_t)out.coeffs.size();
+    chroma_off[0] = P.chroma_qp_off[0];
+    chroma_off[1] = P.chroma_qp_off[1];
     lv.resize(32 * 32);
     for (int i = 0; i < 4; i++) {
         cu_chroma_modes[i] = kModeDC;
@@ -396,7 +398,7 @@ int PicWalker::qpc_table(int qpi, int chroma_idc, int bitd) {
 int PicWalker::tb_qp(int comp, bool act) {
     if (!act) {
         if (comp == 0) return qp_prime(qp_y, P.bd);
-        return qp_prime(qpc_table(qp_y + P.chroma_qp_off[comp - 1], P.chroma_idc, P.bd_c), P.bd_c);
+        return qp_prime(qpc_table(qp_y + chroma_off[comp - 1], P.chroma_idc, P.bd_c), P.bd_c);
     }
     const int qpi = qp_y + P.act_qp[comp];
     if (comp == 0) return qp_prime(std::clamp(qpi, -qp_bd_off, 51), P.bd);

@@ -113,6 +113,11 @@ public:
     // CTB layout: per raster CTB its slice and tile; the walk visits CTBs in
     // ts_to_rs order.  slice_qp per slice.
     void set_layout(const std::vector<uint8_t>& ctb_slice, const std::vector<uint8_t>& ctb_tile);
+    // chroma QP offsets (PPS + slice) of the slice that follows
+    void set_chroma_offsets(int cb, int cr) {
+        chroma_off[0] = cb;
+        chroma_off[1] = cr;
+    }
     // one CTB (SAO, coding quadtree, end flags); first_in_slice / first_in_tile
     // reset the QP predictor; returns the end_of_slice_segment_flag
     int ctb(int rx, int ry, bool first_in_slice, bool first_in_tile, bool last_in_slice, bool last_in_tile,
@@ -133,6 +138,7 @@ private:
     const hpvcd_gen_config* gcfg;
     int w, h, gw, gh, sub_w, sub_h, ctb_cols, ctb_rows;
     int qp_y, slice_qp = 0, qp_prev = 0, qp_bd_off = 0, log2_qg = 6;
+    int chroma_off[2] = {0, 0};
     bool qp_delta_coded = false;
     bool fail_ = false;
     std::vector<uint32_t> cell;

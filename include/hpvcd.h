@@ -114,6 +114,34 @@ hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_records** out)
 hpvcd_status hpvcd_records_concat(hpvcd_records* const* sets, uint32_t count, hpvcd_records** out);
 void hpvcd_records_view(const hpvcd_records* r, hpvcd_batch* out);
 void hpvcd_records_free(hpvcd_records* r);
+
+/* ------------------------------------------------------------------------ */
+/* Coded HEVC: the record producer's bitstream side                        */
+/* ------------------------------------------------------------------------ */
+/* The generator's pictures as an HEVC elementary stream (Annex B start
+ * codes): VPS, SPS, PPS, then each picture as one IDR access unit of CABAC
+ * coded I slices.  Parsing it (hpvcd_parse_hevc) yields the records
+ * hpvcd_gen_records makes for the same configuration.  Inter configurations
+ * are refused.  *data is malloc'd; release it with hpvcd_bytes_free. */
+hpvcd_status hpvcd_gen_hevc(const hpvcd_gen_config* cfg, uint8_t** data, size_t* size);
+void hpvcd_bytes_free(uint8_t* data);
+/* Parses an Annex B stream of intra pictures into records, one output image
+ * per picture (its conformance window).  out_kind is an HPVCD_OUT_* kind, or
+ * -1 for the kind decode_heic returns: luma for 4:0:0, else RGB, 16-bit
+ * samples above 8-bit depth (src/lib.rs:1880-1916).  Replaces the syntax
+ * half of the reference's FullDecoder (src/decode.rs:718-1435, :2495-3560,
+ * :7617-8177; src/cabac; src/config.rs:566 parse_sps, :990 parse_pps). */
+hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32_t out_kind, hpvcd_records** out);
+/* Introspection of the CABAC tables the parser and the writer share (the
+ * context layout, initValues of initType 0..2, rangeTabLps, transIdxLps /
+ * transIdxMps of H.265 9.3.2.2 and 9.3.4.3; the reference's
+ * src/cabac/contexts.rs:139-400 and src/cabac/engine.rs:35-70).  The
+ * element names are the reference's ContextSet field names. */
+int32_t hpvcd_cabac_num_elements(void);
+int32_t hpvcd_cabac_element(int32_t i, const char** name, int32_t* offset, int32_t* count);
+int32_t hpvcd_cabac_tables(uint8_t* init /* 3 x count */, uint8_t* range_lps /* 64 x 4 */,
+                           uint8_t* trans_lps /* 64 */, uint8_t* trans_mps /* 64 */);
+int32_t hpvcd_cabac_init_states(int32_t init_type, int32_t qp, uint8_t* out /* count */);
 /* Checks every range, size and cross reference of a batch on the host (the
  * same checks hpvcd_job_create runs before touching the GPU). */
 hpvcd_status hpvcd_validate_batch(const hpvcd_batch* b);
