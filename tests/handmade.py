@@ -27,7 +27,9 @@ def picture(w, h, chroma_idc, bd, log2_ctb=4, flags=N.PF_DEBLOCK_DISABLED, deblo
 
 
 def output(w, h, chroma_idc, kind=N.OUT_RGB8):
-    return N.Output(width=w, height=h, kind=kind, channels=3 if chroma_idc else 1,
+    """An output image record; the channel count follows the kind (a 4:0:0
+    picture written as RGB gives gray triples)."""
+    return N.Output(width=w, height=h, kind=kind, channels=1 if kind in (N.OUT_LUMA8, N.OUT_LUMA16) else 3,
                     bytes_per_sample=2 if kind in (N.OUT_RGB16, N.OUT_LUMA16) else 1)
 
 
