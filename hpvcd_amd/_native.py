@@ -247,6 +247,8 @@ SIGNATURES = {
     "hpvcd_job_download_planes": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
     "hpvcd_job_time": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hpvcd_gpu_decode_batch": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(_P)]),
+    "hpvcd_gpu_decode_batch_noalloc": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(_P)]),
+    "hpvcd_noalloc_scratch_bytes": (C.c_size_t, []),
     "hpvcd_settings_default": (None, [C.POINTER(Settings)]),
     "hpvcd_decoder_new": (C.c_int, [C.POINTER(Settings), C.POINTER(_P)]),
     "hpvcd_decoder_free": (None, [_P]),

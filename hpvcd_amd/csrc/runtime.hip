@@ -243,7 +243,46 @@ struct hpvcd_gpu {
     // and graph captures are per GPU); decode calls from several threads
     // interleave whole chains
     std::mutex launch_mu;
+    // this device's copy of the allocation-free entry point's scratch array
+    uint8_t* scratch = nullptr;
 };
+
+// Scratch of hpvcd_gpu_decode_batch_noalloc: every device array of one batch
+// (uploaded records, output images, planes, residuals, sync and border
+// words) is laid out inside it.  Each GPU the library runs on holds its own
+// copy until the process ends.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_setup_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+// calls of the allocation-free entry point on one device share its array:
+// they run one at a time within the process
+std::mutex g_scratch_mu[kMaxDevices];
+
+// The device address of `scratch` on `device` (current), looked up once per
+// device; the array's name is a host-side stand-in, never a device address.
+hpvcd_status scratch_address(int device, uint8_t** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return set_error(HPVCD_E_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_setup_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            std::string m = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return set_error(HPVCD_E_DEVICE, m.c_str());
+        }
+        g_scratch_ptr[device] = static_cast<uint8_t*>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return HPVCD_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -334,6 +373,7 @@ struct hpvcd_job {
     int last_stop = HPVCD_STAGE_RGB;
     std::vector<uint32_t> pic_class, pic_local;   // batch picture -> (class, local index)
     double prep_ms = 0.0, upload_ms = 0.0;        // host preparation; staging + H2D + pool clears
+    bool mem_external = false;                    // the pool is the scratch array (not freed)
 };
 
 namespace {
@@ -1066,6 +1106,10 @@ hpvcd_status hpvcd_gpu_open(int device, hpvcd_gpu** out) {
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[1], k_intra<uint8_t, int32_t>, kIntraThreads, 0));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[2], k_intra<uint16_t, int16_t>, kIntraThreads, 0));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g->occ_intra[3], k_intra<uint16_t, int32_t>, kIntraThreads, 0));
+    if (hpvcd_status st = scratch_address(device, &g->scratch)) {
+        hpvcd_gpu_close(g);
+        return st;
+    }
     *out = g;
     return HPVCD_OK;
 }
@@ -1099,11 +1143,19 @@ void hpvcd_job_free(hpvcd_job* j) {
     (void)hipSetDevice(j->g->device);
     for (auto& C : j->classes)
         if (C.graph) (void)hipGraphExecDestroy(C.graph);
-    if (j->mem.base) (void)hipFree(j->mem.base);
+    if (j->mem.base && !j->mem_external) (void)hipFree(j->mem.base);
     delete j;
 }
 
-hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out) {
+}  // extern "C"
+
+namespace {
+
+// Job creation into a pool of the library's own (ext_base null: hipMalloc,
+// pinned staging) or into an existing device region of ext_size bytes
+// (staged from ordinary host memory, nothing allocated on the device).
+hpvcd_status job_create_impl(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out, uint8_t* ext_base,
+                             size_t ext_size) {
     if (!g || !b || !out) return set_error(HPVCD_E_INVALID, "null argument");
     *out = nullptr;
     const auto t0 = std::chrono::steady_clock::now();
@@ -1218,22 +1270,54 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
     DevBuf sizing;
     const size_t upload_bytes = lay_out(sizing, false, nullptr);
     const size_t total = sizing.used + 256;
-    size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total > free_b) {
-        delete j;
-        return set_error(HPVCD_E_DEVICE, "not enough free HBM for the batch");
+    if (ext_base) {
+        if (total > ext_size) {
+            delete j;
+            std::string m = "batch needs " + std::to_string(total) + " bytes of device scratch; the allocation-free "
+                            "entry point holds " + std::to_string(ext_size);
+            return set_error(HPVCD_E_LIMIT, m.c_str());
+        }
+        j->mem.base = reinterpret_cast<char*>(ext_base);
+        j->mem.size = ext_size;
+        j->mem_external = true;
+    } else {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total > free_b) {
+            delete j;
+            return set_error(HPVCD_E_DEVICE, "not enough free HBM for the batch");
+        }
+        if (hipMalloc(&j->mem.base, total) != hipSuccess) {
+            delete j;
+            return set_error(HPVCD_E_DEVICE, "hipMalloc of the job pool failed");
+        }
+        j->mem.size = total;
     }
-    if (hipMalloc(&j->mem.base, total) != hipSuccess) {
-        delete j;
-        return set_error(HPVCD_E_DEVICE, "hipMalloc of the job pool failed");
-    }
-    j->mem.size = total;
     std::vector<Upload> ups;
     lay_out(j->mem, true, &ups);
     if (j->mem.used > j->mem.size) { hpvcd_job_free(j); return set_error(HPVCD_E_DEVICE, "pool overflow"); }
     const auto t1 = std::chrono::steady_clock::now();
     j->prep_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    {
+    if (ext_base) {
+        // one transfer from ordinary host staging, then the same pool clears
+        // (the scratch holds the previous call's contents)
+        std::vector<uint8_t> stage(upload_bytes);
+        for (auto& u : ups) std::memcpy(stage.data() + u.first, u.second.first, u.second.second);
+        hipError_t e = hipSuccess;
+        if (upload_bytes) e = hipMemcpyAsync(j->mem.base, stage.data(), upload_bytes, hipMemcpyHostToDevice, g->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(j->d_out, 0, ob + 1, g->stream);
+        for (auto& C : j->classes) {
+            size_t rb = C.res32 ? 4 : 2;
+            if (e == hipSuccess) e = hipMemsetAsync(C.d_sync, 0, C.sync_words * sizeof(uint32_t), g->stream);
+            if (e == hipSuccess)
+                e = hipMemsetAsync(C.d_border, 0, (C.units.size() * kBorderWords + 1) * sizeof(uint32_t), g->stream);
+            if (e == hipSuccess) e = hipMemsetAsync(C.d_res, 0, C.res_elems * rb, g->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+        if (e != hipSuccess) {
+            hpvcd_job_free(j);
+            return set_error(HPVCD_E_DEVICE, hipGetErrorString(e));
+        }
+    } else {
         // stage the uploaded arrays at their device offsets in the GPU's
         // pinned buffer, copy them in one transfer, clear the written pools
         std::lock_guard<std::mutex> lock(g->mu);
@@ -1268,6 +1352,14 @@ hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** ou
     j->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
     *out = j;
     return HPVCD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out) {
+    return job_create_impl(g, b, out, nullptr, 0);
 }
 
 size_t hpvcd_job_device_bytes(const hpvcd_job* j) { return j ? j->mem.size : 0; }
@@ -1433,5 +1525,21 @@ hpvcd_status hpvcd_gpu_decode_batch(hpvcd_gpu* g, const hpvcd_batch* b, void** o
     hpvcd_job_free(j);
     return st;
 }
+
+hpvcd_status hpvcd_gpu_decode_batch_noalloc(hpvcd_gpu* g, const hpvcd_batch* b, void** outputs) {
+    if (!g || !b || !outputs) return set_error(HPVCD_E_INVALID, "null argument");
+    if (!g->scratch) return set_error(HPVCD_E_DEVICE, "scratch array address unavailable");
+    HIP_TRY(hipSetDevice(g->device));
+    std::lock_guard<std::mutex> lock(g_scratch_mu[g->device]);
+    hpvcd_job* j = nullptr;
+    hpvcd_status st = job_create_impl(g, b, &j, g->scratch, kScratchBytes);
+    if (st) return st;
+    st = hpvcd_job_run(j, HPVCD_MODE_BATCHED, HPVCD_STAGE_RGB);
+    if (!st) st = hpvcd_job_download(j, outputs);
+    hpvcd_job_free(j);
+    return st;
+}
+
+size_t hpvcd_noalloc_scratch_bytes(void) { return kScratchBytes; }
 
 }  // extern "C"

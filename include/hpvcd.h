@@ -209,8 +209,22 @@ hpvcd_status hpvcd_job_download_planes(hpvcd_job* j, uint32_t picture, uint16_t*
 hpvcd_status hpvcd_job_time(hpvcd_job* j, int mode, int warmup, int iters, double* ms_per_run,
                             double* kernel_ms);
 
-/* One-call convenience: upload, reconstruct (batched graph), download. */
+/* One-call convenience: upload, reconstruct (batched launches), download. */
 hpvcd_status hpvcd_gpu_decode_batch(hpvcd_gpu* g, const hpvcd_batch* b, void** outputs);
+
+/* Allocation-free form of hpvcd_gpu_decode_batch (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as tight
+ * serving loops or code that forbids hipMalloc.  Same inputs (host records)
+ * and outputs (host image buffers), same results.  Every device array of the
+ * batch lives in a file-scope device array of hpvcd_noalloc_scratch_bytes()
+ * bytes (64 MiB, one copy per GPU, held until the process ends); no
+ * hipMalloc / hipFree / hipHostMalloc is called, records are staged from
+ * ordinary host memory.  A batch that needs more is refused with
+ * HPVCD_E_LIMIT before any device work.  Calls on one GPU share the array:
+ * within this process they are serialised; overlapping calls from other
+ * processes or library copies on the same GPU would race. */
+hpvcd_status hpvcd_gpu_decode_batch_noalloc(hpvcd_gpu* g, const hpvcd_batch* b, void** outputs);
+size_t hpvcd_noalloc_scratch_bytes(void);
 
 /* ------------------------------------------------------------------------ */
 /* Drop-in decoder: the shape of the reference's public API                 */
