@@ -23,7 +23,7 @@ from . import _native as N
 from ._native import HpvcdError, check
 
 __all__ = ["RecordSet", "Gpu", "Job", "PinnedBuffer", "Decoder", "HpvcdError", "WORKLOADS", "gen_config", "gen_hevc",
-           "N"]
+           "gen_heic", "N"]
 
 
 # --------------------------------------------------------------------------------------
@@ -84,6 +84,14 @@ class RecordSet:
         """Records of an Annex B stream of intra pictures (hpvcd_parse_hevc)."""
         h = C.c_void_p()
         check(N.lib().hpvcd_parse_hevc(data, len(data), out_kind, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def parse_heic(cls, data: bytes, out_kind: int = -1, settings: Optional[N.Settings] = None) -> "RecordSet":
+        """Records of a HEIC file's primary image (hpvcd_parse_heic)."""
+        h = C.c_void_p()
+        check(N.lib().hpvcd_parse_heic(data, len(data), C.byref(settings) if settings is not None else None, out_kind,
+                                       C.byref(h)))
         return cls(h.value)
 
     @classmethod
@@ -150,7 +158,8 @@ class RecordSet:
         return self.batch.num_pictures
 
     def picture(self, i: int) -> N.Picture:
-        return self.batch.pictures[i]
+        """A copy of picture record i (valid after the set is freed)."""
+        return N.Picture.from_buffer_copy(self.batch.pictures[i])
 
     def tbs(self) -> np.ndarray:
         n = self.batch.num_tbs
@@ -183,7 +192,8 @@ class RecordSet:
         return C.string_at(C.addressof(getattr(self.batch, name).contents), n * C.sizeof(ctype))
 
     def outputs(self) -> List[N.Output]:
-        return [self.batch.outputs[i] for i in range(self.batch.num_outputs)]
+        """Copies of the output records."""
+        return [N.Output.from_buffer_copy(self.batch.outputs[i]) for i in range(self.batch.num_outputs)]
 
     def output_buffers(self) -> List[np.ndarray]:
         bufs = []
@@ -247,6 +257,17 @@ def gen_hevc(workload: Optional[str] = None, **overrides) -> bytes:
     p = C.POINTER(C.c_uint8)()
     n = C.c_size_t()
     check(N.lib().hpvcd_gen_hevc(C.byref(cfg), C.byref(p), C.byref(n)))
+    return _take_bytes(p, n)
+
+
+def gen_heic(workload: Optional[str] = None, **overrides) -> bytes:
+    """The generator's pictures as a HEIC file (hpvcd_gen_heic)."""
+    params = dict(WORKLOADS[workload]) if workload else {}
+    params.update(overrides)
+    cfg = gen_config(**params)
+    p = C.POINTER(C.c_uint8)()
+    n = C.c_size_t()
+    check(N.lib().hpvcd_gen_heic(C.byref(cfg), C.byref(p), C.byref(n)))
     return _take_bytes(p, n)
 
 
@@ -410,6 +431,23 @@ class Decoder:
 
     def __exit__(self, *a):
         self.close()
+
+    def decode(self, data: bytes) -> np.ndarray:
+        """A HEIC file's image (hpvcd_decode_heic / Decoder::decode)."""
+        return self._heic(N.lib().hpvcd_decode_heic, data)
+
+    def decode_rgb8(self, data: bytes) -> np.ndarray:
+        """A HEIC file's image as 8-bit RGB (hpvcd_decode_heic_rgb8 / Decoder::decode_rgb8)."""
+        return self._heic(N.lib().hpvcd_decode_heic_rgb8, data)
+
+    def _heic(self, fn, data: bytes) -> np.ndarray:
+        img = N.Image()
+        st = fn(self._h, data, len(data), C.byref(img))
+        try:
+            check(st)
+            return image_array(img)
+        finally:
+            N.lib().hpvcd_image_free(C.byref(img))
 
     def decode_records(self, records: RecordSet) -> List[np.ndarray]:
         """Every output image of a record set (hpvcd_decode_records)."""

@@ -226,6 +226,8 @@ SIGNATURES = {
     "hpvcd_gen_hevc": (C.c_int, [C.POINTER(GenConfig), C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t)]),
     "hpvcd_bytes_free": (None, [C.POINTER(C.c_uint8)]),
     "hpvcd_parse_hevc": (C.c_int, [C.c_char_p, C.c_size_t, C.c_int32, C.POINTER(_P)]),
+    "hpvcd_parse_heic": (C.c_int, [C.c_char_p, C.c_size_t, C.c_void_p, C.c_int32, C.POINTER(_P)]),
+    "hpvcd_gen_heic": (C.c_int, [C.POINTER(GenConfig), C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t)]),
     "hpvcd_cabac_num_elements": (C.c_int32, []),
     "hpvcd_cabac_element": (C.c_int32, [C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "hpvcd_cabac_tables": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -254,6 +256,8 @@ SIGNATURES = {
     "hpvcd_decoder_free": (None, [_P]),
     "hpvcd_decode_records": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(Image)]),
     "hpvcd_image_free": (None, [C.POINTER(Image)]),
+    "hpvcd_decode_heic": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(Image)]),
+    "hpvcd_decode_heic_rgb8": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(Image)]),
 }
 
 _lib = None

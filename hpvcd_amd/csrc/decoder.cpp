@@ -1,9 +1,10 @@
 // decoder.cpp — the drop-in decoder object of include/hpvcd.h: settings and
 // limits (src/settings.rs:59 HeicSettings, src/limits.rs:85-102
 // ParseLimits), one GPU and its backend mode (src/decoder.rs:54
-// Decoder::from_settings), records-in decode into library-owned images
-// (src/lib.rs:430-455 DecodedImage / Rgb8Image; hpvcd_image_free is the Vec
-// ownership hand-off).
+// Decoder::from_settings), HEIC bytes in (src/decoder.rs:224 decode, :234
+// decode_rgb8: container and CABAC parse on the host, reconstruction on the
+// GPU) or records in, into library-owned images (src/lib.rs:430-455
+// DecodedImage / Rgb8Image; hpvcd_image_free is the Vec ownership hand-off).
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "../../include/hpvcd.h"
+#include "heif.h"
 #include "internal.h"
 
 using namespace hpvcd;
@@ -122,6 +124,10 @@ hpvcd_status hpvcd_decode_records(hpvcd_decoder* d, const hpvcd_batch* b, hpvcd_
             return st;
         }
     }
+    if (b->num_pictures == 0) {  // nothing coded (every grid tile failed): black images
+        for (uint32_t o = 0; o < b->num_outputs; o++) std::memset(images[o].pixels, 0, images[o].pixels_len);
+        return HPVCD_OK;
+    }
     hpvcd_job* j = nullptr;
     if ((st = hpvcd_job_create(d->g, b, &j))) {
         for (uint32_t o = 0; o < b->num_outputs; o++) hpvcd_image_free(&images[o]);
@@ -138,6 +144,32 @@ hpvcd_status hpvcd_decode_records(hpvcd_decoder* d, const hpvcd_batch* b, hpvcd_
     if (st)
         for (uint32_t o = 0; o < b->num_outputs; o++) hpvcd_image_free(&images[o]);
     return st;
+}
+
+namespace {
+
+hpvcd_status decode_heic_kind(hpvcd_decoder* d, const uint8_t* data, size_t size, int kind, hpvcd_image* img) {
+    if (!d || !data || !img) return set_error(HPVCD_E_INVALID, "null argument");
+    std::memset(img, 0, sizeof(*img));
+    hpvcd_records* r = nullptr;
+    HeicInfo info;
+    hpvcd_status st = heic_records(data, size, limits_of(&d->s), d->threads, kind, &r, &info);
+    if (st) return st;
+    hpvcd_batch b;
+    hpvcd_records_view(r, &b);
+    st = hpvcd_decode_records(d, &b, img);
+    hpvcd_records_free(r);
+    return st;
+}
+
+}  // namespace
+
+hpvcd_status hpvcd_decode_heic(hpvcd_decoder* d, const uint8_t* data, size_t size, hpvcd_image* img) {
+    return decode_heic_kind(d, data, size, HEIC_KIND_DEFAULT, img);
+}
+
+hpvcd_status hpvcd_decode_heic_rgb8(hpvcd_decoder* d, const uint8_t* data, size_t size, hpvcd_image* img) {
+    return decode_heic_kind(d, data, size, HEIC_KIND_RGB8, img);
 }
 
 }  // extern "C"

@@ -278,8 +278,33 @@ void         hpvcd_decoder_free(hpvcd_decoder* d);
  * decoder's backend.  Each image is filled even when the call fails part
  * way (pixels NULL); free every one with hpvcd_image_free. */
 hpvcd_status hpvcd_decode_records(hpvcd_decoder* d, const hpvcd_batch* b, hpvcd_image* images);
+/* HEIC bytes in (borrowed for the call), one image out: Decoder::decode
+ * (src/decoder.rs:224, src/lib.rs:1497 decode_heic_impl): RGB for colour,
+ * luma for 4:0:0, 16-bit samples above 8-bit depth, orientation applied.
+ * The container, parameter sets and CABAC are parsed on the host (grid tiles
+ * on the decoder's threads); reconstruction runs on the decoder's GPU.
+ * Grid tiles that fail to parse are left black, as the reference leaves
+ * them.  Free the image with hpvcd_image_free. */
+hpvcd_status hpvcd_decode_heic(hpvcd_decoder* d, const uint8_t* data, size_t size, hpvcd_image* img);
+/* Decoder::decode_rgb8 (src/decoder.rs:234, src/lib.rs:1964): always 3 x
+ * 8-bit, deeper samples >> (bit_depth - 8), 4:0:0 as gray triples. */
+hpvcd_status hpvcd_decode_heic_rgb8(hpvcd_decoder* d, const uint8_t* data, size_t size, hpvcd_image* img);
 /* Release an image's pixels (the Vec ownership hand-off of the reference). */
 void         hpvcd_image_free(hpvcd_image* img);
+
+/* The container half on its own: a HEIC file's primary image as records
+ * (one output).  out_kind: -1 the kind hpvcd_decode_heic returns, -2 the
+ * RGB8 kind of hpvcd_decode_heic_rgb8, else an HPVCD_OUT_* kind.  settings
+ * may be NULL (ParseLimits defaults, all host threads).  Replaces
+ * heif::parse (src/heif.rs:242) and the record-producing half of
+ * decode_heic_impl / decode_grid (src/lib.rs:1497, :1724). */
+hpvcd_status hpvcd_parse_heic(const uint8_t* data, size_t size, const hpvcd_settings* settings, int32_t out_kind,
+                              hpvcd_records** out);
+/* The generator's configuration as a HEIC file: one hvc1 item, or a grid
+ * item over one hidden hvc1 item per picture (hvcC, ispe, colr nclx,
+ * irot/imir; iloc v1 with the grid descriptor in idat).  Parsing it gives
+ * the records hpvcd_gen_records makes.  Free with hpvcd_bytes_free. */
+hpvcd_status hpvcd_gen_heic(const hpvcd_gen_config* cfg, uint8_t** data, size_t* size);
 
 #ifdef __cplusplus
 }
