@@ -7,7 +7,7 @@
 namespace hpvcd {
 
 // H.265 Table 9-52 (rangeTabLps) and Table 9-53 (transIdxLps / transIdxMps)
-static const uint8_t kRangeLps[64][4] = {
+const uint8_t kCabacRangeLps[64][4] = {
     {128, 176, 208, 240}, {128, 167, 197, 227}, {128, 158, 187, 216}, {123, 150, 178, 205}, {116, 142, 169, 195},
     {111, 135, 160, 185}, {105, 128, 152, 175}, {100, 122, 144, 166}, {95, 116, 137, 158},  {90, 110, 130, 150},
     {85, 104, 123, 142},  {81, 99, 117, 135},   {77, 94, 111, 128},   {73, 89, 105, 122},   {69, 85, 100, 116},
@@ -21,13 +21,28 @@ static const uint8_t kRangeLps[64][4] = {
     {11, 13, 15, 18},     {10, 12, 15, 17},     {10, 12, 14, 16},     {9, 11, 13, 15},      {9, 11, 12, 14},
     {8, 10, 12, 14},      {8, 9, 11, 13},       {7, 9, 11, 12},       {7, 9, 10, 12},       {7, 8, 10, 11},
     {6, 8, 9, 11},        {6, 7, 9, 10},        {6, 7, 8, 9},         {2, 2, 2, 2}};
-static const uint8_t kTransLps[64] = {0,  0,  1,  2,  2,  4,  4,  5,  6,  7,  8,  9,  9,  11, 11, 12,
+const uint8_t kCabacTransLps[64] = {0,  0,  1,  2,  2,  4,  4,  5,  6,  7,  8,  9,  9,  11, 11, 12,
                                       13, 13, 15, 15, 16, 16, 18, 18, 19, 19, 21, 21, 22, 22, 23, 24,
                                       24, 25, 26, 26, 27, 27, 28, 29, 29, 30, 30, 30, 31, 32, 32, 33,
                                       33, 33, 34, 34, 35, 35, 35, 36, 36, 36, 37, 37, 37, 38, 38, 63};
 
-const uint8_t* cabac_range_lps_row(int s) { return kRangeLps[s]; }
-uint8_t cabac_trans_lps(int s) { return kTransLps[s]; }
+uint32_t kCabacDecision[128][4];
+
+namespace {
+struct DecisionInit {
+    DecisionInit() {
+        for (int packed = 0; packed < 128; packed++) {
+            const int state = packed & 63, mps = packed >> 6;
+            const uint32_t next_mps = (uint32_t)((state < 62 ? state + 1 : state) | (mps << 6));
+            const uint32_t next_lps = (uint32_t)(kCabacTransLps[state] | ((state == 0 ? mps ^ 1 : mps) << 6));
+            for (int q = 0; q < 4; q++) kCabacDecision[packed][q] = kCabacRangeLps[state][q] | next_mps << 8 | next_lps << 16;
+        }
+    }
+} decision_init;
+}  // namespace
+
+const uint8_t* cabac_range_lps_row(int s) { return kCabacRangeLps[s]; }
+uint8_t cabac_trans_lps(int s) { return kCabacTransLps[s]; }
 uint8_t cabac_trans_mps(int s) { return (uint8_t)(s < 62 ? s + 1 : s); }
 
 // ---------------------------------------------------------------------------
@@ -174,32 +189,14 @@ void Contexts::init(int init_type, int slice_qp) {
     std::memset(stat_coeff, 0, sizeof(stat_coeff));
 }
 
-int CabacDec::decode(uint8_t& ctx) {
-    const int state = ctx & 63, mps = ctx >> 6;
-    const uint32_t lps = kRangeLps[state][(range >> 6) & 3];
-    range -= lps;
-    int bin;
-    if (offset >= range) {
-        bin = mps ^ 1;
-        offset -= range;
-        range = lps;
-        ctx = (uint8_t)(kTransLps[state] | ((state == 0 ? mps ^ 1 : mps) << 6));
-    } else {
-        bin = mps;
-        ctx = (uint8_t)(cabac_trans_mps(state) | (mps << 6));
-    }
-    renorm();
-    return bin;
-}
-
 void CabacEnc::encode(uint8_t& ctx, int bin) {
     const int state = ctx & 63, mps = ctx >> 6;
-    const uint32_t lps = kRangeLps[state][(range >> 6) & 3];
+    const uint32_t lps = kCabacRangeLps[state][(range >> 6) & 3];
     range -= lps;
     if (bin != mps) {
         low += range;
         range = lps;
-        ctx = (uint8_t)(kTransLps[state] | ((state == 0 ? mps ^ 1 : mps) << 6));
+        ctx = (uint8_t)(kCabacTransLps[state] | ((state == 0 ? mps ^ 1 : mps) << 6));
     } else {
         ctx = (uint8_t)(cabac_trans_mps(state) | (mps << 6));
     }
@@ -249,6 +246,32 @@ const Scans& scans() {
 }
 
 // sig_coeff_flag context (9.3.4.2.5; src/cabac/residual_full.rs:189)
+int sig_ctx(int xc, int yc, int prev_csbf, int log2n, int scan, bool luma);
+
+// sig_ctx of the 16 positions (raster in the 4x4 sub-block) of a sub-block,
+// per TB size, scan, csbf pattern of the right / below sub-blocks, component
+// class and whether it is the DC sub-block
+struct SigTable {
+    uint8_t v[4][3][4][2][2][16];
+    SigTable() {
+        for (int lg = 0; lg < 4; lg++)
+            for (int sc = 0; sc < 3; sc++)
+                for (int prev = 0; prev < 4; prev++)
+                    for (int luma = 0; luma < 2; luma++)
+                        for (int dc = 0; dc < 2; dc++)
+                            for (int pp = 0; pp < 16; pp++) {
+                                const int sbx = dc ? 0 : 1, xc = (sbx << 2) + (pp & 3), yc = pp >> 2;
+                                v[lg][sc][prev][luma][dc][pp] =
+                                    (uint8_t)std::min(sig_ctx(xc, yc, prev, lg + 2, sc, luma != 0), 43);
+                            }
+    }
+};
+
+const SigTable& sig_table() {
+    static const SigTable t;
+    return t;
+}
+
 int sig_ctx(int xc, int yc, int prev_csbf, int log2n, int scan, bool luma) {
     static const uint8_t map4[16] = {0, 1, 4, 5, 2, 3, 4, 5, 6, 6, 8, 8, 7, 7, 8, 8};
     int s;
@@ -390,7 +413,8 @@ bool rdpcm_active(const ResidualParams& p, const ResidualOut& r) {
 void residual_decode(CabacDec& d, Contexts& c, const ResidualParams& p, int32_t* levels, ResidualOut& r) {
     const RcState S = rc_state(p);
     const int n = S.n;
-    std::fill(levels, levels + n * n, 0);
+    if (r.nz) r.nnz = 0;
+    else std::fill(levels, levels + n * n, 0);
     r.ts = false;
     r.rdpcm = -1;
     const int ci = p.luma ? 0 : 1;
@@ -398,6 +422,7 @@ void residual_decode(CabacDec& d, Contexts& c, const ResidualParams& p, int32_t*
     if (p.explicit_rdpcm && (r.ts || p.tqb)) {
         if (d.decode(c.s[CTX_RDPCM_FLAG + ci])) r.rdpcm = d.decode(c.s[CTX_RDPCM_DIR + ci]);
     }
+    const int ts_ctx = (p.ts_context && (r.ts || p.tqb)) ? (p.luma ? 42 : 43) : -1;
     int off, shift;
     last_ctx(p.log2n, p.luma, &off, &shift);
     const int max_group = kGroupIdx[n - 1];
@@ -423,6 +448,7 @@ void residual_decode(CabacDec& d, Contexts& c, const ResidualParams& p, int32_t*
         const int below = sby + 1 < S.sbw ? (int)((csbf >> (sbp + S.sbw)) & 1) : 0;
         const int prev = right | (below << 1);
         const bool infer_dc = i < last_sb && i > 0;
+        const uint8_t* sig_row = sig_table().v[p.log2n - 2][p.scan][prev][p.luma ? 1 : 0][sbp == 0 ? 1 : 0];
         bool coded;
         if (infer_dc) coded = d.decode(c.s[CTX_CSBF + (p.luma ? 0 : 2) + ((right | below) & 1)]) != 0;
         else coded = true;
@@ -441,8 +467,7 @@ void residual_decode(CabacDec& d, Contexts& c, const ResidualParams& p, int32_t*
             bool sig;
             if (k == 0 && infer_dc && E.n == 0) sig = true;
             else {
-                const int cx = (p.ts_context && (r.ts || p.tqb)) ? (p.luma ? 42 : 43)
-                                                                 : std::min(sig_ctx(xc, yc, prev, p.log2n, p.scan, p.luma), 43);
+                const int cx = ts_ctx >= 0 ? ts_ctx : sig_row[pp];
                 sig = d.decode(c.s[CTX_SIG + cx]) != 0;
             }
             if (sig) {
@@ -497,6 +522,7 @@ void residual_decode(CabacDec& d, Contexts& c, const ResidualParams& p, int32_t*
             const int level = (int)std::min<int64_t>((int64_t)base + rem, 0x7fffffff);
             const bool neg = j < nsign && ((signs >> (nsign - 1 - j)) & 1);
             levels[E.coeff[j]] = neg ? -level : level;
+            if (r.nz) r.nz[r.nnz++] = E.coeff[j];
             parity ^= level & 1;
         }
         if (hidden) {
@@ -536,6 +562,7 @@ void residual_encode(CabacEnc& e, Contexts& c, const ResidualParams& p, const in
         e.encode(c.s[CTX_RDPCM_FLAG + ci], r.rdpcm >= 0 ? 1 : 0);
         if (r.rdpcm >= 0) e.encode(c.s[CTX_RDPCM_DIR + ci], r.rdpcm);
     }
+    const int ts_ctx = (p.ts_context && (r.ts || p.tqb)) ? (p.luma ? 42 : 43) : -1;
     // last significant coefficient in scan order
     int last_sb = -1, last_in_sb = -1;
     for (int i = S.sbw * S.sbw - 1; i >= 0 && last_sb < 0; i--) {
@@ -571,6 +598,7 @@ void residual_encode(CabacEnc& e, Contexts& c, const ResidualParams& p, const in
         const int below = sby + 1 < S.sbw ? (int)((csbf >> (sbp + S.sbw)) & 1) : 0;
         const int prev = right | (below << 1);
         const bool infer_dc = i < last_sb && i > 0;
+        const uint8_t* sig_row = sig_table().v[p.log2n - 2][p.scan][prev][p.luma ? 1 : 0][sbp == 0 ? 1 : 0];
         auto lvl = [&](int k) {
             const int pp = S.pos_order[k];
             return levels[((sby << 2) + (pp >> 2)) * n + (sbx << 2) + (pp & 3)];
@@ -589,11 +617,9 @@ void residual_encode(CabacEnc& e, Contexts& c, const ResidualParams& p, const in
         const int top = i == last_sb ? last_in_sb - 1 : 15;
         for (int k = top; k >= 0; k--) {
             const int pp = S.pos_order[k];
-            const int xc = (sbx << 2) + (pp & 3), yc = (sby << 2) + (pp >> 2);
             const bool sig = lvl(k) != 0;
             if (!(k == 0 && infer_dc && E.n == 0)) {
-                const int cx = (p.ts_context && (r.ts || p.tqb)) ? (p.luma ? 42 : 43)
-                                                                 : std::min(sig_ctx(xc, yc, prev, p.log2n, p.scan, p.luma), 43);
+                const int cx = ts_ctx >= 0 ? ts_ctx : sig_row[pp];
                 e.encode(c.s[CTX_SIG + cx], sig ? 1 : 0);
             }
             if (sig) {

@@ -71,6 +71,14 @@ const uint8_t* cabac_range_lps_row(int s);
 uint8_t cabac_trans_lps(int s);
 uint8_t cabac_trans_mps(int s);
 
+// H.265 Table 9-52 (rangeTabLps) and Table 9-53 (transIdxLps)
+extern const uint8_t kCabacRangeLps[64][4];
+extern const uint8_t kCabacTransLps[64];
+// one lookup per decision, built from those two tables: for a packed
+// context (valMps << 6 | pStateIdx) and range class, rangeLps | the packed
+// context after an MPS << 8 | after an LPS << 16
+extern uint32_t kCabacDecision[128][4];
+
 // packed context: bits 0-5 pStateIdx, bit 6 valMps
 struct Contexts {
     uint8_t s[CTX_COUNT];
@@ -87,40 +95,66 @@ struct CabacDec {
     size_t pos = 0;       // bits consumed
     uint32_t range = 510, offset = 0;
     bool overrun = false;
+    // bits [pos, pos + cache_bits) of the stream, most significant first;
+    // past the end the stream reads as one bits (as the reference reads it)
+    uint64_t cache = 0;
+    int cache_bits = 0;
 
-    uint32_t read_bit() {
-        if (pos >= len * 8) {  // past the end: stuffing ones, as the reference reads
-            pos++;
-            overrun = true;
-            return 1;
+    void refill() {
+        size_t byte = (pos + (size_t)cache_bits) >> 3;
+        while (cache_bits <= 56) {
+            const uint64_t b = byte < len ? data[byte] : 0xffu;
+            cache |= b << (56 - cache_bits);
+            cache_bits += 8;
+            byte++;
         }
-        uint32_t b = (data[pos >> 3] >> (7 - (pos & 7))) & 1u;
-        pos++;
-        return b;
     }
+    // n in 0..32
     uint32_t read_bits(int n) {
-        uint32_t v = 0;
-        for (int i = 0; i < n; i++) v = (v << 1) | read_bit();
+        if (cache_bits < n) refill();
+        const uint32_t v = (uint32_t)((cache >> 1) >> (63 - n));
+        cache <<= n;
+        cache_bits -= n;
+        pos += (size_t)n;
         return v;
     }
+    // read past the end of the data, or flagged malformed by a caller
+    bool failed() const { return overrun || pos > len * 8; }
+    uint32_t read_bit() { return read_bits(1); }
     void start(const uint8_t* d, size_t n) {
         data = d;
         len = n;
         pos = 0;
+        cache = 0;
+        cache_bits = 0;
+        overrun = false;
         init_engine();
     }
     void init_engine() {  // 9.3.2.5
         range = 510;
         offset = read_bits(9);
     }
-    int decode(uint8_t& ctx);
+    // DecodeDecision (9.3.4.3.2) with the state transition (9.3.4.3.2.2)
+    // and renormalisation (9.3.4.3.3), without data-dependent branches
+    int decode(uint8_t& ctx) {
+        const uint32_t st = ctx;
+        const uint32_t e = kCabacDecision[st & 127][(range >> 6) & 3];
+        const uint32_t lps = e & 0xffu;
+        range -= lps;
+        const uint32_t m = 0u - (uint32_t)(offset >= range);  // all ones on an LPS
+        offset -= range & m;
+        range ^= (range ^ lps) & m;
+        ctx = (uint8_t)(((e >> 8) ^ (((e >> 8) ^ (e >> 16)) & m)) & 0xffu);
+        const int n = __builtin_clz(range) - 23;  // 0 when range >= 256
+        range <<= n;
+        offset = (offset << n) | read_bits(n);
+        return (int)((st >> 6) ^ (m & 1u));
+    }
     int bypass() {
-        offset = (offset << 1) | read_bit();
-        if (offset >= range) {
-            offset -= range;
-            return 1;
-        }
-        return 0;
+        offset = (offset << 1) | read_bits(1);
+        const uint32_t m = 0u - (uint32_t)(offset >= range);
+        offset -= range & m;
+        return (int)(m & 1u);
     }
     uint32_t bypass_bits(int n) {
         uint32_t v = 0;
@@ -134,15 +168,19 @@ struct CabacDec {
         return 0;
     }
     void renorm() {
-        while (range < 256) {
-            range <<= 1;
-            offset = (offset << 1) | read_bit();
+        if (range < 256) {
+            const int n = __builtin_clz(range) - 23;  // shifts that bring range to [256, 510]
+            range <<= n;
+            offset = (offset << n) | read_bits(n);
         }
     }
     void align_bypass() { range = 256; }
     // to the next byte boundary after a terminating bin (pcm_flag, end of
     // a sub-stream): the bits the engine holds are given back
-    void byte_align() { pos = (pos + 7) & ~(size_t)7; }
+    void byte_align() {
+        const int drop = (int)((8 - (pos & 7)) & 7);
+        if (drop) read_bits(drop);
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -241,6 +279,11 @@ struct ResidualParams {
 struct ResidualOut {
     bool ts = false;
     int rdpcm = -1;  // -1 none, 0 horizontal, 1 vertical (explicit)
+    // nonzero positions (raster) of the decoded levels, when the caller gives
+    // the buffer (n*n entries); the decoder then writes only those levels
+    // and expects the rest of the array to be zero already
+    uint16_t* nz = nullptr;
+    int nnz = -1;
 };
 
 void residual_decode(CabacDec& d, Contexts& c, const ResidualParams& p, int32_t* levels, ResidualOut& r);

@@ -46,8 +46,8 @@ public:
         len_ = len;
         h_ = h;
     }
-    bool bad() const { return bad_ || d_.overrun; }
-    const char* why() const { return d_.overrun && !bad_ ? "slice data ends inside a coding tree unit" : why_; }
+    bool bad() const { return bad_ || d_.failed(); }
+    const char* why() const { return d_.failed() && !bad_ ? "slice data ends inside a coding tree unit" : why_; }
 
     void begin_slice(int slice_qp) override {
         qp_ = slice_qp;
@@ -173,12 +173,10 @@ public:
     }
     void residual(const ResidualParams& p, int32_t* levels, ResidualOut& r) override {
         residual_decode(d_, c_, p, levels, r);
-        const int n = 1 << p.log2n;
-        for (int i = 0; i < n * n; i++)
-            if (levels[i] > HPVCD_COEFF_LEVEL_MAX || levels[i] < HPVCD_COEFF_LEVEL_MIN) {
-                fail("coefficient level beyond the record range");
-                levels[i] = 0;
-            }
+        for (int i = 0; i < r.nnz; i++) {
+            const int32_t v = levels[r.nz[i]];
+            if (v > HPVCD_COEFF_LEVEL_MAX || v < HPVCD_COEFF_LEVEL_MIN) fail("coefficient level beyond the record range");
+        }
     }
 
 private:

@@ -28,7 +28,8 @@ PicWalker::PicWalker(const PicParams& p, Source& s, PicRecords& o, const hpvcd_g
     coeff_base = (uint32_t)out.coeffs.size();
     chroma_off[0] = P.chroma_qp_off[0];
     chroma_off[1] = P.chroma_qp_off[1];
-    lv.resize(32 * 32);
+    lv.assign(32 * 32, 0);
+    nzpos.resize(32 * 32);
     for (int i = 0; i < 4; i++) {
         cu_chroma_modes[i] = kModeDC;
         cu_chroma_dm[i] = false;
@@ -437,6 +438,7 @@ hpvcd_tb PicWalker::make_tb(int x, int y, int log2n, int comp, uint8_t mode, boo
     rp.extended_precision = P.extended_precision;
     rp.bit_depth = comp ? P.bd_c : P.bd;
     ResidualOut ro;
+    ro.nz = nzpos.data();  // lv is all zero between TBs
     const int n = 1 << log2n;
     src.residual(rp, lv.data(), ro);
     if (ro.ts) tb.flags |= HPVCD_TB_TS;
@@ -444,10 +446,22 @@ hpvcd_tb PicWalker::make_tb(int x, int y, int log2n, int comp, uint8_t mode, boo
     if (ro.rdpcm == 1) tb.flags |= HPVCD_TB_RDPCM_V;
     const uint32_t start = (uint32_t)out.coeffs.size();
     int max_x = 0;
-    for (int pos = 0; pos < n * n; pos++) {
-        if (!lv[pos]) continue;
-        out.coeffs.push_back(HPVCD_COEFF_PACK(pos, lv[pos]));
-        max_x = std::max(max_x, pos & (n - 1));
+    if (ro.nnz >= 0) {
+        // the source listed its nonzero positions: raster order, then clear them
+        std::sort(nzpos.begin(), nzpos.begin() + ro.nnz);
+        for (int i = 0; i < ro.nnz; i++) {
+            const int pos = nzpos[i];
+            out.coeffs.push_back(HPVCD_COEFF_PACK(pos, lv[pos]));
+            max_x = std::max(max_x, pos & (n - 1));
+            lv[pos] = 0;
+        }
+    } else {
+        for (int pos = 0; pos < n * n; pos++) {
+            if (!lv[pos]) continue;
+            out.coeffs.push_back(HPVCD_COEFF_PACK(pos, lv[pos]));
+            max_x = std::max(max_x, pos & (n - 1));
+            lv[pos] = 0;
+        }
     }
     tb.coeff_off = start - coeff_base;
     tb.coeff_cnt = (uint32_t)out.coeffs.size() - start;

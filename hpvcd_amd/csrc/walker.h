@@ -158,7 +158,8 @@ private:
     std::vector<uint32_t> ref_pic;
     std::vector<Motion> motion;
     std::vector<uint8_t> tu_v, tu_h, pu_v, pu_h, nz;
-    std::vector<int32_t> lv;  // one TB's levels
+    std::vector<int32_t> lv;       // one TB's levels (all zero between TBs)
+    std::vector<uint16_t> nzpos;   // the positions a source lists as nonzero
 
     size_t g_of(int x, int y) const { return (size_t)(y >> 2) * gw + (x >> 2); }
     bool same_region(int x, int y) const;

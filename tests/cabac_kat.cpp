@@ -130,7 +130,7 @@ static void prop_engine_inverse() {
                 why = "trial " + std::to_string(trial) + " op " + std::to_string(i);
             }
         }
-        if (ok && (d.terminate() != 1 || d.overrun)) {
+        if (ok && (d.terminate() != 1 || d.failed())) {
             ok = false;
             why = "final terminating bin";
         }
