@@ -251,6 +251,7 @@ SIGNATURES = {
     "hpvcd_gpu_decode_batch": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(_P)]),
     "hpvcd_gpu_decode_batch_noalloc": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(_P)]),
     "hpvcd_noalloc_scratch_bytes": (C.c_size_t, []),
+    "hpvcd_prep_digest": (C.c_int, [C.POINTER(Batch), C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "hpvcd_settings_default": (None, [C.POINTER(Settings)]),
     "hpvcd_decoder_new": (C.c_int, [C.POINTER(Settings), C.POINTER(_P)]),
     "hpvcd_decoder_free": (None, [_P]),

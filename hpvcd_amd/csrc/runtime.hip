@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <type_traits>
@@ -13,6 +14,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -440,14 +442,364 @@ DevResJob make_res_job(const hpvcd_batch& b, const DevPic& D, const hpvcd_tb& T,
     return J;
 }
 
-// Availability stamps, units, levels and residual lists for one class.
+struct UnitKey { uint32_t t, pic, plane, idx; };
+
+// One picture's share of a class's prepared arrays.  TB, unit, descriptor,
+// level and map indices are local to the picture until prepare_class merges
+// the pictures (in order) into the class arrays.
+struct PicPrep {
+    DevPic D;
+    std::vector<hpvcd_tb> tbs;
+    std::vector<uint32_t> order;
+    std::vector<DevUnit> units;
+    std::vector<uint32_t> unit_tbs, unit_refoff, prep_units;
+    std::vector<PrepTb> unit_recs;
+    std::vector<uint16_t> level_off;
+    std::vector<UnitKey> keys;
+    std::vector<uint32_t> res[2][4], act;
+    size_t smap_words = 0, rmap_words = 0;
+    hpvcd_status st = HPVCD_OK;
+    std::string msg;
+};
+
+// Availability stamps, wavefront units with their dependency levels and
+// reference-map layout, and residual lists of class-local picture lp.
+hpvcd_status prepare_picture(const hpvcd_batch& b, uint32_t lp, const hpvcd_picture& P,
+                             const std::vector<size_t>& out_off, PicPrep& Q) {
+    DevPic& D = Q.D;
+    std::memset(&D, 0, sizeof(D));
+    D.w = P.width; D.h = P.height;
+    D.cw = chroma_w(P); D.ch = chroma_h(P);
+    D.gw = (P.width + 3) / 4; D.gh = (P.height + 3) / 4;
+    D.sub_w_log2 = sub_w_of(P.chroma_idc) == 2 ? 1 : 0;
+    D.sub_h_log2 = sub_h_of(P.chroma_idc) == 2 ? 1 : 0;
+    D.ctb_log2 = P.log2_ctb;
+    const int ctb = 1 << P.log2_ctb;
+    D.ctb_cols = (P.width + ctb - 1) / ctb; D.ctb_rows = (P.height + ctb - 1) / ctb;
+    D.bd = P.bit_depth_luma; D.bd_c = P.bit_depth_chroma;
+    D.flags = P.flags;
+    D.num_slices = P.num_slices;
+    std::memcpy(D.slices, P.slices, sizeof(D.slices));
+    D.pps_cb_qp_offset = P.pps_cb_qp_offset; D.pps_cr_qp_offset = P.pps_cr_qp_offset;
+    D.chroma_idc = P.chroma_idc; D.matrix = P.matrix;
+    D.crop_left = P.crop_left; D.crop_top = P.crop_top; D.win_w = P.win_w; D.win_h = P.win_h;
+    D.dst_x = P.dst_x; D.dst_y = P.dst_y; D.copy_w = P.copy_w; D.copy_h = P.copy_h;
+    D.out_kind = b.outputs[P.out_index].kind;
+    D.out_width = b.outputs[P.out_index].width;
+    D.out_height = b.outputs[P.out_index].height;
+    D.orient = b.outputs[P.out_index].orientation;
+    D.out_off = out_off[P.out_index];
+    const size_t ny = (size_t)D.w * D.h, nc = (size_t)D.cw * D.ch;
+    // plane offsets relative to the picture (the merge adds the class base)
+    D.plane_off[0] = 0;
+    D.plane_off[1] = ny;
+    D.plane_off[2] = ny + nc;
+    D.scaling_index = P.scaling_index;
+    bool all_across = true, deb = false;
+    for (int s = 0; s < std::max<int>(1, P.num_slices); s++) {
+        if (P.num_slices) { all_across &= P.slices[s].lf_across != 0; deb |= !P.slices[s].disabled; }
+        else deb = !(P.flags & HPVCD_PF_DEBLOCK_DISABLED);
+    }
+    D.sao_across_slices = (P.flags & HPVCD_PF_LF_ACROSS_SLICES) && all_across;
+    D.deblock_on = deb;
+    D.inter = P.pu_count > 0;
+
+    const hpvcd_tb* src_tb = b.tbs + P.tb_begin;
+    const uint32_t ncell = (uint32_t)(D.gw * D.gh);
+    uint32_t suppress = 0;
+    for (uint32_t g = 0; g < ncell; g++) {
+        uint32_t inf = b.cell_info[P.cell_begin + g];
+        if ((inf & HPVCD_CELL_TQB) || ((P.flags & HPVCD_PF_PCM_LF_DISABLED) && (inf & HPVCD_CELL_PCM))) suppress = 1;
+    }
+    D.suppress_any = suppress;
+
+    // availability stamps: luma TB k gets stamp k (>=1) painted over its
+    // cells; a chroma TB sees every luma TB decoded before it.
+    Q.order.assign(ncell, 0xffffffffu);
+    uint32_t stamp = 0;
+    Q.tbs.reserve(P.tb_count);
+    for (uint32_t t = 0; t < P.tb_count; t++) {
+        hpvcd_tb T = src_tb[t];
+        T.flags &= 0xffu;
+        if (T.comp == 0) {
+            stamp++;
+            T.order = stamp;
+            int n = 1 << T.log2n;
+            for (int yy = T.y; yy < std::min(T.y + n, D.h); yy += 4)
+                for (int xx = T.x; xx < std::min(T.x + n, D.w); xx += 4)
+                    Q.order[(size_t)(yy / 4) * D.gw + xx / 4] = stamp;
+        } else {
+            T.order = stamp + 1;
+        }
+        if (T.flags & HPVCD_TB_PCM) T.flags |= HPVCD_TB_CBF;  // the samples travel as levels
+        // ACT mixes all three residuals: each TB of a triple gets one
+        if ((T.flags & (HPVCD_TB_CBF | HPVCD_TB_ACT)) || T.res_scale != 0) T.flags |= TBX_HAS_RES;
+        Q.tbs.push_back(T);
+    }
+
+    // wavefront units: (plane, CTB) in decode order of their TBs
+    const int nplanes = P.chroma_idc ? 3 : 1;
+    const uint32_t nunits = (uint32_t)(nplanes * D.ctb_cols * D.ctb_rows);
+    std::vector<std::vector<uint32_t>> utbs(nunits);
+    // inter TBs leave K1c reconstructed: the wavefront only publishes
+    // their samples on the unit's bottom row and right column
+    std::vector<uint64_t> inter_bottom(nunits, 0), inter_right(nunits, 0);
+    for (uint32_t t = 0; t < P.tb_count; t++) {
+        const hpvcd_tb& T = Q.tbs[t];
+        int sw = T.comp ? D.sub_w_log2 : 0, sh = T.comp ? D.sub_h_log2 : 0;
+        int cx = (T.x << sw) >> P.log2_ctb, cy = (T.y << sh) >> P.log2_ctb;
+        uint32_t u = (uint32_t)((T.comp * D.ctb_rows + cy) * D.ctb_cols + cx);
+        if (T.flags & HPVCD_TB_INTER) {
+            const int ctbw = (1 << P.log2_ctb) >> sw, ctbh = (1 << P.log2_ctb) >> sh, n = 1 << T.log2n;
+            const int x0 = T.x - cx * ctbw, y0 = T.y - cy * ctbh;
+            const uint64_t run = (n >= 64 ? ~0ull : ((1ull << n) - 1ull));
+            if (y0 + n == ctbh) inter_bottom[u] |= run << x0;
+            if (x0 + n == ctbw) inter_right[u] |= run << y0;
+        } else {
+            utbs[u].push_back(t);
+        }
+        if (T.flags & TBX_HAS_RES) {
+            int pass = T.res_scale != 0 ? 1 : 0;
+            Q.res[pass][T.log2n - 2].push_back(t);
+        }
+        if ((T.flags & HPVCD_TB_ACT) && T.comp == 0) Q.act.push_back(t);
+    }
+    for (int pl = 0; pl < nplanes; pl++)
+        for (int cy = 0; cy < D.ctb_rows; cy++)
+            for (int cx = 0; cx < D.ctb_cols; cx++) {
+                uint32_t u = (uint32_t)((pl * D.ctb_rows + cy) * D.ctb_cols + cx);
+                DevUnit U;
+                std::memset(&U, 0, sizeof(U));
+                U.pic = lp; U.cx = (uint16_t)cx; U.cy = (uint16_t)cy; U.plane = (uint8_t)pl;
+                U.inter_bottom = inter_bottom[u];
+                U.inter_right = inter_right[u];
+                // dependency levels inside the unit
+                int sw = pl ? D.sub_w_log2 : 0, sh = pl ? D.sub_h_log2 : 0;
+                int ctbw = ctb >> sw, ctbh = ctb >> sh, cgw = ctbw / 4, cgh = ctbh / 4;
+                int ux0 = cx * ctbw, uy0 = cy * ctbh;
+                std::vector<uint16_t> lmap((size_t)cgw * cgh, 0);
+                std::vector<std::pair<uint16_t, uint32_t>> lt;  // (level, tb)
+                for (uint32_t tbi : utbs[u]) {
+                    const hpvcd_tb& T = Q.tbs[tbi];
+                    int n = 1 << T.log2n, x0 = T.x - ux0, y0 = T.y - uy0;
+                    int lvl = 0;
+                    auto probe = [&](int rx, int ry) {
+                        if (rx < 0 || ry < 0 || rx >= ctbw || ry >= ctbh) return;
+                        lvl = std::max<int>(lvl, lmap[(ry / 4) * cgw + rx / 4]);
+                    };
+                    for (int k = -1; k < 2 * n; k += 4) { probe(x0 - 1, y0 + k); probe(x0 + k, y0 - 1); }
+                    probe(x0 - 1, y0 + 2 * n - 1);
+                    probe(x0 + 2 * n - 1, y0 - 1);
+                    for (int yy = y0; yy < y0 + n; yy += 4)
+                        for (int xx = x0; xx < x0 + n; xx += 4) lmap[(yy / 4) * cgw + xx / 4] = (uint16_t)(lvl + 1);
+                    lt.push_back({(uint16_t)lvl, tbi});
+                }
+                std::stable_sort(lt.begin(), lt.end(),
+                                 [](const std::pair<uint16_t, uint32_t>& a, const std::pair<uint16_t, uint32_t>& b) {
+                                     return a.first < b.first;
+                                 });
+                int nlev = lt.empty() ? 0 : lt.back().first + 1;
+                if (nlev > 255) return set_error(HPVCD_E_INVALID, "too many dependency levels in one CTB");
+                U.nlevels = (uint8_t)nlev;
+                if (lt.size() > (size_t)kMaxUnitTbs) return set_error(HPVCD_E_INVALID, "too many TBs in one CTB");
+                U.ntb = (uint16_t)lt.size();
+                // groups of up to four consecutive independent (same
+                // level) 4x4 TBs that take the fast path: not DC, or I_PCM
+                auto groupable = [&](uint32_t tbi) {
+                    const hpvcd_tb& T = Q.tbs[tbi];
+                    return T.log2n == 2 && (T.mode != 1 || (T.flags & HPVCD_TB_PCM));
+                };
+                // per step leader also: whether a TB of the step writes
+                // the unit's bottom row or right column, whether one is
+                // planar or H/V (not I_PCM), and the next step's size
+                std::vector<size_t> leaders;
+                for (size_t a = 0; a < lt.size();) {
+                    size_t e = a + 1;
+                    if (groupable(lt[a].second))
+                        while (e < lt.size() && e - a < 4 && lt[e].first == lt[a].first && groupable(lt[e].second)) e++;
+                    uint16_t f = (uint16_t)((e - a) << TBX_GROUP_SHIFT);
+                    for (size_t m = a; m < e; m++) {
+                        const hpvcd_tb& M = Q.tbs[lt[m].second];
+                        const int nm = 1 << M.log2n;
+                        if ((M.y - uy0) + nm == ctbh || (M.x - ux0) + nm == ctbw) f |= TBX_STEP_PUB;
+                        if (!(M.flags & HPVCD_TB_PCM) && (M.mode == 0 || M.mode == 10 || M.mode == 26))
+                            f |= TBX_STEP_PLAIN;
+                    }
+                    Q.tbs[lt[a].second].flags |= f;
+                    leaders.push_back(a);
+                    a = e;
+                }
+                for (size_t k = 0; k + 1 < leaders.size(); k++) {
+                    const size_t gnext = (k + 2 < leaders.size() ? leaders[k + 2] : lt.size()) - leaders[k + 1];
+                    Q.tbs[lt[leaders[k]].second].flags |= (uint16_t)((gnext - 1) << TBX_NEXT_SHIFT);
+                }
+                U.tb_list = (uint32_t)Q.unit_tbs.size();
+                U.level_list = (uint32_t)Q.level_off.size();
+                size_t k = 0;
+                for (int l = 0; l <= nlev; l++) {
+                    while (k < lt.size() && lt[k].first < l) k++;
+                    Q.level_off.push_back((uint16_t)k);
+                }
+                // per TB: reference map offset, sample word offset (a
+                // multiple of 16) and level
+                uint32_t roff = 0, soff = 0;
+                // (map offsets of a unit are multiples of 8 entries: the
+                // wavefront stages them in 16-byte vectors)
+                Q.rmap_words = (Q.rmap_words + 7) & ~(size_t)7;
+                U.sbase = (uint32_t)Q.smap_words;
+                U.rbase = (uint32_t)Q.rmap_words;
+                for (auto& e : lt) {
+                    const uint32_t n = 1u << Q.tbs[e.second].log2n;
+                    if (roff + 4 * n + 1 > (uint32_t)kMaxUnitRefs || soff + n * n > (uint32_t)(kMaxCtb * kMaxCtb))
+                        return set_error(HPVCD_E_INVALID, "too many reference samples in one CTB");
+                    Q.unit_tbs.push_back(e.second);
+                    const hpvcd_tb& R = Q.tbs[e.second];
+                    Q.unit_recs.push_back(PrepTb{R.x, R.y, R.log2n, R.mode, R.flags, R.order});
+                    Q.unit_refoff.push_back(roff | ((soff >> 4) << 13) | ((uint32_t)e.first << 21));
+                    roff += 4 * n + 1;
+                    soff += n * n;
+                }
+                U.nsamp = soff;
+                U.nref = roff;
+                U.desc_base = U.tb_list;
+                // Cr repeats Cb's TB geometry, modes and availability in
+                // every conforming stream: when the two units' TBs agree
+                // in everything K2a reads, Cr takes Cb's descriptors,
+                // reference maps and words (residuals are read whole from
+                // the pool, so residual presence does not matter)
+                bool shared = false;
+                if (pl == 2) {
+                    const DevUnit& B = Q.units[Q.units.size() - (size_t)D.ctb_rows * D.ctb_cols];
+                    shared = B.ntb == U.ntb;
+                    const uint16_t keep = HPVCD_TB_PCM | 0xfe00u;  // PCM and the step bits
+                    for (uint32_t e = 0; shared && e < U.ntb; e++) {
+                        const PrepTb& a = Q.unit_recs[B.tb_list + e];
+                        const PrepTb& c = Q.unit_recs[U.tb_list + e];
+                        shared = a.x == c.x && a.y == c.y && a.log2n == c.log2n && a.mode == c.mode &&
+                                 a.order == c.order && (a.flags & keep) == (c.flags & keep);
+                    }
+                    if (shared) {
+                        U.sbase = B.sbase;
+                        U.rbase = B.rbase;
+                        U.desc_base = B.desc_base;
+                    }
+                }
+                if (!shared) {
+                    Q.smap_words += soff;
+                    Q.rmap_words += roff;
+                    Q.prep_units.push_back(u);
+                }
+                Q.units.push_back(U);
+                Q.keys.push_back({(uint32_t)(cx + 2 * cy), lp, (uint32_t)pl, u});
+            }
+    return HPVCD_OK;
+}
+
+// Availability stamps, units, levels and residual lists for one class: the
+// pictures are prepared on `threads` host threads, then merged in order.
 hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<size_t>& out_off,
                            const std::vector<uint32_t>& depth, const std::vector<uint32_t>& pic_local,
-                           const std::vector<uint8_t>& is_ref) {
+                           const std::vector<uint8_t>& is_ref, int threads) {
     const uint32_t np = (uint32_t)C.batch_pics.size();
+    std::vector<PicPrep> Q(np);
+    {
+        std::atomic<uint32_t> next{0};
+        auto work = [&]() {
+            for (uint32_t lp; (lp = next.fetch_add(1)) < np;) {
+                Q[lp].st = prepare_picture(b, lp, b.pictures[C.batch_pics[lp]], out_off, Q[lp]);
+                if (Q[lp].st) Q[lp].msg = g_last_error;
+            }
+        };
+        const int nth = std::max(1, std::min<int>(threads, (int)np));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nth; t++) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+    }
+    for (uint32_t lp = 0; lp < np; lp++)
+        if (Q[lp].st) return set_error(Q[lp].st, Q[lp].msg.c_str());
+    // class offsets of every picture's arrays
+    struct Base { size_t tb, coeff, ctb, cell, unit, utb, lev, smap, rmap, plane; };
+    std::vector<Base> B(np + 1);
+    Base run{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t lp = 0; lp < np; lp++) {
+        const hpvcd_picture& P = b.pictures[C.batch_pics[lp]];
+        const DevPic& D = Q[lp].D;
+        run.rmap = (run.rmap + 7) & ~(size_t)7;
+        B[lp] = run;
+        run.tb += Q[lp].tbs.size();
+        run.coeff += P.coeff_count;
+        run.ctb += (size_t)D.ctb_cols * D.ctb_rows;
+        run.cell += (size_t)D.gw * D.gh;
+        run.unit += Q[lp].units.size();
+        run.utb += Q[lp].unit_tbs.size();
+        run.lev += Q[lp].level_off.size();
+        run.smap += Q[lp].smap_words;
+        run.rmap += Q[lp].rmap_words;
+        run.plane += (size_t)D.w * D.h + 2 * (size_t)D.cw * D.ch;
+    }
+    B[np] = run;
     C.pics.resize(np);
-    size_t plane_elems = 0;
-    uint32_t unit_base = 0;
+    C.tbs.resize(run.tb);
+    C.tb_pic.resize(run.tb);
+    C.coeffs.resize(run.coeff);
+    C.sao.resize(run.ctb);
+    C.cells.resize(run.cell);
+    C.order.resize(run.cell);
+    C.units.resize(run.unit);
+    C.unit_tbs.resize(run.utb);
+    C.unit_recs.resize(run.utb);
+    C.unit_refoff.resize(run.utb);
+    C.level_off.resize(run.lev);
+    {
+        std::atomic<uint32_t> next{0};
+        auto work = [&]() {
+            for (uint32_t lp; (lp = next.fetch_add(1)) < np;) {
+                const hpvcd_picture& P = b.pictures[C.batch_pics[lp]];
+                PicPrep& R = Q[lp];
+                const Base& o = B[lp];
+                DevPic D = R.D;
+                D.tb_begin = (uint32_t)o.tb;
+                D.coeff_begin = (uint32_t)o.coeff;
+                D.ctb_begin = (uint32_t)o.ctb;
+                D.cell_begin = (uint32_t)o.cell;
+                for (int k = 0; k < 3; k++) {
+                    D.plane_off[k] += o.plane;
+                    D.res_off[k] = D.plane_off[k];
+                }
+                C.pics[lp] = D;
+                std::copy(R.tbs.begin(), R.tbs.end(), C.tbs.begin() + o.tb);
+                std::fill(C.tb_pic.begin() + o.tb, C.tb_pic.begin() + o.tb + R.tbs.size(), lp);
+                std::copy(b.coeffs + P.coeff_begin, b.coeffs + P.coeff_begin + P.coeff_count, C.coeffs.begin() + o.coeff);
+                const size_t nctb = (size_t)D.ctb_cols * D.ctb_rows, ncell = (size_t)D.gw * D.gh;
+                std::copy(b.sao + P.ctb_begin, b.sao + P.ctb_begin + nctb, C.sao.begin() + o.ctb);
+                std::copy(b.cell_info + P.cell_begin, b.cell_info + P.cell_begin + ncell, C.cells.begin() + o.cell);
+                std::copy(R.order.begin(), R.order.end(), C.order.begin() + o.cell);
+                for (size_t u = 0; u < R.units.size(); u++) {
+                    DevUnit U = R.units[u];
+                    U.tb_list += (uint32_t)o.utb;
+                    U.desc_base += (uint32_t)o.utb;
+                    U.level_list += (uint32_t)o.lev;
+                    U.sbase += (uint32_t)o.smap;
+                    U.rbase += (uint32_t)o.rmap;
+                    C.units[o.unit + u] = U;
+                }
+                for (size_t e = 0; e < R.unit_tbs.size(); e++) C.unit_tbs[o.utb + e] = R.unit_tbs[e] + (uint32_t)o.tb;
+                std::copy(R.unit_recs.begin(), R.unit_recs.end(), C.unit_recs.begin() + o.utb);
+                std::copy(R.unit_refoff.begin(), R.unit_refoff.end(), C.unit_refoff.begin() + o.utb);
+                std::copy(R.level_off.begin(), R.level_off.end(), C.level_off.begin() + o.lev);
+            }
+        };
+        const int nth = std::max(1, std::min<int>(threads, (int)np));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nth; t++) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+    }
+    C.smap_words = run.smap;
+    C.rmap_words = run.rmap;
+    const size_t plane_elems = run.plane;
+    const uint32_t unit_base = (uint32_t)run.unit;
     C.pic_units_begin.assign(np + 1, 0);
     for (int pass = 0; pass < 2; pass++)
         for (int s = 0; s < 4; s++) C.pic_res_begin[pass][s].assign(np + 1, 0);
@@ -455,53 +807,26 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
     for (int pass = 0; pass < 2; pass++)
         for (int s = 0; s < 4; s++) per_pic_res[pass][s].resize(np);
     std::vector<std::vector<uint32_t>> per_pic_act(np);
-    struct UnitKey { uint32_t t, pic, plane, idx; };
     std::vector<UnitKey> keys;
-
+    keys.reserve(run.unit);
+    C.prep_list.clear();
     for (uint32_t lp = 0; lp < np; lp++) {
+        const PicPrep& R = Q[lp];
+        const Base& o = B[lp];
+        const DevPic& D = C.pics[lp];
         const hpvcd_picture& P = b.pictures[C.batch_pics[lp]];
-        DevPic& D = C.pics[lp];
-        std::memset(&D, 0, sizeof(D));
-        D.w = P.width; D.h = P.height;
-        D.cw = chroma_w(P); D.ch = chroma_h(P);
-        D.gw = (P.width + 3) / 4; D.gh = (P.height + 3) / 4;
-        D.sub_w_log2 = sub_w_of(P.chroma_idc) == 2 ? 1 : 0;
-        D.sub_h_log2 = sub_h_of(P.chroma_idc) == 2 ? 1 : 0;
-        D.ctb_log2 = P.log2_ctb;
-        int ctb = 1 << P.log2_ctb;
-        D.ctb_cols = (P.width + ctb - 1) / ctb; D.ctb_rows = (P.height + ctb - 1) / ctb;
-        D.bd = P.bit_depth_luma; D.bd_c = P.bit_depth_chroma;
-        D.flags = P.flags;
-        D.num_slices = P.num_slices;
-        std::memcpy(D.slices, P.slices, sizeof(D.slices));
-        D.pps_cb_qp_offset = P.pps_cb_qp_offset; D.pps_cr_qp_offset = P.pps_cr_qp_offset;
-        D.chroma_idc = P.chroma_idc; D.matrix = P.matrix;
-        D.crop_left = P.crop_left; D.crop_top = P.crop_top; D.win_w = P.win_w; D.win_h = P.win_h;
-        D.dst_x = P.dst_x; D.dst_y = P.dst_y; D.copy_w = P.copy_w; D.copy_h = P.copy_h;
-        D.out_kind = b.outputs[P.out_index].kind;
-        D.out_width = b.outputs[P.out_index].width;
-        D.out_height = b.outputs[P.out_index].height;
-        D.orient = b.outputs[P.out_index].orientation;
-        D.out_off = out_off[P.out_index];
-        size_t ny = (size_t)D.w * D.h, nc = (size_t)D.cw * D.ch;
-        D.plane_off[0] = plane_elems;
-        D.plane_off[1] = plane_elems + ny;
-        D.plane_off[2] = plane_elems + ny + nc;
-        D.res_off[0] = D.plane_off[0]; D.res_off[1] = D.plane_off[1]; D.res_off[2] = D.plane_off[2];
-        plane_elems += ny + 2 * nc;
-        D.tb_begin = (uint32_t)C.tbs.size();
-        D.coeff_begin = (uint32_t)C.coeffs.size();
-        D.ctb_begin = (uint32_t)C.sao.size();
-        D.cell_begin = (uint32_t)C.cells.size();
-        D.scaling_index = P.scaling_index;
-        bool all_across = true, deb = false;
-        for (int s = 0; s < std::max<int>(1, P.num_slices); s++) {
-            if (P.num_slices) { all_across &= P.slices[s].lf_across != 0; deb |= !P.slices[s].disabled; }
-            else deb = !(P.flags & HPVCD_PF_DEBLOCK_DISABLED);
-        }
-        D.sao_across_slices = (P.flags & HPVCD_PF_LF_ACROSS_SLICES) && all_across;
-        D.deblock_on = deb;
-        D.inter = P.pu_count > 0;
+        C.pic_units_begin[lp] = (uint32_t)o.unit;
+        for (uint32_t u : R.prep_units) C.prep_list.push_back((uint32_t)o.unit + u);
+        for (const UnitKey& k : R.keys) keys.push_back({k.t, k.pic, k.plane, (uint32_t)o.unit + k.idx});
+        for (int pass = 0; pass < 2; pass++)
+            for (int s = 0; s < 4; s++) {
+                auto& v = per_pic_res[pass][s][lp];
+                v.resize(R.res[pass][s].size());
+                for (size_t e = 0; e < v.size(); e++) v[e] = R.res[pass][s][e] + (uint32_t)o.tb;
+            }
+        per_pic_act[lp].resize(R.act.size());
+        for (size_t e = 0; e < R.act.size(); e++) per_pic_act[lp][e] = R.act[e] + (uint32_t)o.tb;
+        const size_t ny = (size_t)D.w * D.h;
         C.max_luma_v = std::max<uint32_t>(C.max_luma_v, (D.w >= 12 ? (D.w - 4) / 8 : 0) * (D.h / 4));
         C.max_luma_h = std::max<uint32_t>(C.max_luma_h, (D.h >= 12 ? (D.h - 4) / 8 : 0) * (D.w / 4));
         C.max_chroma_v = std::max<uint32_t>(C.max_chroma_v, (D.cw >= 10 ? (D.cw - 2) / 8 : 0) * (D.ch / 4));
@@ -511,205 +836,8 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         C.max_ctbs = std::max<uint32_t>(C.max_ctbs, (uint32_t)(D.ctb_cols * D.ctb_rows));
         C.max_rgb_qx = std::max<uint32_t>(C.max_rgb_qx, (uint32_t)((P.copy_w + 3) / 4));
         C.max_rgb_rows = std::max<uint32_t>(C.max_rgb_rows, (uint32_t)P.copy_h);
-
-        // copy records
-        const hpvcd_tb* src_tb = b.tbs + P.tb_begin;
-        C.coeffs.insert(C.coeffs.end(), b.coeffs + P.coeff_begin, b.coeffs + P.coeff_begin + P.coeff_count);
-        uint32_t nctb = (uint32_t)(D.ctb_cols * D.ctb_rows);
-        C.sao.insert(C.sao.end(), b.sao + P.ctb_begin, b.sao + P.ctb_begin + nctb);
-        uint32_t ncell = (uint32_t)(D.gw * D.gh);
-        C.cells.insert(C.cells.end(), b.cell_info + P.cell_begin, b.cell_info + P.cell_begin + ncell);
-        uint32_t suppress = 0;
-        for (uint32_t g = 0; g < ncell; g++) {
-            uint32_t inf = b.cell_info[P.cell_begin + g];
-            if ((inf & HPVCD_CELL_TQB) || ((P.flags & HPVCD_PF_PCM_LF_DISABLED) && (inf & HPVCD_CELL_PCM))) suppress = 1;
-        }
-        D.suppress_any = suppress;
-
-        // availability stamps: luma TB k gets stamp k (>=1) painted over its
-        // cells; a chroma TB sees every luma TB decoded before it.
-        size_t cell0 = C.order.size();
-        C.order.resize(cell0 + ncell, 0xffffffffu);
-        uint32_t stamp = 0;
-        uint32_t tb0 = (uint32_t)C.tbs.size();
-        for (uint32_t t = 0; t < P.tb_count; t++) {
-            hpvcd_tb T = src_tb[t];
-            T.flags &= 0xffu;
-            if (T.comp == 0) {
-                stamp++;
-                T.order = stamp;
-                int n = 1 << T.log2n;
-                for (int yy = T.y; yy < std::min(T.y + n, D.h); yy += 4)
-                    for (int xx = T.x; xx < std::min(T.x + n, D.w); xx += 4)
-                        C.order[cell0 + (size_t)(yy / 4) * D.gw + xx / 4] = stamp;
-            } else {
-                T.order = stamp + 1;
-            }
-            if (T.flags & HPVCD_TB_PCM) T.flags |= HPVCD_TB_CBF;  // the samples travel as levels
-            // ACT mixes all three residuals: each TB of a triple gets one
-            if ((T.flags & (HPVCD_TB_CBF | HPVCD_TB_ACT)) || T.res_scale != 0) T.flags |= TBX_HAS_RES;
-            C.tbs.push_back(T);
-            C.tb_pic.push_back(lp);
-        }
-
-        // wavefront units: (plane, CTB) in decode order of their TBs
-        int nplanes = P.chroma_idc ? 3 : 1;
-        uint32_t nunits = (uint32_t)(nplanes * D.ctb_cols * D.ctb_rows);
-        std::vector<std::vector<uint32_t>> utbs(nunits);
-        // inter TBs leave K1c reconstructed: the wavefront only publishes
-        // their samples on the unit's bottom row and right column
-        std::vector<uint64_t> inter_bottom(nunits, 0), inter_right(nunits, 0);
-        for (uint32_t t = 0; t < P.tb_count; t++) {
-            const hpvcd_tb& T = C.tbs[tb0 + t];
-            int sw = T.comp ? D.sub_w_log2 : 0, sh = T.comp ? D.sub_h_log2 : 0;
-            int cx = (T.x << sw) >> P.log2_ctb, cy = (T.y << sh) >> P.log2_ctb;
-            uint32_t u = (uint32_t)((T.comp * D.ctb_rows + cy) * D.ctb_cols + cx);
-            if (T.flags & HPVCD_TB_INTER) {
-                const int ctbw = (1 << P.log2_ctb) >> sw, ctbh = (1 << P.log2_ctb) >> sh, n = 1 << T.log2n;
-                const int x0 = T.x - cx * ctbw, y0 = T.y - cy * ctbh;
-                const uint64_t run = (n >= 64 ? ~0ull : ((1ull << n) - 1ull));
-                if (y0 + n == ctbh) inter_bottom[u] |= run << x0;
-                if (x0 + n == ctbw) inter_right[u] |= run << y0;
-            } else {
-                utbs[u].push_back(tb0 + t);
-            }
-            if (T.flags & TBX_HAS_RES) {
-                int pass = T.res_scale != 0 ? 1 : 0;
-                per_pic_res[pass][T.log2n - 2][lp].push_back(tb0 + t);
-            }
-            if ((T.flags & HPVCD_TB_ACT) && T.comp == 0) per_pic_act[lp].push_back(tb0 + t);
-        }
-        C.pic_units_begin[lp] = unit_base;
-        for (int pl = 0; pl < nplanes; pl++)
-            for (int cy = 0; cy < D.ctb_rows; cy++)
-                for (int cx = 0; cx < D.ctb_cols; cx++) {
-                    uint32_t u = (uint32_t)((pl * D.ctb_rows + cy) * D.ctb_cols + cx);
-                    DevUnit U;
-                    std::memset(&U, 0, sizeof(U));
-                    U.pic = lp; U.cx = (uint16_t)cx; U.cy = (uint16_t)cy; U.plane = (uint8_t)pl;
-                    U.inter_bottom = inter_bottom[u];
-                    U.inter_right = inter_right[u];
-                    // dependency levels inside the unit
-                    int sw = pl ? D.sub_w_log2 : 0, sh = pl ? D.sub_h_log2 : 0;
-                    int ctbw = ctb >> sw, ctbh = ctb >> sh, cgw = ctbw / 4, cgh = ctbh / 4;
-                    int ux0 = cx * ctbw, uy0 = cy * ctbh;
-                    std::vector<uint16_t> lmap((size_t)cgw * cgh, 0);
-                    std::vector<std::pair<uint16_t, uint32_t>> lt;  // (level, tb)
-                    for (uint32_t tbi : utbs[u]) {
-                        const hpvcd_tb& T = C.tbs[tbi];
-                        int n = 1 << T.log2n, x0 = T.x - ux0, y0 = T.y - uy0;
-                        int lvl = 0;
-                        auto probe = [&](int rx, int ry) {
-                            if (rx < 0 || ry < 0 || rx >= ctbw || ry >= ctbh) return;
-                            lvl = std::max<int>(lvl, lmap[(ry / 4) * cgw + rx / 4]);
-                        };
-                        for (int k = -1; k < 2 * n; k += 4) { probe(x0 - 1, y0 + k); probe(x0 + k, y0 - 1); }
-                        probe(x0 - 1, y0 + 2 * n - 1);
-                        probe(x0 + 2 * n - 1, y0 - 1);
-                        for (int yy = y0; yy < y0 + n; yy += 4)
-                            for (int xx = x0; xx < x0 + n; xx += 4) lmap[(yy / 4) * cgw + xx / 4] = (uint16_t)(lvl + 1);
-                        lt.push_back({(uint16_t)lvl, tbi});
-                    }
-                    std::stable_sort(lt.begin(), lt.end(),
-                                     [](const std::pair<uint16_t, uint32_t>& a, const std::pair<uint16_t, uint32_t>& b) {
-                                         return a.first < b.first;
-                                     });
-                    int nlev = lt.empty() ? 0 : lt.back().first + 1;
-                    if (nlev > 255) return set_error(HPVCD_E_INVALID, "too many dependency levels in one CTB");
-                    U.nlevels = (uint8_t)nlev;
-                    if (lt.size() > (size_t)kMaxUnitTbs) return set_error(HPVCD_E_INVALID, "too many TBs in one CTB");
-                    U.ntb = (uint16_t)lt.size();
-                    // groups of up to four consecutive independent (same
-                    // level) 4x4 TBs that take the fast path: not DC, or I_PCM
-                    auto groupable = [&](uint32_t tbi) {
-                        const hpvcd_tb& T = C.tbs[tbi];
-                        return T.log2n == 2 && (T.mode != 1 || (T.flags & HPVCD_TB_PCM));
-                    };
-                    // per step leader also: whether a TB of the step writes
-                    // the unit's bottom row or right column, whether one is
-                    // planar or H/V (not I_PCM), and the next step's size
-                    std::vector<size_t> leaders;
-                    for (size_t a = 0; a < lt.size();) {
-                        size_t e = a + 1;
-                        if (groupable(lt[a].second))
-                            while (e < lt.size() && e - a < 4 && lt[e].first == lt[a].first && groupable(lt[e].second)) e++;
-                        uint16_t f = (uint16_t)((e - a) << TBX_GROUP_SHIFT);
-                        for (size_t m = a; m < e; m++) {
-                            const hpvcd_tb& M = C.tbs[lt[m].second];
-                            const int nm = 1 << M.log2n;
-                            if ((M.y - uy0) + nm == ctbh || (M.x - ux0) + nm == ctbw) f |= TBX_STEP_PUB;
-                            if (!(M.flags & HPVCD_TB_PCM) && (M.mode == 0 || M.mode == 10 || M.mode == 26))
-                                f |= TBX_STEP_PLAIN;
-                        }
-                        C.tbs[lt[a].second].flags |= f;
-                        leaders.push_back(a);
-                        a = e;
-                    }
-                    for (size_t k = 0; k + 1 < leaders.size(); k++) {
-                        const size_t gnext = (k + 2 < leaders.size() ? leaders[k + 2] : lt.size()) - leaders[k + 1];
-                        C.tbs[lt[leaders[k]].second].flags |= (uint16_t)((gnext - 1) << TBX_NEXT_SHIFT);
-                    }
-                    U.tb_list = (uint32_t)C.unit_tbs.size();
-                    U.level_list = (uint32_t)C.level_off.size();
-                    size_t k = 0;
-                    for (int l = 0; l <= nlev; l++) {
-                        while (k < lt.size() && lt[k].first < l) k++;
-                        C.level_off.push_back((uint16_t)k);
-                    }
-                    // per TB: reference map offset, sample word offset (a
-                    // multiple of 16) and level
-                    uint32_t roff = 0, soff = 0;
-                    // (map offsets of a unit are multiples of 8 entries: the
-                    // wavefront stages them in 16-byte vectors)
-                    C.rmap_words = (C.rmap_words + 7) & ~(size_t)7;
-                    U.sbase = (uint32_t)C.smap_words;
-                    U.rbase = (uint32_t)C.rmap_words;
-                    for (auto& e : lt) {
-                        const uint32_t n = 1u << C.tbs[e.second].log2n;
-                        if (roff + 4 * n + 1 > (uint32_t)kMaxUnitRefs || soff + n * n > (uint32_t)(kMaxCtb * kMaxCtb))
-                            return set_error(HPVCD_E_INVALID, "too many reference samples in one CTB");
-                        C.unit_tbs.push_back(e.second);
-                        const hpvcd_tb& R = C.tbs[e.second];
-                        C.unit_recs.push_back(PrepTb{R.x, R.y, R.log2n, R.mode, R.flags, R.order});
-                        C.unit_refoff.push_back(roff | ((soff >> 4) << 13) | ((uint32_t)e.first << 21));
-                        roff += 4 * n + 1;
-                        soff += n * n;
-                    }
-                    U.nsamp = soff;
-                    U.nref = roff;
-                    U.desc_base = U.tb_list;
-                    // Cr repeats Cb's TB geometry, modes and availability in
-                    // every conforming stream: when the two units' TBs agree
-                    // in everything K2a reads, Cr takes Cb's descriptors,
-                    // reference maps and words (residuals are read whole from
-                    // the pool, so residual presence does not matter)
-                    bool shared = false;
-                    if (pl == 2) {
-                        const DevUnit& B = C.units[C.units.size() - (size_t)D.ctb_rows * D.ctb_cols];
-                        shared = B.ntb == U.ntb;
-                        const uint16_t keep = HPVCD_TB_PCM | 0xfe00u;  // PCM and the step bits
-                        for (uint32_t e = 0; shared && e < U.ntb; e++) {
-                            const PrepTb& a = C.unit_recs[B.tb_list + e];
-                            const PrepTb& c = C.unit_recs[U.tb_list + e];
-                            shared = a.x == c.x && a.y == c.y && a.log2n == c.log2n && a.mode == c.mode &&
-                                     a.order == c.order && (a.flags & keep) == (c.flags & keep);
-                        }
-                        if (shared) {
-                            U.sbase = B.sbase;
-                            U.rbase = B.rbase;
-                            U.desc_base = B.desc_base;
-                        }
-                    }
-                    if (!shared) {
-                        C.smap_words += soff;
-                        C.rmap_words += roff;
-                        C.prep_list.push_back(unit_base + u);
-                    }
-                    C.units.push_back(U);
-                    keys.push_back({(uint32_t)(cx + 2 * cy), lp, (uint32_t)pl, unit_base + u});
-                }
-        unit_base += nunits;
     }
+    std::vector<PicPrep>().swap(Q);
     C.pic_units_begin[np] = unit_base;
     // K2a takes the units with the most predicted samples first, so that the
     // launch's tail holds the small ones; plain mode prepares each picture's
@@ -736,23 +864,41 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         return a.plane < b.plane;
     });
     // plain tickets: per picture, same order restricted to the picture
-    C.plain_tickets.clear();
-    for (uint32_t lp = 0; lp < np; lp++)
-        for (auto& k : sorted)
-            if (k.pic == lp) C.plain_tickets.push_back(k.idx);
-    // residual lists
+    C.plain_tickets.assign(sorted.size(), 0);
+    {
+        std::vector<uint32_t> fill(np + 1, 0);
+        for (auto& k : sorted) fill[k.pic + 1]++;
+        for (uint32_t lp = 0; lp < np; lp++) fill[lp + 1] += fill[lp];
+        for (auto& k : sorted) C.plain_tickets[fill[k.pic]++] = k.idx;
+    }
+    // residual lists (the K1 jobs of each picture built on the host threads)
     for (int pass = 0; pass < 2; pass++)
         for (int s = 0; s < 4; s++) {
             C.res_list[pass][s].clear();
-            C.res_jobs[pass][s].clear();
             for (uint32_t lp = 0; lp < np; lp++) {
                 C.pic_res_begin[pass][s][lp] = (uint32_t)C.res_list[pass][s].size();
                 auto& v = per_pic_res[pass][s][lp];
                 C.res_list[pass][s].insert(C.res_list[pass][s].end(), v.begin(), v.end());
-                for (uint32_t t : v) C.res_jobs[pass][s].push_back(make_res_job(b, C.pics[lp], C.tbs[t], C.pics[lp].coeff_begin));
             }
             C.pic_res_begin[pass][s][np] = (uint32_t)C.res_list[pass][s].size();
+            C.res_jobs[pass][s].resize(C.res_list[pass][s].size());
         }
+    {
+        std::atomic<uint32_t> next{0};
+        auto work = [&]() {
+            for (uint32_t lp; (lp = next.fetch_add(1)) < np;)
+                for (int pass = 0; pass < 2; pass++)
+                    for (int s = 0; s < 4; s++)
+                        for (uint32_t e = C.pic_res_begin[pass][s][lp]; e < C.pic_res_begin[pass][s][lp + 1]; e++)
+                            C.res_jobs[pass][s][e] =
+                                make_res_job(b, C.pics[lp], C.tbs[C.res_list[pass][s][e]], C.pics[lp].coeff_begin);
+        };
+        const int nth = std::max(1, std::min<int>(threads, (int)np));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nth; t++) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+    }
     C.act_list.clear();
     C.pic_act_begin.assign(np + 1, 0);
     for (uint32_t lp = 0; lp < np; lp++) {
@@ -1151,21 +1297,11 @@ void hpvcd_job_free(hpvcd_job* j) {
 
 namespace {
 
-// Job creation into a pool of the library's own (ext_base null: hipMalloc,
-// pinned staging) or into an existing device region of ext_size bytes
-// (staged from ordinary host memory, nothing allocated on the device).
-hpvcd_status job_create_impl(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out, uint8_t* ext_base,
-                             size_t ext_size) {
-    if (!g || !b || !out) return set_error(HPVCD_E_INVALID, "null argument");
-    *out = nullptr;
-    const auto t0 = std::chrono::steady_clock::now();
-    hpvcd_status st = validate_batch(*b);
-    if (st != HPVCD_OK) return st;
-    HIP_TRY(hipSetDevice(g->device));
-    hpvcd_job* j = new (std::nothrow) hpvcd_job();
-    if (!j) return set_error(HPVCD_E_ALLOC, "out of memory");
-    j->g = g;
-    j->outputs.assign(b->outputs, b->outputs + b->num_outputs);
+// Everything job creation does on the host before touching the GPU:
+// output offsets, format classes, reference depths and the per-class
+// preparation (availability stamps, wavefront units, work lists).
+hpvcd_status job_prepare(const hpvcd_batch& b, hpvcd_job* j, int threads) {
+    j->outputs.assign(b.outputs, b.outputs + b.num_outputs);
     size_t ob = 0;
     for (auto& o : j->outputs) {
         j->out_off.push_back(ob);
@@ -1173,10 +1309,10 @@ hpvcd_status job_create_impl(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out
     }
     j->out_bytes = ob;
     // format classes: (16-bit samples, 32-bit residuals)
-    j->pic_class.resize(b->num_pictures);
-    j->pic_local.resize(b->num_pictures);
-    for (uint32_t p = 0; p < b->num_pictures; p++) {
-        const hpvcd_picture& P = b->pictures[p];
+    j->pic_class.resize(b.num_pictures);
+    j->pic_local.resize(b.num_pictures);
+    for (uint32_t p = 0; p < b.num_pictures; p++) {
+        const hpvcd_picture& P = b.pictures[p];
         bool s16 = P.bit_depth_luma > 8;
         bool r32 = s16 || (P.flags & HPVCD_PF_EXTENDED_PRECISION) || P.chroma_idc == 3;
         int ci = -1;
@@ -1194,13 +1330,13 @@ hpvcd_status job_create_impl(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out
     // reference depth of every picture (references come earlier in the
     // batch and share its format class); each class runs its pictures in
     // order of depth, one wave per depth
-    std::vector<uint32_t> depth(b->num_pictures, 0);
-    std::vector<uint8_t> is_ref(b->num_pictures, 0);
-    for (uint32_t p = 0; p < b->num_pictures; p++) {
-        const hpvcd_picture& P = b->pictures[p];
+    std::vector<uint32_t> depth(b.num_pictures, 0);
+    std::vector<uint8_t> is_ref(b.num_pictures, 0);
+    for (uint32_t p = 0; p < b.num_pictures; p++) {
+        const hpvcd_picture& P = b.pictures[p];
         if (!P.pu_count) continue;
         for (uint32_t k = 0; k < P.ref_count; k++) {
-            uint32_t r = b->refs[P.ref_begin + k].picture;
+            uint32_t r = b.refs[P.ref_begin + k].picture;
             if (r == HPVCD_REF_MISSING) continue;
             is_ref[r] = 1;
             depth[p] = std::max(depth[p], depth[r] + 1);
@@ -1216,10 +1352,40 @@ hpvcd_status job_create_impl(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out
         });
         for (uint32_t i = 0; i < C.batch_pics.size(); i++) j->pic_local[C.batch_pics[i]] = i;
     }
+    hpvcd_status st = HPVCD_OK;
     for (auto& C : j->classes) {
-        st = prepare_class(*b, C, j->out_off, depth, j->pic_local, is_ref);
-        if (st != HPVCD_OK) { delete j; return st; }
+        st = prepare_class(b, C, j->out_off, depth, j->pic_local, is_ref, threads);
+        if (st != HPVCD_OK) return st;
     }
+    return HPVCD_OK;
+}
+
+// host worker threads for batch preparation
+int host_threads() {
+    static const int n = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    return n;
+}
+
+// Job creation into a pool of the library's own (ext_base null: hipMalloc,
+// pinned staging) or into an existing device region of ext_size bytes
+// (staged from ordinary host memory, nothing allocated on the device).
+hpvcd_status job_create_impl(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out, uint8_t* ext_base,
+                             size_t ext_size) {
+    if (!g || !b || !out) return set_error(HPVCD_E_INVALID, "null argument");
+    *out = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    hpvcd_status st = validate_batch(*b);
+    if (st != HPVCD_OK) return st;
+    HIP_TRY(hipSetDevice(g->device));
+    hpvcd_job* j = new (std::nothrow) hpvcd_job();
+    if (!j) return set_error(HPVCD_E_ALLOC, "out of memory");
+    j->g = g;
+    st = job_prepare(*b, j, host_threads());
+    if (st != HPVCD_OK) {
+        delete j;
+        return st;
+    }
+    const size_t ob = j->out_bytes;
     size_t sc_bytes = (size_t)b->num_scaling * HPVCD_SCALING_SET_BYTES;
     // Device layout: first every array uploaded from the host (one
     // contiguous region, copied once from pinned staging), then the pools
@@ -1541,5 +1707,47 @@ hpvcd_status hpvcd_gpu_decode_batch_noalloc(hpvcd_gpu* g, const hpvcd_batch* b, 
 }
 
 size_t hpvcd_noalloc_scratch_bytes(void) { return kScratchBytes; }
+
+hpvcd_status hpvcd_prep_digest(const hpvcd_batch* b, int32_t threads, uint64_t* digest, double* ms) {
+    if (!b || !digest) return set_error(HPVCD_E_INVALID, "null argument");
+    hpvcd_status st = validate_batch(*b);
+    if (st) return st;
+    hpvcd_job* j = new (std::nothrow) hpvcd_job();
+    if (!j) return set_error(HPVCD_E_ALLOC, "out of memory");
+    const auto t0 = std::chrono::steady_clock::now();
+    st = job_prepare(*b, j, threads > 0 ? threads : host_threads());
+    if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    uint64_t h = 1469598103934665603ull;  // FNV-1a over every prepared array
+    auto mix = [&](const void* p, size_t n) {
+        const uint8_t* c = static_cast<const uint8_t*>(p);
+        for (size_t i = 0; i < n; i++) h = (h ^ c[i]) * 1099511628211ull;
+    };
+    auto vec = [&](const auto& v) {
+        const uint64_t n = v.size();
+        mix(&n, sizeof(n));
+        if (n) mix(v.data(), n * sizeof(v[0]));
+    };
+    if (!st)
+        for (const ClassJob& C : j->classes) {
+            vec(C.batch_pics); vec(C.pics); vec(C.tbs); vec(C.tb_pic); vec(C.coeffs); vec(C.cells); vec(C.order);
+            vec(C.sao); vec(C.units); vec(C.tickets); vec(C.plain_tickets); vec(C.unit_tbs); vec(C.unit_recs);
+            vec(C.prep_list); vec(C.plain_prep); vec(C.pic_prep_begin); vec(C.level_off); vec(C.unit_refoff);
+            for (int pass = 0; pass < 2; pass++)
+                for (int sz = 0; sz < 4; sz++) {
+                    vec(C.res_list[pass][sz]);
+                    vec(C.res_jobs[pass][sz]);
+                    vec(C.pic_res_begin[pass][sz]);
+                }
+            vec(C.pic_units_begin); vec(C.act_list); vec(C.pic_act_begin); vec(C.pus); vec(C.mc_tasks);
+            vec(C.pic_mc_begin); vec(C.wave_begin); vec(C.wave_tick_begin); vec(C.pic_is_ref);
+            const uint64_t sc[14] = {C.smap_words, C.rmap_words, C.plane_elems, C.res_elems, C.max_luma_v,
+                                     C.max_luma_h, C.max_chroma_v, C.max_chroma_h, C.max_plane, C.max_copy,
+                                     C.max_ctbs, C.max_rgb_qx, C.max_rgb_rows, C.sync_words};
+            mix(sc, sizeof(sc));
+        }
+    delete j;
+    *digest = h;
+    return st;
+}
 
 }  // extern "C"

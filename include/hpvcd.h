@@ -226,6 +226,12 @@ hpvcd_status hpvcd_gpu_decode_batch(hpvcd_gpu* g, const hpvcd_batch* b, void** o
 hpvcd_status hpvcd_gpu_decode_batch_noalloc(hpvcd_gpu* g, const hpvcd_batch* b, void** outputs);
 size_t hpvcd_noalloc_scratch_bytes(void);
 
+/* Host half of hpvcd_job_create on its own (no GPU needed): validation and
+ * preparation of a batch on `threads` host threads (0 = up to 16), the
+ * milliseconds it took, and a digest of every prepared array (equal for
+ * every thread count). */
+hpvcd_status hpvcd_prep_digest(const hpvcd_batch* b, int32_t threads, uint64_t* digest, double* ms);
+
 /* ------------------------------------------------------------------------ */
 /* Drop-in decoder: the shape of the reference's public API                 */
 /* (src/decoder.rs:54-234 Decoder::{from_settings, decode, decode_rgb8},    */

@@ -360,3 +360,25 @@ def test_decoder_settings_defaults_and_backends():
     assert N.lib().hpvcd_decoder_new(C.byref(s), C.byref(h)) == 11 and not h.value   # E_UNSUPPORTED
     s.backend = 7
     assert N.lib().hpvcd_decoder_new(C.byref(s), C.byref(h)) == 15 and not h.value   # E_INVALID
+
+
+def test_batch_preparation_is_the_same_on_every_thread_count():
+    """Job preparation (availability stamps, wavefront units, dependency
+    levels, reference-map layout, residual and ticket lists) runs per picture
+    on host threads and merges in picture order: its digest does not depend
+    on the thread count."""
+    import ctypes as C
+
+    sets = [H.RecordSet.generate("grid4032", grid_cols=3, grid_rows=2, out_width=1536, out_height=1024),
+            H.RecordSet.generate("rext4096", grid_cols=2, grid_rows=1, out_width=1024, out_height=512, act=1,
+                                 p_act=0.5, p_tqb=0.2),
+            H.RecordSet.generate("video1080", tile_width=256, tile_height=128, grid_rows=5, num_outputs=5,
+                                 out_width=256, out_height=128)]
+    sets.append(H.RecordSet.concat(sets))
+    for rs in sets:
+        digests = set()
+        for th in (1, 2, 5, 16):
+            d, ms = C.c_uint64(), C.c_double()
+            assert N.lib().hpvcd_prep_digest(C.byref(rs.batch), th, C.byref(d), C.byref(ms)) == 0
+            digests.add(d.value)
+        assert len(digests) == 1
