@@ -154,6 +154,40 @@ def end_to_end(H, N, g, rs, iters):
     return out
 
 
+def heic_end_to_end(H, N, workload, iters, want=None):
+    """HEIC bytes to pixels through the drop-in decoder (hpvcd_decode_heic):
+    container and CABAC parse on the host threads, batch preparation, one
+    pinned host-to-device copy, the batched launches, the download into the
+    library-owned page-locked image.  Also the parse alone and the records
+    half alone."""
+    import numpy as np
+
+    f = H.gen_heic(workload)
+    tot, parse, rec = [], [], []
+    img = None
+    with H.Decoder() as d:
+        for _ in range(iters + 1):
+            t0 = time.perf_counter()
+            img = d.decode(f)
+            tot.append((time.perf_counter() - t0) * 1e3)
+            t0 = time.perf_counter()
+            rs = H.RecordSet.parse_heic(f)
+            t1 = time.perf_counter()
+            d.decode_records(rs)
+            t2 = time.perf_counter()
+            parse.append((t1 - t0) * 1e3)
+            rec.append((t2 - t1) * 1e3)
+    sl = slice(1, None)
+    med = lambda v: float(np.median(v[sl]))
+    out = {"bytes": len(f), "ms_per_image": round(med(tot), 3), "parse_ms": round(med(parse), 3),
+           "records_to_image_ms": round(med(rec), 3), "p50_ms": round(float(np.percentile(tot[sl], 50)), 3),
+           "p99_ms": round(float(np.percentile(tot[sl], 99)), 3),
+           "host_threads": os.cpu_count()}
+    if want is not None:
+        out["bit_exact"] = bool(np.array_equal(img.reshape(-1), want.reshape(-1)))
+    return out
+
+
 def config_line(H, N, g, name, steps):
     """Device throughput of another workload (graph replay of a resident job)."""
     rs = H.RecordSet.generate(name)
@@ -245,6 +279,11 @@ def main() -> int:
         print(f"[bench] events {ev_ms:.3f} ms/step, kernels {kms}, plain {plain_ms}", file=sys.stderr, flush=True)
         e2e = end_to_end(H, N, g, rs, 5)
         print(f"[bench] end to end {e2e}", file=sys.stderr, flush=True)
+    heic = None
+    if H.gen_config(**H.WORKLOADS[args.workload]).inter == 0:
+        heic = heic_end_to_end(H, N, args.workload, 5, out_gpu[0])
+        print(f"[bench] HEIC bytes -> image {heic}", file=sys.stderr, flush=True)
+    with H.Gpu(0) as g:
         if not args.no_configs:
             for name in ("single512", "rext4096", "video1080"):
                 if name != args.workload:
@@ -330,6 +369,10 @@ def main() -> int:
                        "kind": "replay of the resident job"},
         "e2e": {**e2e, "value": round(mpix / (e2e["ms_per_step"] / 1e3), 2), "unit": "Mpixel/s",
                 "kind": "host records -> job creation (prep + pinned H2D) -> batched launch -> D2H to pinned"},
+        "heic": None if heic is None else {
+            **heic, "value": round(mpix / (heic["ms_per_image"] / 1e3), 2), "unit": "Mpixel/s",
+            "kind": "HEIC file bytes -> hpvcd_decode_heic -> RGB image in host memory (container and CABAC parse "
+                    "on the host threads, preparation, pinned H2D, batched GPU launches, D2H)"},
         "plain_per_tile": None if plain_ms is None else {
             "ms_per_step": round(plain_ms, 4), "value": round(mpix / (plain_ms / 1e3), 2),
             "batched_speedup": round(plain_ms / wall_ms, 2)},
@@ -354,6 +397,11 @@ def main() -> int:
     }
     if cpu:
         line["vs_cpu"] = round(value / cpu["value"], 1)
+        if heic is not None:
+            # the same host parse in front of the CPU reconstruction
+            cpu_e2e_ms = heic["parse_ms"] + 1e3 * mpix / cpu["value"]
+            line["cpu_baseline"]["e2e_value"] = round(mpix / (cpu_e2e_ms / 1e3), 2)
+            line["cpu_baseline"]["e2e_kind"] = "host parse (as timed above) + CPU reconstruction"
     print(json.dumps(line), flush=True)
     return 0
 

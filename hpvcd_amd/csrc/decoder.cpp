@@ -129,7 +129,7 @@ hpvcd_status hpvcd_decode_records(hpvcd_decoder* d, const hpvcd_batch* b, hpvcd_
         return HPVCD_OK;
     }
     hpvcd_job* j = nullptr;
-    if ((st = hpvcd_job_create(d->g, b, &j))) {
+    if ((st = job_create_validated(d->g, b, &j))) {
         for (uint32_t o = 0; o < b->num_outputs; o++) hpvcd_image_free(&images[o]);
         return st;
     }

@@ -31,5 +31,7 @@ inline int chroma_h(const hpvcd_picture& p) { return p.chroma_idc ? (p.height + 
 
 // Validates ranges, sizes and cross references of a batch (host side).
 hpvcd_status validate_batch(const hpvcd_batch& b);
+// hpvcd_job_create for a batch the caller has validated already
+hpvcd_status job_create_validated(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out);
 
 }  // namespace hpvcd

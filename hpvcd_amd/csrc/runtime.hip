@@ -107,6 +107,105 @@ hpvcd_status validate_inter(const hpvcd_batch& b, uint32_t p) {
     return HPVCD_OK;
 }
 
+// The checks of one picture's records (ranges, cells, TBs, levels, inter
+// records); independent of the other pictures.
+hpvcd_status validate_picture(const hpvcd_batch& b, uint32_t p) {
+    const hpvcd_picture& P = b.pictures[p];
+    if (P.width <= 0 || P.height <= 0 || P.width > 16384 || P.height > 16384 || (P.width & 7) || (P.height & 7))
+        return set_error(HPVCD_E_BAD_DIMENSIONS, "picture size must be a positive multiple of 8");
+    if (P.chroma_idc > 3) return set_error(HPVCD_E_UNSUPPORTED_CHROMA, "chroma_idc");
+    // 4:2:2 (two stacked chroma TBs per luma TB, the remapped chroma
+    // modes, chroma CTBs of ctb/2 x ctb) has no record producer and no
+    // parity evidence yet, so the back end refuses it
+    if (P.chroma_idc == 2) return set_error(HPVCD_E_UNSUPPORTED_CHROMA, "4:2:2 pictures are not supported");
+    if (P.bit_depth_luma < 8 || P.bit_depth_luma > 12 || P.bit_depth_chroma < 8 || P.bit_depth_chroma > 12)
+        return set_error(HPVCD_E_UNSUPPORTED_BITDEPTH, "bit depth must be 8..12");
+    if ((P.bit_depth_luma > 8) != (P.bit_depth_chroma > 8))
+        return set_error(HPVCD_E_UNSUPPORTED_BITDEPTH, "luma and chroma storage classes differ");
+    if (P.log2_ctb < 4 || P.log2_ctb > 6) return set_error(HPVCD_E_PARAMSET, "log2_ctb must be 4..6");
+    if (P.num_slices > HPVCD_MAX_SLICES) return set_error(HPVCD_E_INVALID, "num_slices");
+    if ((uint64_t)P.tb_begin + P.tb_count > b.num_tbs) return set_error(HPVCD_E_INVALID, "tb range");
+    if ((uint64_t)P.coeff_begin + P.coeff_count > b.num_coeffs) return set_error(HPVCD_E_INVALID, "coeff range");
+    int ctb = 1 << P.log2_ctb;
+    uint64_t nctb = (uint64_t)((P.width + ctb - 1) / ctb) * ((P.height + ctb - 1) / ctb);
+    if ((uint64_t)P.ctb_begin + nctb > b.num_ctbs) return set_error(HPVCD_E_INVALID, "sao range");
+    uint64_t ncell = (uint64_t)((P.width + 3) / 4) * ((P.height + 3) / 4);
+    if ((uint64_t)P.cell_begin + ncell > b.num_cells) return set_error(HPVCD_E_INVALID, "cell range");
+    // the loop filters index slices[] by the cell's slice
+    const uint32_t nsl = P.num_slices ? P.num_slices : 1u;
+    for (uint64_t g = 0; g < ncell; g++)
+        if (HPVCD_CELL_SLICE(b.cell_info[P.cell_begin + g]) >= nsl)
+            return set_error(HPVCD_E_INVALID, "cell slice index beyond num_slices");
+    if (P.scaling_index >= 0 && (uint32_t)P.scaling_index >= b.num_scaling)
+        return set_error(HPVCD_E_INVALID, "scaling_index");
+    if (P.out_index >= b.num_outputs) return set_error(HPVCD_E_INVALID, "out_index");
+    const hpvcd_output& O = b.outputs[P.out_index];
+    if (P.copy_w < 0 || P.copy_h < 0 || P.copy_w > P.win_w || P.copy_h > P.win_h || P.dst_x < 0 || P.dst_y < 0 ||
+        (int64_t)P.dst_x + P.copy_w > O.width || (int64_t)P.dst_y + P.copy_h > O.height)
+        return set_error(HPVCD_E_INVALID, "picture window placement outside its output image");
+    const bool luma_out = O.kind == HPVCD_OUT_LUMA8 || O.kind == HPVCD_OUT_LUMA16;
+    if (luma_out && P.chroma_idc != 0) return set_error(HPVCD_E_INVALID, "luma output of a colour picture");
+    if (O.channels != (luma_out ? 1 : 3)) return set_error(HPVCD_E_INVALID, "output channels");
+    bool wide = O.kind == HPVCD_OUT_RGB16 || O.kind == HPVCD_OUT_LUMA16;
+    if (wide && P.bit_depth_luma == 8) return set_error(HPVCD_E_INVALID, "16-bit output of an 8-bit picture");
+    int sw = sub_w_of(P.chroma_idc), sh = sub_h_of(P.chroma_idc);
+    int cw = chroma_w(P), ch = chroma_h(P);
+    int64_t last_ctb[3] = {-1, -1, -1};
+    int ctb_cols = (P.width + ctb - 1) / ctb;
+    for (uint32_t t = 0; t < P.tb_count; t++) {
+        const hpvcd_tb& T = b.tbs[P.tb_begin + t];
+        if (T.log2n < 2 || T.log2n > 5) return set_error(HPVCD_E_INVALID, "tb log2 size");
+        if (T.comp <= 2) {
+            int lx = T.comp ? T.x * sub_w_of(P.chroma_idc) : T.x, ly = T.comp ? T.y * sub_h_of(P.chroma_idc) : T.y;
+            int64_t r = (int64_t)(ly >> P.log2_ctb) * ctb_cols + (lx >> P.log2_ctb);
+            if (r < last_ctb[T.comp]) return set_error(HPVCD_E_INVALID, "tbs not in CTB raster order");
+            last_ctb[T.comp] = r;
+        }
+        if (T.comp > 2 || (T.comp && !P.chroma_idc)) return set_error(HPVCD_E_INVALID, "tb component");
+        int n = 1 << T.log2n;
+        int pw = T.comp ? cw : P.width, ph = T.comp ? ch : P.height;
+        int ctbw = T.comp ? ctb / sw : ctb, ctbh = T.comp ? ctb / sh : ctb;
+        if (n > ctbw || n > ctbh || (T.x % ctbw) + n > ctbw || (T.y % ctbh) + n > ctbh)
+            return set_error(HPVCD_E_INVALID, "tb crosses a CTB boundary");
+        if (T.x >= pw || T.y >= ph) return set_error(HPVCD_E_INVALID, "tb outside its plane");
+        if ((T.x & 3) || (T.y & 3)) return set_error(HPVCD_E_INVALID, "tb not 4-aligned");
+        if (T.mode > 34) return set_error(HPVCD_E_INVALID, "intra mode");
+        if (T.res_scale && (T.comp == 0 || P.chroma_idc != 3)) return set_error(HPVCD_E_INVALID, "res_scale");
+        const bool pcm = (T.flags & HPVCD_TB_PCM) != 0;
+        const int bd = T.comp ? P.bit_depth_chroma : P.bit_depth_luma;
+        if (pcm) {
+            if (T.flags & (HPVCD_TB_TS | HPVCD_TB_TQB | HPVCD_TB_INTER | HPVCD_TB_RDPCM_H | HPVCD_TB_RDPCM_V |
+                           HPVCD_TB_ACT) || T.res_scale)
+                return set_error(HPVCD_E_INVALID, "PCM TB with residual-coding flags");
+            if (T.qp >= bd) return set_error(HPVCD_E_INVALID, "PCM sample shift must be below the bit depth");
+        }
+        if (T.flags & HPVCD_TB_ACT) {
+            // a luma TB opens the triple; its Cb and Cr TBs follow at once
+            if (P.chroma_idc != 3 || T.comp > 2 || t < T.comp || t - T.comp + 2 >= P.tb_count)
+                return set_error(HPVCD_E_INVALID, "ACT TB outside a 4:4:4 luma/Cb/Cr triple");
+            const uint32_t t0 = t - T.comp;
+            for (int k = 0; k <= 2; k++) {
+                const hpvcd_tb& U = b.tbs[P.tb_begin + t0 + k];
+                if (!(U.flags & HPVCD_TB_ACT) || U.comp != k || U.x != T.x || U.y != T.y || U.log2n != T.log2n)
+                    return set_error(HPVCD_E_INVALID, "ACT triple not co-located luma, Cb, Cr of one size");
+            }
+        }
+        if (T.flags & (HPVCD_TB_CBF | HPVCD_TB_PCM)) {
+            if ((uint64_t)T.coeff_off + T.coeff_cnt > P.coeff_count) return set_error(HPVCD_E_INVALID, "tb coeff range");
+            if (T.coeff_cnt > (uint32_t)(n * n)) return set_error(HPVCD_E_INVALID, "more coefficient words than TB positions");
+            for (uint32_t k = 0; k < T.coeff_cnt; k++) {
+                uint32_t w = b.coeffs[P.coeff_begin + T.coeff_off + k];
+                if (HPVCD_COEFF_POS(w) >= (uint32_t)(n * n)) return set_error(HPVCD_E_INVALID, "coefficient position");
+                if (pcm && (HPVCD_COEFF_LEVEL(w) < 0 || HPVCD_COEFF_LEVEL(w) >= (1 << (bd - T.qp))))
+                    return set_error(HPVCD_E_INVALID, "PCM sample outside PcmBitDepth");
+            }
+        }
+    }
+    hpvcd_status st = validate_inter(b, p);
+    if (st != HPVCD_OK) return st;
+    return HPVCD_OK;
+}
+
 hpvcd_status validate_batch(const hpvcd_batch& b) {
     if (b.num_pictures && !b.pictures) return set_error(HPVCD_E_INVALID, "pictures is null");
     if (b.num_tbs && !b.tbs) return set_error(HPVCD_E_INVALID, "tbs is null");
@@ -124,101 +223,26 @@ hpvcd_status validate_batch(const hpvcd_batch& b) {
         if (O.kind > HPVCD_OUT_LUMA16) return set_error(HPVCD_E_INVALID, "output kind");
         if (O.orientation > HPVCD_ORIENT_ROTATE_270) return set_error(HPVCD_E_INVALID, "output orientation");
     }
-    for (uint32_t p = 0; p < b.num_pictures; p++) {
-        const hpvcd_picture& P = b.pictures[p];
-        if (P.width <= 0 || P.height <= 0 || P.width > 16384 || P.height > 16384 || (P.width & 7) || (P.height & 7))
-            return set_error(HPVCD_E_BAD_DIMENSIONS, "picture size must be a positive multiple of 8");
-        if (P.chroma_idc > 3) return set_error(HPVCD_E_UNSUPPORTED_CHROMA, "chroma_idc");
-        // 4:2:2 (two stacked chroma TBs per luma TB, the remapped chroma
-        // modes, chroma CTBs of ctb/2 x ctb) has no record producer and no
-        // parity evidence yet, so the back end refuses it
-        if (P.chroma_idc == 2) return set_error(HPVCD_E_UNSUPPORTED_CHROMA, "4:2:2 pictures are not supported");
-        if (P.bit_depth_luma < 8 || P.bit_depth_luma > 12 || P.bit_depth_chroma < 8 || P.bit_depth_chroma > 12)
-            return set_error(HPVCD_E_UNSUPPORTED_BITDEPTH, "bit depth must be 8..12");
-        if ((P.bit_depth_luma > 8) != (P.bit_depth_chroma > 8))
-            return set_error(HPVCD_E_UNSUPPORTED_BITDEPTH, "luma and chroma storage classes differ");
-        if (P.log2_ctb < 4 || P.log2_ctb > 6) return set_error(HPVCD_E_PARAMSET, "log2_ctb must be 4..6");
-        if (P.num_slices > HPVCD_MAX_SLICES) return set_error(HPVCD_E_INVALID, "num_slices");
-        if ((uint64_t)P.tb_begin + P.tb_count > b.num_tbs) return set_error(HPVCD_E_INVALID, "tb range");
-        if ((uint64_t)P.coeff_begin + P.coeff_count > b.num_coeffs) return set_error(HPVCD_E_INVALID, "coeff range");
-        int ctb = 1 << P.log2_ctb;
-        uint64_t nctb = (uint64_t)((P.width + ctb - 1) / ctb) * ((P.height + ctb - 1) / ctb);
-        if ((uint64_t)P.ctb_begin + nctb > b.num_ctbs) return set_error(HPVCD_E_INVALID, "sao range");
-        uint64_t ncell = (uint64_t)((P.width + 3) / 4) * ((P.height + 3) / 4);
-        if ((uint64_t)P.cell_begin + ncell > b.num_cells) return set_error(HPVCD_E_INVALID, "cell range");
-        // the loop filters index slices[] by the cell's slice
-        const uint32_t nsl = P.num_slices ? P.num_slices : 1u;
-        for (uint64_t g = 0; g < ncell; g++)
-            if (HPVCD_CELL_SLICE(b.cell_info[P.cell_begin + g]) >= nsl)
-                return set_error(HPVCD_E_INVALID, "cell slice index beyond num_slices");
-        if (P.scaling_index >= 0 && (uint32_t)P.scaling_index >= b.num_scaling)
-            return set_error(HPVCD_E_INVALID, "scaling_index");
-        if (P.out_index >= b.num_outputs) return set_error(HPVCD_E_INVALID, "out_index");
-        const hpvcd_output& O = b.outputs[P.out_index];
-        if (P.copy_w < 0 || P.copy_h < 0 || P.copy_w > P.win_w || P.copy_h > P.win_h || P.dst_x < 0 || P.dst_y < 0 ||
-            (int64_t)P.dst_x + P.copy_w > O.width || (int64_t)P.dst_y + P.copy_h > O.height)
-            return set_error(HPVCD_E_INVALID, "picture window placement outside its output image");
-        const bool luma_out = O.kind == HPVCD_OUT_LUMA8 || O.kind == HPVCD_OUT_LUMA16;
-        if (luma_out && P.chroma_idc != 0) return set_error(HPVCD_E_INVALID, "luma output of a colour picture");
-        if (O.channels != (luma_out ? 1 : 3)) return set_error(HPVCD_E_INVALID, "output channels");
-        bool wide = O.kind == HPVCD_OUT_RGB16 || O.kind == HPVCD_OUT_LUMA16;
-        if (wide && P.bit_depth_luma == 8) return set_error(HPVCD_E_INVALID, "16-bit output of an 8-bit picture");
-        int sw = sub_w_of(P.chroma_idc), sh = sub_h_of(P.chroma_idc);
-        int cw = chroma_w(P), ch = chroma_h(P);
-        int64_t last_ctb[3] = {-1, -1, -1};
-        int ctb_cols = (P.width + ctb - 1) / ctb;
-        for (uint32_t t = 0; t < P.tb_count; t++) {
-            const hpvcd_tb& T = b.tbs[P.tb_begin + t];
-            if (T.log2n < 2 || T.log2n > 5) return set_error(HPVCD_E_INVALID, "tb log2 size");
-            if (T.comp <= 2) {
-                int lx = T.comp ? T.x * sub_w_of(P.chroma_idc) : T.x, ly = T.comp ? T.y * sub_h_of(P.chroma_idc) : T.y;
-                int64_t r = (int64_t)(ly >> P.log2_ctb) * ctb_cols + (lx >> P.log2_ctb);
-                if (r < last_ctb[T.comp]) return set_error(HPVCD_E_INVALID, "tbs not in CTB raster order");
-                last_ctb[T.comp] = r;
-            }
-            if (T.comp > 2 || (T.comp && !P.chroma_idc)) return set_error(HPVCD_E_INVALID, "tb component");
-            int n = 1 << T.log2n;
-            int pw = T.comp ? cw : P.width, ph = T.comp ? ch : P.height;
-            int ctbw = T.comp ? ctb / sw : ctb, ctbh = T.comp ? ctb / sh : ctb;
-            if (n > ctbw || n > ctbh || (T.x % ctbw) + n > ctbw || (T.y % ctbh) + n > ctbh)
-                return set_error(HPVCD_E_INVALID, "tb crosses a CTB boundary");
-            if (T.x >= pw || T.y >= ph) return set_error(HPVCD_E_INVALID, "tb outside its plane");
-            if ((T.x & 3) || (T.y & 3)) return set_error(HPVCD_E_INVALID, "tb not 4-aligned");
-            if (T.mode > 34) return set_error(HPVCD_E_INVALID, "intra mode");
-            if (T.res_scale && (T.comp == 0 || P.chroma_idc != 3)) return set_error(HPVCD_E_INVALID, "res_scale");
-            const bool pcm = (T.flags & HPVCD_TB_PCM) != 0;
-            const int bd = T.comp ? P.bit_depth_chroma : P.bit_depth_luma;
-            if (pcm) {
-                if (T.flags & (HPVCD_TB_TS | HPVCD_TB_TQB | HPVCD_TB_INTER | HPVCD_TB_RDPCM_H | HPVCD_TB_RDPCM_V |
-                               HPVCD_TB_ACT) || T.res_scale)
-                    return set_error(HPVCD_E_INVALID, "PCM TB with residual-coding flags");
-                if (T.qp >= bd) return set_error(HPVCD_E_INVALID, "PCM sample shift must be below the bit depth");
-            }
-            if (T.flags & HPVCD_TB_ACT) {
-                // a luma TB opens the triple; its Cb and Cr TBs follow at once
-                if (P.chroma_idc != 3 || T.comp > 2 || t < T.comp || t - T.comp + 2 >= P.tb_count)
-                    return set_error(HPVCD_E_INVALID, "ACT TB outside a 4:4:4 luma/Cb/Cr triple");
-                const uint32_t t0 = t - T.comp;
-                for (int k = 0; k <= 2; k++) {
-                    const hpvcd_tb& U = b.tbs[P.tb_begin + t0 + k];
-                    if (!(U.flags & HPVCD_TB_ACT) || U.comp != k || U.x != T.x || U.y != T.y || U.log2n != T.log2n)
-                        return set_error(HPVCD_E_INVALID, "ACT triple not co-located luma, Cb, Cr of one size");
-                }
-            }
-            if (T.flags & (HPVCD_TB_CBF | HPVCD_TB_PCM)) {
-                if ((uint64_t)T.coeff_off + T.coeff_cnt > P.coeff_count) return set_error(HPVCD_E_INVALID, "tb coeff range");
-                if (T.coeff_cnt > (uint32_t)(n * n)) return set_error(HPVCD_E_INVALID, "more coefficient words than TB positions");
-                for (uint32_t k = 0; k < T.coeff_cnt; k++) {
-                    uint32_t w = b.coeffs[P.coeff_begin + T.coeff_off + k];
-                    if (HPVCD_COEFF_POS(w) >= (uint32_t)(n * n)) return set_error(HPVCD_E_INVALID, "coefficient position");
-                    if (pcm && (HPVCD_COEFF_LEVEL(w) < 0 || HPVCD_COEFF_LEVEL(w) >= (1 << (bd - T.qp))))
-                        return set_error(HPVCD_E_INVALID, "PCM sample outside PcmBitDepth");
-                }
-            }
+    // pictures are checked on the host threads; the first failing picture
+    // (in batch order) reports
+    const uint32_t np = b.num_pictures;
+    std::vector<hpvcd_status> st(np, HPVCD_OK);
+    std::vector<std::string> msg(np);
+    std::atomic<uint32_t> next{0};
+    auto work = [&]() {
+        for (uint32_t p; (p = next.fetch_add(1)) < np;) {
+            st[p] = validate_picture(b, p);
+            if (st[p]) msg[p] = g_last_error;
         }
-        hpvcd_status st = validate_inter(b, p);
-        if (st != HPVCD_OK) return st;
-    }
+    };
+    const int nth = std::max(1, std::min<int>((int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())),
+                                              (int)np));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    for (uint32_t p = 0; p < np; p++)
+        if (st[p]) return set_error(st[p], msg[p].c_str());
     return HPVCD_OK;
 }
 
@@ -1162,6 +1186,57 @@ hpvcd_status check_err(hpvcd_job* j) {
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+namespace {
+
+// Page-locked blocks handed out by hpvcd_host_alloc, and the freed ones kept
+// for reuse (decoded images come and go at the same sizes; hipHostMalloc of
+// tens of MiB costs milliseconds): at most kPinnedKeep blocks and
+// kPinnedKeepBytes, released when the process ends.
+constexpr size_t kPinnedKeep = 8;
+constexpr size_t kPinnedKeepBytes = (size_t)1 << 30;
+std::mutex g_pinned_mu;
+std::vector<std::pair<void*, size_t>> g_pinned_live, g_pinned_free;
+
+void pinned_note(void* p, size_t n) {
+    std::lock_guard<std::mutex> lock(g_pinned_mu);
+    g_pinned_live.push_back({p, n});
+}
+
+// a kept block of at least n bytes and at most twice that, smallest first
+void* pinned_take(size_t n) {
+    std::lock_guard<std::mutex> lock(g_pinned_mu);
+    size_t best = SIZE_MAX;
+    for (size_t i = 0; i < g_pinned_free.size(); i++)
+        if (g_pinned_free[i].second >= n && g_pinned_free[i].second <= 2 * n &&
+            (best == SIZE_MAX || g_pinned_free[i].second < g_pinned_free[best].second))
+            best = i;
+    if (best == SIZE_MAX) return nullptr;
+    auto blk = g_pinned_free[best];
+    g_pinned_free.erase(g_pinned_free.begin() + best);
+    g_pinned_live.push_back(blk);
+    return blk.first;
+}
+
+// true when p is kept for reuse (false: the caller frees it)
+bool pinned_keep(void* p) {
+    std::lock_guard<std::mutex> lock(g_pinned_mu);
+    for (size_t i = 0; i < g_pinned_live.size(); i++)
+        if (g_pinned_live[i].first == p) {
+            auto blk = g_pinned_live[i];
+            g_pinned_live.erase(g_pinned_live.begin() + i);
+            size_t kept = blk.second;
+            for (auto& f : g_pinned_free) kept += f.second;
+            if (g_pinned_free.size() < kPinnedKeep && kept <= kPinnedKeepBytes) {
+                g_pinned_free.push_back(blk);
+                return true;
+            }
+            return false;
+        }
+    return false;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* hpvcd_last_error(void) { return g_last_error.c_str(); }
@@ -1370,11 +1445,11 @@ int host_threads() {
 // pinned staging) or into an existing device region of ext_size bytes
 // (staged from ordinary host memory, nothing allocated on the device).
 hpvcd_status job_create_impl(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out, uint8_t* ext_base,
-                             size_t ext_size) {
+                             size_t ext_size, bool validate) {
     if (!g || !b || !out) return set_error(HPVCD_E_INVALID, "null argument");
     *out = nullptr;
     const auto t0 = std::chrono::steady_clock::now();
-    hpvcd_status st = validate_batch(*b);
+    hpvcd_status st = validate ? validate_batch(*b) : HPVCD_OK;
     if (st != HPVCD_OK) return st;
     HIP_TRY(hipSetDevice(g->device));
     hpvcd_job* j = new (std::nothrow) hpvcd_job();
@@ -1525,8 +1600,18 @@ hpvcd_status job_create_impl(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out
 extern "C" {
 
 hpvcd_status hpvcd_job_create(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out) {
-    return job_create_impl(g, b, out, nullptr, 0);
+    return job_create_impl(g, b, out, nullptr, 0, true);
 }
+
+}  // extern "C"
+
+namespace hpvcd {
+hpvcd_status job_create_validated(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out) {
+    return job_create_impl(g, b, out, nullptr, 0, false);
+}
+}  // namespace hpvcd
+
+extern "C" {
 
 size_t hpvcd_job_device_bytes(const hpvcd_job* j) { return j ? j->mem.size : 0; }
 
@@ -1540,15 +1625,17 @@ hpvcd_status hpvcd_job_host_ms(const hpvcd_job* j, double* prep_ms, double* uplo
 hpvcd_status hpvcd_host_alloc(size_t bytes, void** out) {
     if (!out) return set_error(HPVCD_E_INVALID, "null argument");
     *out = nullptr;
+    if ((*out = pinned_take(bytes ? bytes : 1))) return HPVCD_OK;
     if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
         *out = nullptr;
         return set_error(HPVCD_E_ALLOC, "hipHostMalloc failed");
     }
+    pinned_note(*out, bytes ? bytes : 1);
     return HPVCD_OK;
 }
 
 void hpvcd_host_free(void* p) {
-    if (p) (void)hipHostFree(p);
+    if (p && !pinned_keep(p)) (void)hipHostFree(p);
 }
 
 hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop) {
@@ -1698,7 +1785,7 @@ hpvcd_status hpvcd_gpu_decode_batch_noalloc(hpvcd_gpu* g, const hpvcd_batch* b, 
     HIP_TRY(hipSetDevice(g->device));
     std::lock_guard<std::mutex> lock(g_scratch_mu[g->device]);
     hpvcd_job* j = nullptr;
-    hpvcd_status st = job_create_impl(g, b, &j, g->scratch, kScratchBytes);
+    hpvcd_status st = job_create_impl(g, b, &j, g->scratch, kScratchBytes, true);
     if (st) return st;
     st = hpvcd_job_run(j, HPVCD_MODE_BATCHED, HPVCD_STAGE_RGB);
     if (!st) st = hpvcd_job_download(j, outputs);
