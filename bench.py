@@ -188,6 +188,84 @@ def heic_end_to_end(H, N, workload, iters, want=None):
     return out
 
 
+SERVING_MIX = (
+    ("256x256 8-bit single", dict(tile_width=256, tile_height=256, out_width=256, out_height=256), "single512"),
+    ("1024x768 2x2 grid", dict(grid_cols=2, grid_rows=2, out_width=1024, out_height=768), "grid4032"),
+    ("4032x3024 8x6 grid", {}, "grid4032"),
+    ("2048x2048 4x4 grid 10-bit", dict(grid_cols=4, grid_rows=4, out_width=2048, out_height=2048, bit_depth=10),
+     "grid4032"),
+)
+
+
+def serving(H, seconds, loads=(0.25, 0.5, 0.8)):
+    """The serving loop (hpvcd_server_*) under a mixed request stream:
+    HEIC files of four sizes drawn uniformly, first as one burst (the
+    capacity, aggregate Mpixel/s), then as Poisson arrivals at fractions of
+    that capacity.  Latency is the library's own submit-to-image clock."""
+    import queue
+    import threading
+
+    import numpy as np
+
+    files = [H.gen_heic(w, base_seed=401 + 2 * i, **o) for i, (_, o, w) in enumerate(SERVING_MIX)]
+    cfgs = [H.gen_config(**dict(H.WORKLOADS[w], **o)) for _, o, w in SERVING_MIX]
+    mpix = [c.out_width * c.out_height / 1e6 for c in cfgs]
+    rng = np.random.default_rng(7)
+
+    def run(srv, picks, gaps):
+        q = queue.Queue()
+        lat = []
+        lock = threading.Lock()
+
+        def waiter():
+            while True:
+                t = q.get()
+                if t is None:
+                    return
+                _, st = srv.wait(t, image=False)
+                with lock:
+                    lat.append((st["latency_ms"], st["batch_requests"]))
+
+        ws = [threading.Thread(target=waiter) for _ in range(16)]
+        for w in ws:
+            w.start()
+        t0 = time.perf_counter()
+        due = t0
+        for k, gap in zip(picks, gaps):
+            due += gap
+            d = due - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            q.put(srv.submit(files[k]))
+        for _ in ws:
+            q.put(None)
+        for w in ws:
+            w.join()
+        el = time.perf_counter() - t0
+        l = np.array([x[0] for x in lat])
+        return {"requests": len(picks), "seconds": round(el, 3),
+                "mpix_per_s": round(sum(mpix[k] for k in picks) / el, 2),
+                "p50_ms": round(float(np.percentile(l, 50)), 3), "p99_ms": round(float(np.percentile(l, 99)), 3),
+                "mean_batch_requests": round(float(np.mean([x[1] for x in lat])), 2)}
+
+    out = {"mix": [m for m, _, _ in SERVING_MIX], "mix_mpix": [round(m, 3) for m in mpix]}
+    with H.Server() as srv:
+        run(srv, list(range(len(files))) * 2, [0.0] * (2 * len(files)))  # warm both slots
+        n = max(16, int(seconds * 40))
+        burst = run(srv, list(rng.integers(0, len(files), n)), [0.0] * n)
+        out["burst"] = burst
+        req_rate = burst["requests"] / burst["seconds"]
+        out["poisson"] = []
+        for f in loads:
+            rate = f * req_rate
+            n = max(16, int(rate * seconds))
+            r = run(srv, list(rng.integers(0, len(files), n)), list(rng.exponential(1.0 / rate, n)))
+            out["poisson"].append({"load": f, "offered_req_per_s": round(rate, 2), **r})
+        b, q, p = srv.totals()
+        out["totals"] = {"batches": b, "requests": q, "pictures": p}
+    return out
+
+
 def config_line(H, N, g, name, steps):
     """Device throughput of another workload (graph replay of a resident job)."""
     rs = H.RecordSet.generate(name)
@@ -225,6 +303,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the other workloads' lines")
+    ap.add_argument("--serving-seconds", type=float, default=2.0,
+                    help="length of each arrival-rate run of the serving figure (0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (read + write) three times, for calibrating "
                          "the FETCH_SIZE/WRITE_SIZE counters of a profiling run")
@@ -289,6 +369,11 @@ def main() -> int:
                 if name != args.workload:
                     others[name] = config_line(H, N, g, name, 10)
                     print(f"[bench] {name}: {others[name]}", file=sys.stderr, flush=True)
+
+    serve = None
+    if args.serving_seconds > 0:
+        serve = serving(H, args.serving_seconds)
+        print(f"[bench] serving {serve}", file=sys.stderr, flush=True)
 
     value = mpix / (wall_ms / 1e3)
     alg = kernel_alg_bytes(st, sample_b, res_b, out_b)
@@ -373,6 +458,7 @@ def main() -> int:
             **heic, "value": round(mpix / (heic["ms_per_image"] / 1e3), 2), "unit": "Mpixel/s",
             "kind": "HEIC file bytes -> hpvcd_decode_heic -> RGB image in host memory (container and CABAC parse "
                     "on the host threads, preparation, pinned H2D, batched GPU launches, D2H)"},
+        "serving": serve,
         "plain_per_tile": None if plain_ms is None else {
             "ms_per_step": round(plain_ms, 4), "value": round(mpix / (plain_ms / 1e3), 2),
             "batched_speedup": round(plain_ms / wall_ms, 2)},

@@ -22,7 +22,7 @@ import numpy as np
 from . import _native as N
 from ._native import HpvcdError, check
 
-__all__ = ["RecordSet", "Gpu", "Job", "PinnedBuffer", "Decoder", "HpvcdError", "WORKLOADS", "gen_config", "gen_hevc",
+__all__ = ["RecordSet", "Gpu", "Job", "PinnedBuffer", "Decoder", "Server", "HpvcdError", "WORKLOADS", "gen_config", "gen_hevc",
            "gen_heic", "N"]
 
 
@@ -460,3 +460,62 @@ class Decoder:
         finally:
             for i in range(n):
                 N.lib().hpvcd_image_free(C.byref(imgs[i]))
+
+
+class Server:
+    """The serving loop (hpvcd_server_*): submit HEIC bytes from any thread,
+    wait for each image; requests are parsed on host workers and batched
+    onto the GPU continuously."""
+
+    def __init__(self, backend: int = N.BACKEND_GPU_BATCHED, device: int = 0, threads: int = 0,
+                 max_batch_pictures: Optional[int] = None, max_batch_bytes: Optional[int] = None,
+                 hbm_reserve: Optional[int] = None, parse_workers: Optional[int] = None, **limits):
+        s = N.Settings()
+        N.lib().hpvcd_settings_default(C.byref(s))
+        s.backend, s.device, s.threads = backend, device, threads
+        for k, v in limits.items():
+            if not hasattr(s, k):
+                raise KeyError(f"unknown setting {k!r}")
+            setattr(s, k, v)
+        c = N.ServerConfig()
+        N.lib().hpvcd_server_config_default(C.byref(c))
+        for k, v in (("max_batch_pictures", max_batch_pictures), ("max_batch_bytes", max_batch_bytes),
+                     ("hbm_reserve", hbm_reserve), ("parse_workers", parse_workers)):
+            if v is not None:
+                setattr(c, k, v)
+        h = C.c_void_p()
+        check(N.lib().hpvcd_server_new(C.byref(s), C.byref(c), C.byref(h)))
+        self._h = h
+
+    def submit(self, data: bytes, rgb8: bool = False) -> int:
+        t = C.c_uint64()
+        check(N.lib().hpvcd_server_submit(self._h, data, len(data), 1 if rgb8 else 0, C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, image: bool = True):
+        """(image as (height, width, channels), stats dict) of a ticket;
+        image=False drops the pixels unread and gives (None, stats)."""
+        img = N.Image()
+        st = N.RequestStats()
+        rc = N.lib().hpvcd_server_wait(self._h, ticket, C.byref(img), C.byref(st))
+        try:
+            check(rc)
+            return (image_array(img) if image else None), {f: getattr(st, f) for f, _ in N.RequestStats._fields_}
+        finally:
+            N.lib().hpvcd_image_free(C.byref(img))
+
+    def totals(self) -> Tuple[int, int, int]:
+        b, r, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        N.lib().hpvcd_server_totals(self._h, C.byref(b), C.byref(r), C.byref(p))
+        return b.value, r.value, p.value
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().hpvcd_server_free(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -153,6 +153,16 @@ class Batch(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batch_pictures", C.c_uint32), ("parse_workers", C.c_uint32), ("max_batch_bytes", C.c_uint64),
+                ("hbm_reserve", C.c_uint64), ("_reserved", C.c_uint64)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("latency_ms", C.c_double), ("parse_ms", C.c_double), ("queue_ms", C.c_double),
+                ("device_ms", C.c_double), ("batch_requests", C.c_uint32), ("batch_pictures", C.c_uint32)]
+
+
 class GenConfig(C.Structure):
     _fields_ = [
         ("tile_width", C.c_int32), ("tile_height", C.c_int32),
@@ -258,6 +268,12 @@ SIGNATURES = {
     "hpvcd_decode_records": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(Image)]),
     "hpvcd_image_free": (None, [C.POINTER(Image)]),
     "hpvcd_decode_heic": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(Image)]),
+    "hpvcd_server_config_default": (None, [C.POINTER(ServerConfig)]),
+    "hpvcd_server_new": (C.c_int, [C.POINTER(Settings), C.POINTER(ServerConfig), C.POINTER(_P)]),
+    "hpvcd_server_submit": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.c_int32, C.POINTER(C.c_uint64)]),
+    "hpvcd_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(Image), C.POINTER(RequestStats)]),
+    "hpvcd_server_totals": (None, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "hpvcd_server_free": (None, [_P]),
     "hpvcd_decode_heic_rgb8": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(Image)]),
 }
 

@@ -312,6 +312,49 @@ hpvcd_status hpvcd_parse_heic(const uint8_t* data, size_t size, const hpvcd_sett
  * the records hpvcd_gen_records makes.  Free with hpvcd_bytes_free. */
 hpvcd_status hpvcd_gen_heic(const hpvcd_gen_config* cfg, uint8_t** data, size_t* size);
 
+/* ------------------------------------------------------------------------ */
+/* Serving loop (no counterpart in the reference; Decoder reuse semantics   */
+/* src/decoder.rs:54-92): HEIC requests arrive at any time, are parsed on    */
+/* host worker threads and are continuously batched onto the GPU.            */
+/* ------------------------------------------------------------------------ */
+typedef struct hpvcd_server hpvcd_server;
+
+typedef struct hpvcd_server_config {
+    uint32_t max_batch_pictures;   /* pictures (tiles) in one batch; a larger request still goes alone */
+    uint32_t parse_workers;        /* requests parsed at once, each on threads / parse_workers host threads */
+    uint64_t max_batch_bytes;      /* estimated device bytes of one batch */
+    uint64_t hbm_reserve;          /* free HBM held back from admission */
+    uint64_t _reserved;
+} hpvcd_server_config;
+
+/* Where one request's time went (milliseconds from submission). */
+typedef struct hpvcd_request_stats {
+    double   latency_ms;           /* submission to image in host memory */
+    double   parse_ms;             /* submission to records (queueing for a parse worker included) */
+    double   queue_ms;             /* records to launch (waiting for a slot and the batch) */
+    double   device_ms;            /* launch to image (preparation, upload, kernels, download) */
+    uint32_t batch_requests;       /* requests that shared its batch */
+    uint32_t batch_pictures;       /* pictures in its batch */
+} hpvcd_request_stats;
+
+void         hpvcd_server_config_default(hpvcd_server_config* c);
+/* settings: limits, host threads, backend and device as for the decoder.
+ * The server opens two launch slots (streams) on the device, so that one
+ * batch's host preparation and upload overlap the previous batch's kernels.
+ * Admission: a batch takes requests in arrival order while its pictures,
+ * its estimated bytes and the free HBM (less hbm_reserve) allow. */
+hpvcd_status hpvcd_server_new(const hpvcd_settings* settings, const hpvcd_server_config* c, hpvcd_server** out);
+/* Queue a HEIC decode (the bytes are copied); rgb8 != 0 asks for the
+ * hpvcd_decode_heic_rgb8 kind.  Returns at once with a ticket. */
+hpvcd_status hpvcd_server_submit(hpvcd_server* s, const uint8_t* data, size_t size, int32_t rgb8, uint64_t* ticket);
+/* Block until the request is done; the image (the one hpvcd_decode_heic
+ * gives for it alone) passes to the caller (hpvcd_image_free). */
+hpvcd_status hpvcd_server_wait(hpvcd_server* s, uint64_t ticket, hpvcd_image* img, hpvcd_request_stats* stats);
+/* Batches launched so far, and the requests and pictures they held. */
+void         hpvcd_server_totals(hpvcd_server* s, uint64_t* batches, uint64_t* requests, uint64_t* pictures);
+/* Finishes every queued request, then stops the workers and closes the GPU. */
+void         hpvcd_server_free(hpvcd_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -362,6 +362,26 @@ def test_decoder_settings_defaults_and_backends():
     assert N.lib().hpvcd_decoder_new(C.byref(s), C.byref(h)) == 15 and not h.value   # E_INVALID
 
 
+def test_server_config_and_refusals():
+    """The serving loop's defaults, and the arguments it refuses before any
+    GPU is touched: the CPU_REF backend, zero caps, null arguments."""
+    c = N.ServerConfig()
+    N.lib().hpvcd_server_config_default(C.byref(c))
+    assert (c.max_batch_pictures, c.parse_workers) == (256, 4)
+    assert (c.max_batch_bytes, c.hbm_reserve) == (16 << 30, 8 << 30)
+    s = N.Settings()
+    N.lib().hpvcd_settings_default(C.byref(s))
+    h = C.c_void_p()
+    s.backend = N.BACKEND_CPU_REF
+    assert N.lib().hpvcd_server_new(C.byref(s), C.byref(c), C.byref(h)) == 11 and not h.value   # E_UNSUPPORTED
+    s.backend = N.BACKEND_GPU_BATCHED
+    c.parse_workers = 0
+    assert N.lib().hpvcd_server_new(C.byref(s), C.byref(c), C.byref(h)) == 15 and not h.value   # E_INVALID
+    assert N.lib().hpvcd_server_new(C.byref(s), C.byref(c), None) == 15
+    t = C.c_uint64()
+    assert N.lib().hpvcd_server_submit(None, b"x", 1, 0, C.byref(t)) == 15
+
+
 def test_batch_preparation_is_the_same_on_every_thread_count():
     """Job preparation (availability stamps, wavefront units, dependency
     levels, reference-map layout, residual and ticket lists) runs per picture
