@@ -286,7 +286,7 @@ def config_line(H, N, g, name, steps):
     alg = _pipeline_bytes(st, sample_b, out_b)
     mpix = st["out_pixels"] / 1e6
     return {"ms_per_step": round(ms, 4), "value": round(mpix / (ms / 1e3), 2), "unit": "Mpixel/s",
-            "pictures": st["pictures"], "out_pixels": st["out_pixels"],
+            "pictures": st["pictures"], "out_pixels": st["out_pixels"], "dependency_waves": st["dependency_depth"],
             "plain_ms_per_step": round(plain_ms, 4), "batched_speedup": round(plain_ms / ms, 2),
             "pipeline_hbm_frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
             "kernel_ms": {k: round(v, 4) for k, v in kms.items()},

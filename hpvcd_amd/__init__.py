@@ -46,7 +46,7 @@ WORKLOADS: Dict[str, dict] = {
     # bi-prediction p=0.5, explicit weighted prediction, MV ~ N(0, 16 px), QP 30 -> one RGB8 image per picture
     "video1080": dict(tile_width=1920, tile_height=1080, grid_cols=1, grid_rows=32, out_width=1920, out_height=1080,
                       num_outputs=32, chroma_idc=1, bit_depth=8, qp=30, config_id=3, out_kind=N.OUT_RGB8,
-                      inter=1, num_refs=4, weighted=1, p_bi=0.5, mv_sigma=16.0),
+                      inter=1, num_refs=4, weighted=1, p_bi=0.5, mv_sigma=16.0, gop=8),
 }
 
 
@@ -233,7 +233,18 @@ class RecordSet:
                 mc_write += n * b
         return dict(samples=s, sample_bytes=sample_bytes, nnz=int(self.batch.num_coeffs), out_pixels=out_px,
                     out_bytes=int(out_bytes), pictures=self.num_pictures, tbs=int(self.batch.num_tbs),
-                    pus=int(self.batch.num_pus), mc_read_bytes=mc_read, mc_write_bytes=mc_write)
+                    pus=int(self.batch.num_pus), mc_read_bytes=mc_read, mc_write_bytes=mc_write,
+                    dependency_depth=self.dependency_depth())
+
+    def dependency_depth(self) -> int:
+        """Length of the longest reference chain in pictures (1 + the deepest
+        picture's level): the number of dependent reconstruction waves."""
+        lv = []
+        for i in range(self.num_pictures):
+            p = self.picture(i)
+            refs = [self.batch.refs[p.ref_begin + k].picture for k in range(p.ref_count)]
+            lv.append(1 + max((lv[r] for r in refs), default=-1))
+        return 1 + max(lv, default=-1)
 
 
 _TB_DTYPE = np.dtype([("x", "<u2"), ("y", "<u2"), ("log2n", "u1"), ("comp", "u1"), ("mode", "u1"), ("qp", "u1"),

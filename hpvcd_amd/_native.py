@@ -180,7 +180,7 @@ class GenConfig(C.Structure):
         ("log2_res_scale_max", C.c_uint8),
         ("act", C.c_uint8), ("pcm", C.c_uint8), ("pcm_bit_depth", C.c_uint8), ("pcm_lf_disabled", C.c_uint8),
         ("inter", C.c_uint8), ("num_refs", C.c_uint8), ("weighted", C.c_uint8), ("amp", C.c_uint8),
-        ("_pad", C.c_uint8 * 2),
+        ("gop", C.c_uint8), ("_pad", C.c_uint8),
         ("p_split_cu", C.c_float), ("p_nxn", C.c_float), ("p_split_tu", C.c_float), ("p_cbf", C.c_float),
         ("p_sig", C.c_float), ("p_level_geom", C.c_float), ("p_ts", C.c_float), ("p_tqb", C.c_float),
         ("p_act", C.c_float), ("p_pcm", C.c_float),

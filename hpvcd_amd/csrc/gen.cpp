@@ -283,6 +283,7 @@ hpvcd_status gen_validate(const hpvcd_gen_config& c) {
         c.tile_height > 8192)
         return set_error(HPVCD_E_BAD_DIMENSIONS, "generator: tile size must be a positive multiple of the min CU");
     if (c.chroma_idc == 2) return set_error(HPVCD_E_UNSUPPORTED_CHROMA, "generator: 4:2:2 not generated");
+    if (c.gop > 64 || (c.gop > 1 && !c.inter)) return set_error(HPVCD_E_PARAMSET, "generator: GOP size 2..64 of an inter sequence");
     if (c.bit_depth < 8 || c.bit_depth > 12) return set_error(HPVCD_E_UNSUPPORTED_BITDEPTH, "generator: bit depth");
     if (c.pcm && (c.pcm_bit_depth < 1 || c.pcm_bit_depth > c.bit_depth))
         return set_error(HPVCD_E_PARAMSET, "generator: PCM bit depth must be 1..bit_depth");
@@ -330,6 +331,11 @@ hpvcd_status gen_assemble(const hpvcd_gen_config& c, std::vector<PicRecords>& ti
         ho.orientation = c.orientation;
         r->outputs.push_back(ho);
     }
+    // output of each picture: decode order is display order unless the
+    // pictures form hierarchical GOPs
+    std::vector<int> display(ntiles);
+    for (uint32_t i = 0; i < ntiles; i++) display[i] = (int)i;
+    if (c.inter && c.gop > 1) display = gop_decode_order((int)ntiles, c.gop);
     for (uint32_t i = 0; i < ntiles; i++) {
         PicRecords& t = tiles[i];
         hpvcd_picture p;
@@ -372,7 +378,7 @@ hpvcd_status gen_assemble(const hpvcd_gen_config& c, std::vector<PicRecords>& ti
         p.dst_y = nout > 1 ? 0 : (int32_t)row * win_h;
         p.copy_w = std::max(0, std::min(win_w, c.out_width - p.dst_x));
         p.copy_h = std::max(0, std::min(win_h, c.out_height - p.dst_y));
-        p.out_index = nout > 1 ? i % (uint32_t)nout : 0u;
+        p.out_index = nout > 1 ? (uint32_t)display[i] % (uint32_t)nout : 0u;
         p.scaling_index = c.scaling_list ? 0 : -1;
         p.tb_begin = (uint32_t)r->tbs.size();
         p.tb_count = (uint32_t)t.tbs.size();

@@ -1,7 +1,9 @@
 // walker.cpp — the coding-tree walk of walker.h.
 #include "walker.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace hpvcd {
 
@@ -101,17 +103,95 @@ void PicWalker::finish() {
 // list 1 (B pictures) the same pictures farthest first.  Each entry is a
 // pred_weight_table row (src/inter.rs:141): weights around 1 << denom,
 // offsets in 8-bit units scaled to the bit depth.
+std::vector<int> gop_decode_order(int frames, int gop, std::vector<int>* layer) {
+    std::vector<int> order;
+    if (layer) layer->assign((size_t)std::max(frames, 0), 0);
+    if (frames <= 0) return order;
+    order.push_back(0);
+    // the open interval (a, b) after its ends: midpoint, then each half
+    struct Split {
+        static void run(int a, int b, int depth, std::vector<int>& o, std::vector<int>* layer) {
+            const int m = (a + b) / 2;
+            if (m <= a || m >= b) return;
+            o.push_back(m);
+            if (layer) (*layer)[(size_t)m] = depth;
+            run(a, m, depth + 1, o, layer);
+            run(m, b, depth + 1, o, layer);
+        }
+    };
+    for (int a = 0; a < frames - 1; a += gop) {
+        const int b = std::min(a + gop, frames - 1);
+        order.push_back(b);
+        Split::run(a, b, 1, order, layer);
+    }
+    return order;
+}
+
+void gop_ref_lists(int frames, int gop, int num_refs, uint32_t index, std::vector<uint32_t>& l0,
+                   std::vector<uint32_t>& l1) {
+    l0.clear();
+    l1.clear();
+    std::vector<int> layer;
+    const std::vector<int> order = gop_decode_order(frames, gop, &layer);
+    if (index == 0 || index >= order.size()) return;
+    std::vector<uint32_t> idx_of(order.size());
+    for (size_t i = 0; i < order.size(); i++) idx_of[(size_t)order[i]] = (uint32_t)i;
+    const int p = order[index];
+    const int a = (p - 1) / gop * gop, b = std::min(a + gop, frames - 1);
+    // candidates: for a GOP's last picture the earlier GOP ends (and the intra
+    // picture), for the others the pictures of its own GOP in lower temporal
+    // layers (the GOP's ends and the midpoints of coarser splits) decoded
+    // before it
+    std::vector<int> cand;
+    if (p == b) {
+        for (int q = a; q >= 0; q -= gop) cand.push_back(q);
+    } else {
+        for (int q = a; q <= b; q++)
+            if (idx_of[(size_t)q] < index &&
+                (q == a || q == b || (layer[(size_t)q] > 0 && layer[(size_t)q] < layer[(size_t)p])))
+                cand.push_back(q);
+    }
+    // the nearest in display order, the earlier one first on a tie
+    std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) {
+        const int dx = std::abs(x - p), dy = std::abs(y - p);
+        return dx != dy ? dx < dy : x < y;
+    });
+    if ((int)cand.size() > num_refs) cand.resize((size_t)num_refs);
+    std::vector<int> past, future;
+    for (int q : cand) (q < p ? past : future).push_back(q);
+    std::sort(past.rbegin(), past.rend());
+    std::sort(future.begin(), future.end());
+    for (int q : past) l0.push_back(idx_of[(size_t)q]);
+    for (int q : future) l0.push_back(idx_of[(size_t)q]);
+    if (future.empty()) {
+        l1 = l0;  // no later picture: list 1 repeats list 0 (generalised P/B)
+    } else {
+        for (int q : future) l1.push_back(idx_of[(size_t)q]);
+        for (int q : past) l1.push_back(idx_of[(size_t)q]);
+    }
+}
+
 void PicWalker::setup_refs(uint32_t pic_index) {
     Rng& rng = *src.rng();
     const hpvcd_gen_config& cfg = *gcfg;
     int n = std::max(1, std::min<int>(cfg.num_refs ? cfg.num_refs : 1, (int)pic_index));
-    nref[0] = n;
-    nref[1] = cfg.p_bi > 0.f ? n : 0;
+    std::vector<uint32_t> lists[2];
+    if (cfg.gop > 1) {
+        gop_ref_lists((int)(cfg.grid_cols * cfg.grid_rows), cfg.gop, cfg.num_refs ? cfg.num_refs : 1, pic_index,
+                      lists[0], lists[1]);
+    } else {
+        for (int i = 0; i < n; i++) {
+            lists[0].push_back(pic_index - 1 - (uint32_t)i);
+            lists[1].push_back(pic_index - (uint32_t)n + (uint32_t)i);
+        }
+    }
+    nref[0] = (int)lists[0].size();
+    nref[1] = cfg.p_bi > 0.f ? (int)lists[1].size() : 0;
     for (int l = 0; l < 2; l++)
         for (int i = 0; i < nref[l]; i++) {
             hpvcd_ref e;
             std::memset(&e, 0, sizeof(e));
-            e.picture = pic_index - 1 - (uint32_t)(l == 0 ? i : n - 1 - i);
+            e.picture = lists[l][(size_t)i];
             e.weighted = cfg.weighted ? 1 : 0;
             e.log2_denom[0] = 6;
             e.log2_denom[1] = 6;

@@ -196,4 +196,21 @@ private:
     void coding_quadtree(int x0, int y0, int log2, int depth);
 };
 
+// Random-access coding structure with hierarchical B pictures (GOPs of
+// `gop` pictures after the intra picture): display-order numbers (POCs) in
+// decode order.  Each GOP's last picture comes first, then its interval is
+// split at the midpoint, recursively; a shorter last GOP is split the same way.
+// `layer` (by POC) receives the temporal layer: 0 for the intra picture and
+// the GOP ends, k for the midpoints of the k-th split.
+std::vector<int> gop_decode_order(int frames, int gop, std::vector<int>* layer = nullptr);
+// Reference lists of the picture at decode index `index` (batch indices):
+// a GOP's last picture refers to the earlier GOP ends, the others to the
+// pictures of their own GOP in lower temporal layers decoded before them;
+// at most num_refs,
+// nearest in display order.  List 0 holds the earlier pictures first (nearest
+// first), list 1 the later ones; without a later picture list 1 repeats
+// list 0.
+void gop_ref_lists(int frames, int gop, int num_refs, uint32_t index, std::vector<uint32_t>& l0,
+                   std::vector<uint32_t>& l1);
+
 }  // namespace hpvcd

@@ -67,10 +67,13 @@ typedef struct hpvcd_gen_config {
     uint8_t  pcm_bit_depth;             /* PcmBitDepth (luma and chroma), 1..bit_depth */
     uint8_t  pcm_lf_disabled;           /* pcm_loop_filter_disabled_flag */
     uint8_t  inter;                     /* pictures after the first are inter pictures of one sequence */
-    uint8_t  num_refs;                  /* references per list (1..4): the nearest earlier pictures */
+    uint8_t  num_refs;                  /* references per list (1..4) */
     uint8_t  weighted;                  /* explicit weighted prediction */
     uint8_t  amp;                       /* asymmetric motion partitions */
-    uint8_t  _pad[2];
+    uint8_t  gop;                       /* 0/1: each inter picture refers to the pictures just before it in decode
+                                           order; n > 1: random access, hierarchical B GOPs of n pictures (pictures
+                                           in decode order, outputs in display order) */
+    uint8_t  _pad;
     float    p_split_cu, p_nxn, p_split_tu, p_cbf, p_sig, p_level_geom, p_ts, p_tqb;
     float    p_act;                     /* tu_residual_act_flag where the syntax is present */
     float    p_pcm;                     /* pcm_flag where the syntax is present */

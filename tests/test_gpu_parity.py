@@ -54,6 +54,10 @@ SMALL = {
     # inter pictures: 1 I + 4 low-delay B, 4 references, explicit weighted prediction, AMP
     "inter420_wp": dict(workload="single512", tile_width=256, tile_height=128, grid_cols=1, grid_rows=5,
                         num_outputs=5, out_width=256, out_height=128, inter=1, weighted=1, base_seed=61),
+    # random access: 1 I + 12 pictures in hierarchical B GOPs of 8 (outputs in display order)
+    "inter420_gop8": dict(workload="single512", tile_width=256, tile_height=128, grid_cols=1, grid_rows=13,
+                          num_outputs=13, out_width=256, out_height=128, inter=1, weighted=1, gop=8,
+                          base_seed=67),
     # P pictures without weights, larger motion
     "inter420_p": dict(workload="single512", tile_width=192, tile_height=136, grid_cols=1, grid_rows=4,
                        num_outputs=4, out_width=192, out_height=136, inter=1, p_bi=0.0, mv_sigma=40.0,

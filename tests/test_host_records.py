@@ -362,6 +362,42 @@ def test_decoder_settings_defaults_and_backends():
     assert N.lib().hpvcd_decoder_new(C.byref(s), C.byref(h)) == 15 and not h.value   # E_INVALID
 
 
+def test_hierarchical_gop_structure():
+    """Random access with hierarchical B GOPs of 8 over 32 frames: pictures
+    in decode order (each GOP's end first, then midpoints), outputs in
+    display order, references decoded earlier, inside the GOP (or the earlier
+    GOP ends for a GOP's end), never the top temporal layer, at most 4; the
+    reference chain is 8 pictures deep instead of 32."""
+    rs = H.RecordSet.generate("video1080", tile_width=256, tile_height=128, out_width=256, out_height=128)
+    n = rs.num_pictures
+    poc = [rs.picture(i).out_index for i in range(n)]
+    assert sorted(poc) == list(range(32))
+    assert poc[:9] == [0, 8, 4, 2, 1, 3, 6, 5, 7]
+    assert poc[25:] == [31, 27, 25, 26, 29, 28, 30]
+    referenced = set()
+    for i in range(1, n):
+        p = rs.picture(i)
+        refs = {rs.batch.refs[p.ref_begin + k].picture for k in range(p.ref_count)}
+        assert refs and all(r < i for r in refs) and len(refs) <= 4
+        rp = [poc[r] for r in refs]
+        a = (poc[i] - 1) // 8 * 8
+        b = min(a + 8, 31)
+        if poc[i] == b:
+            assert all(q % 8 == 0 and q < poc[i] for q in rp)
+        else:
+            assert all(a <= q <= b for q in rp) and any(q > poc[i] for q in rp)
+        referenced.update(rp)
+    # the top temporal layer: odd POCs of the full GOPs, the even ones of the 7-frame last GOP
+    assert not referenced & (set(range(1, 24, 2)) | {26, 28, 30})
+    assert rs.dependency_depth() == 8
+    low_delay = H.RecordSet.generate("video1080", tile_width=256, tile_height=128, out_width=256, out_height=128,
+                                     gop=0)
+    assert [low_delay.picture(i).out_index for i in range(n)] == list(range(32))
+    assert low_delay.dependency_depth() == 32
+    with pytest.raises(H.HpvcdError):
+        H.RecordSet.generate("single512", gop=8)
+
+
 def test_server_config_and_refusals():
     """The serving loop's defaults, and the arguments it refuses before any
     GPU is touched: the CPU_REF backend, zero caps, null arguments."""
