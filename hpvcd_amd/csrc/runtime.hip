@@ -12,6 +12,7 @@
 #include <type_traits>
 #include <cstring>
 #include <mutex>
+#include <memory>
 #include <new>
 #include <string>
 #include <thread>
@@ -325,25 +326,41 @@ struct DevBuf {
     }
 };
 
+// Allocator whose resize() leaves trivially constructible elements
+// uninitialised: the merged arrays below are written element by element
+// right after they are sized, so zero-filling them first would only cost a
+// serial pass over tens of megabytes.
+template <typename T>
+struct UninitAlloc : std::allocator<T> {
+    template <typename U> struct rebind { using other = UninitAlloc<U>; };
+    UninitAlloc() = default;
+    template <typename U> UninitAlloc(const UninitAlloc<U>&) noexcept {}
+    template <typename U> void construct(U* p) noexcept { ::new (static_cast<void*>(p)) U; }
+    template <typename U, typename... A> void construct(U* p, A&&... a) {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+};
+template <typename T> using PodVec = std::vector<T, UninitAlloc<T>>;
 
 struct ClassJob {
     bool samp16 = false, res32 = false;
     std::vector<uint32_t> batch_pics;     // batch picture index per class-local picture
     // host-prepared arrays
     std::vector<DevPic> pics;
-    std::vector<hpvcd_tb> tbs;
-    std::vector<uint32_t> tb_pic, coeffs, cells, order;
-    std::vector<hpvcd_sao> sao;
-    std::vector<DevUnit> units;
-    std::vector<uint32_t> tickets, plain_tickets, unit_tbs;
-    std::vector<PrepTb> unit_recs;                 // K2a's fields of each unit_tbs entry
+    PodVec<hpvcd_tb> tbs;
+    PodVec<uint32_t> tb_pic, coeffs, cells, order;
+    PodVec<hpvcd_sao> sao;
+    PodVec<DevUnit> units;
+    std::vector<uint32_t> tickets, plain_tickets;
+    PodVec<uint32_t> unit_tbs;
+    PodVec<PrepTb> unit_recs;                      // K2a's fields of each unit_tbs entry
     std::vector<uint32_t> prep_list;               // units K2a prepares (Cr units sharing Cb's are not)
     std::vector<uint32_t> plain_prep, pic_prep_begin; // the same units per picture (plain mode) and segments
-    std::vector<uint16_t> level_off;
-    std::vector<uint32_t> unit_refoff;             // per unit TB entry: map offset | sample offset / 16 << 13 | level << 21
+    PodVec<uint16_t> level_off;
+    PodVec<uint32_t> unit_refoff;                  // per unit TB entry: map offset | sample offset / 16 << 13 | level << 21
     size_t smap_words = 0, rmap_words = 0;
     std::vector<uint32_t> res_list[2][4];            // [pass][log2-2], batched order
-    std::vector<DevResJob> res_jobs[2][4];           // K1 job of each res_list entry
+    PodVec<DevResJob> res_jobs[2][4];                // K1 job of each res_list entry
     std::vector<uint32_t> pic_units_begin;            // plain_tickets segment per picture (npics+1)
     std::vector<uint32_t> pic_res_begin[2][4];        // per picture list segments (npics+1)
     std::vector<uint32_t> act_list, pic_act_begin;    // luma TB of each ACT triple; per picture segments
@@ -1441,6 +1458,31 @@ int host_threads() {
     return n;
 }
 
+// Copies the uploaded arrays to their offsets in the staging buffer on
+// `threads` host threads, in pieces of at most 1 MiB.
+void stage_copy(uint8_t* stage, const std::vector<std::pair<size_t, std::pair<const void*, size_t>>>& ups,
+                int threads) {
+    constexpr size_t kPiece = (size_t)1 << 20;
+    struct Piece { uint8_t* dst; const uint8_t* src; size_t n; };
+    std::vector<Piece> pieces;
+    size_t total = 0;
+    for (const auto& u : ups)
+        for (size_t o = 0; o < u.second.second; o += kPiece) {
+            const size_t n = std::min(kPiece, u.second.second - o);
+            pieces.push_back({stage + u.first + o, static_cast<const uint8_t*>(u.second.first) + o, n});
+            total += n;
+        }
+    const int nth = (int)std::max<size_t>(1, std::min<size_t>((size_t)threads, total / (4 * kPiece)));
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t k; (k = next.fetch_add(1)) < pieces.size();) std::memcpy(pieces[k].dst, pieces[k].src, pieces[k].n);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+}
+
 // Job creation into a pool of the library's own (ext_base null: hipMalloc,
 // pinned staging) or into an existing device region of ext_size bytes
 // (staged from ordinary host memory, nothing allocated on the device).
@@ -1542,7 +1584,7 @@ hpvcd_status job_create_impl(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out
         // one transfer from ordinary host staging, then the same pool clears
         // (the scratch holds the previous call's contents)
         std::vector<uint8_t> stage(upload_bytes);
-        for (auto& u : ups) std::memcpy(stage.data() + u.first, u.second.first, u.second.second);
+        stage_copy(stage.data(), ups, host_threads());
         hipError_t e = hipSuccess;
         if (upload_bytes) e = hipMemcpyAsync(j->mem.base, stage.data(), upload_bytes, hipMemcpyHostToDevice, g->stream);
         if (e == hipSuccess) e = hipMemsetAsync(j->d_out, 0, ob + 1, g->stream);
@@ -1573,7 +1615,7 @@ hpvcd_status job_create_impl(hpvcd_gpu* g, const hpvcd_batch* b, hpvcd_job** out
             }
             g->pin_size = upload_bytes;
         }
-        for (auto& u : ups) std::memcpy(g->pin + u.first, u.second.first, u.second.second);
+        stage_copy(g->pin, ups, host_threads());
         hipError_t e = hipMemcpyAsync(j->mem.base, g->pin, upload_bytes, hipMemcpyHostToDevice, g->stream);
         if (e == hipSuccess) e = hipMemsetAsync(j->d_out, 0, ob + 1, g->stream);
         for (auto& C : j->classes) {
