@@ -23,6 +23,7 @@ from . import _native as N
 from ._native import HpvcdError, check
 
 __all__ = ["RecordSet", "Gpu", "Job", "PinnedBuffer", "Decoder", "Server", "HpvcdError", "WORKLOADS", "gen_config", "gen_hevc",
+           "default_tuning", "save_tuning", "load_tuning",
            "gen_heic", "N"]
 
 
@@ -336,6 +337,23 @@ class Gpu:
     def job(self, records: RecordSet) -> "Job":
         return Job(self, records)
 
+    @property
+    def tuning(self) -> dict:
+        """The launch shapes this GPU's launches use (hpvcd_gpu_get_tuning)."""
+        t = N.Tuning()
+        check(N.lib().hpvcd_gpu_get_tuning(self._h, C.byref(t)))
+        return tuning_dict(t)
+
+    @tuning.setter
+    def tuning(self, d: dict) -> None:
+        check(N.lib().hpvcd_gpu_set_tuning(self._h, C.byref(tuning_struct(d))))
+
+    @property
+    def tuning_key(self) -> str:
+        b = C.create_string_buffer(64)
+        check(N.lib().hpvcd_gpu_tuning_key(self._h, b, 64))
+        return b.value.decode()
+
     def decode(self, records: RecordSet) -> List[np.ndarray]:
         """Upload, reconstruct and download every output image of the batch."""
         with Job(self, records) as j:
@@ -381,6 +399,13 @@ class Job:
 
     def run(self, mode: int = N.MODE_BATCHED, stop: int = N.STAGE_RGB) -> None:
         check(N.lib().hpvcd_job_run(self._h, mode, stop))
+
+    def tune(self, iters: int = 10) -> Tuple[dict, float, float]:
+        """The autotuner on this job (hpvcd_tune): (best shapes, their ms per
+        run, ms per run of the shapes it started from); the GPU keeps the best."""
+        t, b, d = N.Tuning(), C.c_double(), C.c_double()
+        check(N.lib().hpvcd_tune(self._h, iters, C.byref(t), C.byref(b), C.byref(d)))
+        return tuning_dict(t), b.value, d.value
 
     def download(self) -> List[np.ndarray]:
         bufs = self.records.output_buffers()
@@ -530,3 +555,39 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+def tuning_dict(t: N.Tuning) -> dict:
+    return {"residual_block": list(t.residual_block), "deblock_block": t.deblock_block, "rgb_rows": t.rgb_rows,
+            "intra_per_cu": t.intra_per_cu, "sao_parts": t.sao_parts}
+
+
+def tuning_struct(d: dict) -> N.Tuning:
+    t = N.Tuning()
+    N.lib().hpvcd_tuning_default(C.byref(t))
+    for k, v in d.items():
+        if k == "residual_block":
+            for i, x in enumerate(v):
+                t.residual_block[i] = x
+        elif k in ("deblock_block", "rgb_rows", "intra_per_cu", "sao_parts"):
+            setattr(t, k, v)
+        else:
+            raise KeyError(f"unknown tuning field {k!r}")
+    return t
+
+
+def default_tuning() -> dict:
+    t = N.Tuning()
+    N.lib().hpvcd_tuning_default(C.byref(t))
+    return tuning_dict(t)
+
+
+def save_tuning(path: str, key: str, d: dict) -> None:
+    """Write (or replace) the key's line of a tuning cache file (hpvcd_tuning_save)."""
+    check(N.lib().hpvcd_tuning_save(path.encode(), key.encode(), C.byref(tuning_struct(d))))
+
+
+def load_tuning(path: str, key: str) -> dict:
+    t = N.Tuning()
+    check(N.lib().hpvcd_tuning_load(path.encode(), key.encode(), C.byref(t)))
+    return tuning_dict(t)

@@ -158,6 +158,11 @@ class ServerConfig(C.Structure):
                 ("hbm_reserve", C.c_uint64), ("_reserved", C.c_uint64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("residual_block", C.c_uint16 * 4), ("deblock_block", C.c_uint16), ("rgb_rows", C.c_uint16),
+                ("intra_per_cu", C.c_uint8), ("sao_parts", C.c_uint8), ("_pad", C.c_uint8 * 2)]
+
+
 class RequestStats(C.Structure):
     _fields_ = [("latency_ms", C.c_double), ("parse_ms", C.c_double), ("queue_ms", C.c_double),
                 ("device_ms", C.c_double), ("batch_requests", C.c_uint32), ("batch_pictures", C.c_uint32)]
@@ -275,6 +280,14 @@ SIGNATURES = {
     "hpvcd_server_totals": (None, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "hpvcd_server_free": (None, [_P]),
     "hpvcd_decode_heic_rgb8": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(Image)]),
+    "hpvcd_tuning_default": (None, [C.POINTER(Tuning)]),
+    "hpvcd_tuning_check": (C.c_int, [C.POINTER(Tuning)]),
+    "hpvcd_gpu_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "hpvcd_gpu_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "hpvcd_gpu_tuning_key": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
+    "hpvcd_tune": (C.c_int, [_P, C.c_int, C.POINTER(Tuning), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "hpvcd_tuning_save": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(Tuning)]),
+    "hpvcd_tuning_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(Tuning)]),
 }
 
 _lib = None

@@ -168,15 +168,15 @@ __device__ __forceinline__ void store_res_row(R* p, const int32_t (&v)[N], int v
 // N lanes of one wave per TB: a TB's samples sit in LDS rows of N + 1 words
 // (conflict-free column and row access); lane j owns column j for the
 // vertical pass and row j for the horizontal pass, so the TB needs no
-// barrier beyond its wave.
-template <typename R, int LOG2N>
-__global__ void __launch_bounds__(256) k_residual(const DevResJob* __restrict__ jobs, uint32_t count,
-                                                  const uint32_t* __restrict__ coeffs,
-                                                  const uint8_t* __restrict__ scaling, R* __restrict__ res_pool) {
+// barrier beyond its wave.  BLOCK threads per workgroup (64, 128 or 256).
+template <typename R, int LOG2N, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_residual(const DevResJob* __restrict__ jobs, uint32_t count,
+                                                    const uint32_t* __restrict__ coeffs,
+                                                    const uint8_t* __restrict__ scaling, R* __restrict__ res_pool) {
     constexpr int N = 1 << LOG2N;
     constexpr int NN = N * N;
     constexpr int P = N + 1;        // LDS row pitch in words
-    constexpr int TBS = 256 / N;    // TBs per workgroup
+    constexpr int TBS = BLOCK / N;  // TBs per workgroup
     __shared__ int32_t blk[TBS * N * P];
 
     const int tid = threadIdx.x;
@@ -812,8 +812,8 @@ __global__ void __launch_bounds__(256) k_deblock(const DevPic* __restrict__ pics
 // ---------------------------------------------------------------------------
 // K4: SAO, out of place (src/decode.rs:2258; gated EO src/sao.rs:433)
 //
-// One workgroup per part of a luma CTB's rows (the CTB whole, or quarters
-// when the launch holds few CTBs) and per chroma plane of a CTB, so the CTB's
+// One workgroup per part of a luma CTB's rows (the CTB whole, or 2 or 4
+// bands of rows when the launch holds few CTBs) and per chroma plane of a CTB, so the CTB's
 // SAO parameters are uniform; each thread takes quads of 4 horizontally adjacent samples
 // (plane widths are multiples of 4 and planes start 16-sample aligned, so a
 // quad is one aligned vector load/store).
@@ -836,7 +836,7 @@ __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, co
     const int ctb = 1 << P.ctb_log2;
     const int xs = (rx * ctb) >> sw, ys0 = (ry * ctb) >> sh;
     const int xe = min(((rx + 1) * ctb) >> sw, pw), ye0 = min(((ry + 1) * ctb) >> sh, ph);
-    const int prow = nparts == 4 ? (ye0 - ys0 + 3) >> 2 : ye0 - ys0;  // luma_parts is 1 or 4
+    const int prow = (ye0 - ys0 + nparts - 1) / nparts;
     const int ys = ys0 + part * prow, ye = min(ye0, ys + prow);
     const int qw = (xe - xs) >> 2;
     if (qw <= 0 || ye <= ys) return;
@@ -942,8 +942,8 @@ __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, co
 // K5: YUV -> RGB with crop window and grid placement (src/yuv.rs:187-396)
 //
 // One thread per 4 horizontally adjacent output pixels; a workgroup covers
-// 64 such quads of 4 output rows (grid: quads of a row / 64, rows / 4,
-// pictures).  The Q13 products fit 32 bits for bit depths up to 12
+// 64 such quads of blockDim / 64 output rows (grid: quads of a row / 64,
+// rows / (blockDim / 64), pictures).  The Q13 products fit 32 bits for bit depths up to 12
 // (|coefficient| < 2^15, sample < 2^12).
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -951,7 +951,8 @@ __global__ void __launch_bounds__(256) k_rgb(const DevPic* __restrict__ pics, co
                                              uint8_t* __restrict__ out_pool, uint32_t pic_base) {
     const DevPic& P = pics[pic_base + blockIdx.z];
     const int qx = (P.copy_w + 3) >> 2;
-    const int xq = (int)(blockIdx.x * 64 + (threadIdx.x & 63)), yy = (int)(blockIdx.y * 4 + (threadIdx.x >> 6));
+    const int xq = (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+    const int yy = (int)(blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (xq >= qx || yy >= P.copy_h) return;
     const int xx0 = xq * 4;
     const int npx = min(4, P.copy_w - xx0);
@@ -1116,11 +1117,13 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
 // ---------------------------------------------------------------------------
 // explicit instantiations (sample type x residual type)
 // ---------------------------------------------------------------------------
-#define HPVCD_INST_RES(R)                                                                                 \
-    template __global__ void k_residual<R, 2>(const DevResJob*, uint32_t, const uint32_t*, const uint8_t*, R*); \
-    template __global__ void k_residual<R, 3>(const DevResJob*, uint32_t, const uint32_t*, const uint8_t*, R*); \
-    template __global__ void k_residual<R, 4>(const DevResJob*, uint32_t, const uint32_t*, const uint8_t*, R*); \
-    template __global__ void k_residual<R, 5>(const DevResJob*, uint32_t, const uint32_t*, const uint8_t*, R*);
+#define HPVCD_INST_RES1(R, L, B) \
+    template __global__ void k_residual<R, L, B>(const DevResJob*, uint32_t, const uint32_t*, const uint8_t*, R*);
+#define HPVCD_INST_RES(R)                                                                            \
+    HPVCD_INST_RES1(R, 2, 64) HPVCD_INST_RES1(R, 2, 128) HPVCD_INST_RES1(R, 2, 256)                 \
+    HPVCD_INST_RES1(R, 3, 64) HPVCD_INST_RES1(R, 3, 128) HPVCD_INST_RES1(R, 3, 256)                 \
+    HPVCD_INST_RES1(R, 4, 64) HPVCD_INST_RES1(R, 4, 128) HPVCD_INST_RES1(R, 4, 256)                 \
+    HPVCD_INST_RES1(R, 5, 64) HPVCD_INST_RES1(R, 5, 128) HPVCD_INST_RES1(R, 5, 256)
 HPVCD_INST_RES(int16_t)
 HPVCD_INST_RES(int32_t)
 

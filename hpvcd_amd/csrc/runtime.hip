@@ -9,6 +9,8 @@
 #include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
+#include <initializer_list>
 #include <type_traits>
 #include <cstring>
 #include <mutex>
@@ -272,6 +274,11 @@ struct hpvcd_gpu {
     std::mutex launch_mu;
     // this device's copy of the allocation-free entry point's scratch array
     uint8_t* scratch = nullptr;
+    // launch shapes; `tuning_gen` counts changes, so that graphs captured
+    // with older shapes are captured again
+    hpvcd_tuning tuning;
+    uint32_t tuning_gen = 0;
+    char arch[64] = {0};
 };
 
 // Scratch of hpvcd_gpu_decode_batch_noalloc: every device array of one batch
@@ -400,6 +407,7 @@ struct ClassJob {
     uint32_t* d_border = nullptr; // kBorderWords per unit
     hipGraphExec_t graph = nullptr;
     int graph_stop = -1;
+    uint32_t graph_gen = 0;    // the GPU's tuning generation the graph was captured with
 };
 
 }  // namespace
@@ -1027,17 +1035,27 @@ struct Timer {
 
 inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-template <typename R>
-void launch_residual(hipStream_t s, ClassJob& C, const uint8_t* sc, int log2, const DevResJob* jobs, uint32_t count) {
-    if (!count) return;
+template <typename R, int LOG2N>
+void launch_residual_n(hipStream_t s, ClassJob& C, const uint8_t* sc, int block, const DevResJob* jobs, uint32_t count) {
     R* res = reinterpret_cast<R*>(C.d_res);
-    // 256 / n TBs per workgroup (n lanes each)
-    const uint32_t grid = cdiv(count, 256u >> log2);
+    // block / n TBs per workgroup (n lanes each)
+    const uint32_t grid = cdiv(count, (uint32_t)block >> LOG2N);
+    switch (block) {
+        case 64: k_residual<R, LOG2N, 64><<<grid, 64, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
+        case 128: k_residual<R, LOG2N, 128><<<grid, 128, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
+        default: k_residual<R, LOG2N, 256><<<grid, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
+    }
+}
+
+template <typename R>
+void launch_residual(hipStream_t s, ClassJob& C, const uint8_t* sc, int log2, int block, const DevResJob* jobs,
+                     uint32_t count) {
+    if (!count) return;
     switch (log2) {
-        case 2: k_residual<R, 2><<<grid, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
-        case 3: k_residual<R, 3><<<grid, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
-        case 4: k_residual<R, 4><<<grid, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
-        default: k_residual<R, 5><<<grid, 256, 0, s>>>(jobs, count, C.d_coeffs, sc, res); break;
+        case 2: launch_residual_n<R, 2>(s, C, sc, block, jobs, count); break;
+        case 3: launch_residual_n<R, 3>(s, C, sc, block, jobs, count); break;
+        case 4: launch_residual_n<R, 4>(s, C, sc, block, jobs, count); break;
+        default: launch_residual_n<R, 5>(s, C, sc, block, jobs, count); break;
     }
 }
 
@@ -1057,7 +1075,9 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
         (void)hipEventRecord(e, s);
         marks->push_back({kind, e});
     };
+    const hpvcd_tuning& tn = g->tuning;
     int occ = g->occ_intra[(sizeof(T) == 2 ? 2 : 0) + (sizeof(R) == 4 ? 1 : 0)];
+    if (tn.intra_per_cu > 0 && tn.intra_per_cu < occ) occ = tn.intra_per_cu;
     if (occ < 1) occ = 1;
     const uint32_t max_blocks = (uint32_t)(g->num_cus * occ);
     const uint32_t ncounters = nwaves + np;  // one ticket counter per wave, then one per picture
@@ -1103,7 +1123,8 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
                 for (int sz = 0; sz < 4; sz++) {
                     uint32_t b0 = plain ? C.pic_res_begin[pass][sz][p0] : 0;
                     uint32_t b1 = plain ? C.pic_res_begin[pass][sz][p0 + 1] : (uint32_t)C.res_list[pass][sz].size();
-                    launch_residual<R>(s, C, d_scaling, sz + 2, C.d_res_jobs[pass][sz] + b0, b1 - b0);
+                    launch_residual<R>(s, C, d_scaling, sz + 2, tn.residual_block[sz], C.d_res_jobs[pass][sz] + b0,
+                                       b1 - b0);
                 }
             uint32_t a0 = plain ? C.pic_act_begin[p0] : 0;
             uint32_t a1 = plain ? C.pic_act_begin[p0 + 1] : (uint32_t)C.act_list.size();
@@ -1144,18 +1165,19 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
             uint32_t lh = plain ? (uint32_t)((D0.h >= 12 ? (D0.h - 4) / 8 : 0) * (D0.w / 4)) : C.max_luma_h;
             uint32_t cv = plain ? (uint32_t)((D0.cw >= 10 ? (D0.cw - 2) / 8 : 0) * (D0.ch / 4)) : C.max_chroma_v;
             uint32_t ch = plain ? (uint32_t)((D0.ch >= 10 ? (D0.ch - 2) / 8 : 0) * (D0.cw / 4)) : C.max_chroma_h;
+            const uint32_t db = tn.deblock_block;
             if (lv + cv)
-                k_deblock<T><<<dim3(cdiv(lv, 256) + cdiv(cv, 256), qn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 1, q0,
-                                                                                 cdiv(lv, 256));
+                k_deblock<T><<<dim3(cdiv(lv, db) + cdiv(cv, db), qn), db, 0, s>>>(C.d_pics, C.d_cells, pa, 1, q0,
+                                                                              cdiv(lv, db));
             if (lh + ch)
-                k_deblock<T><<<dim3(cdiv(lh, 256) + cdiv(ch, 256), qn), 256, 0, s>>>(C.d_pics, C.d_cells, pa, 0, q0,
-                                                                                 cdiv(lh, 256));
+                k_deblock<T><<<dim3(cdiv(lh, db) + cdiv(ch, db), qn), db, 0, s>>>(C.d_pics, C.d_cells, pa, 0, q0,
+                                                                              cdiv(lh, db));
             mark(-1);
             if (upto < HPVCD_STAGE_SAO) return;
             mark(HPVCD_KERNEL_SAO);
             uint32_t nctb = plain ? (uint32_t)(D0.ctb_cols * D0.ctb_rows) : C.max_ctbs;
-            // a launch of few CTBs splits each luma CTB's rows over four workgroups
-            const int parts = nctb * qn < 4u * (uint32_t)g->num_cus ? 4 : 1;
+            // a launch of few CTBs splits each luma CTB's rows over several workgroups
+            const int parts = nctb * qn < 4u * (uint32_t)g->num_cus ? (int)tn.sao_parts : 1;
             k_sao<T><<<dim3(nctb, qn, parts + 2), 256, 0, s>>>(C.d_pics, C.d_cells, C.d_sao, pa, pb, q0, parts);
             mark(-1);
         };
@@ -1167,10 +1189,12 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
         }
         if (stop >= HPVCD_STAGE_RGB) {
             mark(HPVCD_KERNEL_RGB);
-            // quads of 4 pixels: 64 of a row by 4 rows per workgroup
+            // quads of 4 pixels: 64 of a row by rgb_rows rows per workgroup
             const uint32_t qx = plain ? (uint32_t)((C.pics[p0].copy_w + 3) / 4) : C.max_rgb_qx;
             const uint32_t rows = plain ? (uint32_t)C.pics[p0].copy_h : C.max_rgb_rows;
-            if (qx && rows) k_rgb<T><<<dim3(cdiv(qx, 64), cdiv(rows, 4), pn), 256, 0, s>>>(C.d_pics, pb, d_out, p0);
+            const uint32_t rr = tn.rgb_rows;
+            if (qx && rows)
+                k_rgb<T><<<dim3(cdiv(qx, 64), cdiv(rows, rr), pn), 64 * rr, 0, s>>>(C.d_pics, pb, d_out, p0);
             mark(-1);
         }
     }
@@ -1336,6 +1360,21 @@ hpvcd_status hpvcd_gpu_open(int device, hpvcd_gpu** out) {
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     g->num_cus = prop.multiProcessorCount;
+    {
+        // the architecture name up to its feature suffix ("gfx950:sramecc+:xnack-")
+        size_t n = 0;
+        while (n + 1 < sizeof(g->arch) && prop.gcnArchName[n] && prop.gcnArchName[n] != ':') {
+            g->arch[n] = prop.gcnArchName[n];
+            n++;
+        }
+        g->arch[n] = 0;
+    }
+    hpvcd_tuning_default(&g->tuning);
+    if (const char* tf = std::getenv("HPVCD_TUNING_FILE")) {
+        char key[64];
+        hpvcd_tuning t;
+        if (*tf && !hpvcd_gpu_tuning_key(g, key, sizeof(key)) && !hpvcd_tuning_load(tf, key, &t)) g->tuning = t;
+    }
     HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
@@ -1367,6 +1406,171 @@ hpvcd_status hpvcd_gpu_mem_info(hpvcd_gpu* g, size_t* free_b, size_t* total_b) {
     if (!g || !free_b || !total_b) return set_error(HPVCD_E_INVALID, "null argument");
     HIP_TRY(hipSetDevice(g->device));
     HIP_TRY(hipMemGetInfo(free_b, total_b));
+    return HPVCD_OK;
+}
+
+void hpvcd_tuning_default(hpvcd_tuning* t) {
+    if (!t) return;
+    std::memset(t, 0, sizeof(*t));
+    for (int k = 0; k < 4; k++) t->residual_block[k] = 256;
+    t->deblock_block = 256;
+    t->rgb_rows = 4;
+    t->intra_per_cu = 0;
+    t->sao_parts = 4;
+}
+
+hpvcd_status hpvcd_tuning_check(const hpvcd_tuning* t) {
+    if (!t) return set_error(HPVCD_E_INVALID, "null argument");
+    auto in = [](unsigned v, std::initializer_list<unsigned> ok) {
+        for (unsigned o : ok)
+            if (v == o) return true;
+        return false;
+    };
+    for (int k = 0; k < 4; k++)
+        if (!in(t->residual_block[k], {64, 128, 256})) return set_error(HPVCD_E_INVALID, "tuning: residual_block");
+    if (!in(t->deblock_block, {64, 128, 256})) return set_error(HPVCD_E_INVALID, "tuning: deblock_block");
+    if (!in(t->rgb_rows, {1, 2, 4})) return set_error(HPVCD_E_INVALID, "tuning: rgb_rows");
+    if (t->intra_per_cu > 8) return set_error(HPVCD_E_INVALID, "tuning: intra_per_cu");
+    if (!in(t->sao_parts, {1, 2, 4})) return set_error(HPVCD_E_INVALID, "tuning: sao_parts");
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_gpu_set_tuning(hpvcd_gpu* g, const hpvcd_tuning* t) {
+    if (!g || !t) return set_error(HPVCD_E_INVALID, "null argument");
+    if (hpvcd_status st = hpvcd_tuning_check(t)) return st;
+    std::lock_guard<std::mutex> launch_lock(g->launch_mu);
+    g->tuning = *t;
+    std::memset(g->tuning._pad, 0, sizeof(g->tuning._pad));
+    g->tuning_gen++;
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_gpu_get_tuning(hpvcd_gpu* g, hpvcd_tuning* t) {
+    if (!g || !t) return set_error(HPVCD_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> launch_lock(g->launch_mu);
+    *t = g->tuning;
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_gpu_tuning_key(hpvcd_gpu* g, char* key, size_t key_size) {
+    if (!g || !key || !key_size) return set_error(HPVCD_E_INVALID, "null argument");
+    const int n = std::snprintf(key, key_size, "%s:%dcu", g->arch[0] ? g->arch : "gpu", g->num_cus);
+    if (n < 0 || (size_t)n >= key_size) return set_error(HPVCD_E_INVALID, "key buffer too small");
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_tune(hpvcd_job* j, int iters, hpvcd_tuning* best, double* best_ms, double* default_ms) {
+    if (!j || iters < 1) return set_error(HPVCD_E_INVALID, "bad arguments");
+    hpvcd_gpu* g = j->g;
+    hpvcd_tuning cur;
+    if (hpvcd_status st = hpvcd_gpu_get_tuning(g, &cur)) return st;
+    auto timed = [&](const hpvcd_tuning& t, double* ms) -> hpvcd_status {
+        if (hpvcd_status st = hpvcd_gpu_set_tuning(g, &t)) return st;
+        return hpvcd_job_time(j, HPVCD_MODE_BATCHED, 1, iters, ms, nullptr);
+    };
+    double cur_ms = 0.0;
+    if (hpvcd_status st = timed(cur, &cur_ms)) return st;
+    if (default_ms) *default_ms = cur_ms;
+    // knobs in launch order: K1 per TU size, K2, K3, K4, K5
+    struct Knob { int field; std::vector<unsigned> values; };
+    const std::vector<Knob> knobs = {
+        {0, {64, 128, 256}}, {1, {64, 128, 256}}, {2, {64, 128, 256}}, {3, {64, 128, 256}},
+        {4, {0, 1, 2, 3, 4}}, {5, {64, 128, 256}}, {6, {1, 2, 4}}, {7, {1, 2, 4}}};
+    auto with = [](hpvcd_tuning t, int field, unsigned v) {
+        if (field < 4) t.residual_block[field] = (uint16_t)v;
+        else if (field == 4) t.intra_per_cu = (uint8_t)v;
+        else if (field == 5) t.deblock_block = (uint16_t)v;
+        else if (field == 6) t.sao_parts = (uint8_t)v;
+        else t.rgb_rows = (uint16_t)v;
+        return t;
+    };
+    for (const Knob& k : knobs)
+        for (unsigned v : k.values) {
+            const hpvcd_tuning t = with(cur, k.field, v);
+            if (std::memcmp(&t, &cur, sizeof(t)) == 0) continue;
+            double ms = 0.0;
+            if (hpvcd_status st = timed(t, &ms)) {
+                (void)hpvcd_gpu_set_tuning(g, &cur);
+                return st;
+            }
+            if (ms < cur_ms) {
+                cur = t;
+                cur_ms = ms;
+            }
+        }
+    if (hpvcd_status st = hpvcd_gpu_set_tuning(g, &cur)) return st;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_tuning_load(const char* path, const char* key, hpvcd_tuning* t) {
+    if (!path || !key || !t) return set_error(HPVCD_E_INVALID, "null argument");
+    FILE* f = std::fopen(path, "r");
+    if (!f) return set_error(HPVCD_E_INVALID, "tuning file cannot be read");
+    char line[256];
+    bool found = false;
+    while (std::fgets(line, sizeof(line), f)) {
+        char k[96];
+        unsigned v[8];
+        if (std::sscanf(line, "%95s %u %u %u %u %u %u %u %u", k, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
+                        &v[7]) != 9 || std::strcmp(k, key) != 0)
+            continue;
+        hpvcd_tuning c;
+        hpvcd_tuning_default(&c);
+        bool fits = true;
+        for (int i = 0; i < 8; i++) fits = fits && v[i] <= 65535u;
+        if (!fits) continue;
+        for (int i = 0; i < 4; i++) c.residual_block[i] = (uint16_t)v[i];
+        c.deblock_block = (uint16_t)v[4];
+        c.rgb_rows = (uint16_t)v[5];
+        c.intra_per_cu = (uint8_t)std::min(v[6], 255u);
+        c.sao_parts = (uint8_t)std::min(v[7], 255u);
+        if (hpvcd_tuning_check(&c) == HPVCD_OK) {
+            *t = c;  // the last valid line of the key wins
+            found = true;
+        }
+    }
+    std::fclose(f);
+    if (!found) return set_error(HPVCD_E_INVALID, "tuning file has no valid line for the key");
+    return HPVCD_OK;
+}
+
+hpvcd_status hpvcd_tuning_save(const char* path, const char* key, const hpvcd_tuning* t) {
+    if (!path || !key || !t) return set_error(HPVCD_E_INVALID, "null argument");
+    if (hpvcd_status st = hpvcd_tuning_check(t)) return st;
+    for (const char* c = key; *c; c++)
+        if (*c == ' ' || *c == '\t' || *c == '\n' || *c == '\r') return set_error(HPVCD_E_INVALID, "key has blanks");
+    if (!*key || std::strlen(key) > 95) return set_error(HPVCD_E_INVALID, "key length");
+    // keep every other key's lines, replace this key's
+    std::vector<std::string> keep;
+    if (FILE* f = std::fopen(path, "r")) {
+        char line[256];
+        while (std::fgets(line, sizeof(line), f)) {
+            char k[96];
+            if (std::sscanf(line, "%95s", k) == 1 && std::strcmp(k, key) == 0) continue;
+            keep.emplace_back(line);
+        }
+        std::fclose(f);
+    }
+    char mine[256];
+    std::snprintf(mine, sizeof(mine), "%s %u %u %u %u %u %u %u %u\n", key, t->residual_block[0], t->residual_block[1],
+                  t->residual_block[2], t->residual_block[3], t->deblock_block, t->rgb_rows, t->intra_per_cu,
+                  t->sao_parts);
+    keep.emplace_back(mine);
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "w");
+    if (!f) return set_error(HPVCD_E_INVALID, "tuning file cannot be written");
+    bool ok = true;
+    for (const std::string& l : keep) {
+        ok = ok && std::fputs(l.c_str(), f) >= 0;
+        if (!l.empty() && l.back() != '\n') ok = ok && std::fputc('\n', f) != EOF;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        return set_error(HPVCD_E_INVALID, "tuning file cannot be written");
+    }
     return HPVCD_OK;
 }
 
@@ -1689,7 +1893,7 @@ hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop) {
     t_launch_err = hipSuccess;
     if (mode == HPVCD_MODE_GRAPH) {
         for (auto& C : j->classes) {
-            if (!C.graph || C.graph_stop != stop) {
+            if (!C.graph || C.graph_stop != stop || C.graph_gen != j->g->tuning_gen) {
                 if (C.graph) { (void)hipGraphExecDestroy(C.graph); C.graph = nullptr; }
                 hipGraph_t gr;
                 HIP_TRY(hipStreamBeginCapture(j->g->stream, hipStreamCaptureModeThreadLocal));
@@ -1709,6 +1913,7 @@ hpvcd_status hpvcd_job_run(hpvcd_job* j, int mode, int stop) {
                 (void)hipGraphDestroy(gr);
                 if (e != hipSuccess) return set_error(HPVCD_E_DEVICE, "hipGraphInstantiate failed");
                 C.graph_stop = stop;
+                C.graph_gen = j->g->tuning_gen;
             }
             HIP_TRY(hipGraphLaunch(C.graph, j->g->stream));
         }

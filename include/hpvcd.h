@@ -212,6 +212,50 @@ hpvcd_status hpvcd_job_download_planes(hpvcd_job* j, uint32_t picture, uint16_t*
 hpvcd_status hpvcd_job_time(hpvcd_job* j, int mode, int warmup, int iters, double* ms_per_run,
                             double* kernel_ms);
 
+/* Launch shapes of the kernels that have a choice: the autotuner's knobs
+ * (no counterpart in the reference, whose decode is scalar).  Every value
+ * gives the same samples; only the speed differs.
+ *   residual_block[k]: K1 threads per workgroup for TBs of 4 << k
+ *                      (64, 128 or 256; block / n TBs per workgroup)
+ *   deblock_block:     K3 threads (edge segments) per workgroup (64, 128, 256)
+ *   rgb_rows:          K5 output rows per workgroup of 64 x 4 pixels (1, 2, 4)
+ *   intra_per_cu:      K2 workgroups per CU of the wavefront (0: as many as
+ *                      fit, else 1..8, capped by what fits)
+ *   sao_parts:         K4 workgroups per luma CTB in launches of fewer CTBs
+ *                      than 4 per CU (1, 2, 4) */
+typedef struct hpvcd_tuning {
+    uint16_t residual_block[4];
+    uint16_t deblock_block;
+    uint16_t rgb_rows;
+    uint8_t  intra_per_cu;
+    uint8_t  sao_parts;
+    uint8_t  _pad[2];
+} hpvcd_tuning;
+
+void hpvcd_tuning_default(hpvcd_tuning* t);
+/* HPVCD_E_INVALID (message names the field) for a value outside the lists. */
+hpvcd_status hpvcd_tuning_check(const hpvcd_tuning* t);
+/* The shapes a GPU's launches use; a job's captured graph is captured again
+ * on its next replay after a change.  hpvcd_gpu_open starts from the
+ * defaults, then from the entry for hpvcd_gpu_tuning_key() in the file that
+ * the environment variable HPVCD_TUNING_FILE names, when it has one. */
+hpvcd_status hpvcd_gpu_set_tuning(hpvcd_gpu* g, const hpvcd_tuning* t);
+hpvcd_status hpvcd_gpu_get_tuning(hpvcd_gpu* g, hpvcd_tuning* t);
+/* "<arch>:<CUs>cu" of the GPU, e.g. "gfx950:256cu" (at most 63 chars). */
+hpvcd_status hpvcd_gpu_tuning_key(hpvcd_gpu* g, char* key, size_t key_size);
+/* Autotuner: coordinate search over the knobs on a resident job.  Each knob
+ * in turn takes each of its values with the others at their best so far;
+ * a value is timed as the mean of `iters` batched runs (HIP events) after one
+ * warm-up run, and kept when it is faster.  The GPU keeps the best shapes
+ * (written to *best); best_ms is its time per run, default_ms that of the
+ * shapes the search started from. */
+hpvcd_status hpvcd_tune(hpvcd_job* j, int iters, hpvcd_tuning* best, double* best_ms, double* default_ms);
+/* Tuning cache: a text file of lines "<key> r4 r8 r16 r32 deblock rgb intra
+ * sao".  save replaces the key's line (or appends one); load fills *t from
+ * it: HPVCD_E_INVALID when the file has no valid line for the key. */
+hpvcd_status hpvcd_tuning_save(const char* path, const char* key, const hpvcd_tuning* t);
+hpvcd_status hpvcd_tuning_load(const char* path, const char* key, hpvcd_tuning* t);
+
 /* One-call convenience: upload, reconstruct (batched launches), download. */
 hpvcd_status hpvcd_gpu_decode_batch(hpvcd_gpu* g, const hpvcd_batch* b, void** outputs);
 

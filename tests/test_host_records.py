@@ -398,6 +398,40 @@ def test_hierarchical_gop_structure():
         H.RecordSet.generate("single512", gop=8)
 
 
+def test_tuning_defaults_checks_and_cache_file(tmp_path):
+    """Launch-shape knobs: defaults, the value lists hpvcd_tuning_check
+    accepts, and the cache file (one line per GPU key, the key's line
+    replaced on save, malformed or out-of-list lines skipped on load)."""
+    d = H.default_tuning()
+    assert d == {"residual_block": [256, 256, 256, 256], "deblock_block": 256, "rgb_rows": 4, "intra_per_cu": 0,
+                 "sao_parts": 4}
+    for bad in ({"residual_block": [256, 96, 256, 256]}, {"deblock_block": 32}, {"rgb_rows": 3},
+                {"intra_per_cu": 9}, {"sao_parts": 0}):
+        t = H.tuning_struct(bad)
+        assert N.lib().hpvcd_tuning_check(C.byref(t)) == 15, bad
+    f = str(tmp_path / "tune.txt")
+    with pytest.raises(H.HpvcdError):
+        H.load_tuning(f, "gfx950:256cu")
+    a = dict(d, residual_block=[64, 128, 256, 64], rgb_rows=2, intra_per_cu=2, sao_parts=1)
+    b = dict(d, deblock_block=128)
+    H.save_tuning(f, "gfx950:256cu", a)
+    H.save_tuning(f, "gfx942:304cu", b)
+    assert H.load_tuning(f, "gfx950:256cu") == a and H.load_tuning(f, "gfx942:304cu") == b
+    c = dict(a, rgb_rows=1)
+    H.save_tuning(f, "gfx950:256cu", c)
+    lines = open(f).read().splitlines()
+    assert len(lines) == 2 and H.load_tuning(f, "gfx950:256cu") == c
+    with open(f, "a") as fh:
+        fh.write("gfx950:256cu 64 64 64\n")                   # too few fields
+        fh.write("gfx950:256cu 64 64 64 64 100 4 0 4\n")      # deblock 100 is not a choice
+        fh.write("# a comment line\n")
+    assert H.load_tuning(f, "gfx950:256cu") == c
+    with pytest.raises(H.HpvcdError):
+        H.load_tuning(f, "gfx950:1cu")
+    with pytest.raises(H.HpvcdError):
+        H.save_tuning(f, "two words", a)
+
+
 def test_server_config_and_refusals():
     """The serving loop's defaults, and the arguments it refuses before any
     GPU is touched: the CPU_REF backend, zero caps, null arguments."""
