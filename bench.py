@@ -303,6 +303,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the other workloads' lines")
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="skip the end-to-end (records and HEIC bytes to image) figures")
     ap.add_argument("--serving-seconds", type=float, default=2.0,
                     help="length of each arrival-rate run of the serving figure (0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -357,10 +359,10 @@ def main() -> int:
             plain_ms, _ = job.time(N.MODE_PLAIN, warmup=1, iters=3, breakdown=False)
         job.close()
         print(f"[bench] events {ev_ms:.3f} ms/step, kernels {kms}, plain {plain_ms}", file=sys.stderr, flush=True)
-        e2e = end_to_end(H, N, g, rs, 5)
+        e2e = None if args.no_e2e else end_to_end(H, N, g, rs, 5)
         print(f"[bench] end to end {e2e}", file=sys.stderr, flush=True)
     heic = None
-    if H.gen_config(**H.WORKLOADS[args.workload]).inter == 0:
+    if not args.no_e2e and H.gen_config(**H.WORKLOADS[args.workload]).inter == 0:
         heic = heic_end_to_end(H, N, args.workload, 5, out_gpu[0])
         print(f"[bench] HEIC bytes -> image {heic}", file=sys.stderr, flush=True)
     with H.Gpu(0) as g:
@@ -452,8 +454,9 @@ def main() -> int:
         "bit_exact": bit_exact,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                        "kind": "replay of the resident job"},
-        "e2e": {**e2e, "value": round(mpix / (e2e["ms_per_step"] / 1e3), 2), "unit": "Mpixel/s",
-                "kind": "host records -> job creation (prep + pinned H2D) -> batched launch -> D2H to pinned"},
+        "e2e": None if e2e is None else {
+            **e2e, "value": round(mpix / (e2e["ms_per_step"] / 1e3), 2), "unit": "Mpixel/s",
+            "kind": "host records -> job creation (prep + pinned H2D) -> batched launch -> D2H to pinned"},
         "heic": None if heic is None else {
             **heic, "value": round(mpix / (heic["ms_per_image"] / 1e3), 2), "unit": "Mpixel/s",
             "kind": "HEIC file bytes -> hpvcd_decode_heic -> RGB image in host memory (container and CABAC parse "

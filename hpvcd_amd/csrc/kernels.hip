@@ -15,7 +15,7 @@
 //   k_sao<T>              band / edge offset, out of place from the
 //                         deblocked planes (reference: src/decode.rs:2258,
 //                         src/sao.rs:179-470)
-//   k_rgb<T>              YUV -> RGB8/RGB16 with crop and grid placement
+//   k_rgb<T, ORIENTED>    YUV -> RGB8/RGB16 with crop, grid placement and orientation
 //                         (reference: src/yuv.rs:187-396, src/lib.rs:1610)
 //
 // None of this is a dense contraction; all arithmetic is integer and stays
@@ -946,7 +946,15 @@ __global__ void __launch_bounds__(256) k_sao(const DevPic* __restrict__ pics, co
 // rows / (blockDim / 64), pictures).  The Q13 products fit 32 bits for bit depths up to 12
 // (|coefficient| < 2^15, sample < 2^12).
 // ---------------------------------------------------------------------------
-template <typename T>
+// Whether a picture's output orientation moves its pixels (the oriented
+// store of k_rgb); the other orientations store the canvas as it is.
+__host__ __device__ __forceinline__ bool orient_moves(int o) {
+    return o >= HPVCD_ORIENT_FLIP_H && o != HPVCD_ORIENT_TRANSPOSE && o != HPVCD_ORIENT_TRANSVERSE;
+}
+
+// ORIENTED: the launch holds a picture whose orientation moves pixels (the
+// per-pixel oriented store is compiled in only then).
+template <typename T, bool ORIENTED>
 __global__ void __launch_bounds__(256) k_rgb(const DevPic* __restrict__ pics, const T* __restrict__ pool,
                                              uint8_t* __restrict__ out_pool, uint32_t pic_base) {
     const DevPic& P = pics[pic_base + blockIdx.z];
@@ -1049,7 +1057,7 @@ __global__ void __launch_bounds__(256) k_rgb(const DevPic* __restrict__ pics, co
     }
     const bool wide = P.out_kind == HPVCD_OUT_RGB16 || P.out_kind == HPVCD_OUT_LUMA16;
     const int orient = P.orient;
-    if (orient >= HPVCD_ORIENT_FLIP_H && orient != HPVCD_ORIENT_TRANSPOSE && orient != HPVCD_ORIENT_TRANSVERSE) {
+    if (ORIENTED && orient_moves(orient)) {
         // oriented image (src/lib.rs:2065 rotate_luma, :2116 rotate_buf): each
         // pixel of the canvas goes to its place in the oriented buffer
         const int W = P.out_width, H = P.out_height, Y = P.dst_y + yy;
@@ -1139,7 +1147,8 @@ HPVCD_INST_INTRA(uint16_t, int32_t)
 #define HPVCD_INST_T(T)                                                                                          \
     template __global__ void k_deblock<T>(const DevPic*, const uint32_t*, T*, int, uint32_t, uint32_t);          \
     template __global__ void k_sao<T>(const DevPic*, const uint32_t*, const hpvcd_sao*, const T*, T*, uint32_t, int); \
-    template __global__ void k_rgb<T>(const DevPic*, const T*, uint8_t*, uint32_t);
+    template __global__ void k_rgb<T, false>(const DevPic*, const T*, uint8_t*, uint32_t);                       \
+    template __global__ void k_rgb<T, true>(const DevPic*, const T*, uint8_t*, uint32_t);
 HPVCD_INST_T(uint8_t)
 HPVCD_INST_T(uint16_t)
 

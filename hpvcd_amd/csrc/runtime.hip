@@ -1193,8 +1193,11 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
             const uint32_t qx = plain ? (uint32_t)((C.pics[p0].copy_w + 3) / 4) : C.max_rgb_qx;
             const uint32_t rows = plain ? (uint32_t)C.pics[p0].copy_h : C.max_rgb_rows;
             const uint32_t rr = tn.rgb_rows;
-            if (qx && rows)
-                k_rgb<T><<<dim3(cdiv(qx, 64), cdiv(rows, rr), pn), 64 * rr, 0, s>>>(C.d_pics, pb, d_out, p0);
+            bool oriented = false;
+            for (uint32_t p = p0; p < p1; p++) oriented = oriented || orient_moves(C.pics[p].orient);
+            const dim3 grid(cdiv(qx, 64), cdiv(rows, rr), pn);
+            if (qx && rows && oriented) k_rgb<T, true><<<grid, 64 * rr, 0, s>>>(C.d_pics, pb, d_out, p0);
+            else if (qx && rows) k_rgb<T, false><<<grid, 64 * rr, 0, s>>>(C.d_pics, pb, d_out, p0);
             mark(-1);
         }
     }
