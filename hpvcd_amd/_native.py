@@ -203,7 +203,7 @@ class GenConfig(C.Structure):
         ("conf_win", C.c_uint16 * 4),
         ("p_sao_merge", C.c_float),
         ("persistent_rice", C.c_uint8), ("ts_context", C.c_uint8),
-        ("bypass_alignment", C.c_uint8), ("_pad2", C.c_uint8),
+        ("bypass_alignment", C.c_uint8), ("wpp", C.c_uint8),
     ]
 
 

@@ -112,8 +112,9 @@ inline std::vector<uint8_t> nal_from_rbsp(uint8_t type, const std::vector<uint8_
 }
 
 // NAL payload (header included) -> RBSP without the emulation prevention
-// bytes; the two header bytes are kept at the front.
-inline std::vector<uint8_t> rbsp_from_nal(const uint8_t* nal, size_t n) {
+// bytes; the two header bytes are kept at the front.  `removed`, when given,
+// receives the NAL positions of the dropped bytes.
+inline std::vector<uint8_t> rbsp_from_nal(const uint8_t* nal, size_t n, std::vector<uint32_t>* removed = nullptr) {
     std::vector<uint8_t> out;
     out.reserve(n);
     int zeros = 0;
@@ -121,6 +122,7 @@ inline std::vector<uint8_t> rbsp_from_nal(const uint8_t* nal, size_t n) {
         const uint8_t b = nal[i];
         if (i >= 2 && zeros >= 2 && b == 3) {
             zeros = 0;
+            if (removed) removed->push_back((uint32_t)i);
             continue;
         }
         out.push_back(b);

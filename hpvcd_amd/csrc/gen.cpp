@@ -65,6 +65,7 @@ PicParams gen_params(const hpvcd_gen_config& c) {
     P.sao_luma = true;
     P.sao_chroma = c.chroma_idc != 0;
     P.amp = c.amp != 0;
+    P.wpp = c.wpp != 0;
     return P;
 }
 
@@ -127,7 +128,7 @@ public:
     RandomSource(const hpvcd_gen_config& c, uint64_t seed) : cfg(c), r{seed} {}
     Rng* rng() override { return &r; }
     void begin_slice(int) override {}
-    void begin_substream() override {}
+    void begin_substream(int) override {}
     int end_ctb(bool last_of_slice, bool) override { return last_of_slice ? 1 : 0; }
     // merged from the left or above CTB (where available) or drawn; offsets
     // |v| <= 7 with edge-offset signs fixed (src/decode.rs:1507-1614)
@@ -236,12 +237,18 @@ void gen_picture(const hpvcd_gen_config& cfg, uint32_t index, PicRecords& out, P
     g.set_layout(ctb_slice, T.tile_of);
     out.num_slices = ctb_slice.empty() ? 1 : ctb_slice[T.ts_to_rs[nctb - 1]] + 1;
     out.num_tiles = T.num_tiles();
+    // substreams: one per tile, and with WPP one per CTB row of a tile
+    auto sub_start = [&](int t) {
+        const uint32_t r = T.ts_to_rs[t];
+        if (t == 0 || T.tile_of[r] != T.tile_of[T.ts_to_rs[t - 1]]) return true;
+        return P.wpp && (r % (uint32_t)cols == 0 || T.tile_of[r - 1] != T.tile_of[r]);
+    };
     for (int ts = 0; ts < nctb; ts++) {
         const uint32_t rs_addr = T.ts_to_rs[ts];
         const bool slice_start = ts == 0 || ctb_slice[rs_addr] != ctb_slice[T.ts_to_rs[ts - 1]];
-        const bool tile_start = ts == 0 || T.tile_of[rs_addr] != T.tile_of[T.ts_to_rs[ts - 1]];
+        const bool tile_start = sub_start(ts);
         const bool slice_end = ts == nctb - 1 || ctb_slice[rs_addr] != ctb_slice[T.ts_to_rs[ts + 1]];
-        const bool tile_end = ts == nctb - 1 || T.tile_of[rs_addr] != T.tile_of[T.ts_to_rs[ts + 1]];
+        const bool tile_end = ts == nctb - 1 || sub_start(ts + 1);
         g.ctb((int)(rs_addr % cols), (int)(rs_addr / cols), slice_start, tile_start, slice_end, tile_end, cfg.qp);
     }
     g.finish();

@@ -57,9 +57,14 @@ public:
         out_.slices.emplace_back();
         open_substream();
     }
-    void begin_substream() override {
-        in_.begin_substream();
+    void begin_substream(int sync_row) override {
+        in_.begin_substream(sync_row);
         open_substream();
+        if (sync_row >= 0 && (size_t)sync_row < saved_.size()) c_ = saved_[(size_t)sync_row];
+    }
+    void wpp_save(int row) override {
+        if (saved_.size() <= (size_t)row) saved_.resize((size_t)row + 1);
+        saved_[(size_t)row] = c_;
     }
     int end_ctb(bool last_of_slice, bool last_of_tile) override {
         const int r = in_.end_ctb(last_of_slice, last_of_tile);
@@ -246,6 +251,7 @@ private:
     const PicParams P_;
     PicStream& out_;
     Contexts c_;
+    std::vector<Contexts> saved_;  // WPP storage per CTB row
     CabacEnc e_;
     BitWriter w_;
     int slice_qp_ = 26;
@@ -332,6 +338,7 @@ void gen_parameter_sets(const hpvcd_gen_config& c, Sps& s, Pps& p) {
     p.tile_cols = (int)T.col_bd.size() - 1;
     p.tile_rows = (int)T.row_bd.size() - 1;
     p.uniform_spacing = true;
+    p.entropy_sync = c.wpp != 0;
     p.lf_across_tiles = c.lf_across_tiles != 0;
     p.lf_across_slices = c.lf_across_slices != 0;
     p.deblocking_control = true;

@@ -408,7 +408,8 @@ hpvcd_status item_picture(const uint8_t* file, size_t n, const HeifItem& it, con
         if (nal.second < 2) return set_error(HPVCD_E_BITSTREAM, "HEIF sample contains an undersized NAL unit");
         const int type = (nal.first[0] >> 1) & 63;
         if (type > 31) continue;
-        cp.slices.push_back(rbsp_from_nal(nal.first, nal.second));
+        cp.removed.emplace_back();
+        cp.slices.push_back(rbsp_from_nal(nal.first, nal.second, &cp.removed.back()));
     }
     if (cp.slices.empty()) return set_error(HPVCD_E_BITSTREAM, "no VCL slice NAL found");
     if (cp.slices[0].size() < 3 || !(cp.slices[0][2] & 0x80))

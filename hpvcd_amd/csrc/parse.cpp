@@ -41,10 +41,14 @@ class CabacSource : public Source {
 public:
     CabacSource(const Pps& p, const PicParams& P) : pps_(p), P_(P) {}
 
-    void set_slice(const uint8_t* data, size_t len, const SliceHeader* h) {
+    // sub_starts: RBSP offsets of the slice's substreams after the first,
+    // from its entry points
+    void set_slice(const uint8_t* data, size_t len, const SliceHeader* h, std::vector<size_t> sub_starts) {
         data_ = data;
         len_ = len;
         h_ = h;
+        sub_starts_ = std::move(sub_starts);
+        substream_ = 0;
     }
     bool bad() const { return bad_ || d_.failed(); }
     const char* why() const { return d_.failed() && !bad_ ? "slice data ends inside a coding tree unit" : why_; }
@@ -54,9 +58,19 @@ public:
         c_.init(0, slice_qp);
         d_.start(data_, len_);
     }
-    void begin_substream() override {
+    void begin_substream(int sync_row) override {
         c_.init(0, qp_);
-        d_.init_engine();
+        if (sync_row >= 0 && (size_t)sync_row < saved_.size()) c_ = saved_[(size_t)sync_row];
+        // the engine starts at the substream's entry point when the slice
+        // header gives one, else where the previous substream ended
+        // (src/decode.rs:1184-1196)
+        const size_t k = substream_++;
+        if (k < sub_starts_.size() && sub_starts_[k] <= len_) d_.start(data_ + sub_starts_[k], len_ - sub_starts_[k]);
+        else d_.init_engine();
+    }
+    void wpp_save(int row) override {
+        if (saved_.size() <= (size_t)row) saved_.resize((size_t)row + 1);
+        saved_[(size_t)row] = c_;
     }
     int end_ctb(bool, bool last_of_tile) override {
         if (d_.terminate()) return 1;  // end_of_slice_segment_flag
@@ -190,6 +204,9 @@ private:
     const uint8_t* data_ = nullptr;
     size_t len_ = 0;
     Contexts c_;
+    std::vector<Contexts> saved_;  // WPP storage per CTB row
+    std::vector<size_t> sub_starts_;
+    size_t substream_ = 0;
     CabacDec d_;
     int qp_ = 26;
     bool bad_ = false;
@@ -236,6 +253,7 @@ PicParams params_of(const Sps& s, const Pps& p) {
     P.sao = s.sao;
     P.amp = s.amp;
     P.constrained_intra = p.constrained_intra;
+    P.wpp = p.entropy_sync;
     return P;
 }
 
@@ -257,7 +275,6 @@ hpvcd_status check_params(const Sps& s, const Pps& p) {
                   s.log2_min_pcm_cb < 3 || s.log2_max_pcm_cb > std::min(5, s.log2_ctb) ||
                   s.log2_min_pcm_cb > s.log2_max_pcm_cb))
         return set_error(HPVCD_E_PARAMSET, "PCM parameters");
-    if (p.entropy_sync) return set_error(HPVCD_E_UNSUPPORTED, "entropy coding sync (WPP) streams");
     if (p.init_qp < -6 * (s.bit_depth_luma - 8) || p.init_qp > 51) return set_error(HPVCD_E_PARAMSET, "init_qp");
     if (p.cb_qp_offset < -12 || p.cb_qp_offset > 12 || p.cr_qp_offset < -12 || p.cr_qp_offset > 12)
         return set_error(HPVCD_E_PARAMSET, "pps chroma QP offsets");
@@ -343,12 +360,40 @@ hpvcd_status parse_picture(const Sps& sps, const Pps& pps, const CodedPicture& c
         if (cb < -12 || cb > 12 || cr < -12 || cr > 12) return set_error(HPVCD_E_BITSTREAM, "slice chroma QP offsets");
         walker.set_chroma_offsets(cb, cr);
         const auto& rb = cp.slices[k];
-        src.set_slice(rb.data() + h.data_offset, rb.size() - h.data_offset, &h);
+        // entry points: byte sizes of the substreams in the NAL unit, where
+        // emulation prevention bytes count; as RBSP offsets from the slice data
+        std::vector<size_t> sub_starts;
+        {
+            const std::vector<uint32_t>& ep = cp.removed.size() > k ? cp.removed[k] : std::vector<uint32_t>();
+            auto nal_of = [&](size_t r) {  // NAL position of RBSP byte r
+                size_t n = r;
+                for (size_t i = 0; i < ep.size() && (size_t)ep[i] - i <= r; i++) n++;
+                return n;
+            };
+            auto rbsp_of = [&](size_t nal) {  // RBSP position of NAL byte nal
+                size_t c = 0;
+                while (c < ep.size() && ep[c] < nal) c++;
+                return nal - c;
+            };
+            size_t at = nal_of(h.data_offset);
+            for (uint32_t e : h.entry_points) {
+                at += e;
+                const size_t r = rbsp_of(at);
+                sub_starts.push_back(r >= h.data_offset ? r - h.data_offset : 0);
+            }
+        }
+        src.set_slice(rb.data() + h.data_offset, rb.size() - h.data_offset, &h, std::move(sub_starts));
+        // substreams: one per tile, and with WPP one per CTB row of a tile
+        auto sub_start = [&](int t) {
+            const uint32_t r = ts_to_rs[t];
+            if (t == 0 || tile_of[r] != tile_of[ts_to_rs[t - 1]]) return true;
+            return pps.entropy_sync && (r % (uint32_t)cols == 0 || tile_of[r - 1] != tile_of[r]);
+        };
         for (int ts = start[k]; ts < start[k + 1]; ts++) {
             const uint32_t rs = ts_to_rs[ts];
-            const bool first_tile = ts == start[k] || tile_of[rs] != tile_of[ts_to_rs[ts - 1]];
+            const bool first_tile = ts == start[k] || sub_start(ts);
             const bool last_slice = ts == start[k + 1] - 1;
-            const bool last_tile = ts == nctb - 1 || tile_of[rs] != tile_of[ts_to_rs[ts + 1]];
+            const bool last_tile = ts == nctb - 1 || sub_start(ts + 1);
             const int eos = walker.ctb((int)(rs % cols), (int)(rs / cols), ts == start[k], first_tile, last_slice,
                                        last_tile, qp);
             if (src.bad()) return set_error(HPVCD_E_CABAC_DESYNC, src.why());
@@ -515,7 +560,8 @@ extern "C" hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32
         if (n.second < 2) return set_error(HPVCD_E_BITSTREAM, "NAL unit header truncated");
         const uint8_t type = (n.first[0] >> 1) & 63;
         if ((n.first[0] & 1) || (n.first[1] >> 3)) continue;  // nuh_layer_id > 0
-        std::vector<uint8_t> rb = rbsp_from_nal(n.first, n.second);
+        std::vector<uint32_t> removed;
+        std::vector<uint8_t> rb = rbsp_from_nal(n.first, n.second, &removed);
         if (type == NAL_SPS) {
             if (int st = parse_sps(rb, sps)) return (hpvcd_status)st;
             have_sps = true;
@@ -534,6 +580,7 @@ extern "C" hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32
                 active.emplace_back(sps, pps);
             }
             coded.back().slices.push_back(std::move(rb));
+            coded.back().removed.push_back(std::move(removed));
         }
     }
     if (coded.empty()) return set_error(HPVCD_E_BITSTREAM, "no coded picture");

@@ -12,9 +12,11 @@
 namespace hpvcd {
 
 // One coded picture's slice segment NAL units, each as RBSP with its two
-// NAL header bytes at the front.
+// NAL header bytes at the front, and the NAL positions of the emulation
+// prevention bytes removed from each (entry points count them).
 struct CodedPicture {
     std::vector<std::vector<uint8_t>> slices;
+    std::vector<std::vector<uint32_t>> removed;
 };
 
 // A parsed picture: its records and the picture record's coding fields

@@ -60,7 +60,15 @@ int PicWalker::ctb(int rx, int ry, bool first_in_slice, bool first_in_tile, bool
         qp_y = slice_qp;
         src.begin_slice(slice_qp);
     } else if (first_in_tile) {
-        src.begin_substream();
+        // WPP: a CTB row that starts inside its tile takes the contexts the
+        // row above stored, when that row's upper-right CTB is in the same
+        // tile and slice (src/decode.rs:1230-1262)
+        int sync = -1;
+        const bool row_start = rx == 0 || ctb_tile[rs - 1] != cur_tile;
+        if (P.wpp && row_start && ry > 0 && ctb_tile[rs - ctb_cols] == cur_tile && rx + 1 < ctb_cols &&
+            ctb_tile[rs - ctb_cols + 1] == cur_tile && ctb_slice[rs - ctb_cols + 1] == cur_slice)
+            sync = ry - 1;
+        src.begin_substream(sync);
     }
     // src/decode.rs:776 slice start, :1262 tile start: qPY_PREV = SliceQpY
     if (first_in_slice || first_in_tile) qp_prev = slice_qp;
@@ -71,6 +79,9 @@ int PicWalker::ctb(int rx, int ry, bool first_in_slice, bool first_in_tile, bool
         src.sao(out.sao[rs], l, u, l ? &out.sao[rs - 1] : nullptr, u ? &out.sao[rs - ctb_cols] : nullptr);
     }
     coding_quadtree(rx << P.log2_ctb, ry << P.log2_ctb, P.log2_ctb, 0);
+    // WPP storage after the second CTB of the row in its tile (src/decode.rs:1269)
+    if (P.wpp && rx >= 1 && ctb_tile[rs - 1] == cur_tile && (rx == 1 || ctb_tile[rs - 2] != cur_tile))
+        src.wpp_save(ry);
     return src.end_ctb(last_in_slice, last_in_tile);
 }
 

@@ -43,6 +43,7 @@ struct PicParams {
     int chroma_qp_off[2] = {0, 0};   // PPS + slice offsets of Cb, Cr (dequantisation)
     bool sao = false, sao_luma = false, sao_chroma = false;
     bool amp = false, constrained_intra = false;
+    bool wpp = false;                // entropy_coding_sync_enabled_flag: a substream per CTB row of a tile
 };
 
 constexpr uint8_t kModePlanar = 0, kModeDC = 1;
@@ -68,7 +69,13 @@ struct Rng {
 struct Source {
     virtual ~Source() = default;
     virtual void begin_slice(int slice_qp) = 0;
-    virtual void begin_substream() = 0;   // a new tile (not the slice's first)
+    // A new substream that is not the slice's first: a tile, or a CTB row of
+    // a tile with WPP.  The contexts start from their initial values, or,
+    // when sync_row >= 0, from the ones wpp_save stored for that CTB row
+    // (9.3.1: the state after the second CTB of the row above).
+    virtual void begin_substream(int sync_row) = 0;
+    // WPP storage after the second CTB of a CTB row of a tile (9.3.2.4)
+    virtual void wpp_save(int row) { (void)row; }
     // end_of_slice_segment_flag / end_of_subset_one_bit after a CTB
     virtual int end_ctb(bool last_of_slice, bool last_of_tile) = 0;
     virtual void sao(hpvcd_sao& s, bool left_avail, bool up_avail, const hpvcd_sao* left, const hpvcd_sao* up) = 0;
@@ -119,7 +126,9 @@ public:
         chroma_off[1] = cr;
     }
     // one CTB (SAO, coding quadtree, end flags); first_in_slice / first_in_tile
-    // reset the QP predictor; returns the end_of_slice_segment_flag
+    // (the first CTB of a substream: a tile, or with WPP a CTB row of a tile)
+    // reset the QP predictor; last_in_tile ends a substream; returns the
+    // end_of_slice_segment_flag
     int ctb(int rx, int ry, bool first_in_slice, bool first_in_tile, bool last_in_slice, bool last_in_tile,
             int slice_qp);
     // cells (QP, slice, tile packed), TB order, inter bS

@@ -48,8 +48,8 @@ const char* hpvcd_last_error(void);
 const char* hpvcd_version(void);
 
 /* ------------------------------------------------------------------------ */
-/* Seeded syntax-level generator: stands in for the CABAC parse and emits   */
-/* conformant reconstruction records for synthetic pictures.                */
+/* Seeded syntax-level generator: conformant reconstruction records for     */
+/* synthetic pictures (and, below, the same pictures as coded HEVC / HEIC). */
 /* ------------------------------------------------------------------------ */
 typedef struct hpvcd_gen_config {
     int32_t  tile_width, tile_height;   /* coded size of each picture */
@@ -104,7 +104,7 @@ typedef struct hpvcd_gen_config {
     uint8_t  persistent_rice;           /* persistent_rice_adaptation_enabled_flag */
     uint8_t  ts_context;                /* transform_skip_context_enabled_flag */
     uint8_t  bypass_alignment;          /* cabac_bypass_alignment_enabled_flag */
-    uint8_t  _pad2;
+    uint8_t  wpp;                       /* entropy_coding_sync_enabled_flag: a substream per CTB row */
 } hpvcd_gen_config;
 
 typedef struct hpvcd_records hpvcd_records;   /* owned record set */

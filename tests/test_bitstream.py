@@ -56,6 +56,16 @@ CASES = {
     "qp51": dict(qp=51),
     "qp0": dict(qp=0, p_level_geom=0.01),
     "full_range_601": dict(full_range=1, matrix=6),
+    # WPP: a substream per CTB row, contexts (and Rice statistics) synced
+    # from the state after the second CTB of the row above
+    # (src/decode.rs:1230-1275)
+    "wpp": dict(wpp=1),
+    "wpp_tiles": dict(wpp=1, tile_cols=3, tile_rows=2),
+    "wpp_slices_mid_row": dict(wpp=1, num_slices=3),
+    "wpp_slices_tiles": dict(wpp=1, num_slices=5, tile_cols=2, tile_rows=2, random_slice_params=1),
+    "wpp_rice_pcm_qp": dict(wpp=1, persistent_rice=1, pcm=1, pcm_bit_depth=7, p_pcm=0.2, cu_qp_delta=1,
+                            diff_cu_qp_delta_depth=1, qp_delta_max=10),
+    "wpp_one_ctb_tiles": dict(wpp=1, tile_cols=4, tile_rows=1),
 }
 
 ARRAYS = ("tbs", "coeffs", "sao", "cell_info", "scaling", "pus", "refs")
@@ -78,6 +88,21 @@ def test_parse_of_generated_stream_equals_records(name):
     assert g.raw("pictures") == p.raw("pictures")
     assert g.raw("outputs") == p.raw("outputs")
     assert N.lib().hpvcd_validate_batch(C.byref(p.batch)) == 0
+
+
+def test_wpp_entry_points_over_escaped_slice_data():
+    """Substreams start at their entry points, which count the emulation
+    prevention bytes of the NAL unit (7.3.8.1, src/decode.rs:1184-1196): a
+    WPP picture whose slice data holds such a byte parses to its records."""
+    import heif_build as B
+
+    params = dict(H.WORKLOADS["single512"], tile_width=1024, tile_height=512, out_width=1024, out_height=512,
+                  wpp=1, qp=22, p_sig=0.5, base_seed=26)
+    bs, g, p = _roundtrip(params)
+    _, _, _, pics = B.split_annexb(bs)
+    assert sum(n.count(b"\x00\x00\x03") for n in pics[0]) >= 1
+    for a in ARRAYS:
+        assert g.raw(a) == p.raw(a), a
 
 
 def test_grid_pictures_parse_to_the_generated_tiles():

@@ -31,6 +31,8 @@ CASES = {
     "slices_tiles_pcm_qp": (dict(H.WORKLOADS["single512"], num_slices=3, tile_cols=2, tile_rows=2, pcm=1, p_pcm=0.2,
                                  pcm_bit_depth=7, random_slice_params=1, cu_qp_delta=1, sign_hiding=1,
                                  base_seed=213), -1),
+    "wpp_grid2x2": (dict(H.WORKLOADS["grid4032"], grid_cols=2, grid_rows=2, out_width=1024, out_height=1024, wpp=1,
+                         persistent_rice=1, base_seed=217), -1),
 }
 
 KIND = {-1: {(False, False): N.OUT_RGB8, (False, True): N.OUT_RGB16, (True, False): N.OUT_LUMA8,
