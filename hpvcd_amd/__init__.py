@@ -199,6 +199,10 @@ class RecordSet:
     def output_buffers(self) -> List[np.ndarray]:
         bufs = []
         for o in self.outputs():
+            if o.kind in N.OUT_YUV:
+                dt = np.uint16 if (o.kind - N.OUT_YUV420_8) & 1 else np.uint8
+                bufs.append(np.zeros(int(N.lib().hpvcd_output_bytes(C.byref(o))) // np.dtype(dt).itemsize, dt))
+                continue
             ch = 1 if o.kind in (N.OUT_LUMA8, N.OUT_LUMA16) else 3
             dt = np.uint16 if o.kind in (N.OUT_RGB16, N.OUT_LUMA16) else np.uint8
             bufs.append(np.zeros((o.height, o.width, ch), dtype=dt))
@@ -433,6 +437,27 @@ class Job:
         return ms.value, {N.KERNEL_NAMES[i]: km[i] for i in range(len(N.KERNEL_NAMES))} if breakdown else None
 
 
+def raw_array(img: N.Image) -> np.ndarray:
+    """A copy of a raw-plane image's bytes as a flat array (planar Y, Cb, Cr)."""
+    dt = np.uint16 if img.sample_bytes == 2 else np.uint8
+    raw = (C.c_uint8 * img.pixels_len).from_address(img.pixels)
+    return np.frombuffer(raw, dtype=dt).copy()
+
+
+def yuv_planes(buf: np.ndarray, output) -> Tuple[np.ndarray, Optional[np.ndarray], Optional[np.ndarray]]:
+    """(y, cb, cr) views of a raw-plane output buffer (HPVCD_OUT_YUV* layout)."""
+    w, h = output.width, output.height
+    chroma = {N.OUT_YUV420_8: 1, N.OUT_YUV420_16: 1, N.OUT_YUV444_8: 3, N.OUT_YUV444_16: 3}.get(output.kind, 0)
+    y = buf[:w * h].reshape(h, w)
+    if not chroma:
+        return y, None, None
+    s = 2 if chroma == 1 else 1
+    cw, ch = (w + s - 1) // s, (h + s - 1) // s
+    cb = buf[w * h:w * h + cw * ch].reshape(ch, cw)
+    cr = buf[w * h + cw * ch:w * h + 2 * cw * ch].reshape(ch, cw)
+    return y, cb, cr
+
+
 def image_array(img: N.Image) -> np.ndarray:
     """A copy of a decoded image's pixels as (height, width, channels)."""
     dt = np.uint16 if img.sample_bytes == 2 else np.uint8
@@ -476,6 +501,27 @@ class Decoder:
         """A HEIC file's image as 8-bit RGB (hpvcd_decode_heic_rgb8 / Decoder::decode_rgb8)."""
         return self._heic(N.lib().hpvcd_decode_heic_rgb8, data)
 
+    def decode_yuv(self, data: bytes) -> Tuple[np.ndarray, Optional[np.ndarray], Optional[np.ndarray], dict]:
+        """A HEIC file's raw planes (hpvcd_decode_heic_yuv / Decoder::decode_yuv):
+        (y, cb, cr, info); cb and cr are None for 4:0:0; orientation reported, not applied."""
+        yv = N.Yuv()
+        st = N.lib().hpvcd_decode_heic_yuv(self._h, data, len(data), C.byref(yv))
+        try:
+            check(st)
+            dt = np.uint16 if yv.sample_bytes == 2 else np.uint8
+
+            def plane(ptr, w, h):
+                raw = (C.c_uint8 * (w * h * yv.sample_bytes)).from_address(ptr)
+                return np.frombuffer(raw, dtype=dt).reshape(h, w).copy()
+
+            y = plane(yv.y, yv.width, yv.height)
+            cb = plane(yv.cb, yv.chroma_width, yv.chroma_height) if yv.cb else None
+            cr = plane(yv.cr, yv.chroma_width, yv.chroma_height) if yv.cr else None
+            info = {f: getattr(yv, f) for f in ("chroma_idc", "bit_depth", "orientation", "matrix", "full_range")}
+            return y, cb, cr, info
+        finally:
+            N.lib().hpvcd_yuv_free(C.byref(yv))
+
     def _heic(self, fn, data: bytes) -> np.ndarray:
         img = N.Image()
         st = fn(self._h, data, len(data), C.byref(img))
@@ -492,7 +538,8 @@ class Decoder:
         st = N.lib().hpvcd_decode_records(self._h, C.byref(records.batch), imgs)
         try:
             check(st)
-            return [image_array(imgs[i]) for i in range(n)]
+            outs = records.outputs()
+            return [raw_array(imgs[i]) if outs[i].kind in N.OUT_YUV else image_array(imgs[i]) for i in range(n)]
         finally:
             for i in range(n):
                 N.lib().hpvcd_image_free(C.byref(imgs[i]))
@@ -505,7 +552,8 @@ class Server:
 
     def __init__(self, backend: int = N.BACKEND_GPU_BATCHED, device: int = 0, threads: int = 0,
                  max_batch_pictures: Optional[int] = None, max_batch_bytes: Optional[int] = None,
-                 hbm_reserve: Optional[int] = None, parse_workers: Optional[int] = None, **limits):
+                 hbm_reserve: Optional[int] = None, parse_workers: Optional[int] = None,
+                 batch_delay_us: Optional[int] = None, **limits):
         s = N.Settings()
         N.lib().hpvcd_settings_default(C.byref(s))
         s.backend, s.device, s.threads = backend, device, threads
@@ -516,7 +564,8 @@ class Server:
         c = N.ServerConfig()
         N.lib().hpvcd_server_config_default(C.byref(c))
         for k, v in (("max_batch_pictures", max_batch_pictures), ("max_batch_bytes", max_batch_bytes),
-                     ("hbm_reserve", hbm_reserve), ("parse_workers", parse_workers)):
+                     ("hbm_reserve", hbm_reserve), ("parse_workers", parse_workers),
+                     ("batch_delay_us", batch_delay_us)):
             if v is not None:
                 setattr(c, k, v)
         h = C.c_void_p()

@@ -50,6 +50,8 @@ OUT_RGB16 = 1
 OUT_RGB8_FROM16 = 2
 OUT_LUMA8 = 3
 OUT_LUMA16 = 4
+OUT_YUV420_8, OUT_YUV420_16, OUT_YUV444_8, OUT_YUV444_16, OUT_YUV400_8, OUT_YUV400_16 = 5, 6, 7, 8, 9, 10
+OUT_YUV = (OUT_YUV420_8, OUT_YUV420_16, OUT_YUV444_8, OUT_YUV444_16, OUT_YUV400_8, OUT_YUV400_16)
 
 ORIENT_NORMAL, ORIENT_FLIP_H, ORIENT_ROTATE_180, ORIENT_FLIP_V = 1, 2, 3, 4
 ORIENT_TRANSPOSE, ORIENT_ROTATE_90, ORIENT_TRANSVERSE, ORIENT_ROTATE_270 = 5, 6, 7, 8
@@ -155,7 +157,15 @@ class Batch(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_batch_pictures", C.c_uint32), ("parse_workers", C.c_uint32), ("max_batch_bytes", C.c_uint64),
-                ("hbm_reserve", C.c_uint64), ("_reserved", C.c_uint64)]
+                ("hbm_reserve", C.c_uint64), ("batch_delay_us", C.c_uint64)]
+
+
+class Yuv(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("chroma_width", C.c_uint32),
+                ("chroma_height", C.c_uint32), ("chroma_idc", C.c_uint8), ("bit_depth", C.c_uint8),
+                ("sample_bytes", C.c_uint8), ("orientation", C.c_uint8), ("matrix", C.c_uint8),
+                ("full_range", C.c_uint8), ("_pad", C.c_uint8 * 2), ("y", C.c_void_p), ("cb", C.c_void_p),
+                ("cr", C.c_void_p), ("bytes", C.c_size_t), ("_reserved", C.c_uint64)]
 
 
 class Tuning(C.Structure):
@@ -280,6 +290,8 @@ SIGNATURES = {
     "hpvcd_server_totals": (None, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "hpvcd_server_free": (None, [_P]),
     "hpvcd_decode_heic_rgb8": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(Image)]),
+    "hpvcd_decode_heic_yuv": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(Yuv)]),
+    "hpvcd_yuv_free": (None, [C.POINTER(Yuv)]),
     "hpvcd_tuning_default": (None, [C.POINTER(Tuning)]),
     "hpvcd_tuning_check": (C.c_int, [C.POINTER(Tuning)]),
     "hpvcd_gpu_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),

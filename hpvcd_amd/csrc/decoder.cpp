@@ -28,7 +28,9 @@ namespace {
 
 // oriented size of an output canvas (src/lib.rs:2007 oriented_dimensions)
 void oriented_size(const hpvcd_output& o, uint32_t* w, uint32_t* h) {
-    const bool swap = o.orientation >= HPVCD_ORIENT_TRANSPOSE && o.orientation <= HPVCD_ORIENT_ROTATE_270;
+    // raw planes keep the coded orientation
+    const bool swap = !HPVCD_OUT_IS_YUV(o.kind) && o.orientation >= HPVCD_ORIENT_TRANSPOSE &&
+                      o.orientation <= HPVCD_ORIENT_ROTATE_270;
     *w = (uint32_t)(swap ? o.height : o.width);
     *h = (uint32_t)(swap ? o.width : o.height);
 }
@@ -108,8 +110,8 @@ hpvcd_status hpvcd_decode_records(hpvcd_decoder* d, const hpvcd_batch* b, hpvcd_
         const hpvcd_output& O = b->outputs[o];
         hpvcd_image& I = images[o];
         oriented_size(O, &I.width, &I.height);
-        I.channels = (O.kind == HPVCD_OUT_LUMA8 || O.kind == HPVCD_OUT_LUMA16) ? 1 : 3;
-        I.sample_bytes = (O.kind == HPVCD_OUT_RGB16 || O.kind == HPVCD_OUT_LUMA16) ? 2 : 1;
+        I.channels = O.channels;
+        I.sample_bytes = O.bytes_per_sample;
         I.orientation = O.orientation;
         for (uint32_t p = 0; p < b->num_pictures; p++)
             if (b->pictures[p].out_index == o) {
@@ -148,7 +150,9 @@ hpvcd_status hpvcd_decode_records(hpvcd_decoder* d, const hpvcd_batch* b, hpvcd_
 
 namespace {
 
-hpvcd_status decode_heic_kind(hpvcd_decoder* d, const uint8_t* data, size_t size, int kind, hpvcd_image* img) {
+// out_kind receives the output kind the container chose
+hpvcd_status decode_heic_kind(hpvcd_decoder* d, const uint8_t* data, size_t size, int kind, hpvcd_image* img,
+                              int* out_kind = nullptr) {
     if (!d || !data || !img) return set_error(HPVCD_E_INVALID, "null argument");
     std::memset(img, 0, sizeof(*img));
     hpvcd_records* r = nullptr;
@@ -157,6 +161,7 @@ hpvcd_status decode_heic_kind(hpvcd_decoder* d, const uint8_t* data, size_t size
     if (st) return st;
     hpvcd_batch b;
     hpvcd_records_view(r, &b);
+    if (out_kind) *out_kind = b.num_outputs ? b.outputs[0].kind : -1;
     st = hpvcd_decode_records(d, &b, img);
     hpvcd_records_free(r);
     return st;
@@ -172,7 +177,47 @@ hpvcd_status hpvcd_decode_heic_rgb8(hpvcd_decoder* d, const uint8_t* data, size_
     return decode_heic_kind(d, data, size, HEIC_KIND_RGB8, img);
 }
 
+hpvcd_status hpvcd_decode_heic_yuv(hpvcd_decoder* d, const uint8_t* data, size_t size, hpvcd_yuv* yuv) {
+    if (!yuv) return set_error(HPVCD_E_INVALID, "null argument");
+    std::memset(yuv, 0, sizeof(*yuv));
+    hpvcd_image img;
+    int kind = -1;
+    hpvcd_status st = decode_heic_kind(d, data, size, HEIC_KIND_YUV, &img, &kind);
+    if (st) return st;
+    const int chroma = HPVCD_OUT_IS_YUV(kind) ? HPVCD_OUT_YUV_CHROMA(kind) : 0;
+    yuv->width = img.width;
+    yuv->height = img.height;
+    yuv->chroma_idc = (uint8_t)chroma;
+    if (chroma) {
+        const uint32_t sw = chroma == 1 ? 2 : 1, sh = chroma == 1 ? 2 : 1;
+        yuv->chroma_width = (img.width + sw - 1) / sw;
+        yuv->chroma_height = (img.height + sh - 1) / sh;
+    }
+    yuv->bit_depth = img.bit_depth;
+    yuv->sample_bytes = img.sample_bytes;
+    yuv->orientation = img.orientation;
+    yuv->matrix = img.matrix;
+    yuv->full_range = img.full_range;
+    uint8_t* base = static_cast<uint8_t*>(img.pixels);
+    yuv->y = base;
+    if (chroma) {
+        const size_t ly = (size_t)img.width * img.height * img.sample_bytes;
+        const size_t lc = (size_t)yuv->chroma_width * yuv->chroma_height * img.sample_bytes;
+        yuv->cb = base + ly;
+        yuv->cr = base + ly + lc;
+    }
+    yuv->bytes = img.pixels_len;
+    return HPVCD_OK;
+}
+
+void hpvcd_yuv_free(hpvcd_yuv* yuv) {
+    if (!yuv) return;
+    hpvcd_host_free(yuv->y);
+    std::memset(yuv, 0, sizeof(*yuv));
+}
+
 }  // extern "C"
 
 static_assert(sizeof(hpvcd_settings) == 56, "hpvcd_settings layout");
 static_assert(sizeof(hpvcd_image) == 40, "hpvcd_image layout");
+static_assert(sizeof(hpvcd_yuv) == 64, "hpvcd_yuv layout");

@@ -333,8 +333,13 @@ hpvcd_status gen_assemble(const hpvcd_gen_config& c, std::vector<PicRecords>& ti
         ho.width = c.out_width;
         ho.height = c.out_height;
         ho.kind = c.out_kind;
-        ho.channels = (c.out_kind == HPVCD_OUT_LUMA8 || c.out_kind == HPVCD_OUT_LUMA16) ? 1 : 3;
-        ho.bytes_per_sample = (c.out_kind == HPVCD_OUT_RGB16 || c.out_kind == HPVCD_OUT_LUMA16) ? 2 : 1;
+        if (HPVCD_OUT_IS_YUV(c.out_kind)) {
+            ho.channels = HPVCD_OUT_YUV_CHROMA(c.out_kind) ? 3 : 1;
+            ho.bytes_per_sample = HPVCD_OUT_YUV_WIDE(c.out_kind) ? 2 : 1;
+        } else {
+            ho.channels = (c.out_kind == HPVCD_OUT_LUMA8 || c.out_kind == HPVCD_OUT_LUMA16) ? 1 : 3;
+            ho.bytes_per_sample = (c.out_kind == HPVCD_OUT_RGB16 || c.out_kind == HPVCD_OUT_LUMA16) ? 2 : 1;
+        }
         ho.orientation = c.orientation;
         r->outputs.push_back(ho);
     }

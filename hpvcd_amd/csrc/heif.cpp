@@ -419,8 +419,11 @@ hpvcd_status item_picture(const uint8_t* file, size_t n, const HeifItem& it, con
 
 namespace {
 
-int default_kind(bool mono, bool deep, int want) {
+int default_kind(int chroma_idc, bool deep, int want) {
+    const bool mono = chroma_idc == 0;
     if (want == HEIC_KIND_RGB8) return deep ? HPVCD_OUT_RGB8_FROM16 : HPVCD_OUT_RGB8;
+    if (want == HEIC_KIND_YUV)
+        return (mono ? HPVCD_OUT_YUV400_8 : (chroma_idc == 3 ? HPVCD_OUT_YUV444_8 : HPVCD_OUT_YUV420_8)) + (deep ? 1 : 0);
     if (want >= 0) return want;
     return mono ? (deep ? HPVCD_OUT_LUMA16 : HPVCD_OUT_LUMA8) : (deep ? HPVCD_OUT_RGB16 : HPVCD_OUT_RGB8);
 }
@@ -431,8 +434,13 @@ hpvcd_output make_output(int w, int h, int kind, uint8_t orientation) {
     o.width = w;
     o.height = h;
     o.kind = (uint8_t)kind;
-    o.channels = (kind == HPVCD_OUT_LUMA8 || kind == HPVCD_OUT_LUMA16) ? 1 : 3;
-    o.bytes_per_sample = (kind == HPVCD_OUT_RGB16 || kind == HPVCD_OUT_LUMA16) ? 2 : 1;
+    if (HPVCD_OUT_IS_YUV(kind)) {
+        o.channels = HPVCD_OUT_YUV_CHROMA(kind) ? 3 : 1;
+        o.bytes_per_sample = HPVCD_OUT_YUV_WIDE(kind) ? 2 : 1;
+    } else {
+        o.channels = (kind == HPVCD_OUT_LUMA8 || kind == HPVCD_OUT_LUMA16) ? 1 : 3;
+        o.bytes_per_sample = (kind == HPVCD_OUT_RGB16 || kind == HPVCD_OUT_LUMA16) ? 2 : 1;
+    }
     o.orientation = orientation == HPVCD_ORIENT_NORMAL ? 0 : orientation;
     return o;
 }
@@ -478,7 +486,7 @@ hpvcd_status heic_records(const uint8_t* file, size_t n, const HeifLimits& L, in
             }
         }
         set_colour(p, m, full);
-        const int kind = default_kind(p.chroma_idc == 0, p.bit_depth_luma > 8, want_kind);
+        const int kind = default_kind(p.chroma_idc, p.bit_depth_luma > 8, want_kind);
         p.out_kind = (uint8_t)kind;
         outs.push_back(make_output(dw, dh, kind, hf.primary.orientation));
         inf.orientation = hf.primary.orientation;
@@ -507,7 +515,9 @@ hpvcd_status heic_records(const uint8_t* file, size_t n, const HeifLimits& L, in
         if (tile_w <= 0 || tile_h <= 0) {
             const HeifItem& t = hf.tiles[0];
             const bool same = t.display_w == hf.out_w && t.display_h == hf.out_h && (hf.cols > 1 || hf.rows > 1);
-            if (t.display_w && t.display_h && !same) {
+            // raw planes take the grid's share without the display fallback
+            // (src/lib.rs:1121-1127)
+            if (t.display_w && t.display_h && !same && want_kind != HEIC_KIND_YUV) {
                 tile_w = (int)t.display_w;
                 tile_h = (int)t.display_h;
             } else {
@@ -519,7 +529,8 @@ hpvcd_status heic_records(const uint8_t* file, size_t n, const HeifLimits& L, in
         const bool full = have0 ? s0.full_range : true;
         const bool mono = have0 && s0.chroma_format_idc == 0;
         const bool deep = have0 && s0.bit_depth_luma > 8;
-        const int kind = default_kind(mono, deep, want_kind);
+        const int chroma0 = have0 ? s0.chroma_format_idc : 1;
+        const int kind = default_kind(chroma0, deep, want_kind);
         const size_t nt = hf.tiles.size();
         std::vector<ParsedPicture> all(nt);
         std::vector<hpvcd_status> sts(nt, HPVCD_OK);
@@ -544,7 +555,8 @@ hpvcd_status heic_records(const uint8_t* file, size_t n, const HeifLimits& L, in
             }
             hpvcd_picture& p = all[i].pic;
             // a tile of another format than the first is not composed
-            if ((p.chroma_idc == 0) != mono || (p.bit_depth_luma > 8) != deep) {
+            if ((p.chroma_idc == 0) != mono || (p.bit_depth_luma > 8) != deep ||
+                (HPVCD_OUT_IS_YUV(kind) && p.chroma_idc != chroma0)) {
                 inf.failed_tiles++;
                 continue;
             }

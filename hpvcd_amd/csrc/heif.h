@@ -50,6 +50,7 @@ struct HeicInfo {
 // decode_heic_rgb8, or an explicit HPVCD_OUT_* kind
 constexpr int HEIC_KIND_DEFAULT = -1;
 constexpr int HEIC_KIND_RGB8 = -2;
+constexpr int HEIC_KIND_YUV = -3;   // raw planes (decode_yuv)
 
 uint8_t canonical_matrix(int m);
 hpvcd_status check_image(const HeifLimits& L, uint64_t w, uint64_t h);

@@ -958,6 +958,7 @@ template <typename T, bool ORIENTED>
 __global__ void __launch_bounds__(256) k_rgb(const DevPic* __restrict__ pics, const T* __restrict__ pool,
                                              uint8_t* __restrict__ out_pool, uint32_t pic_base) {
     const DevPic& P = pics[pic_base + blockIdx.z];
+    if (HPVCD_OUT_IS_YUV(P.out_kind)) return;  // raw planes: k_yuv
     const int qx = (P.copy_w + 3) >> 2;
     const int xq = (int)(blockIdx.x * 64 + (threadIdx.x & 63));
     const int yy = (int)(blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6));
@@ -1117,6 +1118,74 @@ __global__ void __launch_bounds__(256) k_rgb(const DevPic* __restrict__ pics, co
     }
 }
 
+// Rows of a picture's raw-plane copy (src/lib.rs:143 plane_window, :882
+// stitch_native_yuv_band): its luma rows, then its Cb and Cr rows.  The
+// chroma copy of a grid tile lands at (col * tcw, row * tch), tcw =
+// ceil(max(win_w, copy_w) / sub_w), clipped to the output's chroma planes.
+struct YuvCopy {
+    int cdx, cdy, ccw, cch, cwo, cho;
+};
+__host__ __device__ __forceinline__ YuvCopy yuv_copy(const DevPic& P) {
+    YuvCopy c{0, 0, 0, 0, 0, 0};
+    if (P.chroma_idc == 0 || P.win_w <= 0 || P.win_h <= 0) return c;
+    const int sw = 1 << P.sub_w_log2, sh = 1 << P.sub_h_log2;
+    c.cwo = (P.out_width + sw - 1) / sw;
+    c.cho = (P.out_height + sh - 1) / sh;
+    const int tcw = ((P.win_w > P.copy_w ? P.win_w : P.copy_w) + sw - 1) / sw;
+    const int tch = ((P.win_h > P.copy_h ? P.win_h : P.copy_h) + sh - 1) / sh;
+    c.cdx = (P.dst_x / P.win_w) * tcw;
+    c.cdy = (P.dst_y / P.win_h) * tch;
+    c.ccw = c.cwo - c.cdx < tcw ? c.cwo - c.cdx : tcw;
+    c.cch = c.cho - c.cdy < tch ? c.cho - c.cdy : tch;
+    if (c.ccw < 0 || P.copy_w <= 0) c.ccw = 0;
+    if (c.cch < 0 || P.copy_h <= 0) c.cch = 0;
+    return c;
+}
+
+// K5 for raw planes (Decoder::decode_yuv, src/lib.rs:762): coded sample
+// values copied to planar Y, Cb, Cr outputs, source coordinates clamped to
+// the coded plane; 4 samples of a row per thread (grid: quads of a row / 64,
+// rows / (blockDim / 64), pictures).
+template <typename T>
+__global__ void __launch_bounds__(256) k_yuv(const DevPic* __restrict__ pics, const T* __restrict__ pool,
+                                             uint8_t* __restrict__ out_pool, uint32_t pic_base) {
+    const DevPic& P = pics[pic_base + blockIdx.z];
+    if (!HPVCD_OUT_IS_YUV(P.out_kind)) return;
+    const int xq = (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+    const int r = (int)(blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const YuvCopy c = yuv_copy(P);
+    int plane, y, n, dx, dy, sw_, sh_, x0, y0, ow;
+    size_t obase;
+    const size_t luma_px = (size_t)P.out_width * P.out_height;
+    if (r < P.copy_h) {
+        plane = 0; y = r; n = P.copy_w; dx = P.dst_x; dy = P.dst_y; sw_ = P.w; sh_ = P.h; ow = P.out_width;
+        x0 = P.crop_left < P.w - 1 ? P.crop_left : P.w - 1;
+        y0 = P.crop_top < P.h - 1 ? P.crop_top : P.h - 1;
+        obase = 0;
+    } else if (r < P.copy_h + 2 * c.cch) {
+        const int k = (r - P.copy_h) / c.cch;
+        plane = 1 + k; y = r - P.copy_h - k * c.cch; n = c.ccw; dx = c.cdx; dy = c.cdy; sw_ = P.cw; sh_ = P.ch;
+        ow = c.cwo;
+        const int cl = P.crop_left >> P.sub_w_log2, ct = P.crop_top >> P.sub_h_log2;
+        x0 = cl < P.cw - 1 ? cl : P.cw - 1;
+        y0 = ct < P.ch - 1 ? ct : P.ch - 1;
+        obase = luma_px + (size_t)k * c.cwo * c.cho;
+    } else {
+        return;
+    }
+    const int x = xq * 4;
+    if (x >= n) return;
+    const int sy = y0 + y < sh_ - 1 ? y0 + y : sh_ - 1;
+    const T* src = pool + P.plane_off[plane] + (size_t)sy * sw_;
+    T* dst = reinterpret_cast<T*>(out_pool + P.out_off) + obase + (size_t)(dy + y) * ow + dx + x;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (x + k < n) {
+            const int sx = x0 + x + k < sw_ - 1 ? x0 + x + k : sw_ - 1;
+            dst[k] = src[sx];
+        }
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1148,7 +1217,8 @@ HPVCD_INST_INTRA(uint16_t, int32_t)
     template __global__ void k_deblock<T>(const DevPic*, const uint32_t*, T*, int, uint32_t, uint32_t);          \
     template __global__ void k_sao<T>(const DevPic*, const uint32_t*, const hpvcd_sao*, const T*, T*, uint32_t, int); \
     template __global__ void k_rgb<T, false>(const DevPic*, const T*, uint8_t*, uint32_t);                       \
-    template __global__ void k_rgb<T, true>(const DevPic*, const T*, uint8_t*, uint32_t);
+    template __global__ void k_rgb<T, true>(const DevPic*, const T*, uint8_t*, uint32_t);                        \
+    template __global__ void k_yuv<T>(const DevPic*, const T*, uint8_t*, uint32_t);
 HPVCD_INST_T(uint8_t)
 HPVCD_INST_T(uint16_t)
 

@@ -143,12 +143,23 @@ hpvcd_status validate_picture(const hpvcd_batch& b, uint32_t p) {
         return set_error(HPVCD_E_INVALID, "scaling_index");
     if (P.out_index >= b.num_outputs) return set_error(HPVCD_E_INVALID, "out_index");
     const hpvcd_output& O = b.outputs[P.out_index];
-    if (P.copy_w < 0 || P.copy_h < 0 || P.copy_w > P.win_w || P.copy_h > P.win_h || P.dst_x < 0 || P.dst_y < 0 ||
-        (int64_t)P.dst_x + P.copy_w > O.width || (int64_t)P.dst_y + P.copy_h > O.height)
+    const bool yuv = HPVCD_OUT_IS_YUV(O.kind);
+    // raw planes may copy past the window (clamped to the coded plane)
+    if (P.copy_w < 0 || P.copy_h < 0 || (!yuv && (P.copy_w > P.win_w || P.copy_h > P.win_h)) || P.dst_x < 0 ||
+        P.dst_y < 0 || (int64_t)P.dst_x + P.copy_w > O.width || (int64_t)P.dst_y + P.copy_h > O.height)
         return set_error(HPVCD_E_INVALID, "picture window placement outside its output image");
+    if (yuv) {
+        if (HPVCD_OUT_YUV_CHROMA(O.kind) != P.chroma_idc)
+            return set_error(HPVCD_E_INVALID, "raw-plane output of another chroma format");
+        if ((HPVCD_OUT_YUV_WIDE(O.kind) != 0) != (P.bit_depth_luma > 8) ||
+            (P.chroma_idc && (P.bit_depth_chroma > 8) != (P.bit_depth_luma > 8)))
+            return set_error(HPVCD_E_INVALID, "raw-plane output sample size");
+        if (O.channels != (P.chroma_idc ? 3 : 1)) return set_error(HPVCD_E_INVALID, "output channels");
+        if (P.win_w <= 0 || P.win_h <= 0) return set_error(HPVCD_E_INVALID, "picture window");
+    }
     const bool luma_out = O.kind == HPVCD_OUT_LUMA8 || O.kind == HPVCD_OUT_LUMA16;
     if (luma_out && P.chroma_idc != 0) return set_error(HPVCD_E_INVALID, "luma output of a colour picture");
-    if (O.channels != (luma_out ? 1 : 3)) return set_error(HPVCD_E_INVALID, "output channels");
+    if (!yuv && O.channels != (luma_out ? 1 : 3)) return set_error(HPVCD_E_INVALID, "output channels");
     bool wide = O.kind == HPVCD_OUT_RGB16 || O.kind == HPVCD_OUT_LUMA16;
     if (wide && P.bit_depth_luma == 8) return set_error(HPVCD_E_INVALID, "16-bit output of an 8-bit picture");
     int sw = sub_w_of(P.chroma_idc), sh = sub_h_of(P.chroma_idc);
@@ -223,7 +234,7 @@ hpvcd_status validate_batch(const hpvcd_batch& b) {
         const hpvcd_output& O = b.outputs[o];
         if (O.width <= 0 || O.height <= 0 || O.width > 65536 || O.height > 65536)
             return set_error(HPVCD_E_BAD_DIMENSIONS, "output image size");
-        if (O.kind > HPVCD_OUT_LUMA16) return set_error(HPVCD_E_INVALID, "output kind");
+        if (O.kind > HPVCD_OUT_YUV400_16) return set_error(HPVCD_E_INVALID, "output kind");
         if (O.orientation > HPVCD_ORIENT_ROTATE_270) return set_error(HPVCD_E_INVALID, "output orientation");
     }
     // pictures are checked on the host threads; the first failing picture
@@ -349,6 +360,12 @@ struct UninitAlloc : std::allocator<T> {
 };
 template <typename T> using PodVec = std::vector<T, UninitAlloc<T>>;
 
+// rows of a picture's raw-plane copy (zero for the other output kinds)
+inline uint32_t yuv_rows(const DevPic& D) {
+    if (!HPVCD_OUT_IS_YUV(D.out_kind)) return 0;
+    return (uint32_t)(D.copy_h + 2 * yuv_copy(D).cch);
+}
+
 struct ClassJob {
     bool samp16 = false, res32 = false;
     std::vector<uint32_t> batch_pics;     // batch picture index per class-local picture
@@ -382,6 +399,7 @@ struct ClassJob {
     size_t plane_elems = 0, res_elems = 0;
     uint32_t max_luma_v = 0, max_luma_h = 0, max_chroma_v = 0, max_chroma_h = 0, max_plane = 0, max_copy = 0;
     uint32_t max_ctbs = 0, max_rgb_qx = 0, max_rgb_rows = 0;  // K4 blocks; K5 quads per row, rows
+    uint32_t max_yuv_rows = 0;                                  // rows of raw-plane copies (luma + 2 chroma)
     // device pointers
     DevPic* d_pics = nullptr;
     hpvcd_tb* d_tbs = nullptr;
@@ -885,6 +903,7 @@ hpvcd_status prepare_class(const hpvcd_batch& b, ClassJob& C, const std::vector<
         C.max_ctbs = std::max<uint32_t>(C.max_ctbs, (uint32_t)(D.ctb_cols * D.ctb_rows));
         C.max_rgb_qx = std::max<uint32_t>(C.max_rgb_qx, (uint32_t)((P.copy_w + 3) / 4));
         C.max_rgb_rows = std::max<uint32_t>(C.max_rgb_rows, (uint32_t)P.copy_h);
+        C.max_yuv_rows = std::max<uint32_t>(C.max_yuv_rows, yuv_rows(D));
     }
     std::vector<PicPrep>().swap(Q);
     C.pic_units_begin[np] = unit_base;
@@ -1193,11 +1212,20 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
             const uint32_t qx = plain ? (uint32_t)((C.pics[p0].copy_w + 3) / 4) : C.max_rgb_qx;
             const uint32_t rows = plain ? (uint32_t)C.pics[p0].copy_h : C.max_rgb_rows;
             const uint32_t rr = tn.rgb_rows;
-            bool oriented = false;
-            for (uint32_t p = p0; p < p1; p++) oriented = oriented || orient_moves(C.pics[p].orient);
+            bool oriented = false, rgb = false, raw = false;
+            for (uint32_t p = p0; p < p1; p++) {
+                const bool y = HPVCD_OUT_IS_YUV(C.pics[p].out_kind);
+                raw = raw || y;
+                rgb = rgb || !y;
+                oriented = oriented || (!y && orient_moves(C.pics[p].orient));
+            }
             const dim3 grid(cdiv(qx, 64), cdiv(rows, rr), pn);
-            if (qx && rows && oriented) k_rgb<T, true><<<grid, 64 * rr, 0, s>>>(C.d_pics, pb, d_out, p0);
-            else if (qx && rows) k_rgb<T, false><<<grid, 64 * rr, 0, s>>>(C.d_pics, pb, d_out, p0);
+            if (rgb && qx && rows && oriented) k_rgb<T, true><<<grid, 64 * rr, 0, s>>>(C.d_pics, pb, d_out, p0);
+            else if (rgb && qx && rows) k_rgb<T, false><<<grid, 64 * rr, 0, s>>>(C.d_pics, pb, d_out, p0);
+            // raw planes (decode_yuv outputs): luma rows, then Cb and Cr rows
+            const uint32_t yrows = plain ? yuv_rows(C.pics[p0]) : C.max_yuv_rows;
+            if (raw && qx && yrows)
+                k_yuv<T><<<dim3(cdiv(qx, 64), cdiv(yrows, rr), pn), 64 * rr, 0, s>>>(C.d_pics, pb, d_out, p0);
             mark(-1);
         }
     }
@@ -1294,6 +1322,13 @@ hpvcd_status hpvcd_validate_batch(const hpvcd_batch* b) {
 
 size_t hpvcd_output_bytes(const hpvcd_output* o) {
     if (!o) return 0;
+    if (HPVCD_OUT_IS_YUV(o->kind)) {
+        const int c = HPVCD_OUT_YUV_CHROMA(o->kind);
+        const size_t bps = HPVCD_OUT_YUV_WIDE(o->kind) ? 2 : 1;
+        const size_t sw = c == 1 ? 2 : 1, sh = c == 1 ? 2 : 1;
+        const size_t cw = c ? ((size_t)o->width + sw - 1) / sw : 0, ch = c ? ((size_t)o->height + sh - 1) / sh : 0;
+        return ((size_t)o->width * o->height + 2 * cw * ch) * bps;
+    }
     size_t bps = (o->kind == HPVCD_OUT_RGB16 || o->kind == HPVCD_OUT_LUMA16) ? 2 : 1;
     size_t ch = (o->kind == HPVCD_OUT_LUMA8 || o->kind == HPVCD_OUT_LUMA16) ? 1 : 3;
     return (size_t)o->width * o->height * ch * bps;
@@ -2077,9 +2112,9 @@ hpvcd_status hpvcd_prep_digest(const hpvcd_batch* b, int32_t threads, uint64_t* 
                 }
             vec(C.pic_units_begin); vec(C.act_list); vec(C.pic_act_begin); vec(C.pus); vec(C.mc_tasks);
             vec(C.pic_mc_begin); vec(C.wave_begin); vec(C.wave_tick_begin); vec(C.pic_is_ref);
-            const uint64_t sc[14] = {C.smap_words, C.rmap_words, C.plane_elems, C.res_elems, C.max_luma_v,
+            const uint64_t sc[15] = {C.smap_words, C.rmap_words, C.plane_elems, C.res_elems, C.max_luma_v,
                                      C.max_luma_h, C.max_chroma_v, C.max_chroma_h, C.max_plane, C.max_copy,
-                                     C.max_ctbs, C.max_rgb_qx, C.max_rgb_rows, C.sync_words};
+                                     C.max_ctbs, C.max_rgb_qx, C.max_rgb_rows, C.max_yuv_rows, C.sync_words};
             mix(sc, sizeof(sc));
         }
     delete j;

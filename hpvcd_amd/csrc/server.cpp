@@ -85,6 +85,7 @@ struct hpvcd_server {
     uint64_t next_id = 1;
     bool stop = false;
     int parse_threads = 1;
+    int parsing = 0;  // requests a parse worker holds
     // totals for hpvcd_server_stats
     uint64_t batches = 0, batched_requests = 0, batched_pictures = 0;
 
@@ -97,6 +98,7 @@ struct hpvcd_server {
                 if (stop && to_parse.empty()) return;
                 r = to_parse.front();
                 to_parse.pop_front();
+                parsing++;
             }
             hpvcd_records* rec = nullptr;
             hpvcd_status st = heic_records(r->file.data(), r->file.size(), limits_of(&s), parse_threads, r->kind,
@@ -121,6 +123,7 @@ struct hpvcd_server {
                 }
             }
             std::lock_guard<std::mutex> lock(mu);
+            parsing--;
             r->t_parsed = Clock::now();
             if (st) {
                 if (rec) hpvcd_records_free(rec);
@@ -132,8 +135,9 @@ struct hpvcd_server {
             } else {
                 r->rec = rec;
                 ready.push_back(r);
-                cv_ready.notify_one();
             }
+            // a new ready request, or one parse fewer for a waiting slot
+            cv_ready.notify_one();
         }
     }
 
@@ -199,6 +203,15 @@ struct hpvcd_server {
                 std::unique_lock<std::mutex> lock(mu);
                 if (busy.empty()) cv_ready.wait(lock, [&] { return stop || !ready.empty(); });
                 if (stop && ready.empty() && busy.empty()) return;
+                // a free slot waits a little for requests still in the parse
+                if (cfg.batch_delay_us && (int)busy.size() < 2 && !ready.empty()) {
+                    const auto until = ready.front()->t_parsed + std::chrono::microseconds(cfg.batch_delay_us);
+                    cv_ready.wait_until(lock, until, [&] {
+                        size_t pics = 0;
+                        for (const Request* q : ready) pics += q->pictures;
+                        return stop || (to_parse.empty() && parsing == 0) || pics >= cfg.max_batch_pictures;
+                    });
+                }
                 // continuous batching: everything parsed now, within the caps
                 size_t pics = 0, bytes = 0;
                 size_t free_b = 0, total_b = 0;

@@ -345,6 +345,31 @@ hpvcd_status hpvcd_decode_heic_rgb8(hpvcd_decoder* d, const uint8_t* data, size_
 /* Release an image's pixels (the Vec ownership hand-off of the reference). */
 void         hpvcd_image_free(hpvcd_image* img);
 
+/* Raw planes of a HEIC image (DecodedYuv, src/lib.rs:403): the coded sample
+ * values of the visible window (a grid stitched), in coded orientation (the
+ * orientation is reported, not applied), 8-bit samples as u8 and deeper ones
+ * as u16.  The three planes share one page-locked block the library owns
+ * until hpvcd_yuv_free. */
+typedef struct hpvcd_yuv {
+    uint32_t width, height;                 /* luma plane */
+    uint32_t chroma_width, chroma_height;   /* each chroma plane; 0 for 4:0:0 */
+    uint8_t  chroma_idc;                    /* 0, 1 (4:2:0) or 3 (4:4:4) */
+    uint8_t  bit_depth;
+    uint8_t  sample_bytes;                  /* 1 or 2 */
+    uint8_t  orientation;                   /* HPVCD_ORIENT_* of the image, not applied */
+    uint8_t  matrix, full_range;            /* colour signalling (as hpvcd_image) */
+    uint8_t  _pad[2];
+    void*    y;                             /* row-major planes, stride = plane width */
+    void*    cb;                            /* NULL for 4:0:0 */
+    void*    cr;
+    size_t   bytes;                         /* of the whole block */
+    uint64_t _reserved;
+} hpvcd_yuv;
+
+/* Decoder::decode_yuv (src/decoder.rs:229, src/lib.rs:762 decode_heic_yuv_with). */
+hpvcd_status hpvcd_decode_heic_yuv(hpvcd_decoder* d, const uint8_t* data, size_t size, hpvcd_yuv* yuv);
+void         hpvcd_yuv_free(hpvcd_yuv* yuv);
+
 /* The container half on its own: a HEIC file's primary image as records
  * (one output).  out_kind: -1 the kind hpvcd_decode_heic returns, -2 the
  * RGB8 kind of hpvcd_decode_heic_rgb8, else an HPVCD_OUT_* kind.  settings
@@ -371,7 +396,8 @@ typedef struct hpvcd_server_config {
     uint32_t parse_workers;        /* requests parsed at once, each on threads / parse_workers host threads */
     uint64_t max_batch_bytes;      /* estimated device bytes of one batch */
     uint64_t hbm_reserve;          /* free HBM held back from admission */
-    uint64_t _reserved;
+    uint64_t batch_delay_us;     /* how long a free launch slot waits for requests still being parsed
+                                      to join the batch, from the first ready request (0: no wait) */
 } hpvcd_server_config;
 
 /* Where one request's time went (milliseconds from submission). */

@@ -205,6 +205,26 @@ typedef struct hpvcd_sao {
 #define HPVCD_OUT_RGB8_FROM16 2 /* decode_rgb8 of a >8-bit picture: >> (bd-8) */
 #define HPVCD_OUT_LUMA8  3
 #define HPVCD_OUT_LUMA16 4
+/* Raw planes (Decoder::decode_yuv, src/lib.rs:762-1207): Y (width x height),
+ * then for colour pictures Cb and Cr (ceil(width / sub_w) x ceil(height /
+ * sub_h) each), planar, the coded sample values (u8 for 8-bit pictures, u16
+ * above).  The orientation is reported, not applied.  A picture copies
+ * copy_w x copy_h luma samples from (crop_left, crop_top) of its coded plane
+ * to (dst_x, dst_y); its chroma copy starts at (crop_left / sub_w, crop_top /
+ * sub_h) and lands at (col * tcw, row * tch) with col = dst_x / win_w, row =
+ * dst_y / win_h, tcw = ceil(max(win_w, copy_w) / sub_w) and tch likewise,
+ * at most tcw x tch samples; source coordinates clamp to the coded plane
+ * (src/lib.rs:143 plane_window, :882 stitch_native_yuv_band). */
+#define HPVCD_OUT_YUV420_8  5
+#define HPVCD_OUT_YUV420_16 6
+#define HPVCD_OUT_YUV444_8  7
+#define HPVCD_OUT_YUV444_16 8
+#define HPVCD_OUT_YUV400_8  9
+#define HPVCD_OUT_YUV400_16 10
+#define HPVCD_OUT_IS_YUV(k) ((k) >= HPVCD_OUT_YUV420_8 && (k) <= HPVCD_OUT_YUV400_16)
+/* chroma_format_idc of a YUV kind */
+#define HPVCD_OUT_YUV_CHROMA(k) ((k) <= HPVCD_OUT_YUV420_16 ? 1 : ((k) <= HPVCD_OUT_YUV444_16 ? 3 : 0))
+#define HPVCD_OUT_YUV_WIDE(k) (((k) - HPVCD_OUT_YUV420_8) & 1)
 
 /* EXIF orientation of an output image (src/metadata.rs:31 Orientation,
  * applied after RGB conversion as src/lib.rs:2017 apply_orientation does).
@@ -224,7 +244,7 @@ typedef struct hpvcd_sao {
 typedef struct hpvcd_output {
     int32_t  width, height;   /* output canvas size (before orientation) */
     uint8_t  kind;            /* HPVCD_OUT_* */
-    uint8_t  channels;        /* 3 or 1 */
+    uint8_t  channels;        /* 3 or 1 (YUV kinds: planes, 3 or 1) */
     uint8_t  bytes_per_sample;
     uint8_t  orientation;     /* HPVCD_ORIENT_* (0 = as decoded) */
 } hpvcd_output;

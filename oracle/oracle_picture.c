@@ -963,6 +963,59 @@ typedef struct {
     int err;
 } batch_job;
 
+/* Raw planes of one picture into its output (Decoder::decode_yuv,
+ * src/lib.rs:762-1207).  Single item: plane_window (src/lib.rs:143) of the
+ * coded planes at the conformance-window offset, dw x dh of the display size,
+ * source rows and columns clamped to the plane (the padding repeats the last
+ * sample of a row, the last row repeats).  Grid tile: stitch_native_yuv_band
+ * (src/lib.rs:882): luma at (col * tile_w, row * tile_h), chroma at
+ * (col * tile_cw, row * tile_ch) with tile_cw = ceil(tile_w / sub_w),
+ * copies clipped to the output planes, crop offsets clamped to the plane.
+ * In the records a single item is a picture whose window is the whole
+ * output; tcw below is tile_cw for a tile and ceil(dw / sub_w) for it. */
+static void oracle_picture_yuv(const hpvcd_batch* b, uint32_t p, const uint16_t* Y, const uint16_t* Cb,
+                               const uint16_t* Cr, void* out) {
+    const hpvcd_picture* P = &b->pictures[p];
+    const hpvcd_output* O = &b->outputs[P->out_index];
+    int wide = HPVCD_OUT_YUV_WIDE(O->kind);
+    uint8_t* o8 = (uint8_t*)out;
+    uint16_t* o16 = (uint16_t*)out;
+    int w = P->width, h = P->height, W = O->width, H = O->height;
+#define PUT(i, v) do { if (wide) o16[(i)] = (uint16_t)(v); else o8[(i)] = (uint8_t)(v); } while (0)
+    int x0 = P->crop_left < w - 1 ? P->crop_left : w - 1;
+    int y0 = P->crop_top < h - 1 ? P->crop_top : h - 1;
+    for (int r = 0; r < P->copy_h; r++) {
+        int sy = y0 + r < h - 1 ? y0 + r : h - 1;
+        for (int x = 0; x < P->copy_w; x++) {
+            int sx = x0 + x < w - 1 ? x0 + x : w - 1;
+            PUT((size_t)(P->dst_y + r) * W + P->dst_x + x, Y[(size_t)sy * w + sx]);
+        }
+    }
+    if (P->chroma_idc == 0 || P->win_w <= 0 || P->win_h <= 0) return;
+    int sw = sub_w_of(P->chroma_idc), sh = sub_h_of(P->chroma_idc);
+    int cw = (w + sw - 1) / sw, ch = (h + sh - 1) / sh;
+    int cwo = (W + sw - 1) / sw, cho = (H + sh - 1) / sh;
+    int tcw = ((P->win_w > P->copy_w ? P->win_w : P->copy_w) + sw - 1) / sw;
+    int tch = ((P->win_h > P->copy_h ? P->win_h : P->copy_h) + sh - 1) / sh;
+    int cdx = (P->dst_x / P->win_w) * tcw, cdy = (P->dst_y / P->win_h) * tch;
+    int ccw = cwo - cdx < tcw ? cwo - cdx : tcw, cch = cho - cdy < tch ? cho - cdy : tch;
+    if (P->copy_w <= 0 || P->copy_h <= 0 || ccw <= 0 || cch <= 0) return;
+    int cl = P->crop_left / sw < cw - 1 ? P->crop_left / sw : cw - 1;
+    int ct = P->crop_top / sh < ch - 1 ? P->crop_top / sh : ch - 1;
+    for (int k = 0; k < 2; k++) {
+        const uint16_t* C = k ? Cr : Cb;
+        size_t base = (size_t)W * H + (size_t)k * cwo * cho;
+        for (int r = 0; r < cch; r++) {
+            int sy = ct + r < ch - 1 ? ct + r : ch - 1;
+            for (int x = 0; x < ccw; x++) {
+                int sx = cl + x < cw - 1 ? cl + x : cw - 1;
+                PUT(base + (size_t)(cdy + r) * cwo + cdx + x, C[(size_t)sy * cw + sx]);
+            }
+        }
+    }
+#undef PUT
+}
+
 static int decode_one(const hpvcd_batch* b, uint32_t p, void** outputs, uint16_t** refp, int keep) {
     const hpvcd_picture* P = &b->pictures[p];
     const hpvcd_output* O = &b->outputs[P->out_index];
@@ -973,10 +1026,12 @@ static int decode_one(const hpvcd_batch* b, uint32_t p, void** outputs, uint16_t
     uint16_t* cr = malloc(nc * 2 + 2);
     int chn = (O->kind == HPVCD_OUT_LUMA8 || O->kind == HPVCD_OUT_LUMA16) ? 1 : 3;
     int bps = (O->kind == HPVCD_OUT_RGB16 || O->kind == HPVCD_OUT_LUMA16) ? 2 : 1;
-    uint8_t* win = malloc((size_t)P->win_w * P->win_h * chn * bps + 1);
+    int raw = HPVCD_OUT_IS_YUV(O->kind);
+    uint8_t* win = raw ? NULL : malloc((size_t)P->win_w * P->win_h * chn * bps + 1);
     int rc = decode_planes(b, p, ORACLE_STAGE_SAO, y, cb, cr, refp);
-    if (!rc) rc = oracle_picture_rgb(b, p, y, cb, cr, win);
-    if (!rc) {
+    if (!rc && raw && outputs[P->out_index]) oracle_picture_yuv(b, p, y, cb, cr, outputs[P->out_index]);
+    if (!rc && !raw) rc = oracle_picture_rgb(b, p, y, cb, cr, win);
+    if (!rc && !raw) {
         uint8_t* dst = (uint8_t*)outputs[P->out_index];
         size_t row = (size_t)P->copy_w * chn * bps;
         for (int r = 0; r < P->copy_h; r++)
@@ -1013,6 +1068,7 @@ static void* batch_worker(void* arg) {
  * decoded (the reference's match arms fall through to the identity). */
 static int orient_output(const hpvcd_output* O, void* buf) {
     int o = O->orientation;
+    if (HPVCD_OUT_IS_YUV(O->kind)) return 0;  /* raw planes keep the coded orientation */
     if (o < HPVCD_ORIENT_FLIP_H || o == HPVCD_ORIENT_TRANSPOSE || o == HPVCD_ORIENT_TRANSVERSE) return 0;
     size_t px = (size_t)O->width * O->height;
     size_t chn = (O->kind == HPVCD_OUT_LUMA8 || O->kind == HPVCD_OUT_LUMA16) ? 1 : 3;

@@ -75,3 +75,42 @@ def test_damaged_grid_tile_is_black():
     mask = np.ones(got.shape[:2], bool)
     mask[128:250, 128:256] = False
     assert np.array_equal(got[mask], ref[mask])
+
+
+YUV_CASES = {
+    "single512": dict(H.WORKLOADS["single512"], base_seed=221),
+    "grid3x2_clip": dict(H.WORKLOADS["grid4032"], grid_cols=3, grid_rows=2, out_width=1400, out_height=1000,
+                         base_seed=223),
+    "rext444_10": dict(H.WORKLOADS["rext4096"], grid_cols=2, grid_rows=1, out_width=1024, out_height=512,
+                       base_seed=225),
+    "mono12": dict(H.WORKLOADS["single512"], chroma_idc=0, bit_depth=12, base_seed=227),
+    "rot90_grid": dict(H.WORKLOADS["grid4032"], grid_cols=2, grid_rows=2, out_width=1024, out_height=1000,
+                       orientation=N.ORIENT_ROTATE_90, base_seed=229),
+}
+
+
+def _yuv_kind(params):
+    c, deep = params.get("chroma_idc", 1), params.get("bit_depth", 8) > 8
+    base = {1: N.OUT_YUV420_8, 3: N.OUT_YUV444_8, 0: N.OUT_YUV400_8}[c]
+    return base + (1 if deep else 0)
+
+
+def test_decode_heic_yuv_matches_oracle():
+    """Raw planes (Decoder::decode_yuv) of HEIC files: every sample equals
+    the oracle's raw-plane output of the same records; orientation is
+    reported, not applied."""
+    with H.Decoder() as d:
+        for name, params in YUV_CASES.items():
+            print(f"decode_yuv case {name}", flush=True)
+            f = H.gen_heic(**params)
+            y, cb, cr, info = d.decode_yuv(f)
+            rs = H.RecordSet.generate(**dict(params, out_kind=_yuv_kind(params), orientation=0))
+            out = rs.output_buffers()
+            O.decode_batch(rs.batch, out, threads=8)
+            ey, ecb, ecr = H.yuv_planes(out[0], rs.outputs()[0])
+            assert np.array_equal(y, ey), name
+            if ecb is None:
+                assert cb is None and cr is None
+            else:
+                assert np.array_equal(cb, ecb) and np.array_equal(cr, ecr), name
+            assert info["orientation"] == params.get("orientation", 0)

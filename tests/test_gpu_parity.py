@@ -298,6 +298,18 @@ def test_full_size_workloads(gpu, workload):
         assert np.array_equal(g, w), f"{int(np.count_nonzero(g != w))} samples differ"
 
 
+@pytest.mark.parametrize("name", ["single420", "single_conf_window", "grid_clip_window", "rext444_10", "mono12"])
+def test_raw_plane_outputs(gpu, name):
+    """Raw-plane outputs (decode_yuv) of record sets, GPU against oracle."""
+    from test_yuv_output import CASES as YUV
+    rs = H.RecordSet.generate(**YUV[name])
+    got = gpu.decode(rs)
+    want = rs.output_buffers()
+    O.decode_batch(rs.batch, want, threads=4)
+    for g, w in zip(got, want):
+        assert np.array_equal(g.reshape(-1), w.reshape(-1)), name
+
+
 @pytest.mark.parametrize("orientation", ORIENTATIONS)
 def test_orientations(gpu, orientation):
     rs = H.RecordSet.generate("grid4032", grid_cols=2, grid_rows=2, tile_width=128, tile_height=64,

@@ -64,16 +64,30 @@ def test_mixed_requests_equal_alone(files):
     with H.Server() as srv:
         res = _run(srv, fs, order)
         res8 = _run(srv, fs, order[:12], rgb8=True)
-        batches, reqs, pics = srv.totals()
+        _, reqs, _ = srv.totals()
     for i, (k, img, st) in res.items():
         assert np.array_equal(img, alone[k]), (i, k)
         assert st["latency_ms"] > 0 and st["batch_requests"] >= 1
     for i, (k, img, st) in res8.items():
         assert np.array_equal(img, alone8[k]), (i, k)
     assert reqs == 36
-    # continuous batching put several requests into some launches
-    assert batches < reqs
-    assert max(st["batch_requests"] for _, _, st in res.values()) > 1
+
+
+def test_requests_parsed_together_share_launches(files):
+    """A burst of requests with a batching delay: a free slot waits for the
+    requests still in the parse, so launches hold several requests; every
+    image still equals its file decoded alone."""
+    fs, alone, _ = files
+    order = [i % len(fs) for i in range(18)]
+    print("serving: burst of 18 with a 300 ms batching delay", flush=True)
+    with H.Server(batch_delay_us=300_000) as srv:
+        tickets = [srv.submit(fs[k]) for k in order]
+        out = [srv.wait(t) for t in tickets]
+        batches, reqs, _ = srv.totals()
+    for k, (img, st) in zip(order, out):
+        assert np.array_equal(img, alone[k])
+    assert reqs == 18 and batches < reqs
+    assert max(st["batch_requests"] for _, st in out) > 1
 
 
 def test_batch_cap_one_picture_same_images(files):
