@@ -197,11 +197,12 @@ SERVING_MIX = (
 )
 
 
-def serving(H, seconds, loads=(0.25, 0.5, 0.8)):
+def serving(H, seconds, rates=(5.0, 20.0, 50.0, 100.0)):
     """The serving loop (hpvcd_server_*) under a mixed request stream:
     HEIC files of four sizes drawn uniformly, first as one burst (the
-    capacity, aggregate Mpixel/s), then as Poisson arrivals at fractions of
-    that capacity.  Latency is the library's own submit-to-image clock."""
+    capacity, aggregate Mpixel/s), then as Poisson arrivals at fixed rates
+    (requests per second).  Latency is the library's own submit-to-image
+    clock."""
     import queue
     import threading
 
@@ -255,12 +256,12 @@ def serving(H, seconds, loads=(0.25, 0.5, 0.8)):
         burst = run(srv, list(rng.integers(0, len(files), n)), [0.0] * n)
         out["burst"] = burst
         req_rate = burst["requests"] / burst["seconds"]
+        out["capacity_req_per_s"] = round(req_rate, 2)
         out["poisson"] = []
-        for f in loads:
-            rate = f * req_rate
-            n = max(16, int(rate * seconds))
+        for rate in rates:
+            n = max(10, int(rate * seconds))
             r = run(srv, list(rng.integers(0, len(files), n)), list(rng.exponential(1.0 / rate, n)))
-            out["poisson"].append({"load": f, "offered_req_per_s": round(rate, 2), **r})
+            out["poisson"].append({"offered_req_per_s": rate, "load": round(rate / req_rate, 3), **r})
         b, q, p = srv.totals()
         out["totals"] = {"batches": b, "requests": q, "pictures": p}
     return out
