@@ -369,9 +369,10 @@ __global__ void __launch_bounds__(256) k_act(const DevPic* __restrict__ pics, co
 // ---------------------------------------------------------------------------
 // K1c: motion compensation (src/decode.rs:7000 motion_compensate, src/mc.rs)
 //
-// One workgroup per block of at most 16 x 16 luma samples of a PU; for each
-// plane and list the clamped reference window ((bh + taps - 1) x (bw + taps
-// - 1), src/mc.rs:310) is staged in LDS, filtered horizontally into an LDS
+// One wave per block of at most 8 x 8 luma samples of a PU (four blocks per
+// workgroup); for each plane and list the clamped reference window ((bh +
+// taps - 1) x (bw + taps - 1), src/mc.rs:310) is staged in the wave's LDS,
+// filtered horizontally into an LDS
 // row buffer (>> BitDepth - 8) and vertically (>> 6), and narrowed to i16 as
 // the reference's 14-bit intermediates are.  The separable form equals the
 // reference's full-sample / one-direction special cases exactly (the
@@ -389,22 +390,33 @@ __constant__ int8_t kLumaFilter[4][8] = {{0, 0, 0, 64, 0, 0, 0, 0},
 __constant__ int8_t kChromaFilter[8][4] = {{0, 64, 0, 0},    {-2, 58, 10, -2}, {-4, 54, 16, -2}, {-6, 46, 28, -4},
                                            {-4, 36, 36, -4}, {-4, 28, 46, -6}, {-2, 16, 54, -4}, {-2, 10, 58, -2}};
 
-constexpr int kMcBlock = 16;
-constexpr int kMcWin = kMcBlock + 7;
+// The waves of a workgroup share nothing, so a wave needs no barrier beyond
+// its own.
+constexpr int kMcBlock = 8;
+constexpr int kMcWaves = 4;
+constexpr int kMcWinStride = 16;  // window rows: up to 8 + 7 samples
+constexpr int kMcRowStride = 8;   // horizontally filtered rows: up to 8 samples
 
 template <typename T, typename R>
-__global__ void __launch_bounds__(256) k_mc(const DevPic* __restrict__ pics, const DevPu* __restrict__ pus,
-                                            const DevMcTask* __restrict__ tasks, T* __restrict__ cur,
-                                            const T* __restrict__ refs, const R* __restrict__ res) {
-    __shared__ int32_t win[kMcWin * kMcWin];
-    __shared__ int32_t rows[kMcWin * kMcBlock];
-    const int tid = threadIdx.x;
-    const DevMcTask tk = tasks[blockIdx.x];
+__global__ void __launch_bounds__(64 * kMcWaves) k_mc(const DevPic* __restrict__ pics, const DevPu* __restrict__ pus,
+                                                     const DevMcTask* __restrict__ tasks, uint32_t ntasks,
+                                                     T* __restrict__ cur, const T* __restrict__ refs,
+                                                     const R* __restrict__ res) {
+    __shared__ int32_t win_all[kMcWaves][kMcWinStride * kMcWinStride];
+    __shared__ int32_t rows_all[kMcWaves][kMcWinStride * kMcRowStride];
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const uint32_t ti = blockIdx.x * kMcWaves + (uint32_t)wave;
+    if (ti >= ntasks) return;
+    int32_t* win = win_all[wave];
+    int32_t* rows = rows_all[wave];
+    const DevMcTask tk = tasks[ti];
     const DevPu& U = pus[tk.pu];
     const DevPic& P = pics[tk.pic];
     const int flags = (int)U.flags;
     const int bw_l = min(kMcBlock, U.w - (int)tk.ox), bh_l = min(kMcBlock, U.h - (int)tk.oy);
     const int nplanes = P.chroma_idc ? 3 : 1;
+    // the lane's sample of the block
+    const int sx = lane & 7, sy = lane >> 3;
     for (int pl = 0; pl < nplanes; pl++) {
         const int sw = pl ? P.sub_w_log2 : 0, sh = pl ? P.sub_h_log2 : 0;
         const int bw = bw_l >> sw, bh = bh_l >> sh;
@@ -413,8 +425,7 @@ __global__ void __launch_bounds__(256) k_mc(const DevPic* __restrict__ pics, con
         const int bd = pl ? P.bd_c : P.bd, s1 = bd - 8;
         const int taps = pl ? 4 : 8, half = taps / 2 - 1;
         const int ww = bw + taps - 1, wh = bh + taps - 1;
-        const bool mine = tid < bw * bh;
-        const int sx = mine ? tid % bw : 0, sy = mine ? tid / bw : 0;
+        const bool mine = sx < bw && sy < bh;
         int32_t pred[2] = {0, 0};
         for (int l = 0; l < 2; l++) {
             if (!((flags >> l) & 1)) continue;
@@ -429,28 +440,34 @@ __global__ void __launch_bounds__(256) k_mc(const DevPic* __restrict__ pics, con
                 ix = bx + (mx >> 3); fx = mx & 7;
                 iy = by + (my >> 3); fy = my & 7;
             }
-            // clamped reference window
-            for (int e = tid; e < ww * wh; e += 256) {
-                const int r = e / ww, c = e - r * ww;
-                const int x = min(max(ix - half + c, 0), pw - 1), y = min(max(iy - half + r, 0), ph - 1);
-                win[e] = (int32_t)rp[(size_t)y * pw + x];
+            // clamped reference window (RefPlane::sample, src/mc.rs:310)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int e = lane + 64 * k, r = e >> 4, c = e & 15;
+                if (r < wh && c < ww) {
+                    const int x = min(max(ix - half + c, 0), pw - 1), y = min(max(iy - half + r, 0), ph - 1);
+                    win[e] = (int32_t)rp[(size_t)y * pw + x];
+                }
             }
-            __syncthreads();
+            wave_sync();
             const int8_t* hf = pl ? kChromaFilter[fx] : kLumaFilter[fx];
             const int8_t* vf = pl ? kChromaFilter[fy] : kLumaFilter[fy];
-            for (int e = tid; e < wh * bw; e += 256) {
-                const int r = e / bw, c = e - r * bw;
-                int32_t acc = 0;
-                for (int t = 0; t < taps; t++) acc += (int32_t)hf[t] * win[r * ww + c + t];
-                rows[e] = acc >> s1;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int e = lane + 64 * k, r = e >> 3, c = e & 7;
+                if (r < wh && c < bw) {
+                    int32_t acc = 0;
+                    for (int t = 0; t < taps; t++) acc += (int32_t)hf[t] * win[r * kMcWinStride + c + t];
+                    rows[r * kMcRowStride + c] = acc >> s1;
+                }
             }
-            __syncthreads();
+            wave_sync();
             if (mine) {
                 int32_t acc = 0;
-                for (int t = 0; t < taps; t++) acc += (int32_t)vf[t] * rows[(sy + t) * bw + sx];
+                for (int t = 0; t < taps; t++) acc += (int32_t)vf[t] * rows[(sy + t) * kMcRowStride + sx];
                 pred[l] = wrap16(acc >> 6);
             }
-            __syncthreads();
+            wave_sync();
         }
         const int px = bx + sx, py = by + sy;
         if (!mine || px >= pw || py >= ph) continue;

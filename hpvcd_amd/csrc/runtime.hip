@@ -1158,7 +1158,8 @@ void launch_class(hpvcd_gpu* g, ClassJob& C, const uint8_t* d_scaling, uint8_t* 
             const uint32_t t0 = C.pic_mc_begin[p0], t1 = C.pic_mc_begin[p1];
             if (t1 > t0) {
                 mark(HPVCD_KERNEL_MC);
-                k_mc<T, R><<<t1 - t0, 256, 0, s>>>(C.d_pics, C.d_pus, C.d_mc_tasks + t0, pa, pb, res);
+                k_mc<T, R><<<cdiv(t1 - t0, kMcWaves), 64 * kMcWaves, 0, s>>>(C.d_pics, C.d_pus, C.d_mc_tasks + t0,
+                                                                            t1 - t0, pa, pb, res);
             }
         }
         mark(HPVCD_KERNEL_INTRA);
