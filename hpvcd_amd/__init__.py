@@ -43,11 +43,12 @@ WORKLOADS: Dict[str, dict] = {
                      chroma_idc=3, bit_depth=10, qp=22, config_id=2, out_kind=N.OUT_RGB16,
                      transform_skip=1, log2_max_ts=2, p_ts=0.2, implicit_rdpcm=1, cross_component=1,
                      log2_res_scale_max=3),
-    # 1920x1080 4:2:0 8-bit sequence of 32 pictures: 1 I + 31 low-delay B with 4 references,
-    # bi-prediction p=0.5, explicit weighted prediction, MV ~ N(0, 16 px), QP 30 -> one RGB8 image per picture
+    # 1920x1080 4:2:0 8-bit sequence of 32 pictures: 1 I + 31 B in hierarchical GOPs of 8 with up to
+    # 4 references, bi-prediction p=0.5, explicit weighted prediction, merge p=0.4 with temporal MVP,
+    # MV ~ N(0, 16 px), QP 30 -> one RGB8 image per picture in display order
     "video1080": dict(tile_width=1920, tile_height=1080, grid_cols=1, grid_rows=32, out_width=1920, out_height=1080,
                       num_outputs=32, chroma_idc=1, bit_depth=8, qp=30, config_id=3, out_kind=N.OUT_RGB8,
-                      inter=1, num_refs=4, weighted=1, p_bi=0.5, mv_sigma=16.0, gop=8),
+                      inter=1, num_refs=4, weighted=1, p_bi=0.5, mv_sigma=16.0, gop=8, tmvp=1),
 }
 
 
@@ -161,6 +162,10 @@ class RecordSet:
     def picture(self, i: int) -> N.Picture:
         """A copy of picture record i (valid after the set is freed)."""
         return N.Picture.from_buffer_copy(self.batch.pictures[i])
+
+    def ref(self, i: int) -> N.Ref:
+        """A copy of reference-list entry i of the batch."""
+        return N.Ref.from_buffer_copy(self.batch.refs[i])
 
     def tbs(self) -> np.ndarray:
         n = self.batch.num_tbs
@@ -521,6 +526,16 @@ class Decoder:
             return y, cb, cr, info
         finally:
             N.lib().hpvcd_yuv_free(C.byref(yv))
+
+    def decode_hevc(self, data: bytes, out_kind: int = -1) -> List[np.ndarray]:
+        """An HEVC stream's frames in output order (hpvcd_decode_hevc / decode_hevc)."""
+        frames = C.POINTER(N.Image)()
+        n = C.c_uint32()
+        check(N.lib().hpvcd_decode_hevc(self._h, data, len(data), out_kind, C.byref(frames), C.byref(n)))
+        try:
+            return [image_array(frames[i]) for i in range(n.value)]
+        finally:
+            N.lib().hpvcd_frames_free(frames, n)
 
     def _heic(self, fn, data: bytes) -> np.ndarray:
         img = N.Image()

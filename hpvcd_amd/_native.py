@@ -214,6 +214,10 @@ class GenConfig(C.Structure):
         ("p_sao_merge", C.c_float),
         ("persistent_rice", C.c_uint8), ("ts_context", C.c_uint8),
         ("bypass_alignment", C.c_uint8), ("wpp", C.c_uint8),
+        ("tmvp", C.c_uint8), ("merge_cand", C.c_uint8), ("par_merge_level", C.c_uint8),
+        ("mvd_l1_zero", C.c_uint8), ("rps_in_sps", C.c_uint8), ("cabac_init", C.c_uint8),
+        ("_pad2", C.c_uint8 * 2),
+        ("p_merge", C.c_float),
     ]
 
 
@@ -236,6 +240,38 @@ class Image(C.Structure):
     ]
 
 
+class MvCell(C.Structure):
+    _fields_ = [
+        ("inter", C.c_uint8), ("pred", C.c_uint8), ("lt", C.c_uint8), ("avail", C.c_uint8),
+        ("ref_idx", C.c_int8 * 2), ("_pad", C.c_uint8 * 2),
+        ("mv", (C.c_int16 * 2) * 2), ("ref_poc", C.c_int32 * 2),
+    ]
+
+
+class MotionCase(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("log2_ctb", C.c_int32), ("par_mrg_level", C.c_int32), ("max_merge", C.c_int32),
+        ("is_b", C.c_int32), ("tmvp", C.c_int32), ("cur_poc", C.c_int32), ("col_poc", C.c_int32),
+        ("col_from_l0", C.c_int32),
+        ("num_ref", C.c_int32 * 2),
+        ("ref_poc", (C.c_int32 * 16) * 2),
+        ("ref_lt", (C.c_uint8 * 16) * 2),
+        ("pu_x", C.c_int32), ("pu_y", C.c_int32), ("pu_w", C.c_int32), ("pu_h", C.c_int32),
+        ("cu_x", C.c_int32), ("cu_y", C.c_int32), ("cu_size", C.c_int32),
+        ("part_idx", C.c_int32),
+        ("cur", C.POINTER(MvCell)), ("col", C.POINTER(MvCell)),
+    ]
+
+
+class MotionResult(C.Structure):
+    _fields_ = [
+        ("num_merge", C.c_int32),
+        ("merge", MvCell * 5),
+        ("amvp", (((C.c_int16 * 2) * 2) * 16) * 2),
+    ]
+
+
 BACKEND_CPU_REF, BACKEND_GPU_PLAIN, BACKEND_GPU_BATCHED = 0, 1, 2
 
 # every symbol include/hpvcd.h declares, with its ctypes signature
@@ -254,6 +290,7 @@ SIGNATURES = {
     "hpvcd_parse_heic": (C.c_int, [C.c_char_p, C.c_size_t, C.c_void_p, C.c_int32, C.POINTER(_P)]),
     "hpvcd_gen_heic": (C.c_int, [C.POINTER(GenConfig), C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t)]),
     "hpvcd_cabac_num_elements": (C.c_int32, []),
+    "hpvcd_motion_probe": (C.c_int, [C.POINTER(MotionCase), C.POINTER(MotionResult)]),
     "hpvcd_cabac_element": (C.c_int32, [C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "hpvcd_cabac_tables": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hpvcd_cabac_init_states": (C.c_int32, [C.c_int32, C.c_int32, C.c_void_p]),
@@ -292,6 +329,9 @@ SIGNATURES = {
     "hpvcd_decode_heic_rgb8": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(Image)]),
     "hpvcd_decode_heic_yuv": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(Yuv)]),
     "hpvcd_yuv_free": (None, [C.POINTER(Yuv)]),
+    "hpvcd_decode_hevc": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.c_int32, C.POINTER(C.POINTER(Image)),
+                                    C.POINTER(C.c_uint32)]),
+    "hpvcd_frames_free": (None, [C.POINTER(Image), C.c_uint32]),
     "hpvcd_tuning_default": (None, [C.POINTER(Tuning)]),
     "hpvcd_tuning_check": (C.c_int, [C.POINTER(Tuning)]),
     "hpvcd_gpu_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),

@@ -17,9 +17,9 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhpvcd.so")
 
 # host C++ units: record generator and its HEVC writer, coding-tree walker,
-# CABAC, parameter sets, syntax parser, HEIF container, decoder object,
+# motion derivations, CABAC, parameter sets, syntax parser, HEIF container, decoder object,
 # serving loop
-HOST_UNITS = ["gen.cpp", "gen_hevc.cpp", "walker.cpp", "cabac.cpp", "hevc_ps.cpp", "parse.cpp", "heif.cpp", "decoder.cpp", "server.cpp"]
+HOST_UNITS = ["gen.cpp", "gen_hevc.cpp", "walker.cpp", "motion.cpp", "cabac.cpp", "hevc_ps.cpp", "parse.cpp", "heif.cpp", "decoder.cpp", "server.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HPVCD_OFFLOAD_ARCH", "gfx950")

@@ -6,6 +6,7 @@
 // GPU) or records in, into library-owned images (src/lib.rs:430-455
 // DecodedImage / Rgb8Image; hpvcd_image_free is the Vec ownership hand-off).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -214,6 +215,42 @@ void hpvcd_yuv_free(hpvcd_yuv* yuv) {
     if (!yuv) return;
     hpvcd_host_free(yuv->y);
     std::memset(yuv, 0, sizeof(*yuv));
+}
+
+hpvcd_status hpvcd_decode_hevc(hpvcd_decoder* d, const uint8_t* data, size_t size, int32_t out_kind,
+                               hpvcd_image** frames, uint32_t* count) {
+    if (!d || !data || !frames || !count) return set_error(HPVCD_E_INVALID, "null argument");
+    *frames = nullptr;
+    *count = 0;
+    hpvcd_records* r = nullptr;
+    hpvcd_status st = hpvcd_parse_hevc(data, size, out_kind, &r);
+    if (st) return st;
+    hpvcd_batch b;
+    hpvcd_records_view(r, &b);
+    hpvcd_image* imgs = nullptr;
+    if (b.num_outputs) {
+        imgs = static_cast<hpvcd_image*>(std::calloc(b.num_outputs, sizeof(hpvcd_image)));
+        if (!imgs) {
+            hpvcd_records_free(r);
+            return set_error(HPVCD_E_ALLOC, "out of memory");
+        }
+    }
+    st = hpvcd_decode_records(d, &b, imgs);
+    const uint32_t n = b.num_outputs;
+    hpvcd_records_free(r);
+    if (st) {
+        std::free(imgs);
+        return st;
+    }
+    *frames = imgs;
+    *count = n;
+    return HPVCD_OK;
+}
+
+void hpvcd_frames_free(hpvcd_image* frames, uint32_t count) {
+    if (!frames) return;
+    for (uint32_t i = 0; i < count; i++) hpvcd_image_free(&frames[i]);
+    std::free(frames);
 }
 
 }  // extern "C"

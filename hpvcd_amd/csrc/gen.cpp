@@ -126,8 +126,7 @@ namespace {
 class RandomSource : public Source {
 public:
     RandomSource(const hpvcd_gen_config& c, uint64_t seed) : cfg(c), r{seed} {}
-    Rng* rng() override { return &r; }
-    void begin_slice(int) override {}
+    void begin_slice(int, int) override {}
     void begin_substream(int) override {}
     int end_ctb(bool last_of_slice, bool) override { return last_of_slice ? 1 : 0; }
     // merged from the left or above CTB (where available) or drawn; offsets
@@ -209,16 +208,162 @@ public:
         }
         if (p.sign_hiding) residual_hide_signs(p, o, lv);
     }
+    // inter decisions: skip, intra CUs, partitions (NxN at the smallest CU
+    // above 8x8, AMP above the smallest), merge, prediction lists (no bi for
+    // 8x4 / 4x8), reference indices, and vectors ~ N(0, mv_sigma) keeping the
+    // block within 64 samples of the picture, coded against a random
+    // predictor of the two
+    bool cu_skip(int) override { return r.bern(cfg.p_skip); }
+    bool pred_mode_intra() override { return r.bern(cfg.p_intra_cu); }
+    int part_mode_inter(int log2, bool min_size, bool amp) override {
+        int opts[8], n = 0;
+        opts[n++] = PART_2Nx2N;
+        opts[n++] = PART_2NxN;
+        opts[n++] = PART_Nx2N;
+        if (min_size && log2 > 3) opts[n++] = PART_NxN;
+        if (amp)
+            for (int k = PART_2NxnU; k <= PART_nRx2N; k++) opts[n++] = k;
+        return opts[r.uni((uint32_t)n)];
+    }
+    bool merge_flag() override { return r.bern(cfg.p_merge); }
+    int merge_idx(int max_cand) override { return (int)r.uni((uint32_t)max_cand); }
+    int inter_pred_idc(int, bool no_bi) override {
+        if (!no_bi && r.bern(cfg.p_bi)) return 2;
+        return (r.next() & 1) ? 0 : 1;
+    }
+    int ref_idx(int num) override { return (int)r.uni((uint32_t)num); }
+    void mv_syntax(const PuGeom& pu, int, const int16_t cand[2][2], bool mvd_zero, int16_t mvd[2],
+                   int& mvp_flag) override {
+        mvp_flag = (int)(r.next() & 1);
+        if (mvd_zero) {
+            mvd[0] = mvd[1] = 0;
+            return;
+        }
+        const double sig = cfg.mv_sigma * 4.0;
+        int mvx = (int)std::lround(gauss() * sig), mvy = (int)std::lround(gauss() * sig);
+        mvx = std::clamp(mvx, (-64 - pu.x - pu.w) * 4, (cfg.tile_width + 64 - pu.x) * 4);
+        mvy = std::clamp(mvy, (-64 - pu.y - pu.h) * 4, (cfg.tile_height + 64 - pu.y) * 4);
+        mvd[0] = (int16_t)(uint16_t)(uint32_t)(mvx - cand[mvp_flag][0]);
+        mvd[1] = (int16_t)(uint16_t)(uint32_t)(mvy - cand[mvp_flag][1]);
+    }
+    bool rqt_root_cbf() override { return r.bern(0.85f); }
 
 private:
+    double gauss() {
+        const double u1 = ((double)(r.next() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double u2 = (double)(r.next() >> 11) * (1.0 / 9007199254740992.0);
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
     const hpvcd_gen_config& cfg;
     Rng r;
 };
 
 }  // namespace
 
+int32_t gen_poc(const hpvcd_gen_config& c, uint32_t index) {
+    if (!c.inter || c.gop <= 1) return (int32_t)index;
+    const std::vector<int> order = gop_decode_order(c.grid_cols * c.grid_rows, c.gop);
+    return index < order.size() ? order[index] : (int32_t)index;
+}
+
+void gen_lists(const hpvcd_gen_config& c, uint32_t index, std::vector<uint32_t>& l0, std::vector<uint32_t>& l1) {
+    l0.clear();
+    l1.clear();
+    if (!c.inter || index == 0) return;
+    const int nr = c.num_refs ? c.num_refs : 1;
+    if (c.gop > 1) {
+        gop_ref_lists(c.grid_cols * c.grid_rows, c.gop, nr, index, l0, l1);
+    } else {
+        const int n = std::max(1, std::min(nr, (int)index));
+        for (int i = 0; i < n; i++) {
+            l0.push_back(index - 1 - (uint32_t)i);
+            l1.push_back(index - (uint32_t)n + (uint32_t)i);
+        }
+    }
+    if (c.p_bi <= 0.f) l1.clear();
+}
+
+// Each list entry is a pred_weight_table row (src/inter.rs:141): luma and
+// chroma weights each signalled with probability 0.8 (both chroma
+// components under one flag), weights in [32, 96] of denominator 64,
+// offsets in [-16, 16] 8-bit units scaled to the bit depth.
+void gen_inter_slice(const hpvcd_gen_config& c, uint32_t index, InterSlice& s) {
+    s = InterSlice();
+    Rng rng{tile_seed(c.base_seed ^ 0x1e7e5ull, c.config_id, index)};
+    std::vector<uint32_t> lists[2];
+    gen_lists(c, index, lists[0], lists[1]);
+    const bool b = c.p_bi > 0.f;
+    s.slice_type = b ? 0 : 1;
+    for (int l = 0; l < 2; l++)
+        for (uint32_t pic : lists[l]) {
+            RefPic rp;
+            rp.poc = gen_poc(c, pic);
+            rp.pic = (int32_t)pic;
+            s.list[l].push_back(rp);
+            hpvcd_ref e;
+            std::memset(&e, 0, sizeof(e));
+            e.picture = pic;
+            e.weighted = c.weighted ? 1 : 0;
+            e.log2_denom[0] = e.log2_denom[1] = 6;
+            const bool luma = c.weighted && rng.bern(0.8f);
+            const bool chroma = c.weighted && c.chroma_idc && rng.bern(0.8f);
+            for (int k = 0; k < 3; k++) {
+                const bool signalled = k == 0 ? luma : chroma;
+                e.weight[k] = (int16_t)(signalled ? rng.sint(32, 96) : 64);
+                e.offset[k] = (int16_t)(signalled ? rng.sint(-16, 16) * (1 << (c.bit_depth - 8)) : 0);
+            }
+            // a 4:0:0 table has no chroma denominator: ChromaLog2WeightDenom 0
+            // and the neutral weight 1 (src/inter.rs:170)
+            if (c.weighted && !c.chroma_idc) {
+                e.log2_denom[1] = 0;
+                e.weight[1] = e.weight[2] = 1;
+            }
+            s.rows.push_back(e);
+        }
+    s.tmvp = c.tmvp != 0;
+    s.col_from_l0 = b ? rng.bern(0.5f) : true;
+    const size_t ncol = s.list[s.col_from_l0 ? 0 : 1].size();
+    s.col_ref_idx = ncol > 1 ? (int)rng.uni((uint32_t)ncol) : 0;
+    s.max_merge = c.merge_cand ? std::min<int>(c.merge_cand, 5) : 5;
+    s.par_mrg_level = c.par_merge_level ? std::min<int>(c.par_merge_level, c.log2_ctb) : 2;
+    s.mvd_l1_zero = b && c.mvd_l1_zero;
+    s.cabac_init = c.cabac_init && rng.bern(0.5f);
+}
+
+int gen_col_picture(const hpvcd_gen_config& c, uint32_t index) {
+    if (!c.inter || !c.tmvp || index == 0) return -1;
+    InterSlice s;
+    gen_inter_slice(c, index, s);
+    const std::vector<RefPic>& l = s.list[s.col_from_l0 ? 0 : 1];
+    return (size_t)s.col_ref_idx < l.size() ? l[(size_t)s.col_ref_idx].pic : -1;
+}
+
+void gen_for_pictures(const hpvcd_gen_config& c, const std::function<void(uint32_t)>& fn) {
+    const uint32_t n = (uint32_t)(c.grid_cols * c.grid_rows);
+    // waves: a picture one after its collocated picture's
+    std::vector<int> wave(n, 0);
+    int nwaves = 1;
+    for (uint32_t i = 0; i < n; i++) {
+        const int col = gen_col_picture(c, i);
+        wave[i] = col >= 0 && (uint32_t)col < i ? wave[(size_t)col] + 1 : 0;
+        nwaves = std::max(nwaves, wave[i] + 1);
+    }
+    const unsigned nth = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    for (int wv = 0; wv < nwaves; wv++) {
+        std::vector<uint32_t> todo;
+        for (uint32_t i = 0; i < n; i++)
+            if (wave[i] == wv) todo.push_back(i);
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nth && t < todo.size(); t++)
+            th.emplace_back([&, t]() {
+                for (size_t k = t; k < todo.size(); k += nth) fn(todo[k]);
+            });
+        for (auto& x : th) x.join();
+    }
+}
+
 // One generated picture: records, and the coded slice data when `enc` is set.
-void gen_picture(const hpvcd_gen_config& cfg, uint32_t index, PicRecords& out, PicCoder* enc) {
+void gen_picture(const hpvcd_gen_config& cfg, uint32_t index, PicRecords& out, PicCoder* enc, const ColField* col) {
     const PicParams P = gen_params(cfg);
     RandomSource rs(cfg, tile_seed(cfg.base_seed, cfg.config_id, index));
     Source* src = &rs;
@@ -227,7 +372,13 @@ void gen_picture(const hpvcd_gen_config& cfg, uint32_t index, PicRecords& out, P
         coded = enc->wrap(rs, P);
         src = coded.get();
     }
-    PicWalker g(P, *src, out, &cfg, index);
+    PicWalker g(P, *src, out, gen_poc(cfg, index));
+    InterSlice inter;
+    if (cfg.inter && index > 0) {
+        gen_inter_slice(cfg, index, inter);
+        inter.col = col;
+        g.set_inter(&inter);
+    }
     const int ctb = 1 << cfg.log2_ctb;
     const int cols = (P.w + ctb - 1) / ctb, rows = (P.h + ctb - 1) / ctb;
     const int nctb = cols * rows;
@@ -292,6 +443,8 @@ hpvcd_status gen_validate(const hpvcd_gen_config& c) {
     if (c.chroma_idc == 2) return set_error(HPVCD_E_UNSUPPORTED_CHROMA, "generator: 4:2:2 not generated");
     if (c.gop > 64 || (c.gop > 1 && !c.inter)) return set_error(HPVCD_E_PARAMSET, "generator: GOP size 2..64 of an inter sequence");
     if (c.bit_depth < 8 || c.bit_depth > 12) return set_error(HPVCD_E_UNSUPPORTED_BITDEPTH, "generator: bit depth");
+    if (c.merge_cand > 5 || c.par_merge_level == 1 || c.par_merge_level > c.log2_ctb || c.num_refs > 8)
+        return set_error(HPVCD_E_PARAMSET, "generator: merge candidates 0..5, merge level 0 or 2..log2_ctb, 0..8 references");
     if (c.pcm && (c.pcm_bit_depth < 1 || c.pcm_bit_depth > c.bit_depth))
         return set_error(HPVCD_E_PARAMSET, "generator: PCM bit depth must be 1..bit_depth");
     if (c.log2_ctb < 4 || c.log2_ctb > 6 || c.log2_min_cb < 3 || c.log2_min_cb > c.log2_ctb || c.log2_min_tb != 2 ||
@@ -427,18 +580,15 @@ extern "C" hpvcd_status hpvcd_gen_records(const hpvcd_gen_config* cfg, hpvcd_rec
     if (hpvcd_status st = gen_validate(c)) return st;
     const uint32_t ntiles = (uint32_t)(c.grid_cols * c.grid_rows);
     std::vector<PicRecords> tiles(ntiles);
-    const unsigned nth = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nth; t++)
-        th.emplace_back([&, t]() {
-            for (uint32_t i = t; i < ntiles; i += nth) gen_picture(c, i, tiles[i], nullptr);
-        });
-    for (auto& x : th) x.join();
+    gen_for_pictures(c, [&](uint32_t i) {
+        const int col = gen_col_picture(c, i);
+        gen_picture(c, i, tiles[i], nullptr, col >= 0 ? &tiles[(size_t)col].col : nullptr);
+    });
     return gen_assemble(c, tiles, out);
 }
 
 
-static_assert(sizeof(hpvcd_gen_config) == 168, "hpvcd_gen_config layout");
+static_assert(sizeof(hpvcd_gen_config) == 176, "hpvcd_gen_config layout");
 static_assert(sizeof(hpvcd_picture) == 140, "hpvcd_picture layout");
 static_assert(sizeof(hpvcd_tb) == 24 && sizeof(hpvcd_sao) == 36 && sizeof(hpvcd_output) == 12, "record layout");
 static_assert(sizeof(hpvcd_pu) == 20 && sizeof(hpvcd_ref) == 20, "inter record layout");
@@ -482,6 +632,7 @@ extern "C" void hpvcd_gen_default_config(hpvcd_gen_config* c) {
     c->p_skip = 0.3f;
     c->p_bi = 0.5f;
     c->mv_sigma = 16.0f;
+    c->p_merge = 0.4f;
     c->config_id = 0;
     c->base_seed = 0x48505643ull;
     c->qp_delta_max = 4;

@@ -655,6 +655,7 @@ hpvcd_status gen_heic(const hpvcd_gen_config& c, std::vector<uint8_t>& file) {
     if (c.orientation == HPVCD_ORIENT_TRANSPOSE || c.orientation == HPVCD_ORIENT_TRANSVERSE)
         return set_error(HPVCD_E_UNSUPPORTED, "HEIC writer: transpose and transverse have no irot/imir form");
     if (c.num_outputs > 1) return set_error(HPVCD_E_UNSUPPORTED, "HEIC writer: one image per file");
+    if (c.inter) return set_error(HPVCD_E_UNSUPPORTED, "HEIC writer: image items are intra pictures");
     std::vector<PicRecords> recs;
     std::vector<PicStream> streams;
     if (hpvcd_status st = gen_coded(c, recs, streams)) return st;

@@ -233,8 +233,8 @@ std::vector<uint8_t> write_sps(const Sps& s) {
     w.ue((uint32_t)(s.bit_depth_chroma - 8));
     w.ue((uint32_t)(s.log2_max_poc_lsb - 4));
     w.bit(0);       // sps_sub_layer_ordering_info_present_flag
-    w.ue(4);        // sps_max_dec_pic_buffering_minus1
-    w.ue(0);        // sps_max_num_reorder_pics
+    w.ue((uint32_t)(s.max_dec_pic_buffering - 1));
+    w.ue((uint32_t)s.max_num_reorder);
     w.ue(0);        // sps_max_latency_increase_plus1
     w.ue((uint32_t)(s.log2_min_cb - 3));
     w.ue((uint32_t)(s.log2_ctb - s.log2_min_cb));
@@ -257,8 +257,10 @@ std::vector<uint8_t> write_sps(const Sps& s) {
         w.ue((uint32_t)(s.log2_max_pcm_cb - s.log2_min_pcm_cb));
         w.flag(s.pcm_loop_filter_disabled);
     }
-    w.ue((uint32_t)s.num_short_term_rps);
+    w.ue((uint32_t)s.st_rps.size());
+    for (size_t i = 0; i < s.st_rps.size(); i++) write_st_rps(w, (int)i, (int)s.st_rps.size(), s.st_rps, s.st_rps[i]);
     w.flag(s.long_term_refs);
+    if (s.long_term_refs) w.ue(0);  // num_long_term_ref_pics_sps
     w.flag(s.temporal_mvp);
     w.flag(s.strong_intra_smoothing);
     w.flag(s.vui);
@@ -392,6 +394,189 @@ static int ceil_log2(int n) {
     return b;
 }
 
+// 7.4.8 (src/rps.rs:148): the new S0 takes, in decreasing POC, the
+// reference's S1 entries (reversed), the reference picture itself and the
+// reference's S0 entries that land before the current picture; S1 the
+// mirror image.  Candidates are indexed S0 entries, S1 entries, then the
+// reference picture.
+StRps derive_predicted_rps(const StRps& ref, int delta_rps, const std::vector<uint8_t>& used,
+                           const std::vector<uint8_t>& use_delta) {
+    StRps out;
+    const int nn = (int)ref.d0.size(), np = (int)ref.d1.size(), n = nn + np;
+    auto take = [&](int idx) { return idx < (int)use_delta.size() && use_delta[(size_t)idx]; };
+    auto u = [&](int idx) { return (uint8_t)(idx < (int)used.size() ? used[(size_t)idx] : 0); };
+    for (int j = np - 1; j >= 0; j--) {
+        const int d = ref.d1[(size_t)j] + delta_rps;
+        if (d < 0 && take(nn + j)) { out.d0.push_back(d); out.u0.push_back(u(nn + j)); }
+    }
+    if (delta_rps < 0 && take(n)) { out.d0.push_back(delta_rps); out.u0.push_back(u(n)); }
+    for (int j = 0; j < nn; j++) {
+        const int d = ref.d0[(size_t)j] + delta_rps;
+        if (d < 0 && take(j)) { out.d0.push_back(d); out.u0.push_back(u(j)); }
+    }
+    for (int j = nn - 1; j >= 0; j--) {
+        const int d = ref.d0[(size_t)j] + delta_rps;
+        if (d > 0 && take(j)) { out.d1.push_back(d); out.u1.push_back(u(j)); }
+    }
+    if (delta_rps > 0 && take(n)) { out.d1.push_back(delta_rps); out.u1.push_back(u(n)); }
+    for (int j = 0; j < np; j++) {
+        const int d = ref.d1[(size_t)j] + delta_rps;
+        if (d > 0 && take(nn + j)) { out.d1.push_back(d); out.u1.push_back(u(nn + j)); }
+    }
+    return out;
+}
+
+int parse_st_rps(BitReader& r, int idx, int num_sets, const std::vector<StRps>& sets, StRps& out) {
+    out = StRps();
+    const bool predicted = idx != 0 && r.flag();
+    if (predicted) {
+        int delta_idx = 1;
+        if (idx == num_sets) delta_idx = (int)r.ue() + 1;
+        if (delta_idx > idx || idx - delta_idx >= (int)sets.size())
+            return set_error(HPVCD_E_BITSTREAM, "delta_idx_minus1 of a predicted RPS");
+        const StRps& ref = sets[(size_t)(idx - delta_idx)];
+        const bool neg = r.flag();
+        const uint32_t abs_minus1 = r.ue();
+        if (abs_minus1 >= 32768) return set_error(HPVCD_E_BITSTREAM, "abs_delta_rps_minus1");
+        const int delta_rps = neg ? -(int)(abs_minus1 + 1) : (int)(abs_minus1 + 1);
+        const int n = ref.num();
+        std::vector<uint8_t> used((size_t)n + 1), use_delta((size_t)n + 1, 1);
+        for (int j = 0; j <= n; j++) {
+            used[(size_t)j] = r.flag() ? 1 : 0;
+            if (!used[(size_t)j]) use_delta[(size_t)j] = r.flag() ? 1 : 0;
+        }
+        out = derive_predicted_rps(ref, delta_rps, used, use_delta);
+    } else {
+        const uint32_t nn = r.ue(), np = r.ue();
+        if (nn > 16 || np > 16) return set_error(HPVCD_E_BITSTREAM, "short-term RPS size");
+        int prev = 0;
+        for (uint32_t i = 0; i < nn; i++) {
+            prev -= (int)r.ue() + 1;
+            out.d0.push_back(prev);
+            out.u0.push_back(r.flag() ? 1 : 0);
+        }
+        prev = 0;
+        for (uint32_t i = 0; i < np; i++) {
+            prev += (int)r.ue() + 1;
+            out.d1.push_back(prev);
+            out.u1.push_back(r.flag() ? 1 : 0);
+        }
+    }
+    if (r.overrun) return set_error(HPVCD_E_BITSTREAM, "short-term RPS truncated");
+    if (out.num() > 16) return set_error(HPVCD_E_BITSTREAM, "short-term RPS size");
+    return 0;
+}
+
+void write_st_rps(BitWriter& w, int idx, int num_sets, const std::vector<StRps>& sets, const StRps& rps) {
+    if (idx != 0) {
+        // prediction from the set just before (delta_idx_minus1 = 0): the
+        // first delta POC that makes every entry of `rps` derivable
+        const StRps& ref = sets[(size_t)idx - 1];
+        const int n = ref.num();
+        for (int mag = 1; mag <= 64; mag++)
+            for (int sg = 0; sg < 2; sg++) {
+                const int delta = sg ? -mag : mag;
+                std::vector<uint8_t> used((size_t)n + 1, 0), use((size_t)n + 1, 0);
+                for (int j = 0; j <= n; j++) {
+                    const int d = j == n ? delta : (j < (int)ref.d0.size() ? ref.d0[(size_t)j] : ref.d1[(size_t)j - ref.d0.size()]) + delta;
+                    for (size_t k = 0; k < rps.d0.size(); k++)
+                        if (rps.d0[k] == d) { use[(size_t)j] = 1; used[(size_t)j] = rps.u0[k]; }
+                    for (size_t k = 0; k < rps.d1.size(); k++)
+                        if (rps.d1[k] == d) { use[(size_t)j] = 1; used[(size_t)j] = rps.u1[k]; }
+                }
+                if (!(derive_predicted_rps(ref, delta, used, use) == rps)) continue;
+                w.bit(1);  // inter_ref_pic_set_prediction_flag
+                if (idx == num_sets) w.ue(0);  // delta_idx_minus1
+                w.flag(delta < 0);
+                w.ue((uint32_t)(mag - 1));
+                for (int j = 0; j <= n; j++) {
+                    w.flag(used[(size_t)j] != 0);
+                    if (!used[(size_t)j]) w.flag(use[(size_t)j] != 0);
+                }
+                return;
+            }
+        w.bit(0);
+    }
+    w.ue((uint32_t)rps.d0.size());
+    w.ue((uint32_t)rps.d1.size());
+    int prev = 0;
+    for (size_t i = 0; i < rps.d0.size(); i++) {
+        w.ue((uint32_t)(prev - rps.d0[i] - 1));
+        prev = rps.d0[i];
+        w.flag(rps.u0[i] != 0);
+    }
+    prev = 0;
+    for (size_t i = 0; i < rps.d1.size(); i++) {
+        w.ue((uint32_t)(rps.d1[i] - prev - 1));
+        prev = rps.d1[i];
+        w.flag(rps.u1[i] != 0);
+    }
+}
+
+int num_pic_total_curr(const SliceHeader& h) {
+    int n = 0;
+    for (uint8_t u : h.rps.u0) n += u;
+    for (uint8_t u : h.rps.u1) n += u;
+    return n;
+}
+
+bool slice_weighted(const SliceHeader& h, const Pps& p) {
+    return (h.slice_type == 1 && p.weighted_pred) || (h.slice_type == 0 && p.weighted_bipred);
+}
+
+static void write_pred_weights(BitWriter& w, const SliceHeader& h, const Sps& s) {
+    const PredWeights& t = h.wp;
+    w.ue((uint32_t)t.luma_log2_denom);
+    if (s.chroma_format_idc) w.se(t.chroma_log2_denom - t.luma_log2_denom);
+    for (int l = 0; l < (h.slice_type == 0 ? 2 : 1); l++) {
+        const int n = h.num_ref[l];
+        for (int i = 0; i < n; i++) w.flag(t.e[l][(size_t)i].luma);
+        if (s.chroma_format_idc)
+            for (int i = 0; i < n; i++) w.flag(t.e[l][(size_t)i].chroma);
+        for (int i = 0; i < n; i++) {
+            const WpEntry& e = t.e[l][(size_t)i];
+            if (e.luma) {
+                w.se(e.delta_weight);
+                w.se(e.offset);
+            }
+            if (e.chroma)
+                for (int j = 0; j < 2; j++) {
+                    w.se(e.delta_cweight[j]);
+                    w.se(e.delta_coffset[j]);
+                }
+        }
+    }
+}
+
+static int parse_pred_weights(BitReader& r, SliceHeader& h, const Sps& s) {
+    PredWeights& t = h.wp;
+    t = PredWeights();
+    t.luma_log2_denom = (int)r.ue();
+    if (t.luma_log2_denom > 7) return set_error(HPVCD_E_BITSTREAM, "luma_log2_weight_denom");
+    t.chroma_log2_denom = 0;
+    if (s.chroma_format_idc) t.chroma_log2_denom = std::clamp(t.luma_log2_denom + r.se(), 0, 7);
+    for (int l = 0; l < (h.slice_type == 0 ? 2 : 1); l++) {
+        const int n = h.num_ref[l];
+        t.e[l].assign((size_t)n, WpEntry());
+        for (int i = 0; i < n; i++) t.e[l][(size_t)i].luma = r.flag();
+        if (s.chroma_format_idc)
+            for (int i = 0; i < n; i++) t.e[l][(size_t)i].chroma = r.flag();
+        for (int i = 0; i < n; i++) {
+            WpEntry& e = t.e[l][(size_t)i];
+            if (e.luma) {
+                e.delta_weight = r.se();
+                e.offset = r.se();
+            }
+            if (e.chroma)
+                for (int j = 0; j < 2; j++) {
+                    e.delta_cweight[j] = r.se();
+                    e.delta_coffset[j] = r.se();
+                }
+        }
+    }
+    return 0;
+}
+
 void write_slice_header(BitWriter& w, const SliceHeader& h, const Sps& s, const Pps& p, uint8_t nal_type) {
     w.flag(h.first_slice);
     if (nal_type >= 16 && nal_type <= 23) w.bit(0);  // no_output_of_prior_pics_flag
@@ -405,18 +590,42 @@ void write_slice_header(BitWriter& w, const SliceHeader& h, const Sps& s, const 
     if (!h.dependent) {
         w.bits(0, p.num_extra_slice_header_bits);
         w.ue((uint32_t)h.slice_type);
-        if (p.output_flag_present) w.bit(1);
+        if (p.output_flag_present) w.flag(h.pic_output);
         if (nal_type != NAL_IDR_W_RADL && nal_type != NAL_IDR_N_LP) {
             w.bits((uint32_t)h.poc_lsb, s.log2_max_poc_lsb);
-            w.bit(0);  // short_term_ref_pic_set_sps_flag: an empty inline set
-            w.ue(0);   // num_negative_pics
-            w.ue(0);   // num_positive_pics
-            if (s.long_term_refs) w.ue(0);
-            if (s.temporal_mvp) w.bit(0);
+            w.flag(h.rps_from_sps);
+            if (!h.rps_from_sps) write_st_rps(w, (int)s.st_rps.size(), (int)s.st_rps.size(), s.st_rps, h.rps);
+            else if (s.st_rps.size() > 1) w.bits((uint32_t)h.rps_idx, ceil_log2((int)s.st_rps.size()));
+            if (s.long_term_refs) w.ue(0);  // num_long_term_pics
+            if (s.temporal_mvp) w.flag(h.tmvp);
         }
         if (s.sao) {
             w.flag(h.sao_luma);
             if (s.chroma_format_idc) w.flag(h.sao_chroma);
+        }
+        if (h.slice_type != 2) {
+            const bool b = h.slice_type == 0;
+            w.flag(h.num_ref_override);
+            if (h.num_ref_override) {
+                w.ue((uint32_t)(h.num_ref[0] - 1));
+                if (b) w.ue((uint32_t)(h.num_ref[1] - 1));
+            }
+            const int total = num_pic_total_curr(h);
+            if (p.lists_modification && total > 1) {
+                for (int l = 0; l < (b ? 2 : 1); l++) {
+                    w.flag(h.list_mod[l]);
+                    if (h.list_mod[l])
+                        for (int i = 0; i < h.num_ref[l]; i++) w.bits(h.list_entry[l][(size_t)i], ceil_log2(total));
+                }
+            }
+            if (b) w.flag(h.mvd_l1_zero);
+            if (p.cabac_init_present) w.flag(h.cabac_init);
+            if (h.tmvp) {
+                if (b) w.flag(h.col_from_l0);
+                if (h.num_ref[h.col_from_l0 ? 0 : 1] > 1) w.ue((uint32_t)h.col_ref_idx);
+            }
+            if (slice_weighted(h, p)) write_pred_weights(w, h, s);
+            w.ue((uint32_t)(5 - h.max_merge));
         }
         w.se(h.slice_qp_delta);
         if (p.slice_chroma_qp_offsets_present) {
@@ -479,6 +688,7 @@ int parse_sps(const std::vector<uint8_t>& rbsp, Sps& s) {
     if (s.bit_depth_luma > 16 || s.bit_depth_chroma > 16)
         return set_error(HPVCD_E_UNSUPPORTED_BITDEPTH, "bit depth");
     s.log2_max_poc_lsb = (int)r.ue() + 4;
+    if (s.log2_max_poc_lsb > 16) return set_error(HPVCD_E_PARAMSET, "log2_max_pic_order_cnt_lsb");
     const bool ordering = r.flag();
     for (int i = ordering ? 0 : max_sub_layers_minus1; i <= max_sub_layers_minus1; i++) {
         r.ue();
@@ -512,11 +722,20 @@ int parse_sps(const std::vector<uint8_t>& rbsp, Sps& s) {
         s.log2_max_pcm_cb = s.log2_min_pcm_cb + (int)r.ue();
         s.pcm_loop_filter_disabled = r.flag();
     }
-    s.num_short_term_rps = (int)r.ue();
-    if (s.num_short_term_rps) return set_error(HPVCD_E_UNSUPPORTED, "SPS short-term reference picture sets");
+    {
+        const uint32_t n = r.ue();
+        if (n > 64) return set_error(HPVCD_E_PARAMSET, "num_short_term_ref_pic_sets");
+        for (uint32_t i = 0; i < n; i++) {
+            StRps rps;
+            if (int st = parse_st_rps(r, (int)i, (int)n, s.st_rps, rps)) return st;
+            s.st_rps.push_back(rps);
+        }
+    }
     s.long_term_refs = r.flag();
     if (s.long_term_refs) {
         const uint32_t n = r.ue();
+        if (n > 32) return set_error(HPVCD_E_PARAMSET, "num_long_term_ref_pics_sps");
+        s.num_lt_sps = (int)n;
         for (uint32_t i = 0; i < n; i++) {
             r.bits(s.log2_max_poc_lsb);
             r.bit();
@@ -671,12 +890,12 @@ int parse_pps(const std::vector<uint8_t>& rbsp, const Sps& s, Pps& p) {
 }
 
 int parse_slice_header(const std::vector<uint8_t>& rbsp, const Sps& s, const Pps& p, uint8_t nal_type,
-                       SliceHeader& h) {
+                       SliceHeader& h, bool allow_inter) {
     BitReader r(rbsp.data(), rbsp.size());
     h = SliceHeader();
     r.bits(16);
     h.first_slice = r.flag();
-    if (nal_type >= 16 && nal_type <= 23) r.bit();
+    if (nal_type >= 16 && nal_type <= 23) h.no_output_of_prior_pics = r.flag();
     if ((int)r.ue() != p.pps_id) return set_error(HPVCD_E_PARAMSET, "slice refers to another PPS");
     const int ctb = 1 << s.log2_ctb;
     const int nctb = ((s.width + ctb - 1) / ctb) * ((s.height + ctb - 1) / ctb);
@@ -689,28 +908,62 @@ int parse_slice_header(const std::vector<uint8_t>& rbsp, const Sps& s, const Pps
     r.bits(p.num_extra_slice_header_bits);
     h.slice_type = (int)r.ue();
     if (h.slice_type > 2) return set_error(HPVCD_E_BITSTREAM, "slice_type");
-    if (h.slice_type != 2) return set_error(HPVCD_E_UNSUPPORTED, "P/B slices in a still-image item");
-    if (p.output_flag_present) r.bit();
+    if (h.slice_type != 2 && !allow_inter) return set_error(HPVCD_E_UNSUPPORTED, "P/B slices in a still-image item");
+    if (p.output_flag_present) h.pic_output = r.flag();
     if (nal_type != NAL_IDR_W_RADL && nal_type != NAL_IDR_N_LP) {
         h.poc_lsb = (int)r.bits(s.log2_max_poc_lsb);
-        if (!r.flag()) {  // inline short-term set of an intra picture: skip it
-            const uint32_t nn = r.ue(), np = r.ue();
-            if (nn > 16 || np > 16) return set_error(HPVCD_E_BITSTREAM, "short-term RPS");
-            for (uint32_t i = 0; i < nn + np; i++) {
-                r.ue();
-                r.bit();
-            }
-        } else if (s.num_short_term_rps > 1) {
-            r.bits(ceil_log2(s.num_short_term_rps));
+        const int nsets = (int)s.st_rps.size();
+        h.rps_from_sps = r.flag();
+        if (!h.rps_from_sps) {
+            if (int st = parse_st_rps(r, nsets, nsets, s.st_rps, h.rps)) return st;
+        } else {
+            if (nsets == 0) return set_error(HPVCD_E_BITSTREAM, "short_term_ref_pic_set_sps_flag without SPS sets");
+            h.rps_idx = nsets > 1 ? (int)r.bits(ceil_log2(nsets)) : 0;
+            if (h.rps_idx >= nsets) return set_error(HPVCD_E_BITSTREAM, "short_term_ref_pic_set_idx");
+            h.rps = s.st_rps[(size_t)h.rps_idx];
         }
         if (s.long_term_refs) {
-            if (r.ue()) return set_error(HPVCD_E_UNSUPPORTED, "long-term references");
+            const uint32_t n_sps = s.num_lt_sps ? r.ue() : 0;
+            if (n_sps || r.ue()) return set_error(HPVCD_E_UNSUPPORTED, "long-term reference pictures");
         }
-        if (s.temporal_mvp) r.bit();
+        if (s.temporal_mvp) h.tmvp = r.flag();
     }
     if (s.sao) {
         h.sao_luma = r.flag();
         if (s.chroma_format_idc) h.sao_chroma = r.flag();
+    }
+    if (h.slice_type != 2) {
+        const bool b = h.slice_type == 0;
+        h.num_ref[0] = p.num_ref_idx_default[0];
+        h.num_ref[1] = b ? p.num_ref_idx_default[1] : 0;
+        h.num_ref_override = r.flag();
+        if (h.num_ref_override) {
+            h.num_ref[0] = (int)r.ue() + 1;
+            if (b) h.num_ref[1] = (int)r.ue() + 1;
+        }
+        if (h.num_ref[0] > 15 || h.num_ref[1] > 15) return set_error(HPVCD_E_BITSTREAM, "num_ref_idx_active");
+        const int total = num_pic_total_curr(h);
+        if (p.lists_modification && total > 1) {
+            for (int l = 0; l < (b ? 2 : 1); l++) {
+                h.list_mod[l] = r.flag();
+                if (h.list_mod[l])
+                    for (int i = 0; i < h.num_ref[l]; i++) h.list_entry[l].push_back(r.bits(ceil_log2(total)));
+            }
+        }
+        if (b) h.mvd_l1_zero = r.flag();
+        if (p.cabac_init_present) h.cabac_init = r.flag();
+        if (h.tmvp) {
+            if (b) h.col_from_l0 = r.flag();
+            if (h.num_ref[h.col_from_l0 ? 0 : 1] > 1) h.col_ref_idx = (int)r.ue();
+            if (h.col_ref_idx >= h.num_ref[h.col_from_l0 ? 0 : 1])
+                return set_error(HPVCD_E_BITSTREAM, "collocated_ref_idx");
+        }
+        if (slice_weighted(h, p)) {
+            if (int st = parse_pred_weights(r, h, s)) return st;
+        }
+        const uint32_t five_minus = r.ue();
+        if (five_minus > 4) return set_error(HPVCD_E_BITSTREAM, "five_minus_max_num_merge_cand");
+        h.max_merge = 5 - (int)five_minus;
     }
     h.slice_qp_delta = r.se();
     if (p.slice_chroma_qp_offsets_present) {

@@ -13,6 +13,7 @@
 namespace hpvcd {
 
 enum NalType : uint8_t {
+    NAL_TRAIL_N = 0,
     NAL_TRAIL_R = 1,
     NAL_IDR_W_RADL = 19,
     NAL_IDR_N_LP = 20,
@@ -34,6 +35,38 @@ struct ScalingLists {
     void refresh_flat(int size_id, int m);
 };
 
+// A short-term reference picture set (7.3.7, 7.4.8): delta POCs of the
+// pictures before the current one (S0, nearest first) and after it (S1,
+// nearest first), each with used_by_curr_pic_flag.
+struct StRps {
+    std::vector<int32_t> d0, d1;
+    std::vector<uint8_t> u0, u1;
+    int num() const { return (int)(d0.size() + d1.size()); }
+    bool operator==(const StRps& o) const { return d0 == o.d0 && d1 == o.d1 && u0 == o.u0 && u1 == o.u1; }
+};
+
+// st_ref_pic_set(idx) with num_sets sets in the SPS (idx == num_sets: the
+// slice header's own set); predicted sets derive from sets[idx - delta_idx]
+// (src/rps.rs:74 parse_short_term_rps, :148 derive_inter_rps)
+int parse_st_rps(BitReader& r, int idx, int num_sets, const std::vector<StRps>& sets, StRps& out);
+// writes `rps` predicted from the previous set when a delta POC makes it
+// derivable (inter_ref_pic_set_prediction_flag), else explicitly
+void write_st_rps(BitWriter& w, int idx, int num_sets, const std::vector<StRps>& sets, const StRps& rps);
+// 7.4.8 derivation of a predicted set
+StRps derive_predicted_rps(const StRps& ref, int delta_rps, const std::vector<uint8_t>& used,
+                           const std::vector<uint8_t>& use_delta);
+
+// pred_weight_table (7.3.6.3): the syntax values of each list entry
+struct WpEntry {
+    bool luma = false, chroma = false;
+    int delta_weight = 0, offset = 0;            // delta_luma_weight_lX, luma_offset_lX
+    int delta_cweight[2] = {0, 0}, delta_coffset[2] = {0, 0};
+};
+struct PredWeights {
+    int luma_log2_denom = 0, chroma_log2_denom = 0;
+    std::vector<WpEntry> e[2];
+};
+
 struct Sps {
     int sps_id = 0;
     int chroma_format_idc = 1;
@@ -51,7 +84,9 @@ struct Sps {
     int pcm_bit_depth_luma = 8, pcm_bit_depth_chroma = 8;
     int log2_min_pcm_cb = 3, log2_max_pcm_cb = 5;
     bool pcm_loop_filter_disabled = false;
-    int num_short_term_rps = 0;
+    std::vector<StRps> st_rps;      // num_short_term_ref_pic_sets sets
+    int max_dec_pic_buffering = 5, max_num_reorder = 0;
+    int num_lt_sps = 0;             // num_long_term_ref_pics_sps
     bool long_term_refs = false, temporal_mvp = false;
     bool strong_intra_smoothing = false;
     // VUI colour description (E.2.1)
@@ -115,7 +150,27 @@ struct SliceHeader {
     bool lf_across_slices = false;
     std::vector<uint32_t> entry_points;   // sub-stream byte lengths (all but the last)
     size_t data_offset = 0;               // byte offset of the slice data in the RBSP (NAL header included)
+    bool no_output_of_prior_pics = false, pic_output = true;
+    // reference pictures (non-IDR) and P / B fields
+    bool rps_from_sps = false;            // short_term_ref_pic_set_sps_flag
+    int rps_idx = 0;                      // short_term_ref_pic_set_idx
+    StRps rps;                            // the picture's short-term set (inline or the SPS's)
+    bool tmvp = false;                    // slice_temporal_mvp_enabled_flag
+    bool num_ref_override = false;
+    int num_ref[2] = {0, 0};              // active entries (list 1: B slices)
+    bool list_mod[2] = {false, false};    // ref_pic_list_modification_flag_lX
+    std::vector<uint32_t> list_entry[2];
+    bool mvd_l1_zero = false, cabac_init = false;
+    bool col_from_l0 = true;
+    int col_ref_idx = 0;
+    PredWeights wp;                       // when the PPS enables weighting for the slice type
+    int max_merge = 5;                    // MaxNumMergeCand
 };
+
+// NumPicTotalCurr of a slice (7-55): used entries of its short-term set
+int num_pic_total_curr(const SliceHeader& h);
+// whether the slice carries a pred_weight_table
+bool slice_weighted(const SliceHeader& h, const Pps& pps);
 
 // uniform tile spacing (6-3 / 6-4) or explicit widths; column / row CTB boundaries
 void tile_boundaries(const Pps& pps, int ctb_cols, int ctb_rows, std::vector<int>& col_bd, std::vector<int>& row_bd);
@@ -130,7 +185,8 @@ void write_slice_header(BitWriter& w, const SliceHeader& h, const Sps& sps, cons
 // an hpvcd_status (0 = OK) and set the error message
 int parse_sps(const std::vector<uint8_t>& rbsp, Sps& sps);
 int parse_pps(const std::vector<uint8_t>& rbsp, const Sps& sps, Pps& pps);
+// allow_inter: P and B slices are parsed (a still-image item refuses them)
 int parse_slice_header(const std::vector<uint8_t>& rbsp, const Sps& sps, const Pps& pps, uint8_t nal_type,
-                       SliceHeader& h);
+                       SliceHeader& h, bool allow_inter = false);
 
 }  // namespace hpvcd

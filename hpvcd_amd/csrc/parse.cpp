@@ -53,13 +53,14 @@ public:
     bool bad() const { return bad_ || d_.failed(); }
     const char* why() const { return d_.failed() && !bad_ ? "slice data ends inside a coding tree unit" : why_; }
 
-    void begin_slice(int slice_qp) override {
+    void begin_slice(int slice_qp, int init_type) override {
         qp_ = slice_qp;
-        c_.init(0, slice_qp);
+        init_type_ = init_type;
+        c_.init(init_type, slice_qp);
         d_.start(data_, len_);
     }
     void begin_substream(int sync_row) override {
-        c_.init(0, qp_);
+        c_.init(init_type_, qp_);
         if (sync_row >= 0 && (size_t)sync_row < saved_.size()) c_ = saved_[(size_t)sync_row];
         // the engine starts at the substream's entry point when the slice
         // header gives one, else where the previous substream ended
@@ -192,6 +193,66 @@ public:
             if (v > HPVCD_COEFF_LEVEL_MAX || v < HPVCD_COEFF_LEVEL_MIN) fail("coefficient level beyond the record range");
         }
     }
+    // inter syntax (src/decode.rs:6290 cu_skip_flag, :6519 part_mode, :6571
+    // merge_idx, :6771 inter_pred_idc, :6878 ref_idx, :6929 mvd_coding)
+    bool cu_skip(int ctx_inc) override { return d_.decode(c_.s[CTX_SKIP + ctx_inc]) != 0; }
+    bool pred_mode_intra() override { return d_.decode(c_.s[CTX_PRED_MODE]) != 0; }
+    int part_mode_inter(int log2, bool min_size, bool amp) override {
+        if (d_.decode(c_.s[CTX_PART_MODE])) return PART_2Nx2N;
+        const bool horizontal = d_.decode(c_.s[CTX_PART_MODE + 1]) != 0;
+        if (!min_size) {
+            if (!amp || d_.decode(c_.s[CTX_PART_MODE + 3])) return horizontal ? PART_2NxN : PART_Nx2N;
+            const bool far = d_.bypass() != 0;
+            return horizontal ? (far ? PART_2NxnD : PART_2NxnU) : (far ? PART_nRx2N : PART_nLx2N);
+        }
+        if (horizontal) return PART_2NxN;
+        if (log2 == 3) return PART_Nx2N;
+        return d_.decode(c_.s[CTX_PART_MODE + 2]) ? PART_Nx2N : PART_NxN;
+    }
+    bool merge_flag() override { return d_.decode(c_.s[CTX_MERGE_FLAG]) != 0; }
+    int merge_idx(int max_cand) override {
+        if (!d_.decode(c_.s[CTX_MERGE_IDX])) return 0;
+        int idx = 1;
+        while (idx < max_cand - 1 && d_.bypass()) idx++;
+        return idx;
+    }
+    int inter_pred_idc(int ct_depth, bool no_bi) override {
+        if (!no_bi && d_.decode(c_.s[CTX_INTER_PRED + ct_depth])) return 2;
+        return d_.decode(c_.s[CTX_INTER_PRED + 4]) ? 1 : 0;
+    }
+    int ref_idx(int num) override {
+        if (!d_.decode(c_.s[CTX_REF_IDX])) return 0;
+        if (num == 2) return 1;
+        if (!d_.decode(c_.s[CTX_REF_IDX + 1])) return 1;
+        int idx = 2;
+        while (idx < num - 1 && d_.bypass()) idx++;
+        return idx;
+    }
+    void mv_syntax(const PuGeom&, int, const int16_t (*)[2], bool mvd_zero, int16_t mvd[2], int& mvp_flag) override {
+        mvd[0] = mvd[1] = 0;
+        if (!mvd_zero) {
+            bool g0[2], g1[2] = {false, false};
+            for (int c = 0; c < 2; c++) g0[c] = d_.decode(c_.s[CTX_MVD]) != 0;
+            for (int c = 0; c < 2; c++)
+                if (g0[c]) g1[c] = d_.decode(c_.s[CTX_MVD + 1]) != 0;
+            for (int c = 0; c < 2; c++) {
+                if (!g0[c]) continue;
+                uint32_t a = 1;
+                if (g1[c]) {
+                    uint32_t v;
+                    if (!dec_egk(d_, 1, v) || v > 65534) {
+                        fail("abs_mvd_minus2 out of range");
+                        return;
+                    }
+                    a = v + 2;
+                }
+                // the value wraps to 16 bits as the reference's i16 cast does
+                mvd[c] = (int16_t)(uint16_t)(d_.bypass() ? 0u - a : a);
+            }
+        }
+        mvp_flag = d_.decode(c_.s[CTX_MVP_FLAG]);
+    }
+    bool rqt_root_cbf() override { return d_.decode(c_.s[CTX_RQT_ROOT_CBF]) != 0; }
 
 private:
     void fail(const char* why) {
@@ -208,7 +269,7 @@ private:
     std::vector<size_t> sub_starts_;
     size_t substream_ = 0;
     CabacDec d_;
-    int qp_ = 26;
+    int qp_ = 26, init_type_ = 0;
     bool bad_ = false;
     const char* why_ = "";
 };
@@ -303,19 +364,35 @@ hpvcd_status check_params(const Sps& s, const Pps& p) {
 
 }  // namespace
 
-hpvcd_status parse_picture(const Sps& sps, const Pps& pps, const CodedPicture& cp, ParsedPicture& out) {
-    if (hpvcd_status st = check_params(sps, pps)) return st;
+hpvcd_status parse_headers(const Sps& sps, const Pps& pps, const CodedPicture& cp, std::vector<SliceHeader>& hdr,
+                           bool allow_inter) {
     const size_t nsl = cp.slices.size();
     if (nsl == 0) return set_error(HPVCD_E_BITSTREAM, "picture without slices");
     if (nsl > HPVCD_MAX_SLICES) return set_error(HPVCD_E_UNSUPPORTED, "more than 8 slices in a picture");
-    std::vector<SliceHeader> hdr(nsl);
+    hdr.assign(nsl, SliceHeader());
     for (size_t k = 0; k < nsl; k++) {
         const auto& rb = cp.slices[k];
         if (rb.size() < 3) return set_error(HPVCD_E_BITSTREAM, "slice NAL unit truncated");
         const uint8_t type = (rb[0] >> 1) & 63;
-        if (int st = parse_slice_header(rb, sps, pps, type, hdr[k])) return (hpvcd_status)st;
+        if (int st = parse_slice_header(rb, sps, pps, type, hdr[k], allow_inter)) return (hpvcd_status)st;
         if ((k == 0) != hdr[k].first_slice) return set_error(HPVCD_E_BITSTREAM, "first_slice_segment_in_pic_flag");
     }
+    return HPVCD_OK;
+}
+
+hpvcd_status parse_picture(const Sps& sps, const Pps& pps, const CodedPicture& cp, ParsedPicture& out,
+                           const std::vector<SliceHeader>* headers, const std::vector<const InterSlice*>* inter,
+                           int32_t poc) {
+    if (hpvcd_status st = check_params(sps, pps)) return st;
+    std::vector<SliceHeader> own;
+    if (!headers) {
+        if (hpvcd_status st = parse_headers(sps, pps, cp, own, false)) return st;
+        headers = &own;
+    }
+    const std::vector<SliceHeader>& hdr = *headers;
+    const size_t nsl = hdr.size();
+    if (nsl != cp.slices.size() || (inter && inter->size() != nsl))
+        return set_error(HPVCD_E_INVALID, "slice headers of another picture");
     // CTB layout: tile scan (6.5.1) and each CTB's slice
     const int ctb = 1 << sps.log2_ctb;
     const int cols = (sps.width + ctb - 1) / ctb, rows = (sps.height + ctb - 1) / ctb;
@@ -350,10 +427,11 @@ hpvcd_status parse_picture(const Sps& sps, const Pps& pps, const CodedPicture& c
     }
     CabacSource src(pps, P);
     out.rec = PicRecords();
-    PicWalker walker(P, src, out.rec);
+    PicWalker walker(P, src, out.rec, poc);
     walker.set_layout(ctb_slice, tile_of);
     for (size_t k = 0; k < nsl; k++) {
         const SliceHeader& h = hdr[k];
+        walker.set_inter(inter ? (*inter)[k] : nullptr);
         const int qp = pps.init_qp + h.slice_qp_delta;
         if (qp < -6 * (sps.bit_depth_luma - 8) || qp > 51) return set_error(HPVCD_E_BITSTREAM, "SliceQpY");
         const int cb = pps.cb_qp_offset + h.cb_qp_offset, cr = pps.cr_qp_offset + h.cr_qp_offset;
@@ -397,6 +475,7 @@ hpvcd_status parse_picture(const Sps& sps, const Pps& pps, const CodedPicture& c
             const int eos = walker.ctb((int)(rs % cols), (int)(rs / cols), ts == start[k], first_tile, last_slice,
                                        last_tile, qp);
             if (src.bad()) return set_error(HPVCD_E_CABAC_DESYNC, src.why());
+            if (walker.failed()) return set_error(HPVCD_E_BITSTREAM, "merge candidate beyond its reference list");
             if (eos != (last_slice ? 1 : 0))
                 return set_error(HPVCD_E_CABAC_DESYNC, eos ? "slice ends before its last coding tree unit"
                                                           : "slice data continues past the next slice's address");
@@ -543,9 +622,129 @@ hpvcd_status assemble_records(std::vector<ParsedPicture>& pics, const std::vecto
 
 using namespace hpvcd;
 
-// Annex B stream of intra pictures -> records, one output image per picture
-// (its conformance window, kind `out_kind` or, when negative, the kind
-// decode_heic gives: RGB or luma by chroma format, 8 or 16 bit by depth).
+namespace hpvcd {
+namespace {
+
+// A decoded picture in the decoded picture buffer: its POC, batch index,
+// reference marking and stored motion.
+struct DpbPic {
+    int32_t poc;
+    uint32_t index;
+    bool ref;
+    const ColField* col;
+};
+
+// The reference pictures of a P / B slice (src/dpb.rs:217 apply_rps_lt,
+// :298 build_ref_lists, :465 finalize_list): RefPicListTemp0 is
+// StCurrBefore then StCurrAfter (pictures the DPB holds as references;
+// others are left out), RefPicListTemp1 the reverse; each is repeated to
+// max(num_ref_idx_active, its length) entries and then taken in order or
+// through list_entry_lX.
+hpvcd_status slice_lists(const SliceHeader& h, int32_t cur_poc, const std::vector<DpbPic>& dpb, InterSlice& s) {
+    auto find = [&](int32_t poc) -> const DpbPic* {
+        for (const DpbPic& d : dpb)
+            if (d.ref && d.poc == poc) return &d;
+        return nullptr;
+    };
+    std::vector<const DpbPic*> before, after;
+    for (size_t k = 0; k < h.rps.d0.size(); k++)
+        if (h.rps.u0[k])
+            if (const DpbPic* d = find(cur_poc + h.rps.d0[k])) before.push_back(d);
+    for (size_t k = 0; k < h.rps.d1.size(); k++)
+        if (h.rps.u1[k])
+            if (const DpbPic* d = find(cur_poc + h.rps.d1[k])) after.push_back(d);
+    std::vector<const DpbPic*> temp[2];
+    temp[0] = before;
+    temp[0].insert(temp[0].end(), after.begin(), after.end());
+    temp[1] = after;
+    temp[1].insert(temp[1].end(), before.begin(), before.end());
+    for (int l = 0; l < (h.slice_type == 0 ? 2 : 1); l++) {
+        const int num = h.num_ref[l];
+        if (temp[l].empty() || num == 0)
+            return set_error(HPVCD_E_BITSTREAM, "reference picture list without pictures");
+        std::vector<const DpbPic*> ext;
+        const size_t len = std::max((size_t)num, temp[l].size());
+        while (ext.size() < len)
+            for (const DpbPic* d : temp[l])
+                if (ext.size() < len) ext.push_back(d);
+        for (int i = 0; i < num; i++) {
+            size_t e = (size_t)i;
+            if (h.list_mod[l]) {
+                e = h.list_entry[l][(size_t)i];
+                if (e >= ext.size()) return set_error(HPVCD_E_BITSTREAM, "list_entry beyond the reference pictures");
+            }
+            RefPic rp;
+            rp.poc = ext[e]->poc;
+            rp.pic = (int32_t)ext[e]->index;
+            s.list[l].push_back(rp);
+        }
+    }
+    return HPVCD_OK;
+}
+
+// The record rows of a slice's entries: the pred_weight_table values
+// (src/inter.rs:159 PredWeightTable::parse, :60 normalize_luma_offset, :68
+// derive_chroma_offset), or neutral rows of denominator 64 without one.
+void slice_rows(const SliceHeader& h, const Sps& sps, const Pps& pps, InterSlice& s) {
+    const bool weighted = slice_weighted(h, pps);
+    const PredWeights& t = h.wp;
+    for (int l = 0; l < 2; l++)
+        for (size_t i = 0; i < s.list[l].size(); i++) {
+            hpvcd_ref e;
+            std::memset(&e, 0, sizeof(e));
+            e.picture = (uint32_t)s.list[l][i].pic;
+            e.weighted = weighted ? 1 : 0;
+            e.log2_denom[0] = e.log2_denom[1] = 6;
+            for (int k = 0; k < 3; k++) e.weight[k] = 64;
+            if (weighted) {
+                const WpEntry& w = t.e[l][i];
+                e.log2_denom[0] = (uint8_t)t.luma_log2_denom;
+                e.log2_denom[1] = (uint8_t)t.chroma_log2_denom;
+                const int bd_y = sps.bit_depth_luma, bd_c = sps.bit_depth_chroma;
+                const bool hp = sps.high_precision_offsets;
+                const int range_y = hp ? 1 << (bd_y - 1) : 128, scale_y = hp ? 1 : 1 << (bd_y - 8);
+                const int range_c = hp ? 1 << (bd_c - 1) : 128, scale_c = hp ? 1 : 1 << (bd_c - 8);
+                e.weight[0] = (int16_t)((1 << t.luma_log2_denom) + (w.luma ? w.delta_weight : 0));
+                e.offset[0] = (int16_t)(w.luma ? std::clamp(w.offset, -range_y, range_y - 1) * scale_y : 0);
+                for (int j = 0; j < 2; j++) {
+                    const int wc = (1 << t.chroma_log2_denom) + (w.chroma ? w.delta_cweight[j] : 0);
+                    e.weight[1 + j] = (int16_t)wc;
+                    int off = 0;
+                    if (w.chroma) {
+                        const int64_t pred = range_c - (((int64_t)range_c * wc) >> t.chroma_log2_denom);
+                        off = (int)std::clamp<int64_t>(w.delta_coffset[j] + pred, -range_c, range_c - 1) * scale_c;
+                    }
+                    e.offset[1 + j] = (int16_t)off;
+                }
+            }
+            s.rows.push_back(e);
+        }
+}
+
+// 8.3.1 (src/rps.rs:208 derive_poc): the MSB from the previous TemporalId-0
+// picture
+int32_t derive_poc(int poc_lsb, int32_t prev, int log2_max) {
+    const int32_t max = 1 << log2_max;
+    const int32_t prev_lsb = ((prev % max) + max) % max, prev_msb = prev - prev_lsb;
+    int32_t msb = prev_msb;
+    if (poc_lsb < prev_lsb && prev_lsb - poc_lsb >= max / 2) msb = prev_msb + max;
+    else if (poc_lsb > prev_lsb && poc_lsb - prev_lsb > max / 2) msb = prev_msb - max;
+    return msb + poc_lsb;
+}
+
+}  // namespace
+}  // namespace hpvcd
+
+using namespace hpvcd;
+
+// Annex B stream -> records, pictures in decode order, one output image per
+// picture (its conformance window) in output order: by coded video
+// sequence, then POC.  Kind `out_kind`, or when negative the kind
+// decode_heic gives (RGB or luma by chroma format, 8 or 16 bit by depth).
+// P and B pictures follow the reference's video decoder
+// (src/video.rs:1045 decode_access_unit): POC, RPS marking, reference
+// lists, weights and the collocated picture per slice; RASL pictures of a
+// random-access point that starts the stream are skipped.
 extern "C" hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32_t out_kind, hpvcd_records** out) {
     if (!data || !out) return set_error(HPVCD_E_INVALID, "null argument");
     *out = nullptr;
@@ -570,8 +769,7 @@ extern "C" hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32
             if (int st = parse_pps(rb, sps, pps)) return (hpvcd_status)st;
             have_pps = true;
         } else if (type <= 21) {
-            if (type != 19 && type != 20 && type != 21 && type > 9)
-                return set_error(HPVCD_E_UNSUPPORTED, "reserved VCL NAL unit type");
+            if (type > 9 && type < 16) return set_error(HPVCD_E_UNSUPPORTED, "reserved VCL NAL unit type");
             if (!have_pps) return set_error(HPVCD_E_PARAMSET, "slice before its parameter sets");
             if (rb.size() < 3) return set_error(HPVCD_E_BITSTREAM, "slice NAL unit truncated");
             const bool first = (rb[2] & 0x80) != 0;
@@ -584,15 +782,106 @@ extern "C" hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32
         }
     }
     if (coded.empty()) return set_error(HPVCD_E_BITSTREAM, "no coded picture");
-    std::vector<ParsedPicture> pics(coded.size());
-    std::vector<hpvcd_output> outs;
+    std::vector<ParsedPicture> pics;
+    pics.reserve(coded.size());
+    std::vector<std::pair<int, int32_t>> out_key;   // (coded video sequence, POC) per picture
+    std::vector<uint8_t> shown;
+    std::vector<DpbPic> dpb;
+    int32_t prev_tid0 = 0;
+    int cvs = -1;
+    bool rasl_skip = false;
     for (size_t i = 0; i < coded.size(); i++) {
-        if (hpvcd_status st = parse_picture(active[i].first, active[i].second, coded[i], pics[i])) return st;
+        const Sps& S = active[i].first;
+        const Pps& Q = active[i].second;
+        std::vector<SliceHeader> hdr;
+        if (hpvcd_status st = parse_headers(S, Q, coded[i], hdr, true)) return st;
+        const uint8_t nal_type = (coded[i].slices[0][0] >> 1) & 63;
+        const uint8_t tid = (uint8_t)((coded[i].slices[0][1] & 7) - 1);
+        const bool irap = nal_type >= 16 && nal_type <= 23, idr = nal_type == 19 || nal_type == 20;
+        const bool rasl = nal_type == 8 || nal_type == 9, radl = nal_type == 6 || nal_type == 7;
+        // NoRaslOutputFlag: IDR, BLA, or a CRA that starts the stream
+        const bool no_rasl = irap && (nal_type < 21 || cvs < 0);
+        if (rasl && rasl_skip) continue;
+        int32_t poc;
+        if (irap && no_rasl) {
+            poc = idr ? 0 : hdr[0].poc_lsb;
+            for (DpbPic& d : dpb) d.ref = false;
+            cvs++;
+            rasl_skip = true;
+        } else {
+            if (cvs < 0) return set_error(HPVCD_E_BITSTREAM, "stream does not start with a random-access picture");
+            poc = derive_poc(hdr[0].poc_lsb, prev_tid0, S.log2_max_poc_lsb);
+        }
+        if (irap && !no_rasl) rasl_skip = false;
+        // RPS marking (8.3.2): pictures outside the set are no longer references
+        if (!idr) {
+            for (DpbPic& d : dpb) {
+                if (!d.ref) continue;
+                bool keep = false;
+                for (int32_t dp : hdr[0].rps.d0) keep |= d.poc == poc + dp;
+                for (int32_t dp : hdr[0].rps.d1) keep |= d.poc == poc + dp;
+                d.ref = keep;
+            }
+        }
+        std::vector<InterSlice> slices(hdr.size());
+        std::vector<const InterSlice*> inter(hdr.size(), nullptr);
+        for (size_t k = 0; k < hdr.size(); k++) {
+            const SliceHeader& h = hdr[k];
+            if (h.slice_type == 2) continue;
+            InterSlice& s = slices[k];
+            s.slice_type = h.slice_type;
+            s.cabac_init = h.cabac_init;
+            if (hpvcd_status st = slice_lists(h, poc, dpb, s)) return st;
+            slice_rows(h, S, Q, s);
+            s.tmvp = h.tmvp;
+            s.col_from_l0 = h.col_from_l0;
+            s.col_ref_idx = h.col_ref_idx;
+            if (h.tmvp) {
+                const std::vector<RefPic>& l = s.list[h.col_from_l0 ? 0 : 1];
+                if ((size_t)h.col_ref_idx < l.size())
+                    for (const DpbPic& d : dpb)
+                        if ((int32_t)d.index == l[(size_t)h.col_ref_idx].pic) s.col = d.col;
+            }
+            s.max_merge = h.max_merge;
+            s.par_mrg_level = Q.log2_parallel_merge_level;
+            s.mvd_l1_zero = h.mvd_l1_zero;
+            inter[k] = &s;
+        }
+        pics.emplace_back();
+        if (hpvcd_status st = parse_picture(S, Q, coded[i], pics.back(), &hdr, &inter, poc)) return st;
+        const uint32_t index = (uint32_t)(pics.size() - 1);
+        // (pics holds every picture without reallocating: the field stays put)
+        dpb.push_back(DpbPic{poc, index, true, &pics.back().rec.col});
+        // pictures that no longer serve as references or output leave the DPB
+        // bookkeeping once nothing can name them
+        dpb.erase(std::remove_if(dpb.begin(), dpb.end(), [](const DpbPic& d) { return !d.ref; }), dpb.end());
+        out_key.emplace_back(cvs, poc);
+        shown.push_back(hdr[0].pic_output ? 1 : 0);
+        if (tid == 0 && !rasl && !radl && !(nal_type <= 14 && nal_type % 2 == 0)) prev_tid0 = poc;
+    }
+    // output order: coded video sequence, then POC
+    std::vector<uint32_t> order;
+    for (uint32_t i = 0; i < (uint32_t)pics.size(); i++)
+        if (shown[i]) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return out_key[a] < out_key[b]; });
+    std::vector<hpvcd_output> outs;
+    std::vector<uint32_t> rank(pics.size(), 0);
+    for (uint32_t r = 0; r < (uint32_t)order.size(); r++) rank[order[r]] = r;
+    for (size_t i = 0; i < pics.size(); i++) {
         hpvcd_picture& p = pics[i].pic;
         const bool mono = p.chroma_idc == 0, deep = p.bit_depth_luma > 8;
         const int kind = out_kind >= 0 ? out_kind
                                        : (mono ? (deep ? HPVCD_OUT_LUMA16 : HPVCD_OUT_LUMA8)
                                                : (deep ? HPVCD_OUT_RGB16 : HPVCD_OUT_RGB8));
+        p.out_kind = (uint8_t)kind;
+        p.dst_x = p.dst_y = 0;
+        p.copy_w = shown[i] ? p.win_w : 0;
+        p.copy_h = shown[i] ? p.win_h : 0;
+        p.out_index = rank[i];
+    }
+    for (uint32_t r = 0; r < (uint32_t)order.size(); r++) {
+        const hpvcd_picture& p = pics[order[r]].pic;
+        const int kind = p.out_kind;
         hpvcd_output o;
         std::memset(&o, 0, sizeof(o));
         o.width = p.win_w;
@@ -601,11 +890,6 @@ extern "C" hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32
         o.channels = (kind == HPVCD_OUT_LUMA8 || kind == HPVCD_OUT_LUMA16) ? 1 : 3;
         o.bytes_per_sample = (kind == HPVCD_OUT_RGB16 || kind == HPVCD_OUT_LUMA16) ? 2 : 1;
         outs.push_back(o);
-        p.out_kind = (uint8_t)kind;
-        p.dst_x = p.dst_y = 0;
-        p.copy_w = p.win_w;
-        p.copy_h = p.win_h;
-        p.out_index = (uint32_t)i;
     }
     return assemble_records(pics, outs, out);
 }

@@ -27,7 +27,15 @@ struct ParsedPicture {
     std::vector<uint8_t> scaling;  // HPVCD_SCALING_SET_BYTES when scaling lists are on
 };
 
-hpvcd_status parse_picture(const Sps& sps, const Pps& pps, const CodedPicture& cp, ParsedPicture& out);
+// slice segment headers of a coded picture (allow_inter: P and B slices)
+hpvcd_status parse_headers(const Sps& sps, const Pps& pps, const CodedPicture& cp, std::vector<SliceHeader>& hdr,
+                           bool allow_inter);
+// The picture's slice data into records.  Without `headers` the headers are
+// parsed here and must be intra; `inter` gives each slice's inter parameters
+// (null entries: I slices) and `poc` the picture's POC.
+hpvcd_status parse_picture(const Sps& sps, const Pps& pps, const CodedPicture& cp, ParsedPicture& out,
+                           const std::vector<SliceHeader>* headers = nullptr,
+                           const std::vector<const InterSlice*>* inter = nullptr, int32_t poc = 0);
 
 // NAL units of a length-prefixed sample (size_len 1, 2 or 4 bytes) or of an
 // Annex B stream (size_len 0); each escaped, header included

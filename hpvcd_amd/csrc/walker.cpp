@@ -7,8 +7,8 @@
 
 namespace hpvcd {
 
-PicWalker::PicWalker(const PicParams& p, Source& s, PicRecords& o, const hpvcd_gen_config* g, uint32_t pic_index)
-    : P(p), src(s), out(o), gcfg(g) {
+PicWalker::PicWalker(const PicParams& p, Source& s, PicRecords& o, int32_t poc)
+    : P(p), src(s), out(o), poc_(poc) {
     w = P.w;
     h = P.h;
     gw = (w + 3) / 4;
@@ -36,8 +36,53 @@ PicWalker::PicWalker(const PicParams& p, Source& s, PicRecords& o, const hpvcd_g
         cu_chroma_modes[i] = kModeDC;
         cu_chroma_dm[i] = false;
     }
-    inter_pic = gcfg && gcfg->inter && pic_index > 0;
-    if (inter_pic) setup_refs(pic_index);
+    mp_.width = w;
+    mp_.height = h;
+    mp_.cur_poc = poc;
+    mp_.log2_ctb = P.log2_ctb;
+}
+
+void PicWalker::set_inter(const InterSlice* s) {
+    inter_ = s;
+    if (!s) return;
+    if (!have_motion_) {
+        const size_t ncell = (size_t)gw * gh;
+        motion.assign(ncell, MotionCell());
+        decoded.assign(ncell, 0);
+        tu_v.assign(ncell, 0);
+        tu_h.assign(ncell, 0);
+        pu_v.assign(ncell, 0);
+        pu_h.assign(ncell, 0);
+        nz.assign(ncell, 0);
+        have_motion_ = true;
+    }
+    // the slice's rows: shared with the previous inter slice when equal
+    bool same = last_rows_ && last_rows_->rows.size() == s->rows.size();
+    if (same)
+        same = s->rows.empty() ||
+               std::memcmp(last_rows_->rows.data(), s->rows.data(), s->rows.size() * sizeof(hpvcd_ref)) == 0;
+    if (!same) {
+        ref_base_ = (int)out.refs.size();
+        out.refs.insert(out.refs.end(), s->rows.begin(), s->rows.end());
+        last_rows_ = s;
+    }
+    mp_.is_b = s->slice_type == 0;
+    mp_.par_mrg_level = s->par_mrg_level;
+    mp_.max_merge = s->max_merge;
+    mp_.tmvp = s->tmvp;
+    mp_.col_from_l0 = s->col_from_l0;
+    mp_.col = s->col;
+    for (int l = 0; l < 2; l++) {
+        mp_.list[l] = s->list[l].data();
+        mp_.num[l] = (int)s->list[l].size();
+    }
+}
+
+// 6.4.1 availability for motion: decoded, same slice and tile
+// (src/decode.rs:6149)
+bool PicWalker::available(int x, int y) const {
+    if (x < 0 || y < 0 || x >= w || y >= h) return false;
+    return decoded[g_of(x, y)] && same_region(x, y);
 }
 
 void PicWalker::set_layout(const std::vector<uint8_t>& cs, const std::vector<uint8_t>& ct) {
@@ -58,7 +103,10 @@ int PicWalker::ctb(int rx, int ry, bool first_in_slice, bool first_in_tile, bool
     if (first_in_slice) {
         slice_qp = std::clamp(sqp, -qp_bd_off, 51);
         qp_y = slice_qp;
-        src.begin_slice(slice_qp);
+        // initType (9.3.2.2): 0 for I, P: 1 (2 with cabac_init_flag), B: 2 (1)
+        int init_type = 0;
+        if (inter_) init_type = inter_->slice_type == 1 ? (inter_->cabac_init ? 2 : 1) : (inter_->cabac_init ? 1 : 2);
+        src.begin_slice(slice_qp, init_type);
     } else if (first_in_tile) {
         // WPP: a CTB row that starts inside its tile takes the contexts the
         // row above stored, when that row's upper-right CTB is in the same
@@ -86,7 +134,19 @@ int PicWalker::ctb(int rx, int ry, bool first_in_slice, bool first_in_tile, bool
 }
 
 void PicWalker::finish() {
-    if (inter_pic) finalize_bs();
+    // collocated motion: the cell at each 16x16 block's corner
+    out.col.poc = poc_;
+    out.col.cw = out.col.ch = 0;
+    out.col.cells.clear();
+    if (have_motion_) {
+        finalize_bs();
+        out.col.cw = (w + 15) >> 4;
+        out.col.ch = (h + 15) >> 4;
+        out.col.cells.resize((size_t)out.col.cw * out.col.ch);
+        for (int by = 0; by < out.col.ch; by++)
+            for (int bx = 0; bx < out.col.cw; bx++)
+                out.col.cells[(size_t)by * out.col.cw + bx] = motion[g_of(bx << 4, by << 4)];
+    }
     out.cells.resize(cell.size());
     for (size_t i = 0; i < cell.size(); i++) {
         const int gx = (int)(i % (size_t)gw), gy = (int)(i / (size_t)gw);
@@ -108,12 +168,8 @@ void PicWalker::finish() {
 }
 
 // ---------------------------------------------------------------------------
-// inter pictures (generator only): low-delay reference lists, PUs, bS
+// GOP structure of the generator's random-access sequences
 // ---------------------------------------------------------------------------
-// Low-delay reference lists: list 0 holds the nearest earlier pictures,
-// list 1 (B pictures) the same pictures farthest first.  Each entry is a
-// pred_weight_table row (src/inter.rs:141): weights around 1 << denom,
-// offsets in 8-bit units scaled to the bit depth.
 std::vector<int> gop_decode_order(int frames, int gop, std::vector<int>* layer) {
     std::vector<int> order;
     if (layer) layer->assign((size_t)std::max(frames, 0), 0);
@@ -182,47 +238,9 @@ void gop_ref_lists(int frames, int gop, int num_refs, uint32_t index, std::vecto
     }
 }
 
-void PicWalker::setup_refs(uint32_t pic_index) {
-    Rng& rng = *src.rng();
-    const hpvcd_gen_config& cfg = *gcfg;
-    int n = std::max(1, std::min<int>(cfg.num_refs ? cfg.num_refs : 1, (int)pic_index));
-    std::vector<uint32_t> lists[2];
-    if (cfg.gop > 1) {
-        gop_ref_lists((int)(cfg.grid_cols * cfg.grid_rows), cfg.gop, cfg.num_refs ? cfg.num_refs : 1, pic_index,
-                      lists[0], lists[1]);
-    } else {
-        for (int i = 0; i < n; i++) {
-            lists[0].push_back(pic_index - 1 - (uint32_t)i);
-            lists[1].push_back(pic_index - (uint32_t)n + (uint32_t)i);
-        }
-    }
-    nref[0] = (int)lists[0].size();
-    nref[1] = cfg.p_bi > 0.f ? (int)lists[1].size() : 0;
-    for (int l = 0; l < 2; l++)
-        for (int i = 0; i < nref[l]; i++) {
-            hpvcd_ref e;
-            std::memset(&e, 0, sizeof(e));
-            e.picture = lists[l][(size_t)i];
-            e.weighted = cfg.weighted ? 1 : 0;
-            e.log2_denom[0] = 6;
-            e.log2_denom[1] = 6;
-            for (int k = 0; k < 3; k++) {
-                bool signalled = cfg.weighted && rng.bern(0.8f);
-                e.weight[k] = (int16_t)(signalled ? rng.sint(32, 96) : 64);
-                e.offset[k] = (int16_t)(signalled ? rng.sint(-16, 16) * (1 << (P.bd - 8)) : 0);
-            }
-            out.refs.push_back(e);
-            ref_pic.push_back(e.picture);
-        }
-    size_t ncell = (size_t)gw * gh;
-    motion.assign(ncell, Motion());
-    tu_v.assign(ncell, 0);
-    tu_h.assign(ncell, 0);
-    pu_v.assign(ncell, 0);
-    pu_h.assign(ncell, 0);
-    nz.assign(ncell, 0);
-}
-
+// ---------------------------------------------------------------------------
+// inter CUs
+// ---------------------------------------------------------------------------
 // transform-block boundary of an inter CU or TU (edge flags only; the
 // strength is derived once the picture is complete)
 void PicWalker::mark_tu_edges(int x0, int y0, int size) {
@@ -232,22 +250,10 @@ void PicWalker::mark_tu_edges(int x0, int y0, int size) {
         for (int xx = x0; xx < std::min(x0 + size, w); xx += 4) tu_h[g_of(xx, y0)] = 1;
 }
 
-bool PicWalker::motion_differs(const Motion& p, const Motion& q) {
-    // src/decode.rs:620 inter_motion_differs (reference pictures compared by identity)
-    int32_t pr[2] = {(p.flags & 1) ? p.pic[0] : -1, (p.flags & 2) ? p.pic[1] : -1};
-    int32_t qr[2] = {(q.flags & 1) ? q.pic[0] : -1, (q.flags & 2) ? q.pic[1] : -1};
-    bool direct = pr[0] == qr[0] && pr[1] == qr[1];
-    bool crossed = pr[0] == qr[1] && pr[1] == qr[0];
-    if (!direct && !crossed) return true;
-    auto mv = [](const Motion& m, int l, int c) -> int { return (m.flags >> l) & 1 ? m.mv[l][c] : 0; };
-    auto far = [&](int pl, int ql) {
-        return std::abs(mv(p, pl, 0) - mv(q, ql, 0)) >= 4 || std::abs(mv(p, pl, 1) - mv(q, ql, 1)) >= 4;
-    };
-    if (pr[0] != pr[1]) return direct ? (far(0, 0) || far(1, 1)) : (far(0, 1) || far(1, 0));
-    return (far(0, 0) || far(1, 1)) && (far(0, 1) || far(1, 0));
-}
-
-// src/decode.rs:2955 finalize_coeff_bs over the whole inter picture
+// src/decode.rs:2955 finalize_coeff_bs over the whole picture: an intra
+// side gives 2, coefficients at a transform edge or differing motion 1
+// (the per-PU marking of src/decode.rs:6329 gives the same strengths on
+// the 8x8 grid)
 void PicWalker::finalize_bs() {
     for (int gy = 0; gy < gh; gy++)
         for (int gx = 0; gx < gw; gx++) {
@@ -270,46 +276,96 @@ void PicWalker::finalize_bs() {
         }
 }
 
-double PicWalker::gauss(Rng& rng) {
-    double u1 = ((double)(rng.next() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
-    double u2 = (double)(rng.next() >> 11) * (1.0 / 9007199254740992.0);
-    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+// cu_skip_flag context (src/decode.rs:6290): skipped left / above CUs that
+// are decoded and in the same slice and tile
+int PicWalker::skip_ctx(int x0, int y0) const {
+    int inc = 0;
+    if (available(x0 - 1, y0) && motion[g_of(x0 - 1, y0)].skip) inc++;
+    if (available(x0, y0 - 1) && motion[g_of(x0, y0 - 1)].skip) inc++;
+    return inc;
 }
 
-// One PU: prediction lists (no bi for 8x4 / 4x8), reference entries and
-// quarter-sample motion vectors ~ N(0, mv_sigma) keeping the block within
-// 64 samples of the picture.
-void PicWalker::prediction_unit(int px, int py, int pw, int ph) {
-    Rng& rng = *src.rng();
+// a candidate's motion with the pictures of its references
+// (src/decode.rs:6699 cand_to_motion)
+MotionCell PicWalker::cell_of(const MergeCand& c) const {
+    MotionCell m;
+    m.intra = 0;
+    m.pred = c.pred;
+    for (int l = 0; l < 2; l++) {
+        m.ref_idx[l] = c.ref_idx[l];
+        m.mv[l][0] = c.mv[l][0];
+        m.mv[l][1] = c.mv[l][1];
+        if (((c.pred >> l) & 1) && c.ref_idx[l] >= 0 && c.ref_idx[l] < mp_.num[l]) {
+            const RefPic& r = mp_.list[l][c.ref_idx[l]];
+            m.poc[l] = r.poc;
+            m.pic[l] = r.pic;
+            if (r.lt) m.lt |= (uint8_t)(1 << l);
+        }
+    }
+    return m;
+}
+
+// the merge motion of a PU (src/decode.rs:6626 derive_pu_merge)
+MotionCell PicWalker::merge_motion(const PuGeom& g, int merge_idx) const {
+    return cell_of(derive_merge(*this, mp_, g, merge_idx));
+}
+
+// prediction_unit of a CU that is not skipped (src/decode.rs:6738):
+// merge_flag, then merge_idx or inter_pred_idc, ref_idx, mvd and mvp flag
+// per list, the vector the predictor plus the difference (16-bit wrap,
+// src/decode.rs:7521)
+MotionCell PicWalker::prediction_unit(const PuGeom& g, int log2_cb, bool& merge) {
+    merge = src.merge_flag();
+    if (merge) return merge_motion(g, mp_.max_merge > 1 ? src.merge_idx(mp_.max_merge) : 0);
+    int idc = 0;
+    if (mp_.is_b) idc = src.inter_pred_idc(std::min(P.log2_ctb - log2_cb, 4), g.w + g.h == 12);
+    MotionCell m;
+    m.intra = 0;
+    for (int l = 0; l < 2; l++) {
+        if ((l == 0 && idc == 1) || (l == 1 && idc == 0)) continue;
+        const int n = mp_.num[l];
+        const int ridx = n > 1 ? src.ref_idx(n) : 0;
+        const int32_t ref_poc = ridx < n ? mp_.list[l][ridx].poc : mp_.cur_poc;
+        int16_t cand[2][2], mvd[2] = {0, 0};
+        derive_amvp(*this, mp_, g, l, ref_poc, cand);
+        int flag = 0;
+        src.mv_syntax(g, l, cand, l == 1 && idc == 2 && inter_->mvd_l1_zero, mvd, flag);
+        m.pred |= (uint8_t)(1 << l);
+        m.ref_idx[l] = (int8_t)ridx;
+        m.mv[l][0] = (int16_t)(uint16_t)((uint32_t)(uint16_t)cand[flag][0] + (uint32_t)(uint16_t)mvd[0]);
+        m.mv[l][1] = (int16_t)(uint16_t)((uint32_t)(uint16_t)cand[flag][1] + (uint32_t)(uint16_t)mvd[1]);
+        m.poc[l] = ref_poc;
+        if (ridx < n) {
+            m.pic[l] = mp_.list[l][ridx].pic;
+            if (mp_.list[l][ridx].lt) m.lt |= (uint8_t)(1 << l);
+        }
+    }
+    return m;
+}
+
+// the PU's motion over its cells and its record
+void PicWalker::store_pu(const PuGeom& g, const MotionCell& m) {
+    for (int yy = g.y; yy < std::min(g.y + g.h, h); yy += 4)
+        for (int xx = g.x; xx < std::min(g.x + g.w, w); xx += 4) motion[g_of(xx, yy)] = m;
     hpvcd_pu pu;
     std::memset(&pu, 0, sizeof(pu));
-    pu.x = (uint16_t)px; pu.y = (uint16_t)py; pu.w = (uint8_t)pw; pu.h = (uint8_t)ph;
-    if (nref[1] > 0) {
-        if (pw + ph != 12 && rng.bern(gcfg->p_bi)) pu.flags = HPVCD_PU_L0 | HPVCD_PU_L1;
-        else pu.flags = (rng.next() & 1) ? HPVCD_PU_L0 : HPVCD_PU_L1;
-    } else {
-        pu.flags = HPVCD_PU_L0;
-    }
-    Motion m;
-    m.intra = 0;
-    m.flags = pu.flags;
+    pu.x = (uint16_t)g.x;
+    pu.y = (uint16_t)g.y;
+    pu.w = (uint8_t)g.w;
+    pu.h = (uint8_t)g.h;
+    pu.flags = m.pred;
     for (int l = 0; l < 2; l++) {
-        if (!((pu.flags >> l) & 1)) continue;
-        int idx = (int)rng.uni((uint32_t)nref[l]);
-        pu.ref[l] = (uint16_t)(l == 0 ? idx : nref[0] + idx);
-        double sig = gcfg->mv_sigma * 4.0;
-        int mvx = (int)std::lround(gauss(rng) * sig), mvy = (int)std::lround(gauss(rng) * sig);
-        mvx = std::clamp(mvx, (-64 - px - pw) * 4, (w + 64 - px) * 4);
-        mvy = std::clamp(mvy, (-64 - py - ph) * 4, (h + 64 - py) * 4);
-        pu.mv[l][0] = (int16_t)mvx;
-        pu.mv[l][1] = (int16_t)mvy;
-        m.pic[l] = (int32_t)ref_pic[pu.ref[l]];
-        m.mv[l][0] = pu.mv[l][0];
-        m.mv[l][1] = pu.mv[l][1];
+        if (!((m.pred >> l) & 1)) continue;
+        const int idx = m.ref_idx[l];
+        if (idx < 0 || idx >= mp_.num[l]) {
+            fail_ = true;  // a candidate beyond the list (not produced by conforming streams)
+            continue;
+        }
+        pu.ref[l] = (uint16_t)(ref_base_ + (l == 0 ? idx : mp_.num[0] + idx));
+        pu.mv[l][0] = m.mv[l][0];
+        pu.mv[l][1] = m.mv[l][1];
     }
     out.pus.push_back(pu);
-    for (int yy = py; yy < std::min(py + ph, h); yy += 4)
-        for (int xx = px; xx < std::min(px + pw, w); xx += 4) motion[g_of(xx, yy)] = m;
 }
 
 // TBs without residual tiling an inter CU that codes none (skip, or
@@ -326,48 +382,59 @@ void PicWalker::inter_tiles(int x0, int y0, int log2) {
     for (int comp = 1; comp <= 2; comp++) out.tbs.push_back(make_tb(x0 / sub_w, y0 / sub_h, cl, comp, kModeDC, false));
 }
 
-// src/decode.rs:6380 decode_inter_cu: part_mode, PUs in coding order,
-// then the residual tree or none
-void PicWalker::inter_cu(int x0, int y0, int log2) {
-    Rng& rng = *src.rng();
-    int cb = 1 << log2;
+// src/decode.rs:6380 decode_inter_cu: part_mode, PUs in coding order (each
+// PU's motion visible to the next), then rqt_root_cbf (inferred for a
+// 2Nx2N merge) and the residual tree, split at depth 0 by interSplitFlag
+void PicWalker::inter_cu(int x0, int y0, int log2, bool skip) {
+    const int cb = 1 << log2;
     cu_inter = true;
     for (int yy = y0; yy < std::min(y0 + cb, h); yy += 4)
         for (int xx = x0; xx < std::min(x0 + cb, w); xx += 4) cell[g_of(xx, yy)] &= ~HPVCD_CELL_INTRA;
     mark_tu_edges(x0, y0, cb);
-    bool skip = rng.bern(gcfg->p_skip);
-    int part = 0;  // 2Nx2N, 2NxN, Nx2N, NxN, 2NxnU, 2NxnD, nLx2N, nRx2N
-    if (!skip) {
-        int opts[8], n = 0;
-        opts[n++] = 0; opts[n++] = 1; opts[n++] = 2;
-        if (log2 == P.log2_min_cb && log2 > 3) opts[n++] = 3;
-        if (gcfg->amp && log2 > P.log2_min_cb) for (int k = 4; k < 8; k++) opts[n++] = k;
-        part = opts[rng.uni((uint32_t)n)];
-    }
-    int q = cb / 4, hf = cb / 2;
+    int part = PART_2Nx2N;
+    if (!skip) part = src.part_mode_inter(log2, log2 == P.log2_min_cb, P.amp && log2 > P.log2_min_cb);
+    const int q = cb / 4, hf = cb / 2;
     int rects[4][4], nr = 0;
-    auto add = [&](int x, int y, int rw, int rh) { rects[nr][0] = x; rects[nr][1] = y; rects[nr][2] = rw; rects[nr][3] = rh; nr++; };
+    auto add = [&](int x, int y, int rw, int rh) {
+        rects[nr][0] = x; rects[nr][1] = y; rects[nr][2] = rw; rects[nr][3] = rh; nr++;
+    };
     switch (part) {
-        case 0: add(x0, y0, cb, cb); break;
-        case 1: add(x0, y0, cb, hf); add(x0, y0 + hf, cb, hf); break;
-        case 2: add(x0, y0, hf, cb); add(x0 + hf, y0, hf, cb); break;
-        case 3: add(x0, y0, hf, hf); add(x0 + hf, y0, hf, hf); add(x0, y0 + hf, hf, hf); add(x0 + hf, y0 + hf, hf, hf); break;
-        case 4: add(x0, y0, cb, q); add(x0, y0 + q, cb, cb - q); break;
-        case 5: add(x0, y0, cb, cb - q); add(x0, y0 + cb - q, cb, q); break;
-        case 6: add(x0, y0, q, cb); add(x0 + q, y0, cb - q, cb); break;
+        case PART_2Nx2N: add(x0, y0, cb, cb); break;
+        case PART_2NxN: add(x0, y0, cb, hf); add(x0, y0 + hf, cb, hf); break;
+        case PART_Nx2N: add(x0, y0, hf, cb); add(x0 + hf, y0, hf, cb); break;
+        case PART_NxN: add(x0, y0, hf, hf); add(x0 + hf, y0, hf, hf); add(x0, y0 + hf, hf, hf); add(x0 + hf, y0 + hf, hf, hf); break;
+        case PART_2NxnU: add(x0, y0, cb, q); add(x0, y0 + q, cb, cb - q); break;
+        case PART_2NxnD: add(x0, y0, cb, cb - q); add(x0, y0 + cb - q, cb, q); break;
+        case PART_nLx2N: add(x0, y0, q, cb); add(x0 + q, y0, cb - q, cb); break;
         default: add(x0, y0, cb - q, cb); add(x0 + cb - q, y0, q, cb); break;
     }
+    bool last_merge = false;
     for (int i = 0; i < nr; i++) {
-        int px = rects[i][0], py = rects[i][1], pw = rects[i][2], ph = rects[i][3];
-        if (px > x0)
-            for (int yy = py; yy < std::min(py + ph, h); yy += 4) pu_v[g_of(px, yy)] = 1;
-        if (py > y0)
-            for (int xx = px; xx < std::min(px + pw, w); xx += 4) pu_h[g_of(xx, py)] = 1;
-        prediction_unit(px, py, pw, ph);
+        PuGeom g;
+        g.x = rects[i][0]; g.y = rects[i][1]; g.w = rects[i][2]; g.h = rects[i][3];
+        g.cu_x = x0; g.cu_y = y0; g.cu_w = cb; g.cu_h = cb; g.part_idx = i;
+        if (g.x > x0)
+            for (int yy = g.y; yy < std::min(g.y + g.h, h); yy += 4) pu_v[g_of(g.x, yy)] = 1;
+        if (g.y > y0)
+            for (int xx = g.x; xx < std::min(g.x + g.w, w); xx += 4) pu_h[g_of(xx, g.y)] = 1;
+        MotionCell m;
+        if (skip) {
+            m = merge_motion(g, mp_.max_merge > 1 ? src.merge_idx(mp_.max_merge) : 0);
+            last_merge = true;
+        } else {
+            m = prediction_unit(g, log2, last_merge);
+        }
+        m.skip = skip ? 1 : 0;
+        store_pu(g, m);
     }
-    bool root_cbf = !skip && rng.bern(0.85f);
-    if (root_cbf) transform_tree(x0, y0, x0, y0, log2, 0, 0, false, P.max_depth_inter, false, false);
-    else inter_tiles(x0, y0, log2);
+    bool root_cbf = false;
+    if (!skip) root_cbf = (part == PART_2Nx2N && last_merge) ? true : src.rqt_root_cbf();
+    if (root_cbf) {
+        const bool inter_split = P.max_depth_inter == 0 && part != PART_2Nx2N;
+        transform_tree(x0, y0, x0, y0, log2, 0, 0, inter_split, P.max_depth_inter + (inter_split ? 1 : 0), false, false);
+    } else {
+        inter_tiles(x0, y0, log2);
+    }
     for (int yy = y0; yy < std::min(y0 + cb, h); yy += 4)
         for (int xx = x0; xx < std::min(x0 + cb, w); xx += 4) cell_qp[g_of(xx, yy)] = (int8_t)qp_y;
     qp_prev = qp_y;
@@ -601,9 +668,9 @@ void PicWalker::act_unit(int x0, int y0, int log2, uint8_t luma_mode, bool cbf_l
 void PicWalker::transform_unit(int x0, int y0, int xbase, int ybase, int log2, int depth, int blk, bool cbf_cb,
                                bool cbf_cr) {
     // cbf_luma of an inter CU's root TU without chroma residual is inferred
-    // (src/decode.rs:3459); the generator's inter pictures draw it
+    // (src/decode.rs:3459)
     bool cbf_luma;
-    if (cu_inter && depth == 0 && !cbf_cb && !cbf_cr && !src.rng()) cbf_luma = true;
+    if (cu_inter && depth == 0 && !cbf_cb && !cbf_cr) cbf_luma = true;
     else cbf_luma = src.cbf_luma(depth);
     int size = 1 << log2;
     if (cu_inter) {
@@ -689,18 +756,23 @@ void PicWalker::pcm_unit(int x0, int y0, int log2) {
     qp_prev = qp_y;
 }
 
-// src/decode.rs:2585 coding_unit (intra CUs of I slices; inter CUs of the
-// generator's inter pictures)
+// src/decode.rs:2585 coding_unit: cu_transquant_bypass_flag, then in P and
+// B slices cu_skip_flag and pred_mode_flag
 void PicWalker::coding_unit(int x0, int y0, int log2) {
     int size = 1 << log2;
     const int qgm = ~((1 << log2_qg) - 1);
     if ((x0 & qgm) == x0 && (y0 & qgm) == y0) qp_y = predict_qp(x0, y0);
     cu_tqb = P.tqb_enabled && src.tqb();
     if (cu_tqb) fill_cells(x0, y0, size, HPVCD_CELL_TQB);
-    // cu_skip_flag / pred_mode_flag of an inter picture
-    if (inter_pic && !src.rng()->bern(gcfg->p_intra_cu)) {
-        inter_cu(x0, y0, log2);
-        return;
+    if (inter_) {
+        if (src.cu_skip(skip_ctx(x0, y0))) {
+            inter_cu(x0, y0, log2, true);
+            return;
+        }
+        if (!src.pred_mode_intra()) {
+            inter_cu(x0, y0, log2, false);
+            return;
+        }
     }
     bool nxn = log2 == P.log2_min_cb && src.intra_nxn();
     // pcm_flag follows part_mode for 2Nx2N CUs of the PCM size range (src/decode.rs:2647)
@@ -760,6 +832,10 @@ void PicWalker::coding_quadtree(int x0, int y0, int log2, int depth) {
         for (int yy = y0; yy < std::min(y0 + size, h); yy += 4)
             for (int xx = x0; xx < std::min(x0 + size, w); xx += 4) ct_depth[g_of(xx, yy)] = (uint8_t)depth;
         coding_unit(x0, y0, log2);
+        // the CU is decoded: its cells become available to later CUs' motion
+        if (have_motion_)
+            for (int yy = y0; yy < std::min(y0 + size, h); yy += 4)
+                for (int xx = x0; xx < std::min(x0 + size, w); xx += 4) decoded[g_of(xx, yy)] = 1;
     }
 }
 

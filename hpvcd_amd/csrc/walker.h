@@ -21,6 +21,7 @@
 
 #include "../../include/hpvcd.h"
 #include "cabac.h"
+#include "motion.h"
 
 namespace hpvcd {
 
@@ -46,6 +47,26 @@ struct PicParams {
     bool wpp = false;                // entropy_coding_sync_enabled_flag: a substream per CTB row of a tile
 };
 
+// The inter parameters of one P or B slice: reference lists, the record
+// rows of its entries (list 0, then list 1: picture and weights), and the
+// merge / AMVP / temporal parameters of its header.
+struct InterSlice {
+    int slice_type = 1;               // 0 B, 1 P
+    bool cabac_init = false;          // cabac_init_flag
+    std::vector<RefPic> list[2];
+    std::vector<hpvcd_ref> rows;      // list 0 entries, then list 1 entries
+    bool tmvp = false;
+    bool col_from_l0 = true;
+    int col_ref_idx = 0;
+    const ColField* col = nullptr;    // the collocated picture's motion (resolved by the caller)
+    int max_merge = 5;
+    int par_mrg_level = 2;
+    bool mvd_l1_zero = false;
+};
+
+// inter part modes (H.265 Table 7-10 order)
+enum PartMode { PART_2Nx2N = 0, PART_2NxN, PART_Nx2N, PART_NxN, PART_2NxnU, PART_2NxnD, PART_nLx2N, PART_nRx2N };
+
 constexpr uint8_t kModePlanar = 0, kModeDC = 1;
 
 inline uint64_t splitmix64_step(uint64_t& s) {
@@ -68,7 +89,8 @@ struct Rng {
 // walker derived.
 struct Source {
     virtual ~Source() = default;
-    virtual void begin_slice(int slice_qp) = 0;
+    // init_type: 0 for I slices, 1 or 2 for P / B (9.3.2.2 with cabac_init_flag)
+    virtual void begin_slice(int slice_qp, int init_type) = 0;
     // A new substream that is not the slice's first: a tile, or a CTB row of
     // a tile with WPP.  The contexts start from their initial values, or,
     // when sync_row >= 0, from the ones wpp_save stored for that CTB row
@@ -97,8 +119,21 @@ struct Source {
     virtual int qp_delta() = 0;             // CuQpDeltaVal
     virtual int res_scale(int c) = 0;       // ResScaleVal of Cb (0) / Cr (1)
     virtual void residual(const ResidualParams& p, int32_t* levels, ResidualOut& r) = 0;
-    // generator-only inter decisions (the parser refuses P/B slices)
-    virtual Rng* rng() { return nullptr; }
+    // inter syntax (P and B slices)
+    virtual bool cu_skip(int ctx_inc) = 0;
+    virtual bool pred_mode_intra() = 0;                       // pred_mode_flag
+    // part_mode of an inter CU (PartMode); min_size: log2 == MinCbLog2SizeY,
+    // amp: AMP partitions are allowed
+    virtual int part_mode_inter(int log2, bool min_size, bool amp) = 0;
+    virtual bool merge_flag() = 0;
+    virtual int merge_idx(int max_cand) = 0;                  // coded when MaxNumMergeCand > 1
+    virtual int inter_pred_idc(int ct_depth, bool no_bi) = 0; // 0 PRED_L0, 1 PRED_L1, 2 PRED_BI
+    virtual int ref_idx(int num_active) = 0;                  // coded when more than one entry
+    // mvd_coding (unless mvd_zero: MvdL1 inferred zero) and mvp_lX_flag of
+    // one list of a PU; cand are its two AMVP predictors
+    virtual void mv_syntax(const PuGeom& pu, int list, const int16_t cand[2][2], bool mvd_zero, int16_t mvd[2],
+                           int& mvp_flag) = 0;
+    virtual bool rqt_root_cbf() = 0;
 };
 
 // One picture's records as the walker emits them (TBs in decode order, CTBs
@@ -110,13 +145,12 @@ struct PicRecords {
     std::vector<hpvcd_pu> pus;
     std::vector<hpvcd_ref> refs;
     int num_slices = 1, num_tiles = 1;
+    ColField col;   // the picture's motion as a later picture's collocated motion
 };
 
-class PicWalker {
+class PicWalker : private Neighbours {
 public:
-    // gcfg: the generator's configuration for its inter decisions (null when parsing)
-    PicWalker(const PicParams& P, Source& src, PicRecords& out, const hpvcd_gen_config* gcfg = nullptr,
-              uint32_t pic_index = 0);
+    PicWalker(const PicParams& P, Source& src, PicRecords& out, int32_t poc = 0);
     // CTB layout: per raster CTB its slice and tile; the walk visits CTBs in
     // ts_to_rs order.  slice_qp per slice.
     void set_layout(const std::vector<uint8_t>& ctb_slice, const std::vector<uint8_t>& ctb_tile);
@@ -125,26 +159,24 @@ public:
         chroma_off[0] = cb;
         chroma_off[1] = cr;
     }
+    // the inter parameters of the slice that follows (null: an I slice); the
+    // slice's reference rows join the picture's unless equal to the last ones
+    void set_inter(const InterSlice* s);
     // one CTB (SAO, coding quadtree, end flags); first_in_slice / first_in_tile
     // (the first CTB of a substream: a tile, or with WPP a CTB row of a tile)
     // reset the QP predictor; last_in_tile ends a substream; returns the
     // end_of_slice_segment_flag
     int ctb(int rx, int ry, bool first_in_slice, bool first_in_tile, bool last_in_slice, bool last_in_tile,
             int slice_qp);
-    // cells (QP, slice, tile packed), TB order, inter bS
+    // cells (QP, slice, tile packed), TB order, inter bS, collocated motion
     void finish();
     bool failed() const { return fail_; }
 
 private:
-    struct Motion {
-        uint8_t intra = 1, flags = 0;
-        int32_t pic[2] = {-1, -1};
-        int16_t mv[2][2] = {{0, 0}, {0, 0}};
-    };
     PicParams P;
     Source& src;
     PicRecords& out;
-    const hpvcd_gen_config* gcfg;
+    int32_t poc_;
     int w, h, gw, gh, sub_w, sub_h, ctb_cols, ctb_rows;
     int qp_y, slice_qp = 0, qp_prev = 0, qp_bd_off = 0, log2_qg = 6;
     int chroma_off[2] = {0, 0};
@@ -161,25 +193,31 @@ private:
     bool cu_chroma_dm[4];
     int cu_x0 = 0, cu_y0 = 0, cu_pu = 1;
     bool cu_tqb = false, cu_inter = false;
-    // inter pictures (generator only)
-    bool inter_pic = false;
-    int nref[2] = {0, 0};
-    std::vector<uint32_t> ref_pic;
-    std::vector<Motion> motion;
-    std::vector<uint8_t> tu_v, tu_h, pu_v, pu_h, nz;
+    // inter slices
+    const InterSlice* inter_ = nullptr;   // the current slice's (null: I slice)
+    const InterSlice* last_rows_ = nullptr;
+    MotionParams mp_;
+    int ref_base_ = 0;                    // record row of the slice's first list-0 entry
+    bool have_motion_ = false;            // an inter slice was seen: the maps below exist
+    std::vector<MotionCell> motion;
+    std::vector<uint8_t> decoded, tu_v, tu_h, pu_v, pu_h, nz;
     std::vector<int32_t> lv;       // one TB's levels (all zero between TBs)
     std::vector<uint16_t> nzpos;   // the positions a source lists as nonzero
 
     size_t g_of(int x, int y) const { return (size_t)(y >> 2) * gw + (x >> 2); }
     bool same_region(int x, int y) const;
-    void setup_refs(uint32_t pic_index);
+    // Neighbours
+    bool available(int x, int y) const override;
+    const MotionCell& motion_at(int x, int y) const override { return motion[g_of(x, y)]; }
     void mark_tu_edges(int x0, int y0, int size);
-    static bool motion_differs(const Motion& p, const Motion& q);
     void finalize_bs();
-    double gauss(Rng& r);
-    void prediction_unit(int px, int py, int pw, int ph);
+    int skip_ctx(int x0, int y0) const;
+    MotionCell cell_of(const MergeCand& c) const;
+    MotionCell merge_motion(const PuGeom& g, int merge_idx) const;
+    MotionCell prediction_unit(const PuGeom& g, int log2_cb, bool& merge);
+    void store_pu(const PuGeom& g, const MotionCell& m);
     void inter_tiles(int x0, int y0, int log2);
-    void inter_cu(int x0, int y0, int log2);
+    void inter_cu(int x0, int y0, int log2, bool skip);
     void mark_block_edges(int x0, int y0, int size, int bs);
     void fill_cells(int x0, int y0, int size, uint32_t bit);
     void set_mode(int x0, int y0, int size, uint8_t mode);

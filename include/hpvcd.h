@@ -105,6 +105,15 @@ typedef struct hpvcd_gen_config {
     uint8_t  ts_context;                /* transform_skip_context_enabled_flag */
     uint8_t  bypass_alignment;          /* cabac_bypass_alignment_enabled_flag */
     uint8_t  wpp;                       /* entropy_coding_sync_enabled_flag: a substream per CTB row */
+    /* inter syntax (shapes the records and the coded stream) */
+    uint8_t  tmvp;                      /* temporal motion vector prediction (merge and AMVP candidates) */
+    uint8_t  merge_cand;                /* MaxNumMergeCand 1..5 (0 = 5) */
+    uint8_t  par_merge_level;           /* Log2ParMrgLevel 2..log2_ctb (0 = 2) */
+    uint8_t  mvd_l1_zero;               /* mvd_l1_zero_flag of B slices */
+    uint8_t  rps_in_sps;                /* short-term RPSs in the SPS (inter RPS prediction), slices name them */
+    uint8_t  cabac_init;                /* cabac_init_present_flag: each slice draws cabac_init_flag */
+    uint8_t  _pad2[2];
+    float    p_merge;                   /* merge_flag of the PUs of CUs that are not skipped */
 } hpvcd_gen_config;
 
 typedef struct hpvcd_records hpvcd_records;   /* owned record set */
@@ -145,6 +154,42 @@ int32_t hpvcd_cabac_element(int32_t i, const char** name, int32_t* offset, int32
 int32_t hpvcd_cabac_tables(uint8_t* init /* 3 x count */, uint8_t* range_lps /* 64 x 4 */,
                            uint8_t* trans_lps /* 64 */, uint8_t* trans_mps /* 64 */);
 int32_t hpvcd_cabac_init_states(int32_t init_type, int32_t qp, uint8_t* out /* count */);
+/* Motion derivation probe (unit parity tests of the parser's merge / AMVP /
+ * temporal derivations, src/motion.rs:155 derive_merge, :417 derive_amvp,
+ * src/decode.rs:6180 DecoderNeighbors::temporal, :6626 derive_pu_merge).
+ * The caller builds the current picture's motion cells (with each cell's
+ * availability: decoded, same slice and tile) and optionally a collocated
+ * picture's cells on the same grid. */
+typedef struct hpvcd_mv_cell {
+    uint8_t  inter;        /* 1: inter cell, 0: intra or not coded */
+    uint8_t  pred;         /* bit 0 list 0, bit 1 list 1 */
+    uint8_t  lt;           /* bit l: the list-l reference is long-term */
+    uint8_t  avail;        /* current picture: available for motion */
+    int8_t   ref_idx[2];
+    uint8_t  _pad[2];
+    int16_t  mv[2][2];     /* [list][x, y] quarter samples */
+    int32_t  ref_poc[2];
+} hpvcd_mv_cell;
+typedef struct hpvcd_motion_case {
+    int32_t width, height;             /* luma, multiples of 4 */
+    int32_t log2_ctb, par_mrg_level, max_merge;
+    int32_t is_b, tmvp, cur_poc, col_poc, col_from_l0;
+    int32_t num_ref[2];
+    int32_t ref_poc[2][16];
+    uint8_t ref_lt[2][16];
+    int32_t pu_x, pu_y, pu_w, pu_h;    /* the PU */
+    int32_t cu_x, cu_y, cu_size;       /* its coding block */
+    int32_t part_idx;
+    const hpvcd_mv_cell* cur;          /* (width / 4) x (height / 4) cells */
+    const hpvcd_mv_cell* col;          /* collocated picture, same grid, or NULL */
+} hpvcd_motion_case;
+typedef struct hpvcd_motion_result {
+    int32_t       num_merge;           /* length of the merge candidate list */
+    hpvcd_mv_cell merge[5];            /* the PU's merge motion for merge_idx 0..4 */
+    int16_t       amvp[2][16][2][2];   /* [list][ref_idx][predictor][x, y] */
+} hpvcd_motion_result;
+hpvcd_status hpvcd_motion_probe(const hpvcd_motion_case* c, hpvcd_motion_result* r);
+
 /* Checks every range, size and cross reference of a batch on the host (the
  * same checks hpvcd_job_create runs before touching the GPU). */
 hpvcd_status hpvcd_validate_batch(const hpvcd_batch* b);
@@ -369,6 +414,16 @@ typedef struct hpvcd_yuv {
 /* Decoder::decode_yuv (src/decoder.rs:229, src/lib.rs:762 decode_heic_yuv_with). */
 hpvcd_status hpvcd_decode_heic_yuv(hpvcd_decoder* d, const uint8_t* data, size_t size, hpvcd_yuv* yuv);
 void         hpvcd_yuv_free(hpvcd_yuv* yuv);
+/* An HEVC elementary stream (Annex B) decoded to its frames in output order
+ * (decode_hevc, src/video.rs:398; VideoDecoder::decode, :607): the stream's
+ * pictures are parsed on the host (hpvcd_parse_hevc: POC, RPS, reference
+ * lists, merge / AMVP, weights) and reconstructed on the GPU in dependency
+ * waves.  out_kind is an HPVCD_OUT_* kind for every frame, or -1 for RGB /
+ * luma by chroma format, 8 or 16 bit by depth.  *frames is a library-owned
+ * array of *count images; release it with hpvcd_frames_free. */
+hpvcd_status hpvcd_decode_hevc(hpvcd_decoder* d, const uint8_t* data, size_t size, int32_t out_kind,
+                               hpvcd_image** frames, uint32_t* count);
+void         hpvcd_frames_free(hpvcd_image* frames, uint32_t count);
 
 /* The container half on its own: a HEIC file's primary image as records
  * (one output).  out_kind: -1 the kind hpvcd_decode_heic returns, -2 the

@@ -149,7 +149,7 @@ def test_parse_returns_statuses_on_damaged_streams():
     assert N.E_CABAC_DESYNC in statuses
 
 
-def test_parse_refuses_missing_parameter_sets_and_inter_configs():
+def test_parse_refuses_missing_parameter_sets_and_heic_inter_configs():
     bs = H.gen_hevc(**SMALL)
     # drop the PPS NAL unit (third start code)
     parts = bs.split(b"\x00\x00\x00\x01")
@@ -157,8 +157,9 @@ def test_parse_refuses_missing_parameter_sets_and_inter_configs():
     with pytest.raises(H.HpvcdError) as e:
         H.RecordSet.parse_hevc(no_pps)
     assert e.value.status == N.E_PARAMSET
+    # image items are intra pictures: the HEIC writer refuses inter sequences
     with pytest.raises(H.HpvcdError) as e:
-        H.gen_hevc(**dict(SMALL, grid_rows=2, inter=1))
+        H.gen_heic(**dict(SMALL, grid_rows=2, inter=1))
     assert e.value.status == N.E_UNSUPPORTED
     with pytest.raises(H.HpvcdError):
         H.RecordSet.parse_hevc(b"\x01\x02\x03")

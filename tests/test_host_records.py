@@ -43,7 +43,7 @@ def test_struct_layouts_match_c():
     assert C.sizeof(N.Batch) == 144
     assert C.sizeof(N.PU) == 20
     assert C.sizeof(N.Ref) == 20
-    assert C.sizeof(N.GenConfig) == 168
+    assert C.sizeof(N.GenConfig) == 176
     assert C.sizeof(N.Settings) == 56 and C.sizeof(N.Image) == 40
 
 
