@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <string>
+#include <thread>
 
 #include "cabac.h"
 #include "internal.h"
@@ -625,13 +627,12 @@ using namespace hpvcd;
 namespace hpvcd {
 namespace {
 
-// A decoded picture in the decoded picture buffer: its POC, batch index,
-// reference marking and stored motion.
+// A decoded picture in the decoded picture buffer: its POC, batch index and
+// reference marking.
 struct DpbPic {
     int32_t poc;
     uint32_t index;
     bool ref;
-    const ColField* col;
 };
 
 // The reference pictures of a P / B slice (src/dpb.rs:217 apply_rps_lt,
@@ -782,8 +783,19 @@ extern "C" hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32
         }
     }
     if (coded.empty()) return set_error(HPVCD_E_BITSTREAM, "no coded picture");
-    std::vector<ParsedPicture> pics;
-    pics.reserve(coded.size());
+    // Phase 1, in decode order: slice headers, POC, RPS marking, reference
+    // lists and weights of every picture (cheap, serial).  Phase 2: the
+    // slice data of the pictures on host threads, a picture after the
+    // pictures whose motion it reads as collocated motion.
+    struct PicJob {
+        size_t coded;
+        int32_t poc;
+        std::vector<SliceHeader> hdr;
+        std::vector<InterSlice> slices;
+        std::vector<int> col_of;   // per slice: the collocated batch picture, or -1
+        int wave = 0;
+    };
+    std::vector<PicJob> jobs;
     std::vector<std::pair<int, int32_t>> out_key;   // (coded video sequence, POC) per picture
     std::vector<uint8_t> shown;
     std::vector<DpbPic> dpb;
@@ -793,8 +805,10 @@ extern "C" hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32
     for (size_t i = 0; i < coded.size(); i++) {
         const Sps& S = active[i].first;
         const Pps& Q = active[i].second;
-        std::vector<SliceHeader> hdr;
-        if (hpvcd_status st = parse_headers(S, Q, coded[i], hdr, true)) return st;
+        PicJob job;
+        job.coded = i;
+        if (hpvcd_status st = parse_headers(S, Q, coded[i], job.hdr, true)) return st;
+        const std::vector<SliceHeader>& hdr = job.hdr;
         const uint8_t nal_type = (coded[i].slices[0][0] >> 1) & 63;
         const uint8_t tid = (uint8_t)((coded[i].slices[0][1] & 7) - 1);
         const bool irap = nal_type >= 16 && nal_type <= 23, idr = nal_type == 19 || nal_type == 20;
@@ -823,42 +837,75 @@ extern "C" hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32
                 d.ref = keep;
             }
         }
-        std::vector<InterSlice> slices(hdr.size());
-        std::vector<const InterSlice*> inter(hdr.size(), nullptr);
+        job.poc = poc;
+        job.slices.assign(hdr.size(), InterSlice());
+        job.col_of.assign(hdr.size(), -1);
         for (size_t k = 0; k < hdr.size(); k++) {
             const SliceHeader& h = hdr[k];
             if (h.slice_type == 2) continue;
-            InterSlice& s = slices[k];
-            s.slice_type = h.slice_type;
-            s.cabac_init = h.cabac_init;
-            if (hpvcd_status st = slice_lists(h, poc, dpb, s)) return st;
-            slice_rows(h, S, Q, s);
-            s.tmvp = h.tmvp;
-            s.col_from_l0 = h.col_from_l0;
-            s.col_ref_idx = h.col_ref_idx;
+            InterSlice& sl = job.slices[k];
+            sl.slice_type = h.slice_type;
+            sl.cabac_init = h.cabac_init;
+            if (hpvcd_status st = slice_lists(h, poc, dpb, sl)) return st;
+            slice_rows(h, S, Q, sl);
+            sl.tmvp = h.tmvp;
+            sl.col_from_l0 = h.col_from_l0;
+            sl.col_ref_idx = h.col_ref_idx;
             if (h.tmvp) {
-                const std::vector<RefPic>& l = s.list[h.col_from_l0 ? 0 : 1];
-                if ((size_t)h.col_ref_idx < l.size())
-                    for (const DpbPic& d : dpb)
-                        if ((int32_t)d.index == l[(size_t)h.col_ref_idx].pic) s.col = d.col;
+                const std::vector<RefPic>& l = sl.list[h.col_from_l0 ? 0 : 1];
+                if ((size_t)h.col_ref_idx < l.size()) job.col_of[k] = l[(size_t)h.col_ref_idx].pic;
             }
-            s.max_merge = h.max_merge;
-            s.par_mrg_level = Q.log2_parallel_merge_level;
-            s.mvd_l1_zero = h.mvd_l1_zero;
-            inter[k] = &s;
+            sl.max_merge = h.max_merge;
+            sl.par_mrg_level = Q.log2_parallel_merge_level;
+            sl.mvd_l1_zero = h.mvd_l1_zero;
         }
-        pics.emplace_back();
-        if (hpvcd_status st = parse_picture(S, Q, coded[i], pics.back(), &hdr, &inter, poc)) return st;
-        const uint32_t index = (uint32_t)(pics.size() - 1);
-        // (pics holds every picture without reallocating: the field stays put)
-        dpb.push_back(DpbPic{poc, index, true, &pics.back().rec.col});
-        // pictures that no longer serve as references or output leave the DPB
-        // bookkeeping once nothing can name them
+        const uint32_t index = (uint32_t)jobs.size();
+        for (int c : job.col_of)
+            if (c >= 0) job.wave = std::max(job.wave, jobs[(size_t)c].wave + 1);
+        dpb.push_back(DpbPic{poc, index, true});
+        // pictures that serve as references no more leave the bookkeeping
         dpb.erase(std::remove_if(dpb.begin(), dpb.end(), [](const DpbPic& d) { return !d.ref; }), dpb.end());
         out_key.emplace_back(cvs, poc);
         shown.push_back(hdr[0].pic_output ? 1 : 0);
         if (tid == 0 && !rasl && !radl && !(nal_type <= 14 && nal_type % 2 == 0)) prev_tid0 = poc;
+        jobs.push_back(std::move(job));
     }
+    std::vector<ParsedPicture> pics(jobs.size());
+    for (PicJob& job : jobs)
+        for (size_t k = 0; k < job.slices.size(); k++)
+            if (job.col_of[k] >= 0) job.slices[k].col = &pics[(size_t)job.col_of[k]].rec.col;
+    std::vector<hpvcd_status> sts(jobs.size(), HPVCD_OK);
+    std::vector<std::string> why(jobs.size());
+    int nwaves = 0;
+    for (const PicJob& job : jobs) nwaves = std::max(nwaves, job.wave + 1);
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    for (int wv = 0; wv < nwaves; wv++) {
+        std::vector<size_t> todo;
+        for (size_t i = 0; i < jobs.size(); i++)
+            if (jobs[i].wave == wv) todo.push_back(i);
+        auto work = [&](size_t t, size_t stride) {
+            for (size_t k = t; k < todo.size(); k += stride) {
+                PicJob& job = jobs[todo[k]];
+                std::vector<const InterSlice*> inter(job.slices.size(), nullptr);
+                for (size_t s2 = 0; s2 < job.slices.size(); s2++)
+                    if (job.hdr[s2].slice_type != 2) inter[s2] = &job.slices[s2];
+                const size_t c = job.coded;
+                sts[todo[k]] = parse_picture(active[c].first, active[c].second, coded[c], pics[todo[k]], &job.hdr,
+                                             &inter, job.poc);
+                if (sts[todo[k]]) why[todo[k]] = hpvcd_last_error();
+            }
+        };
+        const size_t nw = std::min<size_t>(nth, todo.size());
+        if (nw <= 1) {
+            work(0, 1);
+        } else {
+            std::vector<std::thread> th;
+            for (size_t t = 0; t < nw; t++) th.emplace_back(work, t, nw);
+            for (auto& x : th) x.join();
+        }
+    }
+    for (size_t i = 0; i < jobs.size(); i++)
+        if (sts[i]) return set_error(sts[i], why[i].c_str());
     // output order: coded video sequence, then POC
     std::vector<uint32_t> order;
     for (uint32_t i = 0; i < (uint32_t)pics.size(); i++)
