@@ -21,6 +21,8 @@ hpvcd_status set_error(hpvcd_status st, const char* msg) {
 }
 }  // namespace hpvcd
 
+extern "C" const char* hpvcd_last_error(void) { return hpvcd::g_err.c_str(); }
+
 using namespace hpvcd;
 
 static int failures = 0;

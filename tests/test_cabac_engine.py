@@ -131,7 +131,7 @@ def test_engine_known_answers(tmp_path):
     exe = str(tmp_path / "cabac_kat")
     src = [os.path.join(ROOT, "tests", "cabac_kat.cpp")] + [
         os.path.join(ROOT, "hpvcd_amd", "csrc", f) for f in ("cabac.cpp", "hevc_ps.cpp", "walker.cpp", "motion.cpp", "parse.cpp")]
-    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", exe] + src, check=True,
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", exe] + src + ["-lpthread"], check=True,
                    timeout=300)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     lines = r.stdout.strip().splitlines()
