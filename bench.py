@@ -348,6 +348,9 @@ def main() -> int:
                     help="skip the end-to-end (records and HEIC bytes to image) figures")
     ap.add_argument("--serving-seconds", type=float, default=2.0,
                     help="length of each arrival-rate run of the serving figure (0 skips it)")
+    ap.add_argument("--one-batch", action="store_true",
+                    help="time one resident batch only (no second batch in flight), as the profiling runs do so "
+                         "that kernel durations are not shared with a concurrent batch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (read + write) three times, for calibrating "
                          "the FETCH_SIZE/WRITE_SIZE counters of a profiling run")
@@ -377,23 +380,26 @@ def main() -> int:
         # throughput kernels (K1, K2a, K3-K5) run beside it, as in a stream of
         # images
         job = g.job(rs)
-        job2 = g2.job(rs)
+        job2 = None if args.one_batch else g2.job(rs)
         # warmup (also builds the hipGraphs)
         for _ in range(max(1, args.warmup)):
             job.run(N.MODE_GRAPH)
-            job2.run(N.MODE_GRAPH)
+            if job2:
+                job2.run(N.MODE_GRAPH)
         g.sync()
         g2.sync()
         # throughput: K back-to-back steps (batches), alternating between the
         # two streams, synchronised on both sides
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            (job if k % 2 == 0 else job2).run(N.MODE_GRAPH)
-        g.sync()
-        g2.sync()
-        wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        print(f"[bench] timed {args.steps} steps, two batches in flight: {wall_ms:.3f} ms/step", file=sys.stderr,
-              flush=True)
+        wall_ms = None
+        if not args.one_batch:
+            t0 = time.perf_counter()
+            for k in range(args.steps):
+                (job if k % 2 == 0 else job2).run(N.MODE_GRAPH)
+            g.sync()
+            g2.sync()
+            wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            print(f"[bench] timed {args.steps} steps, two batches in flight: {wall_ms:.3f} ms/step", file=sys.stderr,
+                  flush=True)
         # one batch at a time: K back-to-back replays of one job
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -401,6 +407,8 @@ def main() -> int:
         g.sync()
         single_ms = (time.perf_counter() - t0) * 1e3 / args.steps
         print(f"[bench] one batch at a time: {single_ms:.3f} ms/step", file=sys.stderr, flush=True)
+        if wall_ms is None:
+            wall_ms = single_ms
         # device-side timing and per-kernel breakdown (HIP events on the job's stream)
         ev_ms, kms = job.time(N.MODE_GRAPH, warmup=1, iters=max(3, min(args.steps, 20)))
         # replay latency of the resident job (each step synchronised)
@@ -412,8 +420,9 @@ def main() -> int:
             lat.append((time.perf_counter() - t) * 1e3)
         lat = np.array(lat)
         out_gpu = job.download()
-        out_gpu2 = job2.download()
-        job2.close()
+        out_gpu2 = job2.download() if job2 else out_gpu
+        if job2:
+            job2.close()
         plain_ms = None
         if not args.no_plain:
             plain_ms, _ = job.time(N.MODE_PLAIN, warmup=1, iters=3, breakdown=False)
@@ -521,8 +530,9 @@ def main() -> int:
             "image": f"{rs.outputs()[0].width}x{rs.outputs()[0].height}",
             "pictures": st["pictures"], "tbs": st["tbs"], "nnz": st["nnz"], "samples": st["samples"],
             **({"pus": st["pus"]} if st["pus"] else {}),
-            "mode": "batched, one launch per stage, hipGraph replay; two resident batches on two streams, "
-                    "replayed alternately",
+            "mode": ("batched, one launch per stage, hipGraph replay; " +
+                     ("one resident batch" if args.one_batch else
+                      "two resident batches on two streams, replayed alternately")),
         },
         "single_batch": {"ms_per_step": round(single_ms, 4), "value": round(mpix / (single_ms / 1e3), 2),
                          "kind": "one resident batch replayed back to back"},
