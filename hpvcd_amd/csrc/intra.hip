@@ -402,12 +402,12 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
                   (s.filt ? 1u << 17 : 0u) |
                   ((s.filt && strong_on && n == 32) ? 1u << 18 : 0u) | (y0 + n == ctbh ? 1u << 19 : 0u) |
                   (x0 + n == ctbw ? 1u << 20 : 0u) | ((n <= 8 && !s.filt && s.cls != kClsDc) ? 1u << 21 : 0u) |
-                  ((uint32_t)((tb.flags >> TBX_GROUP_SHIFT) & 7u) << 22) | (gnext << 25) |
+                  ((uint32_t)((tb.flags >> TBX_GROUP_SHIFT) & 7u) << 22) | ((gnext > 1 ? 3u : 0u) << 25) |
                   (pub ? 1u << 28 : 0u) | (plain ? 1u << 29 : 0u) |
                   ((uint32_t)((gsz > 1 ? gsz : (n == 8 ? 4 : 1)) - 1) << 30);
             d.y = (ro & 0x1fffu) | ((uint32_t)soff << 13);
             d.z = u0 | (u1 << 13);
-            d.w = top_need | (left_need << 8);
+            d.w = top_need | (left_need << 8) | ((uint32_t)(e + gsz) << 16);
             desc_g[tb_list + e] = d;
         }
         uint32_t* dst = smap_g + sbase + soff;
@@ -423,19 +423,23 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
 //      19 writes the unit's bottom row  20 writes its right column
 //      21 fast path: at most 8 x 8, references unfiltered, not DC
 //      [22,25) size of the group of independent 4 x 4 TBs this TB leads
-//      (step leaders only:) [25,28) size of the next step  28 some TB of
+//      (step leaders only:) [25,27) 3 if the next step is a group of
+//      several TBs, else 0  28 some TB of
 //      the step writes the unit's bottom row or right column  29 some TB of
 //      the step is planar or horizontal / vertical  [30,32) sixteens of
 //      samples in the step minus one (TBs of at most 8 x 8)
 //   y: [0,13) reference map offset  [13,25) sample word offset
 //   z: uniform references of the class: [0,13) u0  [13,26) u1
+//   w: [0,8) row-above words read  [8,16) left-column rows read
+//      (step leaders:) [16,25) index of the TB after the step
 struct IntraLds {
     uint16_t tile[kLdsSamples];
     alignas(16) uint32_t smap[kMaxCtb * kMaxCtb];
-    uint4 desc[kMaxUnitTbs];
+    uint4 desc[kMaxUnitTbs + 4];      // a zero descriptor (not fast) follows the last TB
     alignas(16) uint16_t refmap[kMaxUnitRefs];
     uint32_t pneed[kMaxUnitTbs];      // prefix maxima over the TB order of the neighbour samples read:
                                       // row-above words | left-column rows << 16
+    uint32_t imask[8];                // intra 4 x 4 blocks of an inter unit: row r in bits 16 (r & 1) of word r / 2
     uint32_t bready;                  // TBs whose neighbour samples are all in the tile
     uint32_t unit;
 };
@@ -469,7 +473,9 @@ __device__ __forceinline__ void intra_predict(const uint16_t* __restrict__ S, co
                 res[c] = r_pf;
             } else {
                 w[c] = W[p];
-                res[c] = (int32_t)rs[(p >> LOG2N) * kMaxCtb + (p & (n - 1))];  // zero where no residual
+                // zero where no residual; 16-bit residuals wait in the tile at the sample itself
+                res[c] = sizeof(R) == 2 ? (int32_t)(int16_t)dst[(p >> LOG2N) * kRowW + (p & (n - 1))]
+                                        : (int32_t)rs[(p >> LOG2N) * kMaxCtb + (p & (n - 1))];
             }
         }
 #pragma unroll
@@ -563,7 +569,7 @@ __device__ __forceinline__ void intra_tb(IntraLds& L, const R* __restrict__ resb
     }
     const int u0 = (int)(d2 & 0x1fff) + fbase, u1 = (int)((d2 >> 13) & 0x1fff) + fbase;
     uint16_t* dst = S + kTopW + y0 * kRowW + x0 + 1;
-    const R* rs = resb + y0 * kMaxCtb + x0;
+    const R* rs = sizeof(R) == 2 ? resb : resb + y0 * kMaxCtb + x0;
     const int pb = ((d0 >> 19) & 1) ? x0 : -1;            // bottom-row word of column x0
     const int pr = ((d0 >> 20) & 1) ? kMaxCtb + y0 : -1;  // right-column word of row y0
     switch (cls) {
@@ -614,40 +620,49 @@ template <typename Q> __device__ __forceinline__ Q quad_zero();
 template <> __device__ __forceinline__ uint2 quad_zero<uint2>() { return make_uint2(0, 0); }
 template <> __device__ __forceinline__ uint4 quad_zero<uint4>() { return make_uint4(0, 0, 0, 0); }
 
+// Four 16-bit residual samples into the tile (rows of odd length: one
+// 16-bit store each).
+__device__ __forceinline__ void store_res_tile(uint16_t* t, uint2 q) {
+    t[0] = (uint16_t)q.x; t[1] = (uint16_t)(q.x >> 16); t[2] = (uint16_t)q.y; t[3] = (uint16_t)(q.y >> 16);
+}
+
 // The common step: one TB of at most 8 x 8 or a group of up to four
 // independent 4 x 4 TBs of one dependency level, references unfiltered, not
 // DC, one sample per lane.  Lane l takes sample l of the step (a group's
 // samples run TB after TB, 16 each); l0/l2 are the descriptor words of its
 // TB, w/rv its prefetched reference word and residual, (x, y) its sample in
-// the TB and loff the sample's tile offset from the TB origin.  Lanes past
+// the TB and ti the sample's tile index.  Lanes past
 // the step's samples write to the sink slot.  Planar and H/V need two more
 // reference slots (descriptor word z); `mixed` says some lane needs them.
-__device__ __forceinline__ void intra_step_fast(uint16_t* __restrict__ S, uint32_t l0, uint32_t l2, uint32_t w,
-                                                int32_t rv, bool b8, bool mixed, int x, int y, int loff,
+struct StepRefs {
+    int32_t r0, r1, ru0, ru1;
+};
+
+__device__ __forceinline__ void intra_step_fast(uint16_t* __restrict__ S, uint32_t l0, const StepRefs& q, uint32_t w,
+                                                int32_t rv, bool b8, bool mixed, int x, int y, int ti,
                                                 bool active, int32_t maxv) {
-    const int32_t r0 = S[w & 0x1fffu], r1 = S[(w >> 13) & 0x1fffu];
+    const int32_t r0 = q.r0, r1 = q.r1;
     const int k = (w >> 26) & 31;
     int32_t r = (mul24(32 - k, r0) + mul24(k, r1) + 16) >> 5;
     if (mixed) {
         const int n = b8 ? 8 : 4, sh = b8 ? 4 : 3;
         const int cls = (l0 >> 14) & 3;
-        const int32_t ru0 = S[l2 & 0x1fffu], ru1 = S[(l2 >> 13) & 0x1fffu];
+        const int32_t ru0 = q.ru0, ru1 = q.ru1;
         const int32_t hv = (int32_t)(w >> 31) ? max(0, min(ru0 + ((r1 - ru1) >> 1), maxv)) : r0;
         const int32_t pl = (mul24(n - 1 - x, r0) + mul24(x + 1, ru0) + mul24(n - 1 - y, r1) + mul24(y + 1, ru1) + n) >> sh;
         r = cls == kClsAngular ? r : (cls == kClsHv ? hv : pl);
     }
     const int32_t v = max(0, min((int32_t)wadd(r, rv), maxv));
-    const int wi = active ? kTopW + 1 + mul24((l0 >> 6) & 63, kRowW) + (l0 & 63) + loff : kSink;
-    S[wi] = (uint16_t)v;
+    S[active ? ti : kSink] = (uint16_t)v;
 }
 
 // The border samples of a fast step: the unit's bottom row and right column
 // go out at once as tagged words for the units below and to the right.
 __device__ __forceinline__ void intra_step_publish(const uint16_t* __restrict__ S, uint32_t l0, bool b8, int x, int y,
-                                                   int loff, bool active, uint32_t* __restrict__ bw, uint32_t tagw) {
+                                                   int ti, bool active, uint32_t* __restrict__ bw, uint32_t tagw) {
     const int n = b8 ? 8 : 4, x0 = l0 & 63, y0 = (l0 >> 6) & 63;
     if (!active) return;
-    const uint32_t v = S[kTopW + 1 + y0 * kRowW + x0 + loff];
+    const uint32_t v = S[ti];
     if (((l0 >> 19) & 1) && y == n - 1) border_store(bw + x0 + x, tagw | v);
     if (((l0 >> 20) & 1) && x == n - 1) border_store(bw + kMaxCtb + y0 + y, tagw | v);
 }
@@ -655,34 +670,55 @@ __device__ __forceinline__ void intra_step_publish(const uint16_t* __restrict__ 
 // TBs of one step of the prediction wave: a group of up to four independent
 // 4 x 4 TBs (descriptor bits 22-24 of its first TB), else one TB.
 __device__ __forceinline__ int step_size(uint32_t d0) { return max(1, (int)((d0 >> 22) & 7)); }
-// Size of the step after the one d0 leads (descriptor bits 25-27).
-__device__ __forceinline__ int next_step_size(uint32_t d0) { return max(1, (int)((d0 >> 25) & 7)); }
+// 3 if the step after the one d0 leads is a group of several TBs, else 0
+// (descriptor bits 25-26): the mask of the lane's TB in it.
+__device__ __forceinline__ int next_step_spread(uint32_t d0) { return (int)((d0 >> 25) & 3u); }
+// Index of the TB after the step whose leader's descriptor word w is d3.
+__device__ __forceinline__ int step_end(uint32_t d3) { return (int)((d3 >> 16) & 0x1ffu); }
 // Whether lane l (lane16 = l / 16) holds a sample of a step of TBs of at
 // most 8 x 8.
 __device__ __forceinline__ bool step_active(uint32_t d0, int lane16) { return lane16 <= (int)(d0 >> 30); }
 
-// The lane's reference word and residual of a step of TBs of at most 8 x 8:
-// lane l takes word l of the step (a group's words are contiguous, 16 per
-// TB).  d0/d1 are the first TB's descriptor words, l0 the lane's own TB's;
-// lr4/lr8 are the lane's residual offsets in a 4 x 4 / 8 x 8 TB.  The
-// residual block holds zeros where a TB has no residual.
+// step_prefetch without the size test, for a step known to be fast (its
+// loads then have a fixed count, which lets the reference reads issued before
+// them be waited for alone).
+template <typename R>
+__device__ __forceinline__ void step_prefetch_fast(const IntraLds& L, const R* __restrict__ resb, uint32_t d0,
+                                                   uint32_t d1, uint32_t l0, int lane, int lt4, int lt8, int lr4,
+                                                   int lr8, uint32_t& w, int32_t& r, int& ti) {
+    const bool b8 = (d0 >> 12) & 1;
+    w = L.smap[min((int)((d1 >> 13) & 0xfff) + lane, kMaxCtb * kMaxCtb - 1)];
+    ti = kTopW + 1 + mul24((l0 >> 6) & 63, kRowW) + (int)(l0 & 63) + (b8 ? lt8 : lt4);
+    if constexpr (sizeof(R) == 2) r = reinterpret_cast<const int16_t*>(L.tile)[ti];
+    else r = (int32_t)resb[(l0 & 0xfffu) + (b8 ? lr8 : lr4)];
+}
+
+// The lane's reference word, residual and tile index of a step of TBs of at
+// most 8 x 8: lane l takes word l of the step (a group's words are
+// contiguous, 16 per TB).  d0/d1 are the first TB's descriptor words, l0 the
+// lane's own TB's; lt4/lt8 (lr4/lr8) are the lane's tile (residual block)
+// offsets in a 4 x 4 / 8 x 8 TB.  16-bit residuals wait in the tile at their
+// samples, 32-bit ones in the residual block; both hold zeros where a TB has
+// no residual.
 template <typename R>
 __device__ __forceinline__ void step_prefetch(const IntraLds& L, const R* __restrict__ resb, uint32_t d0,
-                                              uint32_t d1, uint32_t l0, int lane, int lr4, int lr8, uint32_t& w,
-                                              int32_t& r) {
+                                              uint32_t d1, uint32_t l0, int lane, int lt4, int lt8, int lr4,
+                                              int lr8, uint32_t& w, int32_t& r, int& ti) {
     // (a group's TBs are 4 x 4)
     if (!((d0 >> 13) & 1)) {
+        const bool b8 = (d0 >> 12) & 1;
         w = L.smap[min((int)((d1 >> 13) & 0xfff) + lane, kMaxCtb * kMaxCtb - 1)];
-        r = (int32_t)resb[(l0 & 0xfffu) + (((d0 >> 12) & 1) ? lr8 : lr4)];
+        ti = kTopW + 1 + mul24((l0 >> 6) & 63, kRowW) + (int)(l0 & 63) + (b8 ? lt8 : lt4);
+        if constexpr (sizeof(R) == 2) r = reinterpret_cast<const int16_t*>(L.tile)[ti];
+        else r = (int32_t)resb[(l0 & 0xfffu) + (b8 ? lr8 : lr4)];
     }
 }
 
-// Index of the descriptor of the lane's TB in a step that starts at TB i and
-// holds g TBs (lanes past a group read a later TB; their results go to the
-// sink slot).
-__device__ __forceinline__ int lane_desc(int i, int g, int lane16, int ntb) {
-    return min(i + (g > 1 ? lane16 : 0), ntb - 1);
-}
+// Index of the descriptor of the lane's TB in a step that starts at TB i,
+// spread the step's next_step_spread mask (lanes past a group read a later
+// TB, or the zero descriptors after the last; their results go to the sink
+// slot).
+__device__ __forceinline__ int lane_desc(int i, int spread, int lane16) { return i + (lane16 & spread); }
 
 __device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
 __device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
@@ -703,7 +739,10 @@ __global__ void __launch_bounds__(kIntraThreads)
             const R* __restrict__ res_pool, uint32_t* __restrict__ border, uint32_t* counter, const uint32_t* gen,
             uint32_t* err) {
     __shared__ IntraLds L;
-    __shared__ __align__(16) R resb[kMaxCtb * kMaxCtb];
+    // 32-bit residuals have a block of their own; 16-bit ones are staged
+    // into the tile at their samples and replaced there by the reconstruction
+    constexpr bool kResInTile = sizeof(R) == 2;
+    __shared__ __align__(16) R resb[kResInTile ? 8 : kMaxCtb * kMaxCtb];
     const int tid = threadIdx.x, wave = uni(tid >> 6), lane = tid & 63;
     const uint32_t tag = *gen & 0xffffu;
 
@@ -770,7 +809,13 @@ __global__ void __launch_bounds__(kIntraThreads)
 #pragma unroll
             for (int b = 0; b < kStageRes; b++) {
                 const int e = 4 * (tid + b * kIntraThreads);
-                if (e < nres) *reinterpret_cast<Q*>(resb + (e >> ctbw_log2) * kMaxCtb + (e & (ctbw - 1))) = vr[b];
+                if (e < nres) {
+                    if constexpr (kResInTile) {
+                        if (!P.inter) store_res_tile(L.tile + kTopW + 1 + (e >> ctbw_log2) * kRowW + (e & (ctbw - 1)), vr[b]);
+                    } else {
+                        *reinterpret_cast<Q*>(resb + (e >> ctbw_log2) * kMaxCtb + (e & (ctbw - 1))) = vr[b];
+                    }
+                }
             }
             if (P.inter) {
                 // motion-compensated prediction of the unit, the predictor of its inter TBs
@@ -789,9 +834,29 @@ __global__ void __launch_bounds__(kIntraThreads)
                         if (e < nres) L.tile[kTopW + (e >> ctbw_log2) * kRowW + (e & (ctbw - 1)) + 1] = (uint16_t)v[b];
                     }
                 }
+                if constexpr (kResInTile) {
+                    // residuals into the tile over the predictor, at the intra TBs only
+                    if (tid < 8) L.imask[tid] = 0;
+                    __syncthreads();
+                    for (int i = tid; i < ntb; i += kIntraThreads) {
+                        const uint32_t d0 = L.desc[i].x;
+                        const int bx = (d0 & 63) >> 2, by = ((d0 >> 6) & 63) >> 2, nb = 1 << ((d0 >> 12) & 3);
+                        const uint32_t run = ((1u << nb) - 1u) << bx;
+                        for (int r = by; r < by + nb; r++) atomicOr(&L.imask[r >> 1], run << ((r & 1) * 16));
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int b = 0; b < kStageRes; b++) {
+                        const int e = 4 * (tid + b * kIntraThreads);
+                        const int ry = e >> ctbw_log2, rx = e & (ctbw - 1);
+                        if (e < nres && ((L.imask[(ry >> 3) & 7] >> (((ry >> 2) & 1) * 16 + (rx >> 2))) & 1u))
+                            store_res_tile(L.tile + kTopW + 1 + ry * kRowW + rx, vr[b]);
+                    }
+                }
             }
         }
         if (tid == 0) {
+            L.desc[ntb] = make_uint4(0u, 0u, 0u, 0u);
             L.bready = 0;
             L.tile[kNeutral] = (uint16_t)(1 << (bd - 1));
             L.tile[kZero] = 0;
@@ -905,6 +970,7 @@ __global__ void __launch_bounds__(kIntraThreads)
             // reference words and residuals are loaded while the current TB runs
             uint32_t cd0 = 0, cd1 = 0, cd2 = 0, cw = 0, cl0 = 0, cl2 = 0;
             int32_t cr = 0;
+            int cti = kSink;
             // the lane's sample in a 4 x 4 TB (lane % 16) and an 8 x 8 TB,
             // its tile and residual-block offsets from the TB origin
             const int lane16 = lane >> 4;
@@ -918,21 +984,22 @@ __global__ void __launch_bounds__(kIntraThreads)
             // use, its reference words and residuals while the current step
             // runs.  Only the descriptor words used are loaded: the register
             // of an unused word would hold up the next load into it.
-            uint32_t nr0 = 0, nr1 = 0, nr2 = 0, nl0 = 0, nl2 = 0;
+            uint32_t cd3 = 0, nr0 = 0, nr1 = 0, nr2 = 0, nr3 = 0, nl0 = 0, nl2 = 0;
             const uint32_t* dw = reinterpret_cast<const uint32_t*>(L.desc);
             if (ntb > 0) {
                 cd0 = uni(dw[0]);
                 cd1 = uni(dw[1]);
                 cd2 = uni(dw[2]);
-                const int g0 = step_size(cd0);
-                const int li = lane_desc(0, g0, lane16, ntb);
+                cd3 = uni(dw[3]);
+                const int li = lane_desc(0, step_size(cd0) > 1 ? 3 : 0, lane16);
                 cl0 = dw[4 * li];
                 cl2 = dw[4 * li + 2];
-                step_prefetch(L, resb, cd0, cd1, cl0, lane, lr4, lr8, cw, cr);
-                const int ni = min(g0, kMaxUnitTbs - 1), nli = lane_desc(g0, next_step_size(cd0), lane16, ntb);
+                step_prefetch(L, resb, cd0, cd1, cl0, lane, lt4, lt8, lr4, lr8, cw, cr, cti);
+                const int ni = step_end(cd3), nli = lane_desc(ni, next_step_spread(cd0), lane16);
                 nr0 = dw[4 * ni];
                 nr1 = dw[4 * ni + 1];
                 nr2 = dw[4 * ni + 2];
+                nr3 = dw[4 * ni + 3];
                 nl0 = dw[4 * nli];
                 nl2 = dw[4 * nli + 2];
             }
@@ -956,52 +1023,77 @@ __global__ void __launch_bounds__(kIntraThreads)
                 if ((cd0 >> 21) & 1) {
                     // a run of fast steps, each ready and followed by a fast one
                     for (;;) {
-                        const int next = i + step_size(cd0);
-                        const uint32_t nd0 = uni(nr0), nd1 = uni(nr1), nd2 = uni(nr2);
-                        uint32_t nw = 0, mr0 = nr0, mr1 = nr1, mr2 = nr2, ml0 = nl0, ml2 = nl2;
-                        int32_t nrr = 0;
-                        if (next < ntb) {
-                            step_prefetch(L, resb, nd0, nd1, nl0, lane, lr4, lr8, nw, nrr);
-                            const int nn = next + step_size(nd0);
-                            const int ni = min(nn, kMaxUnitTbs - 1), nli = lane_desc(nn, next_step_size(nd0), lane16, ntb);
-                            mr0 = dw[4 * ni];
-                            mr1 = dw[4 * ni + 1];
-                            mr2 = dw[4 * ni + 2];
+                        // this step's angular reference pair is read first
+                        // (the reads follow the previous step's write); the
+                        // next step's loads, the planar / H-V pair and the
+                        // loop control are issued while it is in flight.
+                        // Past the last TB the loads read clamped slots and
+                        // their values go unused.
+                        const bool mixed = (cd0 >> 29) & 1;
+                        StepRefs q;
+                        q.r0 = L.tile[cw & 0x1fffu];
+                        q.r1 = L.tile[(cw >> 13) & 0x1fffu];
+                        __builtin_amdgcn_sched_barrier(0);
+                        const int next = step_end(cd3);
+                        const uint32_t nd0 = uni(nr0), nd1 = uni(nr1), nd2 = uni(nr2), nd3 = uni(nr3);
+                        uint32_t nw, mr0, mr1, mr2, mr3, ml0, ml2;
+                        int32_t nrr;
+                        int nti;
+                        step_prefetch_fast(L, resb, nd0, nd1, nl0, lane, lt4, lt8, lr4, lr8, nw, nrr, nti);
+                        const int nn = step_end(nd3);
+                        {
+                            const uint4 m = L.desc[nn];
+                            mr0 = m.x;
+                            mr1 = m.y;
+                            mr2 = m.z;
+                            mr3 = m.w;
+                            const int nli = lane_desc(nn, next_step_spread(nd0), lane16);
                             ml0 = dw[4 * nli];
                             ml2 = dw[4 * nli + 2];
                         }
+                        q.ru0 = q.ru1 = 0;
+                        if (mixed) {
+                            q.ru0 = L.tile[cl2 & 0x1fffu];
+                            q.ru1 = L.tile[(cl2 >> 13) & 0x1fffu];
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
                         const bool b8 = (cd0 >> 12) & 1;
-                        const int x = b8 ? x8 : x4, y = b8 ? y8 : y4, loff = b8 ? lt8 : lt4;
+                        const int x = b8 ? x8 : x4, y = b8 ? y8 : y4;
                         const bool active = step_active(cd0, lane16);
-                        intra_step_fast(L.tile, cl0, cl2, cw, cr, b8, (cd0 >> 29) & 1, x, y, loff, active, maxv);
-                        if ((cd0 >> 28) & 1) intra_step_publish(L.tile, cl0, b8, x, y, loff, active, bw, tagw);
+                        intra_step_fast(L.tile, cl0, q, cw, cr, b8, mixed, x, y, cti, active, maxv);
+                        if ((cd0 >> 28) & 1) intra_step_publish(L.tile, cl0, b8, x, y, cti, active, bw, tagw);
                         i = next;
                         cd0 = nd0;
                         cd1 = nd1;
                         cd2 = nd2;
+                        cd3 = nd3;
                         cw = nw;
                         cr = nrr;
+                        cti = nti;
                         cl0 = nl0;
                         cl2 = nl2;
                         nr0 = mr0;
                         nr1 = mr1;
                         nr2 = mr2;
+                        nr3 = mr3;
                         nl0 = ml0;
                         nl2 = ml2;
-                        if (i >= ntb || !((cd0 >> 21) & 1) || i + step_size(cd0) > ready) break;
+                        // the zero descriptor after the last TB is not fast
+                        if (!((cd0 >> 21) & 1) || nn > ready) break;
                     }
                 } else {
-                    const int next = i + step_size(cd0);
-                    const uint32_t nd0 = uni(nr0), nd1 = uni(nr1), nd2 = uni(nr2);
-                    uint32_t nw = 0, mr0 = nr0, mr1 = nr1, mr2 = nr2, ml0 = nl0, ml2 = nl2;
+                    const int next = step_end(cd3);
+                    const uint32_t nd0 = uni(nr0), nd1 = uni(nr1), nd2 = uni(nr2), nd3 = uni(nr3);
+                    uint32_t nw = 0, mr0 = nr0, mr1 = nr1, mr2 = nr2, mr3 = nr3, ml0 = nl0, ml2 = nl2;
                     int32_t nrr = 0;
+                    int nti = kSink;
                     if (next < ntb) {
-                        step_prefetch(L, resb, nd0, nd1, nl0, lane, lr4, lr8, nw, nrr);
-                        const int nn = next + step_size(nd0);
-                        const int ni = min(nn, kMaxUnitTbs - 1), nli = lane_desc(nn, next_step_size(nd0), lane16, ntb);
-                        mr0 = dw[4 * ni];
-                        mr1 = dw[4 * ni + 1];
-                        mr2 = dw[4 * ni + 2];
+                        step_prefetch(L, resb, nd0, nd1, nl0, lane, lt4, lt8, lr4, lr8, nw, nrr, nti);
+                        const int nn = step_end(nd3), nli = lane_desc(nn, next_step_spread(nd0), lane16);
+                        mr0 = dw[4 * nn];
+                        mr1 = dw[4 * nn + 1];
+                        mr2 = dw[4 * nn + 2];
+                        mr3 = dw[4 * nn + 3];
                         ml0 = dw[4 * nli];
                         ml2 = dw[4 * nli + 2];
                     }
@@ -1015,13 +1107,16 @@ __global__ void __launch_bounds__(kIntraThreads)
                     cd0 = nd0;
                     cd1 = nd1;
                     cd2 = nd2;
+                    cd3 = nd3;
                     cw = nw;
                     cr = nrr;
+                    cti = nti;
                     cl0 = nl0;
                     cl2 = nl2;
                     nr0 = mr0;
                     nr1 = mr1;
                     nr2 = mr2;
+                    nr3 = mr3;
                     nl0 = ml0;
                     nl2 = ml2;
                 }
