@@ -397,17 +397,18 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
             const int gsz = max(1, (int)((tb.flags >> TBX_GROUP_SHIFT) & 7u));
             const uint32_t gnext = ((tb.flags >> TBX_NEXT_SHIFT) & 3u) + 1;
             const bool pub = (tb.flags & TBX_STEP_PUB) != 0, plain = (tb.flags & TBX_STEP_PLAIN) != 0;
+            const bool fast = n <= 8 && !s.filt && s.cls != kClsDc;
             uint4 d;
             d.x = (uint32_t)x0 | ((uint32_t)y0 << 6) | ((uint32_t)(log2n - 2) << 12) | ((uint32_t)s.cls << 14) |
                   (s.filt ? 1u << 17 : 0u) |
                   ((s.filt && strong_on && n == 32) ? 1u << 18 : 0u) | (y0 + n == ctbh ? 1u << 19 : 0u) |
-                  (x0 + n == ctbw ? 1u << 20 : 0u) | ((n <= 8 && !s.filt && s.cls != kClsDc) ? 1u << 21 : 0u) |
+                  (x0 + n == ctbw ? 1u << 20 : 0u) | (fast ? 1u << 21 : 0u) |
                   ((uint32_t)((tb.flags >> TBX_GROUP_SHIFT) & 7u) << 22) | ((gnext > 1 ? 3u : 0u) << 25) |
                   (pub ? 1u << 28 : 0u) | (plain ? 1u << 29 : 0u) |
                   ((uint32_t)((gsz > 1 ? gsz : (n == 8 ? 4 : 1)) - 1) << 30);
             d.y = (ro & 0x1fffu) | ((uint32_t)soff << 13);
             d.z = u0 | (u1 << 13);
-            d.w = top_need | (left_need << 8) | ((uint32_t)(e + gsz) << 16);
+            d.w = top_need | (left_need << 8) | ((uint32_t)(e + gsz) << 16) | (fast ? 0u : 1u << 25);
             desc_g[tb_list + e] = d;
         }
         uint32_t* dst = smap_g + sbase + soff;
@@ -431,11 +432,12 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(const DevPic* __restrict_
 //   y: [0,13) reference map offset  [13,25) sample word offset
 //   z: uniform references of the class: [0,13) u0  [13,26) u1
 //   w: [0,8) row-above words read  [8,16) left-column rows read
-//      (step leaders:) [16,25) index of the TB after the step
+//      (step leaders:) [16,25) index of the TB after the step  25 the
+//      step is not fast (bit 21 clear)
 struct IntraLds {
     uint16_t tile[kLdsSamples];
     alignas(16) uint32_t smap[kMaxCtb * kMaxCtb];
-    uint4 desc[kMaxUnitTbs + 4];      // a zero descriptor (not fast) follows the last TB
+    uint4 desc[kMaxUnitTbs + 4];      // a not-fast descriptor follows the last TB
     alignas(16) uint16_t refmap[kMaxUnitRefs];
     uint32_t pneed[kMaxUnitTbs];      // prefix maxima over the TB order of the neighbour samples read:
                                       // row-above words | left-column rows << 16
@@ -631,16 +633,32 @@ __device__ __forceinline__ void store_res_tile(uint16_t* t, uint2 q) {
 // DC, one sample per lane.  Lane l takes sample l of the step (a group's
 // samples run TB after TB, 16 each); l0/l2 are the descriptor words of its
 // TB, w/rv its prefetched reference word and residual, (x, y) its sample in
-// the TB and ti the sample's tile index.  Lanes past
-// the step's samples write to the sink slot.  Planar and H/V need two more
-// reference slots (descriptor word z); `mixed` says some lane needs them.
+// the TB and ti the sample's tile index.  Lanes past the step's samples
+// write to the sink slot.  Planar and H/V need two more reference slots
+// (descriptor word z), read only when `mixed` says some lane needs them.
 struct StepRefs {
     int32_t r0, r1, ru0, ru1;
 };
 
+// The prediction wave's registers between fast steps: the step's first TB
+// index and descriptor (wave-uniform), and per lane its word, residual, tile
+// index and TB descriptor words (w, r, ti, l0, l2) and the next step's
+// descriptor and lane descriptor words as loaded (n*, nl*).  The two kinds
+// are kept apart so that the uniform ones stay in scalar registers.
+struct FastUni {
+    int i;
+    uint32_t d0, d1, d2, d3;
+};
+struct FastLane {
+    uint32_t w;
+    int32_t r;
+    int ti;
+    uint32_t l0, l2, n0, n1, n2, n3, nl0, nl2;
+};
+
 __device__ __forceinline__ void intra_step_fast(uint16_t* __restrict__ S, uint32_t l0, const StepRefs& q, uint32_t w,
-                                                int32_t rv, bool b8, bool mixed, int x, int y, int ti,
-                                                bool active, int32_t maxv) {
+                                                int32_t rv, bool b8, bool mixed, int x, int y, int ti, bool active,
+                                                int32_t maxv) {
     const int32_t r0 = q.r0, r1 = q.r1;
     const int k = (w >> 26) & 31;
     int32_t r = (mul24(32 - k, r0) + mul24(k, r1) + 16) >> 5;
@@ -675,6 +693,9 @@ __device__ __forceinline__ int step_size(uint32_t d0) { return max(1, (int)((d0 
 __device__ __forceinline__ int next_step_spread(uint32_t d0) { return (int)((d0 >> 25) & 3u); }
 // Index of the TB after the step whose leader's descriptor word w is d3.
 __device__ __forceinline__ int step_end(uint32_t d3) { return (int)((d3 >> 16) & 0x1ffu); }
+// step_end, plus 512 when the step is not fast: a fast run goes on while the
+// next step's key is at most the count of ready TBs.
+__device__ __forceinline__ int step_key(uint32_t d3) { return (int)((d3 >> 16) & 0x3ffu); }
 // Whether lane l (lane16 = l / 16) holds a sample of a step of TBs of at
 // most 8 x 8.
 __device__ __forceinline__ bool step_active(uint32_t d0, int lane16) { return lane16 <= (int)(d0 >> 30); }
@@ -684,8 +705,8 @@ __device__ __forceinline__ bool step_active(uint32_t d0, int lane16) { return la
 // them be waited for alone).
 template <typename R>
 __device__ __forceinline__ void step_prefetch_fast(const IntraLds& L, const R* __restrict__ resb, uint32_t d0,
-                                                   uint32_t d1, uint32_t l0, int lane, int lt4, int lt8, int lr4,
-                                                   int lr8, uint32_t& w, int32_t& r, int& ti) {
+                                                   uint32_t d1, uint32_t l0, int lane, int lane16, int lt4, int lt8,
+                                                   int lr4, int lr8, uint32_t& w, int32_t& r, int& ti) {
     const bool b8 = (d0 >> 12) & 1;
     w = L.smap[min((int)((d1 >> 13) & 0xfff) + lane, kMaxCtb * kMaxCtb - 1)];
     ti = kTopW + 1 + mul24((l0 >> 6) & 63, kRowW) + (int)(l0 & 63) + (b8 ? lt8 : lt4);
@@ -702,8 +723,8 @@ __device__ __forceinline__ void step_prefetch_fast(const IntraLds& L, const R* _
 // no residual.
 template <typename R>
 __device__ __forceinline__ void step_prefetch(const IntraLds& L, const R* __restrict__ resb, uint32_t d0,
-                                              uint32_t d1, uint32_t l0, int lane, int lt4, int lt8, int lr4,
-                                              int lr8, uint32_t& w, int32_t& r, int& ti) {
+                                              uint32_t d1, uint32_t l0, int lane, int lane16, int lt4, int lt8,
+                                              int lr4, int lr8, uint32_t& w, int32_t& r, int& ti) {
     // (a group's TBs are 4 x 4)
     if (!((d0 >> 13) & 1)) {
         const bool b8 = (d0 >> 12) & 1;
@@ -716,7 +737,7 @@ __device__ __forceinline__ void step_prefetch(const IntraLds& L, const R* __rest
 
 // Index of the descriptor of the lane's TB in a step that starts at TB i,
 // spread the step's next_step_spread mask (lanes past a group read a later
-// TB, or the zero descriptors after the last; their results go to the sink
+// TB, or the descriptors after the last; their results go to the sink
 // slot).
 __device__ __forceinline__ int lane_desc(int i, int spread, int lane16) { return i + (lane16 & spread); }
 
@@ -856,7 +877,9 @@ __global__ void __launch_bounds__(kIntraThreads)
             }
         }
         if (tid == 0) {
-            L.desc[ntb] = make_uint4(0u, 0u, 0u, 0u);
+            // after the last TB: a descriptor that is not fast (key 512, above
+            // any count of ready TBs), which ends a fast run
+            L.desc[ntb] = make_uint4(0u, 0u, 0u, 1u << 25);
             L.bready = 0;
             L.tile[kNeutral] = (uint16_t)(1 << (bd - 1));
             L.tile[kZero] = 0;
@@ -994,7 +1017,7 @@ __global__ void __launch_bounds__(kIntraThreads)
                 const int li = lane_desc(0, step_size(cd0) > 1 ? 3 : 0, lane16);
                 cl0 = dw[4 * li];
                 cl2 = dw[4 * li + 2];
-                step_prefetch(L, resb, cd0, cd1, cl0, lane, lt4, lt8, lr4, lr8, cw, cr, cti);
+                step_prefetch(L, resb, cd0, cd1, cl0, lane, lane16, lt4, lt8, lr4, lr8, cw, cr, cti);
                 const int ni = step_end(cd3), nli = lane_desc(ni, next_step_spread(cd0), lane16);
                 nr0 = dw[4 * ni];
                 nr1 = dw[4 * ni + 1];
@@ -1021,65 +1044,81 @@ __global__ void __launch_bounds__(kIntraThreads)
                     lds_acquire();
                 }
                 if ((cd0 >> 21) & 1) {
-                    // a run of fast steps, each ready and followed by a fast one
-                    for (;;) {
+                    // a run of fast steps, each ready and followed by a fast
+                    // one, unrolled by two over two register sets
+                    FastUni Au{i, cd0, cd1, cd2, cd3}, Bu;
+                    FastLane Av{cw, cr, cti, cl0, cl2, nr0, nr1, nr2, nr3, nl0, nl2}, Bv;
+                    auto step = [&](const FastUni& cu, const FastLane& c, FastUni& nu, FastLane& n) -> bool {
+                        const bool mixed = (cu.d0 >> 29) & 1;
                         // this step's angular reference pair is read first
                         // (the reads follow the previous step's write); the
                         // next step's loads, the planar / H-V pair and the
                         // loop control are issued while it is in flight.
-                        // Past the last TB the loads read clamped slots and
-                        // their values go unused.
-                        const bool mixed = (cd0 >> 29) & 1;
+                        // Past the last TB the loads read the descriptors
+                        // after it and the slots they name, and their
+                        // values go unused.
                         StepRefs q;
-                        q.r0 = L.tile[cw & 0x1fffu];
-                        q.r1 = L.tile[(cw >> 13) & 0x1fffu];
+                        q.r0 = L.tile[c.w & 0x1fffu];
+                        q.r1 = L.tile[(c.w >> 13) & 0x1fffu];
                         __builtin_amdgcn_sched_barrier(0);
-                        const int next = step_end(cd3);
-                        const uint32_t nd0 = uni(nr0), nd1 = uni(nr1), nd2 = uni(nr2), nd3 = uni(nr3);
-                        uint32_t nw, mr0, mr1, mr2, mr3, ml0, ml2;
-                        int32_t nrr;
-                        int nti;
-                        step_prefetch_fast(L, resb, nd0, nd1, nl0, lane, lt4, lt8, lr4, lr8, nw, nrr, nti);
-                        const int nn = step_end(nd3);
-                        {
-                            const uint4 m = L.desc[nn];
-                            mr0 = m.x;
-                            mr1 = m.y;
-                            mr2 = m.z;
-                            mr3 = m.w;
-                            const int nli = lane_desc(nn, next_step_spread(nd0), lane16);
-                            ml0 = dw[4 * nli];
-                            ml2 = dw[4 * nli + 2];
-                        }
-                        q.ru0 = q.ru1 = 0;
+                        nu.i = step_end(cu.d3);
+                        nu.d0 = uni(c.n0);
+                        nu.d1 = uni(c.n1);
+                        nu.d2 = uni(c.n2);
+                        nu.d3 = uni(c.n3);
+                        n.l0 = c.nl0;
+                        n.l2 = c.nl2;
+                        step_prefetch_fast(L, resb, nu.d0, nu.d1, n.l0, lane, lane16, lt4, lt8, lr4, lr8, n.w, n.r, n.ti);
+                        const int key = step_key(nu.d3), nn = key & 0x1ff;
+                        const uint4 m = L.desc[nn];
+                        n.n0 = m.x;
+                        n.n1 = m.y;
+                        n.n2 = m.z;
+                        n.n3 = m.w;
+                        const int nli = lane_desc(nn, next_step_spread(nu.d0), lane16);
+                        n.nl0 = dw[4 * nli];
+                        n.nl2 = dw[4 * nli + 2];
                         if (mixed) {
-                            q.ru0 = L.tile[cl2 & 0x1fffu];
-                            q.ru1 = L.tile[(cl2 >> 13) & 0x1fffu];
+                            q.ru0 = L.tile[c.l2 & 0x1fffu];
+                            q.ru1 = L.tile[(c.l2 >> 13) & 0x1fffu];
                         }
                         __builtin_amdgcn_sched_barrier(0);
-                        const bool b8 = (cd0 >> 12) & 1;
+                        const bool b8 = (cu.d0 >> 12) & 1;
                         const int x = b8 ? x8 : x4, y = b8 ? y8 : y4;
-                        const bool active = step_active(cd0, lane16);
-                        intra_step_fast(L.tile, cl0, q, cw, cr, b8, mixed, x, y, cti, active, maxv);
-                        if ((cd0 >> 28) & 1) intra_step_publish(L.tile, cl0, b8, x, y, cti, active, bw, tagw);
-                        i = next;
-                        cd0 = nd0;
-                        cd1 = nd1;
-                        cd2 = nd2;
-                        cd3 = nd3;
-                        cw = nw;
-                        cr = nrr;
-                        cti = nti;
-                        cl0 = nl0;
-                        cl2 = nl2;
-                        nr0 = mr0;
-                        nr1 = mr1;
-                        nr2 = mr2;
-                        nr3 = mr3;
-                        nl0 = ml0;
-                        nl2 = ml2;
-                        // the zero descriptor after the last TB is not fast
-                        if (!((cd0 >> 21) & 1) || nn > ready) break;
+                        const bool active = step_active(cu.d0, lane16);
+                        intra_step_fast(L.tile, c.l0, q, c.w, c.r, b8, mixed, x, y, c.ti, active, maxv);
+                        if ((cu.d0 >> 28) & 1) intra_step_publish(L.tile, c.l0, b8, x, y, c.ti, active, bw, tagw);
+                        // the next step runs here if it is fast (the
+                        // descriptor after the last TB is not) and ready
+                        return key <= ready;
+                    };
+                    auto leave = [&](const FastUni& u, const FastLane& v) {
+                        i = u.i;
+                        cd0 = u.d0;
+                        cd1 = u.d1;
+                        cd2 = u.d2;
+                        cd3 = u.d3;
+                        cw = v.w;
+                        cr = v.r;
+                        cti = v.ti;
+                        cl0 = v.l0;
+                        cl2 = v.l2;
+                        nr0 = v.n0;
+                        nr1 = v.n1;
+                        nr2 = v.n2;
+                        nr3 = v.n3;
+                        nl0 = v.nl0;
+                        nl2 = v.nl2;
+                    };
+                    for (;;) {
+                        if (!step(Au, Av, Bu, Bv)) {
+                            leave(Bu, Bv);
+                            break;
+                        }
+                        if (!step(Bu, Bv, Au, Av)) {
+                            leave(Au, Av);
+                            break;
+                        }
                     }
                 } else {
                     const int next = step_end(cd3);
@@ -1088,7 +1127,7 @@ __global__ void __launch_bounds__(kIntraThreads)
                     int32_t nrr = 0;
                     int nti = kSink;
                     if (next < ntb) {
-                        step_prefetch(L, resb, nd0, nd1, nl0, lane, lt4, lt8, lr4, lr8, nw, nrr, nti);
+                        step_prefetch(L, resb, nd0, nd1, nl0, lane, lane16, lt4, lt8, lr4, lr8, nw, nrr, nti);
                         const int nn = step_end(nd3), nli = lane_desc(nn, next_step_spread(nd0), lane16);
                         mr0 = dw[4 * nn];
                         mr1 = dw[4 * nn + 1];
