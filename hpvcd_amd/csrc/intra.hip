@@ -2,8 +2,15 @@
 // (included by kernels.hip inside namespace hpvcd).
 
 constexpr int kMaxCtb = 64;
-constexpr int kTopW = 2 * kMaxCtb + 1;            // row above the unit: x0-1 .. x0+2*ctb-1
-constexpr int kRowW = kMaxCtb + 1;                // body rows: left neighbour + the unit's samples
+// Tile layout: the row above the unit (x0-1 .. x0+2*ctb-1, padded), then 64
+// body rows, each the left neighbour followed by the unit's samples.  The
+// padding puts every body row's first sample on an 8-byte boundary, and the
+// row stride of 72 samples (36 banks) moves each row 4 banks on, so the
+// 64 samples of an 8 x 8 TB, or of four 4 x 4 TBs of one 8 x 8 block, fall
+// in distinct banks.
+constexpr int kTopW = 2 * kMaxCtb + 3;            // row above the unit: x0-1 .. x0+2*ctb-1 (+2 unused)
+constexpr int kRowW = kMaxCtb + 8;                // body row stride (left neighbour at the sample before)
+static_assert((kTopW + 1) % 4 == 0 && kRowW % 4 == 0, "body rows start on 8-byte boundaries");
 constexpr int kNeutral = kTopW + kMaxCtb * kRowW; // tile slot holding 1 << (bitDepth - 1)
 constexpr int kZero = kNeutral + 1;               // tile slot holding 0: the I_PCM predictor
 constexpr int kSink = kNeutral + 2;               // write-only slot of the lanes of a step that hold no sample
@@ -622,11 +629,9 @@ template <typename Q> __device__ __forceinline__ Q quad_zero();
 template <> __device__ __forceinline__ uint2 quad_zero<uint2>() { return make_uint2(0, 0); }
 template <> __device__ __forceinline__ uint4 quad_zero<uint4>() { return make_uint4(0, 0, 0, 0); }
 
-// Four 16-bit residual samples into the tile (rows of odd length: one
-// 16-bit store each).
-__device__ __forceinline__ void store_res_tile(uint16_t* t, uint2 q) {
-    t[0] = (uint16_t)q.x; t[1] = (uint16_t)(q.x >> 16); t[2] = (uint16_t)q.y; t[3] = (uint16_t)(q.y >> 16);
-}
+// Four 16-bit residual samples into the tile, one 8-byte store (t is a
+// body sample whose x is a multiple of 4).
+__device__ __forceinline__ void store_res_tile(uint16_t* t, uint2 q) { *reinterpret_cast<uint2*>(t) = q; }
 
 // The common step: one TB of at most 8 x 8 or a group of up to four
 // independent 4 x 4 TBs of one dependency level, references unfiltered, not
@@ -1168,8 +1173,8 @@ __global__ void __launch_bounds__(kIntraThreads)
             const int ry = e >> ctbw_log2, rx = e - (ry << ctbw_log2);
             const int px = ux0 + rx, py = uy0 + ry;
             if (px < pw && py < ph) {
-                const uint16_t* t = L.tile + kTopW + ry * kRowW + rx + 1;
-                store_quad(pl + (size_t)py * pw + px, t[0], t[1], t[2], t[3]);
+                const uint2 q = *reinterpret_cast<const uint2*>(L.tile + kTopW + 1 + ry * kRowW + rx);
+                store_quad(pl + (size_t)py * pw + px, q.x & 0xffffu, q.x >> 16, q.y & 0xffffu, q.y >> 16);
             }
         }
         __syncthreads();
