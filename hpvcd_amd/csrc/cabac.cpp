@@ -240,10 +240,8 @@ struct Scans {
     }
 };
 
-const Scans& scans() {
-    static const Scans s;
-    return s;
-}
+const Scans kScans;
+inline const Scans& scans() { return kScans; }
 
 // sig_coeff_flag context (9.3.4.2.5; src/cabac/residual_full.rs:189)
 int sig_ctx(int xc, int yc, int prev_csbf, int log2n, int scan, bool luma);
@@ -267,10 +265,8 @@ struct SigTable {
     }
 };
 
-const SigTable& sig_table() {
-    static const SigTable t;
-    return t;
-}
+const SigTable kSigTable;
+inline const SigTable& sig_table() { return kSigTable; }
 
 int sig_ctx(int xc, int yc, int prev_csbf, int log2n, int scan, bool luma) {
     static const uint8_t map4[16] = {0, 1, 4, 5, 2, 3, 4, 5, 6, 6, 8, 8, 7, 7, 8, 8};

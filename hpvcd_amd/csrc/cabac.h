@@ -90,49 +90,72 @@ struct Contexts {
 // arithmetic decoding engine (9.3.4.3)
 // ---------------------------------------------------------------------------
 struct CabacDec {
+    // The 9-bit ivlOffset register and the stream bits after it share one
+    // 64-bit window: val = ivlOffset << k | the next k unread bits.  Reading
+    // n more bits into ivlOffset (renormalisation, bypass) only moves the
+    // boundary (k -= n), and ivlOffset >= x is val >= x << k, so a bin needs
+    // no bit reads; the window takes 32 more bits whenever k drops below 24.
+    // Past the end the stream reads as one bits (as the reference reads it).
     const uint8_t* data = nullptr;
-    size_t len = 0;       // bytes of the slice data
-    size_t pos = 0;       // bits consumed
-    uint32_t range = 510, offset = 0;
+    size_t len = 0;         // bytes of the slice data
+    size_t loaded = 0;      // stream bits taken into the window
+    uint64_t val = 0;
+    int k = 0;              // unread bits held below ivlOffset
+    uint32_t range = 510;
     bool overrun = false;
-    // bits [pos, pos + cache_bits) of the stream, most significant first;
-    // past the end the stream reads as one bits (as the reference reads it)
-    uint64_t cache = 0;
-    int cache_bits = 0;
 
+    // bits consumed: ivlOffset's bits included, the window's unread ones not
+    size_t pos() const { return loaded - (size_t)k; }
+    uint32_t offset() const { return (uint32_t)(val >> k); }
+
+    // k <= 23 on entry: val < 2^(9 + k) has room for 32 more bits
     void refill() {
-        size_t byte = (pos + (size_t)cache_bits) >> 3;
-        while (cache_bits <= 56) {
-            const uint64_t b = byte < len ? data[byte] : 0xffu;
-            cache |= b << (56 - cache_bits);
-            cache_bits += 8;
-            byte++;
+        const size_t byte = loaded >> 3;  // loaded is a multiple of 32
+        uint32_t w;
+        if (byte + 4 <= len) {
+            w = (uint32_t)data[byte] << 24 | (uint32_t)data[byte + 1] << 16 | (uint32_t)data[byte + 2] << 8 |
+                (uint32_t)data[byte + 3];
+        } else {
+            w = 0;
+            for (int i = 0; i < 4; i++) w = w << 8 | (byte + i < len ? data[byte + i] : 0xffu);
         }
+        val = val << 32 | w;
+        k += 32;
+        loaded += 32;
     }
-    // n in 0..32
+    void keep_filled() {
+        if (k < 24) refill();
+    }
+    // n in 0..24 raw bits after ivlOffset (PCM samples, alignment); they do
+    // not enter ivlOffset
     uint32_t read_bits(int n) {
-        if (cache_bits < n) refill();
-        const uint32_t v = (uint32_t)((cache >> 1) >> (63 - n));
-        cache <<= n;
-        cache_bits -= n;
-        pos += (size_t)n;
+        if (n == 0) return 0;
+        if (k < n) refill();
+        const uint64_t low = val & ((1ull << (k - n)) - 1ull);
+        const uint32_t v = (uint32_t)((val >> (k - n)) & ((1ull << n) - 1ull));
+        val = (val >> k << (k - n)) | low;
+        k -= n;
+        keep_filled();
         return v;
     }
     // read past the end of the data, or flagged malformed by a caller
-    bool failed() const { return overrun || pos > len * 8; }
-    uint32_t read_bit() { return read_bits(1); }
+    bool failed() const { return overrun || pos() > len * 8; }
     void start(const uint8_t* d, size_t n) {
         data = d;
         len = n;
-        pos = 0;
-        cache = 0;
-        cache_bits = 0;
+        loaded = 0;
+        val = 0;
+        k = 0;
         overrun = false;
+        refill();
         init_engine();
     }
-    void init_engine() {  // 9.3.2.5
+    void init_engine() {  // 9.3.2.5: ivlOffset = the next 9 bits
         range = 510;
-        offset = read_bits(9);
+        if (k < 9) refill();
+        val &= (1ull << k) - 1ull;
+        k -= 9;
+        keep_filled();
     }
     // DecodeDecision (9.3.4.3.2) with the state transition (9.3.4.3.2.2)
     // and renormalisation (9.3.4.3.3), without data-dependent branches
@@ -141,29 +164,48 @@ struct CabacDec {
         const uint32_t e = kCabacDecision[st & 127][(range >> 6) & 3];
         const uint32_t lps = e & 0xffu;
         range -= lps;
-        const uint32_t m = 0u - (uint32_t)(offset >= range);  // all ones on an LPS
-        offset -= range & m;
+        const uint64_t scaled = (uint64_t)range << k;
+        const uint32_t m = 0u - (uint32_t)(val >= scaled);  // all ones on an LPS
+        val -= scaled & (uint64_t)(int64_t)(int32_t)m;
         range ^= (range ^ lps) & m;
         ctx = (uint8_t)(((e >> 8) ^ (((e >> 8) ^ (e >> 16)) & m)) & 0xffu);
         const int n = __builtin_clz(range) - 23;  // 0 when range >= 256
         range <<= n;
-        offset = (offset << n) | read_bits(n);
+        k -= n;
+        keep_filled();
         return (int)((st >> 6) ^ (m & 1u));
     }
+    // DecodeBypass (9.3.4.3.4)
     int bypass() {
-        offset = (offset << 1) | read_bits(1);
-        const uint32_t m = 0u - (uint32_t)(offset >= range);
-        offset -= range & m;
+        k -= 1;
+        const uint64_t scaled = (uint64_t)range << k;
+        const uint32_t m = 0u - (uint32_t)(val >= scaled);
+        val -= scaled & (uint64_t)(int64_t)(int32_t)m;
+        keep_filled();
         return (int)(m & 1u);
     }
+    // n bypass bins, first bin most significant.  With ivlCurrRange fixed,
+    // n bypass steps are the long division of ivlOffset << n | the next n
+    // bits by ivlCurrRange: the quotient is the bins, the remainder the new
+    // ivlOffset (ivlOffset < ivlCurrRange keeps the quotient below 2^n).
     uint32_t bypass_bits(int n) {
         uint32_t v = 0;
-        for (int i = 0; i < n; i++) v = (v << 1) | (uint32_t)bypass();
+        while (n > 0) {
+            const int c = n < 16 ? n : 16;
+            const int sh = k - c;
+            const uint32_t top = (uint32_t)(val >> sh);  // < 2^(9 + c)
+            const uint32_t q = top / range, r = top - q * range;
+            val = ((uint64_t)r << sh) | (val & ((1ull << sh) - 1ull));
+            k = sh;
+            keep_filled();
+            v = v << c | q;
+            n -= c;
+        }
         return v;
     }
     int terminate() {
         range -= 2;
-        if (offset >= range) return 1;
+        if (val >= ((uint64_t)range << k)) return 1;
         renorm();
         return 0;
     }
@@ -171,14 +213,15 @@ struct CabacDec {
         if (range < 256) {
             const int n = __builtin_clz(range) - 23;  // shifts that bring range to [256, 510]
             range <<= n;
-            offset = (offset << n) | read_bits(n);
+            k -= n;
+            keep_filled();
         }
     }
     void align_bypass() { range = 256; }
     // to the next byte boundary after a terminating bin (pcm_flag, end of
     // a sub-stream): the bits the engine holds are given back
     void byte_align() {
-        const int drop = (int)((8 - (pos & 7)) & 7);
+        const int drop = (int)((8 - (pos() & 7)) & 7);
         if (drop) read_bits(drop);
     }
 };

@@ -164,8 +164,8 @@ public:
         for (int i = 0; i < n_luma; i++) out[i] = r.uni(1u << bd_y);
         for (int i = 0; i < 2 * n_chroma; i++) out[n_luma + i] = r.uni(1u << bd_c);
     }
-    void luma_modes(int npu, const std::function<void(int, uint8_t*)>&,
-                    const std::function<void(int, uint8_t)>& commit) override {
+    void luma_modes(int npu, FnRef<void(int, uint8_t*)>,
+                    FnRef<void(int, uint8_t)> commit) override {
         for (int k = 0; k < npu; k++) commit(k, (uint8_t)r.uni(35));
     }
     int chroma_mode_syntax() override { return (int)r.uni(5); }

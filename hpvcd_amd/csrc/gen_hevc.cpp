@@ -145,8 +145,8 @@ public:
         for (int i = 0; i < 2 * n_chroma; i++) w_.bits(out[n_luma + i], bd_c);
         e_.start(&w_);  // 9.3.2.5 after the PCM samples
     }
-    void luma_modes(int npu, const std::function<void(int, uint8_t*)>& mpm,
-                    const std::function<void(int, uint8_t)>& commit) override {
+    void luma_modes(int npu, FnRef<void(int, uint8_t*)> mpm,
+                    FnRef<void(int, uint8_t)> commit) override {
         int flag[4] = {0, 0, 0, 0}, val[4] = {0, 0, 0, 0};
         in_.luma_modes(npu, mpm, [&](int k, uint8_t mode) {
             uint8_t m[3];

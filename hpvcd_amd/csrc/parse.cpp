@@ -135,8 +135,8 @@ public:
         for (int i = 0; i < 2 * n_chroma; i++) out[n_luma + i] = d_.read_bits(bd_c);
         d_.init_engine();
     }
-    void luma_modes(int npu, const std::function<void(int, uint8_t*)>& mpm,
-                    const std::function<void(int, uint8_t)>& commit) override {
+    void luma_modes(int npu, FnRef<void(int, uint8_t*)> mpm,
+                    FnRef<void(int, uint8_t)> commit) override {
         int flag[4];
         for (int k = 0; k < npu; k++) flag[k] = d_.decode(c_.s[CTX_PREV_INTRA]);
         for (int k = 0; k < npu; k++) {

@@ -605,14 +605,17 @@ hpvcd_tb PicWalker::make_tb(int x, int y, int log2n, int comp, uint8_t mode, boo
     const uint32_t start = (uint32_t)out.coeffs.size();
     int max_x = 0;
     if (ro.nnz >= 0) {
-        // the source listed its nonzero positions: raster order, then clear them
-        std::sort(nzpos.begin(), nzpos.begin() + ro.nnz);
-        for (int i = 0; i < ro.nnz; i++) {
-            const int pos = nzpos[i];
-            out.coeffs.push_back(HPVCD_COEFF_PACK(pos, lv[pos]));
-            max_x = std::max(max_x, pos & (n - 1));
-            lv[pos] = 0;
-        }
+        // the source listed its nonzero positions: out in raster order (a
+        // bit per position of the TB), then cleared
+        uint64_t bits[16] = {};  // n * n <= 1024
+        for (int i = 0; i < ro.nnz; i++) bits[nzpos[i] >> 6] |= 1ull << (nzpos[i] & 63);
+        for (int wi = 0; wi < (n * n + 63) >> 6; wi++)
+            for (uint64_t b = bits[wi]; b; b &= b - 1) {
+                const int pos = wi * 64 + __builtin_ctzll(b);
+                out.coeffs.push_back(HPVCD_COEFF_PACK(pos, lv[pos]));
+                max_x = std::max(max_x, pos & (n - 1));
+                lv[pos] = 0;
+            }
     } else {
         for (int pos = 0; pos < n * n; pos++) {
             if (!lv[pos]) continue;

@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hpvcd.h"
@@ -87,6 +88,20 @@ struct Rng {
 // Decisions of one picture's syntax.  Each is asked only where the syntax
 // element is present; `ctx` arguments are the context increments the
 // walker derived.
+// A non-owning reference to a callable (a lambda at a call site), called
+// through one function pointer: no allocation per call, unlike std::function.
+template <typename Sig> class FnRef;
+template <typename R, typename... A> class FnRef<R(A...)> {
+    void* obj_;
+    R (*call_)(void*, A...);
+
+public:
+    template <typename F>
+    FnRef(F&& f)  // NOLINT: implicit from a callable that outlives the call
+        : obj_((void*)&f), call_([](void* o, A... a) -> R { return (*(std::remove_reference_t<F>*)o)(a...); }) {}
+    R operator()(A... a) const { return call_(obj_, a...); }
+};
+
 struct Source {
     virtual ~Source() = default;
     // init_type: 0 for I slices, 1 or 2 for P / B (9.3.2.2 with cabac_init_flag)
@@ -109,8 +124,7 @@ struct Source {
     virtual void pcm_samples(int n_luma, int n_chroma, int bd_y, int bd_c, uint32_t* out) = 0;
     // luma modes of npu PUs: mpm(k, list) fills PU k's candidates from the
     // modes committed so far; commit(k, mode) records PU k's mode
-    virtual void luma_modes(int npu, const std::function<void(int, uint8_t*)>& mpm,
-                            const std::function<void(int, uint8_t)>& commit) = 0;
+    virtual void luma_modes(int npu, FnRef<void(int, uint8_t*)> mpm, FnRef<void(int, uint8_t)> commit) = 0;
     virtual int chroma_mode_syntax() = 0;   // intra_chroma_pred_mode 0..4
     virtual bool split_tf(int log2) = 0;
     virtual bool cbf_chroma(int depth) = 0;

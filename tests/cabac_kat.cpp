@@ -42,8 +42,8 @@ static void kat_pcm_realign() {
     const uint32_t pcm = d.read_bits(16);
     d.init_engine();
     const uint32_t want = ((uint32_t)data[4] << 1) | (uint32_t)(data[5] >> 7);
-    check("pcm_realign", pcm == 0x1234 && d.offset == want && d.range == 510,
-          "pcm=" + std::to_string(pcm) + " offset=" + std::to_string(d.offset));
+    check("pcm_realign", pcm == 0x1234 && d.offset() == want && d.range == 510,
+          "pcm=" + std::to_string(pcm) + " offset=" + std::to_string(d.offset()));
 }
 
 // The reference engine test bypass_alignment_changes_only_range: alignment
@@ -54,11 +54,11 @@ static void kat_bypass_alignment() {
     d.start(data, sizeof(data));
     uint8_t ctx = (uint8_t)(23 | (1 << 6));
     for (int i = 0; i < 5; i++) d.decode(ctx);
-    const uint32_t off = d.offset;
-    const size_t pos = d.pos;
+    const uint32_t off = d.offset();
+    const size_t pos = d.pos();
     d.range = 319;
     d.align_bypass();
-    check("bypass_alignment", d.range == 256 && d.offset == off && d.pos == pos);
+    check("bypass_alignment", d.range == 256 && d.offset() == off && d.pos() == pos);
 }
 
 // Decoding what the encoder wrote gives back every bin, for random context
@@ -139,9 +139,9 @@ static void prop_engine_inverse() {
         // the decoder reads exactly the bits the encoder wrote, stop bit included
         if (ok) {
             d.byte_align();
-            if (d.pos != w.buf.size() * 8) {
+            if (d.pos() != w.buf.size() * 8) {
                 ok = false;
-                why = "decoder position " + std::to_string(d.pos) + " vs " + std::to_string(w.buf.size() * 8);
+                why = "decoder position " + std::to_string(d.pos()) + " vs " + std::to_string(w.buf.size() * 8);
             }
         }
     }
@@ -257,7 +257,42 @@ static void kat_param_sets() {
           "status " + std::to_string((int)st4));
 }
 
+// A run of n bypass bins read at once (one long division per 16 bins)
+// equals n bins read one at a time, for random streams, ranges after a
+// decision and run lengths 0..32, interleaved with context-coded bins.
+static void prop_bypass_runs() {
+    std::mt19937 rng(777);
+    std::vector<uint8_t> data(4096);
+    bool ok = true;
+    std::string why;
+    for (int trial = 0; trial < 200 && ok; trial++) {
+        for (auto& b : data) b = (uint8_t)rng();
+        CabacDec a, b;
+        a.start(data.data(), data.size());
+        b.start(data.data(), data.size());
+        if (a.offset() >= a.range) continue;  // not a conforming start (9.3.2.5)
+        uint8_t ca = (uint8_t)(rng() & 127), cb = ca;
+        for (int op = 0; op < 300 && ok; op++) {
+            if (rng() % 3 == 0) {
+                const int x = a.decode(ca), y = b.decode(cb);
+                if (x != y) { ok = false; why = "decision"; }
+                continue;
+            }
+            const int n = (int)(rng() % 33);
+            const uint32_t x = a.bypass_bits(n);
+            uint32_t y = 0;
+            for (int i = 0; i < n; i++) y = (y << 1) | (uint32_t)b.bypass();
+            if (x != y || a.offset() != b.offset() || a.pos() != b.pos()) {
+                ok = false;
+                why = "trial " + std::to_string(trial) + " op " + std::to_string(op) + " n " + std::to_string(n);
+            }
+        }
+    }
+    check("bypass_runs", ok, why);
+}
+
 int main() {
+    prop_bypass_runs();
     kat_pcm_realign();
     kat_bypass_alignment();
     prop_engine_inverse();

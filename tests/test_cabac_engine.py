@@ -137,5 +137,5 @@ def test_engine_known_answers(tmp_path):
     lines = r.stdout.strip().splitlines()
     assert r.returncode == 0, r.stdout
     names = {l.split()[0] for l in lines if l.endswith(" ok")}
-    assert names == {"pcm_realign", "bypass_alignment", "engine_inverse", "expgolomb", "emulation_prevention",
+    assert names == {"bypass_runs", "pcm_realign", "bypass_alignment", "engine_inverse", "expgolomb", "emulation_prevention",
                      "sps_roundtrip", "pps_roundtrip", "tile_uniform_spacing", "chroma_422_refused"}, r.stdout
