@@ -457,37 +457,42 @@ void residual_decode(CabacDec& d, Contexts& c, const ResidualParams& p, int32_t*
             E.n = 1;
         }
         const int top = i == last_sb ? last_in_sb - 1 : 15;
-        for (int k = top; k >= 0; k--) {
+        // every position is written and counted only when significant (no
+        // branch on the decoded flag)
+        const int sb_base = (sby << 2) * n + (sbx << 2);
+        for (int k = top; k > 0; k--) {
             const int pp = S.pos_order[k];
-            const int xc = (sbx << 2) + (pp & 3), yc = (sby << 2) + (pp >> 2);
-            bool sig;
-            if (k == 0 && infer_dc && E.n == 0) sig = true;
-            else {
-                const int cx = ts_ctx >= 0 ? ts_ctx : sig_row[pp];
-                sig = d.decode(c.s[CTX_SIG + cx]) != 0;
-            }
-            if (sig) {
-                E.coeff[E.n] = (uint16_t)(yc * n + xc);
-                E.k[E.n] = (uint8_t)k;
-                E.n++;
-            }
+            const int cx = ts_ctx >= 0 ? ts_ctx : sig_row[pp];
+            const int sig = d.decode(c.s[CTX_SIG + cx]);
+            E.coeff[E.n] = (uint16_t)(sb_base + (pp >> 2) * n + (pp & 3));
+            E.k[E.n] = (uint8_t)k;
+            E.n += sig;
+        }
+        if (top >= 0) {  // position 0 of the sub-block, inferred when nothing else is
+            const int pp = S.pos_order[0];
+            int sig;
+            if (infer_dc && E.n == 0) sig = 1;
+            else sig = d.decode(c.s[CTX_SIG + (ts_ctx >= 0 ? ts_ctx : sig_row[pp])]);
+            E.coeff[E.n] = (uint16_t)(sb_base + (pp >> 2) * n + (pp & 3));
+            E.k[E.n] = 0;
+            E.n += sig;
         }
         if (!E.n) continue;
         int ctx_set = (i == 0 || !p.luma) ? 0 : 2;
         if (!first_sb && c1_carry == 0) ctx_set++;
         first_sb = false;
-        int c1 = 1, first_g1 = 16;
+        int c1 = 1;
         uint32_t g1mask = 0;
         const int ng1 = std::min(E.n, 8);
+        const int g1_base = CTX_G1 + ctx_set * 4 + (p.luma ? 0 : 16);
         for (int j = 0; j < ng1; j++) {
-            const int cx = ctx_set * 4 + std::min(c1, 3) + (p.luma ? 0 : 16);
-            const int g = d.decode(c.s[CTX_G1 + cx]);
+            const int g = d.decode(c.s[g1_base + std::min(c1, 3)]);
             g1mask |= (uint32_t)g << j;
-            if (g) {
-                c1 = 0;
-                if (first_g1 == 16) first_g1 = j;
-            } else if (c1 > 0 && c1 < 3) c1++;
+            // greater1Ctx: 0 after a level above one (it stays 0), else one
+            // more up to 3
+            c1 = g ? 0 : c1 + (c1 > 0 && c1 < 3);
         }
+        const int first_g1 = g1mask ? __builtin_ctz(g1mask) : 16;
         c1_carry = c1;
         const bool g2 = first_g1 < 16 ? d.decode(c.s[CTX_G2 + ctx_set + (p.luma ? 0 : 4)]) != 0 : false;
         const bool escape = E.n > 8 || __builtin_popcount(g1mask) > 1 || g2;
