@@ -323,7 +323,7 @@ int scan_for(int mode, int log2n, bool luma, bool chroma444) {
 }
 
 // --- binarisations of coeff_abs_level_remaining (9.3.3.11 / 9.3.3.12) ------
-static uint32_t dec_remaining(CabacDec& d, int rice, bool ep, int bd) {
+static inline __attribute__((always_inline)) uint32_t dec_remaining(CabacDec& d, int rice, bool ep, int bd) {
     int q = 0;
     while (q < 4 && d.bypass()) q++;
     if (q < 4) return ((uint32_t)q << rice) + d.bypass_bits(rice);
@@ -406,7 +406,19 @@ bool rdpcm_active(const ResidualParams& p, const ResidualOut& r) {
 
 }  // namespace
 
-void residual_decode(CabacDec& d, Contexts& c, const ResidualParams& p, int32_t* levels, ResidualOut& r) {
+static void residual_decode_local(CabacDec& d, Contexts& c, const ResidualParams& p, int32_t* levels, ResidualOut& r);
+
+// The engine runs on a local copy: the context stores (bytes, which may alias
+// anything) then cannot force its state out of registers after every bin.
+void residual_decode(CabacDec& dref, Contexts& c, const ResidualParams& p, int32_t* levels, ResidualOut& r) {
+    CabacDec d = dref;
+    residual_decode_local(d, c, p, levels, r);
+    dref = d;
+}
+
+static inline __attribute__((always_inline)) void residual_decode_local(CabacDec& d, Contexts& c,
+                                                                        const ResidualParams& p, int32_t* levels,
+                                                                        ResidualOut& r) {
     const RcState S = rc_state(p);
     const int n = S.n;
     if (r.nz) r.nnz = 0;

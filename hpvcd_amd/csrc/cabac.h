@@ -109,7 +109,7 @@ struct CabacDec {
     uint32_t offset() const { return (uint32_t)(val >> k); }
 
     // k <= 23 on entry: val < 2^(9 + k) has room for 32 more bits
-    void refill() {
+    __attribute__((always_inline)) void refill() {
         const size_t byte = loaded >> 3;  // loaded is a multiple of 32
         uint32_t w;
         if (byte + 4 <= len) {
@@ -123,7 +123,7 @@ struct CabacDec {
         k += 32;
         loaded += 32;
     }
-    void keep_filled() {
+    __attribute__((always_inline)) void keep_filled() {
         if (k < 24) refill();
     }
     // n in 0..24 raw bits after ivlOffset (PCM samples, alignment); they do
@@ -159,7 +159,7 @@ struct CabacDec {
     }
     // DecodeDecision (9.3.4.3.2) with the state transition (9.3.4.3.2.2)
     // and renormalisation (9.3.4.3.3), without data-dependent branches
-    int decode(uint8_t& ctx) {
+    __attribute__((always_inline)) int decode(uint8_t& ctx) {
         const uint32_t st = ctx;
         const uint32_t e = kCabacDecision[st & 127][(range >> 6) & 3];
         const uint32_t lps = e & 0xffu;
@@ -176,7 +176,7 @@ struct CabacDec {
         return (int)((st >> 6) ^ (m & 1u));
     }
     // DecodeBypass (9.3.4.3.4)
-    int bypass() {
+    __attribute__((always_inline)) int bypass() {
         k -= 1;
         const uint64_t scaled = (uint64_t)range << k;
         const uint32_t m = 0u - (uint32_t)(val >= scaled);
@@ -188,8 +188,12 @@ struct CabacDec {
     // n bypass steps are the long division of ivlOffset << n | the next n
     // bits by ivlCurrRange: the quotient is the bins, the remainder the new
     // ivlOffset (ivlOffset < ivlCurrRange keeps the quotient below 2^n).
-    uint32_t bypass_bits(int n) {
+    __attribute__((always_inline)) uint32_t bypass_bits(int n) {
         uint32_t v = 0;
+        if (n < 8) {  // a divide costs more than a few single bins
+            for (int i = 0; i < n; i++) v = v << 1 | (uint32_t)bypass();
+            return v;
+        }
         while (n > 0) {
             const int c = n < 16 ? n : 16;
             const int sh = k - c;
