@@ -582,6 +582,23 @@ hpvcd_status assemble_records(std::vector<ParsedPicture>& pics, const std::vecto
     hpvcd_records* r = new (std::nothrow) hpvcd_records();
     if (!r) return set_error(HPVCD_E_ALLOC, "out of memory");
     r->outputs = outputs;
+    // every array at its final size once: no reallocation while appending
+    size_t ntb = 0, nco = 0, nsao = 0, ncell = 0, npu = 0, nref = 0;
+    for (const ParsedPicture& pp : pics) {
+        ntb += pp.rec.tbs.size();
+        nco += pp.rec.coeffs.size();
+        nsao += pp.rec.sao.size();
+        ncell += pp.rec.cells.size();
+        npu += pp.rec.pus.size();
+        nref += pp.rec.refs.size();
+    }
+    r->pictures.reserve(pics.size());
+    r->tbs.reserve(ntb);
+    r->coeffs.reserve(nco);
+    r->sao.reserve(nsao);
+    r->cells.reserve(ncell);
+    r->pus.reserve(npu);
+    r->refs.reserve(nref);
     std::vector<const uint8_t*> sets;
     for (ParsedPicture& pp : pics) {
         hpvcd_picture p = pp.pic;
