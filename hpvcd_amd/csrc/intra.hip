@@ -633,14 +633,8 @@ template <> __device__ __forceinline__ uint4 quad_zero<uint4>() { return make_ui
 // body sample whose x is a multiple of 4).
 __device__ __forceinline__ void store_res_tile(uint16_t* t, uint2 q) { *reinterpret_cast<uint2*>(t) = q; }
 
-// The common step: one TB of at most 8 x 8 or a group of up to four
-// independent 4 x 4 TBs of one dependency level, references unfiltered, not
-// DC, one sample per lane.  Lane l takes sample l of the step (a group's
-// samples run TB after TB, 16 each); l0/l2 are the descriptor words of its
-// TB, w/rv its prefetched reference word and residual, (x, y) its sample in
-// the TB and ti the sample's tile index.  Lanes past the step's samples
-// write to the sink slot.  Planar and H/V need two more reference slots
-// (descriptor word z), read only when `mixed` says some lane needs them.
+// The reference samples of a fast step's lane: the angular pair (r0, r1) and,
+// for planar and H/V, the class's uniform pair (ru0, ru1).
 struct StepRefs {
     int32_t r0, r1, ru0, ru1;
 };
@@ -661,6 +655,14 @@ struct FastLane {
     uint32_t l0, l2, n0, n1, n2, n3, nl0, nl2;
 };
 
+// The common step: one TB of at most 8 x 8 or a group of up to four
+// independent 4 x 4 TBs of one dependency level, references unfiltered, not
+// DC, one sample per lane.  Lane l takes sample l of the step (a group's
+// samples run TB after TB, 16 each); l0/l2 are the descriptor words of its
+// TB, w/rv its prefetched reference word and residual, (x, y) its sample in
+// the TB and ti the sample's tile index.  Lanes past the step's samples
+// write to the sink slot.  Planar and H/V need two more reference slots
+// (descriptor word z), read only when `mixed` says some lane needs them.
 __device__ __forceinline__ void intra_step_fast(uint16_t* __restrict__ S, uint32_t l0, const StepRefs& q, uint32_t w,
                                                 int32_t rv, bool b8, bool mixed, int x, int y, int ti, bool active,
                                                 int32_t maxv) {
@@ -710,7 +712,7 @@ __device__ __forceinline__ bool step_active(uint32_t d0, int lane16) { return la
 // them be waited for alone).
 template <typename R>
 __device__ __forceinline__ void step_prefetch_fast(const IntraLds& L, const R* __restrict__ resb, uint32_t d0,
-                                                   uint32_t d1, uint32_t l0, int lane, int lane16, int lt4, int lt8,
+                                                   uint32_t d1, uint32_t l0, int lane, int lt4, int lt8,
                                                    int lr4, int lr8, uint32_t& w, int32_t& r, int& ti) {
     const bool b8 = (d0 >> 12) & 1;
     w = L.smap[min((int)((d1 >> 13) & 0xfff) + lane, kMaxCtb * kMaxCtb - 1)];
@@ -728,7 +730,7 @@ __device__ __forceinline__ void step_prefetch_fast(const IntraLds& L, const R* _
 // no residual.
 template <typename R>
 __device__ __forceinline__ void step_prefetch(const IntraLds& L, const R* __restrict__ resb, uint32_t d0,
-                                              uint32_t d1, uint32_t l0, int lane, int lane16, int lt4, int lt8,
+                                              uint32_t d1, uint32_t l0, int lane, int lt4, int lt8,
                                               int lr4, int lr8, uint32_t& w, int32_t& r, int& ti) {
     // (a group's TBs are 4 x 4)
     if (!((d0 >> 13) & 1)) {
@@ -1022,7 +1024,7 @@ __global__ void __launch_bounds__(kIntraThreads)
                 const int li = lane_desc(0, step_size(cd0) > 1 ? 3 : 0, lane16);
                 cl0 = dw[4 * li];
                 cl2 = dw[4 * li + 2];
-                step_prefetch(L, resb, cd0, cd1, cl0, lane, lane16, lt4, lt8, lr4, lr8, cw, cr, cti);
+                step_prefetch(L, resb, cd0, cd1, cl0, lane, lt4, lt8, lr4, lr8, cw, cr, cti);
                 const int ni = step_end(cd3), nli = lane_desc(ni, next_step_spread(cd0), lane16);
                 nr0 = dw[4 * ni];
                 nr1 = dw[4 * ni + 1];
@@ -1073,7 +1075,7 @@ __global__ void __launch_bounds__(kIntraThreads)
                         nu.d3 = uni(c.n3);
                         n.l0 = c.nl0;
                         n.l2 = c.nl2;
-                        step_prefetch_fast(L, resb, nu.d0, nu.d1, n.l0, lane, lane16, lt4, lt8, lr4, lr8, n.w, n.r, n.ti);
+                        step_prefetch_fast(L, resb, nu.d0, nu.d1, n.l0, lane, lt4, lt8, lr4, lr8, n.w, n.r, n.ti);
                         const int key = step_key(nu.d3), nn = key & 0x1ff;
                         const uint4 m = L.desc[nn];
                         n.n0 = m.x;
@@ -1132,7 +1134,7 @@ __global__ void __launch_bounds__(kIntraThreads)
                     int32_t nrr = 0;
                     int nti = kSink;
                     if (next < ntb) {
-                        step_prefetch(L, resb, nd0, nd1, nl0, lane, lane16, lt4, lt8, lr4, lr8, nw, nrr, nti);
+                        step_prefetch(L, resb, nd0, nd1, nl0, lane, lt4, lt8, lr4, lr8, nw, nrr, nti);
                         const int nn = step_end(nd3), nli = lane_desc(nn, next_step_spread(nd0), lane16);
                         mr0 = dw[4 * nn];
                         mr1 = dw[4 * nn + 1];
