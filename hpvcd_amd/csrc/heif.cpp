@@ -13,6 +13,7 @@
 // conformance window and colour, tiles that fail to parse are left black,
 // and a second irot/imir after a first non-identity one is ignored.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <new>
@@ -535,11 +536,14 @@ hpvcd_status heic_records(const uint8_t* file, size_t n, const HeifLimits& L, in
         std::vector<ParsedPicture> all(nt);
         std::vector<hpvcd_status> sts(nt, HPVCD_OK);
         // tiles parse on the decoder's worker threads
+        // (each thread takes the next unparsed tile: tiles of unequal size
+        // balance out)
         const unsigned nth = (unsigned)std::max(1, std::min<int>(threads, (int)nt));
+        std::atomic<size_t> next{0};
         std::vector<std::thread> th;
         for (unsigned w = 0; w < nth; w++)
-            th.emplace_back([&, w]() {
-                for (size_t i = w; i < nt; i += nth) {
+            th.emplace_back([&]() {
+                for (size_t i = next++; i < nt; i = next++) {
                     const HeifItem& t = hf.tiles[i];
                     const std::vector<uint8_t>& hv = !t.hvcc.empty() ? t.hvcc : *fallback;
                     Sps s;
