@@ -14,6 +14,7 @@
 #include "parse.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <new>
 #include <string>
@@ -900,8 +901,11 @@ extern "C" hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32
         std::vector<size_t> todo;
         for (size_t i = 0; i < jobs.size(); i++)
             if (jobs[i].wave == wv) todo.push_back(i);
-        auto work = [&](size_t t, size_t stride) {
-            for (size_t k = t; k < todo.size(); k += stride) {
+        // each thread takes the wave's next unparsed picture (pictures of a
+        // wave can differ much in size)
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t k = next++; k < todo.size(); k = next++) {
                 PicJob& job = jobs[todo[k]];
                 std::vector<const InterSlice*> inter(job.slices.size(), nullptr);
                 for (size_t s2 = 0; s2 < job.slices.size(); s2++)
@@ -914,10 +918,10 @@ extern "C" hpvcd_status hpvcd_parse_hevc(const uint8_t* data, size_t size, int32
         };
         const size_t nw = std::min<size_t>(nth, todo.size());
         if (nw <= 1) {
-            work(0, 1);
+            work();
         } else {
             std::vector<std::thread> th;
-            for (size_t t = 0; t < nw; t++) th.emplace_back(work, t, nw);
+            for (size_t t = 0; t < nw; t++) th.emplace_back(work);
             for (auto& x : th) x.join();
         }
     }
